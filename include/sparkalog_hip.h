@@ -1,0 +1,243 @@
+/*
+ * sparkalog_hip: MI355X (gfx950) native binary recursive operator pipeline.
+ *
+ * Part 1 is the drop-in operator ABI. Each sparkalog_hip_* operator replaces
+ * the reference's extern "C" function of the same suffix in
+ * crates/sparkalog-execution/native/column_ops.cu (declared for Rust at
+ * crates/sparkalog-execution/src/lib.rs:28-169) with identical arguments and
+ * conventions:
+ *   - the return value is a hipError_t (0 = success);
+ *   - all work is asynchronous on `stream` (a hipStream_t, or NULL);
+ *   - nothing is allocated: buffers are caller-owned device memory, and
+ *     scratch comes from `temporary`, sized by the matching
+ *     *_temporary_bytes query;
+ *   - an empty input writes a zero count on the stream and succeeds;
+ *   - a null pointer with a non-zero length is hipErrorInvalidValue;
+ *   - counts are written on the device (u32 for filter and anti-join, u64 for
+ *     distinct, union and the join total).
+ * Scratch arrays the reference needed for its CUB pipeline (filter flags,
+ * anti-join flags/selection, union packed copies, distinct packed/scratch)
+ * keep their parameters; the kernels here use them only where noted.
+ *
+ * Part 2 is the device-resident relation store and semi-naive executor
+ * (RelationStore / RecursiveExecutor of
+ * crates/sparkalog-recursion/src/generic.rs:16-534), with FULL/DELTA/NEWT
+ * kept in HBM between iterations.
+ */
+#ifndef SPARKALOG_HIP_H
+#define SPARKALOG_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int sparkalog_hip_error; /* hipError_t value */
+
+/* ------------------------------------------------------------------ */
+/* Part 1: operator ABI                                                */
+
+/* column_ops.cu:231-246 (smoke test: data[i] += 1). */
+sparkalog_hip_error sparkalog_hip_add_one_i32(int32_t* data, size_t len, void* stream);
+
+/* column_ops.cu:248-250. */
+sparkalog_hip_error sparkalog_hip_stream_synchronize(void* stream);
+
+/* column_ops.cu:252-276. `flags` is unused (ballot compaction needs no flag
+ * array) and may be NULL. */
+sparkalog_hip_error sparkalog_hip_filter_u32_temporary_bytes(const uint32_t* flags, uint32_t* output,
+                                                             uint32_t* count, size_t len,
+                                                             size_t* temporary_bytes, void* stream);
+
+/* column_ops.cu:278-318: output[0..*count) = ordered row ids i with
+ * compare(input[i], operation, operand); operation 0..5 = Eq, Ne, Lt, Le,
+ * Gt, Ge (crates/sparkalog-execution/src/lib.rs:1769-1778). */
+sparkalog_hip_error sparkalog_hip_filter_u32(const uint32_t* input, size_t len, uint32_t operation,
+                                             uint32_t operand, uint32_t* flags, uint32_t* output,
+                                             uint32_t* count, void* temporary, size_t temporary_bytes,
+                                             void* stream);
+
+/* column_ops.cu:320-337: output[i] = i % modulus. */
+sparkalog_hip_error sparkalog_hip_fill_mod_u32(uint32_t* output, size_t len, uint32_t modulus,
+                                               void* stream);
+
+/* column_ops.cu:339-354. */
+sparkalog_hip_error sparkalog_hip_join_u32_temporary_bytes(const uint64_t* counts, uint64_t* offsets,
+                                                           size_t left_rows, size_t* temporary_bytes,
+                                                           void* stream);
+
+/* column_ops.cu:356-399: counts[l] = matches of left_keys[l] in the range
+ * index (keys[unique_keys] sorted unique, starts[unique_keys+1]); offsets =
+ * exclusive sum of counts; *total = sum. */
+sparkalog_hip_error sparkalog_hip_join_u32_count(const uint32_t* left_keys, size_t left_rows,
+                                                 const uint32_t* index_keys, const uint32_t* index_starts,
+                                                 size_t unique_keys, uint64_t* counts, uint64_t* offsets,
+                                                 uint64_t* total, void* temporary, size_t temporary_bytes,
+                                                 void* stream);
+
+/* column_ops.cu:401-440: writes output rows offsets[l] .. offsets[l]+counts[l]
+ * of left row l in index order; side 0 takes projection[l], side 1 takes
+ * projection[index_rows[match]]. */
+sparkalog_hip_error sparkalog_hip_join_u32_emit(const uint32_t* left_keys, size_t left_rows,
+                                                const uint32_t* index_keys, const uint32_t* index_starts,
+                                                const uint32_t* index_rows, size_t unique_keys,
+                                                const uint32_t* projection0, uint32_t projection0_side,
+                                                const uint32_t* projection1, uint32_t projection1_side,
+                                                const uint64_t* offsets, uint32_t* output0, uint32_t* output1,
+                                                void* stream);
+
+/* column_ops.cu:442-484. */
+sparkalog_hip_error sparkalog_hip_distinct_u32_temporary_bytes(uint64_t* packed, uint64_t* scratch,
+                                                               uint64_t* unique_rows, size_t rows,
+                                                               size_t* temporary_bytes, void* stream);
+
+/* column_ops.cu:486-547: sorted unique (first, second) pairs. `packed` and
+ * `scratch` (rows entries each) are the radix-sort ping-pong buffers. */
+sparkalog_hip_error sparkalog_hip_distinct_u32(const uint32_t* first, const uint32_t* second, size_t rows,
+                                               uint64_t* packed, uint64_t* scratch, uint64_t* unique_rows,
+                                               uint32_t* output_first, uint32_t* output_second,
+                                               void* temporary, size_t temporary_bytes, void* stream);
+
+/* column_ops.cu:549-575. `flags` and `selected` are unused and may be NULL. */
+sparkalog_hip_error sparkalog_hip_anti_join_u32_temporary_bytes(const uint32_t* flags, uint32_t* selected,
+                                                                uint32_t* selected_rows, size_t left_rows,
+                                                                size_t* temporary_bytes, void* stream);
+
+/* column_ops.cu:577-647: left rows absent from the sorted right relation, in
+ * left order; *selected_rows = count. */
+sparkalog_hip_error sparkalog_hip_anti_join_u32(const uint32_t* left_first, const uint32_t* left_second,
+                                                size_t left_rows, const uint32_t* right_first,
+                                                const uint32_t* right_second, size_t right_rows,
+                                                uint32_t* flags, uint32_t* selected, uint32_t* selected_rows,
+                                                uint32_t* output_first, uint32_t* output_second,
+                                                void* temporary, size_t temporary_bytes, void* stream);
+
+/* column_ops.cu:649-689. left/right/merged/unique are unused and may be NULL. */
+sparkalog_hip_error sparkalog_hip_union_u32_temporary_bytes(const uint64_t* left, size_t left_rows,
+                                                            const uint64_t* right, size_t right_rows,
+                                                            uint64_t* merged, uint64_t* unique,
+                                                            uint64_t* unique_rows, size_t* temporary_bytes,
+                                                            void* stream);
+
+/* column_ops.cu:691-775: merge of two sorted relations with every adjacent
+ * duplicate dropped; *unique_rows = count. The packed scratch arrays are
+ * unused (the merge reads the columns directly) and may be NULL. */
+sparkalog_hip_error sparkalog_hip_union_u32(const uint32_t* left_first, const uint32_t* left_second,
+                                            size_t left_rows, const uint32_t* right_first,
+                                            const uint32_t* right_second, size_t right_rows, uint64_t* left,
+                                            uint64_t* right, uint64_t* merged, uint64_t* unique,
+                                            uint64_t* unique_rows, uint32_t* output_first,
+                                            uint32_t* output_second, void* temporary, size_t temporary_bytes,
+                                            void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Part 2: device-resident relation store and semi-naive executor      */
+
+/* Status codes of the store/executor calls (GenericExecutionError and
+ * RelationStoreError, crates/sparkalog-recursion/src/generic.rs:129-135 and
+ * :544-559; execution::Error, crates/sparkalog-execution/src/lib.rs:171-188). */
+enum sparkalog_hip_status {
+    SPARKALOG_HIP_OK = 0,
+    SPARKALOG_HIP_ERR_HIP = 1,               /* a HIP call failed: sparkalog_hip_last_hip_error() */
+    SPARKALOG_HIP_ERR_TOO_MANY_ROWS = 2,     /* > u32::MAX rows */
+    SPARKALOG_HIP_ERR_MISSING_COLUMN = 3,    /* plan column >= 2 */
+    SPARKALOG_HIP_ERR_ITERATION_LIMIT = 4,
+    SPARKALOG_HIP_ERR_INCOMPATIBLE_PLANS = 5,
+    SPARKALOG_HIP_ERR_NOT_RECURSIVE = 6,
+    SPARKALOG_HIP_ERR_MISSING_RELATION = 7,
+    SPARKALOG_HIP_ERR_DUPLICATE_RELATION = 8,
+    SPARKALOG_HIP_ERR_EMPTY_SCC = 9,
+    SPARKALOG_HIP_ERR_DELTA_OUTSIDE_SCC = 10,
+    SPARKALOG_HIP_ERR_TARGET_OUTSIDE_SCC = 11,
+    SPARKALOG_HIP_ERR_MISSING_RULE_TARGET = 12,
+    SPARKALOG_HIP_ERR_OUT_OF_MEMORY = 13,
+    SPARKALOG_HIP_ERR_UNSUPPORTED_PLAN = 14, /* anti-join/union columns other than [0,1] */
+    SPARKALOG_HIP_ERR_DEVICE = 15,           /* a kernel raised its device error flag */
+    SPARKALOG_HIP_ERR_INVALID_ARGUMENT = 16
+};
+
+/* RecursiveRulePlan (crates/sparkalog-relational/src/lib.rs:74-82). */
+typedef struct {
+    uint32_t target, delta_input, right_input;
+    uint32_t left_key, right_key;     /* BinaryEqualityJoin keys */
+    uint32_t out_input[2];            /* JoinInput: 0 = Left, 1 = Right */
+    uint32_t out_column[2];
+    uint32_t distinct_columns[2];     /* BinaryDistinct */
+    uint32_t anti_left[2], anti_right[2];   /* SortedBinaryAntiJoin */
+    uint32_t union_left[2], union_right[2]; /* SortedBinaryUnion */
+} sparkalog_hip_rule;
+
+/* SccSummary (crates/sparkalog-recursion/src/generic.rs:536-542) plus
+ * device timing and traffic of the run. */
+typedef struct {
+    uint64_t iterations;
+    uint64_t total_new_rows;
+    uint64_t delta_rows_left;   /* on ITERATION_LIMIT: DELTA rows still pending */
+    double device_ms;           /* wall time of the run on the stream */
+    double algorithmic_bytes;   /* sum over operators of their algorithmic bytes */
+} sparkalog_hip_summary;
+
+typedef struct sparkalog_hip_store sparkalog_hip_store;
+
+/* A store bound to one device stream (NULL = the default stream). */
+int sparkalog_hip_store_create(sparkalog_hip_store** out, void* stream);
+void sparkalog_hip_store_destroy(sparkalog_hip_store* store);
+
+/* RelationStore::insert_static (generic.rs:36-53). Columns are host memory
+ * when on_device == 0 (staged through pinned memory) or device memory. */
+int sparkalog_hip_store_insert_static(sparkalog_hip_store* store, uint32_t id, const uint32_t* c0,
+                                      const uint32_t* c1, uint64_t rows, int on_device);
+
+/* RelationStore::insert_recursive (generic.rs:55-79). FULL and DELTA must be
+ * sorted unique pairs, as the reference requires of its callers. */
+int sparkalog_hip_store_insert_recursive(sparkalog_hip_store* store, uint32_t id, const uint32_t* full0,
+                                         const uint32_t* full1, uint64_t full_rows, const uint32_t* delta0,
+                                         const uint32_t* delta1, uint64_t delta_rows, int on_device);
+
+/* Forget every relation but keep the device buffers (and the executor's
+ * operator workspaces) for reuse, so a repeated run allocates nothing. */
+int sparkalog_hip_store_clear(sparkalog_hip_store* store);
+
+/* Row count of FULL (which = 0) or DELTA (which = 1). */
+int sparkalog_hip_store_rows(sparkalog_hip_store* store, uint32_t id, int which, uint64_t* rows);
+
+/* Copy FULL / DELTA out as two columns of `capacity` rows (host memory when
+ * to_device == 0, through pinned staging; device memory otherwise). */
+int sparkalog_hip_store_read(sparkalog_hip_store* store, uint32_t id, int which, uint32_t* c0, uint32_t* c1,
+                             uint64_t capacity, int to_device);
+
+/* Order-independent digest (sum of mix64(((a<<32)|b) + 0x9E3779B97F4A7C15)),
+ * strict-sortedness flag and row count of FULL, computed on the device. */
+int sparkalog_hip_store_digest(sparkalog_hip_store* store, uint32_t id, uint64_t* digest, int* sorted_unique,
+                               uint64_t* rows);
+
+/* RecursiveExecutor::compile (generic.rs:407-470): validation only. */
+int sparkalog_hip_executor_compile(sparkalog_hip_store* store, const uint32_t* relations, uint32_t n_relations,
+                                   const sparkalog_hip_rule* rules, uint32_t n_rules);
+
+/* RecursiveExecutor::compile + run (generic.rs:407-516) on the device. */
+int sparkalog_hip_executor_run(sparkalog_hip_store* store, const uint32_t* relations, uint32_t n_relations,
+                               const sparkalog_hip_rule* rules, uint32_t n_rules, uint64_t max_iterations,
+                               sparkalog_hip_summary* summary);
+
+/* Per-operator profiling: when enabled, every operator of a run is bracketed
+ * by HIP events on the store's stream. op: 0 join, 1 distinct, 2 anti-join,
+ * 3 union, 4 index build. Reports launches, summed device milliseconds and
+ * summed algorithmic bytes (8 bytes per pair read or written). */
+int sparkalog_hip_store_set_profiling(sparkalog_hip_store* store, int enabled);
+int sparkalog_hip_store_op_stats(sparkalog_hip_store* store, int op, uint64_t* launches, double* ms,
+                                 double* bytes);
+
+/* The hipError_t of the last failed HIP call in this thread (0 if none). */
+int sparkalog_hip_last_hip_error(void);
+
+/* Human-readable text of a status code. */
+const char* sparkalog_hip_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

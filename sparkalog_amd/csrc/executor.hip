@@ -1,0 +1,1070 @@
+// Device-resident relation store and semi-naive executor (part 2 of
+// include/sparkalog_hip.h).
+//
+// Restates RelationStore / RuleRuntime / TargetRuntime / RecursiveExecutor of
+// crates/sparkalog-recursion/src/generic.rs:16-534 with every relation kept in
+// HBM between iterations:
+//   * a relation is one packed u64 array (first<<32 | second), so the
+//     lexicographic pair order is the integer order of the keys;
+//   * the join's right input is indexed on the device: rows sorted by
+//     (key, other) plus a dense CSR start array over key values, rebuilt only
+//     when the right relation's version changes (generic.rs:217-229);
+//   * row counts stay on the device inside an iteration; the host reads them
+//     twice per iteration (join output sizes, then the new DELTA/FULL sizes),
+//     which is also the termination test of generic.rs:518-525.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ops.hpp"
+#include "prims.hpp"
+#include "sparkalog_hip.h"
+
+namespace spk {
+namespace {
+
+thread_local int g_last_hip_error = 0;
+
+struct HipFailure {
+    hipError_t code;
+};
+
+inline void check(hipError_t e) {
+    if (e != hipSuccess) {
+        g_last_hip_error = static_cast<int>(e);
+        throw HipFailure{e};
+    }
+}
+
+struct StatusError {
+    int code;
+};
+
+__host__ __device__ inline size_t tiles(size_t n, size_t t) { return (n + t - 1) / t; }
+
+inline uint64_t bit_mask_for(uint64_t max_value) {
+    uint64_t m = 0;
+    while (m < max_value) m = (m << 1) | 1;
+    return m;
+}
+
+// Device buffer with capacity; grow() keeps no contents.
+template <typename T>
+struct DevBuf {
+    T* ptr = nullptr;
+    size_t cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), cap(o.cap) {
+        o.ptr = nullptr;
+        o.cap = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        std::swap(ptr, o.ptr);
+        std::swap(cap, o.cap);
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+    void reserve(size_t n) {
+        if (n <= cap && ptr) return;
+        size_t c = cap ? cap : 1024;
+        while (c < n) c += c / 2 + 1;
+        // queued kernels may still use the old buffer
+        if (ptr) (void)hipDeviceSynchronize();
+        release();
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, c * sizeof(T));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            g_last_hip_error = static_cast<int>(e);
+            throw StatusError{SPARKALOG_HIP_ERR_OUT_OF_MEMORY};
+        }
+        ptr = static_cast<T*>(p);
+        cap = c;
+    }
+};
+
+struct DevRelation {
+    DevBuf<uint64_t> keys;
+    uint64_t rows = 0;
+};
+
+struct RelationState {
+    bool present = false;
+    bool recursive = false;
+    DevRelation full, delta, newt;
+    uint64_t version = 0;
+    uint32_t max_value = 0;  // max over both columns of every row ever stored
+};
+
+// Per-op timing (hipEvents) and algorithmic bytes when profiling is on.
+struct OpStat {
+    const char* name;
+    uint64_t launches = 0;
+    double ms = 0.0;
+    double bytes = 0.0;
+};
+
+enum OpKind { kOpJoin = 0, kOpDistinct, kOpAntiJoin, kOpUnion, kOpIndex, kOpCount };
+const char* const kOpNames[kOpCount] = {"join", "distinct", "anti_join", "union", "index"};
+
+// ------------------------------------------------------------------------
+// executor kernels
+
+__global__ void gather_index_keys_kernel(const uint64_t* rows, size_t n, uint32_t key_col, uint64_t* out) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t r = rows[i];
+        const uint32_t c0 = static_cast<uint32_t>(r >> 32), c1 = static_cast<uint32_t>(r);
+        out[i] = key_col == 0 ? r : pack_pair(c1, c0);
+    }
+}
+
+// starts[v] = first sorted row with key >= v, for v in [0, n_keys].
+__global__ void dense_starts_kernel(const uint64_t* sorted, size_t n, uint32_t* starts, size_t n_keys) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t v = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; v <= n_keys; v += stride) {
+        const uint64_t target = static_cast<uint64_t>(v) << 32;
+        size_t lo = 0, hi = n;
+        while (lo < hi) {
+            const size_t mid = (lo + hi) >> 1;
+            if (sorted[mid] < target)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        starts[v] = static_cast<uint32_t>(lo);
+    }
+}
+
+__device__ __forceinline__ uint32_t column_of(uint64_t key, uint32_t col) {
+    return col == 0 ? static_cast<uint32_t>(key >> 32) : static_cast<uint32_t>(key);
+}
+
+// Join count over a dense CSR: counts -> exclusive offsets (u64) with
+// decoupled look-back; the last tile writes the total.
+__global__ __launch_bounds__(kBlock) void dense_join_count_kernel(const uint64_t* delta, size_t n, uint32_t key_col,
+                                                                  const uint32_t* starts, uint32_t n_keys,
+                                                                  uint64_t* offsets, uint64_t* total,
+                                                                  uint64_t* status, uint32_t* counter,
+                                                                  uint32_t* error) {
+    constexpr int kItems = 8;
+    constexpr int kTile = kBlock * kItems;
+    __shared__ uint64_t scan_lds[kWavesPerBlock];
+    __shared__ uint64_t offset_slot;
+    __shared__ uint32_t tile_slot;
+    const uint32_t tile = next_tile(counter, &tile_slot);
+    const size_t base = static_cast<size_t>(tile) * kTile;
+    uint64_t c[kItems];
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        const size_t row = base + static_cast<size_t>(i) * kBlock + threadIdx.x;
+        c[i] = 0;
+        if (row < n) {
+            const uint32_t key = column_of(delta[row], key_col);
+            if (key < n_keys) c[i] = starts[key + 1] - starts[key];
+        }
+    }
+    uint64_t running = 0;
+    for (int i = 0; i < kItems; ++i) {
+        uint64_t part;
+        const uint64_t ex = block_exclusive_sum<uint64_t>(c[i], &part, scan_lds);
+        c[i] = running + ex;
+        running += part;
+    }
+    if (wave_id() == 0) {
+        const uint64_t excl = wave_lookback(status, tile, running, 0, error);
+        if (lane_id() == 0) offset_slot = excl;
+    }
+    __syncthreads();
+    const uint64_t offset = offset_slot;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        const size_t row = base + static_cast<size_t>(i) * kBlock + threadIdx.x;
+        if (row < n) offsets[row] = offset + c[i];
+    }
+    if (tile == tiles(n, kTile) - 1 && threadIdx.x == 0) *total = offset + running;
+}
+
+struct EmitSpec {
+    uint32_t key_col;          // left key column
+    uint32_t right_key;        // right key column
+    uint32_t side[2];          // projection input of output column 0/1 (0 left, 1 right)
+    uint32_t col[2];           // projection column
+    uint32_t order[2];         // distinct columns: key = out[order0] << shift | out[order1]
+    uint32_t shift;            // 32, or fewer bits to compress the sort key
+};
+
+// Output-balanced emit of the dense join, writing compressed sort keys.
+__global__ __launch_bounds__(kBlock) void dense_join_emit_kernel(const uint64_t* delta, size_t n,
+                                                                 const uint32_t* starts, uint32_t n_keys,
+                                                                 const uint64_t* right_sorted, const uint64_t* offsets,
+                                                                 size_t total, EmitSpec spec, uint64_t* out) {
+    constexpr int kTile = 2048;
+    constexpr int kStage = 2048;
+    __shared__ uint64_t row_offset[kStage];
+    __shared__ uint64_t row_key[kStage];
+    __shared__ uint32_t row_match[kStage];
+    __shared__ size_t bounds[2];
+    const uint64_t n_tiles = (total + kTile - 1) / kTile;
+    for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const uint64_t o0 = t * kTile;
+        const uint64_t o1 = o0 + kTile < total ? o0 + kTile : total;
+        if (threadIdx.x < 2) {
+            const uint64_t o = threadIdx.x ? o1 - 1 : o0;
+            size_t lo = 0, hi = n;
+            while (lo < hi) {
+                const size_t mid = (lo + hi) >> 1;
+                if (offsets[mid] <= o)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            bounds[threadIdx.x] = lo - 1;
+        }
+        __syncthreads();
+        const size_t l0 = bounds[0], l1 = bounds[1] + 1;
+        const size_t span = l1 - l0;
+        const bool staged = span <= static_cast<size_t>(kStage);
+        if (staged) {
+            for (uint32_t i = threadIdx.x; i < span; i += kBlock) {
+                const uint64_t dk = delta[l0 + i];
+                row_offset[i] = offsets[l0 + i];
+                row_key[i] = dk;
+                const uint32_t key = column_of(dk, spec.key_col);
+                row_match[i] = key < n_keys ? starts[key] : 0;
+            }
+        }
+        __syncthreads();
+        for (uint64_t o = o0 + threadIdx.x; o < o1; o += kBlock) {
+            uint64_t dk, row_start;
+            uint32_t match;
+            if (staged) {
+                uint32_t lo = 0, hi = static_cast<uint32_t>(span);
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (row_offset[mid] <= o)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                dk = row_key[lo - 1];
+                row_start = row_offset[lo - 1];
+                match = row_match[lo - 1];
+            } else {
+                size_t lo = l0, hi = l1;
+                while (lo < hi) {
+                    const size_t mid = (lo + hi) >> 1;
+                    if (offsets[mid] <= o)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                dk = delta[lo - 1];
+                row_start = offsets[lo - 1];
+                match = starts[column_of(dk, spec.key_col)];
+            }
+            const uint64_t r = right_sorted[match + static_cast<uint32_t>(o - row_start)];
+            // right_sorted rows are (key, other): column right_key is the high half
+            uint32_t v[2];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                if (spec.side[c] == 0)
+                    v[c] = column_of(dk, spec.col[c]);
+                else
+                    v[c] = spec.col[c] == spec.right_key ? static_cast<uint32_t>(r >> 32) : static_cast<uint32_t>(r);
+            }
+            out[o] = (static_cast<uint64_t>(v[spec.order[0]]) << spec.shift) | v[spec.order[1]];
+        }
+        __syncthreads();
+    }
+}
+
+// Expand compressed keys (a << shift | b) to packed pairs (a << 32 | b).
+__global__ void expand_keys_kernel(uint64_t* keys, const uint64_t* rows_dev, size_t bound, uint32_t shift) {
+    const size_t n = rows_dev ? (*rows_dev < bound ? *rows_dev : bound) : bound;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    const uint64_t low_mask = (1ull << shift) - 1;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t k = keys[i];
+        keys[i] = ((k >> shift) << 32) | (k & low_mask);
+    }
+}
+
+__global__ void digest_kernel(const uint64_t* keys, size_t n, unsigned long long* digest, uint32_t* unsorted) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    unsigned long long acc = 0;
+    uint32_t bad = 0;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint64_t z = keys[i] + 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        acc += z ^ (z >> 31);
+        if (i > 0 && !(keys[i - 1] < keys[i])) bad = 1;
+    }
+    acc = wave_sum<unsigned long long>(acc);
+    if (lane_id() == 0) atomicAdd(digest, acc);
+    if (bad) atomicOr(unsorted, 1u);
+}
+
+__global__ void max_value_kernel(const uint32_t* a, const uint32_t* b, size_t n, uint32_t* out) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    uint32_t m = 0;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        m = a[i] > m ? a[i] : m;
+        m = b[i] > m ? b[i] : m;
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const uint32_t o = __shfl_xor(m, off, kWave);
+        m = o > m ? o : m;
+    }
+    if (lane_id() == 0) atomicMax(out, m);
+}
+
+__global__ void pack_columns_kernel(const uint32_t* a, const uint32_t* b, size_t n, uint64_t* out) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        out[i] = pack_pair(a[i], b[i]);
+}
+
+__global__ void unpack_columns_kernel(const uint64_t* keys, size_t n, uint32_t* a, uint32_t* b) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        a[i] = static_cast<uint32_t>(keys[i] >> 32);
+        b[i] = static_cast<uint32_t>(keys[i]);
+    }
+}
+
+__global__ void clear_words_kernel(uint32_t* p, size_t n) {
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+unsigned grid_for(size_t n) {
+    size_t b = tiles(n, kBlock * 4);
+    if (b == 0) b = 1;
+    return static_cast<unsigned>(b < 16384 ? b : 16384);
+}
+
+// ------------------------------------------------------------------------
+// host-side runtime
+
+struct CachedIndex {
+    bool valid = false;
+    uint32_t relation = 0, column = 0;
+    uint64_t version = 0;
+    DevBuf<uint64_t> sorted;  // (key, other) rows sorted
+    DevBuf<uint32_t> starts;  // dense CSR over key values [0, n_keys]
+    uint32_t n_keys = 0;
+    uint64_t rows = 0;
+};
+
+struct RuleRuntime {
+    sparkalog_hip_rule plan{};
+    CachedIndex index;
+    DevBuf<uint64_t> status;    // look-back words of the join count
+    DevBuf<uint64_t> offsets;   // join offsets per delta row
+    DevBuf<uint64_t> cand_a, cand_b;
+    DevBuf<uint64_t> distinct_out;
+    uint64_t join_rows = 0;     // M (host, after the first sync)
+    uint64_t delta_rows = 0;
+    uint32_t shift = 32;
+};
+
+struct TargetRuntime {
+    uint32_t target = 0;
+    std::vector<uint32_t> rules;
+    DevBuf<uint64_t> contributions, union_tmp;
+    DevBuf<uint64_t> newt;
+    DevBuf<uint64_t> full_union;
+    uint64_t contrib_bound = 0;
+};
+
+// Small device counters read back once per phase.
+struct Counters {
+    uint64_t join_total[64];
+    uint64_t distinct_rows[64];
+    uint64_t contrib_rows[64];
+    uint64_t newt_rows[64];
+    uint64_t full_rows[64];
+};
+
+}  // namespace
+}  // namespace spk
+
+struct sparkalog_hip_store {
+    hipStream_t stream = nullptr;
+    std::vector<spk::RelationState> rel;
+    spk::DevBuf<unsigned char> temp;
+    spk::DevBuf<uint64_t> counters_dev;   // Counters on device
+    spk::DevBuf<uint32_t> misc_dev;       // lookback counters / error words
+    void* pinned = nullptr;
+    size_t pinned_bytes = 0;
+    bool profiling = false;
+    spk::OpStat stats[spk::kOpCount];
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> event_pool;
+    size_t events_used = 0;
+    std::vector<std::pair<int, double>> pending_bytes;  // (op, bytes) aligned with events
+    uint64_t generation = 0;                    // bumped by every insert; versions derive from it
+    // operator workspaces kept across runs so repeated runs allocate nothing
+    std::vector<spk::TargetRuntime> ws_targets;
+    std::vector<spk::RuleRuntime> ws_rules;
+
+    ~sparkalog_hip_store() {
+        if (pinned) (void)hipHostFree(pinned);
+        for (auto& e : event_pool) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+    }
+
+    void* staging(size_t bytes) {
+        if (bytes > pinned_bytes) {
+            if (pinned) (void)hipHostFree(pinned);
+            pinned = nullptr;
+            size_t b = pinned_bytes ? pinned_bytes : (1u << 20);
+            while (b < bytes) b *= 2;
+            spk::check(hipHostMalloc(&pinned, b, hipHostMallocDefault));
+            pinned_bytes = b;
+        }
+        return pinned;
+    }
+
+    void* scratch(size_t bytes) {
+        temp.reserve(bytes < 4096 ? 4096 : bytes);
+        return temp.ptr;
+    }
+
+    // Profiling brackets around one operator.
+    int begin_op() {
+        if (!profiling) return -1;
+        if (events_used == event_pool.size()) {
+            hipEvent_t a, b;
+            spk::check(hipEventCreate(&a));
+            spk::check(hipEventCreate(&b));
+            event_pool.emplace_back(a, b);
+        }
+        spk::check(hipEventRecord(event_pool[events_used].first, stream));
+        return static_cast<int>(events_used++);
+    }
+    void end_op(int slot, int op, double bytes) {
+        if (slot < 0) return;
+        spk::check(hipEventRecord(event_pool[slot].second, stream));
+        pending_bytes.resize(events_used);
+        pending_bytes[slot] = {op, bytes};
+    }
+    // Call after a stream synchronize.
+    void collect_ops() {
+        for (size_t i = 0; i < events_used; ++i) {
+            float ms = 0.f;
+            spk::check(hipEventElapsedTime(&ms, event_pool[i].first, event_pool[i].second));
+            auto& s = stats[pending_bytes[i].first];
+            s.launches += 1;
+            s.ms += ms;
+            s.bytes += pending_bytes[i].second;
+        }
+        events_used = 0;
+        pending_bytes.clear();
+    }
+};
+
+namespace spk {
+namespace {
+
+RelationState* find(sparkalog_hip_store* s, uint32_t id) {
+    if (id >= s->rel.size() || !s->rel[id].present) return nullptr;
+    return &s->rel[id];
+}
+
+RelationState* slot(sparkalog_hip_store* s, uint32_t id) {
+    if (id >= s->rel.size()) s->rel.resize(id + 1);
+    if (s->rel[id].present) throw StatusError{SPARKALOG_HIP_ERR_DUPLICATE_RELATION};
+    // a fresh version space per insert keeps cached join indexes of an earlier
+    // relation with the same id from matching
+    s->rel[id].version = (++s->generation) << 32;
+    return &s->rel[id];
+}
+
+// Upload two columns into a packed device relation; returns the max value.
+uint32_t upload(sparkalog_hip_store* s, DevRelation& dst, const uint32_t* c0, const uint32_t* c1, uint64_t rows,
+                int on_device) {
+    if (rows > 0xFFFFFFFFull) throw StatusError{SPARKALOG_HIP_ERR_TOO_MANY_ROWS};
+    dst.keys.reserve(rows ? rows : 1);
+    dst.rows = rows;
+    if (rows == 0) return 0;
+    uint32_t max_value = 0;
+    if (on_device) {
+        pack_columns_kernel<<<grid_for(rows), kBlock, 0, s->stream>>>(c0, c1, rows, dst.keys.ptr);
+        s->misc_dev.reserve(64);
+        clear_words_kernel<<<1, 64, 0, s->stream>>>(s->misc_dev.ptr, 1);
+        max_value_kernel<<<grid_for(rows), kBlock, 0, s->stream>>>(c0, c1, rows, s->misc_dev.ptr);
+        check(hipGetLastError());
+        check(hipMemcpyAsync(&max_value, s->misc_dev.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
+        check(hipStreamSynchronize(s->stream));
+        return max_value;
+    }
+    const size_t chunk = 1u << 22;  // rows per staging chunk (32 MiB)
+    uint64_t* stage = static_cast<uint64_t*>(s->staging(chunk * sizeof(uint64_t)));
+    for (uint64_t base = 0; base < rows; base += chunk) {
+        const uint64_t n = rows - base < chunk ? rows - base : chunk;
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint32_t a = c0[base + i], b = c1[base + i];
+            max_value = a > max_value ? a : max_value;
+            max_value = b > max_value ? b : max_value;
+            stage[i] = (static_cast<uint64_t>(a) << 32) | b;
+        }
+        check(hipMemcpyAsync(dst.keys.ptr + base, stage, n * sizeof(uint64_t), hipMemcpyHostToDevice, s->stream));
+        check(hipStreamSynchronize(s->stream));
+    }
+    return max_value;
+}
+
+void download(sparkalog_hip_store* s, const DevRelation& src, uint32_t* c0, uint32_t* c1, int to_device) {
+    if (src.rows == 0) return;
+    if (to_device) {
+        unpack_columns_kernel<<<grid_for(src.rows), kBlock, 0, s->stream>>>(src.keys.ptr, src.rows, c0, c1);
+        check(hipGetLastError());
+        check(hipStreamSynchronize(s->stream));
+        return;
+    }
+    const size_t chunk = 1u << 22;
+    uint64_t* stage = static_cast<uint64_t*>(s->staging(chunk * sizeof(uint64_t)));
+    for (uint64_t base = 0; base < src.rows; base += chunk) {
+        const uint64_t n = src.rows - base < chunk ? src.rows - base : chunk;
+        check(hipMemcpyAsync(stage, src.keys.ptr + base, n * sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
+        check(hipStreamSynchronize(s->stream));
+        for (uint64_t i = 0; i < n; ++i) {
+            c0[base + i] = static_cast<uint32_t>(stage[i] >> 32);
+            c1[base + i] = static_cast<uint32_t>(stage[i]);
+        }
+    }
+}
+
+bool identity(const uint32_t c[2]) { return c[0] == 0 && c[1] == 1; }
+
+void build_index(sparkalog_hip_store* s, RuleRuntime& rt, RelationState& right, uint32_t relation) {
+    const uint32_t col = rt.plan.right_key;
+    CachedIndex& ix = rt.index;
+    if (ix.valid && ix.relation == relation && ix.version == right.version && ix.column == col) return;
+    const int ev = s->begin_op();
+    const uint64_t n = right.full.rows;
+    if (right.max_value >= (1u << 28)) throw StatusError{SPARKALOG_HIP_ERR_UNSUPPORTED_PLAN};
+    const uint32_t n_keys = right.max_value + 1;
+    ix.sorted.reserve(n ? n : 1);
+    ix.starts.reserve(static_cast<size_t>(n_keys) + 1);
+    DevBuf<uint64_t> tmp;
+    tmp.reserve(n ? n : 1);
+    if (n) {
+        gather_index_keys_kernel<<<grid_for(n), kBlock, 0, s->stream>>>(right.full.keys.ptr, n, col, ix.sorted.ptr);
+        check(hipGetLastError());
+        const uint64_t vm = bit_mask_for(right.max_value);
+        const uint64_t known = (vm << 32) | vm;
+        void* temp = s->scratch(distinct_temp_bytes(n));
+        uint64_t* sorted = nullptr;
+        PairInput in{ix.sorted.ptr, nullptr, nullptr, nullptr};
+        check(radix_sort(in, n, ix.sorted.ptr, tmp.ptr, temp, distinct_temp_bytes(n), known, s->stream, &sorted));
+        if (sorted != ix.sorted.ptr) std::swap(ix.sorted, tmp);
+    }
+    dense_starts_kernel<<<grid_for(static_cast<size_t>(n_keys) + 1), kBlock, 0, s->stream>>>(ix.sorted.ptr, n,
+                                                                                          ix.starts.ptr, n_keys);
+    check(hipGetLastError());
+    check(hipStreamSynchronize(s->stream));
+    ix.valid = true;
+    ix.relation = relation;
+    ix.version = right.version;
+    ix.column = col;
+    ix.n_keys = n_keys;
+    ix.rows = n;
+    s->end_op(ev, kOpIndex, 16.0 * n);
+}
+
+// RecursiveExecutor::compile (generic.rs:407-470): validation and runtime
+// construction; returns a status code.
+int compile_plan(sparkalog_hip_store* s, const uint32_t* relations, uint32_t n_relations,
+                 const sparkalog_hip_rule* rules, uint32_t n_rules, std::vector<TargetRuntime>& targets,
+                 std::vector<RuleRuntime>& rts) {
+    if (n_rules > 64) return SPARKALOG_HIP_ERR_UNSUPPORTED_PLAN;
+    if (n_relations == 0) return SPARKALOG_HIP_ERR_EMPTY_SCC;
+    for (uint32_t i = 0; i < n_relations; ++i) {
+        for (uint32_t j = 0; j < i; ++j)
+            if (relations[j] == relations[i]) return SPARKALOG_HIP_ERR_DUPLICATE_RELATION;
+        RelationState* st = find(s, relations[i]);
+        if (!st) return SPARKALOG_HIP_ERR_MISSING_RELATION;
+        if (!st->recursive) return SPARKALOG_HIP_ERR_NOT_RECURSIVE;
+    }
+    for (uint32_t r = 0; r < n_rules; ++r) {
+        if (!find(s, rules[r].delta_input) || !find(s, rules[r].right_input)) return SPARKALOG_HIP_ERR_MISSING_RELATION;
+        bool delta_in = false, target_in = false;
+        for (uint32_t i = 0; i < n_relations; ++i) {
+            delta_in |= relations[i] == rules[r].delta_input;
+            target_in |= relations[i] == rules[r].target;
+        }
+        if (!delta_in) return SPARKALOG_HIP_ERR_DELTA_OUTSIDE_SCC;
+        if (!target_in) return SPARKALOG_HIP_ERR_TARGET_OUTSIDE_SCC;
+    }
+    if (targets.size() != n_relations) {
+        targets.clear();
+        targets.resize(n_relations);
+    }
+    for (uint32_t i = 0; i < n_relations; ++i) {
+        targets[i].target = relations[i];
+        targets[i].rules.clear();
+        for (uint32_t r = 0; r < n_rules; ++r)
+            if (rules[r].target == relations[i]) targets[i].rules.push_back(r);
+        if (targets[i].rules.empty()) return SPARKALOG_HIP_ERR_MISSING_RULE_TARGET;
+        const sparkalog_hip_rule& f = rules[targets[i].rules[0]];
+        for (uint32_t k : targets[i].rules) {
+            const sparkalog_hip_rule& o = rules[k];
+            if (std::memcmp(o.anti_left, f.anti_left, sizeof(f.anti_left)) ||
+                std::memcmp(o.anti_right, f.anti_right, sizeof(f.anti_right)) ||
+                std::memcmp(o.union_left, f.union_left, sizeof(f.union_left)) ||
+                std::memcmp(o.union_right, f.union_right, sizeof(f.union_right)))
+                return SPARKALOG_HIP_ERR_INCOMPATIBLE_PLANS;
+        }
+    }
+    if (rts.size() != n_rules) {
+        rts.clear();
+        rts.resize(n_rules);
+    }
+    for (uint32_t r = 0; r < n_rules; ++r) {
+        const sparkalog_hip_rule& p = rules[r];
+        if (p.left_key > 1 || p.right_key > 1 || p.out_column[0] > 1 || p.out_column[1] > 1 ||
+            p.distinct_columns[0] > 1 || p.distinct_columns[1] > 1 || p.out_input[0] > 1 || p.out_input[1] > 1)
+            return SPARKALOG_HIP_ERR_MISSING_COLUMN;
+        if (!identity(p.anti_left) || !identity(p.anti_right) || !identity(p.union_left) || !identity(p.union_right))
+            return SPARKALOG_HIP_ERR_UNSUPPORTED_PLAN;
+        rts[r].plan = p;
+    }
+
+    return SPARKALOG_HIP_OK;
+}
+
+}  // namespace
+}  // namespace spk
+
+using namespace spk;
+
+extern "C" int sparkalog_hip_last_hip_error(void) { return g_last_hip_error; }
+
+extern "C" const char* sparkalog_hip_status_string(int status) {
+    switch (status) {
+        case SPARKALOG_HIP_OK: return "ok";
+        case SPARKALOG_HIP_ERR_HIP: return "HIP call failed";
+        case SPARKALOG_HIP_ERR_TOO_MANY_ROWS: return "rows cannot be represented by u32 row IDs";
+        case SPARKALOG_HIP_ERR_MISSING_COLUMN: return "plan names a missing column";
+        case SPARKALOG_HIP_ERR_ITERATION_LIMIT: return "recursive SCC did not converge within the iteration limit";
+        case SPARKALOG_HIP_ERR_INCOMPATIBLE_PLANS: return "clauses for one relation use incompatible set plans";
+        case SPARKALOG_HIP_ERR_NOT_RECURSIVE: return "relation has no DELTA/NEWT state";
+        case SPARKALOG_HIP_ERR_MISSING_RELATION: return "relation is not registered";
+        case SPARKALOG_HIP_ERR_DUPLICATE_RELATION: return "relation is already registered";
+        case SPARKALOG_HIP_ERR_EMPTY_SCC: return "recursive SCC has no relations";
+        case SPARKALOG_HIP_ERR_DELTA_OUTSIDE_SCC: return "DELTA relation is outside its SCC";
+        case SPARKALOG_HIP_ERR_TARGET_OUTSIDE_SCC: return "rule target is outside its SCC";
+        case SPARKALOG_HIP_ERR_MISSING_RULE_TARGET: return "relation has no recursive rule";
+        case SPARKALOG_HIP_ERR_OUT_OF_MEMORY: return "device allocation failed";
+        case SPARKALOG_HIP_ERR_UNSUPPORTED_PLAN: return "plan shape not supported by the device executor";
+        case SPARKALOG_HIP_ERR_DEVICE: return "a kernel raised its device error flag";
+        case SPARKALOG_HIP_ERR_INVALID_ARGUMENT: return "invalid argument";
+        default: return "unknown status";
+    }
+}
+
+#define SPK_GUARD_BEGIN try {
+#define SPK_GUARD_END                                    \
+    }                                                    \
+    catch (const StatusError& e) {                       \
+        return e.code;                                   \
+    }                                                    \
+    catch (const HipFailure&) {                          \
+        return SPARKALOG_HIP_ERR_HIP;                    \
+    }                                                    \
+    catch (...) {                                        \
+        return SPARKALOG_HIP_ERR_OUT_OF_MEMORY;          \
+    }
+
+extern "C" int sparkalog_hip_store_create(sparkalog_hip_store** out, void* stream) {
+    if (!out) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    SPK_GUARD_BEGIN
+    auto* s = new sparkalog_hip_store();
+    s->stream = static_cast<hipStream_t>(stream);
+    for (int i = 0; i < kOpCount; ++i) s->stats[i].name = kOpNames[i];
+    *out = s;
+    return SPARKALOG_HIP_OK;
+    SPK_GUARD_END
+}
+
+extern "C" void sparkalog_hip_store_destroy(sparkalog_hip_store* store) {
+    if (!store) return;
+    (void)hipStreamSynchronize(store->stream);
+    delete store;
+}
+
+extern "C" int sparkalog_hip_store_insert_static(sparkalog_hip_store* s, uint32_t id, const uint32_t* c0,
+                                                 const uint32_t* c1, uint64_t rows, int on_device) {
+    if (!s || (rows && (!c0 || !c1))) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    SPK_GUARD_BEGIN
+    RelationState* st = slot(s, id);
+    st->max_value = upload(s, st->full, c0, c1, rows, on_device);
+    st->present = true;
+    st->recursive = false;
+    return SPARKALOG_HIP_OK;
+    SPK_GUARD_END
+}
+
+extern "C" int sparkalog_hip_store_insert_recursive(sparkalog_hip_store* s, uint32_t id, const uint32_t* full0,
+                                                    const uint32_t* full1, uint64_t full_rows, const uint32_t* delta0,
+                                                    const uint32_t* delta1, uint64_t delta_rows, int on_device) {
+    if (!s || (full_rows && (!full0 || !full1)) || (delta_rows && (!delta0 || !delta1)))
+        return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    SPK_GUARD_BEGIN
+    RelationState* st = slot(s, id);
+    const uint32_t m0 = upload(s, st->full, full0, full1, full_rows, on_device);
+    const uint32_t m1 = upload(s, st->delta, delta0, delta1, delta_rows, on_device);
+    st->max_value = m0 > m1 ? m0 : m1;
+    st->newt.rows = 0;
+    st->present = true;
+    st->recursive = true;
+    return SPARKALOG_HIP_OK;
+    SPK_GUARD_END
+}
+
+extern "C" int sparkalog_hip_store_clear(sparkalog_hip_store* s) {
+    if (!s) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    for (auto& st : s->rel) {
+        st.present = false;
+        st.recursive = false;
+        st.full.rows = st.delta.rows = st.newt.rows = 0;
+    }
+    return SPARKALOG_HIP_OK;
+}
+
+extern "C" int sparkalog_hip_store_rows(sparkalog_hip_store* s, uint32_t id, int which, uint64_t* rows) {
+    if (!s || !rows) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    RelationState* st = find(s, id);
+    if (!st) return SPARKALOG_HIP_ERR_MISSING_RELATION;
+    if (which == 1 && !st->recursive) return SPARKALOG_HIP_ERR_NOT_RECURSIVE;
+    *rows = which == 0 ? st->full.rows : st->delta.rows;
+    return SPARKALOG_HIP_OK;
+}
+
+extern "C" int sparkalog_hip_store_read(sparkalog_hip_store* s, uint32_t id, int which, uint32_t* c0, uint32_t* c1,
+                                        uint64_t capacity, int to_device) {
+    if (!s) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    RelationState* st = find(s, id);
+    if (!st) return SPARKALOG_HIP_ERR_MISSING_RELATION;
+    if (which == 1 && !st->recursive) return SPARKALOG_HIP_ERR_NOT_RECURSIVE;
+    const DevRelation& r = which == 0 ? st->full : st->delta;
+    if (r.rows > capacity || (r.rows && (!c0 || !c1))) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    SPK_GUARD_BEGIN
+    download(s, r, c0, c1, to_device);
+    return SPARKALOG_HIP_OK;
+    SPK_GUARD_END
+}
+
+extern "C" int sparkalog_hip_store_digest(sparkalog_hip_store* s, uint32_t id, uint64_t* digest, int* sorted_unique,
+                                          uint64_t* rows) {
+    if (!s) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    RelationState* st = find(s, id);
+    if (!st) return SPARKALOG_HIP_ERR_MISSING_RELATION;
+    SPK_GUARD_BEGIN
+    s->misc_dev.reserve(64);
+    clear_words_kernel<<<1, 64, 0, s->stream>>>(s->misc_dev.ptr, 4);
+    const uint64_t n = st->full.rows;
+    if (n) {
+        digest_kernel<<<grid_for(n), kBlock, 0, s->stream>>>(
+            st->full.keys.ptr, n, reinterpret_cast<unsigned long long*>(s->misc_dev.ptr), s->misc_dev.ptr + 2);
+    }
+    check(hipGetLastError());
+    uint32_t host[4];
+    check(hipMemcpyAsync(host, s->misc_dev.ptr, sizeof(host), hipMemcpyDeviceToHost, s->stream));
+    check(hipStreamSynchronize(s->stream));
+    uint64_t d;
+    std::memcpy(&d, host, sizeof(d));
+    if (digest) *digest = d;
+    if (sorted_unique) *sorted_unique = host[2] == 0;
+    if (rows) *rows = n;
+    return SPARKALOG_HIP_OK;
+    SPK_GUARD_END
+}
+
+extern "C" int sparkalog_hip_store_set_profiling(sparkalog_hip_store* s, int enabled) {
+    if (!s) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    s->profiling = enabled != 0;
+    for (auto& st : s->stats) {
+        st.launches = 0;
+        st.ms = 0;
+        st.bytes = 0;
+    }
+    return SPARKALOG_HIP_OK;
+}
+
+extern "C" int sparkalog_hip_store_op_stats(sparkalog_hip_store* s, int op, uint64_t* launches, double* ms,
+                                            double* bytes) {
+    if (!s || op < 0 || op >= kOpCount) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    if (launches) *launches = s->stats[op].launches;
+    if (ms) *ms = s->stats[op].ms;
+    if (bytes) *bytes = s->stats[op].bytes;
+    return SPARKALOG_HIP_OK;
+}
+
+extern "C" int sparkalog_hip_executor_compile(sparkalog_hip_store* s, const uint32_t* relations,
+                                              uint32_t n_relations, const sparkalog_hip_rule* rules,
+                                              uint32_t n_rules) {
+    if (!s || (n_relations && !relations) || (n_rules && !rules)) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    SPK_GUARD_BEGIN
+    std::vector<TargetRuntime> targets;
+    std::vector<RuleRuntime> rts;
+    return compile_plan(s, relations, n_relations, rules, n_rules, targets, rts);
+    SPK_GUARD_END
+}
+
+extern "C" int sparkalog_hip_executor_run(sparkalog_hip_store* s, const uint32_t* relations, uint32_t n_relations,
+                                          const sparkalog_hip_rule* rules, uint32_t n_rules, uint64_t max_iterations,
+                                          sparkalog_hip_summary* summary) {
+    if (!s || (n_relations && !relations) || (n_rules && !rules)) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    SPK_GUARD_BEGIN
+    std::vector<TargetRuntime>& targets = s->ws_targets;
+    std::vector<RuleRuntime>& rts = s->ws_rules;
+    {
+        const int compiled = compile_plan(s, relations, n_relations, rules, n_rules, targets, rts);
+        if (compiled != SPARKALOG_HIP_OK) return compiled;
+    }
+
+    hipEvent_t t_begin, t_end;
+    check(hipEventCreate(&t_begin));
+    check(hipEventCreate(&t_end));
+    check(hipEventRecord(t_begin, s->stream));
+
+    s->counters_dev.reserve(sizeof(Counters) / sizeof(uint64_t));
+    Counters* cdev = reinterpret_cast<Counters*>(s->counters_dev.ptr);
+    Counters chost;
+    // look-back scratch: per launch one counter + error word
+    s->misc_dev.reserve(256);
+    uint32_t* lb_counters = s->misc_dev.ptr;  // [0..127] counters, [128] error
+    uint32_t* lb_error = s->misc_dev.ptr + 128;
+
+    uint64_t iterations = 0, total_new = 0;
+    double algo_bytes = 0.0;
+    int rc = SPARKALOG_HIP_OK;
+    // max value across the SCC's relations and inputs bounds every key
+    uint32_t vmax = 0;
+    for (const auto& st : s->rel)
+        if (st.present) vmax = st.max_value > vmax ? st.max_value : vmax;
+    const uint64_t vmask = bit_mask_for(vmax);
+    uint32_t vbits = 0;
+    while ((1ull << vbits) <= vmask && vbits < 32) ++vbits;
+    if (vbits == 0) vbits = 1;
+    const uint32_t shift = vbits <= 24 ? vbits : 32;  // compressed sort key width per column
+    const uint64_t known_compressed = shift == 32 ? ((vmask << 32) | vmask) : ((vmask << shift) | vmask);
+
+    for (;;) {
+        uint64_t pending = 0;
+        for (uint32_t i = 0; i < n_relations; ++i) pending += find(s, relations[i])->delta.rows;
+        if (pending == 0) break;
+        if (iterations >= max_iterations) {
+            if (summary) summary->delta_rows_left = pending;
+            rc = SPARKALOG_HIP_ERR_ITERATION_LIMIT;
+            break;
+        }
+        clear_words_kernel<<<1, 256, 0, s->stream>>>(s->misc_dev.ptr, 256);
+        // ---- RuleRuntime::evaluate, join count (generic.rs:197-241) ----
+        for (uint32_t r = 0; r < n_rules; ++r) {
+            RuleRuntime& rt = rts[r];
+            RelationState& ds = *find(s, rt.plan.delta_input);
+            RelationState& rs = *find(s, rt.plan.right_input);
+            build_index(s, rt, rs, rt.plan.right_input);
+            rt.delta_rows = ds.delta.rows;
+            rt.shift = shift;
+            const uint64_t L = rt.delta_rows;
+            if (L == 0) {
+                check(set_u64(&cdev->join_total[r], 0, s->stream));
+                continue;
+            }
+            rt.offsets.reserve(L);
+            const size_t n_tiles = tiles(L, kBlock * 8);
+            rt.status.reserve(n_tiles);
+            check(fill_u64(rt.status.ptr, n_tiles, 0, s->stream));
+            const int ev = s->begin_op();
+            dense_join_count_kernel<<<static_cast<unsigned>(n_tiles), kBlock, 0, s->stream>>>(
+                ds.delta.keys.ptr, L, rt.plan.left_key, rt.index.starts.ptr, rt.index.n_keys, rt.offsets.ptr,
+                &cdev->join_total[r], rt.status.ptr, &lb_counters[r], lb_error);
+            check(hipGetLastError());
+            s->end_op(ev, kOpJoin, 8.0 * L);
+        }
+        check(hipMemcpyAsync(&chost, cdev, sizeof(Counters), hipMemcpyDeviceToHost, s->stream));
+        check(hipStreamSynchronize(s->stream));
+        // ---- join emit + distinct (generic.rs:231-249) ----
+        for (uint32_t r = 0; r < n_rules; ++r) {
+            RuleRuntime& rt = rts[r];
+            const uint64_t M = rt.delta_rows ? chost.join_total[r] : 0;
+            rt.join_rows = M;
+            if (M > 0xFFFFFFFFull) throw StatusError{SPARKALOG_HIP_ERR_TOO_MANY_ROWS};
+            RelationState& ds = *find(s, rt.plan.delta_input);
+            rt.cand_a.reserve(M ? M : 1);
+            rt.cand_b.reserve(M ? M : 1);
+            rt.distinct_out.reserve(M ? M : 1);
+            if (M == 0) {
+                check(set_u64(&cdev->distinct_rows[r], 0, s->stream));
+                continue;
+            }
+            EmitSpec spec;
+            spec.key_col = rt.plan.left_key;
+            spec.right_key = rt.plan.right_key;
+            spec.side[0] = rt.plan.out_input[0];
+            spec.side[1] = rt.plan.out_input[1];
+            spec.col[0] = rt.plan.out_column[0];
+            spec.col[1] = rt.plan.out_column[1];
+            spec.order[0] = rt.plan.distinct_columns[0];
+            spec.order[1] = rt.plan.distinct_columns[1];
+            spec.shift = rt.shift;
+            int ev = s->begin_op();
+            const uint64_t emit_tiles = tiles(M, 2048);
+            const unsigned emit_grid = static_cast<unsigned>(emit_tiles < 65536 ? emit_tiles : 65536);
+            dense_join_emit_kernel<<<emit_grid, kBlock, 0, s->stream>>>(
+                ds.delta.keys.ptr, rt.delta_rows, rt.index.starts.ptr, rt.index.n_keys, rt.index.sorted.ptr,
+                rt.offsets.ptr, M, spec, rt.cand_a.ptr);
+            check(hipGetLastError());
+            s->end_op(ev, kOpJoin, 8.0 * rt.index.rows + 8.0 * M);
+            ev = s->begin_op();
+            const size_t tb = distinct_temp_bytes(M);
+            void* temp = s->scratch(tb);
+            PairInput in{rt.cand_a.ptr, nullptr, nullptr, nullptr};
+            PairOutput out{rt.distinct_out.ptr, nullptr, nullptr};
+            check(distinct(in, M, rt.cand_a.ptr, rt.cand_b.ptr, out, &cdev->distinct_rows[r], temp, tb,
+                           known_compressed, s->stream));
+            if (rt.shift != 32)
+                expand_keys_kernel<<<grid_for(M), kBlock, 0, s->stream>>>(rt.distinct_out.ptr, &cdev->distinct_rows[r],
+                                                                         M, rt.shift);
+            check(hipGetLastError());
+            s->end_op(ev, kOpDistinct, 16.0 * M);  // + 8U added after the sync
+        }
+        // ---- TargetRuntime::combine (generic.rs:299-358) ----
+        for (auto& t : targets) {
+            RelationState& ts = *find(s, t.target);
+            const uint64_t* contrib = nullptr;
+            const uint64_t* contrib_rows = nullptr;
+            uint64_t bound = 0;
+            for (size_t k = 0; k < t.rules.size(); ++k) {
+                RuleRuntime& rt = rts[t.rules[k]];
+                if (k == 0) {
+                    contrib = rt.distinct_out.ptr;
+                    contrib_rows = &cdev->distinct_rows[t.rules[k]];
+                    bound = rt.join_rows;
+                    continue;
+                }
+                const uint64_t nb = bound + rt.join_rows;
+                t.union_tmp.reserve(nb ? nb : 1);
+                const size_t tb = union_temp_bytes(bound, rt.join_rows);
+                void* temp = s->scratch(tb);
+                PairInput a{contrib, nullptr, nullptr, contrib_rows};
+                PairInput b{rt.distinct_out.ptr, nullptr, nullptr, &cdev->distinct_rows[t.rules[k]]};
+                PairOutput o{t.union_tmp.ptr, nullptr, nullptr};
+                const int ev = s->begin_op();
+                check(sorted_union(a, bound, b, rt.join_rows, o, &cdev->contrib_rows[t.rules[k]], temp, tb,
+                                   s->stream));
+                s->end_op(ev, kOpUnion, 0.0);
+                std::swap(t.union_tmp, t.contributions);
+                contrib = t.contributions.ptr;
+                contrib_rows = &cdev->contrib_rows[t.rules[k]];
+                bound = nb;
+            }
+            t.contrib_bound = bound;
+            // NEWT = contributions \ FULL
+            const uint32_t ti = static_cast<uint32_t>(&t - targets.data());
+            t.newt.reserve(bound ? bound : 1);
+            if (bound == 0) {
+                check(set_u64(&cdev->newt_rows[ti], 0, s->stream));
+            } else {
+                const size_t tb = anti_join_temp_bytes(bound);
+                void* temp = s->scratch(tb);
+                PairInput a{contrib, nullptr, nullptr, contrib_rows};
+                PairInput b{ts.full.keys.ptr, nullptr, nullptr, nullptr};
+                PairOutput o{t.newt.ptr, nullptr, nullptr};
+                const int ev = s->begin_op();
+                check(sorted_anti_join(a, bound, b, ts.full.rows, o, nullptr, &cdev->newt_rows[ti], temp, tb,
+                                       s->stream));
+                s->end_op(ev, kOpAntiJoin, 8.0 * ts.full.rows);  // + 8L + 8O after the sync
+            }
+        }
+        // ---- TargetRuntime::apply (generic.rs:360-397) ----
+        for (auto& t : targets) {
+            RelationState& ts = *find(s, t.target);
+            const uint32_t ti = static_cast<uint32_t>(&t - targets.data());
+            if (t.contrib_bound == 0) {
+                // nothing new: FULL is unchanged (the union would copy it)
+                check(set_u64(&cdev->full_rows[ti], ts.full.rows, s->stream));
+                continue;
+            }
+            const uint64_t nb = ts.full.rows + t.contrib_bound;
+            t.full_union.reserve(nb ? nb : 1);
+            const size_t tb = union_temp_bytes(ts.full.rows, t.contrib_bound);
+            void* temp = s->scratch(tb);
+            PairInput a{ts.full.keys.ptr, nullptr, nullptr, nullptr};
+            PairInput b{t.newt.ptr, nullptr, nullptr, &cdev->newt_rows[ti]};
+            PairOutput o{t.full_union.ptr, nullptr, nullptr};
+            const int ev = s->begin_op();
+            check(sorted_union(a, ts.full.rows, b, t.contrib_bound, o, &cdev->full_rows[ti], temp, tb, s->stream));
+            s->end_op(ev, kOpUnion, 8.0 * ts.full.rows);  // + 8R + 8O after the sync
+        }
+        check(hipMemcpyAsync(&chost, cdev, sizeof(Counters), hipMemcpyDeviceToHost, s->stream));
+        uint32_t err = 0;
+        check(hipMemcpyAsync(&err, lb_error, sizeof(err), hipMemcpyDeviceToHost, s->stream));
+        check(hipStreamSynchronize(s->stream));
+        if (err) throw StatusError{SPARKALOG_HIP_ERR_DEVICE};
+        // per-iteration algorithmic bytes (pairs are 8 bytes)
+        for (uint32_t r = 0; r < n_rules; ++r) {
+            const double L = static_cast<double>(rts[r].delta_rows), M = static_cast<double>(rts[r].join_rows);
+            const double U = rts[r].join_rows ? static_cast<double>(chost.distinct_rows[r]) : 0.0;
+            const double R = static_cast<double>(rts[r].index.rows);
+            if (L > 0) algo_bytes += 8 * L + 8 * R + 8 * M + (M > 0 ? 8 * M + 8 * U : 0.0);
+        }
+        for (uint32_t i = 0; i < n_relations; ++i) {
+            TargetRuntime& t = targets[i];
+            RelationState& ts = *find(s, t.target);
+            const double C = static_cast<double>(t.contrib_bound ? chost.distinct_rows[t.rules[0]] : 0);
+            const double N = static_cast<double>(chost.newt_rows[i]);
+            const double F = static_cast<double>(ts.full.rows), F2 = static_cast<double>(chost.full_rows[i]);
+            if (t.contrib_bound) algo_bytes += 8 * C + 8 * F + 8 * N + 8 * F + 8 * N + 8 * F2;
+        }
+        if (s->profiling) {
+            s->collect_ops();
+        }
+        // swaps: NEWT <- anti-join output, FULL <- union output, DELTA <-> NEWT
+        for (uint32_t i = 0; i < n_relations; ++i) {
+            TargetRuntime& t = targets[i];
+            RelationState& ts = *find(s, t.target);
+            std::swap(ts.newt.keys, t.newt);
+            ts.newt.rows = chost.newt_rows[i];
+            if (t.contrib_bound) std::swap(ts.full.keys, t.full_union);
+            ts.full.rows = chost.full_rows[i];
+            std::swap(ts.delta, ts.newt);
+            ts.version += 1;
+            total_new += ts.delta.rows;
+        }
+        ++iterations;
+    }
+    check(hipEventRecord(t_end, s->stream));
+    check(hipEventSynchronize(t_end));
+    float ms = 0.f;
+    check(hipEventElapsedTime(&ms, t_begin, t_end));
+    (void)hipEventDestroy(t_begin);
+    (void)hipEventDestroy(t_end);
+    if (summary) {
+        summary->iterations = iterations;
+        summary->total_new_rows = total_new;
+        summary->device_ms = ms;
+        summary->algorithmic_bytes = algo_bytes;
+        if (rc == SPARKALOG_HIP_OK) summary->delta_rows_left = 0;
+    }
+    return rc;
+    SPK_GUARD_END
+}

@@ -1,0 +1,1105 @@
+// gfx950 kernels of the binary operator pipeline and their launchers.
+//
+// Design (all integer work, HBM-bound, no MFMA):
+//  * distinct  = onesweep LSD radix sort (Adinets & Merrill 2022) over the
+//    varying 8-bit digits of the packed key, with stable wave64 match-ranking
+//    in LDS, followed by a single-pass unique compaction.
+//  * union     = merge-path partitioned merge (Green, McColl & Bader 2012)
+//    with fused duplicate drop and single-pass compaction.
+//  * anti-join = left-tile partitioning with a per-tile right range found by
+//    binary search; ranges that fit LDS are staged there, larger ones are
+//    searched in place, so work is O(L log(R/L)) rather than O(L+R) when
+//    R >> L.
+//  * join      = single-pass count + exclusive scan, then an output-balanced
+//    emit (every block writes an equal slice of the output, whatever the key
+//    skew).
+//  * filter    = ballot compaction of matching row ids.
+// Variable-size outputs use decoupled look-back, so every operator reads
+// its input once.
+#include "ops.hpp"
+#include "prims.hpp"
+
+namespace spk {
+namespace {
+
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kBlock * kSortItems;       // 4096 keys
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;              // 256 digits
+constexpr int kPasses = 8;                           // 64-bit keys
+constexpr int kCompactItems = 16;
+constexpr int kCompactTile = kBlock * kCompactItems; // 4096 rows
+constexpr int kMergeItems = 8;
+constexpr int kMergeTile = kBlock * kMergeItems;     // 2048 merged rows
+constexpr int kAntiTile = 2048;                      // left rows per tile
+constexpr int kAntiStage = 4096;                     // right rows staged in LDS
+constexpr int kEmitTile = 2048;                      // output rows per tile
+constexpr int kEmitStage = 2048;                     // left rows staged in LDS
+
+__host__ __device__ inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+__host__ __device__ inline size_t tiles_of(size_t n, size_t t) { return (n + t - 1) / t; }
+
+// Scratch header shared by every launcher (offset 0 of `temp`).
+struct Header {
+    uint32_t error;
+    uint32_t counters[15];
+};
+
+// ------------------------------------------------------------------------
+// generic helpers
+
+__global__ void clear_u64_kernel(uint64_t* p, size_t n) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0;
+}
+
+__global__ void clear_header_kernel(Header* h) {
+    if (threadIdx.x < 16) reinterpret_cast<uint32_t*>(h)[threadIdx.x] = 0;
+}
+
+hipError_t clear_u64(uint64_t* p, size_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const size_t blocks = tiles_of(n, kBlock * 4);
+    const unsigned grid = static_cast<unsigned>(blocks < 8192 ? blocks : 8192);
+    clear_u64_kernel<<<grid, kBlock, 0, stream>>>(p, n);
+    return hipGetLastError();
+}
+
+__device__ __forceinline__ size_t actual_rows(const PairInput& in, size_t bound) {
+    if (!in.rows_dev) return bound;
+    const uint64_t n = *in.rows_dev;
+    return n < bound ? n : bound;
+}
+
+__device__ __forceinline__ uint64_t load_key(const PairInput& in, size_t i) {
+    return in.packed ? in.packed[i] : pack_pair(in.first[i], in.second[i]);
+}
+
+__device__ __forceinline__ void store_key(const PairOutput& out, size_t i, uint64_t k) {
+    if (out.packed) out.packed[i] = k;
+    if (out.first) {
+        out.first[i] = static_cast<uint32_t>(k >> 32);
+        out.second[i] = static_cast<uint32_t>(k);
+    }
+}
+
+// ------------------------------------------------------------------------
+// radix sort
+
+struct SortDesc {
+    Header header;                       // counters[0..7]: per-pass tile counters
+    uint64_t rows;                       // actual row count of this sort
+    uint64_t or_mask;
+    uint64_t and_mask;
+    uint32_t hist[kPasses][kRadix];
+    uint64_t digit_start[kPasses][kRadix];
+};
+
+__device__ __forceinline__ uint32_t varying_digits(const SortDesc* d) {
+    const uint64_t v = d->or_mask ^ d->and_mask;
+    uint32_t m = 0;
+#pragma unroll
+    for (int p = 0; p < kPasses; ++p)
+        if ((v >> (p * kRadixBits)) & (kRadix - 1)) m |= 1u << p;
+    return m;
+}
+
+// Pass 0 of the sort: copy/pack keys into keys_a, reduce OR/AND of the keys,
+// build the digit histograms of every pass, and clear status region 0.
+// Launched over a fixed grid with a grid-stride loop.
+__global__ __launch_bounds__(kBlock) void sort_histogram_kernel(PairInput in, size_t rows,
+                                                                uint64_t* keys_a, SortDesc* desc,
+                                                                uint64_t known_varying,
+                                                                uint64_t* status0, size_t status_words) {
+    __shared__ uint32_t hist[kPasses][kRadix];
+    __shared__ uint64_t red_or[kWavesPerBlock], red_and[kWavesPerBlock];
+    for (int i = threadIdx.x; i < kPasses * kRadix; i += kBlock) (&hist[0][0])[i] = 0;
+    __syncthreads();
+    const size_t stride = static_cast<size_t>(gridDim.x) * kBlock;
+    const size_t gtid = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x;
+    for (size_t i = gtid; i < status_words; i += stride) status0[i] = 0;
+    rows = desc->rows;
+    uint64_t acc_or = 0, acc_and = ~0ull;
+    // Every lane of a wave runs the same number of trips so that the wave
+    // ballots below see all 64 lanes.
+    const size_t trips = tiles_of(rows, stride);
+    for (size_t t = 0; t < trips; ++t) {
+        const size_t i = t * stride + gtid;
+        const bool valid = i < rows;
+        uint64_t key = 0;
+        if (valid) {
+            key = load_key(in, i);
+            if (!in.packed || in.packed != keys_a) keys_a[i] = key;
+            acc_or |= key;
+            acc_and &= key;
+        }
+        const uint64_t valid_mask = ballot(valid);
+        if (valid_mask == 0) continue;
+#pragma unroll
+        for (int p = 0; p < kPasses; ++p) {
+            const uint32_t digit = static_cast<uint32_t>(key >> (p * kRadixBits)) & (kRadix - 1);
+            const uint32_t lead = __builtin_amdgcn_readfirstlane(digit);
+            const uint64_t same = ballot(valid && digit == lead);
+            if (same == valid_mask) {
+                if (lane_id() == 0) atomicAdd(&hist[p][lead], static_cast<uint32_t>(__popcll(valid_mask)));
+            } else if (valid) {
+                atomicAdd(&hist[p][digit], 1u);
+            }
+        }
+    }
+    // OR/AND reduction
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        acc_or |= __shfl_xor(acc_or, off, kWave);
+        acc_and &= __shfl_xor(acc_and, off, kWave);
+    }
+    if (lane_id() == 0) {
+        red_or[wave_id()] = acc_or;
+        red_and[wave_id()] = acc_and;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t o = 0, a = ~0ull;
+        for (int w = 0; w < kWavesPerBlock; ++w) {
+            o |= red_or[w];
+            a &= red_and[w];
+        }
+        if (known_varying == 0) {
+            atomicOr(reinterpret_cast<unsigned long long*>(&desc->or_mask), static_cast<unsigned long long>(o));
+            atomicAnd(reinterpret_cast<unsigned long long*>(&desc->and_mask), static_cast<unsigned long long>(a));
+        }
+    }
+    for (int i = threadIdx.x; i < kPasses * kRadix; i += kBlock) {
+        const uint32_t c = (&hist[0][0])[i];
+        if (c) atomicAdd(&(&desc->hist[0][0])[i], c);
+    }
+}
+
+__global__ void sort_init_kernel(SortDesc* desc, uint64_t known_varying, PairInput in, size_t bound) {
+    for (int i = threadIdx.x; i < kPasses * kRadix; i += blockDim.x) (&desc->hist[0][0])[i] = 0;
+    if (threadIdx.x < 16) reinterpret_cast<uint32_t*>(&desc->header)[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        desc->rows = actual_rows(in, bound);
+        desc->or_mask = known_varying;  // 0 when unknown: OR-reduced on device
+        desc->and_mask = known_varying ? 0 : ~0ull;
+    }
+}
+
+// Exclusive scan of each pass's 256-digit histogram (one block, 256 threads).
+__global__ __launch_bounds__(kBlock) void sort_digit_scan_kernel(SortDesc* desc) {
+    __shared__ uint64_t lds[kWavesPerBlock];
+    for (int p = 0; p < kPasses; ++p) {
+        uint64_t total;
+        const uint64_t v = desc->hist[p][threadIdx.x];
+        desc->digit_start[p][threadIdx.x] = block_exclusive_sum<uint64_t>(v, &total, lds);
+    }
+}
+
+// One onesweep pass over digit `pass`. Keys move from src to dst (selected on
+// the device from the count of varying digits below this pass), ranked
+// stably within the tile by wave64 match masks, and placed at the global
+// digit offset found by per-digit decoupled look-back (thread d owns digit d).
+// The next active pass's status region is cleared slice by slice.
+__global__ __launch_bounds__(kBlock) void sort_pass_kernel(uint64_t* keys_a, uint64_t* keys_b, size_t rows,
+                                                           SortDesc* desc, int pass, uint64_t* status_r0,
+                                                           uint64_t* status_r1) {
+    __shared__ uint64_t tile_keys[kSortTile];
+    __shared__ uint32_t wave_count[kWavesPerBlock][kRadix];
+    __shared__ uint64_t digit_base[kRadix];  // global start - local start
+    __shared__ uint32_t scan_lds[kWavesPerBlock];
+    __shared__ uint32_t tile_slot;
+
+    const uint32_t varying = varying_digits(desc);
+    if (!((varying >> pass) & 1u)) return;  // constant digit: identity pass
+    const uint32_t ordinal = __popc(varying & ((1u << pass) - 1u));
+    const uint64_t* src = (ordinal & 1u) ? keys_b : keys_a;
+    uint64_t* dst = (ordinal & 1u) ? keys_a : keys_b;
+    uint64_t* status = (ordinal & 1u) ? status_r1 : status_r0;
+    uint64_t* next_status = (ordinal & 1u) ? status_r0 : status_r1;
+    const int shift = pass * kRadixBits;
+
+    rows = desc->rows;
+    const uint32_t tile = next_tile(&desc->header.counters[pass], &tile_slot);
+    const size_t tile_base = static_cast<size_t>(tile) * kSortTile;
+    if (tile_base >= rows) return;  // launch sized for an upper bound
+    const uint32_t lane = lane_id();
+    const uint32_t wave = wave_id();
+
+    for (int i = threadIdx.x; i < kWavesPerBlock * kRadix; i += kBlock) (&wave_count[0][0])[i] = 0;
+    // clear this tile's slice of the next active pass's status region
+    for (int d = threadIdx.x; d < kRadix; d += kBlock) next_status[static_cast<size_t>(tile) * kRadix + d] = 0;
+    __syncthreads();
+
+    // Load: wave w owns keys [w*1024, w*1024+1024) of the tile; item i of lane
+    // j is key w*1024 + i*64 + j (coalesced 512-byte rows).
+    uint64_t keys[kSortItems];
+    uint32_t ranks[kSortItems];
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+        const size_t idx = tile_base + wave * (kSortItems * kWave) + i * kWave + lane;
+        keys[i] = idx < rows ? src[idx] : ~0ull;
+    }
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+        const size_t idx = tile_base + wave * (kSortItems * kWave) + i * kWave + lane;
+        const bool valid = idx < rows;
+        const uint32_t digit = static_cast<uint32_t>(keys[i] >> shift) & (kRadix - 1);
+        uint64_t peers = ballot(valid);
+#pragma unroll
+        for (int b = 0; b < kRadixBits; ++b) {
+            const bool bit = (digit >> b) & 1u;
+            const uint64_t m = ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t before = valid ? wave_count[wave][digit] : 0;
+        const uint32_t below = lanes_below(peers);
+        ranks[i] = before + below;
+        if (valid && below == 0) wave_count[wave][digit] = before + static_cast<uint32_t>(__popcll(peers));
+    }
+    __syncthreads();
+
+    // Thread d: tile count of digit d, exclusive over waves, then look-back.
+    const uint32_t d = threadIdx.x;
+    uint32_t tile_count = 0;
+#pragma unroll
+    for (int w = 0; w < kWavesPerBlock; ++w) {
+        const uint32_t c = wave_count[w][d];
+        wave_count[w][d] = tile_count;
+        tile_count += c;
+    }
+    uint32_t tile_total;
+    const uint32_t local_start = block_exclusive_sum<uint32_t>(tile_count, &tile_total, scan_lds);
+
+    // Per-digit look-back (single thread per digit chain).
+    uint64_t* my_status = status + static_cast<size_t>(tile) * kRadix + d;
+    uint64_t exclusive = 0;
+    if (tile == 0) {
+        exclusive = desc->digit_start[pass][d];
+        store_status(my_status, kFlagPrefix | ((exclusive + tile_count) & kValueMask));
+    } else {
+        store_status(my_status, kFlagAggregate | tile_count);
+        int64_t j = static_cast<int64_t>(tile) - 1;
+        uint32_t spins = 0;
+        for (;;) {
+            const uint64_t w = load_status(status + static_cast<size_t>(j) * kRadix + d);
+            const uint64_t flag = w >> 62;
+            if (flag == 0) {
+                if (++spins >= kSpinLimit) {
+                    atomicOr(&desc->header.error, kErrLookback);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            exclusive += w & kValueMask;
+            if (flag == 2) break;
+            --j;
+            spins = 0;
+        }
+        store_status(my_status, kFlagPrefix | ((exclusive + tile_count) & kValueMask));
+    }
+    digit_base[d] = exclusive - local_start;
+    // local positions: reuse wave_count as local start per (wave, digit)
+#pragma unroll
+    for (int w = 0; w < kWavesPerBlock; ++w) wave_count[w][d] += local_start;
+    __syncthreads();
+
+    // Scatter into LDS in tile-sorted order.
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+        const size_t idx = tile_base + wave * (kSortItems * kWave) + i * kWave + lane;
+        if (idx < rows) {
+            const uint32_t digit = static_cast<uint32_t>(keys[i] >> shift) & (kRadix - 1);
+            tile_keys[wave_count[wave][digit] + ranks[i]] = keys[i];
+        }
+    }
+    __syncthreads();
+    // Write out in local order: consecutive threads write consecutive
+    // positions of the same digit run.
+    const uint32_t valid_rows = static_cast<uint32_t>(rows - tile_base < static_cast<size_t>(kSortTile)
+                                                          ? rows - tile_base
+                                                          : kSortTile);
+    for (uint32_t i = threadIdx.x; i < valid_rows; i += kBlock) {
+        const uint64_t k = tile_keys[i];
+        const uint32_t digit = static_cast<uint32_t>(k >> shift) & (kRadix - 1);
+        dst[digit_base[digit] + i] = k;
+    }
+}
+
+// Single-pass unique compaction of sorted keys (in keys_a or keys_b as the
+// sort descriptor says). Writes unique keys to `out` in order and the count.
+__global__ __launch_bounds__(kBlock) void unique_kernel(const uint64_t* keys_a, const uint64_t* keys_b,
+                                                        const SortDesc* sort_desc, int sorted_in_b,
+                                                        size_t rows, PairOutput out, uint64_t* unique_rows,
+                                                        uint64_t* status, Header* header) {
+    __shared__ uint64_t stage[kCompactTile];
+    __shared__ uint32_t scan_lds[kWavesPerBlock];
+    __shared__ uint64_t offset_slot;
+    __shared__ uint32_t tile_slot;
+    const uint64_t* keys = keys_a;
+    if (sort_desc) {
+        keys = (__popc(varying_digits(sort_desc)) & 1u) ? keys_b : keys_a;
+        rows = sort_desc->rows;
+    } else if (sorted_in_b) {
+        keys = keys_b;
+    }
+    const uint32_t tile = next_tile(&header->counters[0], &tile_slot);
+    const size_t base = static_cast<size_t>(tile) * kCompactTile;
+    if (rows == 0) {
+        if (tile == 0 && threadIdx.x == 0) *unique_rows = 0;
+        return;
+    }
+    if (base >= rows) return;
+    // blocked arrangement: thread t owns rows base + t*16 .. +16
+    uint64_t k[kCompactItems];
+    uint32_t keep = 0;
+    const size_t first = base + static_cast<size_t>(threadIdx.x) * kCompactItems;
+    uint64_t prev = 0;
+    bool have_prev = first > 0 && first - 1 < rows;
+    if (have_prev) prev = keys[first - 1];
+#pragma unroll
+    for (int i = 0; i < kCompactItems; ++i) {
+        const size_t idx = first + i;
+        const bool valid = idx < rows;
+        k[i] = valid ? keys[idx] : 0;
+        const bool flag = valid && (!have_prev || k[i] != prev);
+        keep |= static_cast<uint32_t>(flag) << i;
+        prev = k[i];
+        have_prev = valid;
+    }
+    const uint32_t count = __popc(keep);
+    uint32_t tile_total;
+    const uint32_t local = block_exclusive_sum<uint32_t>(count, &tile_total, scan_lds);
+    if (wave_id() == 0) {
+        const uint64_t excl = wave_lookback(status, tile, tile_total, 0, &header->error);
+        if (lane_id() == 0) offset_slot = excl;
+    }
+    // stage kept keys in order
+    uint32_t pos = local;
+#pragma unroll
+    for (int i = 0; i < kCompactItems; ++i)
+        if ((keep >> i) & 1u) stage[pos++] = k[i];
+    __syncthreads();
+    const uint64_t offset = offset_slot;
+    for (uint32_t i = threadIdx.x; i < tile_total; i += kBlock) store_key(out, offset + i, stage[i]);
+    const size_t tiles = tiles_of(rows, kCompactTile);
+    if (tile == tiles - 1 && threadIdx.x == 0) *unique_rows = offset + tile_total;
+}
+
+__global__ void set_u64_kernel(uint64_t* p, uint64_t v) { *p = v; }
+__global__ void set_u32_kernel(uint32_t* p, uint32_t v) { *p = v; }
+
+struct SortLayout {
+    size_t desc_off, status0_off, status1_off, unique_status_off, total;
+    size_t sort_tiles, unique_tiles;
+};
+
+SortLayout sort_layout(size_t rows) {
+    SortLayout l{};
+    l.sort_tiles = tiles_of(rows, kSortTile);
+    l.unique_tiles = tiles_of(rows, kCompactTile);
+    l.desc_off = 0;
+    size_t off = align_up(sizeof(SortDesc), 256);
+    l.status0_off = off;
+    off += align_up(l.sort_tiles * kRadix * sizeof(uint64_t), 256);
+    l.status1_off = off;
+    off += align_up(l.sort_tiles * kRadix * sizeof(uint64_t), 256);
+    l.unique_status_off = off;
+    off += align_up(l.unique_tiles * sizeof(uint64_t), 256);
+    l.total = off;
+    return l;
+}
+
+hipError_t run_sort(PairInput in, size_t rows, uint64_t* keys_a, uint64_t* keys_b, void* temp,
+                    const SortLayout& l, uint64_t known_varying, hipStream_t stream) {
+    char* base = static_cast<char*>(temp);
+    SortDesc* desc = reinterpret_cast<SortDesc*>(base + l.desc_off);
+    uint64_t* s0 = reinterpret_cast<uint64_t*>(base + l.status0_off);
+    uint64_t* s1 = reinterpret_cast<uint64_t*>(base + l.status1_off);
+    sort_init_kernel<<<1, kBlock, 0, stream>>>(desc, known_varying, in, rows);
+    const size_t hist_blocks = tiles_of(rows, kSortTile);
+    const unsigned hist_grid = static_cast<unsigned>(hist_blocks < 2048 ? (hist_blocks ? hist_blocks : 1) : 2048);
+    sort_histogram_kernel<<<hist_grid, kBlock, 0, stream>>>(in, rows, keys_a, desc, known_varying, s0,
+                                                           l.sort_tiles * kRadix);
+    sort_digit_scan_kernel<<<1, kBlock, 0, stream>>>(desc);
+    for (int p = 0; p < kPasses; ++p) {
+        if (known_varying && !((known_varying >> (p * kRadixBits)) & (kRadix - 1))) continue;
+        sort_pass_kernel<<<static_cast<unsigned>(l.sort_tiles), kBlock, 0, stream>>>(keys_a, keys_b, rows, desc,
+                                                                                    p, s0, s1);
+    }
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// merge-path union
+
+__device__ __forceinline__ size_t merge_path(const PairInput& a, size_t na, const PairInput& b, size_t nb,
+                                             size_t diag) {
+    size_t lo = diag > nb ? diag - nb : 0;
+    size_t hi = diag < na ? diag : na;
+    while (lo < hi) {
+        const size_t mid = (lo + hi) >> 1;
+        // A goes first on ties.
+        if (!(load_key(b, diag - 1 - mid) < load_key(a, mid)))
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t merge_path_lds(const uint64_t* a, uint32_t na, const uint64_t* b, uint32_t nb,
+                                                   uint32_t diag) {
+    uint32_t lo = diag > nb ? diag - nb : 0;
+    uint32_t hi = diag < na ? diag : na;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (!(b[diag - 1 - mid] < a[mid]))
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void union_kernel(PairInput left, size_t nl, PairInput right, size_t nr,
+                                                       PairOutput out, uint64_t* unique_rows, uint64_t* status,
+                                                       Header* header) {
+    __shared__ uint64_t lds[kMergeTile + 2];
+    __shared__ uint32_t scan_lds[kWavesPerBlock];
+    __shared__ size_t part[2];
+    __shared__ uint64_t offset_slot;
+    __shared__ uint32_t tile_slot;
+    nl = actual_rows(left, nl);
+    nr = actual_rows(right, nr);
+    const uint32_t tile = next_tile(&header->counters[0], &tile_slot);
+    const size_t total = nl + nr;
+    const size_t d0 = static_cast<size_t>(tile) * kMergeTile;
+    if (total == 0) {
+        if (tile == 0 && threadIdx.x == 0) *unique_rows = 0;
+        return;
+    }
+    if (d0 >= total) return;
+    const size_t d1 = d0 + kMergeTile < total ? d0 + kMergeTile : total;
+    if (threadIdx.x < 2) part[threadIdx.x] = merge_path(left, nl, right, nr, threadIdx.x ? d1 : d0);
+    __syncthreads();
+    const size_t a0 = part[0], a1 = part[1];
+    const size_t b0 = d0 - a0, b1 = d1 - a1;
+    const uint32_t na = static_cast<uint32_t>(a1 - a0), nb = static_cast<uint32_t>(b1 - b0);
+    uint64_t* la = lds;
+    uint64_t* lb = lds + na;
+    for (uint32_t i = threadIdx.x; i < na; i += kBlock) la[i] = load_key(left, a0 + i);
+    for (uint32_t i = threadIdx.x; i < nb; i += kBlock) lb[i] = load_key(right, b0 + i);
+    // element preceding the tile in merged order: the larger of the last A
+    // and last B consumed before it
+    __shared__ uint64_t prev_slot;
+    __shared__ int have_prev_slot;
+    if (threadIdx.x == 0) {
+        uint64_t p = 0;
+        int h = 0;
+        if (a0 > 0) {
+            p = load_key(left, a0 - 1);
+            h = 1;
+        }
+        if (b0 > 0) {
+            const uint64_t q = load_key(right, b0 - 1);
+            if (!h || q > p) p = q;
+            h = 1;
+        }
+        prev_slot = p;
+        have_prev_slot = h;
+    }
+    __syncthreads();
+    // Each thread merges kMergeItems consecutive outputs.
+    const uint32_t n = na + nb;
+    const uint32_t diag = threadIdx.x * kMergeItems < n ? threadIdx.x * kMergeItems : n;
+    uint32_t ia = merge_path_lds(la, na, lb, nb, diag);
+    uint32_t ib = diag - ia;
+    uint64_t v[kMergeItems];
+#pragma unroll
+    for (int i = 0; i < kMergeItems; ++i) {
+        const bool take_a = ib >= nb || (ia < na && la[ia] <= lb[ib]);
+        if (diag + i < n) {
+            v[i] = take_a ? la[ia] : lb[ib];
+            if (take_a)
+                ++ia;
+            else
+                ++ib;
+        } else {
+            v[i] = 0;
+        }
+    }
+    __syncthreads();
+    // write merged run to LDS to find each element's predecessor
+#pragma unroll
+    for (int i = 0; i < kMergeItems; ++i)
+        if (diag + i < n) lds[diag + i] = v[i];
+    __syncthreads();
+    uint32_t keep = 0;
+#pragma unroll
+    for (int i = 0; i < kMergeItems; ++i) {
+        const uint32_t pos = diag + i;
+        if (pos < n) {
+            bool flag;
+            if (pos == 0)
+                flag = !have_prev_slot || v[i] != prev_slot;
+            else
+                flag = v[i] != lds[pos - 1];
+            keep |= static_cast<uint32_t>(flag) << i;
+        }
+    }
+    const uint32_t count = __popc(keep);
+    uint32_t tile_total;
+    const uint32_t local = block_exclusive_sum<uint32_t>(count, &tile_total, scan_lds);
+    if (wave_id() == 0) {
+        const uint64_t excl = wave_lookback(status, tile, tile_total, 0, &header->error);
+        if (lane_id() == 0) offset_slot = excl;
+    }
+    // compact in LDS (reads of lds[] above are complete after the scan's barriers)
+    uint32_t pos = local;
+#pragma unroll
+    for (int i = 0; i < kMergeItems; ++i)
+        if ((keep >> i) & 1u) lds[pos++] = v[i];
+    __syncthreads();
+    const uint64_t offset = offset_slot;
+    for (uint32_t i = threadIdx.x; i < tile_total; i += kBlock) store_key(out, offset + i, lds[i]);
+    if (tile == tiles_of(total, kMergeTile) - 1 && threadIdx.x == 0) *unique_rows = offset + tile_total;
+}
+
+// ------------------------------------------------------------------------
+// anti-join
+
+__device__ __forceinline__ size_t lower_bound_key(const PairInput& r, size_t lo, size_t hi, uint64_t key) {
+    while (lo < hi) {
+        const size_t mid = (lo + hi) >> 1;
+        if (load_key(r, mid) < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void anti_join_kernel(PairInput left, size_t nl, PairInput right, size_t nr,
+                                                           PairOutput out, uint32_t* count32, uint64_t* count64,
+                                                           uint64_t* status, Header* header) {
+    constexpr int kItems = kAntiTile / kBlock;  // 8
+    __shared__ uint64_t stage[kAntiStage];
+    __shared__ uint64_t outbuf[kAntiTile];
+    __shared__ uint32_t scan_lds[kWavesPerBlock];
+    __shared__ size_t range[2];
+    __shared__ uint64_t offset_slot;
+    __shared__ uint32_t tile_slot;
+    nl = actual_rows(left, nl);
+    nr = actual_rows(right, nr);
+    const uint32_t tile = next_tile(&header->counters[0], &tile_slot);
+    const size_t base = static_cast<size_t>(tile) * kAntiTile;
+    if (nl == 0) {
+        if (tile == 0 && threadIdx.x == 0) {
+            if (count32) *count32 = 0;
+            if (count64) *count64 = 0;
+        }
+        return;
+    }
+    if (base >= nl) return;
+    const uint32_t rows = static_cast<uint32_t>(nl - base < static_cast<size_t>(kAntiTile) ? nl - base : kAntiTile);
+    // blocked arrangement: thread t owns left rows t*8 .. t*8+7 of the tile
+    uint64_t k[kItems];
+    uint64_t kmin = ~0ull, kmax = 0;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        const uint32_t li = threadIdx.x * kItems + i;
+        k[i] = li < rows ? load_key(left, base + li) : 0;
+        if (li < rows) {
+            kmin = k[i] < kmin ? k[i] : kmin;
+            kmax = k[i] > kmax ? k[i] : kmax;
+        }
+    }
+    // The tile's key range bounds the right rows that can match (min/max
+    // rather than first/last keeps pure membership semantics for any order).
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const uint64_t omin = __shfl_xor(kmin, off, kWave), omax = __shfl_xor(kmax, off, kWave);
+        kmin = omin < kmin ? omin : kmin;
+        kmax = omax > kmax ? omax : kmax;
+    }
+    __shared__ uint64_t red_min[kWavesPerBlock], red_max[kWavesPerBlock];
+    if (lane_id() == 0) {
+        red_min[wave_id()] = kmin;
+        red_max[wave_id()] = kmax;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        uint64_t lo_key = ~0ull, hi_key = 0;
+        for (int w = 0; w < kWavesPerBlock; ++w) {
+            lo_key = red_min[w] < lo_key ? red_min[w] : lo_key;
+            hi_key = red_max[w] > hi_key ? red_max[w] : hi_key;
+        }
+        if (threadIdx.x == 0) {
+            range[0] = lower_bound_key(right, 0, nr, lo_key);
+        } else {
+            const size_t lb = lower_bound_key(right, 0, nr, hi_key);
+            range[1] = (lb < nr && load_key(right, lb) == hi_key) ? lb + 1 : lb;
+        }
+    }
+    __syncthreads();
+    const size_t r0 = range[0];
+    const size_t r1 = range[1] > r0 ? range[1] : r0;
+    const size_t span = r1 - r0;
+    const bool staged = span <= static_cast<size_t>(kAntiStage);
+    if (staged)
+        for (uint32_t i = threadIdx.x; i < span; i += kBlock) stage[i] = load_key(right, r0 + i);
+    __syncthreads();
+    uint32_t keep = 0;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        const uint32_t li = threadIdx.x * kItems + i;
+        if (li < rows) {
+            bool found;
+            if (staged) {
+                uint32_t lo = 0, hi = static_cast<uint32_t>(span);
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (stage[mid] < k[i])
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                found = lo < span && stage[lo] == k[i];
+            } else {
+                const size_t p = lower_bound_key(right, r0, r1, k[i]);
+                found = p < r1 && load_key(right, p) == k[i];
+            }
+            keep |= static_cast<uint32_t>(!found) << i;
+        }
+    }
+    const uint32_t count = __popc(keep);
+    uint32_t tile_total;
+    const uint32_t local = block_exclusive_sum<uint32_t>(count, &tile_total, scan_lds);
+    if (wave_id() == 0) {
+        const uint64_t excl = wave_lookback(status, tile, tile_total, 0, &header->error);
+        if (lane_id() == 0) offset_slot = excl;
+    }
+    uint32_t pos = local;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i)
+        if ((keep >> i) & 1u) outbuf[pos++] = k[i];
+    __syncthreads();
+    const uint64_t offset = offset_slot;
+    for (uint32_t i = threadIdx.x; i < tile_total; i += kBlock) store_key(out, offset + i, outbuf[i]);
+    if (tile == tiles_of(nl, kAntiTile) - 1 && threadIdx.x == 0) {
+        if (count32) *count32 = static_cast<uint32_t>(offset + tile_total);
+        if (count64) *count64 = offset + tile_total;
+    }
+}
+
+// ------------------------------------------------------------------------
+// join (range-index layout: keys[K] sorted unique, starts[K+1], rows[R])
+
+__device__ __forceinline__ int64_t find_index_key(const uint32_t* keys, size_t k, uint32_t key) {
+    size_t lo = 0, hi = k;
+    while (lo < hi) {
+        const size_t mid = (lo + hi) >> 1;
+        if (keys[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return (lo < k && keys[lo] == key) ? static_cast<int64_t>(lo) : -1;
+}
+
+__global__ __launch_bounds__(kBlock) void join_count_kernel(const uint32_t* left_keys, size_t nl,
+                                                            const uint32_t* index_keys, const uint32_t* index_starts,
+                                                            size_t k, uint64_t* counts, uint64_t* offsets,
+                                                            uint64_t* total, uint64_t* status, Header* header) {
+    constexpr int kItems = 8;
+    constexpr int kTile = kBlock * kItems;
+    __shared__ uint64_t scan_lds[kWavesPerBlock];
+    __shared__ uint64_t offset_slot;
+    __shared__ uint32_t tile_slot;
+    const uint32_t tile = next_tile(&header->counters[0], &tile_slot);
+    const size_t base = static_cast<size_t>(tile) * kTile;
+    // striped arrangement for coalescing: item i of thread t is row base + i*256 + t
+    uint64_t c[kItems];
+    uint64_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        const size_t row = base + static_cast<size_t>(i) * kBlock + threadIdx.x;
+        c[i] = 0;
+        if (row < nl) {
+            const int64_t idx = find_index_key(index_keys, k, left_keys[row]);
+            if (idx >= 0) c[i] = index_starts[idx + 1] - index_starts[idx];
+            counts[row] = c[i];
+        }
+    }
+    // exclusive scan in row order: rows of item i precede rows of item i+1
+    uint64_t running = 0;
+    for (int i = 0; i < kItems; ++i) {
+        uint64_t part_total;
+        const uint64_t ex = block_exclusive_sum<uint64_t>(c[i], &part_total, scan_lds);
+        c[i] = running + ex;
+        running += part_total;
+    }
+    sum = running;
+    if (wave_id() == 0) {
+        const uint64_t excl = wave_lookback(status, tile, sum, 0, &header->error);
+        if (lane_id() == 0) offset_slot = excl;
+    }
+    __syncthreads();
+    const uint64_t offset = offset_slot;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        const size_t row = base + static_cast<size_t>(i) * kBlock + threadIdx.x;
+        if (row < nl) offsets[row] = offset + c[i];
+    }
+    if (tile == tiles_of(nl, kTile) - 1 && threadIdx.x == 0) *total = offset + sum;
+}
+
+// Output-balanced emit: block b writes output rows [b*T, (b+1)*T). Runs over a
+// fixed grid; the output size comes from the device (last offset + count).
+__global__ __launch_bounds__(kBlock) void join_emit_kernel(const uint32_t* left_keys, size_t nl,
+                                                           const uint32_t* index_keys, const uint32_t* index_starts,
+                                                           const uint32_t* index_rows, size_t k,
+                                                           const uint32_t* p0, uint32_t side0, const uint32_t* p1,
+                                                           uint32_t side1, const uint64_t* offsets, uint32_t* out0,
+                                                           uint32_t* out1) {
+    __shared__ uint64_t row_offset[kEmitStage + 1];
+    __shared__ uint32_t row_match[kEmitStage];
+    __shared__ size_t bounds[2];
+    __shared__ uint64_t total_slot;
+    if (threadIdx.x == 0) {
+        const int64_t idx = find_index_key(index_keys, k, left_keys[nl - 1]);
+        const uint64_t last = idx >= 0 ? index_starts[idx + 1] - index_starts[idx] : 0;
+        total_slot = offsets[nl - 1] + last;
+    }
+    __syncthreads();
+    const uint64_t total = total_slot;
+    const uint64_t tiles = (total + kEmitTile - 1) / kEmitTile;
+    for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const uint64_t o0 = tile * kEmitTile;
+        const uint64_t o1 = o0 + kEmitTile < total ? o0 + kEmitTile : total;
+        if (threadIdx.x < 2) {
+            // first row whose range contains o (upper_bound(offsets, o) - 1)
+            const uint64_t o = threadIdx.x ? o1 - 1 : o0;
+            size_t lo = 0, hi = nl;
+            while (lo < hi) {
+                const size_t mid = (lo + hi) >> 1;
+                if (offsets[mid] <= o)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            bounds[threadIdx.x] = lo - 1;
+        }
+        __syncthreads();
+        const size_t l0 = bounds[0], l1 = bounds[1] + 1;  // rows [l0, l1)
+        const size_t span = l1 - l0;
+        const bool staged = span <= static_cast<size_t>(kEmitStage);
+        if (staged) {
+            for (uint32_t i = threadIdx.x; i < span; i += kBlock) {
+                const size_t row = l0 + i;
+                row_offset[i] = offsets[row];
+                const int64_t idx = find_index_key(index_keys, k, left_keys[row]);
+                row_match[i] = idx >= 0 ? index_starts[idx] : 0;
+            }
+        }
+        __syncthreads();
+        for (uint64_t o = o0 + threadIdx.x; o < o1; o += kBlock) {
+            size_t row;
+            uint64_t row_start;
+            uint32_t match_start;
+            if (staged) {
+                uint32_t lo = 0, hi = static_cast<uint32_t>(span);
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (row_offset[mid] <= o)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                row = l0 + lo - 1;
+                row_start = row_offset[lo - 1];
+                match_start = row_match[lo - 1];
+            } else {
+                size_t lo = l0, hi = l1;
+                while (lo < hi) {
+                    const size_t mid = (lo + hi) >> 1;
+                    if (offsets[mid] <= o)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                row = lo - 1;
+                row_start = offsets[row];
+                match_start = index_starts[find_index_key(index_keys, k, left_keys[row])];
+            }
+            const uint32_t right_row = index_rows[match_start + static_cast<uint32_t>(o - row_start)];
+            out0[o] = side0 == 0 ? p0[row] : p0[right_row];
+            out1[o] = side1 == 0 ? p1[row] : p1[right_row];
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------
+// filter
+
+__device__ __forceinline__ bool compare_u32(uint32_t value, uint32_t op, uint32_t operand) {
+    switch (op) {
+        case 0: return value == operand;
+        case 1: return value != operand;
+        case 2: return value < operand;
+        case 3: return value <= operand;
+        case 4: return value > operand;
+        case 5: return value >= operand;
+        default: return false;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void filter_kernel(const uint32_t* input, size_t n, uint32_t op,
+                                                        uint32_t operand, uint32_t* out_rows, uint32_t* count,
+                                                        uint64_t* status, Header* header) {
+    __shared__ uint32_t stage[kCompactTile];
+    __shared__ uint32_t wave_totals[kWavesPerBlock][kCompactItems];
+    __shared__ uint64_t offset_slot;
+    __shared__ uint32_t tile_slot;
+    const uint32_t tile = next_tile(&header->counters[0], &tile_slot);
+    const size_t base = static_cast<size_t>(tile) * kCompactTile;
+    const uint32_t lane = lane_id(), wave = wave_id();
+    // wave w owns rows [w*1024, w*1024+1024) of the tile; item i of lane j is
+    // row w*1024 + i*64 + j. Ballots rank matches in row order per item.
+    uint64_t masks[kCompactItems];
+    uint32_t wave_count = 0;
+#pragma unroll
+    for (int i = 0; i < kCompactItems; ++i) {
+        const size_t row = base + wave * (kCompactItems * kWave) + i * kWave + lane;
+        const bool hit = row < n && compare_u32(input[row], op, operand);
+        masks[i] = ballot(hit);
+        if (lane == 0) wave_totals[wave][i] = static_cast<uint32_t>(__popcll(masks[i]));
+        wave_count += static_cast<uint32_t>(__popcll(masks[i]));
+    }
+    __syncthreads();
+    // tile-local start of each wave
+    uint32_t wave_start = 0, tile_total = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w) {
+        uint32_t s = 0;
+        for (int i = 0; i < kCompactItems; ++i) s += wave_totals[w][i];
+        if (w < static_cast<int>(wave)) wave_start += s;
+        tile_total += s;
+    }
+    if (wave == 0) {
+        const uint64_t excl = wave_lookback(status, tile, tile_total, 0, &header->error);
+        if (lane == 0) offset_slot = excl;
+    }
+    uint32_t pos = wave_start;
+#pragma unroll
+    for (int i = 0; i < kCompactItems; ++i) {
+        const size_t row = base + wave * (kCompactItems * kWave) + i * kWave + lane;
+        if ((masks[i] >> lane) & 1ull) stage[pos + lanes_below(masks[i])] = static_cast<uint32_t>(row);
+        pos += static_cast<uint32_t>(__popcll(masks[i]));
+    }
+    __syncthreads();
+    const uint64_t offset = offset_slot;
+    for (uint32_t i = threadIdx.x; i < tile_total; i += kBlock) out_rows[offset + i] = stage[i];
+    if (tile == tiles_of(n, kCompactTile) - 1 && threadIdx.x == 0) *count = static_cast<uint32_t>(offset + tile_total);
+}
+
+// ------------------------------------------------------------------------
+// small kernels
+
+__global__ void add_one_kernel(int32_t* data, size_t len) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < len) data[i] += 1;
+}
+
+__global__ void fill_mod_kernel(uint32_t* out, size_t len, uint32_t modulus) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < len; i += stride)
+        out[i] = static_cast<uint32_t>(i % modulus);
+}
+
+__global__ void fill_u64_kernel(uint64_t* out, size_t len, uint64_t value) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < len; i += stride) out[i] = value;
+}
+
+unsigned grid_for(size_t n, size_t per_block, unsigned cap) {
+    const size_t b = tiles_of(n, per_block);
+    return static_cast<unsigned>(b == 0 ? 1 : (b < cap ? b : cap));
+}
+
+// Scratch of single-pass compaction kernels: header + one status word per tile.
+size_t compact_temp(size_t tiles) { return kTempHeader + align_up(tiles * sizeof(uint64_t), 256); }
+
+hipError_t prepare_compact(void* temp, size_t tiles, hipStream_t stream) {
+    clear_header_kernel<<<1, 64, 0, stream>>>(static_cast<Header*>(temp));
+    return clear_u64(reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), tiles, stream);
+}
+
+}  // namespace
+
+// ========================================================================
+// launchers
+
+size_t distinct_temp_bytes(size_t rows) {
+    const SortLayout l = sort_layout(rows);
+    return kTempHeader + l.total;
+}
+
+hipError_t radix_sort(PairInput in, size_t rows, uint64_t* keys_a, uint64_t* keys_b, void* temp,
+                      size_t temp_bytes, uint64_t known_varying, hipStream_t stream, uint64_t** sorted_host) {
+    const SortLayout l = sort_layout(rows);
+    if (temp_bytes < kTempHeader + l.total) return hipErrorInvalidValue;
+    if (rows == 0) return hipSuccess;
+    char* sort_temp = static_cast<char*>(temp) + kTempHeader;
+    hipError_t st = run_sort(in, rows, keys_a, keys_b, sort_temp, l, known_varying, stream);
+    if (sorted_host && known_varying) {
+        int active = 0;
+        for (int p = 0; p < kPasses; ++p)
+            if ((known_varying >> (p * kRadixBits)) & (kRadix - 1)) ++active;
+        *sorted_host = (active & 1) ? keys_b : keys_a;
+    }
+    return st;
+}
+
+hipError_t distinct(PairInput in, size_t rows, uint64_t* keys_a, uint64_t* keys_b, PairOutput out,
+                    uint64_t* unique_rows, void* temp, size_t temp_bytes, uint64_t known_varying,
+                    hipStream_t stream) {
+    if (rows == 0) {
+        set_u64_kernel<<<1, 1, 0, stream>>>(unique_rows, 0);
+        return hipGetLastError();
+    }
+    const SortLayout l = sort_layout(rows);
+    if (temp_bytes < kTempHeader + l.total) return hipErrorInvalidValue;
+    char* sort_temp = static_cast<char*>(temp) + kTempHeader;
+    hipError_t st = run_sort(in, rows, keys_a, keys_b, sort_temp, l, known_varying, stream);
+    if (st != hipSuccess) return st;
+    Header* header = static_cast<Header*>(temp);
+    uint64_t* ustatus = reinterpret_cast<uint64_t*>(sort_temp + l.unique_status_off);
+    clear_header_kernel<<<1, 64, 0, stream>>>(header);
+    st = clear_u64(ustatus, l.unique_tiles, stream);
+    if (st != hipSuccess) return st;
+    const SortDesc* desc = reinterpret_cast<const SortDesc*>(sort_temp + l.desc_off);
+    unique_kernel<<<static_cast<unsigned>(l.unique_tiles), kBlock, 0, stream>>>(keys_a, keys_b, desc, 0, rows, out,
+                                                                              unique_rows, ustatus, header);
+    return hipGetLastError();
+}
+
+size_t union_temp_bytes(size_t left_rows, size_t right_rows) {
+    return compact_temp(tiles_of(left_rows + right_rows, kMergeTile));
+}
+
+hipError_t sorted_union(PairInput left, size_t left_rows, PairInput right, size_t right_rows, PairOutput out,
+                        uint64_t* unique_rows, void* temp, size_t temp_bytes, hipStream_t stream) {
+    const size_t total = left_rows + right_rows;
+    if (total == 0) {
+        set_u64_kernel<<<1, 1, 0, stream>>>(unique_rows, 0);
+        return hipGetLastError();
+    }
+    const size_t tiles = tiles_of(total, kMergeTile);
+    if (temp_bytes < compact_temp(tiles)) return hipErrorInvalidValue;
+    hipError_t st = prepare_compact(temp, tiles, stream);
+    if (st != hipSuccess) return st;
+    union_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(
+        left, left_rows, right, right_rows, out, unique_rows,
+        reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp));
+    return hipGetLastError();
+}
+
+size_t anti_join_temp_bytes(size_t left_rows) { return compact_temp(tiles_of(left_rows, kAntiTile)); }
+
+hipError_t sorted_anti_join(PairInput left, size_t left_rows, PairInput right, size_t right_rows, PairOutput out,
+                            uint32_t* count32, uint64_t* count64, void* temp, size_t temp_bytes,
+                            hipStream_t stream) {
+    if (left_rows == 0) {
+        if (count32) set_u32_kernel<<<1, 1, 0, stream>>>(count32, 0);
+        if (count64) set_u64_kernel<<<1, 1, 0, stream>>>(count64, 0);
+        return hipGetLastError();
+    }
+    const size_t tiles = tiles_of(left_rows, kAntiTile);
+    if (temp_bytes < compact_temp(tiles)) return hipErrorInvalidValue;
+    hipError_t st = prepare_compact(temp, tiles, stream);
+    if (st != hipSuccess) return st;
+    anti_join_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(
+        left, left_rows, right, right_rows, out, count32, count64,
+        reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp));
+    return hipGetLastError();
+}
+
+size_t join_temp_bytes(size_t left_rows) { return compact_temp(tiles_of(left_rows, kBlock * 8)); }
+
+hipError_t join_count(const uint32_t* left_keys, size_t left_rows, const uint32_t* index_keys,
+                      const uint32_t* index_starts, size_t unique_keys, uint64_t* counts, uint64_t* offsets,
+                      uint64_t* total, void* temp, size_t temp_bytes, hipStream_t stream) {
+    if (left_rows == 0) {
+        set_u64_kernel<<<1, 1, 0, stream>>>(total, 0);
+        return hipGetLastError();
+    }
+    const size_t tiles = tiles_of(left_rows, kBlock * 8);
+    if (temp_bytes < compact_temp(tiles)) return hipErrorInvalidValue;
+    hipError_t st = prepare_compact(temp, tiles, stream);
+    if (st != hipSuccess) return st;
+    join_count_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(
+        left_keys, left_rows, index_keys, index_starts, unique_keys, counts, offsets, total,
+        reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp));
+    return hipGetLastError();
+}
+
+hipError_t join_emit(const uint32_t* left_keys, size_t left_rows, const uint32_t* index_keys,
+                     const uint32_t* index_starts, const uint32_t* index_rows, size_t unique_keys,
+                     const uint32_t* projection0, uint32_t side0, const uint32_t* projection1, uint32_t side1,
+                     const uint64_t* offsets, uint32_t* out0, uint32_t* out1, hipStream_t stream) {
+    if (left_rows == 0) return hipSuccess;
+    join_emit_kernel<<<2048, kBlock, 0, stream>>>(left_keys, left_rows, index_keys, index_starts, index_rows,
+                                                 unique_keys, projection0, side0, projection1, side1, offsets,
+                                                 out0, out1);
+    return hipGetLastError();
+}
+
+size_t filter_temp_bytes(size_t rows) { return compact_temp(tiles_of(rows, kCompactTile)); }
+
+hipError_t filter(const uint32_t* input, size_t rows, uint32_t op, uint32_t operand, uint32_t* out_rows,
+                  uint32_t* count, void* temp, size_t temp_bytes, hipStream_t stream) {
+    if (rows == 0) {
+        set_u32_kernel<<<1, 1, 0, stream>>>(count, 0);
+        return hipGetLastError();
+    }
+    const size_t tiles = tiles_of(rows, kCompactTile);
+    if (temp_bytes < compact_temp(tiles)) return hipErrorInvalidValue;
+    hipError_t st = prepare_compact(temp, tiles, stream);
+    if (st != hipSuccess) return st;
+    filter_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(
+        input, rows, op, operand, out_rows, count,
+        reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp));
+    return hipGetLastError();
+}
+
+hipError_t add_one_i32(int32_t* data, size_t len, hipStream_t stream) {
+    if (len == 0) return hipSuccess;
+    add_one_kernel<<<static_cast<unsigned>(tiles_of(len, kBlock)), kBlock, 0, stream>>>(data, len);
+    return hipGetLastError();
+}
+
+hipError_t fill_mod_u32(uint32_t* out, size_t len, uint32_t modulus, hipStream_t stream) {
+    if (len == 0) return hipSuccess;
+    fill_mod_kernel<<<grid_for(len, kBlock * 4, 8192), kBlock, 0, stream>>>(out, len, modulus);
+    return hipGetLastError();
+}
+
+hipError_t fill_u64(uint64_t* out, size_t len, uint64_t value, hipStream_t stream) {
+    if (len == 0) return hipSuccess;
+    fill_u64_kernel<<<grid_for(len, kBlock * 4, 8192), kBlock, 0, stream>>>(out, len, value);
+    return hipGetLastError();
+}
+
+hipError_t set_u32(uint32_t* out, uint32_t value, hipStream_t stream) {
+    set_u32_kernel<<<1, 1, 0, stream>>>(out, value);
+    return hipGetLastError();
+}
+
+hipError_t set_u64(uint64_t* out, uint64_t value, hipStream_t stream) {
+    set_u64_kernel<<<1, 1, 0, stream>>>(out, value);
+    return hipGetLastError();
+}
+
+}  // namespace spk

@@ -1,0 +1,90 @@
+// Host-side launchers of the gfx950 operator kernels (namespace spk). All
+// launchers are asynchronous on `stream`, allocate nothing, and carve their
+// scratch from a caller-provided temporary buffer whose size the matching
+// *_temp_bytes function returns. Row counts are device-resident.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace spk {
+
+// Key sources for the sort and set operators: packed u64 keys (first<<32 |
+// second) or two u32 columns (structure of arrays).
+// `rows_dev`, when non-null, holds the actual row count on the device; the
+// host-side row count passed beside the input is then an upper bound used
+// only to size the launch.
+struct PairInput {
+    const uint64_t* packed;   // used when non-null
+    const uint32_t* first;
+    const uint32_t* second;
+    const uint64_t* rows_dev;
+};
+
+struct PairOutput {
+    uint64_t* packed;   // written when non-null
+    uint32_t* first;    // written when non-null
+    uint32_t* second;
+};
+
+// ---- radix sort + unique (distinct) ----
+size_t distinct_temp_bytes(size_t rows);
+// Sort `rows` keys (from `in`) using keys_a / keys_b (rows entries each) as
+// ping-pong buffers, drop adjacent duplicates, write the unique keys to `out`
+// and the unique count to *unique_rows (u64, device). `known_varying` is a
+// host-side superset of the key bits that differ between keys (0 = unknown:
+// the kernels find it on the device).
+hipError_t distinct(PairInput in, size_t rows, uint64_t* keys_a, uint64_t* keys_b,
+                    PairOutput out, uint64_t* unique_rows, void* temp, size_t temp_bytes,
+                    uint64_t known_varying, hipStream_t stream);
+// Sort only (stable LSD); the sorted keys end in keys_a or keys_b: the
+// returned pointer names which when known_varying != 0, else the device
+// flag word at `temp` holds it and sorted_out (device u64*) is written.
+hipError_t radix_sort(PairInput in, size_t rows, uint64_t* keys_a, uint64_t* keys_b,
+                      void* temp, size_t temp_bytes, uint64_t known_varying, hipStream_t stream,
+                      uint64_t** sorted_host);
+
+// ---- sorted set operators ----
+size_t union_temp_bytes(size_t left_rows, size_t right_rows);
+hipError_t sorted_union(PairInput left, size_t left_rows, PairInput right, size_t right_rows,
+                        PairOutput out, uint64_t* unique_rows, void* temp, size_t temp_bytes,
+                        hipStream_t stream);
+
+size_t anti_join_temp_bytes(size_t left_rows);
+// left \ right; `count32`/`count64` receive the output row count (either may
+// be null).
+hipError_t sorted_anti_join(PairInput left, size_t left_rows, PairInput right, size_t right_rows,
+                            PairOutput out, uint32_t* count32, uint64_t* count64, void* temp,
+                            size_t temp_bytes, hipStream_t stream);
+
+// ---- indexed join (U32RangeIndex layout) ----
+size_t join_temp_bytes(size_t left_rows);
+hipError_t join_count(const uint32_t* left_keys, size_t left_rows, const uint32_t* index_keys,
+                      const uint32_t* index_starts, size_t unique_keys, uint64_t* counts,
+                      uint64_t* offsets, uint64_t* total, void* temp, size_t temp_bytes,
+                      hipStream_t stream);
+hipError_t join_emit(const uint32_t* left_keys, size_t left_rows, const uint32_t* index_keys,
+                     const uint32_t* index_starts, const uint32_t* index_rows, size_t unique_keys,
+                     const uint32_t* projection0, uint32_t side0, const uint32_t* projection1,
+                     uint32_t side1, const uint64_t* offsets, uint32_t* out0, uint32_t* out1,
+                     hipStream_t stream);
+
+// ---- filter / compaction ----
+size_t filter_temp_bytes(size_t rows);
+hipError_t filter(const uint32_t* input, size_t rows, uint32_t op, uint32_t operand,
+                  uint32_t* out_rows, uint32_t* count, void* temp, size_t temp_bytes,
+                  hipStream_t stream);
+
+// ---- small kernels ----
+hipError_t add_one_i32(int32_t* data, size_t len, hipStream_t stream);
+hipError_t fill_mod_u32(uint32_t* out, size_t len, uint32_t modulus, hipStream_t stream);
+hipError_t fill_u64(uint64_t* out, size_t len, uint64_t value, hipStream_t stream);
+hipError_t set_u32(uint32_t* out, uint32_t value, hipStream_t stream);
+hipError_t set_u64(uint64_t* out, uint64_t value, hipStream_t stream);
+
+// Device error word of a temporary buffer (look-back failure or capacity
+// overflow bits); every launcher reserves it at offset 0 of `temp`.
+constexpr size_t kTempHeader = 256;
+
+}  // namespace spk

@@ -1,0 +1,111 @@
+"""Seeded synthetic workloads (include/sparkalog_workload.h) as numpy arrays.
+
+The five configurations of the binary pipeline's benchmark:
+  random_dag         - plain TC input: DAG of 1,000 nodes, 4,000 edges
+  component_graph    - single-instance TC: 2^20 nodes in Zipf(1.5) components
+  operator inputs    - pairs / distinct_input / overlapping_sets
+  batch_graphs       - 4,096 independent instances of uneven size
+  (served requests reuse component_graph with a per-request seed)
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native
+
+C1_SEED = 0xC1
+C2_SEED = 0xC2
+C3_SEED = 0xC3
+C4_SEED = 0xC4
+C5_SEED = 0xC5
+
+
+def _take(ptr, n, dtype) -> np.ndarray:
+    lib = _native.workload()
+    arr = np.ctypeslib.as_array(ptr, shape=(max(n, 1),))[:n].astype(dtype, copy=True)
+    lib.sparkalog_wl_free(ctypes.cast(ptr, ctypes.c_void_p))
+    return arr
+
+
+def random_dag(seed: int = C1_SEED, nodes: int = 1000, edges: int = 4000) -> tuple[np.ndarray, np.ndarray]:
+    lib = _native.workload()
+    src = np.empty(edges, dtype=np.uint32)
+    dst = np.empty(edges, dtype=np.uint32)
+    rc = lib.sparkalog_wl_random_dag(seed, nodes, edges, src.ctypes.data_as(ctypes.c_void_p),
+                                     dst.ctypes.data_as(ctypes.c_void_p))
+    if rc != 0:
+        raise ValueError(f"cannot draw {edges} distinct edges over {nodes} nodes")
+    return src, dst
+
+
+def component_graph(seed: int = C2_SEED, nodes: int = 1 << 20, zipf_s: float = 1.5, size_min: int = 8,
+                    size_max: int = 4096, mean_out_degree: float = 2.0):
+    """Sorted unique edges (src, dst) and the component count."""
+    lib = _native.workload()
+    s = _native.u32p()
+    d = _native.u32p()
+    comps = ctypes.c_uint32(0)
+    n = lib.sparkalog_wl_component_graph(seed, nodes, zipf_s, size_min, size_max, mean_out_degree,
+                                         ctypes.byref(s), ctypes.byref(d), ctypes.byref(comps))
+    if n == 0xFFFFFFFFFFFFFFFF:
+        raise MemoryError("component graph generation failed")
+    return _take(s, n, np.uint32), _take(d, n, np.uint32), comps.value
+
+
+def batch_graphs(seed: int = C4_SEED, instances: int = 4096, n_min: int = 16, n_max: int = 16384,
+                 lambda_min: float = 1.2, lambda_max: float = 3.0):
+    """(src, dst, offsets[instances+1], node_counts[instances]); local ids."""
+    lib = _native.workload()
+    s = _native.u32p()
+    d = _native.u32p()
+    o = _native.u64p()
+    c = _native.u32p()
+    n = lib.sparkalog_wl_batch_graphs(seed, instances, n_min, n_max, lambda_min, lambda_max,
+                                      ctypes.byref(s), ctypes.byref(d), ctypes.byref(o), ctypes.byref(c))
+    if n == 0xFFFFFFFFFFFFFFFF:
+        raise MemoryError("batch generation failed")
+    return (_take(s, n, np.uint32), _take(d, n, np.uint32), _take(o, instances + 1, np.uint64),
+            _take(c, instances, np.uint32))
+
+
+KEY_UNIFORM = 0
+KEY_ZIPF = 1
+
+
+def pairs(seed: int, rows: int, key_mode: int = KEY_UNIFORM, key_space: int = 1 << 24, zipf_s: float = 1.1):
+    lib = _native.workload()
+    a = np.empty(rows, dtype=np.uint32)
+    b = np.empty(rows, dtype=np.uint32)
+    lib.sparkalog_wl_pairs(seed, rows, key_mode, key_space, zipf_s, a.ctypes.data_as(ctypes.c_void_p),
+                           b.ctypes.data_as(ctypes.c_void_p))
+    return a, b
+
+
+def distinct_input(seed: int, rows: int, unique_fraction: float = 0.7, key_mode: int = KEY_UNIFORM,
+                   key_space: int = 1 << 24, zipf_s: float = 1.1):
+    lib = _native.workload()
+    a = np.empty(rows, dtype=np.uint32)
+    b = np.empty(rows, dtype=np.uint32)
+    lib.sparkalog_wl_distinct_input(seed, rows, unique_fraction, key_mode, key_space, zipf_s,
+                                    a.ctypes.data_as(ctypes.c_void_p), b.ctypes.data_as(ctypes.c_void_p))
+    return a, b
+
+
+def overlapping_sets(seed: int, rows: int, overlap: float = 0.5):
+    lib = _native.workload()
+    arrs = [np.empty(rows, dtype=np.uint32) for _ in range(4)]
+    lib.sparkalog_wl_overlapping_sets(seed, rows, overlap, *[x.ctypes.data_as(ctypes.c_void_p) for x in arrs])
+    return (arrs[0], arrs[1]), (arrs[2], arrs[3])
+
+
+def pair_digest(first, second) -> int:
+    a = np.ascontiguousarray(first, dtype=np.uint32)
+    b = np.ascontiguousarray(second, dtype=np.uint32)
+    return _native.workload().sparkalog_wl_pair_digest(a.ctypes.data_as(ctypes.c_void_p),
+                                                       b.ctypes.data_as(ctypes.c_void_p), a.size)
+
+
+def mix64(key: int) -> int:
+    return _native.workload().sparkalog_wl_mix64(key)

@@ -1,0 +1,141 @@
+"""Device-resident semi-naive executor on the GPU against the oracle:
+the reference's recursion tests, plain TC on the 1,000-node DAG (exact
+arrays), and single-instance TC on component graphs (exact arrays at 2^14
+nodes; row count, order-independent digest, strict sortedness and iteration
+count against brute-force BFS fixtures at 2^17 and 2^20 nodes)."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import chain, cols, pairs
+from sparkalog_amd import plans, workload
+from sparkalog_amd.recursion import GenericExecutionError, RecursiveExecutor, RelationStore
+
+pytestmark = pytest.mark.gpu
+
+R = plans.RelationId
+
+
+def rule(target, delta, right):
+    return plans.RecursiveRulePlan(target=R(target), delta_input=R(delta), right_input=R(right),
+                                   join=plans.path_join_plan())
+
+
+def test_reference_recursion_vectors(golden, stream):
+    for case in golden["recursion"]:
+        store = RelationStore(stream)
+        for rid, rel in case["static"].items():
+            store.insert_static(R(int(rid)), cols(rel))
+        for rid, st in case["recursive"].items():
+            store.insert_recursive(R(int(rid)), cols(st["full"]), cols(st["delta"]))
+        plan = plans.RecursiveSccPlan(tuple(R(r) for r in case["relations"]),
+                                      tuple(rule(r["target"], r["delta_input"], r["right_input"]) for r in case["rules"]))
+        summary = RecursiveExecutor.compile(plan, store).run(store, 16)
+        if "iterations" in case:
+            assert summary.iterations == case["iterations"], case["name"]
+        if "total_new_rows" in case:
+            assert summary.total_new_rows == case["total_new_rows"], case["name"]
+        for rid, rel in case["full"].items():
+            assert pairs(store.full(R(int(rid)))) == [tuple(p) for p in rel], case["name"]
+            assert store.rows(R(int(rid)), 1) == 0
+
+
+def test_clause_order_independence(stream):
+    def run(reverse):
+        store = RelationStore(stream)
+        store.insert_static(R(0), cols([(2, 3), (3, 4)]))
+        store.insert_static(R(1), cols([(2, 3), (3, 4), (4, 5)]))
+        store.insert_recursive(R(2), cols([(1, 2)]), cols([(1, 2)]))
+        rules = [rule(2, 2, 0), rule(2, 2, 1)]
+        if reverse:
+            rules.reverse()
+        s = RecursiveExecutor.compile(plans.RecursiveSccPlan((R(2),), tuple(rules)), store).run(store, 16)
+        return s.iterations, s.total_new_rows, pairs(store.full(R(2)))
+    assert run(False) == run(True) == (4, 3, [(1, 2), (1, 3), (1, 4), (1, 5)])
+
+
+def test_compile_errors(stream):
+    store = RelationStore(stream)
+    store.insert_static(R(0), cols([(1, 2)]))
+    store.insert_recursive(R(1), cols([(1, 2)]), cols([(1, 2)]))
+    first = rule(1, 1, 0)
+    bad = plans.RecursiveRulePlan(R(1), R(1), R(0), plans.path_join_plan(),
+                                  union=plans.SortedBinaryUnion((0, 1), (1, 0)))
+    with pytest.raises(GenericExecutionError) as err:
+        RecursiveExecutor.compile(plans.RecursiveSccPlan((R(1),), (first, bad)), store)
+    assert err.value.kind == "IncompatibleTargetPlans"
+    with pytest.raises(GenericExecutionError) as err:
+        RecursiveExecutor.compile(plans.RecursiveSccPlan((), (first,)), store)
+    assert err.value.kind == "EmptyScc"
+    with pytest.raises(GenericExecutionError) as err:
+        RecursiveExecutor.compile(plans.RecursiveSccPlan((R(0),), (rule(0, 0, 0),)), store)
+    assert err.value.kind == "NotRecursive"
+    with pytest.raises(GenericExecutionError) as err:
+        store.insert_static(R(0), cols([(5, 6)]))
+    assert err.value.kind == "Duplicate"
+
+
+def test_iteration_limit(stream):
+    store = RelationStore(stream)
+    e = chain(10)
+    store.insert_static(R(0), e)
+    store.insert_recursive(R(1), e, e)
+    with pytest.raises(GenericExecutionError) as err:
+        RecursiveExecutor.compile(plans.transitive_closure_scc(R(1), R(0)), store).run(store, 3)
+    assert err.value.kind == "IterationLimit"
+
+
+def _tc(edges, stream, max_iterations=10_000):
+    store = RelationStore(stream)
+    store.insert_static(R(0), edges)
+    store.insert_recursive(R(1), edges, edges)
+    summary = RecursiveExecutor.compile(plans.transitive_closure_scc(R(1), R(0)), store).run(store, max_iterations)
+    return store, summary
+
+
+@pytest.mark.parametrize("n", [2, 3, 64, 1000])
+def test_chain_closed_form(n, stream):
+    store, s = _tc(chain(n), stream)
+    assert s.relation_rows == [(R(1), n * (n - 1) // 2)]
+    assert s.iterations == n - 1
+
+
+def test_plain_tc_exact(expected, stream):
+    e = expected["random_dag_1000_4000"]
+    edges = workload.random_dag()
+    store, s = _tc(edges, stream)
+    ref = oracle.Store()
+    ref.insert_static(0, edges)
+    ref.insert_recursive(1, edges, edges)
+    rc, rs = ref.run([1], [oracle.rule_from_plan(rule(1, 1, 0))], 10_000, 4)
+    assert rc == 0
+    full = store.full(R(1))
+    expect = ref.get(1)
+    assert np.array_equal(full[0], expect[0]) and np.array_equal(full[1], expect[1])
+    assert s.iterations == rs.iterations == e["max_dist"]
+    assert s.total_new_rows == rs.total_new_rows == e["path_rows"] - e["edges"]
+    assert workload.pair_digest(*full) == e["path_digest"]
+
+
+def test_component_graph_exact_small(expected, stream):
+    e = expected["component_graph_16384"]
+    s, d, _ = workload.component_graph(nodes=1 << 14)
+    store, summ = _tc((s, d), stream)
+    full = store.full(R(1))
+    bfs = oracle.tc_bfs(s, d, 1 << 14, closure=True)
+    assert np.array_equal(full[0], bfs["closure"][0]) and np.array_equal(full[1], bfs["closure"][1])
+    assert summ.iterations == e["max_dist"]
+    assert summ.total_new_rows == e["path_rows"] - e["edges"]
+
+
+@pytest.mark.parametrize("nodes", [1 << 17, 1 << 20])
+def test_component_graph_digest(nodes, expected, stream):
+    e = expected[f"component_graph_{nodes}"]
+    s, d, _ = workload.component_graph(nodes=nodes)
+    store, summ = _tc((s, d), stream)
+    digest, sorted_unique, rows = store.digest(R(1))
+    assert rows == e["path_rows"]
+    assert sorted_unique
+    assert digest == e["path_digest"]
+    assert summ.iterations == e["max_dist"]
+    assert summ.total_new_rows == e["path_rows"] - e["edges"]

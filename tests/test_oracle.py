@@ -1,0 +1,268 @@
+"""Pins the oracle (CPU restatement of the reference) before it is trusted.
+
+Every oracle function is checked against (a) the reference's own known-answer
+vectors (tests/golden/reference_vectors.json), (b) an independent set or
+brute-force formulation in Python, and (c) closed forms of transitive closure.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import chain, cols, cycle, pairs, random_pairs
+from sparkalog_amd import plans, workload
+
+
+# ---- known-answer vectors ----------------------------------------------------
+
+def test_distinct_vectors(golden):
+    for case in golden["distinct"]:
+        assert pairs(oracle.distinct(*cols(case["input"]))) == [tuple(p) for p in case["expected"]]
+
+
+def test_anti_join_vectors(golden):
+    for case in golden["anti_join"]:
+        out = oracle.anti_join(cols(case["left"]), cols(case["right"]))
+        assert pairs(out) == [tuple(p) for p in case["expected"]]
+
+
+def test_union_vectors(golden):
+    for case in golden["union"]:
+        out = oracle.union(cols(case["left"]), cols(case["right"]))
+        assert pairs(out) == [tuple(p) for p in case["expected"]]
+
+
+def test_join_vectors(golden):
+    for case in golden["join"]:
+        p = case["plan"]
+        out = oracle.join(cols(case["left"]), cols(case["right"]), p["left_key"], p["right_key"],
+                          [tuple(o) for o in p["output"]])
+        assert pairs(out) == [tuple(x) for x in case["expected"]]
+
+
+def test_filter_vectors(golden):
+    values = np.arange(10000, dtype=np.uint32) % 97
+    assert list(oracle.filter_rows(values, 5, 51)) == [i for i, v in enumerate(values) if v >= 51]
+    case = golden["filter"][1]
+    for op, rows in case["expected"].items():
+        assert list(oracle.filter_rows(np.arange(10), int(op), 4)) == rows
+    assert list(oracle.filter_rows(np.empty(0, np.uint32), 0, 0)) == []
+
+
+def test_range_index_vectors(golden):
+    for case in golden["range_index"]:
+        keys, starts, rows = oracle.range_index(case["values"])
+        assert keys.size == case["unique_keys"]
+        for key, expect in case["lookup"].items():
+            hit = np.searchsorted(keys, int(key))
+            got = [] if hit >= keys.size or keys[hit] != int(key) else list(rows[starts[hit]:starts[hit + 1]])
+            assert sorted(got) == expect
+
+
+def _rule(target, delta, right):
+    return oracle.rule_from_plan(plans.RecursiveRulePlan(
+        target=plans.RelationId(target), delta_input=plans.RelationId(delta), right_input=plans.RelationId(right),
+        join=plans.path_join_plan()))
+
+
+def test_recursion_vectors(golden):
+    for case in golden["recursion"]:
+        store = oracle.Store()
+        for rid, rel in case["static"].items():
+            assert store.insert_static(int(rid), cols(rel)) == 0
+        for rid, st in case["recursive"].items():
+            assert store.insert_recursive(int(rid), cols(st["full"]), cols(st["delta"])) == 0
+        rules = [_rule(r["target"], r["delta_input"], r["right_input"]) for r in case["rules"]]
+        rc, summary = store.run(case["relations"], rules, 16)
+        assert rc == 0, case["name"]
+        if "iterations" in case:
+            assert summary.iterations == case["iterations"], case["name"]
+        if "total_new_rows" in case:
+            assert summary.total_new_rows == case["total_new_rows"], case["name"]
+        for rid, rel in case["full"].items():
+            assert pairs(store.get(int(rid))) == [tuple(p) for p in rel], case["name"]
+            assert store.get(int(rid), 1)[0].size == 0
+
+
+def test_clause_order_independence():
+    # generic.rs:902-914: the overlapping-clause result does not depend on rule order
+    def run(reverse):
+        store = oracle.Store()
+        store.insert_static(0, cols([(2, 3), (3, 4)]))
+        store.insert_static(1, cols([(2, 3), (3, 4), (4, 5)]))
+        store.insert_recursive(2, cols([(1, 2)]), cols([(1, 2)]))
+        rules = [_rule(2, 2, 0), _rule(2, 2, 1)]
+        if reverse:
+            rules.reverse()
+        rc, s = store.run([2], rules, 16)
+        assert rc == 0
+        return s.iterations, s.total_new_rows, pairs(store.get(2))
+    assert run(False) == run(True)
+
+
+def test_incompatible_plans_rejected():
+    # generic.rs:917-946
+    store = oracle.Store()
+    store.insert_static(0, cols([(1, 2)]))
+    store.insert_recursive(1, cols([(1, 2)]), cols([(1, 2)]))
+    first = _rule(1, 1, 0)
+    bad = _rule(1, 1, 0)
+    bad.union_right[0], bad.union_right[1] = 1, 0
+    rc, _ = store.run([1], [first, bad], 16)
+    assert rc == 5
+
+
+def test_iteration_limit():
+    store = oracle.Store()
+    e = chain(10)
+    store.insert_static(0, e)
+    store.insert_recursive(1, e, e)
+    rc, s = store.run([1], [_rule(1, 1, 0)], 3)
+    assert rc == oracle.ORACLE_ITERATION_LIMIT
+    assert s.delta_rows_left > 0
+
+
+# ---- independent formulations on random inputs --------------------------------
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("threads", [1, 4])
+def test_set_operators_match_python_sets(seed, threads):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(0, 3000))
+    hi = int(rng.choice([4, 50, 1 << 32]))
+    a = random_pairs(rng, n, hi)
+    got = pairs(oracle.distinct(*a, threads=threads))
+    assert got == sorted(set(pairs(a)))
+    left = sorted(set(pairs(random_pairs(rng, int(rng.integers(0, 2000)), hi))))
+    right = sorted(set(pairs(random_pairs(rng, int(rng.integers(0, 2000)), hi))))
+    rs = set(right)
+    assert pairs(oracle.anti_join(cols(left), cols(right), threads=threads)) == [p for p in left if p not in rs]
+    assert pairs(oracle.union(cols(left), cols(right), threads=threads)) == sorted(set(left) | rs)
+
+
+@pytest.mark.parametrize("threads", [1, 4])
+def test_union_drops_duplicates_within_one_input(threads):
+    # SortedBinaryUnion drops every adjacent duplicate, also inside one input
+    # (execution/src/lib.rs:2208-2246), unlike std::set_union
+    left = [(1, 1), (1, 1), (2, 2)]
+    right = [(2, 2), (2, 2), (3, 3)]
+    assert pairs(oracle.union(cols(left), cols(right), threads=threads)) == [(1, 1), (2, 2), (3, 3)]
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("threads", [1, 3])
+def test_join_matches_nested_loop_order(seed, threads):
+    rng = np.random.default_rng(100 + seed)
+    left = random_pairs(rng, int(rng.integers(0, 400)), 20)
+    right = random_pairs(rng, int(rng.integers(0, 400)), 20)
+    for left_key, right_key, output in [(1, 0, ((0, 0), (1, 1))), (0, 1, ((1, 0), (0, 1))), (1, 1, ((0, 1), (1, 0)))]:
+        expect = []
+        lp, rp = pairs(left), pairs(right)
+        for li, l in enumerate(lp):
+            # right rows of a key ascending by row id (storage/src/lib.rs:366-367)
+            for ri, r in enumerate(rp):
+                if l[left_key] == r[right_key]:
+                    vals = [(l if side == 0 else r)[col] for side, col in output]
+                    expect.append(tuple(vals))
+        assert pairs(oracle.join(left, right, left_key, right_key, output, threads=threads)) == expect
+
+
+def test_filter_matches_predicates():
+    rng = np.random.default_rng(7)
+    v = rng.integers(0, 100, size=5000, dtype=np.uint32)
+    for op in range(6):
+        pred = plans.U32Predicate(op, 40)
+        assert list(oracle.filter_rows(v, op, 40)) == [i for i, x in enumerate(v) if pred.matches(int(x))]
+
+
+def test_sort_matches_numpy():
+    rng = np.random.default_rng(3)
+    for n, threads in [(0, 1), (1, 1), (1000, 1), (300000, 4)]:
+        k = rng.integers(0, 1 << 63, size=n, dtype=np.uint64) | (rng.integers(0, 2, size=n, dtype=np.uint64) << 63)
+        got = k.copy()
+        oracle.lib().oracle_sort_u64(got.ctypes.data_as(__import__("ctypes").c_void_p), n, threads)
+        assert np.array_equal(got, np.sort(k))
+
+
+# ---- transitive closure: brute force and closed forms ------------------------
+
+def _seminaive(edges, max_iterations=10000, threads=1):
+    store = oracle.Store()
+    store.insert_static(0, edges)
+    store.insert_recursive(1, edges, edges)
+    rc, s = store.run([1], [_rule(1, 1, 0)], max_iterations, threads)
+    assert rc == 0
+    return store.get(1), s
+
+
+@pytest.mark.parametrize("n", [2, 3, 17, 128])
+def test_chain_closed_form(n):
+    full, s = _seminaive(chain(n))
+    assert full[0].size == n * (n - 1) // 2
+    assert s.iterations == n - 1  # longest shortest path (generic.rs:734: 4 edges, 4 iterations)
+    assert s.total_new_rows == n * (n - 1) // 2 - (n - 1)
+
+
+def test_chain_4096_closed_form():
+    # tc-fixpoint's upper bound: n(n-1)/2 rows for n = 4096 (BFS, not the loop)
+    s, d = chain(4096)
+    assert oracle.tc_bfs(s, d, 4096)["pairs"] == 4096 * 4095 // 2
+
+
+@pytest.mark.parametrize("n", [1, 5, 40])
+def test_cycle_closed_form(n):
+    full, s = _seminaive(cycle(n))
+    assert full[0].size == n * n
+    bfs = oracle.tc_bfs(*cycle(n), n)
+    assert bfs["pairs"] == n * n
+    assert s.iterations == bfs["max_dist"]
+
+
+def test_disjoint_components_add():
+    a = chain(30)
+    b = cycle(12, start=100)
+    edges = (np.concatenate([a[0], b[0]]), np.concatenate([a[1], b[1]]))
+    order = np.lexsort((edges[1], edges[0]))
+    edges = (edges[0][order], edges[1][order])
+    full, _ = _seminaive(edges)
+    assert full[0].size == 30 * 29 // 2 + 12 * 12
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_seminaive_matches_bfs(seed):
+    rng = np.random.default_rng(seed)
+    nodes = int(rng.integers(5, 200))
+    m = int(rng.integers(1, 600))
+    e = sorted(set(pairs(random_pairs(rng, m, nodes))))
+    edges = cols(e)
+    full, s = _seminaive(edges, threads=1 + seed % 3)
+    bfs = oracle.tc_bfs(*edges, nodes, closure=True)
+    assert pairs(full) == pairs(bfs["closure"])
+    assert s.iterations == bfs["max_dist"]
+    assert s.total_new_rows == bfs["pairs"] - len(e)
+
+
+def test_c1_fixture_three_ways(expected):
+    e = expected["random_dag_1000_4000"]
+    s, d = workload.random_dag()
+    assert workload.pair_digest(s, d) == e["edge_digest"]
+    full, summ = _seminaive((s, d), threads=4)
+    assert full[0].size == e["path_rows"]
+    assert workload.pair_digest(*full) == e["path_digest"]
+    assert summ.iterations == e["max_dist"]
+    assert summ.total_new_rows == e["path_rows"] - e["edges"]
+    g = oracle.general_tc(s, d)
+    assert pairs(g["path"]) == pairs(full)
+    assert g["iterations"] == summ.iterations
+    # nonlocal = path \ edge (stratified negation)
+    assert pairs(g["nonlocal"]) == pairs(oracle.anti_join(full, (s, d)))
+    assert g["nonlocal"][0].size == e["nonlocal_rows"]
+
+
+def test_general_path_matches_bfs_small():
+    rng = np.random.default_rng(11)
+    e = sorted(set(pairs(random_pairs(rng, 300, 60))))
+    g = oracle.general_tc(*cols(e))
+    bfs = oracle.tc_bfs(*cols(e), 60, closure=True)
+    assert pairs(g["path"]) == pairs(bfs["closure"])
+    es = set(e)
+    assert pairs(g["nonlocal"]) == [p for p in pairs(bfs["closure"]) if p not in es]

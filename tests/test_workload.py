@@ -1,0 +1,52 @@
+"""Seeded workload generators: determinism and the shape each config asks for."""
+import numpy as np
+
+from helpers import packed
+from sparkalog_amd import workload
+
+
+def test_random_dag_shape(expected):
+    s, d = workload.random_dag()
+    assert s.size == 4000
+    assert np.all(s < d) and np.all(d < 1000)           # i -> j with i < j
+    k = packed((s, d))
+    assert np.all(k[1:] > k[:-1])                        # sorted, duplicates removed
+    assert workload.pair_digest(s, d) == expected["random_dag_1000_4000"]["edge_digest"]
+    s2, d2 = workload.random_dag()
+    assert np.array_equal(s, s2) and np.array_equal(d, d2)
+
+
+def test_component_graph_digests(expected):
+    for nodes in (1 << 14, 1 << 17):
+        s, d, comps = workload.component_graph(nodes=nodes)
+        e = expected[f"component_graph_{nodes}"]
+        assert s.size == e["edges"] and comps == e["components"]
+        assert workload.pair_digest(s, d) == e["edge_digest"]
+        k = packed((s, d))
+        assert np.all(k[1:] > k[:-1])
+        assert s.max() < nodes and d.max() < nodes
+        # Poisson(2) out-degree: about 2 edges per node after duplicate removal
+        assert 1.8 < s.size / nodes < 2.05
+
+
+def test_batch_graphs_shape():
+    s, d, offs, counts = workload.batch_graphs(instances=64, n_max=2048)
+    assert offs.size == 65 and offs[-1] == s.size
+    assert counts.min() >= 16 and counts.max() <= 2048
+    for i in range(64):
+        a, b = s[offs[i]:offs[i + 1]], d[offs[i]:offs[i + 1]]
+        assert a.size == 0 or (a.max() < counts[i] and b.max() < counts[i])
+
+
+def test_operator_inputs():
+    a, b = workload.distinct_input(5, 100000)
+    uniq = np.unique(packed((a, b)))
+    assert uniq.size == 70000
+    assert a.max() < (1 << 24)
+    (lf, ls), (rf, rs) = workload.overlapping_sets(6, 50000, 0.5)
+    lk, rk = packed((lf, ls)), packed((rf, rs))
+    assert np.all(lk[1:] > lk[:-1]) and np.all(rk[1:] > rk[:-1])
+    assert np.intersect1d(lk, rk).size == 25000
+    z, _ = workload.pairs(7, 200000, workload.KEY_ZIPF)
+    counts = np.bincount(z[z < 1000], minlength=1000)
+    assert counts[0] > counts[10] > counts[500]          # Zipf: rank 1 is the most frequent
