@@ -37,7 +37,32 @@ def parse_args():
     p.add_argument("--nodes", type=int, default=1 << 20)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-nodes", type=int, default=1 << 16)
+    p.add_argument("--mode", choices=["tc", "ops"], default="tc",
+                   help="tc: the headline fixpoint; ops: operator microbench at 2^k rows")
+    p.add_argument("--sizes", default="24,26,28", help="ops mode: log2 row counts")
+    p.add_argument("--ops", default="distinct,union,anti_join,join", help="ops mode: operators")
+    p.add_argument("--samples", type=int, default=5, help="ops mode: timed launches per case")
     return p.parse_args()
+
+
+def run_ops(args) -> None:
+    """Operator drop-in microbench: one JSON line per (operator, rows, keys)."""
+    import torch
+
+    from sparkalog_amd import opbench, workload
+
+    torch.cuda.set_device(0)
+    for name in args.ops.split(","):
+        modes = [workload.KEY_UNIFORM, workload.KEY_ZIPF] if name in ("distinct", "join") else [workload.KEY_UNIFORM]
+        for k in [int(x) for x in args.sizes.split(",")]:
+            for mode in modes:
+                op = opbench.make(name, 1 << k, mode)
+                r = opbench.time_op(op, args.samples)
+                r.update({"keys": "zipf1.1" if mode == workload.KEY_ZIPF else "uniform",
+                          "frac_of_hbm_peak": round(r["gbs"] / HBM_PEAK_GBS, 4)})
+                print(json.dumps(r), flush=True)
+                del op
+                torch.cuda.empty_cache()
 
 
 def cpu_baseline(sample_nodes: int) -> dict:
@@ -66,6 +91,9 @@ def cpu_baseline(sample_nodes: int) -> dict:
 
 def main():
     args = parse_args()
+    if args.mode == "ops":
+        run_ops(args)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
               "the multi-GPU sharding is described in DESIGN.md", file=sys.stderr)
