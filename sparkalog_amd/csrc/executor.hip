@@ -71,7 +71,11 @@ struct DevBuf {
     }
     ~DevBuf() { release(); }
     void release() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr) {
+            // kernels queued on any stream may still use the buffer
+            (void)hipDeviceSynchronize();
+            (void)hipFree(ptr);
+        }
         ptr = nullptr;
         cap = 0;
     }
@@ -79,8 +83,6 @@ struct DevBuf {
         if (n <= cap && ptr) return;
         size_t c = cap ? cap : 1024;
         while (c < n) c += c / 2 + 1;
-        // queued kernels may still use the old buffer
-        if (ptr) (void)hipDeviceSynchronize();
         release();
         void* p = nullptr;
         hipError_t e = hipMalloc(&p, c * sizeof(T));
@@ -102,7 +104,12 @@ struct DevRelation {
 struct RelationState {
     bool present = false;
     bool recursive = false;
+    // FULL = full ∪ runs[0] ∪ runs[1] ∪ ...: sorted unique arrays, pairwise
+    // disjoint, sizes decreasing. Inside a run new rows land in small runs so
+    // that FULL is not rewritten every iteration; outside a run FULL is one
+    // array (runs is empty).
     DevRelation full, delta, newt;
+    std::vector<DevRelation> runs;
     uint64_t version = 0;
     uint32_t max_value = 0;  // max over both columns of every row ever stored
 };
@@ -151,8 +158,11 @@ __device__ __forceinline__ uint32_t column_of(uint64_t key, uint32_t col) {
     return col == 0 ? static_cast<uint32_t>(key >> 32) : static_cast<uint32_t>(key);
 }
 
-// Join count over a dense CSR: counts -> exclusive offsets (u64) with
-// decoupled look-back; the last tile writes the total.
+// Join count over a dense CSR: per-row match counts -> exclusive offsets
+// (u64) with decoupled look-back; the last tile writes the total. Layout:
+// wave w owns tile rows [w*512, w*512+512), item i of lane j is row
+// w*512 + i*64 + j, so loads are coalesced and the row-order prefix is one
+// wave scan per item plus one block step.
 __global__ __launch_bounds__(kBlock) void dense_join_count_kernel(const uint64_t* delta, size_t n, uint32_t key_col,
                                                                   const uint32_t* starts, uint32_t n_keys,
                                                                   uint64_t* offsets, uint64_t* total,
@@ -160,40 +170,47 @@ __global__ __launch_bounds__(kBlock) void dense_join_count_kernel(const uint64_t
                                                                   uint32_t* error) {
     constexpr int kItems = 8;
     constexpr int kTile = kBlock * kItems;
-    __shared__ uint64_t scan_lds[kWavesPerBlock];
+    constexpr int kWaveRows = kItems * kWave;
+    __shared__ uint64_t wave_totals[kWavesPerBlock];
     __shared__ uint64_t offset_slot;
     __shared__ uint32_t tile_slot;
     const uint32_t tile = next_tile(counter, &tile_slot);
     const size_t base = static_cast<size_t>(tile) * kTile;
+    const uint32_t lane = lane_id(), wave = wave_id();
     uint64_t c[kItems];
+    uint64_t running = 0;  // wave-local prefix in row order
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
-        const size_t row = base + static_cast<size_t>(i) * kBlock + threadIdx.x;
-        c[i] = 0;
+        const size_t row = base + wave * kWaveRows + i * kWave + lane;
+        uint64_t cnt = 0;
         if (row < n) {
             const uint32_t key = column_of(delta[row], key_col);
-            if (key < n_keys) c[i] = starts[key + 1] - starts[key];
+            if (key < n_keys) cnt = starts[key + 1] - starts[key];
         }
+        const uint64_t incl = wave_inclusive_sum<uint64_t>(cnt);
+        c[i] = running + incl - cnt;
+        running += __shfl(incl, kWave - 1, kWave);
     }
-    uint64_t running = 0;
-    for (int i = 0; i < kItems; ++i) {
-        uint64_t part;
-        const uint64_t ex = block_exclusive_sum<uint64_t>(c[i], &part, scan_lds);
-        c[i] = running + ex;
-        running += part;
+    if (lane == 0) wave_totals[wave] = running;
+    __syncthreads();
+    uint64_t wave_base = 0, tile_sum = 0;
+#pragma unroll
+    for (int w = 0; w < kWavesPerBlock; ++w) {
+        if (w < static_cast<int>(wave)) wave_base += wave_totals[w];
+        tile_sum += wave_totals[w];
     }
-    if (wave_id() == 0) {
-        const uint64_t excl = wave_lookback(status, tile, running, 0, error);
-        if (lane_id() == 0) offset_slot = excl;
+    if (wave == 0) {
+        const uint64_t excl = wave_lookback(status, tile, tile_sum, 0, error);
+        if (lane == 0) offset_slot = excl;
     }
     __syncthreads();
-    const uint64_t offset = offset_slot;
+    const uint64_t offset = offset_slot + wave_base;
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
-        const size_t row = base + static_cast<size_t>(i) * kBlock + threadIdx.x;
+        const size_t row = base + wave * kWaveRows + i * kWave + lane;
         if (row < n) offsets[row] = offset + c[i];
     }
-    if (tile == tiles(n, kTile) - 1 && threadIdx.x == 0) *total = offset + running;
+    if (tile == tiles(n, kTile) - 1 && threadIdx.x == 0) *total = offset_slot + tile_sum;
 }
 
 struct EmitSpec {
@@ -205,62 +222,107 @@ struct EmitSpec {
     uint32_t shift;            // 32, or fewer bits to compress the sort key
 };
 
-// Output-balanced emit of the dense join, writing compressed sort keys.
+constexpr int kEmitTile = 2048;   // output rows per tile
+constexpr int kEmitStage = 2048;  // delta rows staged per tile
+
+// row_start[t] = the delta row that contains output t*kEmitTile (one thread
+// per tile; the last entry is n).
+__global__ void emit_partition_kernel(const uint64_t* offsets, size_t n, size_t total, size_t* row_start,
+                                      size_t n_tiles) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; t <= n_tiles; t += stride) {
+        if (t == n_tiles) {
+            row_start[t] = n;
+            continue;
+        }
+        const uint64_t o = static_cast<uint64_t>(t) * kEmitTile;
+        size_t lo = 0, hi = n;
+        while (lo < hi) {
+            const size_t mid = (lo + hi) >> 1;
+            if (offsets[mid] <= o)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        row_start[t] = lo - 1;
+    }
+    (void)total;
+}
+
+// Output-balanced emit of the dense join, writing compressed sort keys. Each
+// tile stages its delta rows in LDS, marks where each row's outputs begin,
+// and a block max-scan gives every output position its row.
 __global__ __launch_bounds__(kBlock) void dense_join_emit_kernel(const uint64_t* delta, size_t n,
                                                                  const uint32_t* starts, uint32_t n_keys,
                                                                  const uint64_t* right_sorted, const uint64_t* offsets,
-                                                                 size_t total, EmitSpec spec, uint64_t* out) {
-    constexpr int kTile = 2048;
-    constexpr int kStage = 2048;
-    __shared__ uint64_t row_offset[kStage];
-    __shared__ uint64_t row_key[kStage];
-    __shared__ uint32_t row_match[kStage];
-    __shared__ size_t bounds[2];
-    const uint64_t n_tiles = (total + kTile - 1) / kTile;
+                                                                 size_t total, const size_t* row_start, EmitSpec spec,
+                                                                 uint64_t* out) {
+    constexpr int kPer = kEmitTile / kBlock;  // 8 output positions per thread
+    __shared__ uint64_t row_offset[kEmitStage];
+    __shared__ uint64_t row_key[kEmitStage];
+    __shared__ uint32_t row_match[kEmitStage];
+    __shared__ uint32_t row_of[kEmitTile];
+    __shared__ uint32_t scan_lds[kWavesPerBlock];
+    const uint64_t n_tiles = (total + kEmitTile - 1) / kEmitTile;
     for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-        const uint64_t o0 = t * kTile;
-        const uint64_t o1 = o0 + kTile < total ? o0 + kTile : total;
-        if (threadIdx.x < 2) {
-            const uint64_t o = threadIdx.x ? o1 - 1 : o0;
-            size_t lo = 0, hi = n;
-            while (lo < hi) {
-                const size_t mid = (lo + hi) >> 1;
-                if (offsets[mid] <= o)
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
-            bounds[threadIdx.x] = lo - 1;
-        }
-        __syncthreads();
-        const size_t l0 = bounds[0], l1 = bounds[1] + 1;
+        const uint64_t o0 = t * kEmitTile;
+        const uint64_t o1 = o0 + kEmitTile < total ? o0 + kEmitTile : total;
+        const size_t l0 = row_start[t];
+        const size_t l1 = row_start[t + 1] + 1 < n ? row_start[t + 1] + 1 : n;
         const size_t span = l1 - l0;
-        const bool staged = span <= static_cast<size_t>(kStage);
+        const bool staged = span <= static_cast<size_t>(kEmitStage);
         if (staged) {
+            for (uint32_t j = threadIdx.x; j < kEmitTile; j += kBlock) row_of[j] = 0;
+            __syncthreads();
             for (uint32_t i = threadIdx.x; i < span; i += kBlock) {
                 const uint64_t dk = delta[l0 + i];
-                row_offset[i] = offsets[l0 + i];
-                row_key[i] = dk;
+                const uint64_t off = offsets[l0 + i];
                 const uint32_t key = column_of(dk, spec.key_col);
-                row_match[i] = key < n_keys ? starts[key] : 0;
+                const uint32_t m0 = key < n_keys ? starts[key] : 0;
+                const uint32_t cnt = key < n_keys ? starts[key + 1] - m0 : 0;
+                row_offset[i] = off;
+                row_key[i] = dk;
+                row_match[i] = m0;
+                // first output position of row i inside the tile (row 0 owns o0)
+                if (i > 0 && cnt > 0 && off < o1) row_of[off - o0] = i;
             }
+            __syncthreads();
+            // inclusive max-scan of row_of: thread owns positions [8t, 8t+8)
+            uint32_t local[kPer];
+            uint32_t m = 0;
+#pragma unroll
+            for (int q = 0; q < kPer; ++q) {
+                const uint32_t v = row_of[threadIdx.x * kPer + q];
+                m = v > m ? v : m;
+                local[q] = m;
+            }
+            // exclusive max over previous threads
+            uint32_t incl = m;
+#pragma unroll
+            for (int off = 1; off < kWave; off <<= 1) {
+                const uint32_t other = __shfl_up(incl, off, kWave);
+                if (static_cast<int>(lane_id()) >= off) incl = other > incl ? other : incl;
+            }
+            if (lane_id() == kWave - 1) scan_lds[wave_id()] = incl;
+            __syncthreads();
+            uint32_t prior = __shfl_up(incl, 1, kWave);
+            if (lane_id() == 0) prior = 0;
+            for (int w = 0; w < static_cast<int>(wave_id()); ++w) prior = scan_lds[w] > prior ? scan_lds[w] : prior;
+#pragma unroll
+            for (int q = 0; q < kPer; ++q) {
+                const uint32_t v = local[q] > prior ? local[q] : prior;
+                row_of[threadIdx.x * kPer + q] = v;
+            }
+            __syncthreads();
         }
-        __syncthreads();
         for (uint64_t o = o0 + threadIdx.x; o < o1; o += kBlock) {
-            uint64_t dk, row_start;
+            uint64_t dk, row_start_off;
             uint32_t match;
             if (staged) {
-                uint32_t lo = 0, hi = static_cast<uint32_t>(span);
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (row_offset[mid] <= o)
-                        lo = mid + 1;
-                    else
-                        hi = mid;
-                }
-                dk = row_key[lo - 1];
-                row_start = row_offset[lo - 1];
-                match = row_match[lo - 1];
+                const uint32_t i = row_of[o - o0];
+                dk = row_key[i];
+                row_start_off = row_offset[i];
+                match = row_match[i];
             } else {
                 size_t lo = l0, hi = l1;
                 while (lo < hi) {
@@ -271,10 +333,10 @@ __global__ __launch_bounds__(kBlock) void dense_join_emit_kernel(const uint64_t*
                         hi = mid;
                 }
                 dk = delta[lo - 1];
-                row_start = offsets[lo - 1];
+                row_start_off = offsets[lo - 1];
                 match = starts[column_of(dk, spec.key_col)];
             }
-            const uint64_t r = right_sorted[match + static_cast<uint32_t>(o - row_start)];
+            const uint64_t r = right_sorted[match + static_cast<uint32_t>(o - row_start_off)];
             // right_sorted rows are (key, other): column right_key is the high half
             uint32_t v[2];
 #pragma unroll
@@ -373,9 +435,11 @@ struct RuleRuntime {
     CachedIndex index;
     DevBuf<uint64_t> status;    // look-back words of the join count
     DevBuf<uint64_t> offsets;   // join offsets per delta row
+    DevBuf<size_t> row_start;   // first delta row of each emit tile
     DevBuf<uint64_t> cand_a, cand_b;
     DevBuf<uint64_t> distinct_out;
     uint64_t join_rows = 0;     // M (host, after the first sync)
+    int distinct_slot = -1;     // profiling event of this iteration's distinct
     uint64_t delta_rows = 0;
     uint32_t shift = 32;
 };
@@ -386,7 +450,9 @@ struct TargetRuntime {
     DevBuf<uint64_t> contributions, union_tmp;
     DevBuf<uint64_t> newt;
     DevBuf<uint64_t> full_union;
+    DevBuf<uint64_t> new_run;      // NEWT's second copy, becomes a run of FULL
     uint64_t contrib_bound = 0;
+    int anti_slot = -1;            // profiling event of this iteration's anti-join
 };
 
 // Small device counters read back once per phase.
@@ -396,6 +462,7 @@ struct Counters {
     uint64_t contrib_rows[64];
     uint64_t newt_rows[64];
     uint64_t full_rows[64];
+    uint64_t merge_rows;         // output count of a run merge (equal to the summed sizes)
 };
 
 }  // namespace
@@ -415,6 +482,25 @@ struct sparkalog_hip_store {
     size_t events_used = 0;
     std::vector<std::pair<int, double>> pending_bytes;  // (op, bytes) aligned with events
     uint64_t generation = 0;                    // bumped by every insert; versions derive from it
+    std::vector<spk::DevBuf<uint64_t>> pool;    // released run buffers, reused by later merges
+    uint32_t merge_ratio = 4;                   // merge two runs when the older is < ratio x the newer
+
+    spk::DevBuf<uint64_t> take(size_t n) {
+        size_t best = pool.size();
+        for (size_t i = 0; i < pool.size(); ++i)
+            if (pool[i].cap >= n && (best == pool.size() || pool[i].cap < pool[best].cap)) best = i;
+        spk::DevBuf<uint64_t> b;
+        if (best < pool.size()) {
+            b = std::move(pool[best]);
+            pool.erase(pool.begin() + static_cast<long>(best));
+        } else {
+            b.reserve(n ? n : 1);
+        }
+        return b;
+    }
+    void give(spk::DevBuf<uint64_t>&& b) {
+        if (b.ptr) pool.push_back(std::move(b));
+    }
     // operator workspaces kept across runs so repeated runs allocate nothing
     std::vector<spk::TargetRuntime> ws_targets;
     std::vector<spk::RuleRuntime> ws_rules;
@@ -461,6 +547,10 @@ struct sparkalog_hip_store {
         spk::check(hipEventRecord(event_pool[slot].second, stream));
         pending_bytes.resize(events_used);
         pending_bytes[slot] = {op, bytes};
+    }
+    // Bytes that depend on device-written counts, added after the sync.
+    void add_op_bytes(int slot, double bytes) {
+        if (slot >= 0 && static_cast<size_t>(slot) < pending_bytes.size()) pending_bytes[slot].second += bytes;
     }
     // Call after a stream synchronize.
     void collect_ops() {
@@ -551,10 +641,14 @@ void download(sparkalog_hip_store* s, const DevRelation& src, uint32_t* c0, uint
 
 bool identity(const uint32_t c[2]) { return c[0] == 0 && c[1] == 1; }
 
+double settle_runs(sparkalog_hip_store* s, RelationState& ts, bool all);
+
 void build_index(sparkalog_hip_store* s, RuleRuntime& rt, RelationState& right, uint32_t relation) {
     const uint32_t col = rt.plan.right_key;
     CachedIndex& ix = rt.index;
     if (ix.valid && ix.relation == relation && ix.version == right.version && ix.column == col) return;
+    // a recursive right input is indexed as one array
+    if (!right.runs.empty()) settle_runs(s, right, true);
     const int ev = s->begin_op();
     const uint64_t n = right.full.rows;
     if (right.max_value >= (1u << 28)) throw StatusError{SPARKALOG_HIP_ERR_UNSUPPORTED_PLAN};
@@ -585,6 +679,64 @@ void build_index(sparkalog_hip_store* s, RuleRuntime& rt, RelationState& right, 
     ix.n_keys = n_keys;
     ix.rows = n;
     s->end_op(ev, kOpIndex, 16.0 * n);
+}
+
+constexpr size_t kMaxRuns = 15;
+
+__global__ void copy_keys_kernel(const uint64_t* in, const uint64_t* rows_dev, size_t bound, uint64_t* out,
+                                 uint64_t* out_copy, uint64_t* out_rows) {
+    const size_t n = *rows_dev < bound ? *rows_dev : bound;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    const size_t gtid = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (size_t i = gtid; i < n; i += stride) {
+        out[i] = in[i];
+        out_copy[i] = in[i];
+    }
+    if (gtid == 0) *out_rows = n;
+}
+
+hipError_t copy_keys(const uint64_t* in, const uint64_t* rows_dev, size_t bound, uint64_t* out, uint64_t* out_copy,
+                     uint64_t* out_rows, hipStream_t stream) {
+    copy_keys_kernel<<<grid_for(bound), kBlock, 0, stream>>>(in, rows_dev, bound, out, out_copy, out_rows);
+    return hipGetLastError();
+}
+
+// Merge FULL's newest runs while the older run is smaller than merge_ratio
+// times the newer (or always, when `all`), leaving FULL as one array when
+// `all` is set. Inputs are disjoint sorted unique arrays, so a merge's output
+// size is the sum of its inputs' sizes. Returns the algorithmic bytes moved.
+double settle_runs(sparkalog_hip_store* s, RelationState& ts, bool all) {
+    double bytes = 0.0;
+    Counters* cdev = reinterpret_cast<Counters*>(s->counters_dev.ptr);
+    while (!ts.runs.empty()) {
+        DevRelation& last = ts.runs.back();
+        DevRelation& prev = ts.runs.size() >= 2 ? ts.runs[ts.runs.size() - 2] : ts.full;
+        const bool merge = all || ts.runs.size() > kMaxRuns ||
+                           prev.rows < static_cast<uint64_t>(s->merge_ratio) * last.rows;
+        if (!merge) break;
+        const uint64_t n = prev.rows + last.rows;
+        if (prev.rows == 0) {
+            std::swap(prev.keys, last.keys);
+            prev.rows = last.rows;
+        } else if (last.rows) {
+            DevBuf<uint64_t> out = s->take(n);
+            const size_t tb = union_temp_bytes(prev.rows, last.rows);
+            void* temp = s->scratch(tb);
+            PairInput a{prev.keys.ptr, nullptr, nullptr, nullptr};
+            PairInput b{last.keys.ptr, nullptr, nullptr, nullptr};
+            PairOutput o{out.ptr, nullptr, nullptr};
+            const int ev = s->begin_op();
+            check(sorted_union(a, prev.rows, b, last.rows, o, &cdev->merge_rows, temp, tb, s->stream));
+            s->end_op(ev, kOpUnion, 16.0 * n);
+            s->give(std::move(prev.keys));
+            prev.keys = std::move(out);
+            prev.rows = n;
+            bytes += 16.0 * n;
+        }
+        s->give(std::move(last.keys));
+        ts.runs.pop_back();
+    }
+    return bytes;
 }
 
 // RecursiveExecutor::compile (generic.rs:407-470): validation and runtime
@@ -743,6 +895,8 @@ extern "C" int sparkalog_hip_store_clear(sparkalog_hip_store* s) {
         st.present = false;
         st.recursive = false;
         st.full.rows = st.delta.rows = st.newt.rows = 0;
+        for (auto& r : st.runs) s->give(std::move(r.keys));
+        st.runs.clear();
     }
     return SPARKALOG_HIP_OK;
 }
@@ -752,7 +906,9 @@ extern "C" int sparkalog_hip_store_rows(sparkalog_hip_store* s, uint32_t id, int
     RelationState* st = find(s, id);
     if (!st) return SPARKALOG_HIP_ERR_MISSING_RELATION;
     if (which == 1 && !st->recursive) return SPARKALOG_HIP_ERR_NOT_RECURSIVE;
-    *rows = which == 0 ? st->full.rows : st->delta.rows;
+    uint64_t full_rows = st->full.rows;
+    for (const auto& r : st->runs) full_rows += r.rows;
+    *rows = which == 0 ? full_rows : st->delta.rows;
     return SPARKALOG_HIP_OK;
 }
 
@@ -762,9 +918,10 @@ extern "C" int sparkalog_hip_store_read(sparkalog_hip_store* s, uint32_t id, int
     RelationState* st = find(s, id);
     if (!st) return SPARKALOG_HIP_ERR_MISSING_RELATION;
     if (which == 1 && !st->recursive) return SPARKALOG_HIP_ERR_NOT_RECURSIVE;
+    SPK_GUARD_BEGIN
+    if (which == 0 && !st->runs.empty()) settle_runs(s, *st, true);
     const DevRelation& r = which == 0 ? st->full : st->delta;
     if (r.rows > capacity || (r.rows && (!c0 || !c1))) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
-    SPK_GUARD_BEGIN
     download(s, r, c0, c1, to_device);
     return SPARKALOG_HIP_OK;
     SPK_GUARD_END
@@ -776,6 +933,7 @@ extern "C" int sparkalog_hip_store_digest(sparkalog_hip_store* s, uint32_t id, u
     RelationState* st = find(s, id);
     if (!st) return SPARKALOG_HIP_ERR_MISSING_RELATION;
     SPK_GUARD_BEGIN
+    if (!st->runs.empty()) settle_runs(s, *st, true);
     s->misc_dev.reserve(64);
     clear_words_kernel<<<1, 64, 0, s->stream>>>(s->misc_dev.ptr, 4);
     const uint64_t n = st->full.rows;
@@ -839,6 +997,8 @@ extern "C" int sparkalog_hip_executor_run(sparkalog_hip_store* s, const uint32_t
         if (compiled != SPARKALOG_HIP_OK) return compiled;
     }
 
+    s->events_used = 0;
+    s->pending_bytes.clear();
     hipEvent_t t_begin, t_end;
     check(hipEventCreate(&t_begin));
     check(hipEventCreate(&t_end));
@@ -884,6 +1044,7 @@ extern "C" int sparkalog_hip_executor_run(sparkalog_hip_store* s, const uint32_t
             build_index(s, rt, rs, rt.plan.right_input);
             rt.delta_rows = ds.delta.rows;
             rt.shift = shift;
+            rt.distinct_slot = -1;
             const uint64_t L = rt.delta_rows;
             if (L == 0) {
                 check(set_u64(&cdev->join_total[r], 0, s->stream));
@@ -927,14 +1088,18 @@ extern "C" int sparkalog_hip_executor_run(sparkalog_hip_store* s, const uint32_t
             spec.order[1] = rt.plan.distinct_columns[1];
             spec.shift = rt.shift;
             int ev = s->begin_op();
-            const uint64_t emit_tiles = tiles(M, 2048);
+            const uint64_t emit_tiles = tiles(M, kEmitTile);
             const unsigned emit_grid = static_cast<unsigned>(emit_tiles < 65536 ? emit_tiles : 65536);
+            rt.row_start.reserve(emit_tiles + 1);
+            emit_partition_kernel<<<grid_for(emit_tiles + 1), kBlock, 0, s->stream>>>(rt.offsets.ptr, rt.delta_rows, M,
+                                                                                   rt.row_start.ptr, emit_tiles);
             dense_join_emit_kernel<<<emit_grid, kBlock, 0, s->stream>>>(
                 ds.delta.keys.ptr, rt.delta_rows, rt.index.starts.ptr, rt.index.n_keys, rt.index.sorted.ptr,
-                rt.offsets.ptr, M, spec, rt.cand_a.ptr);
+                rt.offsets.ptr, M, rt.row_start.ptr, spec, rt.cand_a.ptr);
             check(hipGetLastError());
             s->end_op(ev, kOpJoin, 8.0 * rt.index.rows + 8.0 * M);
             ev = s->begin_op();
+            rt.distinct_slot = ev;
             const size_t tb = distinct_temp_bytes(M);
             void* temp = s->scratch(tb);
             PairInput in{rt.cand_a.ptr, nullptr, nullptr, nullptr};
@@ -978,82 +1143,99 @@ extern "C" int sparkalog_hip_executor_run(sparkalog_hip_store* s, const uint32_t
                 bound = nb;
             }
             t.contrib_bound = bound;
-            // NEWT = contributions \ FULL
+            // NEWT = contributions \ FULL, as a chain of anti-joins over FULL's
+            // runs (largest first). The last one also writes NEWT's copy that
+            // becomes FULL's newest run.
             const uint32_t ti = static_cast<uint32_t>(&t - targets.data());
             t.newt.reserve(bound ? bound : 1);
+            t.new_run = s->take(bound ? bound : 1);
+            if (ts.runs.size() >= kMaxRuns) settle_runs(s, ts, true);
+            AntiRight rights{};
+            rights.count = 0;
+            double right_rows = 0;
+            auto add_right = [&](const DevRelation& r) {
+                if (!r.rows) return;
+                rights.in[rights.count] = PairInput{r.keys.ptr, nullptr, nullptr, nullptr};
+                rights.rows[rights.count] = r.rows;
+                ++rights.count;
+                right_rows += static_cast<double>(r.rows);
+            };
+            add_right(ts.full);
+            for (const auto& r : ts.runs) add_right(r);
+            t.anti_slot = -1;
             if (bound == 0) {
                 check(set_u64(&cdev->newt_rows[ti], 0, s->stream));
+            } else if (rights.count == 0) {
+                check(copy_keys(contrib, contrib_rows, bound, t.newt.ptr, t.new_run.ptr, &cdev->newt_rows[ti],
+                                s->stream));
             } else {
-                const size_t tb = anti_join_temp_bytes(bound);
+                const size_t tb = anti_join_temp_bytes(bound, rights.count);
                 void* temp = s->scratch(tb);
                 PairInput a{contrib, nullptr, nullptr, contrib_rows};
-                PairInput b{ts.full.keys.ptr, nullptr, nullptr, nullptr};
-                PairOutput o{t.newt.ptr, nullptr, nullptr};
+                PairOutput o{t.newt.ptr, nullptr, nullptr, t.new_run.ptr};
                 const int ev = s->begin_op();
-                check(sorted_anti_join(a, bound, b, ts.full.rows, o, nullptr, &cdev->newt_rows[ti], temp, tb,
-                                       s->stream));
-                s->end_op(ev, kOpAntiJoin, 8.0 * ts.full.rows);  // + 8L + 8O after the sync
+                check(sorted_anti_join_multi(a, bound, rights, o, nullptr, &cdev->newt_rows[ti], temp, tb, s->stream));
+                s->end_op(ev, kOpAntiJoin, 8.0 * right_rows);
+                t.anti_slot = ev;
+                algo_bytes += 8.0 * right_rows;
             }
-        }
-        // ---- TargetRuntime::apply (generic.rs:360-397) ----
-        for (auto& t : targets) {
-            RelationState& ts = *find(s, t.target);
-            const uint32_t ti = static_cast<uint32_t>(&t - targets.data());
-            if (t.contrib_bound == 0) {
-                // nothing new: FULL is unchanged (the union would copy it)
-                check(set_u64(&cdev->full_rows[ti], ts.full.rows, s->stream));
-                continue;
-            }
-            const uint64_t nb = ts.full.rows + t.contrib_bound;
-            t.full_union.reserve(nb ? nb : 1);
-            const size_t tb = union_temp_bytes(ts.full.rows, t.contrib_bound);
-            void* temp = s->scratch(tb);
-            PairInput a{ts.full.keys.ptr, nullptr, nullptr, nullptr};
-            PairInput b{t.newt.ptr, nullptr, nullptr, &cdev->newt_rows[ti]};
-            PairOutput o{t.full_union.ptr, nullptr, nullptr};
-            const int ev = s->begin_op();
-            check(sorted_union(a, ts.full.rows, b, t.contrib_bound, o, &cdev->full_rows[ti], temp, tb, s->stream));
-            s->end_op(ev, kOpUnion, 8.0 * ts.full.rows);  // + 8R + 8O after the sync
         }
         check(hipMemcpyAsync(&chost, cdev, sizeof(Counters), hipMemcpyDeviceToHost, s->stream));
         uint32_t err = 0;
         check(hipMemcpyAsync(&err, lb_error, sizeof(err), hipMemcpyDeviceToHost, s->stream));
         check(hipStreamSynchronize(s->stream));
         if (err) throw StatusError{SPARKALOG_HIP_ERR_DEVICE};
-        // per-iteration algorithmic bytes (pairs are 8 bytes)
+        // per-iteration algorithmic bytes of the operators run (pairs are 8 bytes)
         for (uint32_t r = 0; r < n_rules; ++r) {
             const double L = static_cast<double>(rts[r].delta_rows), M = static_cast<double>(rts[r].join_rows);
             const double U = rts[r].join_rows ? static_cast<double>(chost.distinct_rows[r]) : 0.0;
             const double R = static_cast<double>(rts[r].index.rows);
             if (L > 0) algo_bytes += 8 * L + 8 * R + 8 * M + (M > 0 ? 8 * M + 8 * U : 0.0);
+            s->add_op_bytes(rts[r].distinct_slot, 8 * U);
         }
         for (uint32_t i = 0; i < n_relations; ++i) {
             TargetRuntime& t = targets[i];
-            RelationState& ts = *find(s, t.target);
-            const double C = static_cast<double>(t.contrib_bound ? chost.distinct_rows[t.rules[0]] : 0);
-            const double N = static_cast<double>(chost.newt_rows[i]);
-            const double F = static_cast<double>(ts.full.rows), F2 = static_cast<double>(chost.full_rows[i]);
-            if (t.contrib_bound) algo_bytes += 8 * C + 8 * F + 8 * N + 8 * F + 8 * N + 8 * F2;
+            const uint64_t C = t.rules.size() > 1 ? chost.contrib_rows[t.rules.back()]
+                                                  : (t.contrib_bound ? chost.distinct_rows[t.rules[0]] : 0);
+            // the anti-join reads the contributions and writes NEWT twice (DELTA
+            // and FULL's new run)
+            if (t.anti_slot >= 0 || t.contrib_bound) {
+                const double extra = 8.0 * static_cast<double>(C) + 16.0 * static_cast<double>(chost.newt_rows[i]);
+                s->add_op_bytes(t.anti_slot, extra);
+                if (t.contrib_bound) algo_bytes += extra;
+            }
         }
-        if (s->profiling) {
-            s->collect_ops();
-        }
-        // swaps: NEWT <- anti-join output, FULL <- union output, DELTA <-> NEWT
+        // ---- TargetRuntime::apply (generic.rs:360-397) ----
+        // FULL ∪ NEWT: NEWT is disjoint from every run of FULL, so it becomes
+        // FULL's newest run; runs are merged (pure merges of disjoint sorted
+        // arrays, output size known on the host) while the older run is less
+        // than merge_ratio times the newer one.
         for (uint32_t i = 0; i < n_relations; ++i) {
             TargetRuntime& t = targets[i];
             RelationState& ts = *find(s, t.target);
+            const uint64_t N = chost.newt_rows[i];
             std::swap(ts.newt.keys, t.newt);
-            ts.newt.rows = chost.newt_rows[i];
-            if (t.contrib_bound) std::swap(ts.full.keys, t.full_union);
-            ts.full.rows = chost.full_rows[i];
+            ts.newt.rows = N;
+            if (N) {
+                DevRelation run;
+                run.keys = std::move(t.new_run);
+                run.rows = N;
+                ts.runs.push_back(std::move(run));
+            } else {
+                s->give(std::move(t.new_run));
+            }
+            algo_bytes += settle_runs(s, ts, false);
             std::swap(ts.delta, ts.newt);
             ts.version += 1;
             total_new += ts.delta.rows;
         }
         ++iterations;
     }
+    // FULL leaves the run as one sorted array
+    for (uint32_t i = 0; i < n_relations; ++i) algo_bytes += settle_runs(s, *find(s, relations[i]), true);
     check(hipEventRecord(t_end, s->stream));
     check(hipEventSynchronize(t_end));
+    if (s->profiling) s->collect_ops();
     float ms = 0.f;
     check(hipEventElapsedTime(&ms, t_begin, t_end));
     (void)hipEventDestroy(t_begin);

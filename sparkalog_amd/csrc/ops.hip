@@ -32,7 +32,10 @@ constexpr int kCompactTile = kBlock * kCompactItems; // 4096 rows
 constexpr int kMergeItems = 8;
 constexpr int kMergeTile = kBlock * kMergeItems;     // 2048 merged rows
 constexpr int kAntiTile = 2048;                      // left rows per tile
-constexpr int kAntiStage = 4096;                     // right rows staged in LDS
+constexpr int kAntiStage = 2048;                     // right rows staged in LDS per chunk
+// Above this many right rows per 2048-row left tile, reading the range costs
+// more HBM traffic than ~log2(span) probes per left row.
+constexpr size_t kAntiStreamLimit = 1u << 18;
 constexpr int kEmitTile = 2048;                      // output rows per tile
 constexpr int kEmitStage = 2048;                     // left rows staged in LDS
 
@@ -77,6 +80,7 @@ __device__ __forceinline__ uint64_t load_key(const PairInput& in, size_t i) {
 
 __device__ __forceinline__ void store_key(const PairOutput& out, size_t i, uint64_t k) {
     if (out.packed) out.packed[i] = k;
+    if (out.packed_copy) out.packed_copy[i] = k;
     if (out.first) {
         out.first[i] = static_cast<uint32_t>(k >> 32);
         out.second[i] = static_cast<uint32_t>(k);
@@ -462,14 +466,30 @@ __device__ __forceinline__ uint32_t merge_path_lds(const uint64_t* a, uint32_t n
     return lo;
 }
 
+// Merge-path partition of every tile boundary: part[t] = number of left rows
+// among the first t*kMergeTile merged rows. One thread per boundary, so the
+// searches of all tiles overlap instead of stalling each tile's start.
+__global__ void merge_partition_kernel(PairInput left, size_t nl, PairInput right, size_t nr, size_t* part,
+                                       size_t n_parts) {
+    nl = actual_rows(left, nl);
+    nr = actual_rows(right, nr);
+    const size_t total = nl + nr;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n_parts; t += stride) {
+        const size_t d = t * kMergeTile < total ? t * kMergeTile : total;
+        part[t] = merge_path(left, nl, right, nr, d);
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void union_kernel(PairInput left, size_t nl, PairInput right, size_t nr,
-                                                       PairOutput out, uint64_t* unique_rows, uint64_t* status,
-                                                       Header* header) {
+                                                       PairOutput out, uint64_t* unique_rows, const size_t* part,
+                                                       uint64_t* status, Header* header) {
     __shared__ uint64_t lds[kMergeTile + 2];
     __shared__ uint32_t scan_lds[kWavesPerBlock];
-    __shared__ size_t part[2];
     __shared__ uint64_t offset_slot;
     __shared__ uint32_t tile_slot;
+    __shared__ uint64_t prev_slot;
+    __shared__ int have_prev_slot;
     nl = actual_rows(left, nl);
     nr = actual_rows(right, nr);
     const uint32_t tile = next_tile(&header->counters[0], &tile_slot);
@@ -481,9 +501,7 @@ __global__ __launch_bounds__(kBlock) void union_kernel(PairInput left, size_t nl
     }
     if (d0 >= total) return;
     const size_t d1 = d0 + kMergeTile < total ? d0 + kMergeTile : total;
-    if (threadIdx.x < 2) part[threadIdx.x] = merge_path(left, nl, right, nr, threadIdx.x ? d1 : d0);
-    __syncthreads();
-    const size_t a0 = part[0], a1 = part[1];
+    const size_t a0 = part[tile], a1 = part[tile + 1];
     const size_t b0 = d0 - a0, b1 = d1 - a1;
     const uint32_t na = static_cast<uint32_t>(a1 - a0), nb = static_cast<uint32_t>(b1 - b0);
     uint64_t* la = lds;
@@ -492,8 +510,6 @@ __global__ __launch_bounds__(kBlock) void union_kernel(PairInput left, size_t nl
     for (uint32_t i = threadIdx.x; i < nb; i += kBlock) lb[i] = load_key(right, b0 + i);
     // element preceding the tile in merged order: the larger of the last A
     // and last B consumed before it
-    __shared__ uint64_t prev_slot;
-    __shared__ int have_prev_slot;
     if (threadIdx.x == 0) {
         uint64_t p = 0;
         int h = 0;
@@ -580,18 +596,48 @@ __device__ __forceinline__ size_t lower_bound_key(const PairInput& r, size_t lo,
     return lo;
 }
 
-__global__ __launch_bounds__(kBlock) void anti_join_kernel(PairInput left, size_t nl, PairInput right, size_t nr,
-                                                           PairOutput out, uint32_t* count32, uint64_t* count64,
-                                                           uint64_t* status, Header* header) {
+// For every left tile and every right input: the right rows that can match a
+// sorted tile, [lower_bound(first key), upper_bound(last key)). One thread per
+// (tile, right input).
+__global__ void anti_partition_kernel(PairInput left, size_t nl, AntiRight rights, size_t n_tiles,
+                                      size_t* ranges) {
+    nl = actual_rows(left, nl);
+    const size_t tiles = tiles_of(nl, kAntiTile) < n_tiles ? tiles_of(nl, kAntiTile) : n_tiles;
+    const size_t jobs = tiles * rights.count;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t j = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < jobs; j += stride) {
+        const size_t t = j / rights.count;
+        const uint32_t r = static_cast<uint32_t>(j % rights.count);
+        const PairInput& right = rights.in[r];
+        const size_t nr = rights.rows[r];
+        const size_t first = t * kAntiTile;
+        const size_t last = (first + kAntiTile < nl ? first + kAntiTile : nl) - 1;
+        const uint64_t lo_key = load_key(left, first), hi_key = load_key(left, last);
+        const size_t lo = lower_bound_key(right, 0, nr, lo_key);
+        size_t hi = lower_bound_key(right, lo, nr, hi_key);
+        if (hi < nr && load_key(right, hi) == hi_key) ++hi;
+        ranges[2 * j] = lo;
+        ranges[2 * j + 1] = hi > lo ? hi : lo;
+    }
+}
+
+// left \ (right_0 ∪ right_1 ∪ ...), keeping left order. Layout: wave w owns
+// tile rows [w*512, w*512+512); item i of lane j is row w*512 + i*64 + j, so
+// every load and the ballot compaction are coalesced and order-preserving.
+// Each right input's candidate range for the tile comes from the partition
+// pass (or, for a tile that is not sorted, from a block-cooperative search);
+// ranges are streamed through LDS once, or probed in place when huge.
+__global__ __launch_bounds__(kBlock) void anti_join_kernel(PairInput left, size_t nl, AntiRight rights,
+                                                           const size_t* ranges, PairOutput out, uint32_t* count32,
+                                                           uint64_t* count64, uint64_t* status, Header* header) {
     constexpr int kItems = kAntiTile / kBlock;  // 8
+    constexpr int kWaveRows = kItems * kWave;   // 512
     __shared__ uint64_t stage[kAntiStage];
-    __shared__ uint64_t outbuf[kAntiTile];
-    __shared__ uint32_t scan_lds[kWavesPerBlock];
-    __shared__ size_t range[2];
+    __shared__ uint32_t wave_totals[kWavesPerBlock][kItems];
+    __shared__ uint64_t red_min[kWavesPerBlock], red_max[kWavesPerBlock];
     __shared__ uint64_t offset_slot;
     __shared__ uint32_t tile_slot;
     nl = actual_rows(left, nl);
-    nr = actual_rows(right, nr);
     const uint32_t tile = next_tile(&header->counters[0], &tile_slot);
     const size_t base = static_cast<size_t>(tile) * kAntiTile;
     if (nl == 0) {
@@ -602,91 +648,120 @@ __global__ __launch_bounds__(kBlock) void anti_join_kernel(PairInput left, size_
         return;
     }
     if (base >= nl) return;
-    const uint32_t rows = static_cast<uint32_t>(nl - base < static_cast<size_t>(kAntiTile) ? nl - base : kAntiTile);
-    // blocked arrangement: thread t owns left rows t*8 .. t*8+7 of the tile
+    const uint32_t lane = lane_id(), wave = wave_id();
+    const size_t rows = nl - base < static_cast<size_t>(kAntiTile) ? nl - base : kAntiTile;
+    const uint64_t first_key = load_key(left, base), last_key = load_key(left, base + rows - 1);
     uint64_t k[kItems];
     uint64_t kmin = ~0ull, kmax = 0;
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
-        const uint32_t li = threadIdx.x * kItems + i;
-        k[i] = li < rows ? load_key(left, base + li) : 0;
-        if (li < rows) {
+        const size_t r = wave * kWaveRows + i * kWave + lane;
+        k[i] = r < rows ? load_key(left, base + r) : 0;
+        if (r < rows) {
             kmin = k[i] < kmin ? k[i] : kmin;
             kmax = k[i] > kmax ? k[i] : kmax;
         }
     }
-    // The tile's key range bounds the right rows that can match (min/max
-    // rather than first/last keeps pure membership semantics for any order).
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) {
         const uint64_t omin = __shfl_xor(kmin, off, kWave), omax = __shfl_xor(kmax, off, kWave);
         kmin = omin < kmin ? omin : kmin;
         kmax = omax > kmax ? omax : kmax;
     }
-    __shared__ uint64_t red_min[kWavesPerBlock], red_max[kWavesPerBlock];
-    if (lane_id() == 0) {
-        red_min[wave_id()] = kmin;
-        red_max[wave_id()] = kmax;
+    if (lane == 0) {
+        red_min[wave] = kmin;
+        red_max[wave] = kmax;
     }
     __syncthreads();
-    if (threadIdx.x < 2) {
-        uint64_t lo_key = ~0ull, hi_key = 0;
-        for (int w = 0; w < kWavesPerBlock; ++w) {
-            lo_key = red_min[w] < lo_key ? red_min[w] : lo_key;
-            hi_key = red_max[w] > hi_key ? red_max[w] : hi_key;
-        }
-        if (threadIdx.x == 0) {
-            range[0] = lower_bound_key(right, 0, nr, lo_key);
+    uint64_t lo_key = ~0ull, hi_key = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w) {
+        lo_key = red_min[w] < lo_key ? red_min[w] : lo_key;
+        hi_key = red_max[w] > hi_key ? red_max[w] : hi_key;
+    }
+    // a sorted tile may use the precomputed ranges; any other order gets its
+    // own (pure membership semantics for every input order)
+    const bool sorted_tile = ranges != nullptr && lo_key == first_key && hi_key == last_key;
+    uint32_t found = 0;  // bit i: item i matched
+    for (uint32_t r = 0; r < rights.count; ++r) {
+        const PairInput& right = rights.in[r];
+        const size_t nr = rights.rows[r];
+        size_t r0, r1;
+        if (sorted_tile) {
+            const size_t j = static_cast<size_t>(tile) * rights.count + r;
+            r0 = ranges[2 * j];
+            r1 = ranges[2 * j + 1];
         } else {
-            const size_t lb = lower_bound_key(right, 0, nr, hi_key);
-            range[1] = (lb < nr && load_key(right, lb) == hi_key) ? lb + 1 : lb;
+            r0 = block_search(0, nr, [&](size_t i) { return load_key(right, i) < lo_key; });
+            r1 = block_search(r0, nr, [&](size_t i) { return load_key(right, i) <= hi_key; });
+        }
+        const size_t span = r1 - r0;
+        if (span <= kAntiStreamLimit) {
+            for (size_t c0 = r0; c0 < r1; c0 += kAntiStage) {
+                const uint32_t n =
+                    static_cast<uint32_t>(r1 - c0 < static_cast<size_t>(kAntiStage) ? r1 - c0 : kAntiStage);
+                for (uint32_t i = threadIdx.x; i < n; i += kBlock) stage[i] = load_key(right, c0 + i);
+                __syncthreads();
+                const uint64_t cmin = stage[0], cmax = stage[n - 1];
+#pragma unroll
+                for (int i = 0; i < kItems; ++i) {
+                    const size_t row = wave * kWaveRows + i * kWave + lane;
+                    if (row < rows && !((found >> i) & 1u) && k[i] >= cmin && k[i] <= cmax) {
+                        uint32_t lo = 0, hi = n;
+                        while (lo < hi) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            if (stage[mid] < k[i])
+                                lo = mid + 1;
+                            else
+                                hi = mid;
+                        }
+                        if (lo < n && stage[lo] == k[i]) found |= 1u << i;
+                    }
+                }
+                __syncthreads();
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < kItems; ++i) {
+                const size_t row = wave * kWaveRows + i * kWave + lane;
+                if (row < rows && !((found >> i) & 1u)) {
+                    const size_t p = lower_bound_key(right, r0, r1, k[i]);
+                    if (p < r1 && load_key(right, p) == k[i]) found |= 1u << i;
+                }
+            }
         }
     }
-    __syncthreads();
-    const size_t r0 = range[0];
-    const size_t r1 = range[1] > r0 ? range[1] : r0;
-    const size_t span = r1 - r0;
-    const bool staged = span <= static_cast<size_t>(kAntiStage);
-    if (staged)
-        for (uint32_t i = threadIdx.x; i < span; i += kBlock) stage[i] = load_key(right, r0 + i);
-    __syncthreads();
-    uint32_t keep = 0;
+    // order-preserving compaction of the rows not found
+    uint64_t masks[kItems];
+    uint32_t wave_count = 0;
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
-        const uint32_t li = threadIdx.x * kItems + i;
-        if (li < rows) {
-            bool found;
-            if (staged) {
-                uint32_t lo = 0, hi = static_cast<uint32_t>(span);
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (stage[mid] < k[i])
-                        lo = mid + 1;
-                    else
-                        hi = mid;
-                }
-                found = lo < span && stage[lo] == k[i];
-            } else {
-                const size_t p = lower_bound_key(right, r0, r1, k[i]);
-                found = p < r1 && load_key(right, p) == k[i];
-            }
-            keep |= static_cast<uint32_t>(!found) << i;
-        }
+        const size_t row = wave * kWaveRows + i * kWave + lane;
+        masks[i] = ballot(row < rows && !((found >> i) & 1u));
+        if (lane == 0) wave_totals[wave][i] = static_cast<uint32_t>(__popcll(masks[i]));
+        wave_count += static_cast<uint32_t>(__popcll(masks[i]));
     }
-    const uint32_t count = __popc(keep);
-    uint32_t tile_total;
-    const uint32_t local = block_exclusive_sum<uint32_t>(count, &tile_total, scan_lds);
-    if (wave_id() == 0) {
+    __syncthreads();
+    uint32_t wave_start = 0, tile_total = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w) {
+        uint32_t c = 0;
+        for (int i = 0; i < kItems; ++i) c += wave_totals[w][i];
+        if (w < static_cast<int>(wave)) wave_start += c;
+        tile_total += c;
+    }
+    (void)wave_count;
+    if (wave == 0) {
         const uint64_t excl = wave_lookback(status, tile, tile_total, 0, &header->error);
-        if (lane_id() == 0) offset_slot = excl;
+        if (lane == 0) offset_slot = excl;
     }
-    uint32_t pos = local;
+    uint32_t pos = wave_start;
 #pragma unroll
-    for (int i = 0; i < kItems; ++i)
-        if ((keep >> i) & 1u) outbuf[pos++] = k[i];
+    for (int i = 0; i < kItems; ++i) {
+        if ((masks[i] >> lane) & 1ull) stage[pos + lanes_below(masks[i])] = k[i];
+        pos += static_cast<uint32_t>(__popcll(masks[i]));
+    }
     __syncthreads();
     const uint64_t offset = offset_slot;
-    for (uint32_t i = threadIdx.x; i < tile_total; i += kBlock) store_key(out, offset + i, outbuf[i]);
+    for (uint32_t i = threadIdx.x; i < tile_total; i += kBlock) store_key(out, offset + i, stage[i]);
     if (tile == tiles_of(nl, kAntiTile) - 1 && threadIdx.x == 0) {
         if (count32) *count32 = static_cast<uint32_t>(offset + tile_total);
         if (count64) *count64 = offset + tile_total;
@@ -778,21 +853,10 @@ __global__ __launch_bounds__(kBlock) void join_emit_kernel(const uint32_t* left_
     for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         const uint64_t o0 = tile * kEmitTile;
         const uint64_t o1 = o0 + kEmitTile < total ? o0 + kEmitTile : total;
-        if (threadIdx.x < 2) {
-            // first row whose range contains o (upper_bound(offsets, o) - 1)
-            const uint64_t o = threadIdx.x ? o1 - 1 : o0;
-            size_t lo = 0, hi = nl;
-            while (lo < hi) {
-                const size_t mid = (lo + hi) >> 1;
-                if (offsets[mid] <= o)
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
-            bounds[threadIdx.x] = lo - 1;
-        }
-        __syncthreads();
-        const size_t l0 = bounds[0], l1 = bounds[1] + 1;  // rows [l0, l1)
+        // rows [l0, l1) cover outputs [o0, o1): l = upper_bound(offsets, o) - 1
+        const size_t l0 = block_search(0, nl, [&](size_t i) { return offsets[i] <= o0; }) - 1;
+        const size_t l1 = block_search(l0, nl, [&](size_t i) { return offsets[i] <= o1 - 1; });
+        (void)bounds;
         const size_t span = l1 - l0;
         const bool staged = span <= static_cast<size_t>(kEmitStage);
         if (staged) {
@@ -986,7 +1050,8 @@ hipError_t distinct(PairInput in, size_t rows, uint64_t* keys_a, uint64_t* keys_
 }
 
 size_t union_temp_bytes(size_t left_rows, size_t right_rows) {
-    return compact_temp(tiles_of(left_rows + right_rows, kMergeTile));
+    const size_t tiles = tiles_of(left_rows + right_rows, kMergeTile);
+    return compact_temp(tiles) + align_up((tiles + 1) * sizeof(size_t), 256);
 }
 
 hipError_t sorted_union(PairInput left, size_t left_rows, PairInput right, size_t right_rows, PairOutput out,
@@ -997,33 +1062,55 @@ hipError_t sorted_union(PairInput left, size_t left_rows, PairInput right, size_
         return hipGetLastError();
     }
     const size_t tiles = tiles_of(total, kMergeTile);
-    if (temp_bytes < compact_temp(tiles)) return hipErrorInvalidValue;
+    if (temp_bytes < union_temp_bytes(left_rows, right_rows)) return hipErrorInvalidValue;
     hipError_t st = prepare_compact(temp, tiles, stream);
     if (st != hipSuccess) return st;
+    size_t* part = reinterpret_cast<size_t*>(static_cast<char*>(temp) + compact_temp(tiles));
+    merge_partition_kernel<<<grid_for(tiles + 1, kBlock, 4096), kBlock, 0, stream>>>(left, left_rows, right,
+                                                                                    right_rows, part, tiles + 1);
     union_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(
-        left, left_rows, right, right_rows, out, unique_rows,
+        left, left_rows, right, right_rows, out, unique_rows, part,
         reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp));
     return hipGetLastError();
 }
 
-size_t anti_join_temp_bytes(size_t left_rows) { return compact_temp(tiles_of(left_rows, kAntiTile)); }
+size_t anti_join_temp_bytes(size_t left_rows, uint32_t right_inputs) {
+    const size_t tiles = tiles_of(left_rows, kAntiTile);
+    return compact_temp(tiles) + align_up(tiles * (right_inputs ? right_inputs : 1) * 2 * sizeof(size_t), 256);
+}
 
-hipError_t sorted_anti_join(PairInput left, size_t left_rows, PairInput right, size_t right_rows, PairOutput out,
-                            uint32_t* count32, uint64_t* count64, void* temp, size_t temp_bytes,
-                            hipStream_t stream) {
+hipError_t sorted_anti_join_multi(PairInput left, size_t left_rows, const AntiRight& rights, PairOutput out,
+                                  uint32_t* count32, uint64_t* count64, void* temp, size_t temp_bytes,
+                                  hipStream_t stream) {
     if (left_rows == 0) {
         if (count32) set_u32_kernel<<<1, 1, 0, stream>>>(count32, 0);
         if (count64) set_u64_kernel<<<1, 1, 0, stream>>>(count64, 0);
         return hipGetLastError();
     }
+    if (rights.count > static_cast<uint32_t>(kMaxAntiRights)) return hipErrorInvalidValue;
     const size_t tiles = tiles_of(left_rows, kAntiTile);
-    if (temp_bytes < compact_temp(tiles)) return hipErrorInvalidValue;
+    if (temp_bytes < anti_join_temp_bytes(left_rows, rights.count)) return hipErrorInvalidValue;
     hipError_t st = prepare_compact(temp, tiles, stream);
     if (st != hipSuccess) return st;
+    size_t* ranges = reinterpret_cast<size_t*>(static_cast<char*>(temp) + compact_temp(tiles));
+    if (rights.count) {
+        anti_partition_kernel<<<grid_for(tiles * rights.count, kBlock, 4096), kBlock, 0, stream>>>(
+            left, left_rows, rights, tiles, ranges);
+    }
     anti_join_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(
-        left, left_rows, right, right_rows, out, count32, count64,
+        left, left_rows, rights, ranges, out, count32, count64,
         reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp));
     return hipGetLastError();
+}
+
+hipError_t sorted_anti_join(PairInput left, size_t left_rows, PairInput right, size_t right_rows, PairOutput out,
+                            uint32_t* count32, uint64_t* count64, void* temp, size_t temp_bytes,
+                            hipStream_t stream) {
+    AntiRight rights{};
+    rights.count = right_rows ? 1 : 0;
+    rights.in[0] = right;
+    rights.rows[0] = right_rows;
+    return sorted_anti_join_multi(left, left_rows, rights, out, count32, count64, temp, temp_bytes, stream);
 }
 
 size_t join_temp_bytes(size_t left_rows) { return compact_temp(tiles_of(left_rows, kBlock * 8)); }

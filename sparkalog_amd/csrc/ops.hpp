@@ -26,6 +26,7 @@ struct PairOutput {
     uint64_t* packed;   // written when non-null
     uint32_t* first;    // written when non-null
     uint32_t* second;
+    uint64_t* packed_copy = nullptr;  // second packed copy, written when non-null
 };
 
 // ---- radix sort + unique (distinct) ----
@@ -45,18 +46,31 @@ hipError_t radix_sort(PairInput in, size_t rows, uint64_t* keys_a, uint64_t* key
                       void* temp, size_t temp_bytes, uint64_t known_varying, hipStream_t stream,
                       uint64_t** sorted_host);
 
+// Right inputs of a multi-way anti-join: sorted relations (at most
+// kMaxAntiRights), rows known on the host.
+constexpr int kMaxAntiRights = 16;
+struct AntiRight {
+    PairInput in[kMaxAntiRights];
+    uint64_t rows[kMaxAntiRights];
+    uint32_t count;
+};
+
 // ---- sorted set operators ----
 size_t union_temp_bytes(size_t left_rows, size_t right_rows);
 hipError_t sorted_union(PairInput left, size_t left_rows, PairInput right, size_t right_rows,
                         PairOutput out, uint64_t* unique_rows, void* temp, size_t temp_bytes,
                         hipStream_t stream);
 
-size_t anti_join_temp_bytes(size_t left_rows);
+size_t anti_join_temp_bytes(size_t left_rows, uint32_t right_inputs = 1);
 // left \ right; `count32`/`count64` receive the output row count (either may
 // be null).
 hipError_t sorted_anti_join(PairInput left, size_t left_rows, PairInput right, size_t right_rows,
                             PairOutput out, uint32_t* count32, uint64_t* count64, void* temp,
                             size_t temp_bytes, hipStream_t stream);
+// left \ (rights[0] ∪ rights[1] ∪ ...) in one pass over the left input.
+hipError_t sorted_anti_join_multi(PairInput left, size_t left_rows, const AntiRight& rights, PairOutput out,
+                                  uint32_t* count32, uint64_t* count64, void* temp, size_t temp_bytes,
+                                  hipStream_t stream);
 
 // ---- indexed join (U32RangeIndex layout) ----
 size_t join_temp_bytes(size_t left_rows);

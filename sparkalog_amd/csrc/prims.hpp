@@ -138,6 +138,30 @@ __device__ __forceinline__ uint32_t next_tile(uint32_t* counter, uint32_t* lds_s
     return t;
 }
 
+// Block-cooperative search: the first index i in [lo, hi) where the monotone
+// predicate `before(i)` (true, ..., true, false, ..., false) is false, or hi.
+// Every thread of the block must call it with the same arguments. Each step
+// probes 256 evenly spaced indexes in parallel, so a range of 2^32 rows takes
+// four dependent rounds instead of 32.
+template <typename Pred>
+__device__ __forceinline__ size_t block_search(size_t lo, size_t hi, Pred before) {
+    while (lo < hi) {
+        const size_t span = hi - lo;
+        const size_t stride = span <= static_cast<size_t>(kBlock) ? 1 : (span + kBlock - 1) / kBlock;
+        const size_t probe = lo + static_cast<size_t>(threadIdx.x) * stride;
+        const bool in_range = probe < hi;
+        const int count = __syncthreads_count(in_range && before(probe));
+        if (count == 0) return lo;
+        if (stride == 1) return lo + static_cast<size_t>(count);
+        // answer in (probe[count-1], probe[count]]
+        const size_t last_true = lo + static_cast<size_t>(count - 1) * stride;
+        const size_t next = lo + static_cast<size_t>(count) * stride;
+        lo = last_true + 1;
+        if (next < hi) hi = next;
+    }
+    return lo;
+}
+
 __device__ __forceinline__ uint64_t pack_pair(uint32_t a, uint32_t b) {
     return (static_cast<uint64_t>(a) << 32) | b;
 }
