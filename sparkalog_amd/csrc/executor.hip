@@ -436,7 +436,7 @@ struct RuleRuntime {
     DevBuf<uint64_t> status;    // look-back words of the join count
     DevBuf<uint64_t> offsets;   // join offsets per delta row
     DevBuf<size_t> row_start;   // first delta row of each emit tile
-    DevBuf<uint64_t> cand_a, cand_b;
+    DevBuf<uint64_t> cand_a, cand_b, cand_c;
     DevBuf<uint64_t> distinct_out;
     uint64_t join_rows = 0;     // M (host, after the first sync)
     int distinct_slot = -1;     // profiling event of this iteration's distinct
@@ -484,6 +484,7 @@ struct sparkalog_hip_store {
     uint64_t generation = 0;                    // bumped by every insert; versions derive from it
     std::vector<spk::DevBuf<uint64_t>> pool;    // released run buffers, reused by later merges
     uint32_t merge_ratio = 4;                   // merge two runs when the older is < ratio x the newer
+    bool segmented_distinct = true;             // sort grouped candidates per run in LDS
 
     spk::DevBuf<uint64_t> take(size_t n) {
         size_t best = pool.size();
@@ -1100,15 +1101,27 @@ extern "C" int sparkalog_hip_executor_run(sparkalog_hip_store* s, const uint32_t
             s->end_op(ev, kOpJoin, 8.0 * rt.index.rows + 8.0 * M);
             ev = s->begin_op();
             rt.distinct_slot = ev;
-            const size_t tb = distinct_temp_bytes(M);
-            void* temp = s->scratch(tb);
-            PairInput in{rt.cand_a.ptr, nullptr, nullptr, nullptr};
-            PairOutput out{rt.distinct_out.ptr, nullptr, nullptr};
-            check(distinct(in, M, rt.cand_a.ptr, rt.cand_b.ptr, out, &cdev->distinct_rows[r], temp, tb,
-                           known_compressed, s->stream));
-            if (rt.shift != 32)
-                expand_keys_kernel<<<grid_for(M), kBlock, 0, s->stream>>>(rt.distinct_out.ptr, &cdev->distinct_rows[r],
-                                                                         M, rt.shift);
+            // The join writes candidates in DELTA order; when the distinct's major
+            // column is DELTA's first column they are grouped by it already.
+            const bool grouped = spec.side[spec.order[0]] == 0 && spec.col[spec.order[0]] == 0;
+            if (grouped && s->segmented_distinct) {
+                rt.cand_c.reserve(M ? M : 1);
+                const size_t tb = segmented_distinct_temp_bytes(M);
+                void* temp = s->scratch(tb);
+                PairOutput out{rt.distinct_out.ptr, nullptr, nullptr};
+                check(segmented_distinct(rt.cand_a.ptr, M, rt.shift, known_compressed, rt.cand_b.ptr, rt.cand_c.ptr,
+                                         out, &cdev->distinct_rows[r], temp, tb, s->stream));
+            } else {
+                const size_t tb = distinct_temp_bytes(M);
+                void* temp = s->scratch(tb);
+                PairInput in{rt.cand_a.ptr, nullptr, nullptr, nullptr};
+                PairOutput out{rt.distinct_out.ptr, nullptr, nullptr};
+                check(distinct(in, M, rt.cand_a.ptr, rt.cand_b.ptr, out, &cdev->distinct_rows[r], temp, tb,
+                               known_compressed, s->stream));
+                if (rt.shift != 32)
+                    expand_keys_kernel<<<grid_for(M), kBlock, 0, s->stream>>>(rt.distinct_out.ptr,
+                                                                             &cdev->distinct_rows[r], M, rt.shift);
+            }
             check(hipGetLastError());
             s->end_op(ev, kOpDistinct, 16.0 * M);  // + 8U added after the sync
         }

@@ -334,8 +334,8 @@ __global__ __launch_bounds__(kBlock) void sort_pass_kernel(uint64_t* keys_a, uin
 // sort descriptor says). Writes unique keys to `out` in order and the count.
 __global__ __launch_bounds__(kBlock) void unique_kernel(const uint64_t* keys_a, const uint64_t* keys_b,
                                                         const SortDesc* sort_desc, int sorted_in_b,
-                                                        size_t rows, PairOutput out, uint64_t* unique_rows,
-                                                        uint64_t* status, Header* header) {
+                                                        size_t rows, const uint64_t* rows_dev, PairOutput out,
+                                                        uint64_t* unique_rows, uint64_t* status, Header* header) {
     __shared__ uint64_t stage[kCompactTile];
     __shared__ uint32_t scan_lds[kWavesPerBlock];
     __shared__ uint64_t offset_slot;
@@ -347,6 +347,7 @@ __global__ __launch_bounds__(kBlock) void unique_kernel(const uint64_t* keys_a, 
     } else if (sorted_in_b) {
         keys = keys_b;
     }
+    if (rows_dev) rows = *rows_dev < rows ? *rows_dev : rows;
     const uint32_t tile = next_tile(&header->counters[0], &tile_slot);
     const size_t base = static_cast<size_t>(tile) * kCompactTile;
     if (rows == 0) {
@@ -1044,8 +1045,21 @@ hipError_t distinct(PairInput in, size_t rows, uint64_t* keys_a, uint64_t* keys_
     st = clear_u64(ustatus, l.unique_tiles, stream);
     if (st != hipSuccess) return st;
     const SortDesc* desc = reinterpret_cast<const SortDesc*>(sort_temp + l.desc_off);
-    unique_kernel<<<static_cast<unsigned>(l.unique_tiles), kBlock, 0, stream>>>(keys_a, keys_b, desc, 0, rows, out,
-                                                                              unique_rows, ustatus, header);
+    unique_kernel<<<static_cast<unsigned>(l.unique_tiles), kBlock, 0, stream>>>(keys_a, keys_b, desc, 0, rows, nullptr,
+                                                                              out, unique_rows, ustatus, header);
+    return hipGetLastError();
+}
+
+hipError_t unique_sorted(const uint64_t* keys, const uint64_t* rows_dev, size_t bound, PairOutput out,
+                         uint64_t* unique_rows, void* temp, size_t temp_bytes, hipStream_t stream) {
+    if (bound == 0) return set_u64(unique_rows, 0, stream);
+    const size_t tiles = tiles_of(bound, kCompactTile);
+    if (temp_bytes < compact_temp(tiles)) return hipErrorInvalidValue;
+    hipError_t st = prepare_compact(temp, tiles, stream);
+    if (st != hipSuccess) return st;
+    unique_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(
+        keys, keys, nullptr, 0, bound, rows_dev, out, unique_rows,
+        reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp));
     return hipGetLastError();
 }
 

@@ -55,6 +55,21 @@ struct AntiRight {
     uint32_t count;
 };
 
+// Unique compaction of sorted keys; the row count is *rows_dev (<= bound)
+// when rows_dev is non-null.
+hipError_t unique_sorted(const uint64_t* keys, const uint64_t* rows_dev, size_t bound, PairOutput out,
+                         uint64_t* unique_rows, void* temp, size_t temp_bytes, hipStream_t stream);
+
+// Distinct of compressed keys (major << minor_bits | minor) whose major part
+// is non-decreasing along the input; writes sorted unique expanded pairs
+// (major << 32 | minor). side_a/side_b (rows entries each) hold keys of runs
+// too long for one tile; side_varying is a superset of the varying bits of
+// the compressed keys.
+size_t segmented_distinct_temp_bytes(size_t rows);
+hipError_t segmented_distinct(const uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t side_varying,
+                              uint64_t* side_a, uint64_t* side_b, PairOutput out, uint64_t* unique_rows, void* temp,
+                              size_t temp_bytes, hipStream_t stream);
+
 // ---- sorted set operators ----
 size_t union_temp_bytes(size_t left_rows, size_t right_rows);
 hipError_t sorted_union(PairInput left, size_t left_rows, PairInput right, size_t right_rows,

@@ -1,0 +1,378 @@
+// Segmented distinct for keys whose major part is already non-decreasing.
+//
+// In the semi-naive loop the join emits candidates in DELTA order, and DELTA
+// is sorted by its first column, so whenever the first output column is
+// DELTA's first column the candidates arrive grouped by that column: only the
+// runs of one major value need sorting. This replaces the multi-pass global
+// radix sort (read + write of every key per 8-bit digit) by one read and one
+// write per key:
+//   1. partition: tile boundaries are moved back to the start of the run
+//      that contains them (one thread per boundary), so no run crosses a tile;
+//   2. runs longer than a tile go, gathered, through the global onesweep sort
+//      (rare: a run is one source's candidates in one iteration);
+//   3. every tile sorts its keys in LDS by (local run rank, minor) with stable
+//      8-bit LSD passes over the varying digits only, drops duplicates, and
+//      writes the expanded pairs (major << 32 | minor) at an offset found by
+//      decoupled look-back.
+#include "ops.hpp"
+#include "prims.hpp"
+
+namespace spk {
+namespace {
+
+constexpr int kSegTile = 2048;                // nominal tile
+constexpr int kSegCap = 4096;                 // largest tile sorted in LDS
+constexpr int kSegItems = kSegCap / kBlock;   // 16 keys per thread
+constexpr int kSegWaveRows = kSegItems * kWave;
+constexpr int kSegRadix = 256;
+
+__device__ __forceinline__ void store_pair(const PairOutput& out, size_t i, uint64_t k) {
+    if (out.packed) out.packed[i] = k;
+    if (out.packed_copy) out.packed_copy[i] = k;
+    if (out.first) {
+        out.first[i] = static_cast<uint32_t>(k >> 32);
+        out.second[i] = static_cast<uint32_t>(k);
+    }
+}
+
+__host__ __device__ inline size_t seg_tiles(size_t n) { return (n + kSegTile - 1) / kSegTile; }
+__host__ __device__ inline size_t seg_align(size_t v) { return (v + 255) / 256 * 256; }
+
+struct SegHeader {
+    uint32_t error;
+    uint32_t counter;
+    uint64_t side_rows;     // keys of oversized runs gathered for the global sort
+    uint64_t side_unique;   // their unique count
+};
+
+struct SegLayout {
+    size_t status_off, bounds_off, sort_off, total, tiles;
+};
+
+SegLayout seg_layout(size_t rows) {
+    SegLayout l{};
+    l.tiles = seg_tiles(rows);
+    size_t off = seg_align(sizeof(SegHeader));
+    l.status_off = off;
+    off += seg_align(l.tiles * sizeof(uint64_t));
+    l.bounds_off = off;
+    off += seg_align((l.tiles + 1) * sizeof(size_t));
+    l.sort_off = off;
+    off += distinct_temp_bytes(rows);
+    l.total = off;
+    return l;
+}
+
+__global__ void seg_init_kernel(SegHeader* h, uint64_t* status, size_t tiles) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    const size_t gtid = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gtid == 0) {
+        h->error = 0;
+        h->counter = 0;
+        h->side_rows = 0;
+        h->side_unique = 0;
+    }
+    for (size_t i = gtid; i < tiles; i += stride) status[i] = 0;
+}
+
+// bounds[t] = start of the run containing position t*kSegTile (or rows).
+__global__ void seg_partition_kernel(const uint64_t* keys, size_t rows, uint32_t minor_bits, size_t* bounds,
+                                     size_t tiles) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; t <= tiles; t += stride) {
+        const size_t p = t * kSegTile;
+        if (p >= rows) {
+            bounds[t] = rows;
+            continue;
+        }
+        const uint64_t major = keys[p] >> minor_bits;
+        size_t lo = 0, hi = p;
+        while (lo < hi) {
+            const size_t mid = (lo + hi) >> 1;
+            if ((keys[mid] >> minor_bits) < major)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        bounds[t] = lo;
+    }
+}
+
+// Copy the keys of tiles larger than kSegCap to the side buffer (any order:
+// they are sorted globally afterwards).
+__global__ __launch_bounds__(kBlock) void seg_gather_kernel(const uint64_t* keys, const size_t* bounds, size_t tiles,
+                                                            uint64_t* side, SegHeader* h) {
+    __shared__ uint64_t slot;
+    for (size_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const size_t b0 = bounds[t], b1 = bounds[t + 1];
+        if (b1 - b0 <= static_cast<size_t>(kSegCap)) continue;  // block-uniform
+        if (threadIdx.x == 0) slot = atomicAdd(reinterpret_cast<unsigned long long*>(&h->side_rows),
+                                               static_cast<unsigned long long>(b1 - b0));
+        __syncthreads();
+        const uint64_t base = slot;
+        for (size_t i = b0 + threadIdx.x; i < b1; i += kBlock) side[base + (i - b0)] = keys[i];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void seg_tile_kernel(const uint64_t* keys, size_t rows, uint32_t minor_bits,
+                                                          const size_t* bounds, const uint64_t* side_sorted,
+                                                          PairOutput out, uint64_t* unique_rows, uint64_t* status,
+                                                          SegHeader* h) {
+    __shared__ uint64_t tile_keys[kSegCap];
+    __shared__ uint32_t major_of[kSegCap];
+    __shared__ uint32_t wave_count[kWavesPerBlock][kSegRadix];
+    __shared__ uint32_t item_totals[kWavesPerBlock][kSegItems];
+    __shared__ uint32_t scan_lds[kWavesPerBlock];
+    __shared__ uint64_t red_or[kWavesPerBlock], red_and[kWavesPerBlock];
+    __shared__ uint64_t offset_slot;
+    __shared__ uint32_t tile_slot;
+    const uint32_t tile = next_tile(&h->counter, &tile_slot);
+    const size_t tiles = seg_tiles(rows);
+    if (tile >= tiles) return;
+    const uint32_t lane = lane_id(), wave = wave_id();
+    const size_t b0 = bounds[tile], b1 = bounds[tile + 1];
+    const size_t size = b1 - b0;
+    const uint64_t minor_mask = (1ull << minor_bits) - 1;
+    const bool big = size > static_cast<size_t>(kSegCap);
+    // each wave owns a contiguous share of the tile: items = ceil(size / 256)
+    const int items = static_cast<int>((size + kBlock - 1) / kBlock);
+    const size_t wave_len = static_cast<size_t>(items) * kWave;
+
+    uint64_t k[kSegItems];
+    uint32_t count = 0;        // unique keys of this tile
+    size_t side0 = 0;          // big tile: its slice of the side output
+    uint64_t keep[kSegItems];  // small tile: ballot masks of kept keys per item
+    if (!big) {
+        // load; rank runs: local rank = run starts at or before the position
+        uint64_t or_acc = 0, and_acc = ~0ull;
+#pragma unroll
+        for (int i = 0; i < kSegItems; ++i) {
+            if (i >= items) continue;
+            const size_t p = wave * wave_len + i * kWave + lane;
+            k[i] = p < size ? keys[b0 + p] : 0;
+        }
+        uint32_t lrank[kSegItems];   // run starts at or before the position, within this wave
+        uint32_t starts_before = 0;  // run starts in earlier items of this wave
+#pragma unroll
+        for (int i = 0; i < kSegItems; ++i) {
+            if (i >= items) continue;
+            const size_t p = wave * wave_len + i * kWave + lane;
+            const bool valid = p < size;
+            const uint64_t major = k[i] >> minor_bits;
+            // predecessor's major: the lane below, or (lane 0) position p-1 in memory
+            const uint64_t up = __shfl_up(major, 1, kWave);
+            bool start = false;
+            if (valid) {
+                if (p == 0)
+                    start = true;
+                else if (lane > 0)
+                    start = up != major;
+                else
+                    start = (keys[b0 + p - 1] >> minor_bits) != major;
+            }
+            const uint64_t m = ballot(start);
+            if (lane == 0) item_totals[wave][i] = static_cast<uint32_t>(__popcll(m));
+            lrank[i] = starts_before + lanes_below(m) + (start ? 1u : 0u);
+            starts_before += static_cast<uint32_t>(__popcll(m));
+        }
+        __syncthreads();
+        uint32_t wave_base = 0;
+        for (int w = 0; w < static_cast<int>(wave); ++w)
+            for (int i = 0; i < items; ++i) wave_base += item_totals[w][i];
+        // rank (0-based) and the major value of every run
+#pragma unroll
+        for (int i = 0; i < kSegItems; ++i) {
+            if (i >= items) continue;
+            const size_t p = wave * wave_len + i * kWave + lane;
+            if (p < size) {
+                const uint32_t rank = lrank[i] + wave_base - 1;
+                major_of[rank] = static_cast<uint32_t>(k[i] >> minor_bits);
+                k[i] = (static_cast<uint64_t>(rank) << minor_bits) | (k[i] & minor_mask);
+                or_acc |= k[i];
+                and_acc &= k[i];
+            }
+        }
+        __syncthreads();
+        // varying digits of the tile's (rank, minor) keys
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+            or_acc |= __shfl_xor(or_acc, off, kWave);
+            and_acc &= __shfl_xor(and_acc, off, kWave);
+        }
+        if (lane == 0) {
+            red_or[wave] = or_acc;
+            red_and[wave] = and_acc;
+        }
+        __syncthreads();
+        uint64_t vary = 0;
+        {
+            uint64_t o = 0, a = ~0ull;
+            for (int w = 0; w < kWavesPerBlock; ++w) {
+                o |= red_or[w];
+                a &= red_and[w];
+            }
+            vary = size ? (o ^ a) : 0;
+        }
+        // stable LSD passes in LDS over the varying 8-bit digits
+        for (int shift = 0; shift < 64; shift += 8) {
+            if (!((vary >> shift) & 0xFF)) continue;  // block-uniform
+            for (int i = threadIdx.x; i < kWavesPerBlock * kSegRadix; i += kBlock) (&wave_count[0][0])[i] = 0;
+            __syncthreads();
+            uint32_t ranks[kSegItems];
+#pragma unroll
+            for (int i = 0; i < kSegItems; ++i) {
+                if (i >= items) continue;
+                const size_t p = wave * wave_len + i * kWave + lane;
+                const bool valid = p < size;
+                const uint32_t digit = static_cast<uint32_t>(k[i] >> shift) & 0xFF;
+                uint64_t peers = ballot(valid);
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    const bool bit = (digit >> b) & 1u;
+                    const uint64_t mm = ballot(bit);
+                    peers &= bit ? mm : ~mm;
+                }
+                const uint32_t before = valid ? wave_count[wave][digit] : 0;
+                const uint32_t below = lanes_below(peers);
+                ranks[i] = before + below;
+                if (valid && below == 0) wave_count[wave][digit] = before + static_cast<uint32_t>(__popcll(peers));
+            }
+            __syncthreads();
+            const uint32_t d = threadIdx.x;
+            uint32_t tc = 0;
+#pragma unroll
+            for (int w = 0; w < kWavesPerBlock; ++w) {
+                const uint32_t c = wave_count[w][d];
+                wave_count[w][d] = tc;
+                tc += c;
+            }
+            uint32_t total_unused;
+            const uint32_t local_start = block_exclusive_sum<uint32_t>(tc, &total_unused, scan_lds);
+#pragma unroll
+            for (int w = 0; w < kWavesPerBlock; ++w) wave_count[w][d] += local_start;
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < kSegItems; ++i) {
+                if (i >= items) continue;
+                const size_t p = wave * wave_len + i * kWave + lane;
+                if (p < size) {
+                    const uint32_t digit = static_cast<uint32_t>(k[i] >> shift) & 0xFF;
+                    tile_keys[wave_count[wave][digit] + ranks[i]] = k[i];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < kSegItems; ++i) {
+                if (i >= items) continue;
+                const size_t p = wave * wave_len + i * kWave + lane;
+                if (p < size) k[i] = tile_keys[p];
+            }
+            __syncthreads();
+        }
+        // sorted keys by position; also in LDS for the predecessor of lane 0
+#pragma unroll
+        for (int i = 0; i < kSegItems; ++i) {
+            if (i >= items) continue;
+            const size_t p = wave * wave_len + i * kWave + lane;
+            if (p < size) tile_keys[p] = k[i];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kSegItems; ++i) {
+            if (i >= items) continue;
+            const size_t p = wave * wave_len + i * kWave + lane;
+            const bool valid = p < size;
+            const uint64_t prev = p > 0 && valid ? tile_keys[p - 1] : ~k[i];
+            keep[i] = ballot(valid && (p == 0 || prev != k[i]));
+            if (lane == 0) item_totals[wave][i] = static_cast<uint32_t>(__popcll(keep[i]));
+        }
+        __syncthreads();
+        for (int w = 0; w < kWavesPerBlock; ++w)
+            for (int i = 0; i < items; ++i) count += item_totals[w][i];
+    } else {
+        // oversized tile: its unique keys are a slice of the globally sorted side output
+        const uint64_t first_major = keys[b0] >> minor_bits;
+        const uint64_t last_major = keys[b1 - 1] >> minor_bits;
+        const size_t n_side = h->side_unique;
+        side0 = block_search(0, n_side, [&](size_t i) { return (side_sorted[i] >> minor_bits) < first_major; });
+        const size_t side1 =
+            block_search(side0, n_side, [&](size_t i) { return (side_sorted[i] >> minor_bits) <= last_major; });
+        count = static_cast<uint32_t>(side1 - side0);
+    }
+    if (wave == 0) {
+        const uint64_t excl = wave_lookback(status, tile, count, 0, &h->error);
+        if (lane == 0) offset_slot = excl;
+    }
+    __syncthreads();
+    const uint64_t offset = offset_slot;
+    if (!big) {
+        uint32_t wave_start = 0;
+        for (int w = 0; w < static_cast<int>(wave); ++w)
+            for (int i = 0; i < items; ++i) wave_start += item_totals[w][i];
+        uint32_t pos = wave_start;
+#pragma unroll
+        for (int i = 0; i < kSegItems; ++i) {
+            if (i >= items) continue;
+            if ((keep[i] >> lane) & 1ull) {
+                const uint32_t rank = static_cast<uint32_t>(k[i] >> minor_bits);
+                const uint64_t pair = (static_cast<uint64_t>(major_of[rank]) << 32) | (k[i] & minor_mask);
+                store_pair(out, offset + pos + lanes_below(keep[i]), pair);
+            }
+            pos += static_cast<uint32_t>(__popcll(keep[i]));
+        }
+    } else {
+        for (uint32_t i = threadIdx.x; i < count; i += kBlock) {
+            const uint64_t kk = side_sorted[side0 + i];
+            store_pair(out, offset + i, ((kk >> minor_bits) << 32) | (kk & minor_mask));
+        }
+    }
+    if (tile == tiles - 1 && threadIdx.x == 0) *unique_rows = offset + count;
+}
+
+}  // namespace
+
+size_t segmented_distinct_temp_bytes(size_t rows) { return seg_layout(rows).total; }
+
+hipError_t segmented_distinct(const uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t side_varying,
+                              uint64_t* side_a, uint64_t* side_b, PairOutput out, uint64_t* unique_rows, void* temp,
+                              size_t temp_bytes, hipStream_t stream) {
+    if (rows == 0) return set_u64(unique_rows, 0, stream);
+    const SegLayout l = seg_layout(rows);
+    if (temp_bytes < l.total || minor_bits == 0 || minor_bits > 32) return hipErrorInvalidValue;
+    char* base = static_cast<char*>(temp);
+    SegHeader* h = reinterpret_cast<SegHeader*>(base);
+    uint64_t* status = reinterpret_cast<uint64_t*>(base + l.status_off);
+    size_t* bounds = reinterpret_cast<size_t*>(base + l.bounds_off);
+    void* sort_temp = base + l.sort_off;
+    const size_t sort_bytes = distinct_temp_bytes(rows);
+    const unsigned small_grid = static_cast<unsigned>(l.tiles / kBlock + 1 < 4096 ? l.tiles / kBlock + 1 : 4096);
+    seg_init_kernel<<<small_grid, kBlock, 0, stream>>>(h, status, l.tiles);
+    seg_partition_kernel<<<small_grid, kBlock, 0, stream>>>(keys, rows, minor_bits, bounds, l.tiles);
+    seg_gather_kernel<<<static_cast<unsigned>(l.tiles < 1024 ? l.tiles : 1024), kBlock, 0, stream>>>(
+        keys, bounds, l.tiles, side_a, h);
+    hipError_t st = hipGetLastError();
+    if (st != hipSuccess) return st;
+    // oversized runs (usually none): one 8-byte read tells whether the global
+    // sort of the gathered keys is needed, and sizes it exactly
+    uint64_t side_rows = 0;
+    st = hipMemcpyAsync(&side_rows, &h->side_rows, sizeof(side_rows), hipMemcpyDeviceToHost, stream);
+    if (st == hipSuccess) st = hipStreamSynchronize(stream);
+    if (st != hipSuccess) return st;
+    uint64_t* side_unique_out = side_b;
+    if (side_rows) {
+        PairInput side_in{side_a, nullptr, nullptr, nullptr};
+        uint64_t* sorted = nullptr;
+        st = radix_sort(side_in, side_rows, side_a, side_b, sort_temp, sort_bytes, side_varying, stream, &sorted);
+        if (st != hipSuccess) return st;
+        side_unique_out = sorted == side_a ? side_b : side_a;
+        st = unique_sorted(sorted, nullptr, side_rows, PairOutput{side_unique_out, nullptr, nullptr},
+                           &h->side_unique, sort_temp, sort_bytes, stream);
+        if (st != hipSuccess) return st;
+    }
+    seg_tile_kernel<<<static_cast<unsigned>(l.tiles), kBlock, 0, stream>>>(keys, rows, minor_bits, bounds,
+                                                                           side_unique_out, out, unique_rows, status, h);
+    return hipGetLastError();
+}
+
+}  // namespace spk

@@ -139,3 +139,49 @@ def test_component_graph_digest(nodes, expected, stream):
     assert digest == e["path_digest"]
     assert summ.iterations == e["max_dist"]
     assert summ.total_new_rows == e["path_rows"] - e["edges"]
+
+
+def _oracle_run(static, recursive, rules, relations, max_iterations=10_000):
+    ref = oracle.Store()
+    for rid, rel in static.items():
+        ref.insert_static(rid, rel)
+    for rid, (full, delta) in recursive.items():
+        ref.insert_recursive(rid, full, delta)
+    rc, summ = ref.run(relations, [oracle.rule_from_plan(r) for r in rules], max_iterations, 4)
+    assert rc == 0
+    return ref, summ
+
+
+def test_long_candidate_runs(stream):
+    # source 0 reaches 6,000 nodes in one step, each with 4 successors: one
+    # candidate run of 24,000 rows, far longer than an LDS tile
+    hub = [(0, i) for i in range(1, 6001)]
+    fan = [(i, 10_000 + (i * 7 + j) % 3000) for i in range(1, 6001) for j in range(4)]
+    chain_tail = [(10_000 + i, 10_000 + i + 1) for i in range(40)]
+    e = sorted(set(hub + fan + chain_tail))
+    edges = cols(e)
+    store, summ = _tc(edges, stream)
+    rule0 = rule(1, 1, 0)
+    ref, rsum = _oracle_run({0: edges}, {1: (edges, edges)}, [rule0], [1])
+    full, expect = store.full(R(1)), ref.get(1)
+    assert np.array_equal(full[0], expect[0]) and np.array_equal(full[1], expect[1])
+    assert summ.iterations == rsum.iterations and summ.total_new_rows == rsum.total_new_rows
+
+
+def test_ungrouped_projection_plan(stream):
+    # output (edge.1, path.0): the distinct's major column is not DELTA's first
+    # column, so candidates take the general radix-sort path
+    s, d, _ = workload.component_graph(nodes=1 << 12)
+    edges = (s, d)
+    plan_rule = plans.RecursiveRulePlan(
+        target=R(1), delta_input=R(1), right_input=R(0),
+        join=plans.BinaryEqualityJoin(1, 0, (plans.JoinProjection(plans.JoinInput.RIGHT, 1),
+                                             plans.JoinProjection(plans.JoinInput.LEFT, 0))))
+    store = RelationStore(stream)
+    store.insert_static(R(0), edges)
+    store.insert_recursive(R(1), edges, edges)
+    summ = RecursiveExecutor.compile(plans.RecursiveSccPlan((R(1),), (plan_rule,)), store).run(store, 64)
+    ref, rsum = _oracle_run({0: edges}, {1: (edges, edges)}, [plan_rule], [1], 64)
+    full, expect = store.full(R(1)), ref.get(1)
+    assert np.array_equal(full[0], expect[0]) and np.array_equal(full[1], expect[1])
+    assert summ.iterations == rsum.iterations and summ.total_new_rows == rsum.total_new_rows
