@@ -37,8 +37,14 @@ def parse_args():
     p.add_argument("--nodes", type=int, default=1 << 20)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-nodes", type=int, default=1 << 16)
-    p.add_argument("--mode", choices=["tc", "ops"], default="tc",
-                   help="tc: the headline fixpoint; ops: operator microbench at 2^k rows")
+    p.add_argument("--mode", choices=["tc", "ops", "batched", "served"], default="tc",
+                   help="tc: the headline fixpoint; ops: operator microbench at 2^k rows; "
+                        "batched: 4,096 instances in one loop vs one run each; served: Poisson stream, "
+                        "continuous batching vs sequential")
+    p.add_argument("--budget-pairs", type=float, default=4.5e9, help="batched/served: HBM budget in closure pairs")
+    p.add_argument("--requests", type=int, default=200, help="served: stream length")
+    p.add_argument("--utilization", type=float, default=0.7, help="served: offered load of the plain path")
+    p.add_argument("--plain-stride", type=int, default=1, help="batched: time the plain path on every k-th instance")
     p.add_argument("--sizes", default="24,26,28", help="ops mode: log2 row counts")
     p.add_argument("--ops", default="distinct,union,anti_join,join", help="ops mode: operators")
     p.add_argument("--samples", type=int, default=5, help="ops mode: timed launches per case")
@@ -63,6 +69,68 @@ def run_ops(args) -> None:
                 print(json.dumps(r), flush=True)
                 del op
                 torch.cuda.empty_cache()
+
+
+def run_batched_mode(args) -> None:
+    """Config 4: 4,096 instances (nodes log-uniform on [16, 16384], Poisson
+    out-degree mean uniform on [1.2, 3.0]) in one semi-naive loop per HBM
+    wave, against one executor run per instance."""
+    import numpy as np
+    import torch
+
+    from sparkalog_amd import batch, workload
+    from sparkalog_amd.execution import HipStream
+    from sparkalog_amd.recursion import RelationStore
+
+    torch.cuda.set_device(0)
+    s, d, offs, counts = workload.batch_graphs()
+    store = RelationStore(HipStream())
+    res = batch.run_batched(s, d, offs, counts, store, budget_pairs=args.budget_pairs)
+    picks = list(range(0, counts.size, args.plain_stride))
+    plain = batch.run_plain(s, d, offs, counts, store, instances=picks)
+    same = bool(np.array_equal(plain.rows[picks], res.rows[picks]) and
+                np.array_equal(plain.digests[picks], res.digests[picks]))
+    scale = counts.size / len(picks)
+    tuples = int(res.rows.sum())
+    print(json.dumps({
+        "mode": "batched", "instances": int(counts.size), "edges": int(s.size), "result_tuples": tuples,
+        "waves": len(res.waves), "batched_s": round(res.seconds, 3),
+        "batched_tuples_per_s": round(tuples / res.seconds, 1),
+        "plain_s_measured": round(plain.seconds, 3), "plain_instances_measured": len(picks),
+        "plain_s_estimated_all": round(plain.seconds * scale, 3),
+        "speedup_vs_plain": round(plain.seconds * scale / res.seconds, 3),
+        "plain_results_equal": same, "max_iterations": int(res.iterations.max()), "wave_log": res.waves}),
+        flush=True)
+
+
+def run_served_mode(args) -> None:
+    """Config 5: Poisson stream of TC requests (edges log-uniform on
+    [2^8, 2^20]); continuous batching against sequential plain execution of
+    the same stream, at an arrival rate giving the plain path the requested
+    utilisation."""
+    import numpy as np
+    import torch
+
+    from sparkalog_amd import serving
+    from sparkalog_amd.execution import HipStream
+    from sparkalog_amd.recursion import RelationStore
+
+    torch.cuda.set_device(0)
+    store = RelationStore(HipStream())
+    # service time of the plain path on a calibration stream (all at t=0)
+    calib = serving.make_requests(24, rate_per_s=1e9, seed=0xCA1)
+    serving.serve_plain(calib[:2], store)  # warm-up
+    done, secs = serving.serve_plain(calib, store)
+    mean_service = secs / len(calib)
+    rate = args.utilization / mean_service
+    reqs = serving.make_requests(args.requests, rate_per_s=rate)
+    serving.serve_batched(reqs[:4], store, budget_pairs=args.budget_pairs)  # warm-up
+    mixed, t_mixed = serving.serve_batched(reqs, store, budget_pairs=args.budget_pairs)
+    alone, t_alone = serving.serve_plain(reqs, store)
+    equal = all((m.rows, m.digest) == (a.rows, a.digest) for m, a in zip(mixed, alone))
+    print(json.dumps({"mode": "served", "rate_per_s": round(rate, 3), "mean_plain_service_ms": round(1000 * mean_service, 3),
+                      "batched": serving.summarize(mixed, t_mixed), "plain": serving.summarize(alone, t_alone),
+                      "results_equal_alone_vs_mix": equal}), flush=True)
 
 
 def cpu_baseline(sample_nodes: int) -> dict:
@@ -93,6 +161,12 @@ def main():
     args = parse_args()
     if args.mode == "ops":
         run_ops(args)
+        return
+    if args.mode == "batched":
+        run_batched_mode(args)
+        return
+    if args.mode == "served":
+        run_served_mode(args)
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "

@@ -230,6 +230,48 @@ int sparkalog_hip_store_set_profiling(sparkalog_hip_store* store, int enabled);
 int sparkalog_hip_store_op_stats(sparkalog_hip_store* store, int op, uint64_t* launches, double* ms,
                                  double* bytes);
 
+/* ---- stepwise fixpoint (for callers that change relations between steps) ----
+ * begin = RecursiveExecutor::compile; step = one loop trip of
+ * RecursiveExecutor::run (generic.rs:481-512) unless every DELTA is empty,
+ * returning the DELTA rows left; end = leave FULL as one array and report. */
+int sparkalog_hip_session_begin(sparkalog_hip_store* store, const uint32_t* relations, uint32_t n_relations,
+                                const sparkalog_hip_rule* rules, uint32_t n_rules);
+int sparkalog_hip_session_step(sparkalog_hip_store* store, uint64_t* delta_rows);
+int sparkalog_hip_session_end(sparkalog_hip_store* store, sparkalog_hip_summary* summary);
+
+/* ---- many independent instances in one relation ----
+ * Instance of a row = first column >> instance_bits; local pair = both
+ * columns masked to instance_bits. No counterpart in the reference: these
+ * serve the batched and served fixpoints. */
+
+/* Record, per instance, 1 + the last iteration that produced new rows (0 =
+ * none); applies to later session_begin / executor_run calls. */
+int sparkalog_hip_set_instance_tracking(sparkalog_hip_store* store, uint32_t instance_bits, uint32_t instances);
+int sparkalog_hip_instance_iterations(sparkalog_hip_store* store, uint32_t* last_new, uint32_t instances);
+
+/* Per instance: FULL rows and digest of its local pairs. */
+int sparkalog_hip_store_instance_stats(sparkalog_hip_store* store, uint32_t id, uint32_t instance_bits,
+                                       uint32_t instances, uint64_t* rows, uint64_t* digests);
+
+/* Rows and digest of the FULL (which = 0) or DELTA (1) rows whose first
+ * column is in [first_lo, first_hi), pairs taken relative to `base`. */
+int sparkalog_hip_store_range_stats(sparkalog_hip_store* store, uint32_t id, int which, uint32_t first_lo,
+                                    uint32_t first_hi, uint32_t base, uint64_t* rows, uint64_t* digest);
+
+/* Row counts of FULL (which = 0) or DELTA (1) per range of the first
+ * column: range j is [first_bounds[2j], first_bounds[2j+1]). */
+int sparkalog_hip_store_range_rows(sparkalog_hip_store* store, uint32_t id, int which, const uint32_t* first_bounds,
+                                   uint32_t ranges, uint64_t* rows);
+
+/* Add sorted unique rows: to FULL (which = 0; for a recursive relation they
+ * must be disjoint from FULL and become a run of their own) or DELTA (1). */
+int sparkalog_hip_store_merge_rows(sparkalog_hip_store* store, uint32_t id, int which, const uint32_t* c0,
+                                   const uint32_t* c1, uint64_t rows, int on_device);
+
+/* Remove every row whose first column is in [first_lo, first_hi) from FULL
+ * and DELTA. */
+int sparkalog_hip_store_remove_range(sparkalog_hip_store* store, uint32_t id, uint32_t first_lo, uint32_t first_hi);
+
 /* The hipError_t of the last failed HIP call in this thread (0 if none). */
 int sparkalog_hip_last_hip_error(void);
 

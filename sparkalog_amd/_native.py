@@ -68,6 +68,16 @@ _HIP_SIGNATURES = {
     "sparkalog_hip_store_op_stats": (c_int, [c_void_p, c_int, u64p, POINTER(c_double), POINTER(c_double)]),
     "sparkalog_hip_executor_compile": (c_int, [c_void_p, u32p, c_uint32, POINTER(Rule), c_uint32]),
     "sparkalog_hip_executor_run": (c_int, [c_void_p, u32p, c_uint32, POINTER(Rule), c_uint32, c_uint64, POINTER(Summary)]),
+    "sparkalog_hip_session_begin": (c_int, [c_void_p, u32p, c_uint32, POINTER(Rule), c_uint32]),
+    "sparkalog_hip_session_step": (c_int, [c_void_p, u64p]),
+    "sparkalog_hip_session_end": (c_int, [c_void_p, POINTER(Summary)]),
+    "sparkalog_hip_set_instance_tracking": (c_int, [c_void_p, c_uint32, c_uint32]),
+    "sparkalog_hip_instance_iterations": (c_int, [c_void_p, c_void_p, c_uint32]),
+    "sparkalog_hip_store_instance_stats": (c_int, [c_void_p, c_uint32, c_uint32, c_uint32, c_void_p, c_void_p]),
+    "sparkalog_hip_store_range_stats": (c_int, [c_void_p, c_uint32, c_int, c_uint32, c_uint32, c_uint32, u64p, u64p]),
+    "sparkalog_hip_store_range_rows": (c_int, [c_void_p, c_uint32, c_int, c_void_p, c_uint32, c_void_p]),
+    "sparkalog_hip_store_merge_rows": (c_int, [c_void_p, c_uint32, c_int, c_void_p, c_void_p, c_uint64, c_int]),
+    "sparkalog_hip_store_remove_range": (c_int, [c_void_p, c_uint32, c_uint32, c_uint32]),
     "sparkalog_hip_last_hip_error": (c_int, []),
     "sparkalog_hip_status_string": (ctypes.c_char_p, [c_int]),
 }

@@ -437,6 +437,7 @@ struct RuleRuntime {
     DevBuf<uint64_t> offsets;   // join offsets per delta row
     DevBuf<size_t> row_start;   // first delta row of each emit tile
     DevBuf<uint64_t> cand_a, cand_b, cand_c;
+    DevBuf<unsigned char> seg_meta;  // tile bounds and look-back words of the segmented distinct
     DevBuf<uint64_t> distinct_out;
     uint64_t join_rows = 0;     // M (host, after the first sync)
     int distinct_slot = -1;     // profiling event of this iteration's distinct
@@ -484,6 +485,7 @@ struct sparkalog_hip_store {
     uint64_t generation = 0;                    // bumped by every insert; versions derive from it
     std::vector<spk::DevBuf<uint64_t>> pool;    // released run buffers, reused by later merges
     uint32_t merge_ratio = 4;                   // merge two runs when the older is < ratio x the newer
+    void* session = nullptr;                    // spk::Session of the running fixpoint
     bool segmented_distinct = true;             // sort grouped candidates per run in LDS
 
     spk::DevBuf<uint64_t> take(size_t n) {
@@ -495,6 +497,12 @@ struct sparkalog_hip_store {
             b = std::move(pool[best]);
             pool.erase(pool.begin() + static_cast<long>(best));
         } else {
+            // runs only grow: pooled buffers smaller than this request are
+            // unlikely to be reused, so give their memory back first
+            for (auto& pb : pool)
+                if (pb.cap < n) pb.release();
+            pool.erase(std::remove_if(pool.begin(), pool.end(), [](const spk::DevBuf<uint64_t>& x) { return x.ptr == nullptr; }),
+                       pool.end());
             b.reserve(n ? n : 1);
         }
         return b;
@@ -740,6 +748,138 @@ double settle_runs(sparkalog_hip_store* s, RelationState& ts, bool all) {
     return bytes;
 }
 
+
+// ------------------------------------------------------------------------
+// per-instance and per-range statistics, row merges and range removal (the
+// batched and served fixpoints keep many independent instances in one
+// relation, instance = first column >> instance bits)
+
+__global__ void instance_stats_kernel(const uint64_t* keys, size_t n, uint32_t bits, uint32_t instances,
+                                      unsigned long long* rows, unsigned long long* digests) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    const size_t gtid = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t trips = (n + stride - 1) / stride;
+    const uint32_t mask = bits >= 32 ? 0xFFFFFFFFu : ((1u << bits) - 1u);
+    for (size_t t = 0; t < trips; ++t) {
+        const size_t i = t * stride + gtid;
+        const bool valid = i < n;
+        uint32_t inst = 0xFFFFFFFFu;
+        unsigned long long c = 0;
+        if (valid) {
+            const uint64_t k = keys[i];
+            const uint32_t x = static_cast<uint32_t>(k >> 32), z = static_cast<uint32_t>(k);
+            inst = bits >= 32 ? 0 : x >> bits;
+            uint64_t zz = ((static_cast<uint64_t>(x & mask) << 32) | (z & mask)) + 0x9E3779B97F4A7C15ull;
+            zz = (zz ^ (zz >> 30)) * 0xBF58476D1CE4E5B9ull;
+            zz = (zz ^ (zz >> 27)) * 0x94D049BB133111EBull;
+            c = zz ^ (zz >> 31);
+        }
+        const uint32_t lead = __builtin_amdgcn_readfirstlane(inst);
+        const uint64_t valid_mask = ballot(valid);
+        if (valid_mask == 0) continue;
+        if (ballot(valid && inst == lead) == valid_mask) {
+            const unsigned long long sum = wave_sum<unsigned long long>(c);
+            if (lane_id() == 0 && lead < instances) {
+                atomicAdd(&digests[lead], sum);
+                atomicAdd(&rows[lead], static_cast<unsigned long long>(__popcll(valid_mask)));
+            }
+        } else if (valid && inst < instances) {
+            atomicAdd(&digests[inst], c);
+            atomicAdd(&rows[inst], 1ull);
+        }
+    }
+}
+
+// lower bounds of a set of keys (one thread each)
+__global__ void lower_bounds_kernel(const uint64_t* keys, size_t n, const uint64_t* probes, size_t count,
+                                    uint64_t* out) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t j = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < count; j += stride) {
+        size_t lo = 0, hi = n;
+        while (lo < hi) {
+            const size_t mid = (lo + hi) >> 1;
+            if (keys[mid] < probes[j])
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        out[j] = lo;
+    }
+}
+
+__global__ void range_stats_kernel(const uint64_t* keys, size_t begin, size_t end, uint32_t base,
+                                   unsigned long long* rows_digest) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    unsigned long long acc = 0, cnt = 0;
+    for (size_t i = begin + static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < end; i += stride) {
+        const uint64_t k = keys[i];
+        const uint64_t local = ((static_cast<uint64_t>(static_cast<uint32_t>(k >> 32) - base)) << 32) |
+                               static_cast<uint32_t>(static_cast<uint32_t>(k) - base);
+        uint64_t zz = local + 0x9E3779B97F4A7C15ull;
+        zz = (zz ^ (zz >> 30)) * 0xBF58476D1CE4E5B9ull;
+        zz = (zz ^ (zz >> 27)) * 0x94D049BB133111EBull;
+        acc += zz ^ (zz >> 31);
+        ++cnt;
+    }
+    acc = wave_sum<unsigned long long>(acc);
+    cnt = wave_sum<unsigned long long>(cnt);
+    if (lane_id() == 0 && cnt) {
+        atomicAdd(&rows_digest[0], cnt);
+        atomicAdd(&rows_digest[1], acc);
+    }
+}
+
+// Positions [lo, hi) of the rows whose first column lies in [first_lo, first_hi).
+std::pair<uint64_t, uint64_t> first_range(sparkalog_hip_store* s, const DevRelation& r, uint32_t first_lo,
+                                          uint32_t first_hi) {
+    if (r.rows == 0) return {0, 0};
+    uint64_t probes[2] = {static_cast<uint64_t>(first_lo) << 32, static_cast<uint64_t>(first_hi) << 32};
+    if (first_hi == 0) probes[1] = 0;
+    DevBuf<uint64_t> dev;
+    dev.reserve(4);
+    check(hipMemcpyAsync(dev.ptr, probes, sizeof(probes), hipMemcpyHostToDevice, s->stream));
+    lower_bounds_kernel<<<1, 64, 0, s->stream>>>(r.keys.ptr, r.rows, dev.ptr, 2, dev.ptr + 2);
+    check(hipGetLastError());
+    uint64_t out[2];
+    check(hipMemcpyAsync(out, dev.ptr + 2, sizeof(out), hipMemcpyDeviceToHost, s->stream));
+    check(hipStreamSynchronize(s->stream));
+    return {out[0], out[1] > out[0] ? out[1] : out[0]};
+}
+
+void remove_rows(sparkalog_hip_store* s, DevRelation& r, uint64_t lo, uint64_t hi) {
+    if (hi <= lo) return;
+    const uint64_t n = r.rows - (hi - lo);
+    DevBuf<uint64_t> out = s->take(n ? n : 1);
+    if (lo) check(hipMemcpyAsync(out.ptr, r.keys.ptr, lo * sizeof(uint64_t), hipMemcpyDeviceToDevice, s->stream));
+    if (r.rows > hi)
+        check(hipMemcpyAsync(out.ptr + lo, r.keys.ptr + hi, (r.rows - hi) * sizeof(uint64_t),
+                             hipMemcpyDeviceToDevice, s->stream));
+    s->give(std::move(r.keys));
+    r.keys = std::move(out);
+    r.rows = n;
+}
+
+// dst = dst ∪ keys (both sorted); the result size comes back from the device.
+void union_into(sparkalog_hip_store* s, DevRelation& dst, const uint64_t* keys, uint64_t rows) {
+    if (rows == 0) return;
+    const uint64_t bound = dst.rows + rows;
+    DevBuf<uint64_t> out = s->take(bound);
+    const size_t tb = union_temp_bytes(dst.rows, rows);
+    void* temp = s->scratch(tb);
+    s->counters_dev.reserve(sizeof(Counters) / sizeof(uint64_t));
+    Counters* cdev = reinterpret_cast<Counters*>(s->counters_dev.ptr);
+    PairInput a{dst.keys.ptr, nullptr, nullptr, nullptr};
+    PairInput b{keys, nullptr, nullptr, nullptr};
+    check(sorted_union(a, dst.rows, b, rows, PairOutput{out.ptr, nullptr, nullptr}, &cdev->merge_rows, temp, tb,
+                       s->stream));
+    uint64_t n = 0;
+    check(hipMemcpyAsync(&n, &cdev->merge_rows, sizeof(n), hipMemcpyDeviceToHost, s->stream));
+    check(hipStreamSynchronize(s->stream));
+    s->give(std::move(dst.keys));
+    dst.keys = std::move(out);
+    dst.rows = n;
+}
+
 // RecursiveExecutor::compile (generic.rs:407-470): validation and runtime
 // construction; returns a status code.
 int compile_plan(sparkalog_hip_store* s, const uint32_t* relations, uint32_t n_relations,
@@ -855,9 +995,16 @@ extern "C" int sparkalog_hip_store_create(sparkalog_hip_store** out, void* strea
     SPK_GUARD_END
 }
 
+namespace spk {
+namespace {
+void destroy_session(sparkalog_hip_store* s);
+}  // namespace
+}  // namespace spk
+
 extern "C" void sparkalog_hip_store_destroy(sparkalog_hip_store* store) {
     if (!store) return;
     (void)hipStreamSynchronize(store->stream);
+    destroy_session(store);
     delete store;
 }
 
@@ -986,37 +1133,97 @@ extern "C" int sparkalog_hip_executor_compile(sparkalog_hip_store* s, const uint
     SPK_GUARD_END
 }
 
-extern "C" int sparkalog_hip_executor_run(sparkalog_hip_store* s, const uint32_t* relations, uint32_t n_relations,
-                                          const sparkalog_hip_rule* rules, uint32_t n_rules, uint64_t max_iterations,
-                                          sparkalog_hip_summary* summary) {
-    if (!s || (n_relations && !relations) || (n_rules && !rules)) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
-    SPK_GUARD_BEGIN
-    std::vector<TargetRuntime>& targets = s->ws_targets;
-    std::vector<RuleRuntime>& rts = s->ws_rules;
-    {
-        const int compiled = compile_plan(s, relations, n_relations, rules, n_rules, targets, rts);
-        if (compiled != SPARKALOG_HIP_OK) return compiled;
-    }
+namespace spk {
+namespace {
 
-    s->events_used = 0;
-    s->pending_bytes.clear();
-    hipEvent_t t_begin, t_end;
-    check(hipEventCreate(&t_begin));
-    check(hipEventCreate(&t_end));
-    check(hipEventRecord(t_begin, s->stream));
-
-    s->counters_dev.reserve(sizeof(Counters) / sizeof(uint64_t));
-    Counters* cdev = reinterpret_cast<Counters*>(s->counters_dev.ptr);
-    Counters chost;
-    // look-back scratch: per launch one counter + error word
-    s->misc_dev.reserve(256);
-    uint32_t* lb_counters = s->misc_dev.ptr;  // [0..127] counters, [128] error
-    uint32_t* lb_error = s->misc_dev.ptr + 128;
-
+// State of one fixpoint between begin and end (the loop of generic.rs:472-516
+// split into steps so that a caller may change relations between steps).
+struct Session {
+    bool active = false;
+    std::vector<uint32_t> relations;
+    std::vector<sparkalog_hip_rule> rules;
     uint64_t iterations = 0, total_new = 0;
     double algo_bytes = 0.0;
-    int rc = SPARKALOG_HIP_OK;
-    // max value across the SCC's relations and inputs bounds every key
+    hipEvent_t t_begin = nullptr, t_end = nullptr;
+    // per-instance tracking: instance of a row = first >> instance_bits
+    uint32_t instance_bits = 0, instances = 0;
+    DevBuf<uint32_t> last_new;  // per instance: 1 + last iteration with new rows
+};
+
+Session& session_of(sparkalog_hip_store* s) {
+    if (!s->session) s->session = new Session();
+    return *static_cast<Session*>(s->session);
+}
+
+void destroy_session(sparkalog_hip_store* s) {
+    if (!s->session) return;
+    Session* ss = static_cast<Session*>(s->session);
+    if (ss->t_begin) (void)hipEventDestroy(ss->t_begin);
+    if (ss->t_end) (void)hipEventDestroy(ss->t_end);
+    delete ss;
+    s->session = nullptr;
+}
+
+__global__ void track_instances_kernel(const uint64_t* keys, size_t n, uint32_t bits, uint32_t instances,
+                                       uint32_t value, uint32_t* last_new) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t inst = static_cast<uint32_t>(keys[i] >> 32) >> bits;
+        if (inst < instances && (i == 0 || (static_cast<uint32_t>(keys[i - 1] >> 32) >> bits) != inst))
+            atomicMax(&last_new[inst], value);
+    }
+}
+
+int session_begin(sparkalog_hip_store* s, const uint32_t* relations, uint32_t n_relations,
+                  const sparkalog_hip_rule* rules, uint32_t n_rules) {
+    Session& ss = session_of(s);
+    const int compiled = compile_plan(s, relations, n_relations, rules, n_rules, s->ws_targets, s->ws_rules);
+    if (compiled != SPARKALOG_HIP_OK) return compiled;
+    ss.relations.assign(relations, relations + n_relations);
+    ss.rules.assign(rules, rules + n_rules);
+    ss.iterations = ss.total_new = 0;
+    ss.algo_bytes = 0.0;
+    s->events_used = 0;
+    s->pending_bytes.clear();
+    if (!ss.t_begin) {
+        check(hipEventCreate(&ss.t_begin));
+        check(hipEventCreate(&ss.t_end));
+    }
+    check(hipEventRecord(ss.t_begin, s->stream));
+    s->counters_dev.reserve(sizeof(Counters) / sizeof(uint64_t));
+    s->misc_dev.reserve(256);
+    if (ss.instance_bits) {
+        ss.last_new.reserve(ss.instances ? ss.instances : 1);
+        clear_words_kernel<<<1, 1024, 0, s->stream>>>(ss.last_new.ptr, ss.instances);
+        check(hipGetLastError());
+    }
+    ss.active = true;
+    return SPARKALOG_HIP_OK;
+}
+
+uint64_t pending_delta(sparkalog_hip_store* s) {
+    uint64_t pending = 0;
+    for (uint32_t r : session_of(s).relations) pending += find(s, r)->delta.rows;
+    return pending;
+}
+
+// One semi-naive iteration: evaluate every rule, combine per target, apply
+// per target (generic.rs:481-512).
+void session_step(sparkalog_hip_store* s) {
+    Session& ss = session_of(s);
+    std::vector<TargetRuntime>& targets = s->ws_targets;
+    std::vector<RuleRuntime>& rts = s->ws_rules;
+    const uint32_t n_rules = static_cast<uint32_t>(ss.rules.size());
+    const uint32_t n_relations = static_cast<uint32_t>(ss.relations.size());
+    const uint32_t* relations = ss.relations.data();
+    Counters* cdev = reinterpret_cast<Counters*>(s->counters_dev.ptr);
+    Counters chost;
+    uint32_t* lb_counters = s->misc_dev.ptr;  // [0..127] look-back counters, [128] error
+    uint32_t* lb_error = s->misc_dev.ptr + 128;
+    uint64_t& iterations = ss.iterations;
+    uint64_t& total_new = ss.total_new;
+    double& algo_bytes = ss.algo_bytes;
+    // the largest value of any stored relation bounds every key of this step
     uint32_t vmax = 0;
     for (const auto& st : s->rel)
         if (st.present) vmax = st.max_value > vmax ? st.max_value : vmax;
@@ -1027,239 +1234,459 @@ extern "C" int sparkalog_hip_executor_run(sparkalog_hip_store* s, const uint32_t
     const uint32_t shift = vbits <= 24 ? vbits : 32;  // compressed sort key width per column
     const uint64_t known_compressed = shift == 32 ? ((vmask << 32) | vmask) : ((vmask << shift) | vmask);
 
+    clear_words_kernel<<<1, 256, 0, s->stream>>>(s->misc_dev.ptr, 256);
+    // ---- RuleRuntime::evaluate, join count (generic.rs:197-241) ----
+    for (uint32_t r = 0; r < n_rules; ++r) {
+        RuleRuntime& rt = rts[r];
+        RelationState& ds = *find(s, rt.plan.delta_input);
+        RelationState& rs = *find(s, rt.plan.right_input);
+        build_index(s, rt, rs, rt.plan.right_input);
+        rt.delta_rows = ds.delta.rows;
+        rt.shift = shift;
+        rt.distinct_slot = -1;
+        const uint64_t L = rt.delta_rows;
+        if (L == 0) {
+            check(set_u64(&cdev->join_total[r], 0, s->stream));
+            continue;
+        }
+        rt.offsets.reserve(L);
+        const size_t n_tiles = tiles(L, kBlock * 8);
+        rt.status.reserve(n_tiles);
+        check(fill_u64(rt.status.ptr, n_tiles, 0, s->stream));
+        const int ev = s->begin_op();
+        dense_join_count_kernel<<<static_cast<unsigned>(n_tiles), kBlock, 0, s->stream>>>(
+            ds.delta.keys.ptr, L, rt.plan.left_key, rt.index.starts.ptr, rt.index.n_keys, rt.offsets.ptr,
+            &cdev->join_total[r], rt.status.ptr, &lb_counters[r], lb_error);
+        check(hipGetLastError());
+        s->end_op(ev, kOpJoin, 8.0 * L);
+    }
+    check(hipMemcpyAsync(&chost, cdev, sizeof(Counters), hipMemcpyDeviceToHost, s->stream));
+    check(hipStreamSynchronize(s->stream));
+    // ---- join emit + distinct (generic.rs:231-249) ----
+    for (uint32_t r = 0; r < n_rules; ++r) {
+        RuleRuntime& rt = rts[r];
+        const uint64_t M = rt.delta_rows ? chost.join_total[r] : 0;
+        rt.join_rows = M;
+        if (M > 0xFFFFFFFFull) throw StatusError{SPARKALOG_HIP_ERR_TOO_MANY_ROWS};
+        RelationState& ds = *find(s, rt.plan.delta_input);
+        rt.cand_a.reserve(M ? M : 1);
+        rt.cand_b.reserve(M ? M : 1);
+        rt.distinct_out.reserve(M ? M : 1);
+        if (M == 0) {
+            check(set_u64(&cdev->distinct_rows[r], 0, s->stream));
+            continue;
+        }
+        EmitSpec spec;
+        spec.key_col = rt.plan.left_key;
+        spec.right_key = rt.plan.right_key;
+        spec.side[0] = rt.plan.out_input[0];
+        spec.side[1] = rt.plan.out_input[1];
+        spec.col[0] = rt.plan.out_column[0];
+        spec.col[1] = rt.plan.out_column[1];
+        spec.order[0] = rt.plan.distinct_columns[0];
+        spec.order[1] = rt.plan.distinct_columns[1];
+        spec.shift = rt.shift;
+        int ev = s->begin_op();
+        const uint64_t emit_tiles = tiles(M, kEmitTile);
+        const unsigned emit_grid = static_cast<unsigned>(emit_tiles < 65536 ? emit_tiles : 65536);
+        rt.row_start.reserve(emit_tiles + 1);
+        emit_partition_kernel<<<grid_for(emit_tiles + 1), kBlock, 0, s->stream>>>(rt.offsets.ptr, rt.delta_rows, M,
+                                                                               rt.row_start.ptr, emit_tiles);
+        dense_join_emit_kernel<<<emit_grid, kBlock, 0, s->stream>>>(
+            ds.delta.keys.ptr, rt.delta_rows, rt.index.starts.ptr, rt.index.n_keys, rt.index.sorted.ptr,
+            rt.offsets.ptr, M, rt.row_start.ptr, spec, rt.cand_a.ptr);
+        check(hipGetLastError());
+        s->end_op(ev, kOpJoin, 8.0 * rt.index.rows + 8.0 * M);
+        ev = s->begin_op();
+        rt.distinct_slot = ev;
+        // The join writes candidates in DELTA order; when the distinct's major
+        // column is DELTA's first column they are grouped by it already.
+        const bool grouped = spec.side[spec.order[0]] == 0 && spec.col[spec.order[0]] == 0;
+        if (grouped && s->segmented_distinct) {
+            const size_t mb = segmented_distinct_meta_bytes(M);
+            rt.seg_meta.reserve(mb);
+            uint64_t side_rows = 0;
+            check(segmented_distinct_prepare(rt.cand_a.ptr, M, rt.shift, rt.cand_b.ptr, rt.seg_meta.ptr, mb,
+                                             &side_rows, s->stream));
+            size_t tb = 0;
+            void* temp = nullptr;
+            if (side_rows) {
+                rt.cand_c.reserve(side_rows);
+                tb = distinct_temp_bytes(side_rows);
+                temp = s->scratch(tb);
+            }
+            PairOutput out{rt.distinct_out.ptr, nullptr, nullptr};
+            check(segmented_distinct_finish(rt.cand_a.ptr, M, rt.shift, known_compressed, rt.cand_b.ptr,
+                                            side_rows ? rt.cand_c.ptr : nullptr, side_rows, out,
+                                            &cdev->distinct_rows[r], rt.seg_meta.ptr, mb, temp, tb, s->stream));
+        } else {
+            const size_t tb = distinct_temp_bytes(M);
+            void* temp = s->scratch(tb);
+            PairInput in{rt.cand_a.ptr, nullptr, nullptr, nullptr};
+            PairOutput out{rt.distinct_out.ptr, nullptr, nullptr};
+            check(distinct(in, M, rt.cand_a.ptr, rt.cand_b.ptr, out, &cdev->distinct_rows[r], temp, tb,
+                           known_compressed, s->stream));
+            if (rt.shift != 32)
+                expand_keys_kernel<<<grid_for(M), kBlock, 0, s->stream>>>(rt.distinct_out.ptr,
+                                                                         &cdev->distinct_rows[r], M, rt.shift);
+        }
+        check(hipGetLastError());
+        s->end_op(ev, kOpDistinct, 16.0 * M);  // + 8U added after the sync
+    }
+    // ---- TargetRuntime::combine (generic.rs:299-358) ----
+    for (auto& t : targets) {
+        RelationState& ts = *find(s, t.target);
+        const uint64_t* contrib = nullptr;
+        const uint64_t* contrib_rows = nullptr;
+        uint64_t bound = 0;
+        for (size_t k = 0; k < t.rules.size(); ++k) {
+            RuleRuntime& rt = rts[t.rules[k]];
+            if (k == 0) {
+                contrib = rt.distinct_out.ptr;
+                contrib_rows = &cdev->distinct_rows[t.rules[k]];
+                bound = rt.join_rows;
+                continue;
+            }
+            const uint64_t nb = bound + rt.join_rows;
+            t.union_tmp.reserve(nb ? nb : 1);
+            const size_t tb = union_temp_bytes(bound, rt.join_rows);
+            void* temp = s->scratch(tb);
+            PairInput a{contrib, nullptr, nullptr, contrib_rows};
+            PairInput b{rt.distinct_out.ptr, nullptr, nullptr, &cdev->distinct_rows[t.rules[k]]};
+            PairOutput o{t.union_tmp.ptr, nullptr, nullptr};
+            const int ev = s->begin_op();
+            check(sorted_union(a, bound, b, rt.join_rows, o, &cdev->contrib_rows[t.rules[k]], temp, tb,
+                               s->stream));
+            s->end_op(ev, kOpUnion, 0.0);
+            std::swap(t.union_tmp, t.contributions);
+            contrib = t.contributions.ptr;
+            contrib_rows = &cdev->contrib_rows[t.rules[k]];
+            bound = nb;
+        }
+        t.contrib_bound = bound;
+        // NEWT = contributions \ FULL, as a chain of anti-joins over FULL's
+        // runs (largest first). The last one also writes NEWT's copy that
+        // becomes FULL's newest run.
+        const uint32_t ti = static_cast<uint32_t>(&t - targets.data());
+        t.newt.reserve(bound ? bound : 1);
+        t.new_run = s->take(bound ? bound : 1);
+        if (ts.runs.size() >= kMaxRuns) settle_runs(s, ts, true);
+        AntiRight rights{};
+        rights.count = 0;
+        double right_rows = 0;
+        auto add_right = [&](const DevRelation& r) {
+            if (!r.rows) return;
+            rights.in[rights.count] = PairInput{r.keys.ptr, nullptr, nullptr, nullptr};
+            rights.rows[rights.count] = r.rows;
+            ++rights.count;
+            right_rows += static_cast<double>(r.rows);
+        };
+        add_right(ts.full);
+        for (const auto& r : ts.runs) add_right(r);
+        t.anti_slot = -1;
+        if (bound == 0) {
+            check(set_u64(&cdev->newt_rows[ti], 0, s->stream));
+        } else if (rights.count == 0) {
+            check(copy_keys(contrib, contrib_rows, bound, t.newt.ptr, t.new_run.ptr, &cdev->newt_rows[ti],
+                            s->stream));
+        } else {
+            const size_t tb = anti_join_temp_bytes(bound, rights.count);
+            void* temp = s->scratch(tb);
+            PairInput a{contrib, nullptr, nullptr, contrib_rows};
+            PairOutput o{t.newt.ptr, nullptr, nullptr, t.new_run.ptr};
+            const int ev = s->begin_op();
+            check(sorted_anti_join_multi(a, bound, rights, o, nullptr, &cdev->newt_rows[ti], temp, tb, s->stream));
+            s->end_op(ev, kOpAntiJoin, 8.0 * right_rows);
+            t.anti_slot = ev;
+            algo_bytes += 8.0 * right_rows;
+        }
+    }
+    check(hipMemcpyAsync(&chost, cdev, sizeof(Counters), hipMemcpyDeviceToHost, s->stream));
+    uint32_t err = 0;
+    check(hipMemcpyAsync(&err, lb_error, sizeof(err), hipMemcpyDeviceToHost, s->stream));
+    check(hipStreamSynchronize(s->stream));
+    if (err) throw StatusError{SPARKALOG_HIP_ERR_DEVICE};
+    // per-iteration algorithmic bytes of the operators run (pairs are 8 bytes)
+    for (uint32_t r = 0; r < n_rules; ++r) {
+        const double L = static_cast<double>(rts[r].delta_rows), M = static_cast<double>(rts[r].join_rows);
+        const double U = rts[r].join_rows ? static_cast<double>(chost.distinct_rows[r]) : 0.0;
+        const double R = static_cast<double>(rts[r].index.rows);
+        if (L > 0) algo_bytes += 8 * L + 8 * R + 8 * M + (M > 0 ? 8 * M + 8 * U : 0.0);
+        s->add_op_bytes(rts[r].distinct_slot, 8 * U);
+    }
+    for (uint32_t i = 0; i < n_relations; ++i) {
+        TargetRuntime& t = targets[i];
+        const uint64_t C = t.rules.size() > 1 ? chost.contrib_rows[t.rules.back()]
+                                              : (t.contrib_bound ? chost.distinct_rows[t.rules[0]] : 0);
+        // the anti-join reads the contributions and writes NEWT twice (DELTA
+        // and FULL's new run)
+        if (t.anti_slot >= 0 || t.contrib_bound) {
+            const double extra = 8.0 * static_cast<double>(C) + 16.0 * static_cast<double>(chost.newt_rows[i]);
+            s->add_op_bytes(t.anti_slot, extra);
+            if (t.contrib_bound) algo_bytes += extra;
+        }
+    }
+    // ---- TargetRuntime::apply (generic.rs:360-397) ----
+    // FULL ∪ NEWT: NEWT is disjoint from every run of FULL, so it becomes
+    // FULL's newest run; runs are merged (pure merges of disjoint sorted
+    // arrays, output size known on the host) while the older run is less
+    // than merge_ratio times the newer one.
+    for (uint32_t i = 0; i < n_relations; ++i) {
+        TargetRuntime& t = targets[i];
+        RelationState& ts = *find(s, t.target);
+        const uint64_t N = chost.newt_rows[i];
+        std::swap(ts.newt.keys, t.newt);
+        ts.newt.rows = N;
+        if (N) {
+            DevRelation run;
+            run.keys = std::move(t.new_run);
+            run.rows = N;
+            ts.runs.push_back(std::move(run));
+        } else {
+            s->give(std::move(t.new_run));
+        }
+        algo_bytes += settle_runs(s, ts, false);
+        std::swap(ts.delta, ts.newt);
+        ts.version += 1;
+        total_new += ts.delta.rows;
+    }
+    if (ss.instance_bits) {
+        for (uint32_t i = 0; i < n_relations; ++i) {
+            const RelationState& ts = *find(s, relations[i]);
+            if (ts.delta.rows)
+                track_instances_kernel<<<grid_for(ts.delta.rows), kBlock, 0, s->stream>>>(
+                    ts.delta.keys.ptr, ts.delta.rows, ss.instance_bits, ss.instances,
+                    static_cast<uint32_t>(iterations + 1), ss.last_new.ptr);
+        }
+        check(hipGetLastError());
+    }
+    ++iterations;
+}
+
+int session_end(sparkalog_hip_store* s, sparkalog_hip_summary* summary, int rc) {
+    Session& ss = session_of(s);
+    // FULL leaves the run as one sorted array
+    for (uint32_t r : ss.relations) ss.algo_bytes += settle_runs(s, *find(s, r), true);
+    check(hipEventRecord(ss.t_end, s->stream));
+    check(hipEventSynchronize(ss.t_end));
+    float ms = 0.f;
+    check(hipEventElapsedTime(&ms, ss.t_begin, ss.t_end));
+    if (s->profiling) s->collect_ops();
+    if (summary) {
+        summary->iterations = ss.iterations;
+        summary->total_new_rows = ss.total_new;
+        summary->device_ms = ms;
+        summary->algorithmic_bytes = ss.algo_bytes;
+        if (rc == SPARKALOG_HIP_OK) summary->delta_rows_left = 0;
+    }
+    ss.active = false;
+    return rc;
+}
+
+}  // namespace
+}  // namespace spk
+
+extern "C" int sparkalog_hip_executor_run(sparkalog_hip_store* s, const uint32_t* relations, uint32_t n_relations,
+                                          const sparkalog_hip_rule* rules, uint32_t n_rules, uint64_t max_iterations,
+                                          sparkalog_hip_summary* summary) {
+    if (!s || (n_relations && !relations) || (n_rules && !rules)) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    SPK_GUARD_BEGIN
+    const int begun = session_begin(s, relations, n_relations, rules, n_rules);
+    if (begun != SPARKALOG_HIP_OK) return begun;
+    int rc = SPARKALOG_HIP_OK;
     for (;;) {
-        uint64_t pending = 0;
-        for (uint32_t i = 0; i < n_relations; ++i) pending += find(s, relations[i])->delta.rows;
+        const uint64_t pending = pending_delta(s);
         if (pending == 0) break;
-        if (iterations >= max_iterations) {
+        if (session_of(s).iterations >= max_iterations) {
             if (summary) summary->delta_rows_left = pending;
             rc = SPARKALOG_HIP_ERR_ITERATION_LIMIT;
             break;
         }
-        clear_words_kernel<<<1, 256, 0, s->stream>>>(s->misc_dev.ptr, 256);
-        // ---- RuleRuntime::evaluate, join count (generic.rs:197-241) ----
-        for (uint32_t r = 0; r < n_rules; ++r) {
-            RuleRuntime& rt = rts[r];
-            RelationState& ds = *find(s, rt.plan.delta_input);
-            RelationState& rs = *find(s, rt.plan.right_input);
-            build_index(s, rt, rs, rt.plan.right_input);
-            rt.delta_rows = ds.delta.rows;
-            rt.shift = shift;
-            rt.distinct_slot = -1;
-            const uint64_t L = rt.delta_rows;
-            if (L == 0) {
-                check(set_u64(&cdev->join_total[r], 0, s->stream));
-                continue;
-            }
-            rt.offsets.reserve(L);
-            const size_t n_tiles = tiles(L, kBlock * 8);
-            rt.status.reserve(n_tiles);
-            check(fill_u64(rt.status.ptr, n_tiles, 0, s->stream));
-            const int ev = s->begin_op();
-            dense_join_count_kernel<<<static_cast<unsigned>(n_tiles), kBlock, 0, s->stream>>>(
-                ds.delta.keys.ptr, L, rt.plan.left_key, rt.index.starts.ptr, rt.index.n_keys, rt.offsets.ptr,
-                &cdev->join_total[r], rt.status.ptr, &lb_counters[r], lb_error);
-            check(hipGetLastError());
-            s->end_op(ev, kOpJoin, 8.0 * L);
-        }
-        check(hipMemcpyAsync(&chost, cdev, sizeof(Counters), hipMemcpyDeviceToHost, s->stream));
-        check(hipStreamSynchronize(s->stream));
-        // ---- join emit + distinct (generic.rs:231-249) ----
-        for (uint32_t r = 0; r < n_rules; ++r) {
-            RuleRuntime& rt = rts[r];
-            const uint64_t M = rt.delta_rows ? chost.join_total[r] : 0;
-            rt.join_rows = M;
-            if (M > 0xFFFFFFFFull) throw StatusError{SPARKALOG_HIP_ERR_TOO_MANY_ROWS};
-            RelationState& ds = *find(s, rt.plan.delta_input);
-            rt.cand_a.reserve(M ? M : 1);
-            rt.cand_b.reserve(M ? M : 1);
-            rt.distinct_out.reserve(M ? M : 1);
-            if (M == 0) {
-                check(set_u64(&cdev->distinct_rows[r], 0, s->stream));
-                continue;
-            }
-            EmitSpec spec;
-            spec.key_col = rt.plan.left_key;
-            spec.right_key = rt.plan.right_key;
-            spec.side[0] = rt.plan.out_input[0];
-            spec.side[1] = rt.plan.out_input[1];
-            spec.col[0] = rt.plan.out_column[0];
-            spec.col[1] = rt.plan.out_column[1];
-            spec.order[0] = rt.plan.distinct_columns[0];
-            spec.order[1] = rt.plan.distinct_columns[1];
-            spec.shift = rt.shift;
-            int ev = s->begin_op();
-            const uint64_t emit_tiles = tiles(M, kEmitTile);
-            const unsigned emit_grid = static_cast<unsigned>(emit_tiles < 65536 ? emit_tiles : 65536);
-            rt.row_start.reserve(emit_tiles + 1);
-            emit_partition_kernel<<<grid_for(emit_tiles + 1), kBlock, 0, s->stream>>>(rt.offsets.ptr, rt.delta_rows, M,
-                                                                                   rt.row_start.ptr, emit_tiles);
-            dense_join_emit_kernel<<<emit_grid, kBlock, 0, s->stream>>>(
-                ds.delta.keys.ptr, rt.delta_rows, rt.index.starts.ptr, rt.index.n_keys, rt.index.sorted.ptr,
-                rt.offsets.ptr, M, rt.row_start.ptr, spec, rt.cand_a.ptr);
-            check(hipGetLastError());
-            s->end_op(ev, kOpJoin, 8.0 * rt.index.rows + 8.0 * M);
-            ev = s->begin_op();
-            rt.distinct_slot = ev;
-            // The join writes candidates in DELTA order; when the distinct's major
-            // column is DELTA's first column they are grouped by it already.
-            const bool grouped = spec.side[spec.order[0]] == 0 && spec.col[spec.order[0]] == 0;
-            if (grouped && s->segmented_distinct) {
-                rt.cand_c.reserve(M ? M : 1);
-                const size_t tb = segmented_distinct_temp_bytes(M);
-                void* temp = s->scratch(tb);
-                PairOutput out{rt.distinct_out.ptr, nullptr, nullptr};
-                check(segmented_distinct(rt.cand_a.ptr, M, rt.shift, known_compressed, rt.cand_b.ptr, rt.cand_c.ptr,
-                                         out, &cdev->distinct_rows[r], temp, tb, s->stream));
-            } else {
-                const size_t tb = distinct_temp_bytes(M);
-                void* temp = s->scratch(tb);
-                PairInput in{rt.cand_a.ptr, nullptr, nullptr, nullptr};
-                PairOutput out{rt.distinct_out.ptr, nullptr, nullptr};
-                check(distinct(in, M, rt.cand_a.ptr, rt.cand_b.ptr, out, &cdev->distinct_rows[r], temp, tb,
-                               known_compressed, s->stream));
-                if (rt.shift != 32)
-                    expand_keys_kernel<<<grid_for(M), kBlock, 0, s->stream>>>(rt.distinct_out.ptr,
-                                                                             &cdev->distinct_rows[r], M, rt.shift);
-            }
-            check(hipGetLastError());
-            s->end_op(ev, kOpDistinct, 16.0 * M);  // + 8U added after the sync
-        }
-        // ---- TargetRuntime::combine (generic.rs:299-358) ----
-        for (auto& t : targets) {
-            RelationState& ts = *find(s, t.target);
-            const uint64_t* contrib = nullptr;
-            const uint64_t* contrib_rows = nullptr;
-            uint64_t bound = 0;
-            for (size_t k = 0; k < t.rules.size(); ++k) {
-                RuleRuntime& rt = rts[t.rules[k]];
-                if (k == 0) {
-                    contrib = rt.distinct_out.ptr;
-                    contrib_rows = &cdev->distinct_rows[t.rules[k]];
-                    bound = rt.join_rows;
-                    continue;
-                }
-                const uint64_t nb = bound + rt.join_rows;
-                t.union_tmp.reserve(nb ? nb : 1);
-                const size_t tb = union_temp_bytes(bound, rt.join_rows);
-                void* temp = s->scratch(tb);
-                PairInput a{contrib, nullptr, nullptr, contrib_rows};
-                PairInput b{rt.distinct_out.ptr, nullptr, nullptr, &cdev->distinct_rows[t.rules[k]]};
-                PairOutput o{t.union_tmp.ptr, nullptr, nullptr};
-                const int ev = s->begin_op();
-                check(sorted_union(a, bound, b, rt.join_rows, o, &cdev->contrib_rows[t.rules[k]], temp, tb,
-                                   s->stream));
-                s->end_op(ev, kOpUnion, 0.0);
-                std::swap(t.union_tmp, t.contributions);
-                contrib = t.contributions.ptr;
-                contrib_rows = &cdev->contrib_rows[t.rules[k]];
-                bound = nb;
-            }
-            t.contrib_bound = bound;
-            // NEWT = contributions \ FULL, as a chain of anti-joins over FULL's
-            // runs (largest first). The last one also writes NEWT's copy that
-            // becomes FULL's newest run.
-            const uint32_t ti = static_cast<uint32_t>(&t - targets.data());
-            t.newt.reserve(bound ? bound : 1);
-            t.new_run = s->take(bound ? bound : 1);
-            if (ts.runs.size() >= kMaxRuns) settle_runs(s, ts, true);
-            AntiRight rights{};
-            rights.count = 0;
-            double right_rows = 0;
-            auto add_right = [&](const DevRelation& r) {
-                if (!r.rows) return;
-                rights.in[rights.count] = PairInput{r.keys.ptr, nullptr, nullptr, nullptr};
-                rights.rows[rights.count] = r.rows;
-                ++rights.count;
-                right_rows += static_cast<double>(r.rows);
-            };
-            add_right(ts.full);
-            for (const auto& r : ts.runs) add_right(r);
-            t.anti_slot = -1;
-            if (bound == 0) {
-                check(set_u64(&cdev->newt_rows[ti], 0, s->stream));
-            } else if (rights.count == 0) {
-                check(copy_keys(contrib, contrib_rows, bound, t.newt.ptr, t.new_run.ptr, &cdev->newt_rows[ti],
-                                s->stream));
-            } else {
-                const size_t tb = anti_join_temp_bytes(bound, rights.count);
-                void* temp = s->scratch(tb);
-                PairInput a{contrib, nullptr, nullptr, contrib_rows};
-                PairOutput o{t.newt.ptr, nullptr, nullptr, t.new_run.ptr};
-                const int ev = s->begin_op();
-                check(sorted_anti_join_multi(a, bound, rights, o, nullptr, &cdev->newt_rows[ti], temp, tb, s->stream));
-                s->end_op(ev, kOpAntiJoin, 8.0 * right_rows);
-                t.anti_slot = ev;
-                algo_bytes += 8.0 * right_rows;
-            }
-        }
-        check(hipMemcpyAsync(&chost, cdev, sizeof(Counters), hipMemcpyDeviceToHost, s->stream));
-        uint32_t err = 0;
-        check(hipMemcpyAsync(&err, lb_error, sizeof(err), hipMemcpyDeviceToHost, s->stream));
-        check(hipStreamSynchronize(s->stream));
-        if (err) throw StatusError{SPARKALOG_HIP_ERR_DEVICE};
-        // per-iteration algorithmic bytes of the operators run (pairs are 8 bytes)
-        for (uint32_t r = 0; r < n_rules; ++r) {
-            const double L = static_cast<double>(rts[r].delta_rows), M = static_cast<double>(rts[r].join_rows);
-            const double U = rts[r].join_rows ? static_cast<double>(chost.distinct_rows[r]) : 0.0;
-            const double R = static_cast<double>(rts[r].index.rows);
-            if (L > 0) algo_bytes += 8 * L + 8 * R + 8 * M + (M > 0 ? 8 * M + 8 * U : 0.0);
-            s->add_op_bytes(rts[r].distinct_slot, 8 * U);
-        }
-        for (uint32_t i = 0; i < n_relations; ++i) {
-            TargetRuntime& t = targets[i];
-            const uint64_t C = t.rules.size() > 1 ? chost.contrib_rows[t.rules.back()]
-                                                  : (t.contrib_bound ? chost.distinct_rows[t.rules[0]] : 0);
-            // the anti-join reads the contributions and writes NEWT twice (DELTA
-            // and FULL's new run)
-            if (t.anti_slot >= 0 || t.contrib_bound) {
-                const double extra = 8.0 * static_cast<double>(C) + 16.0 * static_cast<double>(chost.newt_rows[i]);
-                s->add_op_bytes(t.anti_slot, extra);
-                if (t.contrib_bound) algo_bytes += extra;
-            }
-        }
-        // ---- TargetRuntime::apply (generic.rs:360-397) ----
-        // FULL ∪ NEWT: NEWT is disjoint from every run of FULL, so it becomes
-        // FULL's newest run; runs are merged (pure merges of disjoint sorted
-        // arrays, output size known on the host) while the older run is less
-        // than merge_ratio times the newer one.
-        for (uint32_t i = 0; i < n_relations; ++i) {
-            TargetRuntime& t = targets[i];
-            RelationState& ts = *find(s, t.target);
-            const uint64_t N = chost.newt_rows[i];
-            std::swap(ts.newt.keys, t.newt);
-            ts.newt.rows = N;
-            if (N) {
-                DevRelation run;
-                run.keys = std::move(t.new_run);
-                run.rows = N;
-                ts.runs.push_back(std::move(run));
-            } else {
-                s->give(std::move(t.new_run));
-            }
-            algo_bytes += settle_runs(s, ts, false);
-            std::swap(ts.delta, ts.newt);
-            ts.version += 1;
-            total_new += ts.delta.rows;
-        }
-        ++iterations;
+        session_step(s);
     }
-    // FULL leaves the run as one sorted array
-    for (uint32_t i = 0; i < n_relations; ++i) algo_bytes += settle_runs(s, *find(s, relations[i]), true);
-    check(hipEventRecord(t_end, s->stream));
-    check(hipEventSynchronize(t_end));
-    if (s->profiling) s->collect_ops();
-    float ms = 0.f;
-    check(hipEventElapsedTime(&ms, t_begin, t_end));
-    (void)hipEventDestroy(t_begin);
-    (void)hipEventDestroy(t_end);
-    if (summary) {
-        summary->iterations = iterations;
-        summary->total_new_rows = total_new;
-        summary->device_ms = ms;
-        summary->algorithmic_bytes = algo_bytes;
-        if (rc == SPARKALOG_HIP_OK) summary->delta_rows_left = 0;
+    return session_end(s, summary, rc);
+    SPK_GUARD_END
+}
+
+extern "C" int sparkalog_hip_session_begin(sparkalog_hip_store* s, const uint32_t* relations, uint32_t n_relations,
+                                           const sparkalog_hip_rule* rules, uint32_t n_rules) {
+    if (!s || (n_relations && !relations) || (n_rules && !rules)) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    SPK_GUARD_BEGIN
+    return session_begin(s, relations, n_relations, rules, n_rules);
+    SPK_GUARD_END
+}
+
+extern "C" int sparkalog_hip_session_step(sparkalog_hip_store* s, uint64_t* delta_rows) {
+    if (!s || !session_of(s).active) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    SPK_GUARD_BEGIN
+    if (pending_delta(s)) session_step(s);
+    if (delta_rows) *delta_rows = pending_delta(s);
+    return SPARKALOG_HIP_OK;
+    SPK_GUARD_END
+}
+
+extern "C" int sparkalog_hip_session_end(sparkalog_hip_store* s, sparkalog_hip_summary* summary) {
+    if (!s || !session_of(s).active) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    SPK_GUARD_BEGIN
+    return session_end(s, summary, SPARKALOG_HIP_OK);
+    SPK_GUARD_END
+}
+
+extern "C" int sparkalog_hip_store_instance_stats(sparkalog_hip_store* s, uint32_t id, uint32_t instance_bits,
+                                                  uint32_t instances, uint64_t* rows, uint64_t* digests) {
+    if (!s || (instances && (!rows || !digests))) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    RelationState* st = find(s, id);
+    if (!st) return SPARKALOG_HIP_ERR_MISSING_RELATION;
+    SPK_GUARD_BEGIN
+    if (!st->runs.empty()) settle_runs(s, *st, true);
+    DevBuf<unsigned long long> acc;
+    acc.reserve(2 * static_cast<size_t>(instances) + 2);
+    check(fill_u64(reinterpret_cast<uint64_t*>(acc.ptr), 2 * static_cast<size_t>(instances), 0, s->stream));
+    if (st->full.rows && instances)
+        instance_stats_kernel<<<grid_for(st->full.rows), kBlock, 0, s->stream>>>(
+            st->full.keys.ptr, st->full.rows, instance_bits, instances, acc.ptr, acc.ptr + instances);
+    check(hipGetLastError());
+    if (instances) {
+        check(hipMemcpyAsync(rows, acc.ptr, instances * sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
+        check(hipMemcpyAsync(digests, acc.ptr + instances, instances * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                             s->stream));
     }
-    return rc;
+    check(hipStreamSynchronize(s->stream));
+    return SPARKALOG_HIP_OK;
+    SPK_GUARD_END
+}
+
+extern "C" int sparkalog_hip_store_range_stats(sparkalog_hip_store* s, uint32_t id, int which, uint32_t first_lo,
+                                               uint32_t first_hi, uint32_t base, uint64_t* rows, uint64_t* digest) {
+    if (!s) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    RelationState* st = find(s, id);
+    if (!st) return SPARKALOG_HIP_ERR_MISSING_RELATION;
+    if (which == 1 && !st->recursive) return SPARKALOG_HIP_ERR_NOT_RECURSIVE;
+    SPK_GUARD_BEGIN
+    std::vector<const DevRelation*> parts;
+    if (which == 1) {
+        parts.push_back(&st->delta);
+    } else {
+        parts.push_back(&st->full);
+        for (const auto& r : st->runs) parts.push_back(&r);
+    }
+    DevBuf<unsigned long long> acc;
+    acc.reserve(2);
+    check(fill_u64(reinterpret_cast<uint64_t*>(acc.ptr), 2, 0, s->stream));
+    for (const DevRelation* r : parts) {
+        const auto range = first_range(s, *r, first_lo, first_hi);
+        if (range.second > range.first)
+            range_stats_kernel<<<grid_for(range.second - range.first), kBlock, 0, s->stream>>>(
+                r->keys.ptr, range.first, range.second, base, acc.ptr);
+    }
+    check(hipGetLastError());
+    unsigned long long host[2];
+    check(hipMemcpyAsync(host, acc.ptr, sizeof(host), hipMemcpyDeviceToHost, s->stream));
+    check(hipStreamSynchronize(s->stream));
+    if (rows) *rows = host[0];
+    if (digest) *digest = host[1];
+    return SPARKALOG_HIP_OK;
+    SPK_GUARD_END
+}
+
+extern "C" int sparkalog_hip_store_merge_rows(sparkalog_hip_store* s, uint32_t id, int which, const uint32_t* c0,
+                                              const uint32_t* c1, uint64_t rows, int on_device) {
+    if (!s || (rows && (!c0 || !c1))) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    RelationState* st = find(s, id);
+    if (!st) return SPARKALOG_HIP_ERR_MISSING_RELATION;
+    if (which == 1 && !st->recursive) return SPARKALOG_HIP_ERR_NOT_RECURSIVE;
+    SPK_GUARD_BEGIN
+    if (rows == 0) return SPARKALOG_HIP_OK;
+    DevRelation incoming;
+    const uint32_t mv = upload(s, incoming, c0, c1, rows, on_device);
+    st->max_value = mv > st->max_value ? mv : st->max_value;
+    if (which == 1) {
+        union_into(s, st->delta, incoming.keys.ptr, incoming.rows);
+    } else if (st->recursive) {
+        // rows disjoint from FULL become a run of their own
+        st->runs.push_back(std::move(incoming));
+        st->version += 1;
+    } else {
+        union_into(s, st->full, incoming.keys.ptr, incoming.rows);
+        st->version += 1;
+    }
+    return SPARKALOG_HIP_OK;
+    SPK_GUARD_END
+}
+
+extern "C" int sparkalog_hip_store_remove_range(sparkalog_hip_store* s, uint32_t id, uint32_t first_lo,
+                                                uint32_t first_hi) {
+    if (!s) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    RelationState* st = find(s, id);
+    if (!st) return SPARKALOG_HIP_ERR_MISSING_RELATION;
+    SPK_GUARD_BEGIN
+    auto strip = [&](DevRelation& r) {
+        const auto range = first_range(s, r, first_lo, first_hi);
+        remove_rows(s, r, range.first, range.second);
+    };
+    strip(st->full);
+    for (auto& r : st->runs) strip(r);
+    if (st->recursive) strip(st->delta);
+    st->version += 1;
+    return SPARKALOG_HIP_OK;
+    SPK_GUARD_END
+}
+
+extern "C" int sparkalog_hip_store_range_rows(sparkalog_hip_store* s, uint32_t id, int which,
+                                              const uint32_t* first_bounds, uint32_t ranges, uint64_t* rows) {
+    if (!s || (ranges && (!first_bounds || !rows))) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    RelationState* st = find(s, id);
+    if (!st) return SPARKALOG_HIP_ERR_MISSING_RELATION;
+    if (which == 1 && !st->recursive) return SPARKALOG_HIP_ERR_NOT_RECURSIVE;
+    SPK_GUARD_BEGIN
+    for (uint32_t j = 0; j < ranges; ++j) rows[j] = 0;
+    if (ranges == 0) return SPARKALOG_HIP_OK;
+    std::vector<const DevRelation*> parts;
+    if (which == 1) {
+        parts.push_back(&st->delta);
+    } else {
+        parts.push_back(&st->full);
+        for (const auto& r : st->runs) parts.push_back(&r);
+    }
+    std::vector<uint64_t> probes(2 * static_cast<size_t>(ranges));
+    for (uint32_t j = 0; j < 2 * ranges; ++j) probes[j] = static_cast<uint64_t>(first_bounds[j]) << 32;
+    DevBuf<uint64_t> dev;
+    dev.reserve(4 * static_cast<size_t>(ranges));
+    check(hipMemcpyAsync(dev.ptr, probes.data(), probes.size() * sizeof(uint64_t), hipMemcpyHostToDevice, s->stream));
+    std::vector<uint64_t> pos(probes.size());
+    for (const DevRelation* r : parts) {
+        if (r->rows == 0) continue;
+        lower_bounds_kernel<<<grid_for(probes.size()), kBlock, 0, s->stream>>>(r->keys.ptr, r->rows, dev.ptr,
+                                                                             probes.size(), dev.ptr + probes.size());
+        check(hipGetLastError());
+        check(hipMemcpyAsync(pos.data(), dev.ptr + probes.size(), pos.size() * sizeof(uint64_t),
+                             hipMemcpyDeviceToHost, s->stream));
+        check(hipStreamSynchronize(s->stream));
+        for (uint32_t j = 0; j < ranges; ++j)
+            if (pos[2 * j + 1] > pos[2 * j]) rows[j] += pos[2 * j + 1] - pos[2 * j];
+    }
+    return SPARKALOG_HIP_OK;
+    SPK_GUARD_END
+}
+
+extern "C" int sparkalog_hip_set_instance_tracking(sparkalog_hip_store* s, uint32_t instance_bits,
+                                                   uint32_t instances) {
+    if (!s || instance_bits >= 32) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    Session& ss = session_of(s);
+    ss.instance_bits = instances ? instance_bits : 0;
+    ss.instances = instances;
+    return SPARKALOG_HIP_OK;
+}
+
+extern "C" int sparkalog_hip_instance_iterations(sparkalog_hip_store* s, uint32_t* iterations, uint32_t instances) {
+    if (!s || (instances && !iterations)) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    Session& ss = session_of(s);
+    if (!ss.instance_bits || instances > ss.instances) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    SPK_GUARD_BEGIN
+    if (instances)
+        check(hipMemcpyAsync(iterations, ss.last_new.ptr, instances * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             s->stream));
+    check(hipStreamSynchronize(s->stream));
+    return SPARKALOG_HIP_OK;
     SPK_GUARD_END
 }

@@ -62,13 +62,18 @@ hipError_t unique_sorted(const uint64_t* keys, const uint64_t* rows_dev, size_t 
 
 // Distinct of compressed keys (major << minor_bits | minor) whose major part
 // is non-decreasing along the input; writes sorted unique expanded pairs
-// (major << 32 | minor). side_a/side_b (rows entries each) hold keys of runs
-// too long for one tile; side_varying is a superset of the varying bits of
-// the compressed keys.
-size_t segmented_distinct_temp_bytes(size_t rows);
-hipError_t segmented_distinct(const uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t side_varying,
-                              uint64_t* side_a, uint64_t* side_b, PairOutput out, uint64_t* unique_rows, void* temp,
-                              size_t temp_bytes, hipStream_t stream);
+// (major << 32 | minor). Two phases: prepare partitions the input into tiles
+// aligned to runs, copies runs longer than a tile to `side` and returns their
+// row count on the host; finish sorts those (side/side_b, sort_temp sized by
+// distinct_temp_bytes(side_rows); unused when side_rows is 0) and every tile.
+// `meta` (segmented_distinct_meta_bytes(rows)) must persist between phases.
+size_t segmented_distinct_meta_bytes(size_t rows);
+hipError_t segmented_distinct_prepare(const uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t* side,
+                                      void* meta, size_t meta_bytes, uint64_t* side_rows, hipStream_t stream);
+hipError_t segmented_distinct_finish(const uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t side_varying,
+                                     uint64_t* side_a, uint64_t* side_b, uint64_t side_rows, PairOutput out,
+                                     uint64_t* unique_rows, void* meta, size_t meta_bytes, void* sort_temp,
+                                     size_t sort_bytes, hipStream_t stream);
 
 // ---- sorted set operators ----
 size_t union_temp_bytes(size_t left_rows, size_t right_rows);

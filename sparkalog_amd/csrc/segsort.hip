@@ -46,9 +46,10 @@ struct SegHeader {
 };
 
 struct SegLayout {
-    size_t status_off, bounds_off, sort_off, total, tiles;
+    size_t status_off, bounds_off, total, tiles;
 };
 
+// Metadata of one segmented distinct: header, look-back status and tile bounds.
 SegLayout seg_layout(size_t rows) {
     SegLayout l{};
     l.tiles = seg_tiles(rows);
@@ -57,8 +58,6 @@ SegLayout seg_layout(size_t rows) {
     off += seg_align(l.tiles * sizeof(uint64_t));
     l.bounds_off = off;
     off += seg_align((l.tiles + 1) * sizeof(size_t));
-    l.sort_off = off;
-    off += distinct_temp_bytes(rows);
     l.total = off;
     return l;
 }
@@ -332,42 +331,52 @@ __global__ __launch_bounds__(kBlock) void seg_tile_kernel(const uint64_t* keys, 
 
 }  // namespace
 
-size_t segmented_distinct_temp_bytes(size_t rows) { return seg_layout(rows).total; }
+size_t segmented_distinct_meta_bytes(size_t rows) { return seg_layout(rows).total; }
 
-hipError_t segmented_distinct(const uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t side_varying,
-                              uint64_t* side_a, uint64_t* side_b, PairOutput out, uint64_t* unique_rows, void* temp,
-                              size_t temp_bytes, hipStream_t stream) {
-    if (rows == 0) return set_u64(unique_rows, 0, stream);
+hipError_t segmented_distinct_prepare(const uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t* side,
+                                      void* meta, size_t meta_bytes, uint64_t* side_rows, hipStream_t stream) {
+    *side_rows = 0;
+    if (rows == 0) return hipSuccess;
     const SegLayout l = seg_layout(rows);
-    if (temp_bytes < l.total || minor_bits == 0 || minor_bits > 32) return hipErrorInvalidValue;
-    char* base = static_cast<char*>(temp);
+    if (meta_bytes < l.total || minor_bits == 0 || minor_bits > 32) return hipErrorInvalidValue;
+    char* base = static_cast<char*>(meta);
     SegHeader* h = reinterpret_cast<SegHeader*>(base);
     uint64_t* status = reinterpret_cast<uint64_t*>(base + l.status_off);
     size_t* bounds = reinterpret_cast<size_t*>(base + l.bounds_off);
-    void* sort_temp = base + l.sort_off;
-    const size_t sort_bytes = distinct_temp_bytes(rows);
     const unsigned small_grid = static_cast<unsigned>(l.tiles / kBlock + 1 < 4096 ? l.tiles / kBlock + 1 : 4096);
     seg_init_kernel<<<small_grid, kBlock, 0, stream>>>(h, status, l.tiles);
     seg_partition_kernel<<<small_grid, kBlock, 0, stream>>>(keys, rows, minor_bits, bounds, l.tiles);
     seg_gather_kernel<<<static_cast<unsigned>(l.tiles < 1024 ? l.tiles : 1024), kBlock, 0, stream>>>(
-        keys, bounds, l.tiles, side_a, h);
+        keys, bounds, l.tiles, side, h);
     hipError_t st = hipGetLastError();
-    if (st != hipSuccess) return st;
-    // oversized runs (usually none): one 8-byte read tells whether the global
-    // sort of the gathered keys is needed, and sizes it exactly
-    uint64_t side_rows = 0;
-    st = hipMemcpyAsync(&side_rows, &h->side_rows, sizeof(side_rows), hipMemcpyDeviceToHost, stream);
+    // runs longer than a tile (usually none): one 8-byte read sizes their sort
+    if (st == hipSuccess) st = hipMemcpyAsync(side_rows, &h->side_rows, sizeof(uint64_t), hipMemcpyDeviceToHost, stream);
     if (st == hipSuccess) st = hipStreamSynchronize(stream);
-    if (st != hipSuccess) return st;
-    uint64_t* side_unique_out = side_b;
+    return st;
+}
+
+hipError_t segmented_distinct_finish(const uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t side_varying,
+                                     uint64_t* side_a, uint64_t* side_b, uint64_t side_rows, PairOutput out,
+                                     uint64_t* unique_rows, void* meta, size_t meta_bytes, void* sort_temp,
+                                     size_t sort_bytes, hipStream_t stream) {
+    if (rows == 0) return set_u64(unique_rows, 0, stream);
+    const SegLayout l = seg_layout(rows);
+    if (meta_bytes < l.total) return hipErrorInvalidValue;
+    char* base = static_cast<char*>(meta);
+    SegHeader* h = reinterpret_cast<SegHeader*>(base);
+    uint64_t* status = reinterpret_cast<uint64_t*>(base + l.status_off);
+    size_t* bounds = reinterpret_cast<size_t*>(base + l.bounds_off);
+    hipError_t st = hipSuccess;
+    uint64_t* side_unique_out = side_a;
     if (side_rows) {
+        if (!side_b || sort_bytes < distinct_temp_bytes(side_rows)) return hipErrorInvalidValue;
         PairInput side_in{side_a, nullptr, nullptr, nullptr};
         uint64_t* sorted = nullptr;
         st = radix_sort(side_in, side_rows, side_a, side_b, sort_temp, sort_bytes, side_varying, stream, &sorted);
         if (st != hipSuccess) return st;
         side_unique_out = sorted == side_a ? side_b : side_a;
-        st = unique_sorted(sorted, nullptr, side_rows, PairOutput{side_unique_out, nullptr, nullptr},
-                           &h->side_unique, sort_temp, sort_bytes, stream);
+        st = unique_sorted(sorted, nullptr, side_rows, PairOutput{side_unique_out, nullptr, nullptr}, &h->side_unique,
+                           sort_temp, sort_bytes, stream);
         if (st != hipSuccess) return st;
     }
     seg_tile_kernel<<<static_cast<unsigned>(l.tiles), kBlock, 0, stream>>>(keys, rows, minor_bits, bounds,

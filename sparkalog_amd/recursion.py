@@ -136,6 +136,47 @@ class RelationStore:
         _check(self._lib.sparkalog_hip_store_digest(self._h, int(id), ctypes.byref(d), ctypes.byref(ok), ctypes.byref(n)))
         return d.value, bool(ok.value), n.value
 
+    # ---- many instances in one relation (batched and served fixpoints) ----
+
+    def set_instance_tracking(self, instance_bits: int, instances: int) -> None:
+        _check(self._lib.sparkalog_hip_set_instance_tracking(self._h, instance_bits, instances))
+
+    def instance_iterations(self, instances: int) -> np.ndarray:
+        out = np.zeros(max(instances, 1), dtype=np.uint32)
+        _check(self._lib.sparkalog_hip_instance_iterations(self._h, out.ctypes.data_as(ctypes.c_void_p), instances))
+        return out[:instances]
+
+    def instance_stats(self, id: RelationId, instance_bits: int, instances: int) -> tuple[np.ndarray, np.ndarray]:
+        rows = np.zeros(max(instances, 1), dtype=np.uint64)
+        digests = np.zeros(max(instances, 1), dtype=np.uint64)
+        _check(self._lib.sparkalog_hip_store_instance_stats(self._h, int(id), instance_bits, instances,
+                                                            rows.ctypes.data_as(ctypes.c_void_p),
+                                                            digests.ctypes.data_as(ctypes.c_void_p)))
+        return rows[:instances], digests[:instances]
+
+    def range_stats(self, id: RelationId, which: int, first_lo: int, first_hi: int, base: int) -> tuple[int, int]:
+        rows = ctypes.c_uint64(0)
+        digest = ctypes.c_uint64(0)
+        _check(self._lib.sparkalog_hip_store_range_stats(self._h, int(id), which, first_lo, first_hi, base,
+                                                         ctypes.byref(rows), ctypes.byref(digest)))
+        return rows.value, digest.value
+
+    def range_rows(self, id: RelationId, which: int, bounds) -> np.ndarray:
+        """Rows per first-column range [bounds[2j], bounds[2j+1])."""
+        b = np.ascontiguousarray(np.asarray(bounds, dtype=np.uint32))
+        n = b.size // 2
+        out = np.zeros(max(n, 1), dtype=np.uint64)
+        _check(self._lib.sparkalog_hip_store_range_rows(self._h, int(id), which, b.ctypes.data_as(ctypes.c_void_p), n,
+                                                        out.ctypes.data_as(ctypes.c_void_p)))
+        return out[:n]
+
+    def merge_rows(self, id: RelationId, which: int, relation) -> None:
+        c0, c1, n, dev, _keep = _columns(relation)
+        _check(self._lib.sparkalog_hip_store_merge_rows(self._h, int(id), which, c0, c1, n, dev))
+
+    def remove_range(self, id: RelationId, first_lo: int, first_hi: int) -> None:
+        _check(self._lib.sparkalog_hip_store_remove_range(self._h, int(id), first_lo, first_hi))
+
     def set_profiling(self, enabled: bool) -> None:
         _check(self._lib.sparkalog_hip_store_set_profiling(self._h, int(enabled)))
 
@@ -178,6 +219,25 @@ class RecursiveExecutor:
         _check(_native.hip().sparkalog_hip_executor_compile(store._h, ex._relations, len(plan.relations),
                                                             ex._rules, len(plan.rules)))
         return ex
+
+    def begin(self, store: RelationStore) -> None:
+        """Stepwise run: compile and start (see step/end)."""
+        _check(_native.hip().sparkalog_hip_session_begin(store._h, self._relations, len(self.plan.relations),
+                                                         self._rules, len(self.plan.rules)))
+
+    def step(self, store: RelationStore) -> int:
+        """One semi-naive iteration (none when every DELTA is empty); returns the
+        DELTA rows left."""
+        left = ctypes.c_uint64(0)
+        _check(_native.hip().sparkalog_hip_session_step(store._h, ctypes.byref(left)))
+        return left.value
+
+    def end(self, store: RelationStore) -> SccSummary:
+        summary = _native.Summary()
+        _check(_native.hip().sparkalog_hip_session_end(store._h, ctypes.byref(summary)))
+        return SccSummary(iterations=summary.iterations, total_new_rows=summary.total_new_rows,
+                          relation_rows=[(r, store.rows(r, 0)) for r in self.plan.relations],
+                          device_ms=summary.device_ms, algorithmic_bytes=summary.algorithmic_bytes)
 
     def run(self, store: RelationStore, max_iterations: int) -> SccSummary:
         summary = _native.Summary()

@@ -1,0 +1,74 @@
+"""Batched and served fixpoints on the GPU.
+
+Batched: many independent TC instances in one semi-naive loop (in waves
+under an HBM budget) give, per instance, the same closure rows, digest and
+iteration count as brute-force BFS and as the plain one-instance-per-run
+path; the full 4,096-instance batch matches committed BFS fixtures.
+Served: every request's result is the same alone (plain, sequential) and in
+the continuously batched mix, and equals BFS.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from sparkalog_amd import batch, serving, workload
+from sparkalog_amd.recursion import RelationStore
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def bfs_per_instance(s, d, offs, counts):
+    out = []
+    for i in range(counts.size):
+        r = oracle.tc_bfs(s[offs[i]:offs[i + 1]], d[offs[i]:offs[i + 1]], int(counts[i]), threads=8)
+        out.append((r["pairs"], r["digest"], r["max_dist"]))
+    return np.array(out, dtype=np.uint64)
+
+
+def test_batched_matches_bfs_and_plain(stream):
+    s, d, offs, counts = workload.batch_graphs(seed=0x5B, instances=160, n_max=2048)
+    ref = bfs_per_instance(s, d, offs, counts)
+    store = RelationStore(stream)
+    total_est = sum(batch.estimate_closure(int(n), int(e)) for n, e in zip(counts, np.diff(offs.astype(np.int64))))
+    res = batch.run_batched(s, d, offs, counts, store, budget_pairs=total_est / 3)
+    assert len(res.waves) >= 3
+    assert np.array_equal(res.rows, ref[:, 0])
+    assert np.array_equal(res.digests, ref[:, 1])
+    assert np.array_equal(res.iterations.astype(np.uint64), ref[:, 2])
+    plain = batch.run_plain(s, d, offs, counts, store)
+    assert np.array_equal(plain.rows, res.rows) and np.array_equal(plain.digests, res.digests)
+    assert np.array_equal(plain.iterations, res.iterations)
+
+
+def test_full_batch_matches_fixture(stream):
+    exp = np.load(GOLDEN / "batch_expected.npz")
+    s, d, offs, counts = workload.batch_graphs()
+    assert workload.pair_digest(s, d) == int(exp["edge_digest"])
+    store = RelationStore(stream)
+    res = batch.run_batched(s, d, offs, counts, store, budget_pairs=4.5e9)
+    assert int(res.rows.sum()) == int(exp["pairs"].sum())
+    assert np.array_equal(res.rows, exp["pairs"])
+    assert np.array_equal(res.digests, exp["digest"])
+    assert np.array_equal(res.iterations, exp["max_dist"])
+    # spot-check the plain path on every 256th instance
+    picks = list(range(0, counts.size, 256))
+    plain = batch.run_plain(s, d, offs, counts, store, instances=picks)
+    assert np.array_equal(plain.rows[picks], exp["pairs"][picks])
+    assert np.array_equal(plain.digests[picks], exp["digest"][picks])
+
+
+def test_served_alone_equals_mix(stream):
+    reqs = serving.make_requests(24, rate_per_s=400.0, seed=0x5E, min_log2_edges=8, max_log2_edges=13)
+    store = RelationStore(stream)
+    mixed, _ = serving.serve_batched(reqs, store, budget_pairs=1e9, max_slots=64)
+    alone, _ = serving.serve_plain(reqs, store)
+    assert [r.rid for r in mixed] == list(range(24))
+    for m, a, req in zip(mixed, alone, reqs):
+        assert (m.rows, m.digest) == (a.rows, a.digest), req.rid
+        bfs = oracle.tc_bfs(req.src, req.dst, req.nodes)
+        assert (m.rows, m.digest) == (bfs["pairs"], bfs["digest"]), req.rid
+    # requests overlapped: some were admitted before an earlier one finished
+    assert any(mixed[i + 1].admitted < mixed[i].done for i in range(len(mixed) - 1))
