@@ -37,10 +37,12 @@ def parse_args():
     p.add_argument("--nodes", type=int, default=1 << 20)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-nodes", type=int, default=1 << 16)
-    p.add_argument("--mode", choices=["tc", "ops", "batched", "served"], default="tc",
+    p.add_argument("--mode", choices=["tc", "ops", "batched", "served", "copy"], default="tc",
                    help="tc: the headline fixpoint; ops: operator microbench at 2^k rows; "
                         "batched: 4,096 instances in one loop vs one run each; served: Poisson stream, "
-                        "continuous batching vs sequential")
+                        "continuous batching vs sequential; copy: plain device copy of --copy-bytes "
+                        "(the known-size reference for HBM byte counters)")
+    p.add_argument("--copy-bytes", type=int, default=1 << 30, help="copy mode: bytes copied per launch")
     p.add_argument("--budget-pairs", type=float, default=4.5e9, help="batched/served: HBM budget in closure pairs")
     p.add_argument("--requests", type=int, default=200, help="served: stream length")
     p.add_argument("--utilization", type=float, default=0.7, help="served: offered load of the plain path")
@@ -133,6 +135,36 @@ def run_served_mode(args) -> None:
                       "results_equal_alone_vs_mix": equal}), flush=True)
 
 
+def run_copy_mode(args) -> None:
+    """The library's plain device copy of --copy-bytes, launched --steps
+    times and timed with HIP events on its stream: the measured HBM copy
+    ceiling, and (under rocprofv3 --pmc) the known-size kernel whose
+    FETCH_SIZE / WRITE_SIZE readings scale those of the fixpoint's kernels."""
+    import torch
+
+    from sparkalog_amd.execution import HipStream, copy_u64, fill_mod_u32
+
+    torch.cuda.set_device(0)
+    stream = HipStream()
+    words = args.copy_bytes // 8
+    src = torch.empty(words * 2, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    fill_mod_u32(src, 1_000_003, stream)
+    copy_u64(src, dst, stream)  # warm-up
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream.torch_stream)
+    for _ in range(args.steps):
+        copy_u64(src, dst, stream)
+    stop.record(stream.torch_stream)
+    stop.synchronize()
+    ms = start.elapsed_time(stop) / args.steps
+    ok = bool(torch.equal(dst, src))
+    print(json.dumps({"mode": "copy", "kernel": "copy_u64_kernel", "bytes": words * 8, "launches": args.steps + 1,
+                      "ms_per_copy": round(ms, 4), "gbs_read_plus_write": round(2 * words * 8 / ms / 1e6, 1),
+                      "frac_of_hbm_peak": round(2 * words * 8 / ms / 1e6 / HBM_PEAK_GBS, 4), "equal": ok}),
+          flush=True)
+
+
 def cpu_baseline(sample_nodes: int) -> dict:
     """The oracle's parallel semi-naive executor (Rayon-style chunking) on a
     smaller draw of the same component-graph distribution."""
@@ -167,6 +199,9 @@ def main():
         return
     if args.mode == "served":
         run_served_mode(args)
+        return
+    if args.mode == "copy":
+        run_copy_mode(args)
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -214,17 +249,23 @@ def main():
     algo_bytes = summaries[-1].algorithmic_bytes
     device_ms = sum(s.device_ms for s in summaries) / len(summaries)
 
-    # dominant operator by device time over the timed steps
-    dom_name, dom = max(((k, v) for k, v in ops.items() if v["launches"]), key=lambda kv: kv[1]["ms"])
-    achieved = (dom["bytes"] / dom["launches"]) / (dom["ms"] / dom["launches"] / 1000.0) / 1e9
+    # dominant kernel: the operator main kernel with the most device time
+    # over the timed steps (HIP events around that kernel alone); its
+    # algorithmic bytes are the operator's (8 bytes per pair read or written)
+    timed = [(k, v) for k, v in ops.items() if v["kernel_launches"] and v["kernel_launches"] == v["launches"]]
+    dom_op, dom = max(timed, key=lambda kv: kv[1]["kernel_ms"])
+    launches = dom["kernel_launches"]
+    avg_ms = dom["kernel_ms"] / launches
+    bytes_per_launch = dom["bytes"] / launches
+    achieved = bytes_per_launch / (avg_ms / 1000.0) / 1e9
     traffic = None
     if PROFILE_SUMMARY.exists():
         prof = json.loads(PROFILE_SUMMARY.read_text())
-        traffic = prof.get("traffic_bytes_per_launch", {}).get(dom_name)
+        traffic = prof.get("traffic_bytes_per_launch", {}).get(dom["kernel"])
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom_name,
-                "launches_per_step": dom["launches"] // args.steps,
-                "avg_launch_ms": round(dom["ms"] / dom["launches"], 4)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom["kernel"],
+                "operator": dom_op, "launches_per_step": launches // args.steps,
+                "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": round(bytes_per_launch, 1)}
 
     line = {
         "metric": "TC result tuples/sec + HBM GB/s (1 GPU); batched/served p50/p99 vs plain",
@@ -245,6 +286,7 @@ def main():
         "hbm_gbs": round(algo_bytes / (ms_per_step / 1000.0) / 1e9, 1),
         "device_ms_per_step": round(device_ms, 3),
         "ops": {k: {"launches": v["launches"] // args.steps, "ms": round(v["ms"] / args.steps, 3),
+                    "kernel_ms": round(v["kernel_ms"] / args.steps, 3),
                     "gbs": round(v["bytes"] / (v["ms"] / 1000.0) / 1e9, 1) if v["ms"] else None}
                 for k, v in ops.items() if v["launches"]},
         "roofline": roofline,

@@ -63,6 +63,12 @@ sparkalog_hip_error sparkalog_hip_filter_u32(const uint32_t* input, size_t len, 
 sparkalog_hip_error sparkalog_hip_fill_mod_u32(uint32_t* output, size_t len, uint32_t modulus,
                                                void* stream);
 
+/* Bench helper with no counterpart in the reference: a plain device copy of
+ * `count` u64 words (both pointers 16-byte aligned). It measures the HBM copy
+ * ceiling and is the known-size kernel that calibrates HBM byte counters. */
+sparkalog_hip_error sparkalog_hip_copy_u64(const uint64_t* source, uint64_t* destination, size_t count,
+                                           void* stream);
+
 /* column_ops.cu:339-354. */
 sparkalog_hip_error sparkalog_hip_join_u32_temporary_bytes(const uint64_t* counts, uint64_t* offsets,
                                                            size_t left_rows, size_t* temporary_bytes,
@@ -223,12 +229,18 @@ int sparkalog_hip_executor_run(sparkalog_hip_store* store, const uint32_t* relat
                                sparkalog_hip_summary* summary);
 
 /* Per-operator profiling: when enabled, every operator of a run is bracketed
- * by HIP events on the store's stream. op: 0 join, 1 distinct, 2 anti-join,
- * 3 union, 4 index build. Reports launches, summed device milliseconds and
+ * by HIP events on the store's stream. op: 0 join count, 1 join emit,
+ * 2 distinct, 3 anti-join, 4 union of rule contributions, 5 merge of FULL's
+ * runs, 6 index build. Reports launches, summed device milliseconds and
  * summed algorithmic bytes (8 bytes per pair read or written). */
 int sparkalog_hip_store_set_profiling(sparkalog_hip_store* store, int enabled);
 int sparkalog_hip_store_op_stats(sparkalog_hip_store* store, int op, uint64_t* launches, double* ms,
                                  double* bytes);
+/* The operator's name and main kernel, and that kernel's own launches and
+ * time (events around it alone). `store` may be null to query names only;
+ * returns SPARKALOG_HIP_ERR_INVALID_ARGUMENT past the last operator. */
+int sparkalog_hip_store_op_kernel_stats(sparkalog_hip_store* store, int op, const char** name, const char** kernel,
+                                        uint64_t* kernel_launches, double* kernel_ms);
 
 /* ---- stepwise fixpoint (for callers that change relations between steps) ----
  * begin = RecursiveExecutor::compile; step = one loop trip of

@@ -47,6 +47,7 @@ _HIP_SIGNATURES = {
     "sparkalog_hip_filter_u32_temporary_bytes": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, POINTER(c_size_t), c_void_p]),
     "sparkalog_hip_filter_u32": (c_int, [c_void_p, c_size_t, c_uint32, c_uint32, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
     "sparkalog_hip_fill_mod_u32": (c_int, [c_void_p, c_size_t, c_uint32, c_void_p]),
+    "sparkalog_hip_copy_u64": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
     "sparkalog_hip_join_u32_temporary_bytes": (c_int, [c_void_p, c_void_p, c_size_t, POINTER(c_size_t), c_void_p]),
     "sparkalog_hip_join_u32_count": (c_int, [c_void_p, c_size_t, c_void_p, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
     "sparkalog_hip_join_u32_emit": (c_int, [c_void_p, c_size_t, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_uint32, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p, c_void_p]),
@@ -66,6 +67,8 @@ _HIP_SIGNATURES = {
     "sparkalog_hip_store_digest": (c_int, [c_void_p, c_uint32, u64p, POINTER(c_int), u64p]),
     "sparkalog_hip_store_set_profiling": (c_int, [c_void_p, c_int]),
     "sparkalog_hip_store_op_stats": (c_int, [c_void_p, c_int, u64p, POINTER(c_double), POINTER(c_double)]),
+    "sparkalog_hip_store_op_kernel_stats": (c_int, [c_void_p, c_int, POINTER(ctypes.c_char_p), POINTER(ctypes.c_char_p),
+                                                     u64p, POINTER(c_double)]),
     "sparkalog_hip_executor_compile": (c_int, [c_void_p, u32p, c_uint32, POINTER(Rule), c_uint32]),
     "sparkalog_hip_executor_run": (c_int, [c_void_p, u32p, c_uint32, POINTER(Rule), c_uint32, c_uint64, POINTER(Summary)]),
     "sparkalog_hip_session_begin": (c_int, [c_void_p, u32p, c_uint32, POINTER(Rule), c_uint32]),

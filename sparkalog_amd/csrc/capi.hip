@@ -54,6 +54,15 @@ extern "C" sparkalog_hip_error sparkalog_hip_fill_mod_u32(uint32_t* output, size
     return spk::fill_mod_u32(output, len, modulus, as_stream(stream));
 }
 
+extern "C" sparkalog_hip_error sparkalog_hip_copy_u64(const uint64_t* source, uint64_t* destination, size_t count,
+                                                      void* stream) {
+    if (count == 0) return hipSuccess;
+    if (source == nullptr || destination == nullptr || (reinterpret_cast<uintptr_t>(source) & 15) ||
+        (reinterpret_cast<uintptr_t>(destination) & 15))
+        return hipErrorInvalidValue;
+    return spk::copy_u64(source, destination, count, as_stream(stream));
+}
+
 extern "C" sparkalog_hip_error sparkalog_hip_join_u32_temporary_bytes(const uint64_t* counts, uint64_t* offsets,
                                                                       size_t left_rows, size_t* temporary_bytes,
                                                                       void* stream) {

@@ -114,16 +114,32 @@ struct RelationState {
     uint32_t max_value = 0;  // max over both columns of every row ever stored
 };
 
-// Per-op timing (hipEvents) and algorithmic bytes when profiling is on.
+// Per-op timing (hipEvents) and algorithmic bytes when profiling is on: the
+// whole operator (helper kernels included) and, separately, its main kernel.
 struct OpStat {
     const char* name;
+    const char* kernel;
     uint64_t launches = 0;
     double ms = 0.0;
     double bytes = 0.0;
+    uint64_t kernel_launches = 0;
+    double kernel_ms = 0.0;
 };
 
-enum OpKind { kOpJoin = 0, kOpDistinct, kOpAntiJoin, kOpUnion, kOpIndex, kOpCount };
-const char* const kOpNames[kOpCount] = {"join", "distinct", "anti_join", "union", "index"};
+enum OpKind { kOpJoinCount = 0, kOpJoinEmit, kOpDistinct, kOpAntiJoin, kOpUnion, kOpMergeRuns, kOpIndex, kOpCount };
+const char* const kOpNames[kOpCount] = {"join_count", "join_emit", "distinct", "anti_join",
+                                        "union", "merge_runs", "index"};
+const char* const kOpKernels[kOpCount] = {"dense_join_count_kernel", "dense_join_emit_kernel", "seg_tile_kernel",
+                                          "anti_join_kernel", "union_kernel", "union_kernel", ""};
+
+// Events of one profiled operator launch.
+struct OpEvents {
+    hipEvent_t begin = nullptr, end = nullptr;
+    KernelTimer kernel;
+    bool kernel_used = false;
+    int op = 0;
+    double bytes = 0.0;
+};
 
 // ------------------------------------------------------------------------
 // executor kernels
@@ -454,6 +470,7 @@ struct TargetRuntime {
     DevBuf<uint64_t> new_run;      // NEWT's second copy, becomes a run of FULL
     uint64_t contrib_bound = 0;
     int anti_slot = -1;            // profiling event of this iteration's anti-join
+    std::vector<int> union_slots;  // profiling events of this iteration's contribution unions
 };
 
 // Small device counters read back once per phase.
@@ -479,9 +496,8 @@ struct sparkalog_hip_store {
     size_t pinned_bytes = 0;
     bool profiling = false;
     spk::OpStat stats[spk::kOpCount];
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> event_pool;
+    std::vector<spk::OpEvents> event_pool;
     size_t events_used = 0;
-    std::vector<std::pair<int, double>> pending_bytes;  // (op, bytes) aligned with events
     uint64_t generation = 0;                    // bumped by every insert; versions derive from it
     std::vector<spk::DevBuf<uint64_t>> pool;    // released run buffers, reused by later merges
     uint32_t merge_ratio = 4;                   // merge two runs when the older is < ratio x the newer
@@ -517,8 +533,10 @@ struct sparkalog_hip_store {
     ~sparkalog_hip_store() {
         if (pinned) (void)hipHostFree(pinned);
         for (auto& e : event_pool) {
-            (void)hipEventDestroy(e.first);
-            (void)hipEventDestroy(e.second);
+            (void)hipEventDestroy(e.begin);
+            (void)hipEventDestroy(e.end);
+            (void)hipEventDestroy(e.kernel.start);
+            (void)hipEventDestroy(e.kernel.stop);
         }
     }
 
@@ -543,36 +561,55 @@ struct sparkalog_hip_store {
     int begin_op() {
         if (!profiling) return -1;
         if (events_used == event_pool.size()) {
-            hipEvent_t a, b;
-            spk::check(hipEventCreate(&a));
-            spk::check(hipEventCreate(&b));
-            event_pool.emplace_back(a, b);
+            spk::OpEvents e;
+            spk::check(hipEventCreate(&e.begin));
+            spk::check(hipEventCreate(&e.end));
+            spk::check(hipEventCreate(&e.kernel.start));
+            spk::check(hipEventCreate(&e.kernel.stop));
+            event_pool.push_back(e);
         }
-        spk::check(hipEventRecord(event_pool[events_used].first, stream));
+        spk::OpEvents& e = event_pool[events_used];
+        e.kernel_used = false;
+        e.op = -1;
+        e.bytes = 0.0;
+        spk::check(hipEventRecord(e.begin, stream));
         return static_cast<int>(events_used++);
+    }
+    // Events around the operator's main kernel (null when not profiling).
+    const spk::KernelTimer* kernel_timer(int slot) {
+        if (slot < 0) return nullptr;
+        event_pool[slot].kernel_used = true;
+        return &event_pool[slot].kernel;
     }
     void end_op(int slot, int op, double bytes) {
         if (slot < 0) return;
-        spk::check(hipEventRecord(event_pool[slot].second, stream));
-        pending_bytes.resize(events_used);
-        pending_bytes[slot] = {op, bytes};
+        spk::check(hipEventRecord(event_pool[slot].end, stream));
+        event_pool[slot].op = op;
+        event_pool[slot].bytes = bytes;
     }
     // Bytes that depend on device-written counts, added after the sync.
     void add_op_bytes(int slot, double bytes) {
-        if (slot >= 0 && static_cast<size_t>(slot) < pending_bytes.size()) pending_bytes[slot].second += bytes;
+        if (slot >= 0 && static_cast<size_t>(slot) < events_used) event_pool[slot].bytes += bytes;
     }
     // Call after a stream synchronize.
     void collect_ops() {
         for (size_t i = 0; i < events_used; ++i) {
+            const spk::OpEvents& e = event_pool[i];
+            if (e.op < 0) continue;
             float ms = 0.f;
-            spk::check(hipEventElapsedTime(&ms, event_pool[i].first, event_pool[i].second));
-            auto& s = stats[pending_bytes[i].first];
-            s.launches += 1;
-            s.ms += ms;
-            s.bytes += pending_bytes[i].second;
+            spk::check(hipEventElapsedTime(&ms, e.begin, e.end));
+            auto& st = stats[e.op];
+            st.launches += 1;
+            st.ms += ms;
+            st.bytes += e.bytes;
+            if (e.kernel_used) {
+                float kms = 0.f;
+                spk::check(hipEventElapsedTime(&kms, e.kernel.start, e.kernel.stop));
+                st.kernel_launches += 1;
+                st.kernel_ms += kms;
+            }
         }
         events_used = 0;
-        pending_bytes.clear();
     }
 };
 
@@ -735,8 +772,9 @@ double settle_runs(sparkalog_hip_store* s, RelationState& ts, bool all) {
             PairInput b{last.keys.ptr, nullptr, nullptr, nullptr};
             PairOutput o{out.ptr, nullptr, nullptr};
             const int ev = s->begin_op();
-            check(sorted_union(a, prev.rows, b, last.rows, o, &cdev->merge_rows, temp, tb, s->stream));
-            s->end_op(ev, kOpUnion, 16.0 * n);
+            check(sorted_union(a, prev.rows, b, last.rows, o, &cdev->merge_rows, temp, tb, s->stream,
+                               s->kernel_timer(ev)));
+            s->end_op(ev, kOpMergeRuns, 16.0 * n);
             s->give(std::move(prev.keys));
             prev.keys = std::move(out);
             prev.rows = n;
@@ -989,7 +1027,10 @@ extern "C" int sparkalog_hip_store_create(sparkalog_hip_store** out, void* strea
     SPK_GUARD_BEGIN
     auto* s = new sparkalog_hip_store();
     s->stream = static_cast<hipStream_t>(stream);
-    for (int i = 0; i < kOpCount; ++i) s->stats[i].name = kOpNames[i];
+    for (int i = 0; i < kOpCount; ++i) {
+        s->stats[i].name = kOpNames[i];
+        s->stats[i].kernel = kOpKernels[i];
+    }
     *out = s;
     return SPARKALOG_HIP_OK;
     SPK_GUARD_END
@@ -1109,6 +1150,8 @@ extern "C" int sparkalog_hip_store_set_profiling(sparkalog_hip_store* s, int ena
         st.launches = 0;
         st.ms = 0;
         st.bytes = 0;
+        st.kernel_launches = 0;
+        st.kernel_ms = 0;
     }
     return SPARKALOG_HIP_OK;
 }
@@ -1119,6 +1162,17 @@ extern "C" int sparkalog_hip_store_op_stats(sparkalog_hip_store* s, int op, uint
     if (launches) *launches = s->stats[op].launches;
     if (ms) *ms = s->stats[op].ms;
     if (bytes) *bytes = s->stats[op].bytes;
+    return SPARKALOG_HIP_OK;
+}
+
+extern "C" int sparkalog_hip_store_op_kernel_stats(sparkalog_hip_store* s, int op, const char** name,
+                                                   const char** kernel, uint64_t* kernel_launches,
+                                                   double* kernel_ms) {
+    if (op < 0 || op >= kOpCount) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    if (name) *name = kOpNames[op];
+    if (kernel) *kernel = kOpKernels[op];
+    if (kernel_launches) *kernel_launches = s ? s->stats[op].kernel_launches : 0;
+    if (kernel_ms) *kernel_ms = s ? s->stats[op].kernel_ms : 0.0;
     return SPARKALOG_HIP_OK;
 }
 
@@ -1184,7 +1238,6 @@ int session_begin(sparkalog_hip_store* s, const uint32_t* relations, uint32_t n_
     ss.iterations = ss.total_new = 0;
     ss.algo_bytes = 0.0;
     s->events_used = 0;
-    s->pending_bytes.clear();
     if (!ss.t_begin) {
         check(hipEventCreate(&ss.t_begin));
         check(hipEventCreate(&ss.t_end));
@@ -1254,11 +1307,13 @@ void session_step(sparkalog_hip_store* s) {
         rt.status.reserve(n_tiles);
         check(fill_u64(rt.status.ptr, n_tiles, 0, s->stream));
         const int ev = s->begin_op();
+        timer_start(s->kernel_timer(ev), s->stream);
         dense_join_count_kernel<<<static_cast<unsigned>(n_tiles), kBlock, 0, s->stream>>>(
             ds.delta.keys.ptr, L, rt.plan.left_key, rt.index.starts.ptr, rt.index.n_keys, rt.offsets.ptr,
             &cdev->join_total[r], rt.status.ptr, &lb_counters[r], lb_error);
+        timer_stop(s->kernel_timer(ev), s->stream);
         check(hipGetLastError());
-        s->end_op(ev, kOpJoin, 8.0 * L);
+        s->end_op(ev, kOpJoinCount, 8.0 * L);
     }
     check(hipMemcpyAsync(&chost, cdev, sizeof(Counters), hipMemcpyDeviceToHost, s->stream));
     check(hipStreamSynchronize(s->stream));
@@ -1292,11 +1347,13 @@ void session_step(sparkalog_hip_store* s) {
         rt.row_start.reserve(emit_tiles + 1);
         emit_partition_kernel<<<grid_for(emit_tiles + 1), kBlock, 0, s->stream>>>(rt.offsets.ptr, rt.delta_rows, M,
                                                                                rt.row_start.ptr, emit_tiles);
+        timer_start(s->kernel_timer(ev), s->stream);
         dense_join_emit_kernel<<<emit_grid, kBlock, 0, s->stream>>>(
             ds.delta.keys.ptr, rt.delta_rows, rt.index.starts.ptr, rt.index.n_keys, rt.index.sorted.ptr,
             rt.offsets.ptr, M, rt.row_start.ptr, spec, rt.cand_a.ptr);
+        timer_stop(s->kernel_timer(ev), s->stream);
         check(hipGetLastError());
-        s->end_op(ev, kOpJoin, 8.0 * rt.index.rows + 8.0 * M);
+        s->end_op(ev, kOpJoinEmit, 8.0 * rt.index.rows + 8.0 * M);
         ev = s->begin_op();
         rt.distinct_slot = ev;
         // The join writes candidates in DELTA order; when the distinct's major
@@ -1318,7 +1375,8 @@ void session_step(sparkalog_hip_store* s) {
             PairOutput out{rt.distinct_out.ptr, nullptr, nullptr};
             check(segmented_distinct_finish(rt.cand_a.ptr, M, rt.shift, known_compressed, rt.cand_b.ptr,
                                             side_rows ? rt.cand_c.ptr : nullptr, side_rows, out,
-                                            &cdev->distinct_rows[r], rt.seg_meta.ptr, mb, temp, tb, s->stream));
+                                            &cdev->distinct_rows[r], rt.seg_meta.ptr, mb, temp, tb, s->stream,
+                                            s->kernel_timer(ev)));
         } else {
             const size_t tb = distinct_temp_bytes(M);
             void* temp = s->scratch(tb);
@@ -1331,10 +1389,11 @@ void session_step(sparkalog_hip_store* s) {
                                                                          &cdev->distinct_rows[r], M, rt.shift);
         }
         check(hipGetLastError());
-        s->end_op(ev, kOpDistinct, 16.0 * M);  // + 8U added after the sync
+        s->end_op(ev, kOpDistinct, 8.0 * M);  // + 8U added after the sync
     }
     // ---- TargetRuntime::combine (generic.rs:299-358) ----
     for (auto& t : targets) {
+        t.union_slots.clear();
         RelationState& ts = *find(s, t.target);
         const uint64_t* contrib = nullptr;
         const uint64_t* contrib_rows = nullptr;
@@ -1355,9 +1414,10 @@ void session_step(sparkalog_hip_store* s) {
             PairInput b{rt.distinct_out.ptr, nullptr, nullptr, &cdev->distinct_rows[t.rules[k]]};
             PairOutput o{t.union_tmp.ptr, nullptr, nullptr};
             const int ev = s->begin_op();
+            t.union_slots.push_back(ev);
             check(sorted_union(a, bound, b, rt.join_rows, o, &cdev->contrib_rows[t.rules[k]], temp, tb,
-                               s->stream));
-            s->end_op(ev, kOpUnion, 0.0);
+                               s->stream, s->kernel_timer(ev)));
+            s->end_op(ev, kOpUnion, 0.0);  // bytes added after the sync
             std::swap(t.union_tmp, t.contributions);
             contrib = t.contributions.ptr;
             contrib_rows = &cdev->contrib_rows[t.rules[k]];
@@ -1395,7 +1455,8 @@ void session_step(sparkalog_hip_store* s) {
             PairInput a{contrib, nullptr, nullptr, contrib_rows};
             PairOutput o{t.newt.ptr, nullptr, nullptr, t.new_run.ptr};
             const int ev = s->begin_op();
-            check(sorted_anti_join_multi(a, bound, rights, o, nullptr, &cdev->newt_rows[ti], temp, tb, s->stream));
+            check(sorted_anti_join_multi(a, bound, rights, o, nullptr, &cdev->newt_rows[ti], temp, tb, s->stream,
+                                         s->kernel_timer(ev)));
             s->end_op(ev, kOpAntiJoin, 8.0 * right_rows);
             t.anti_slot = ev;
             algo_bytes += 8.0 * right_rows;
@@ -1418,6 +1479,16 @@ void session_step(sparkalog_hip_store* s) {
         TargetRuntime& t = targets[i];
         const uint64_t C = t.rules.size() > 1 ? chost.contrib_rows[t.rules.back()]
                                               : (t.contrib_bound ? chost.distinct_rows[t.rules[0]] : 0);
+        // unions of the rules' contributions: 8 bytes per input and output row
+        for (size_t k = 1; k < t.rules.size(); ++k) {
+            const uint32_t rk = t.rules[k];
+            const double a = static_cast<double>(k == 1 ? (rts[t.rules[0]].join_rows ? chost.distinct_rows[t.rules[0]] : 0)
+                                                        : chost.contrib_rows[t.rules[k - 1]]);
+            const double b = static_cast<double>(rts[rk].join_rows ? chost.distinct_rows[rk] : 0);
+            const double ub = 8.0 * (a + b + static_cast<double>(chost.contrib_rows[rk]));
+            if (k - 1 < t.union_slots.size()) s->add_op_bytes(t.union_slots[k - 1], ub);
+            algo_bytes += ub;
+        }
         // the anti-join reads the contributions and writes NEWT twice (DELTA
         // and FULL's new run)
         if (t.anti_slot >= 0 || t.contrib_bound) {

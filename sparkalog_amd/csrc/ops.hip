@@ -988,6 +988,25 @@ __global__ void fill_u64_kernel(uint64_t* out, size_t len, uint64_t value) {
     for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < len; i += stride) out[i] = value;
 }
 
+// Plain device copy: four 16-byte loads in flight per lane, then the stores.
+__global__ __launch_bounds__(kBlock) void copy_u64_kernel(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst,
+                                                          size_t count) {
+    const uint4* s = reinterpret_cast<const uint4*>(src);
+    uint4* d = reinterpret_cast<uint4*>(dst);
+    const size_t n16 = count / 2;
+    const size_t lanes = static_cast<size_t>(gridDim.x) * kBlock;
+    size_t i = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x;
+    for (; i + 3 * lanes < n16; i += 4 * lanes) {
+        const uint4 a = s[i], b = s[i + lanes], c = s[i + 2 * lanes], e = s[i + 3 * lanes];
+        d[i] = a;
+        d[i + lanes] = b;
+        d[i + 2 * lanes] = c;
+        d[i + 3 * lanes] = e;
+    }
+    for (; i < n16; i += lanes) d[i] = s[i];
+    if ((count & 1) && blockIdx.x == 0 && threadIdx.x == 0) dst[count - 1] = src[count - 1];
+}
+
 unsigned grid_for(size_t n, size_t per_block, unsigned cap) {
     const size_t b = tiles_of(n, per_block);
     return static_cast<unsigned>(b == 0 ? 1 : (b < cap ? b : cap));
@@ -1069,7 +1088,8 @@ size_t union_temp_bytes(size_t left_rows, size_t right_rows) {
 }
 
 hipError_t sorted_union(PairInput left, size_t left_rows, PairInput right, size_t right_rows, PairOutput out,
-                        uint64_t* unique_rows, void* temp, size_t temp_bytes, hipStream_t stream) {
+                        uint64_t* unique_rows, void* temp, size_t temp_bytes, hipStream_t stream,
+                        const KernelTimer* timer) {
     const size_t total = left_rows + right_rows;
     if (total == 0) {
         set_u64_kernel<<<1, 1, 0, stream>>>(unique_rows, 0);
@@ -1082,9 +1102,11 @@ hipError_t sorted_union(PairInput left, size_t left_rows, PairInput right, size_
     size_t* part = reinterpret_cast<size_t*>(static_cast<char*>(temp) + compact_temp(tiles));
     merge_partition_kernel<<<grid_for(tiles + 1, kBlock, 4096), kBlock, 0, stream>>>(left, left_rows, right,
                                                                                     right_rows, part, tiles + 1);
+    timer_start(timer, stream);
     union_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(
         left, left_rows, right, right_rows, out, unique_rows, part,
         reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp));
+    timer_stop(timer, stream);
     return hipGetLastError();
 }
 
@@ -1095,7 +1117,7 @@ size_t anti_join_temp_bytes(size_t left_rows, uint32_t right_inputs) {
 
 hipError_t sorted_anti_join_multi(PairInput left, size_t left_rows, const AntiRight& rights, PairOutput out,
                                   uint32_t* count32, uint64_t* count64, void* temp, size_t temp_bytes,
-                                  hipStream_t stream) {
+                                  hipStream_t stream, const KernelTimer* timer) {
     if (left_rows == 0) {
         if (count32) set_u32_kernel<<<1, 1, 0, stream>>>(count32, 0);
         if (count64) set_u64_kernel<<<1, 1, 0, stream>>>(count64, 0);
@@ -1111,9 +1133,11 @@ hipError_t sorted_anti_join_multi(PairInput left, size_t left_rows, const AntiRi
         anti_partition_kernel<<<grid_for(tiles * rights.count, kBlock, 4096), kBlock, 0, stream>>>(
             left, left_rows, rights, tiles, ranges);
     }
+    timer_start(timer, stream);
     anti_join_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(
         left, left_rows, rights, ranges, out, count32, count64,
         reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp));
+    timer_stop(timer, stream);
     return hipGetLastError();
 }
 
@@ -1190,6 +1214,12 @@ hipError_t fill_mod_u32(uint32_t* out, size_t len, uint32_t modulus, hipStream_t
 hipError_t fill_u64(uint64_t* out, size_t len, uint64_t value, hipStream_t stream) {
     if (len == 0) return hipSuccess;
     fill_u64_kernel<<<grid_for(len, kBlock * 4, 8192), kBlock, 0, stream>>>(out, len, value);
+    return hipGetLastError();
+}
+
+hipError_t copy_u64(const uint64_t* src, uint64_t* dst, size_t count, hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    copy_u64_kernel<<<grid_for(count / 2 + 1, kBlock * 4, 16384), kBlock, 0, stream>>>(src, dst, count);
     return hipGetLastError();
 }
 

@@ -22,6 +22,21 @@ struct PairInput {
     const uint64_t* rows_dev;
 };
 
+// Optional events recorded on the launch stream immediately before and after
+// an operator's main kernel (the one that moves the operator's rows), so a
+// caller can time that kernel alone; helper kernels fall outside.
+struct KernelTimer {
+    hipEvent_t start = nullptr;
+    hipEvent_t stop = nullptr;
+};
+
+inline void timer_start(const KernelTimer* t, hipStream_t stream) {
+    if (t && t->start) (void)hipEventRecord(t->start, stream);
+}
+inline void timer_stop(const KernelTimer* t, hipStream_t stream) {
+    if (t && t->stop) (void)hipEventRecord(t->stop, stream);
+}
+
 struct PairOutput {
     uint64_t* packed;   // written when non-null
     uint32_t* first;    // written when non-null
@@ -73,13 +88,13 @@ hipError_t segmented_distinct_prepare(const uint64_t* keys, size_t rows, uint32_
 hipError_t segmented_distinct_finish(const uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t side_varying,
                                      uint64_t* side_a, uint64_t* side_b, uint64_t side_rows, PairOutput out,
                                      uint64_t* unique_rows, void* meta, size_t meta_bytes, void* sort_temp,
-                                     size_t sort_bytes, hipStream_t stream);
+                                     size_t sort_bytes, hipStream_t stream, const KernelTimer* timer = nullptr);
 
 // ---- sorted set operators ----
 size_t union_temp_bytes(size_t left_rows, size_t right_rows);
 hipError_t sorted_union(PairInput left, size_t left_rows, PairInput right, size_t right_rows,
                         PairOutput out, uint64_t* unique_rows, void* temp, size_t temp_bytes,
-                        hipStream_t stream);
+                        hipStream_t stream, const KernelTimer* timer = nullptr);
 
 size_t anti_join_temp_bytes(size_t left_rows, uint32_t right_inputs = 1);
 // left \ right; `count32`/`count64` receive the output row count (either may
@@ -90,7 +105,7 @@ hipError_t sorted_anti_join(PairInput left, size_t left_rows, PairInput right, s
 // left \ (rights[0] ∪ rights[1] ∪ ...) in one pass over the left input.
 hipError_t sorted_anti_join_multi(PairInput left, size_t left_rows, const AntiRight& rights, PairOutput out,
                                   uint32_t* count32, uint64_t* count64, void* temp, size_t temp_bytes,
-                                  hipStream_t stream);
+                                  hipStream_t stream, const KernelTimer* timer = nullptr);
 
 // ---- indexed join (U32RangeIndex layout) ----
 size_t join_temp_bytes(size_t left_rows);
@@ -114,6 +129,8 @@ hipError_t filter(const uint32_t* input, size_t rows, uint32_t op, uint32_t oper
 hipError_t add_one_i32(int32_t* data, size_t len, hipStream_t stream);
 hipError_t fill_mod_u32(uint32_t* out, size_t len, uint32_t modulus, hipStream_t stream);
 hipError_t fill_u64(uint64_t* out, size_t len, uint64_t value, hipStream_t stream);
+// dst[0..count) = src[0..count); both 16-byte aligned.
+hipError_t copy_u64(const uint64_t* src, uint64_t* dst, size_t count, hipStream_t stream);
 hipError_t set_u32(uint32_t* out, uint32_t value, hipStream_t stream);
 hipError_t set_u64(uint64_t* out, uint64_t value, hipStream_t stream);
 

@@ -358,7 +358,7 @@ hipError_t segmented_distinct_prepare(const uint64_t* keys, size_t rows, uint32_
 hipError_t segmented_distinct_finish(const uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t side_varying,
                                      uint64_t* side_a, uint64_t* side_b, uint64_t side_rows, PairOutput out,
                                      uint64_t* unique_rows, void* meta, size_t meta_bytes, void* sort_temp,
-                                     size_t sort_bytes, hipStream_t stream) {
+                                     size_t sort_bytes, hipStream_t stream, const KernelTimer* timer) {
     if (rows == 0) return set_u64(unique_rows, 0, stream);
     const SegLayout l = seg_layout(rows);
     if (meta_bytes < l.total) return hipErrorInvalidValue;
@@ -379,8 +379,10 @@ hipError_t segmented_distinct_finish(const uint64_t* keys, size_t rows, uint32_t
                            sort_temp, sort_bytes, stream);
         if (st != hipSuccess) return st;
     }
+    timer_start(timer, stream);
     seg_tile_kernel<<<static_cast<unsigned>(l.tiles), kBlock, 0, stream>>>(keys, rows, minor_bits, bounds,
                                                                            side_unique_out, out, unique_rows, status, h);
+    timer_stop(timer, stream);
     return hipGetLastError();
 }
 

@@ -126,6 +126,14 @@ def add_one_i32(column: torch.Tensor, stream: HipStream) -> None:
     stream.synchronize()
 
 
+def copy_u64(src: torch.Tensor, dst: torch.Tensor, stream: HipStream) -> None:
+    """Plain device copy of src's bytes into dst (asynchronous on `stream`)."""
+    if src.nbytes % 8 or dst.nbytes < src.nbytes:
+        raise ValueError("copy_u64: source must be whole u64 words and fit the destination")
+    _check("sparkalog_hip_copy_u64", _native.hip().sparkalog_hip_copy_u64(_ptr(src), _ptr(dst), src.nbytes // 8,
+                                                                          stream.raw))
+
+
 def fill_mod_u32(column: torch.Tensor, modulus: int, stream: HipStream) -> None:
     """GPU producer of the crossover harnesses (lib.rs:1929-1948)."""
     _check("sparkalog_hip_fill_mod_u32",

@@ -24,7 +24,6 @@ STATUS_NAMES = {
     9: "EmptyScc", 10: "DeltaOutsideScc", 11: "TargetOutsideScc", 12: "MissingRuleTarget",
     13: "OutOfMemory", 14: "UnsupportedPlan", 15: "DeviceError", 16: "InvalidArgument",
 }
-OPS = ("join", "distinct", "anti_join", "union", "index")
 
 
 class GenericExecutionError(Exception):
@@ -181,14 +180,23 @@ class RelationStore:
         _check(self._lib.sparkalog_hip_store_set_profiling(self._h, int(enabled)))
 
     def op_stats(self) -> dict[str, dict]:
+        """Per operator: launches, device ms (helper kernels included),
+        algorithmic bytes, and the main kernel's name, launches and ms."""
         out = {}
-        for i, name in enumerate(OPS):
-            launches = ctypes.c_uint64(0)
-            ms = ctypes.c_double(0)
-            nbytes = ctypes.c_double(0)
+        i = 0
+        while True:
+            name, kernel = ctypes.c_char_p(), ctypes.c_char_p()
+            k_launches, k_ms = ctypes.c_uint64(0), ctypes.c_double(0)
+            if self._lib.sparkalog_hip_store_op_kernel_stats(self._h, i, ctypes.byref(name), ctypes.byref(kernel),
+                                                             ctypes.byref(k_launches), ctypes.byref(k_ms)) != OK:
+                break
+            launches, ms, nbytes = ctypes.c_uint64(0), ctypes.c_double(0), ctypes.c_double(0)
             _check(self._lib.sparkalog_hip_store_op_stats(self._h, i, ctypes.byref(launches), ctypes.byref(ms),
                                                           ctypes.byref(nbytes)))
-            out[name] = {"launches": launches.value, "ms": ms.value, "bytes": nbytes.value}
+            out[name.value.decode()] = {"launches": launches.value, "ms": ms.value, "bytes": nbytes.value,
+                                        "kernel": kernel.value.decode() or None, "kernel_launches": k_launches.value,
+                                        "kernel_ms": k_ms.value}
+            i += 1
         return out
 
 

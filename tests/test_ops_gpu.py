@@ -63,6 +63,16 @@ def test_fill_and_add_one(stream):
     assert list(col.cpu().numpy()) == [i % 17 + 1 for i in range(257)]
 
 
+@pytest.mark.parametrize("words", [1, 2, 7, 4096 * 4 + 3, 1_000_001])
+def test_copy_u64(words, stream):
+    src = torch.randint(-2**31, 2**31 - 1, (words * 2,), dtype=torch.int32, device="cuda")
+    dst = torch.zeros(words * 2 + 2, dtype=torch.int32, device="cuda")
+    ex.copy_u64(src, dst, stream)
+    stream.synchronize()
+    assert torch.equal(dst[:words * 2], src)
+    assert int(dst[words * 2:].abs().sum()) == 0  # nothing past the copy
+
+
 # ---- seeded parity -------------------------------------------------------------
 
 DISTINCT_CASES = [
