@@ -36,6 +36,9 @@ def parse_args():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--nodes", type=int, default=1 << 20)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--merge-ratio", type=int, default=4, help="tc: FULL run-merge ratio (results are identical)")
+    p.add_argument("--indexed-anti-join", action="store_true",
+                   help="tc: anti-join FULL through its first-column index (results are identical)")
     p.add_argument("--cpu-sample-nodes", type=int, default=1 << 16)
     p.add_argument("--mode", choices=["tc", "ops", "batched", "served", "copy"], default="tc",
                    help="tc: the headline fixpoint; ops: operator microbench at 2^k rows; "
@@ -222,6 +225,7 @@ def main():
     d_dst = torch.from_numpy(dst.view(np.int32)).cuda()
     stream = HipStream()
     store = RelationStore(stream)
+    store.set_tuning(merge_ratio=args.merge_ratio, indexed_anti_join=args.indexed_anti_join)
     R = plans.RelationId
     plan = plans.transitive_closure_scc(R(1), R(0))
 

@@ -234,6 +234,17 @@ int sparkalog_hip_executor_run(sparkalog_hip_store* store, const uint32_t* relat
  * runs, 6 index build. Reports launches, summed device milliseconds and
  * summed algorithmic bytes (8 bytes per pair read or written). */
 int sparkalog_hip_store_set_profiling(sparkalog_hip_store* store, int enabled);
+/* Executor tuning (no counterpart in the reference, whose union rewrites all
+ * of FULL each iteration): FULL is kept as size-tiered sorted runs, and the
+ * newest run is merged into the one before while that one holds fewer than
+ * merge_ratio (>= 2, default 4) times its rows; segmented_distinct (default
+ * 1) sorts candidates grouped by their first column per group in LDS instead
+ * of a global radix sort; indexed_anti_join (default 0) gives FULL's runs a
+ * first-column index (while first values are below 2^24) so the anti-join
+ * reads only the rows of first values it holds. Results do not depend on
+ * any of them. */
+int sparkalog_hip_store_set_tuning(sparkalog_hip_store* store, uint32_t merge_ratio, int segmented_distinct,
+                                   int indexed_anti_join);
 int sparkalog_hip_store_op_stats(sparkalog_hip_store* store, int op, uint64_t* launches, double* ms,
                                  double* bytes);
 /* The operator's name and main kernel, and that kernel's own launches and

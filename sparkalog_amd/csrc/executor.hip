@@ -99,6 +99,19 @@ struct DevBuf {
 struct DevRelation {
     DevBuf<uint64_t> keys;
     uint64_t rows = 0;
+    // first-column index (xstart, n_x + 1 entries) of a FULL part, valid
+    // while xidx_n is the current n_x and keys/rows are those it was built on
+    DevBuf<uint32_t> xidx;
+    uint32_t xidx_n = 0;
+    const uint64_t* xidx_keys = nullptr;
+    uint64_t xidx_rows = 0;
+    void touched() { xidx_n = 0; }
+    bool xidx_valid(uint32_t n_x) const { return xidx_n == n_x && xidx_keys == keys.ptr && xidx_rows == rows; }
+    void xidx_set(uint32_t n_x) {
+        xidx_n = n_x;
+        xidx_keys = keys.ptr;
+        xidx_rows = rows;
+    }
 };
 
 struct RelationState {
@@ -526,6 +539,23 @@ struct sparkalog_hip_store {
     void give(spk::DevBuf<uint64_t>&& b) {
         if (b.ptr) pool.push_back(std::move(b));
     }
+    // first-column index buffers (all n_x + 1 entries within a fixpoint)
+    std::vector<spk::DevBuf<uint32_t>> xpool;
+    spk::DevBuf<uint32_t> take_x(size_t n) {
+        for (size_t i = 0; i < xpool.size(); ++i)
+            if (xpool[i].cap >= n) {
+                spk::DevBuf<uint32_t> b = std::move(xpool[i]);
+                xpool.erase(xpool.begin() + static_cast<long>(i));
+                return b;
+            }
+        spk::DevBuf<uint32_t> b;
+        b.reserve(n);
+        return b;
+    }
+    void give_x(spk::DevBuf<uint32_t>&& b) {
+        if (b.ptr) xpool.push_back(std::move(b));
+    }
+    bool x_anti = false;  // anti-join FULL through its first-column index when it fits
     // operator workspaces kept across runs so repeated runs allocate nothing
     std::vector<spk::TargetRuntime> ws_targets;
     std::vector<spk::RuleRuntime> ws_rules;
@@ -636,6 +666,7 @@ uint32_t upload(sparkalog_hip_store* s, DevRelation& dst, const uint32_t* c0, co
     if (rows > 0xFFFFFFFFull) throw StatusError{SPARKALOG_HIP_ERR_TOO_MANY_ROWS};
     dst.keys.reserve(rows ? rows : 1);
     dst.rows = rows;
+    dst.touched();
     if (rows == 0) return 0;
     uint32_t max_value = 0;
     if (on_device) {
@@ -751,6 +782,17 @@ hipError_t copy_keys(const uint64_t* in, const uint64_t* rows_dev, size_t bound,
 // times the newer (or always, when `all`), leaving FULL as one array when
 // `all` is set. Inputs are disjoint sorted unique arrays, so a merge's output
 // size is the sum of its inputs' sizes. Returns the algorithmic bytes moved.
+// Build a FULL part's first-column index unless it is current.
+void ensure_xidx(sparkalog_hip_store* s, DevRelation& r, uint32_t n_x) {
+    if (r.xidx_valid(n_x)) return;
+    if (r.xidx.cap < static_cast<size_t>(n_x) + 1) {
+        s->give_x(std::move(r.xidx));
+        r.xidx = s->take_x(static_cast<size_t>(n_x) + 1);
+    }
+    check(xindex_build(r.keys.ptr, nullptr, r.rows, n_x, r.xidx.ptr, s->stream));
+    r.xidx_set(n_x);
+}
+
 double settle_runs(sparkalog_hip_store* s, RelationState& ts, bool all) {
     double bytes = 0.0;
     Counters* cdev = reinterpret_cast<Counters*>(s->counters_dev.ptr);
@@ -762,9 +804,18 @@ double settle_runs(sparkalog_hip_store* s, RelationState& ts, bool all) {
         if (!merge) break;
         const uint64_t n = prev.rows + last.rows;
         if (prev.rows == 0) {
+            const uint32_t last_nx = last.xidx_n && last.xidx_valid(last.xidx_n) ? last.xidx_n : 0;
             std::swap(prev.keys, last.keys);
+            std::swap(prev.xidx, last.xidx);
             prev.rows = last.rows;
+            if (last_nx)
+                prev.xidx_set(last_nx);
+            else
+                prev.touched();
         } else if (last.rows) {
+            // the merged run's index is the sum of the two indexes
+            const uint32_t nx = prev.xidx_n;
+            const bool add = nx && prev.xidx_valid(nx) && last.xidx_valid(nx);
             DevBuf<uint64_t> out = s->take(n);
             const size_t tb = union_temp_bytes(prev.rows, last.rows);
             void* temp = s->scratch(tb);
@@ -775,12 +826,18 @@ double settle_runs(sparkalog_hip_store* s, RelationState& ts, bool all) {
             check(sorted_union(a, prev.rows, b, last.rows, o, &cdev->merge_rows, temp, tb, s->stream,
                                s->kernel_timer(ev)));
             s->end_op(ev, kOpMergeRuns, 16.0 * n);
+            if (add) check(xindex_add(prev.xidx.ptr, last.xidx.ptr, nx, s->stream));
             s->give(std::move(prev.keys));
             prev.keys = std::move(out);
             prev.rows = n;
+            if (add)
+                prev.xidx_set(nx);
+            else
+                prev.touched();
             bytes += 16.0 * n;
         }
         s->give(std::move(last.keys));
+        s->give_x(std::move(last.xidx));
         ts.runs.pop_back();
     }
     return bytes;
@@ -895,6 +952,7 @@ void remove_rows(sparkalog_hip_store* s, DevRelation& r, uint64_t lo, uint64_t h
     s->give(std::move(r.keys));
     r.keys = std::move(out);
     r.rows = n;
+    r.touched();
 }
 
 // dst = dst ∪ keys (both sorted); the result size comes back from the device.
@@ -916,6 +974,7 @@ void union_into(sparkalog_hip_store* s, DevRelation& dst, const uint64_t* keys, 
     s->give(std::move(dst.keys));
     dst.keys = std::move(out);
     dst.rows = n;
+    dst.touched();
 }
 
 // RecursiveExecutor::compile (generic.rs:407-470): validation and runtime
@@ -1084,7 +1143,11 @@ extern "C" int sparkalog_hip_store_clear(sparkalog_hip_store* s) {
         st.present = false;
         st.recursive = false;
         st.full.rows = st.delta.rows = st.newt.rows = 0;
-        for (auto& r : st.runs) s->give(std::move(r.keys));
+        st.full.touched();
+        for (auto& r : st.runs) {
+            s->give(std::move(r.keys));
+            s->give_x(std::move(r.xidx));
+        }
         st.runs.clear();
     }
     return SPARKALOG_HIP_OK;
@@ -1143,6 +1206,15 @@ extern "C" int sparkalog_hip_store_digest(sparkalog_hip_store* s, uint32_t id, u
     SPK_GUARD_END
 }
 
+extern "C" int sparkalog_hip_store_set_tuning(sparkalog_hip_store* s, uint32_t merge_ratio, int segmented_distinct,
+                                              int indexed_anti_join) {
+    if (!s || merge_ratio < 2) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    s->merge_ratio = merge_ratio;
+    s->segmented_distinct = segmented_distinct != 0;
+    s->x_anti = indexed_anti_join != 0;
+    return SPARKALOG_HIP_OK;
+}
+
 extern "C" int sparkalog_hip_store_set_profiling(sparkalog_hip_store* s, int enabled) {
     if (!s) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
     s->profiling = enabled != 0;
@@ -1170,7 +1242,7 @@ extern "C" int sparkalog_hip_store_op_kernel_stats(sparkalog_hip_store* s, int o
                                                    double* kernel_ms) {
     if (op < 0 || op >= kOpCount) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
     if (name) *name = kOpNames[op];
-    if (kernel) *kernel = kOpKernels[op];
+    if (kernel) *kernel = s && s->stats[op].kernel ? s->stats[op].kernel : kOpKernels[op];
     if (kernel_launches) *kernel_launches = s ? s->stats[op].kernel_launches : 0;
     if (kernel_ms) *kernel_ms = s ? s->stats[op].kernel_ms : 0.0;
     return SPARKALOG_HIP_OK;
@@ -1286,6 +1358,10 @@ void session_step(sparkalog_hip_store* s) {
     if (vbits == 0) vbits = 1;
     const uint32_t shift = vbits <= 24 ? vbits : 32;  // compressed sort key width per column
     const uint64_t known_compressed = shift == 32 ? ((vmask << 32) | vmask) : ((vmask << shift) | vmask);
+    // FULL's parts carry a first-column index (n_x + 1 u32 entries each)
+    // while first values stay below 2^24
+    const uint32_t n_x = vmax + 1;
+    const bool use_x = s->x_anti && vmax < (1u << 24);
 
     clear_words_kernel<<<1, 256, 0, s->stream>>>(s->misc_dev.ptr, 256);
     // ---- RuleRuntime::evaluate, join count (generic.rs:197-241) ----
@@ -1444,11 +1520,36 @@ void session_step(sparkalog_hip_store* s) {
         add_right(ts.full);
         for (const auto& r : ts.runs) add_right(r);
         t.anti_slot = -1;
+        bool parts_fit = true;  // index entries are u32 row counts
+        if (ts.full.rows > 0xFFFFFFFFull) parts_fit = false;
+        for (const auto& r : ts.runs) parts_fit = parts_fit && r.rows <= 0xFFFFFFFFull;
         if (bound == 0) {
             check(set_u64(&cdev->newt_rows[ti], 0, s->stream));
         } else if (rights.count == 0) {
             check(copy_keys(contrib, contrib_rows, bound, t.newt.ptr, t.new_run.ptr, &cdev->newt_rows[ti],
                             s->stream));
+        } else if (use_x && parts_fit) {
+            XRights xr{};
+            xr.count = 0;
+            auto add_part = [&](DevRelation& r) {
+                if (!r.rows) return;
+                ensure_xidx(s, r, n_x);
+                xr.keys[xr.count] = r.keys.ptr;
+                xr.xstart[xr.count] = r.xidx.ptr;
+                ++xr.count;
+            };
+            add_part(ts.full);
+            for (auto& r : ts.runs) add_part(r);
+            const size_t tb = anti_join_x_temp_bytes(bound);
+            void* temp = s->scratch(tb);
+            PairOutput o{t.newt.ptr, nullptr, nullptr, t.new_run.ptr};
+            const int ev = s->begin_op();
+            check(anti_join_x(contrib, contrib_rows, bound, xr, n_x, o, &cdev->newt_rows[ti], temp, tb, s->stream,
+                              s->kernel_timer(ev)));
+            s->stats[kOpAntiJoin].kernel = "anti_join_x_kernel";
+            s->end_op(ev, kOpAntiJoin, 8.0 * right_rows);
+            t.anti_slot = ev;
+            algo_bytes += 8.0 * right_rows;
         } else {
             const size_t tb = anti_join_temp_bytes(bound, rights.count);
             void* temp = s->scratch(tb);
@@ -1457,6 +1558,7 @@ void session_step(sparkalog_hip_store* s) {
             const int ev = s->begin_op();
             check(sorted_anti_join_multi(a, bound, rights, o, nullptr, &cdev->newt_rows[ti], temp, tb, s->stream,
                                          s->kernel_timer(ev)));
+            s->stats[kOpAntiJoin].kernel = "anti_join_kernel";
             s->end_op(ev, kOpAntiJoin, 8.0 * right_rows);
             t.anti_slot = ev;
             algo_bytes += 8.0 * right_rows;
@@ -1512,6 +1614,7 @@ void session_step(sparkalog_hip_store* s) {
             DevRelation run;
             run.keys = std::move(t.new_run);
             run.rows = N;
+            if (use_x && N <= 0xFFFFFFFFull) ensure_xidx(s, run, n_x);
             ts.runs.push_back(std::move(run));
         } else {
             s->give(std::move(t.new_run));

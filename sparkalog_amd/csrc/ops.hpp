@@ -107,6 +107,28 @@ hipError_t sorted_anti_join_multi(PairInput left, size_t left_rows, const AntiRi
                                   uint32_t* count32, uint64_t* count64, void* temp, size_t temp_bytes,
                                   hipStream_t stream, const KernelTimer* timer = nullptr);
 
+// ---- first-column index of sorted packed runs, and the anti-join using it ----
+// xstart (n_x + 1 u32 entries) of a sorted packed relation with fewer than
+// 2^32 rows whose first values are all below n_x: xstart[x] = rows with first
+// column < x. The row count is *rows_dev (<= bound) when rows_dev is non-null.
+hipError_t xindex_build(const uint64_t* keys, const uint64_t* rows_dev, size_t bound, uint32_t n_x, uint32_t* xstart,
+                        hipStream_t stream);
+// into[x] += other[x]: the index of the merge of two disjoint runs.
+hipError_t xindex_add(uint32_t* into, const uint32_t* other, uint32_t n_x, hipStream_t stream);
+
+struct XRights {
+    const uint64_t* keys[kMaxAntiRights];
+    const uint32_t* xstart[kMaxAntiRights];
+    uint32_t count;
+};
+// left \ (rights' union) for sorted unique packed left rows (count *left_rows
+// <= bound) whose first values are below n_x; writes out.packed and
+// out.packed_copy (out.first must be null) and the row count to *count64.
+size_t anti_join_x_temp_bytes(size_t bound);
+hipError_t anti_join_x(const uint64_t* left, const uint64_t* left_rows, size_t bound, const XRights& rights,
+                       uint32_t n_x, PairOutput out, uint64_t* count64, void* temp, size_t temp_bytes,
+                       hipStream_t stream, const KernelTimer* timer = nullptr);
+
 // ---- indexed join (U32RangeIndex layout) ----
 size_t join_temp_bytes(size_t left_rows);
 hipError_t join_count(const uint32_t* left_keys, size_t left_rows, const uint32_t* index_keys,

@@ -52,16 +52,17 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
-// Block-wide exclusive sum of one value per thread. `lds` holds kWavesPerBlock
-// entries. Returns the exclusive prefix and the block total in *total.
-template <typename T>
+// Block-wide exclusive sum of one value per thread over a block of kWaves
+// waves. `lds` holds kWaves entries. Returns the exclusive prefix and the
+// block total in *total.
+template <typename T, int kWaves = kWavesPerBlock>
 __device__ __forceinline__ T block_exclusive_sum(T v, T* total, T* lds) {
     const T inclusive = wave_inclusive_sum(v);
     if (lane_id() == kWave - 1) lds[wave_id()] = inclusive;
     __syncthreads();
     T wave_offset = 0, sum = 0;
 #pragma unroll
-    for (int w = 0; w < kWavesPerBlock; ++w) {
+    for (int w = 0; w < kWaves; ++w) {
         const T t = lds[w];
         if (w < static_cast<int>(wave_id())) wave_offset += t;
         sum += t;

@@ -22,7 +22,9 @@ namespace {
 
 constexpr int kSegTile = 2048;                // nominal tile
 constexpr int kSegCap = 4096;                 // largest tile sorted in LDS
-constexpr int kSegItems = kSegCap / kBlock;   // 16 keys per thread
+constexpr int kSegBlock = 512;                // threads of a tile block (8 waves)
+constexpr int kSegWaves = kSegBlock / kWave;
+constexpr int kSegItems = kSegCap / kSegBlock;  // 8 keys per thread
 constexpr int kSegWaveRows = kSegItems * kWave;
 constexpr int kSegRadix = 256;
 
@@ -114,16 +116,22 @@ __global__ __launch_bounds__(kBlock) void seg_gather_kernel(const uint64_t* keys
     }
 }
 
-__global__ __launch_bounds__(kBlock) void seg_tile_kernel(const uint64_t* keys, size_t rows, uint32_t minor_bits,
+// One tile (whole runs, at most kSegCap keys) per block. K is the in-LDS
+// sort key (local run rank << minor_bits | minor): u32 when rank and minor
+// fit in 32 bits (minor_bits <= 20), else u64.
+template <typename K>
+__global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(const uint64_t* keys, size_t rows, uint32_t minor_bits,
                                                           const size_t* bounds, const uint64_t* side_sorted,
                                                           PairOutput out, uint64_t* unique_rows, uint64_t* status,
                                                           SegHeader* h) {
-    __shared__ uint64_t tile_keys[kSegCap];
+    constexpr int kKeyBits = static_cast<int>(sizeof(K) * 8);
+    __shared__ K tile_keys[kSegCap];
     __shared__ uint32_t major_of[kSegCap];
-    __shared__ uint32_t wave_count[kWavesPerBlock][kSegRadix];
-    __shared__ uint32_t item_totals[kWavesPerBlock][kSegItems];
-    __shared__ uint32_t scan_lds[kWavesPerBlock];
-    __shared__ uint64_t red_or[kWavesPerBlock], red_and[kWavesPerBlock];
+    __shared__ uint32_t wave_count[kSegWaves][kSegRadix];
+    __shared__ uint32_t item_totals[kSegWaves][kSegItems];
+    __shared__ uint64_t keep_lds[kSegWaves][kSegItems];  // ballot masks of kept keys
+    __shared__ uint32_t scan_lds[kSegWaves];
+    __shared__ K red_or[kSegWaves], red_and[kSegWaves];
     __shared__ uint64_t offset_slot;
     __shared__ uint32_t tile_slot;
     const uint32_t tile = next_tile(&h->counter, &tile_slot);
@@ -134,65 +142,65 @@ __global__ __launch_bounds__(kBlock) void seg_tile_kernel(const uint64_t* keys, 
     const size_t size = b1 - b0;
     const uint64_t minor_mask = (1ull << minor_bits) - 1;
     const bool big = size > static_cast<size_t>(kSegCap);
-    // each wave owns a contiguous share of the tile: items = ceil(size / 256)
-    const int items = static_cast<int>((size + kBlock - 1) / kBlock);
-    const size_t wave_len = static_cast<size_t>(items) * kWave;
+    // each wave owns a contiguous share of the tile: items = ceil(size / 512)
+    const int items = static_cast<int>((size + kSegBlock - 1) / kSegBlock);
+    const uint32_t wave_len = static_cast<uint32_t>(items) * kWave;
+    const uint32_t n = static_cast<uint32_t>(size);
 
-    uint64_t k[kSegItems];
+    K k[kSegItems];
     uint32_t count = 0;        // unique keys of this tile
     size_t side0 = 0;          // big tile: its slice of the side output
-    uint64_t keep[kSegItems];  // small tile: ballot masks of kept keys per item
     if (!big) {
-        // load; rank runs: local rank = run starts at or before the position
-        uint64_t or_acc = 0, and_acc = ~0ull;
-#pragma unroll
-        for (int i = 0; i < kSegItems; ++i) {
-            if (i >= items) continue;
-            const size_t p = wave * wave_len + i * kWave + lane;
-            k[i] = p < size ? keys[b0 + p] : 0;
-        }
-        uint32_t lrank[kSegItems];   // run starts at or before the position, within this wave
+        // load; k = (run starts at or before the position within this wave)
+        // << minor_bits | minor; majors of run starts are staged in LDS (in
+        // tile_keys, free until the first pass) at their wave-local rank
+        uint32_t* staged = reinterpret_cast<uint32_t*>(tile_keys) + wave * (kSegCap / kSegWaves);
         uint32_t starts_before = 0;  // run starts in earlier items of this wave
+        uint32_t last_major = 0;     // major of lane 63 of the previous item
 #pragma unroll
         for (int i = 0; i < kSegItems; ++i) {
             if (i >= items) continue;
-            const size_t p = wave * wave_len + i * kWave + lane;
-            const bool valid = p < size;
-            const uint64_t major = k[i] >> minor_bits;
-            // predecessor's major: the lane below, or (lane 0) position p-1 in memory
-            const uint64_t up = __shfl_up(major, 1, kWave);
+            const uint32_t p = wave * wave_len + i * kWave + lane;
+            const bool valid = p < n;
+            const uint64_t raw = valid ? keys[b0 + p] : 0;
+            const uint32_t major = static_cast<uint32_t>(raw >> minor_bits);
+            const uint32_t up = __shfl_up(major, 1, kWave);
             bool start = false;
             if (valid) {
                 if (p == 0)
                     start = true;
                 else if (lane > 0)
                     start = up != major;
+                else if (i > 0)
+                    start = last_major != major;
                 else
-                    start = (keys[b0 + p - 1] >> minor_bits) != major;
+                    start = static_cast<uint32_t>(keys[b0 + p - 1] >> minor_bits) != major;
             }
+            last_major = __shfl(major, kWave - 1, kWave);
             const uint64_t m = ballot(start);
-            if (lane == 0) item_totals[wave][i] = static_cast<uint32_t>(__popcll(m));
-            lrank[i] = starts_before + lanes_below(m) + (start ? 1u : 0u);
+            const uint32_t local = starts_before + lanes_below(m) + (start ? 1u : 0u);
+            if (start) staged[local - 1] = major;
+            k[i] = (static_cast<K>(local) << minor_bits) | static_cast<K>(raw & minor_mask);
             starts_before += static_cast<uint32_t>(__popcll(m));
         }
+        if (lane == 0) item_totals[wave][0] = starts_before;
         __syncthreads();
         uint32_t wave_base = 0;
-        for (int w = 0; w < static_cast<int>(wave); ++w)
-            for (int i = 0; i < items; ++i) wave_base += item_totals[w][i];
-        // rank (0-based) and the major value of every run
+        for (int w = 0; w < static_cast<int>(wave); ++w) wave_base += item_totals[w][0];
+        for (uint32_t j = lane; j < starts_before; j += kWave) major_of[wave_base + j] = staged[j];
+        // rank = wave_base + local - 1 (unsigned wrap-around is exact)
+        const K rank_add = static_cast<K>(static_cast<K>(static_cast<K>(wave_base) - static_cast<K>(1)) << minor_bits);
+        K or_acc = 0, and_acc = static_cast<K>(~static_cast<K>(0));
 #pragma unroll
         for (int i = 0; i < kSegItems; ++i) {
             if (i >= items) continue;
-            const size_t p = wave * wave_len + i * kWave + lane;
-            if (p < size) {
-                const uint32_t rank = lrank[i] + wave_base - 1;
-                major_of[rank] = static_cast<uint32_t>(k[i] >> minor_bits);
-                k[i] = (static_cast<uint64_t>(rank) << minor_bits) | (k[i] & minor_mask);
+            const uint32_t p = wave * wave_len + i * kWave + lane;
+            if (p < n) {
+                k[i] = static_cast<K>(k[i] + rank_add);
                 or_acc |= k[i];
                 and_acc &= k[i];
             }
         }
-        __syncthreads();
         // varying digits of the tile's (rank, minor) keys
 #pragma unroll
         for (int off = kWave / 2; off > 0; off >>= 1) {
@@ -204,27 +212,27 @@ __global__ __launch_bounds__(kBlock) void seg_tile_kernel(const uint64_t* keys, 
             red_and[wave] = and_acc;
         }
         __syncthreads();
-        uint64_t vary = 0;
+        K vary = 0;
         {
-            uint64_t o = 0, a = ~0ull;
-            for (int w = 0; w < kWavesPerBlock; ++w) {
+            K o = 0, a = static_cast<K>(~static_cast<K>(0));
+            for (int w = 0; w < kSegWaves; ++w) {
                 o |= red_or[w];
                 a &= red_and[w];
             }
-            vary = size ? (o ^ a) : 0;
+            vary = n ? (o ^ a) : 0;
         }
         // stable LSD passes in LDS over the varying 8-bit digits
-        for (int shift = 0; shift < 64; shift += 8) {
+        for (int shift = 0; shift < kKeyBits; shift += 8) {
             if (!((vary >> shift) & 0xFF)) continue;  // block-uniform
-            for (int i = threadIdx.x; i < kWavesPerBlock * kSegRadix; i += kBlock) (&wave_count[0][0])[i] = 0;
+            for (int i = threadIdx.x; i < kSegWaves * kSegRadix; i += kSegBlock) (&wave_count[0][0])[i] = 0;
             __syncthreads();
             uint32_t ranks[kSegItems];
 #pragma unroll
             for (int i = 0; i < kSegItems; ++i) {
                 if (i >= items) continue;
-                const size_t p = wave * wave_len + i * kWave + lane;
-                const bool valid = p < size;
-                const uint32_t digit = static_cast<uint32_t>(k[i] >> shift) & 0xFF;
+                const uint32_t p = wave * wave_len + i * kWave + lane;
+                const bool valid = p < n;
+                const uint32_t digit = static_cast<uint32_t>(k[i] >> shift) & 0xFFu;
                 uint64_t peers = ballot(valid);
 #pragma unroll
                 for (int b = 0; b < 8; ++b) {
@@ -238,25 +246,30 @@ __global__ __launch_bounds__(kBlock) void seg_tile_kernel(const uint64_t* keys, 
                 if (valid && below == 0) wave_count[wave][digit] = before + static_cast<uint32_t>(__popcll(peers));
             }
             __syncthreads();
+            // digit d's start per wave: the first 256 threads own one digit each
             const uint32_t d = threadIdx.x;
             uint32_t tc = 0;
+            if (d < kSegRadix) {
 #pragma unroll
-            for (int w = 0; w < kWavesPerBlock; ++w) {
-                const uint32_t c = wave_count[w][d];
-                wave_count[w][d] = tc;
-                tc += c;
+                for (int w = 0; w < kSegWaves; ++w) {
+                    const uint32_t c = wave_count[w][d];
+                    wave_count[w][d] = tc;
+                    tc += c;
+                }
             }
             uint32_t total_unused;
-            const uint32_t local_start = block_exclusive_sum<uint32_t>(tc, &total_unused, scan_lds);
+            const uint32_t local_start = block_exclusive_sum<uint32_t, kSegWaves>(tc, &total_unused, scan_lds);
+            if (d < kSegRadix) {
 #pragma unroll
-            for (int w = 0; w < kWavesPerBlock; ++w) wave_count[w][d] += local_start;
+                for (int w = 0; w < kSegWaves; ++w) wave_count[w][d] += local_start;
+            }
             __syncthreads();
 #pragma unroll
             for (int i = 0; i < kSegItems; ++i) {
                 if (i >= items) continue;
-                const size_t p = wave * wave_len + i * kWave + lane;
-                if (p < size) {
-                    const uint32_t digit = static_cast<uint32_t>(k[i] >> shift) & 0xFF;
+                const uint32_t p = wave * wave_len + i * kWave + lane;
+                if (p < n) {
+                    const uint32_t digit = static_cast<uint32_t>(k[i] >> shift) & 0xFFu;
                     tile_keys[wave_count[wave][digit] + ranks[i]] = k[i];
                 }
             }
@@ -264,8 +277,8 @@ __global__ __launch_bounds__(kBlock) void seg_tile_kernel(const uint64_t* keys, 
 #pragma unroll
             for (int i = 0; i < kSegItems; ++i) {
                 if (i >= items) continue;
-                const size_t p = wave * wave_len + i * kWave + lane;
-                if (p < size) k[i] = tile_keys[p];
+                const uint32_t p = wave * wave_len + i * kWave + lane;
+                if (p < n) k[i] = tile_keys[p];
             }
             __syncthreads();
         }
@@ -273,21 +286,24 @@ __global__ __launch_bounds__(kBlock) void seg_tile_kernel(const uint64_t* keys, 
 #pragma unroll
         for (int i = 0; i < kSegItems; ++i) {
             if (i >= items) continue;
-            const size_t p = wave * wave_len + i * kWave + lane;
-            if (p < size) tile_keys[p] = k[i];
+            const uint32_t p = wave * wave_len + i * kWave + lane;
+            if (p < n) tile_keys[p] = k[i];
         }
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < kSegItems; ++i) {
             if (i >= items) continue;
-            const size_t p = wave * wave_len + i * kWave + lane;
-            const bool valid = p < size;
-            const uint64_t prev = p > 0 && valid ? tile_keys[p - 1] : ~k[i];
-            keep[i] = ballot(valid && (p == 0 || prev != k[i]));
-            if (lane == 0) item_totals[wave][i] = static_cast<uint32_t>(__popcll(keep[i]));
+            const uint32_t p = wave * wave_len + i * kWave + lane;
+            const bool valid = p < n;
+            const K prev = p > 0 && valid ? tile_keys[p - 1] : static_cast<K>(~k[i]);
+            const uint64_t kept = ballot(valid && (p == 0 || prev != k[i]));
+            if (lane == 0) {
+                keep_lds[wave][i] = kept;
+                item_totals[wave][i] = static_cast<uint32_t>(__popcll(kept));
+            }
         }
         __syncthreads();
-        for (int w = 0; w < kWavesPerBlock; ++w)
+        for (int w = 0; w < kSegWaves; ++w)
             for (int i = 0; i < items; ++i) count += item_totals[w][i];
     } else {
         // oversized tile: its unique keys are a slice of the globally sorted side output
@@ -313,15 +329,17 @@ __global__ __launch_bounds__(kBlock) void seg_tile_kernel(const uint64_t* keys, 
 #pragma unroll
         for (int i = 0; i < kSegItems; ++i) {
             if (i >= items) continue;
-            if ((keep[i] >> lane) & 1ull) {
+            const uint64_t kept = keep_lds[wave][i];
+            if ((kept >> lane) & 1ull) {
                 const uint32_t rank = static_cast<uint32_t>(k[i] >> minor_bits);
-                const uint64_t pair = (static_cast<uint64_t>(major_of[rank]) << 32) | (k[i] & minor_mask);
-                store_pair(out, offset + pos + lanes_below(keep[i]), pair);
+                const uint64_t pair = (static_cast<uint64_t>(major_of[rank]) << 32) |
+                                      (static_cast<uint64_t>(k[i]) & minor_mask);
+                store_pair(out, offset + pos + lanes_below(kept), pair);
             }
-            pos += static_cast<uint32_t>(__popcll(keep[i]));
+            pos += static_cast<uint32_t>(__popcll(kept));
         }
     } else {
-        for (uint32_t i = threadIdx.x; i < count; i += kBlock) {
+        for (uint32_t i = threadIdx.x; i < count; i += kSegBlock) {
             const uint64_t kk = side_sorted[side0 + i];
             store_pair(out, offset + i, ((kk >> minor_bits) << 32) | (kk & minor_mask));
         }
@@ -380,8 +398,13 @@ hipError_t segmented_distinct_finish(const uint64_t* keys, size_t rows, uint32_t
         if (st != hipSuccess) return st;
     }
     timer_start(timer, stream);
-    seg_tile_kernel<<<static_cast<unsigned>(l.tiles), kBlock, 0, stream>>>(keys, rows, minor_bits, bounds,
-                                                                           side_unique_out, out, unique_rows, status, h);
+    // rank bits: at most log2(kSegCap) = 12 (a run per key)
+    if (minor_bits + 12 <= 32)
+        seg_tile_kernel<uint32_t><<<static_cast<unsigned>(l.tiles), kSegBlock, 0, stream>>>(
+            keys, rows, minor_bits, bounds, side_unique_out, out, unique_rows, status, h);
+    else
+        seg_tile_kernel<uint64_t><<<static_cast<unsigned>(l.tiles), kSegBlock, 0, stream>>>(
+            keys, rows, minor_bits, bounds, side_unique_out, out, unique_rows, status, h);
     timer_stop(timer, stream);
     return hipGetLastError();
 }

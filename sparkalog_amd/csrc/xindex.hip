@@ -1,0 +1,429 @@
+// First-column index of sorted packed relations, and the anti-join that uses
+// it.
+//
+// FULL is kept as a few sorted runs of packed pairs (first << 32 | second).
+// For a run R and every first value x in [0, n_x], xstart[x] is the number
+// of rows of R whose first column is below x, so the rows of R with first
+// column x are R[xstart[x], xstart[x+1]). Merging two runs adds their
+// indexes element-wise.
+//
+// The anti-join `left \ (R_0 ∪ R_1 ∪ ...)` (SortedBinaryAntiJoin,
+// crates/sparkalog-execution/src/lib.rs:672-710, with FULL as right input)
+// then reads, for a tile of left rows, only the segments of the first values
+// present in the tile: no search, and none of the rows of first values
+// without left rows. Each tile
+//   1. loads its rows (sorted, unique) into LDS and lists its first-value
+//      groups,
+//   2. per run: looks up each group's segment, scans the segment lengths,
+//      and streams the concatenated segments with four rows in flight per
+//      lane, marking every left row found (binary search within its group),
+//   3. compacts the unmarked rows in order at an offset from decoupled
+//      look-back, writing them to both outputs (NEWT and FULL's new run).
+#include "ops.hpp"
+#include "prims.hpp"
+
+namespace spk {
+namespace {
+
+constexpr int kXTile = 1024;                 // left rows per anti-join tile
+constexpr int kXItems = kXTile / kBlock;     // 4 rows per thread
+constexpr int kXWaveRows = kXItems * kWave;  // 256 rows per wave
+constexpr int kXUnroll = 8;                  // segment rows in flight per lane
+constexpr int kXHashBits = 11;               // hash set of a tile: 2048 slots (load <= 1/2)
+constexpr int kXHashSlots = 1 << kXHashBits;
+static_assert(kXHashSlots >= 2 * kXTile, "hash set load factor");
+constexpr uint32_t kXPairCap = 1024;         // (run, group) segments looked up per batch
+constexpr int kXPairItems = kXPairCap / kBlock;
+constexpr int kCTile = 2048;                 // rows per compaction tile
+constexpr int kCItems = kCTile / kBlock;
+constexpr int kCWaveRows = kCItems * kWave;
+constexpr uint32_t kXBuildSpan = 4096;       // first values per index-build block
+
+struct XHeader {
+    uint32_t error;
+    uint32_t counter;
+};
+
+__device__ __forceinline__ size_t rows_of(const uint64_t* rows_dev, size_t bound) {
+    if (!rows_dev) return bound;
+    const uint64_t n = *rows_dev;
+    return n < bound ? n : bound;
+}
+
+// First row with first column >= x (rows sorted), by block-cooperative search.
+__device__ __forceinline__ size_t block_first_at_least(const uint64_t* keys, size_t n, uint64_t x) {
+    return block_search(0, n, [&](size_t i) { return (keys[i] >> 32) < x; });
+}
+
+// One block per kXBuildSpan first values: mark where each value's rows start,
+// then xstart[x] = the smallest start at or after x (a suffix minimum).
+__global__ __launch_bounds__(kBlock) void xindex_build_kernel(const uint64_t* keys, const uint64_t* rows_dev,
+                                                              size_t bound, uint32_t n_x, uint32_t* xstart) {
+    __shared__ uint32_t first_row[kXBuildSpan];
+    __shared__ uint32_t scan_lds[kWavesPerBlock];
+    const size_t n = rows_of(rows_dev, bound);
+    const uint64_t xb = static_cast<uint64_t>(blockIdx.x) * kXBuildSpan;
+    if (xb > n_x) return;  // block-uniform
+    const uint64_t xe = xb + kXBuildSpan < static_cast<uint64_t>(n_x) + 1 ? xb + kXBuildSpan
+                                                                          : static_cast<uint64_t>(n_x) + 1;
+    const size_t r0 = block_first_at_least(keys, n, xb);
+    const size_t r1 = block_first_at_least(keys, n, xe);
+    for (uint32_t j = threadIdx.x; j < kXBuildSpan; j += kBlock) first_row[j] = static_cast<uint32_t>(r1);
+    __syncthreads();
+    // rows [r0, r1) have first values in [xb, xe); a row starts its value's
+    // group when the row before it has another first value
+    for (size_t c = r0; c < r1; c += static_cast<size_t>(kBlock) * kXUnroll) {
+        uint64_t cur[kXUnroll], prev[kXUnroll];
+#pragma unroll
+        for (int u = 0; u < kXUnroll; ++u) {
+            const size_t i = c + static_cast<size_t>(u) * kBlock + threadIdx.x;
+            cur[u] = i < r1 ? keys[i] : 0;
+            prev[u] = (i < r1 && i > r0) ? keys[i - 1] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kXUnroll; ++u) {
+            const size_t i = c + static_cast<size_t>(u) * kBlock + threadIdx.x;
+            if (i < r1 && (i == r0 || (prev[u] >> 32) != (cur[u] >> 32)))
+                first_row[static_cast<uint32_t>((cur[u] >> 32) - xb)] = static_cast<uint32_t>(i);
+        }
+    }
+    __syncthreads();
+    // suffix minimum over the span: thread t owns 16 consecutive values
+    constexpr int kPer = kXBuildSpan / kBlock;
+    const uint32_t t = threadIdx.x;
+    uint32_t v[kPer];
+    uint32_t m = static_cast<uint32_t>(r1);
+#pragma unroll
+    for (int j = kPer - 1; j >= 0; --j) {
+        const uint32_t f = first_row[t * kPer + j];
+        m = f < m ? f : m;
+        v[j] = m;
+    }
+    // minimum over later threads: inclusive scan of min over reversed order
+    uint32_t later = m;
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t o = __shfl_down(later, off, kWave);
+        if (lane_id() + off < static_cast<uint32_t>(kWave)) later = o < later ? o : later;
+    }
+    // `later` = min over lanes >= this lane of this wave
+    if (lane_id() == 0) scan_lds[wave_id()] = later;
+    __syncthreads();
+    uint32_t after_wave = static_cast<uint32_t>(r1);
+    for (int w = static_cast<int>(wave_id()) + 1; w < kWavesPerBlock; ++w)
+        after_wave = scan_lds[w] < after_wave ? scan_lds[w] : after_wave;
+    const uint32_t next_lane = __shfl_down(later, 1, kWave);
+    uint32_t after = lane_id() + 1 < static_cast<uint32_t>(kWave) ? next_lane : static_cast<uint32_t>(r1);
+    after = after_wave < after ? after_wave : after;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const uint64_t x = xb + t * kPer + j;
+        if (x < xe) xstart[x] = v[j] < after ? v[j] : after;
+    }
+}
+
+__global__ void xindex_add_kernel(uint32_t* into, const uint32_t* other, size_t n) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        into[i] += other[i];
+}
+
+__global__ void xanti_init_kernel(XHeader* h, uint64_t* status, size_t tiles) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    const size_t gtid = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gtid == 0) {
+        h->error = 0;
+        h->counter = 0;
+    }
+    for (size_t i = gtid; i < tiles; i += stride) status[i] = 0;
+}
+
+// Largest j in [j, last] with pre[j] <= t (pre non-decreasing, pre[j] <= t).
+__device__ __forceinline__ uint32_t advance_pair(const uint64_t* pre, uint32_t j, uint32_t last, uint64_t t) {
+    if (j + 1 > last || pre[j + 1] > t) return j;
+    uint32_t lo = j + 1, hi = last;  // answer in [lo, hi]
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Slot of a packed pair in the tile's hash set (multiplicative mixing of
+// both halves; top bits of the product).
+__device__ __forceinline__ uint32_t pair_slot(uint64_t key) {
+    const uint32_t a = static_cast<uint32_t>(key), b = static_cast<uint32_t>(key >> 32);
+    const uint32_t h = (a ^ (b * 0x9E3779B9u)) * 0x85EBCA6Bu;
+    return h >> (32 - kXHashBits);
+}
+
+// Marks, for every left row of a tile, whether FULL holds it: bit r of
+// found_bits[tile * kXTile / 32 + r / 32]. Tiles are independent (their work
+// varies with the segment sizes), so no tile waits for another.
+__global__ __launch_bounds__(kBlock) void anti_join_x_kernel(const uint64_t* left, const uint64_t* left_rows,
+                                                             size_t bound, XRights rights, uint32_t n_x,
+                                                             uint32_t* found_bits) {
+    __shared__ uint64_t tkeys[kXTile];
+    __shared__ uint32_t table[kXHashSlots];  // tile row + 1, or 0 for an empty slot
+    __shared__ uint32_t grp_x[kXTile];           // first value of each group of equal first values
+    __shared__ uint64_t pair_pre[kXPairCap + 1];  // exclusive prefix of the batch's segment lengths
+    __shared__ uint32_t pair_beg[kXPairCap];      // segment start row within its run
+    __shared__ uint8_t pair_run[kXPairCap];       // the segment's run
+    __shared__ const uint64_t* run_keys[kMaxAntiRights];
+    __shared__ uint8_t found[kXTile];
+    __shared__ uint32_t item_cnt[kWavesPerBlock][kXItems];
+    __shared__ uint64_t scan_lds[kWavesPerBlock];
+    const size_t nl = rows_of(left_rows, bound);
+    const uint32_t tile = blockIdx.x;
+    const size_t base = static_cast<size_t>(tile) * kXTile;
+    if (base >= nl) return;
+    const uint32_t lane = lane_id(), wave = wave_id();
+    const uint32_t rows = static_cast<uint32_t>(nl - base < static_cast<size_t>(kXTile) ? nl - base : kXTile);
+    uint64_t k[kXItems];
+#pragma unroll
+    for (int i = 0; i < kXItems; ++i) {
+        const uint32_t r = wave * kXWaveRows + i * kWave + lane;
+        k[i] = r < rows ? left[base + r] : ~0ull;
+        if (r < rows) {
+            tkeys[r] = k[i];
+            found[r] = 0;
+        }
+    }
+    for (uint32_t j = threadIdx.x; j < kXHashSlots; j += kBlock) table[j] = 0;
+    if (threadIdx.x < rights.count) run_keys[threadIdx.x] = rights.keys[threadIdx.x];
+    __syncthreads();
+    // hash set of the tile's rows (unique keys: every insert takes a new slot)
+    uint64_t starts[kXItems];
+#pragma unroll
+    for (int i = 0; i < kXItems; ++i) {
+        const uint32_t r = wave * kXWaveRows + i * kWave + lane;
+        if (r < rows) {
+            uint32_t slot = pair_slot(k[i]);
+            while (atomicCAS(&table[slot], 0u, r + 1) != 0u) slot = (slot + 1) & (kXHashSlots - 1);
+        }
+        // groups of equal first value, in tile order
+        const bool st = r < rows && (r == 0 || (tkeys[r - 1] >> 32) != (k[i] >> 32));
+        starts[i] = ballot(st);
+        if (lane == 0) item_cnt[wave][i] = static_cast<uint32_t>(__popcll(starts[i]));
+    }
+    __syncthreads();
+    uint32_t pos = 0, groups = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w)
+        for (int i = 0; i < kXItems; ++i) {
+            if (w < static_cast<int>(wave)) pos += item_cnt[w][i];
+            groups += item_cnt[w][i];
+        }
+#pragma unroll
+    for (int i = 0; i < kXItems; ++i) {
+        if ((starts[i] >> lane) & 1ull) grp_x[pos + lanes_below(starts[i])] = static_cast<uint32_t>(k[i] >> 32);
+        pos += static_cast<uint32_t>(__popcll(starts[i]));
+    }
+    __syncthreads();
+
+    // (run, group) pairs in batches of whole runs: each batch looks up all
+    // its segments at once, scans their lengths and streams them
+    const uint32_t runs_per_batch = groups <= kXPairCap ? kXPairCap / groups : 1;
+    for (uint32_t q0 = 0; q0 < rights.count; q0 += runs_per_batch) {
+        const uint32_t q1 = q0 + runs_per_batch < rights.count ? q0 + runs_per_batch : rights.count;
+        const uint32_t pairs = (q1 - q0) * groups;
+        // a batch with more pairs than LDS slots (one run of > kXPairCap
+        // groups) is cut into pieces of kXPairCap groups
+        for (uint32_t p0 = 0; p0 < pairs; p0 += kXPairCap) {
+            const uint32_t np = pairs - p0 < kXPairCap ? pairs - p0 : kXPairCap;
+            uint32_t beg[kXPairItems], len[kXPairItems];
+            uint64_t sum = 0;
+#pragma unroll
+            for (int g = 0; g < kXPairItems; ++g) {
+                const uint32_t pi = threadIdx.x * kXPairItems + g;
+                beg[g] = len[g] = 0;
+                if (pi < np) {
+                    const uint32_t pair = p0 + pi;
+                    const uint32_t q = q0 + pair / groups, j = pair % groups;
+                    const uint32_t x = grp_x[j];
+                    if (x < n_x) {
+                        const uint32_t* xs = rights.xstart[q];
+                        beg[g] = xs[x];
+                        len[g] = xs[x + 1] - beg[g];
+                    }
+                    pair_run[pi] = static_cast<uint8_t>(q);
+                }
+                sum += len[g];
+            }
+            uint64_t total = 0;
+            uint64_t run = block_exclusive_sum<uint64_t>(sum, &total, scan_lds);
+#pragma unroll
+            for (int g = 0; g < kXPairItems; ++g) {
+                const uint32_t pi = threadIdx.x * kXPairItems + g;
+                if (pi < np) {
+                    pair_pre[pi] = run;
+                    pair_beg[pi] = beg[g];
+                }
+                run += len[g];
+            }
+            if (threadIdx.x == 0) pair_pre[np] = total;
+            __syncthreads();
+            // stream the concatenated segments, kXUnroll rows in flight per lane
+            uint32_t pj = 0;
+            const uint64_t* rk = run_keys[pair_run[0]];
+            uint64_t row_off = pair_beg[0];  // row = row_off + t - pair_pre[pj]
+            uint64_t pre = 0;
+            for (uint64_t c = 0; c < total; c += kBlock * kXUnroll) {
+                uint64_t row[kXUnroll];
+#pragma unroll
+                for (int u = 0; u < kXUnroll; ++u) {
+                    const uint64_t t = c + u * kBlock + threadIdx.x;
+                    row[u] = 0;
+                    if (t < total) {
+                        const uint32_t nj = advance_pair(pair_pre, pj, np - 1, t);
+                        if (nj != pj) {
+                            pj = nj;
+                            rk = run_keys[pair_run[pj]];
+                            row_off = pair_beg[pj];
+                            pre = pair_pre[pj];
+                        }
+                        row[u] = rk[row_off + (t - pre)];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kXUnroll; ++u) {
+                    const uint64_t t = c + u * kBlock + threadIdx.x;
+                    if (t >= total) continue;
+                    uint32_t slot = pair_slot(row[u]);
+                    for (;;) {
+                        const uint32_t v = table[slot];
+                        if (v == 0) break;
+                        if (tkeys[v - 1] == row[u]) {
+                            found[v - 1] = 1;
+                            break;
+                        }
+                        slot = (slot + 1) & (kXHashSlots - 1);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // found bits of the tile, one word per 32 rows
+    if (threadIdx.x < kXTile / 32) {
+        uint32_t w = 0;
+        for (int b = 0; b < 32; ++b) {
+            const uint32_t r = threadIdx.x * 32 + b;
+            if (r < rows && found[r]) w |= 1u << b;
+        }
+        found_bits[static_cast<size_t>(tile) * (kXTile / 32) + threadIdx.x] = w;
+    }
+}
+
+// Order-preserving compaction of the left rows whose found bit is clear,
+// written to both outputs at offsets from decoupled look-back (uniform work
+// per tile).
+__global__ __launch_bounds__(kBlock) void xanti_compact_kernel(const uint64_t* left, const uint64_t* left_rows,
+                                                               size_t bound, const uint32_t* found_bits,
+                                                               PairOutput out, uint64_t* count64, uint64_t* status,
+                                                               XHeader* h) {
+    __shared__ uint32_t item_cnt[kWavesPerBlock][kCItems];
+    __shared__ uint64_t offset_slot;
+    __shared__ uint32_t tile_slot;
+    const size_t nl = rows_of(left_rows, bound);
+    const uint32_t tile = next_tile(&h->counter, &tile_slot);
+    const size_t base = static_cast<size_t>(tile) * kCTile;
+    if (nl == 0) {
+        if (tile == 0 && threadIdx.x == 0) *count64 = 0;
+        return;
+    }
+    if (base >= nl) return;
+    const uint32_t lane = lane_id(), wave = wave_id();
+    const uint32_t rows = static_cast<uint32_t>(nl - base < static_cast<size_t>(kCTile) ? nl - base : kCTile);
+    uint64_t k[kCItems], keep[kCItems];
+#pragma unroll
+    for (int i = 0; i < kCItems; ++i) {
+        const uint32_t r = wave * kCWaveRows + i * kWave + lane;
+        const size_t g = base + r;
+        bool alive = false;
+        k[i] = 0;
+        if (r < rows) {
+            k[i] = left[g];
+            alive = !((found_bits[g >> 5] >> (g & 31)) & 1u);
+        }
+        keep[i] = ballot(alive);
+        if (lane == 0) item_cnt[wave][i] = static_cast<uint32_t>(__popcll(keep[i]));
+    }
+    __syncthreads();
+    uint32_t wave_start = 0, tile_total = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w)
+        for (int i = 0; i < kCItems; ++i) {
+            if (w < static_cast<int>(wave)) wave_start += item_cnt[w][i];
+            tile_total += item_cnt[w][i];
+        }
+    if (wave == 0) {
+        const uint64_t excl = wave_lookback(status, tile, tile_total, 0, &h->error);
+        if (lane == 0) offset_slot = excl;
+    }
+    __syncthreads();
+    const uint64_t offset = offset_slot;
+    uint32_t p = wave_start;
+#pragma unroll
+    for (int i = 0; i < kCItems; ++i) {
+        if ((keep[i] >> lane) & 1ull) {
+            const size_t o = offset + p + lanes_below(keep[i]);
+            if (out.packed) out.packed[o] = k[i];
+            if (out.packed_copy) out.packed_copy[o] = k[i];
+        }
+        p += static_cast<uint32_t>(__popcll(keep[i]));
+    }
+    if (base + rows == nl && threadIdx.x == 0) *count64 = offset + tile_total;
+}
+
+size_t xanti_tiles(size_t bound) { return (bound + kXTile - 1) / kXTile; }
+size_t xcompact_tiles(size_t bound) { return (bound + kCTile - 1) / kCTile; }
+
+}  // namespace
+
+hipError_t xindex_build(const uint64_t* keys, const uint64_t* rows_dev, size_t bound, uint32_t n_x, uint32_t* xstart,
+                        hipStream_t stream) {
+    const size_t blocks = (static_cast<size_t>(n_x) + 1 + kXBuildSpan - 1) / kXBuildSpan;
+    xindex_build_kernel<<<static_cast<unsigned>(blocks), kBlock, 0, stream>>>(keys, rows_dev, bound, n_x, xstart);
+    return hipGetLastError();
+}
+
+hipError_t xindex_add(uint32_t* into, const uint32_t* other, uint32_t n_x, hipStream_t stream) {
+    const size_t n = static_cast<size_t>(n_x) + 1;
+    const size_t blocks = (n + kBlock * 4 - 1) / (kBlock * 4);
+    xindex_add_kernel<<<static_cast<unsigned>(blocks < 8192 ? blocks : 8192), kBlock, 0, stream>>>(into, other, n);
+    return hipGetLastError();
+}
+
+size_t anti_join_x_temp_bytes(size_t bound) {
+    const size_t b = bound ? bound : 1;
+    return 256 + (xcompact_tiles(b) * sizeof(uint64_t) + 255) / 256 * 256 +
+           (xanti_tiles(b) * (kXTile / 32) * sizeof(uint32_t) + 255) / 256 * 256;
+}
+
+hipError_t anti_join_x(const uint64_t* left, const uint64_t* left_rows, size_t bound, const XRights& rights,
+                       uint32_t n_x, PairOutput out, uint64_t* count64, void* temp, size_t temp_bytes,
+                       hipStream_t stream, const KernelTimer* timer) {
+    if (bound == 0) return set_u64(count64, 0, stream);
+    if (temp_bytes < anti_join_x_temp_bytes(bound) || rights.count > static_cast<uint32_t>(kMaxAntiRights) ||
+        out.first != nullptr)
+        return hipErrorInvalidValue;
+    const size_t probe_tiles = xanti_tiles(bound), tiles = xcompact_tiles(bound);
+    char* t = static_cast<char*>(temp);
+    XHeader* h = reinterpret_cast<XHeader*>(t);
+    uint64_t* status = reinterpret_cast<uint64_t*>(t + 256);
+    uint32_t* found_bits = reinterpret_cast<uint32_t*>(t + 256 + (tiles * sizeof(uint64_t) + 255) / 256 * 256);
+    const size_t init_blocks = tiles / kBlock + 1;
+    xanti_init_kernel<<<static_cast<unsigned>(init_blocks < 4096 ? init_blocks : 4096), kBlock, 0, stream>>>(
+        h, status, tiles);
+    timer_start(timer, stream);
+    anti_join_x_kernel<<<static_cast<unsigned>(probe_tiles), kBlock, 0, stream>>>(left, left_rows, bound, rights,
+                                                                                  n_x, found_bits);
+    timer_stop(timer, stream);
+    xanti_compact_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(left, left_rows, bound, found_bits, out,
+                                                                             count64, status, h);
+    return hipGetLastError();
+}
+
+}  // namespace spk

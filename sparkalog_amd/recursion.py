@@ -176,6 +176,13 @@ class RelationStore:
     def remove_range(self, id: RelationId, first_lo: int, first_hi: int) -> None:
         _check(self._lib.sparkalog_hip_store_remove_range(self._h, int(id), first_lo, first_hi))
 
+    def set_tuning(self, merge_ratio: int = 4, segmented_distinct: bool = True, indexed_anti_join: bool = False) -> None:
+        """Run-merge ratio of FULL's sorted runs, the grouped (per-run LDS)
+        distinct and the first-column-indexed anti-join; results are
+        identical for every setting."""
+        _check(self._lib.sparkalog_hip_store_set_tuning(self._h, merge_ratio, int(segmented_distinct),
+                                                        int(indexed_anti_join)))
+
     def set_profiling(self, enabled: bool) -> None:
         _check(self._lib.sparkalog_hip_store_set_profiling(self._h, int(enabled)))
 

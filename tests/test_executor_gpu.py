@@ -185,3 +185,20 @@ def test_ungrouped_projection_plan(stream):
     full, expect = store.full(R(1)), ref.get(1)
     assert np.array_equal(full[0], expect[0]) and np.array_equal(full[1], expect[1])
     assert summ.iterations == rsum.iterations and summ.total_new_rows == rsum.total_new_rows
+
+
+@pytest.mark.parametrize("merge_ratio,segmented,indexed", [(2, True, True), (4, False, True), (4, True, False),
+                                                           (16, False, False)])
+def test_tuning_does_not_change_results(merge_ratio, segmented, indexed, expected, stream):
+    e = expected["component_graph_16384"]
+    s, d, _ = workload.component_graph(nodes=1 << 14)
+    store = RelationStore(stream)
+    store.set_tuning(merge_ratio=merge_ratio, segmented_distinct=segmented, indexed_anti_join=indexed)
+    store.insert_static(R(0), (s, d))
+    store.insert_recursive(R(1), (s, d), (s, d))
+    summ = RecursiveExecutor.compile(plans.transitive_closure_scc(R(1), R(0)), store).run(store, 10_000)
+    full = store.full(R(1))
+    bfs = oracle.tc_bfs(s, d, 1 << 14, closure=True)
+    assert np.array_equal(full[0], bfs["closure"][0]) and np.array_equal(full[1], bfs["closure"][1])
+    assert summ.iterations == e["max_dist"]
+    assert summ.total_new_rows == e["path_rows"] - e["edges"]
