@@ -143,7 +143,7 @@ enum OpKind { kOpJoinCount = 0, kOpJoinEmit, kOpDistinct, kOpAntiJoin, kOpUnion,
 const char* const kOpNames[kOpCount] = {"join_count", "join_emit", "distinct", "anti_join",
                                         "union", "merge_runs", "index"};
 const char* const kOpKernels[kOpCount] = {"dense_join_count_kernel", "dense_join_emit_kernel", "seg_tile_kernel",
-                                          "anti_join_kernel", "union_kernel", "union_kernel", ""};
+                                          "anti_join_kernel", "union_kernel", "merge_disjoint_kernel", ""};
 
 // Events of one profiled operator launch.
 struct OpEvents {
@@ -795,7 +795,6 @@ void ensure_xidx(sparkalog_hip_store* s, DevRelation& r, uint32_t n_x) {
 
 double settle_runs(sparkalog_hip_store* s, RelationState& ts, bool all) {
     double bytes = 0.0;
-    Counters* cdev = reinterpret_cast<Counters*>(s->counters_dev.ptr);
     while (!ts.runs.empty()) {
         DevRelation& last = ts.runs.back();
         DevRelation& prev = ts.runs.size() >= 2 ? ts.runs[ts.runs.size() - 2] : ts.full;
@@ -817,14 +816,11 @@ double settle_runs(sparkalog_hip_store* s, RelationState& ts, bool all) {
             const uint32_t nx = prev.xidx_n;
             const bool add = nx && prev.xidx_valid(nx) && last.xidx_valid(nx);
             DevBuf<uint64_t> out = s->take(n);
-            const size_t tb = union_temp_bytes(prev.rows, last.rows);
+            const size_t tb = merge_disjoint_temp_bytes(prev.rows, last.rows);
             void* temp = s->scratch(tb);
-            PairInput a{prev.keys.ptr, nullptr, nullptr, nullptr};
-            PairInput b{last.keys.ptr, nullptr, nullptr, nullptr};
-            PairOutput o{out.ptr, nullptr, nullptr};
             const int ev = s->begin_op();
-            check(sorted_union(a, prev.rows, b, last.rows, o, &cdev->merge_rows, temp, tb, s->stream,
-                               s->kernel_timer(ev)));
+            check(merge_disjoint(prev.keys.ptr, prev.rows, last.keys.ptr, last.rows, out.ptr, temp, tb, s->stream,
+                                 s->kernel_timer(ev)));
             s->end_op(ev, kOpMergeRuns, 16.0 * n);
             if (add) check(xindex_add(prev.xidx.ptr, last.xidx.ptr, nx, s->stream));
             s->give(std::move(prev.keys));

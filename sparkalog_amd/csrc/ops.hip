@@ -470,6 +470,51 @@ __device__ __forceinline__ uint32_t merge_path_lds(const uint64_t* a, uint32_t n
 // Merge-path partition of every tile boundary: part[t] = number of left rows
 // among the first t*kMergeTile merged rows. One thread per boundary, so the
 // searches of all tiles overlap instead of stalling each tile's start.
+// Merge of two sorted arrays with no key in common (FULL's runs): merged row
+// d goes to out[d], so every tile knows its output range and no tile waits
+// for another. Tile t covers merged rows [t*kMergeTile, (t+1)*kMergeTile),
+// whose split between the inputs comes from the partition pass.
+__global__ __launch_bounds__(kBlock) void merge_disjoint_kernel(const uint64_t* a, size_t na, const uint64_t* b,
+                                                                size_t nb, uint64_t* out, const size_t* part) {
+    __shared__ uint64_t lds[kMergeTile];
+    const size_t tile = blockIdx.x;
+    const size_t total = na + nb;
+    const size_t d0 = tile * kMergeTile;
+    if (d0 >= total) return;
+    const size_t d1 = d0 + kMergeTile < total ? d0 + kMergeTile : total;
+    const size_t a0 = part[tile], a1 = part[tile + 1];
+    const size_t b0 = d0 - a0, b1 = d1 - a1;
+    const uint32_t n_a = static_cast<uint32_t>(a1 - a0), n_b = static_cast<uint32_t>(b1 - b0);
+    uint64_t* la = lds;
+    uint64_t* lb = lds + n_a;
+    for (uint32_t i = threadIdx.x; i < n_a; i += kBlock) la[i] = a[a0 + i];
+    for (uint32_t i = threadIdx.x; i < n_b; i += kBlock) lb[i] = b[b0 + i];
+    __syncthreads();
+    const uint32_t n = n_a + n_b;
+    const uint32_t diag = threadIdx.x * kMergeItems < n ? threadIdx.x * kMergeItems : n;
+    uint32_t ia = merge_path_lds(la, n_a, lb, n_b, diag);
+    uint32_t ib = diag - ia;
+    uint64_t v[kMergeItems];
+#pragma unroll
+    for (int i = 0; i < kMergeItems; ++i) {
+        const bool take_a = ib >= n_b || (ia < n_a && la[ia] <= lb[ib]);
+        v[i] = 0;
+        if (diag + i < n) {
+            v[i] = take_a ? la[ia] : lb[ib];
+            if (take_a)
+                ++ia;
+            else
+                ++ib;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kMergeItems; ++i)
+        if (diag + i < n) lds[diag + i] = v[i];
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += kBlock) out[d0 + i] = lds[i];
+}
+
 __global__ void merge_partition_kernel(PairInput left, size_t nl, PairInput right, size_t nr, size_t* part,
                                        size_t n_parts) {
     nl = actual_rows(left, nl);
@@ -1106,6 +1151,27 @@ hipError_t sorted_union(PairInput left, size_t left_rows, PairInput right, size_
     union_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(
         left, left_rows, right, right_rows, out, unique_rows, part,
         reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp));
+    timer_stop(timer, stream);
+    return hipGetLastError();
+}
+
+size_t merge_disjoint_temp_bytes(size_t left_rows, size_t right_rows) {
+    return align_up((tiles_of(left_rows + right_rows, kMergeTile) + 1) * sizeof(size_t), 256);
+}
+
+hipError_t merge_disjoint(const uint64_t* left, size_t left_rows, const uint64_t* right, size_t right_rows,
+                          uint64_t* out, void* temp, size_t temp_bytes, hipStream_t stream, const KernelTimer* timer) {
+    const size_t total = left_rows + right_rows;
+    if (total == 0) return hipSuccess;
+    if (temp_bytes < merge_disjoint_temp_bytes(left_rows, right_rows)) return hipErrorInvalidValue;
+    const size_t tiles = tiles_of(total, kMergeTile);
+    size_t* part = static_cast<size_t*>(temp);
+    const PairInput a{left, nullptr, nullptr, nullptr}, b{right, nullptr, nullptr, nullptr};
+    merge_partition_kernel<<<grid_for(tiles + 1, kBlock, 4096), kBlock, 0, stream>>>(a, left_rows, b, right_rows,
+                                                                                    part, tiles + 1);
+    timer_start(timer, stream);
+    merge_disjoint_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(left, left_rows, right, right_rows,
+                                                                               out, part);
     timer_stop(timer, stream);
     return hipGetLastError();
 }

@@ -96,6 +96,13 @@ hipError_t sorted_union(PairInput left, size_t left_rows, PairInput right, size_
                         PairOutput out, uint64_t* unique_rows, void* temp, size_t temp_bytes,
                         hipStream_t stream, const KernelTimer* timer = nullptr);
 
+// Merge of two sorted packed arrays that share no key (FULL's runs):
+// out[0, left_rows + right_rows) in order.
+size_t merge_disjoint_temp_bytes(size_t left_rows, size_t right_rows);
+hipError_t merge_disjoint(const uint64_t* left, size_t left_rows, const uint64_t* right, size_t right_rows,
+                          uint64_t* out, void* temp, size_t temp_bytes, hipStream_t stream,
+                          const KernelTimer* timer = nullptr);
+
 size_t anti_join_temp_bytes(size_t left_rows, uint32_t right_inputs = 1);
 // left \ right; `count32`/`count64` receive the output row count (either may
 // be null).
