@@ -50,6 +50,10 @@ def parse_args():
     p.add_argument("--requests", type=int, default=200, help="served: stream length")
     p.add_argument("--utilization", type=float, default=0.7, help="served: offered load of the plain path")
     p.add_argument("--plain-stride", type=int, default=1, help="batched: time the plain path on every k-th instance")
+    p.add_argument("--lane-bounds", default="1e6,3e7",
+                   help="served: closure-size bounds (pairs) between continuous-batching lanes")
+    p.add_argument("--lane-slots", default="256,256,2",
+                   help="served: instances in flight per lane (comma list; empty = 256 each)")
     p.add_argument("--sizes", default="24,26,28", help="ops mode: log2 row counts")
     p.add_argument("--ops", default="distinct,union,anti_join,join", help="ops mode: operators")
     p.add_argument("--samples", type=int, default=5, help="ops mode: timed launches per case")
@@ -77,7 +81,7 @@ def run_ops(args) -> None:
 
 
 def run_batched_mode(args) -> None:
-    """Config 4: 4,096 instances (nodes log-uniform on [16, 16384], Poisson
+    """Batched fixpoints: 4,096 instances (nodes log-uniform on [16, 16384], Poisson
     out-degree mean uniform on [1.2, 3.0]) in one semi-naive loop per HBM
     wave, against one executor run per instance."""
     import numpy as np
@@ -109,10 +113,10 @@ def run_batched_mode(args) -> None:
 
 
 def run_served_mode(args) -> None:
-    """Config 5: Poisson stream of TC requests (edges log-uniform on
-    [2^8, 2^20]); continuous batching against sequential plain execution of
-    the same stream, at an arrival rate giving the plain path the requested
-    utilisation."""
+    """Served fixpoints: a Poisson stream of TC requests (edges log-uniform
+    on [2^8, 2^20]); continuous batching in size-class lanes (and in one
+    lane) against sequential plain execution of the same stream, at an
+    arrival rate giving the plain path the requested utilisation."""
     import numpy as np
     import torch
 
@@ -129,12 +133,19 @@ def run_served_mode(args) -> None:
     mean_service = secs / len(calib)
     rate = args.utilization / mean_service
     reqs = serving.make_requests(args.requests, rate_per_s=rate)
-    serving.serve_batched(reqs[:4], store, budget_pairs=args.budget_pairs)  # warm-up
-    mixed, t_mixed = serving.serve_batched(reqs, store, budget_pairs=args.budget_pairs)
+    bounds = [float(b) for b in args.lane_bounds.split(",")] if args.lane_bounds else []
+    lanes = [RelationStore(HipStream(torch.cuda.Stream())) for _ in range(len(bounds) + 1)]
+    slots = [int(x) for x in args.lane_slots.split(",")] if args.lane_slots else None
+    serving.serve_lanes(reqs[:6], lanes, bounds, budget_pairs=args.budget_pairs, lane_slots=slots)  # warm-up
+    mixed, t_mixed = serving.serve_lanes(reqs, lanes, bounds, budget_pairs=args.budget_pairs, lane_slots=slots)
+    one, t_one = serving.serve_batched(reqs, store, budget_pairs=args.budget_pairs)
     alone, t_alone = serving.serve_plain(reqs, store)
-    equal = all((m.rows, m.digest) == (a.rows, a.digest) for m, a in zip(mixed, alone))
+    equal = all((m.rows, m.digest) == (a.rows, a.digest) for m, a in zip(mixed, alone)) and \
+        all((m.rows, m.digest) == (a.rows, a.digest) for m, a in zip(one, alone))
     print(json.dumps({"mode": "served", "rate_per_s": round(rate, 3), "mean_plain_service_ms": round(1000 * mean_service, 3),
-                      "batched": serving.summarize(mixed, t_mixed), "plain": serving.summarize(alone, t_alone),
+                      "lane_bounds_pairs": bounds, "lane_slots": slots,
+                      "batched_lanes": serving.summarize(mixed, t_mixed),
+                      "batched_one_lane": serving.summarize(one, t_one), "plain": serving.summarize(alone, t_alone),
                       "results_equal_alone_vs_mix": equal}), flush=True)
 
 
@@ -194,6 +205,10 @@ def cpu_baseline(sample_nodes: int) -> dict:
 
 def main():
     args = parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
+              "components and instances are independent, so each GPU would run its own share", file=sys.stderr)
+        sys.exit(2)
     if args.mode == "ops":
         run_ops(args)
         return
@@ -206,11 +221,6 @@ def main():
     if args.mode == "copy":
         run_copy_mode(args)
         return
-    if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
-              "the multi-GPU sharding is described in DESIGN.md", file=sys.stderr)
-        sys.exit(2)
-
     import numpy as np
     import torch
 

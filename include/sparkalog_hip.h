@@ -294,6 +294,11 @@ int sparkalog_hip_store_merge_rows(sparkalog_hip_store* store, uint32_t id, int 
 /* Remove every row whose first column is in [first_lo, first_hi) from FULL
  * and DELTA. */
 int sparkalog_hip_store_remove_range(sparkalog_hip_store* store, uint32_t id, uint32_t first_lo, uint32_t first_hi);
+/* Remove the rows of several ranges at once: first_bounds holds `ranges`
+ * pairs [lo, hi), ascending and disjoint; each part of FULL and DELTA is
+ * copied once. */
+int sparkalog_hip_store_remove_ranges(sparkalog_hip_store* store, uint32_t id, const uint32_t* first_bounds,
+                                      uint32_t ranges);
 
 /* The hipError_t of the last failed HIP call in this thread (0 if none). */
 int sparkalog_hip_last_hip_error(void);

@@ -556,6 +556,17 @@ struct sparkalog_hip_store {
         if (b.ptr) xpool.push_back(std::move(b));
     }
     bool x_anti = false;  // anti-join FULL through its first-column index when it fits
+    // small persistent device buffers for range probes and statistics (a
+    // fresh allocation would synchronize the whole device when freed)
+    spk::DevBuf<uint64_t> small_dev, probe_dev;
+    uint64_t* small16() {
+        small_dev.reserve(16);
+        return small_dev.ptr;
+    }
+    uint64_t* probes(size_t n) {
+        probe_dev.reserve(n ? n : 1);
+        return probe_dev.ptr;
+    }
     // operator workspaces kept across runs so repeated runs allocate nothing
     std::vector<spk::TargetRuntime> ws_targets;
     std::vector<spk::RuleRuntime> ws_rules;
@@ -732,8 +743,7 @@ void build_index(sparkalog_hip_store* s, RuleRuntime& rt, RelationState& right, 
     const uint32_t n_keys = right.max_value + 1;
     ix.sorted.reserve(n ? n : 1);
     ix.starts.reserve(static_cast<size_t>(n_keys) + 1);
-    DevBuf<uint64_t> tmp;
-    tmp.reserve(n ? n : 1);
+    DevBuf<uint64_t> tmp = s->take(n ? n : 1);
     if (n) {
         gather_index_keys_kernel<<<grid_for(n), kBlock, 0, s->stream>>>(right.full.keys.ptr, n, col, ix.sorted.ptr);
         check(hipGetLastError());
@@ -755,6 +765,7 @@ void build_index(sparkalog_hip_store* s, RuleRuntime& rt, RelationState& right, 
     ix.column = col;
     ix.n_keys = n_keys;
     ix.rows = n;
+    s->give(std::move(tmp));
     s->end_op(ev, kOpIndex, 16.0 * n);
 }
 
@@ -926,13 +937,12 @@ std::pair<uint64_t, uint64_t> first_range(sparkalog_hip_store* s, const DevRelat
     if (r.rows == 0) return {0, 0};
     uint64_t probes[2] = {static_cast<uint64_t>(first_lo) << 32, static_cast<uint64_t>(first_hi) << 32};
     if (first_hi == 0) probes[1] = 0;
-    DevBuf<uint64_t> dev;
-    dev.reserve(4);
-    check(hipMemcpyAsync(dev.ptr, probes, sizeof(probes), hipMemcpyHostToDevice, s->stream));
-    lower_bounds_kernel<<<1, 64, 0, s->stream>>>(r.keys.ptr, r.rows, dev.ptr, 2, dev.ptr + 2);
+    uint64_t* dev = s->small16() + 8;  // [0, 8) is left to the callers
+    check(hipMemcpyAsync(dev, probes, sizeof(probes), hipMemcpyHostToDevice, s->stream));
+    lower_bounds_kernel<<<1, 64, 0, s->stream>>>(r.keys.ptr, r.rows, dev, 2, dev + 2);
     check(hipGetLastError());
     uint64_t out[2];
-    check(hipMemcpyAsync(out, dev.ptr + 2, sizeof(out), hipMemcpyDeviceToHost, s->stream));
+    check(hipMemcpyAsync(out, dev + 2, sizeof(out), hipMemcpyDeviceToHost, s->stream));
     check(hipStreamSynchronize(s->stream));
     return {out[0], out[1] > out[0] ? out[1] : out[0]};
 }
@@ -1709,16 +1719,16 @@ extern "C" int sparkalog_hip_store_instance_stats(sparkalog_hip_store* s, uint32
     if (!st) return SPARKALOG_HIP_ERR_MISSING_RELATION;
     SPK_GUARD_BEGIN
     if (!st->runs.empty()) settle_runs(s, *st, true);
-    DevBuf<unsigned long long> acc;
-    acc.reserve(2 * static_cast<size_t>(instances) + 2);
-    check(fill_u64(reinterpret_cast<uint64_t*>(acc.ptr), 2 * static_cast<size_t>(instances), 0, s->stream));
+    unsigned long long* acc =
+        reinterpret_cast<unsigned long long*>(s->probes(2 * static_cast<size_t>(instances) + 2));
+    check(fill_u64(reinterpret_cast<uint64_t*>(acc), 2 * static_cast<size_t>(instances), 0, s->stream));
     if (st->full.rows && instances)
         instance_stats_kernel<<<grid_for(st->full.rows), kBlock, 0, s->stream>>>(
-            st->full.keys.ptr, st->full.rows, instance_bits, instances, acc.ptr, acc.ptr + instances);
+            st->full.keys.ptr, st->full.rows, instance_bits, instances, acc, acc + instances);
     check(hipGetLastError());
     if (instances) {
-        check(hipMemcpyAsync(rows, acc.ptr, instances * sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
-        check(hipMemcpyAsync(digests, acc.ptr + instances, instances * sizeof(uint64_t), hipMemcpyDeviceToHost,
+        check(hipMemcpyAsync(rows, acc, instances * sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
+        check(hipMemcpyAsync(digests, acc + instances, instances * sizeof(uint64_t), hipMemcpyDeviceToHost,
                              s->stream));
     }
     check(hipStreamSynchronize(s->stream));
@@ -1740,18 +1750,17 @@ extern "C" int sparkalog_hip_store_range_stats(sparkalog_hip_store* s, uint32_t 
         parts.push_back(&st->full);
         for (const auto& r : st->runs) parts.push_back(&r);
     }
-    DevBuf<unsigned long long> acc;
-    acc.reserve(2);
-    check(fill_u64(reinterpret_cast<uint64_t*>(acc.ptr), 2, 0, s->stream));
+    unsigned long long* acc = reinterpret_cast<unsigned long long*>(s->small16());
+    check(fill_u64(reinterpret_cast<uint64_t*>(acc), 2, 0, s->stream));
     for (const DevRelation* r : parts) {
         const auto range = first_range(s, *r, first_lo, first_hi);
         if (range.second > range.first)
             range_stats_kernel<<<grid_for(range.second - range.first), kBlock, 0, s->stream>>>(
-                r->keys.ptr, range.first, range.second, base, acc.ptr);
+                r->keys.ptr, range.first, range.second, base, acc);
     }
     check(hipGetLastError());
     unsigned long long host[2];
-    check(hipMemcpyAsync(host, acc.ptr, sizeof(host), hipMemcpyDeviceToHost, s->stream));
+    check(hipMemcpyAsync(host, acc, sizeof(host), hipMemcpyDeviceToHost, s->stream));
     check(hipStreamSynchronize(s->stream));
     if (rows) *rows = host[0];
     if (digest) *digest = host[1];
@@ -1780,6 +1789,60 @@ extern "C" int sparkalog_hip_store_merge_rows(sparkalog_hip_store* s, uint32_t i
         union_into(s, st->full, incoming.keys.ptr, incoming.rows);
         st->version += 1;
     }
+    return SPARKALOG_HIP_OK;
+    SPK_GUARD_END
+}
+
+extern "C" int sparkalog_hip_store_remove_ranges(sparkalog_hip_store* s, uint32_t id, const uint32_t* first_bounds,
+                                                 uint32_t ranges) {
+    if (!s || (ranges && !first_bounds)) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    for (uint32_t j = 0; j < ranges; ++j) {
+        if (first_bounds[2 * j] > first_bounds[2 * j + 1]) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+        if (j && first_bounds[2 * j] < first_bounds[2 * j - 1]) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    }
+    RelationState* st = find(s, id);
+    if (!st) return SPARKALOG_HIP_ERR_MISSING_RELATION;
+    SPK_GUARD_BEGIN
+    if (ranges == 0) return SPARKALOG_HIP_OK;
+    std::vector<uint64_t> probes(2 * static_cast<size_t>(ranges));
+    for (uint32_t j = 0; j < 2 * ranges; ++j) probes[j] = static_cast<uint64_t>(first_bounds[j]) << 32;
+    uint64_t* dev = s->probes(4 * static_cast<size_t>(ranges));
+    check(hipMemcpyAsync(dev, probes.data(), probes.size() * sizeof(uint64_t), hipMemcpyHostToDevice, s->stream));
+    std::vector<uint64_t> pos(probes.size());
+    // one pass per part: the rows outside every range are copied, in order,
+    // into a new buffer
+    auto strip = [&](DevRelation& r) {
+        if (r.rows == 0) return;
+        lower_bounds_kernel<<<grid_for(probes.size()), kBlock, 0, s->stream>>>(r.keys.ptr, r.rows, dev, probes.size(),
+                                                                             dev + probes.size());
+        check(hipGetLastError());
+        check(hipMemcpyAsync(pos.data(), dev + probes.size(), pos.size() * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                             s->stream));
+        check(hipStreamSynchronize(s->stream));
+        uint64_t removed = 0;
+        for (uint32_t j = 0; j < ranges; ++j) removed += pos[2 * j + 1] > pos[2 * j] ? pos[2 * j + 1] - pos[2 * j] : 0;
+        if (removed == 0) return;
+        const uint64_t n = r.rows - removed;
+        DevBuf<uint64_t> out = s->take(n ? n : 1);
+        uint64_t from = 0, to = 0;
+        for (uint32_t j = 0; j <= ranges; ++j) {
+            const uint64_t end = j < ranges ? pos[2 * j] : r.rows;
+            if (end > from) {
+                check(hipMemcpyAsync(out.ptr + to, r.keys.ptr + from, (end - from) * sizeof(uint64_t),
+                                     hipMemcpyDeviceToDevice, s->stream));
+                to += end - from;
+            }
+            if (j < ranges) from = pos[2 * j + 1] > from ? pos[2 * j + 1] : from;
+        }
+        s->give(std::move(r.keys));
+        r.keys = std::move(out);
+        r.rows = n;
+        r.touched();
+    };
+    strip(st->full);
+    for (auto& r : st->runs) strip(r);
+    if (st->recursive) strip(st->delta);
+    st->version += 1;
     return SPARKALOG_HIP_OK;
     SPK_GUARD_END
 }
@@ -1820,16 +1883,15 @@ extern "C" int sparkalog_hip_store_range_rows(sparkalog_hip_store* s, uint32_t i
     }
     std::vector<uint64_t> probes(2 * static_cast<size_t>(ranges));
     for (uint32_t j = 0; j < 2 * ranges; ++j) probes[j] = static_cast<uint64_t>(first_bounds[j]) << 32;
-    DevBuf<uint64_t> dev;
-    dev.reserve(4 * static_cast<size_t>(ranges));
-    check(hipMemcpyAsync(dev.ptr, probes.data(), probes.size() * sizeof(uint64_t), hipMemcpyHostToDevice, s->stream));
+    uint64_t* dev = s->probes(4 * static_cast<size_t>(ranges));
+    check(hipMemcpyAsync(dev, probes.data(), probes.size() * sizeof(uint64_t), hipMemcpyHostToDevice, s->stream));
     std::vector<uint64_t> pos(probes.size());
     for (const DevRelation* r : parts) {
         if (r->rows == 0) continue;
-        lower_bounds_kernel<<<grid_for(probes.size()), kBlock, 0, s->stream>>>(r->keys.ptr, r->rows, dev.ptr,
-                                                                             probes.size(), dev.ptr + probes.size());
+        lower_bounds_kernel<<<grid_for(probes.size()), kBlock, 0, s->stream>>>(r->keys.ptr, r->rows, dev,
+                                                                             probes.size(), dev + probes.size());
         check(hipGetLastError());
-        check(hipMemcpyAsync(pos.data(), dev.ptr + probes.size(), pos.size() * sizeof(uint64_t),
+        check(hipMemcpyAsync(pos.data(), dev + probes.size(), pos.size() * sizeof(uint64_t),
                              hipMemcpyDeviceToHost, s->stream));
         check(hipStreamSynchronize(s->stream));
         for (uint32_t j = 0; j < ranges; ++j)

@@ -176,6 +176,15 @@ class RelationStore:
     def remove_range(self, id: RelationId, first_lo: int, first_hi: int) -> None:
         _check(self._lib.sparkalog_hip_store_remove_range(self._h, int(id), first_lo, first_hi))
 
+    def remove_ranges(self, id: RelationId, bounds) -> None:
+        """Remove every row whose first column is in one of the [lo, hi)
+        ranges (a flat array lo0, hi0, lo1, hi1, ... ascending, disjoint)."""
+        b = np.ascontiguousarray(bounds, dtype=np.uint32)
+        if b.size % 2:
+            raise ValueError("remove_ranges: bounds come in (lo, hi) pairs")
+        _check(self._lib.sparkalog_hip_store_remove_ranges(self._h, int(id), b.ctypes.data_as(_native.u32p),
+                                                           b.size // 2))
+
     def set_tuning(self, merge_ratio: int = 4, segmented_distinct: bool = True, indexed_anti_join: bool = False) -> None:
         """Run-merge ratio of FULL's sorted runs, the grouped (per-run LDS)
         distinct and the first-column-indexed anti-join; results are

@@ -10,12 +10,17 @@ and runs one semi-naive step at a time over all of them:
     into the edge relation and their seeds join FULL (as a run of their own)
     and DELTA;
   * after a step, every instance whose DELTA is empty has converged: its
-    rows and digest are read from FULL, its rows are removed from the store,
-    and its slot is freed.
+    rows and digest are read from FULL; its rows are removed later, together
+    with those of other converged instances, and only then is its slot
+    reused.
 
 So requests that arrive while others are on the GPU join the running batch
 at the next step, and their work overlaps on the device. Latency is measured
 from arrival to the moment its result has been read.
+
+Requests can be split into size-class lanes (serve_lanes), each a
+continuous batch of its own on its own stream and host thread, so that small
+requests do not step at the pace of the largest instances.
 
 The plain baseline serves the same stream one request at a time, in arrival
 order, each with its own executor run.
@@ -23,6 +28,7 @@ order, each with its own executor run.
 from __future__ import annotations
 
 import math
+import threading
 import time
 from dataclasses import dataclass
 
@@ -112,9 +118,10 @@ def serve_plain(requests: list[Request], store: RelationStore) -> tuple[list[Ser
     return results, time.perf_counter() - t0
 
 
-def serve_batched(requests: list[Request], store: RelationStore, budget_pairs: float = 6e9,
-                  max_slots: int = 256) -> tuple[list[Served], float]:
-    bits = max(1, max(r.nodes for r in requests) - 1).bit_length()
+def _lane(requests: list[Request], store: RelationStore, t0: float, budget_pairs: float, max_slots: int,
+          bits: int, results: list[Served]) -> None:
+    """One continuous-batching loop over `requests` (arrival order) on
+    `store`'s stream; appends to `results`."""
     if (max_slots << bits) > (1 << 28):
         max_slots = (1 << 28) >> bits
     store.clear()
@@ -125,12 +132,21 @@ def serve_batched(requests: list[Request], store: RelationStore, budget_pairs: f
     ex.begin(store)
     free = list(range(max_slots))[::-1]
     active = {}  # slot -> (request, admitted time, estimate)
+    dead: list[int] = []  # converged slots whose rows are still stored
+    dead_pairs = 0
     pending = list(requests)
-    results = []
     used = 0.0
-    t0 = time.perf_counter()
     while pending or active:
         now = time.perf_counter() - t0
+        # converged instances' rows are removed in batches: all dead slots in
+        # one pass over each relation, when slots run out or their rows grow
+        if dead and ((not free and pending) or dead_pairs > budget_pairs / 4):
+            bounds = np.array([[s << bits, (s + 1) << bits] for s in sorted(dead)], dtype=np.uint32).reshape(-1)
+            store.remove_ranges(PATH, bounds)
+            store.remove_ranges(EDGE, bounds)
+            free.extend(dead)
+            dead.clear()
+            dead_pairs = 0
         # admission between steps, in arrival order
         while pending and pending[0].arrival <= now and free:
             req = pending[0]
@@ -160,11 +176,68 @@ def serve_batched(requests: list[Request], store: RelationStore, budget_pairs: f
             lo, hi = slot << bits, (slot + 1) << bits
             req, admitted, est = active.pop(slot)
             rows, digest = store.range_stats(PATH, 0, lo, hi, lo)
-            store.remove_range(PATH, lo, hi)
-            store.remove_range(EDGE, lo, hi)
             results.append(Served(req.rid, req.arrival, admitted, time.perf_counter() - t0, rows, digest))
+            # its rows stay until the next batched removal: with an empty
+            # DELTA they take part in no join, and no new instance uses the
+            # slot before they are gone
             used -= est
-            free.append(slot)
+            dead.append(slot)
+            dead_pairs += rows
     ex.end(store)
+
+
+def _instance_bits(requests: list[Request]) -> int:
+    return max(1, max(r.nodes for r in requests) - 1).bit_length()
+
+
+def serve_batched(requests: list[Request], store: RelationStore, budget_pairs: float = 6e9,
+                  max_slots: int = 256) -> tuple[list[Served], float]:
+    """Every request in one continuous batch on one stream."""
+    results: list[Served] = []
+    t0 = time.perf_counter()
+    _lane(requests, store, t0, budget_pairs, max_slots, _instance_bits(requests), results)
     results.sort(key=lambda r: r.rid)
     return results, time.perf_counter() - t0
+
+
+def serve_lanes(requests: list[Request], stores: list[RelationStore], bounds: list[float],
+                budget_pairs: float = 6e9, max_slots: int = 256,
+                lane_slots: list[int] | None = None) -> tuple[list[Served], float]:
+    """Continuous batching in size classes: request r goes to the first lane
+    whose bound exceeds its estimated closure (the last lane takes the rest).
+    Every request of a component graph takes about as many steps as its
+    diameter, and a batch step costs about as much as its largest instances,
+    so small requests get lanes of their own; each lane batches on its own
+    store and stream, driven by its own host thread (the native calls release
+    the interpreter lock), and the HBM budget is split evenly. lane_slots
+    caps the instances in flight per lane (default max_slots each); a cap of
+    1 serves that lane's requests one at a time, in arrival order."""
+    if len(stores) != len(bounds) + 1:
+        raise ValueError("serve_lanes: one store per lane (len(bounds) + 1)")
+    lanes: list[list[Request]] = [[] for _ in stores]
+    for req in requests:
+        est = batch.estimate_closure(req.nodes, req.src.size)
+        lane = next((i for i, b in enumerate(bounds) if est < b), len(bounds))
+        lanes[lane].append(req)
+    bits = _instance_bits(requests)
+    results: list[list[Served]] = [[] for _ in stores]
+    errors: list[BaseException] = []
+    t0 = time.perf_counter()
+
+    def run(i: int) -> None:
+        try:
+            if lanes[i]:
+                slots = lane_slots[i] if lane_slots else max_slots
+                _lane(lanes[i], stores[i], t0, budget_pairs / len(stores), slots, bits, results[i])
+        except BaseException as e:  # re-raised on the calling thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=run, args=(i,)) for i in range(len(stores))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if errors:
+        raise errors[0]
+    merged = sorted((r for lane in results for r in lane), key=lambda r: r.rid)
+    return merged, time.perf_counter() - t0

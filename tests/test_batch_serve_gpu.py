@@ -13,8 +13,10 @@ import numpy as np
 import pytest
 
 import oracle
-from sparkalog_amd import batch, serving, workload
+from sparkalog_amd import batch, plans, serving, workload
 from sparkalog_amd.recursion import RelationStore
+
+R = plans.RelationId
 
 pytestmark = pytest.mark.gpu
 GOLDEN = Path(__file__).resolve().parent / "golden"
@@ -72,3 +74,36 @@ def test_served_alone_equals_mix(stream):
         assert (m.rows, m.digest) == (bfs["pairs"], bfs["digest"]), req.rid
     # requests overlapped: some were admitted before an earlier one finished
     assert any(mixed[i + 1].admitted < mixed[i].done for i in range(len(mixed) - 1))
+
+
+def test_served_lanes_equal_alone(stream):
+    # three size-class lanes, each on its own stream and host thread
+    import torch
+    from sparkalog_amd.execution import HipStream
+
+    reqs = serving.make_requests(30, rate_per_s=600.0, seed=0x5F, min_log2_edges=8, max_log2_edges=14)
+    stores = [RelationStore(HipStream(torch.cuda.Stream())) for _ in range(3)]
+    mixed, _ = serving.serve_lanes(reqs, stores, bounds=[2e4, 2e5], budget_pairs=1e9, max_slots=64)
+    alone, _ = serving.serve_plain(reqs, RelationStore(stream))
+    assert [r.rid for r in mixed] == list(range(30))
+    for m, a, req in zip(mixed, alone, reqs):
+        assert (m.rows, m.digest) == (a.rows, a.digest), req.rid
+
+
+def test_remove_ranges_matches_filter(stream):
+    rng = np.random.default_rng(0x77)
+    a = rng.integers(0, 4000, 20000).astype(np.uint32)
+    b = rng.integers(0, 4000, 20000).astype(np.uint32)
+    key = np.unique((a.astype(np.uint64) << 32) | b)
+    cols = ((key >> 32).astype(np.uint32), (key & 0xFFFFFFFF).astype(np.uint32))
+    store = RelationStore(stream)
+    store.insert_recursive(R(1), cols, cols)
+    bounds = np.array([0, 10, 500, 777, 1000, 1001, 3990, 5000], dtype=np.uint32)
+    store.remove_ranges(R(1), bounds)
+    x = cols[0]
+    keep = np.ones(x.size, bool)
+    for lo, hi in bounds.reshape(-1, 2):
+        keep &= ~((x >= lo) & (x < hi))
+    full = store.full(R(1))
+    assert np.array_equal(full[0], cols[0][keep]) and np.array_equal(full[1], cols[1][keep])
+    assert store.rows(R(1), 1) == int(keep.sum())
