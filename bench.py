@@ -37,6 +37,8 @@ def parse_args():
     p.add_argument("--nodes", type=int, default=1 << 20)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--merge-ratio", type=int, default=4, help="tc: FULL run-merge ratio (results are identical)")
+    p.add_argument("--host-io", action="store_true",
+                   help="tc: also time one fixpoint from host arrays to the closure back in host memory (PCIe)")
     p.add_argument("--indexed-anti-join", action="store_true",
                    help="tc: anti-join FULL through its first-column index (results are identical)")
     p.add_argument("--cpu-sample-nodes", type=int, default=1 << 16)
@@ -305,6 +307,23 @@ def main():
                 for k, v in ops.items() if v["launches"]},
         "roofline": roofline,
     }
+    if args.host_io:
+        # edges uploaded from host arrays (pinned staging), fixpoint, closure
+        # read back to host memory: the rate a caller handing over host
+        # buffers sees
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        store.clear()
+        store.insert_static(R(0), (src, dst))
+        store.insert_recursive(R(1), (src, dst), (src, dst))
+        RecursiveExecutor.compile(plan, store).run(store, 10_000)
+        t1 = time.perf_counter()
+        host_full = store.full(R(1))
+        t2 = time.perf_counter()
+        line["host_io"] = {"seconds": round(t2 - t0, 4), "upload_and_fixpoint_s": round(t1 - t0, 4),
+                           "readback_s": round(t2 - t1, 4), "rows": int(host_full[0].size),
+                           "tuples_per_s_pcie_inclusive": round(host_full[0].size / (t2 - t0), 1)}
+        del host_full
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample_nodes)
     print(json.dumps(line), flush=True)

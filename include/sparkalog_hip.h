@@ -293,6 +293,15 @@ int sparkalog_hip_store_merge_rows(sparkalog_hip_store* store, uint32_t id, int 
 
 /* Remove every row whose first column is in [first_lo, first_hi) from FULL
  * and DELTA. */
+/* A non-recursive rule with stratified negation, target(x, y) :- left(x, y),
+ * !right(x, y), over two stored relations (FULL of each): target becomes a
+ * static relation holding left \ right in order (SortedBinaryAntiJoin,
+ * crates/sparkalog-execution/src/lib.rs:672-710, as the general path's
+ * negation against an earlier stratum, crates/sparkalog-datalog/src/
+ * general.rs:522-536). Writes the row count to *rows (may be null). */
+int sparkalog_hip_store_difference(sparkalog_hip_store* store, uint32_t target, uint32_t left, uint32_t right,
+                                   uint64_t* rows);
+
 int sparkalog_hip_store_remove_range(sparkalog_hip_store* store, uint32_t id, uint32_t first_lo, uint32_t first_hi);
 /* Remove the rows of several ranges at once: first_bounds holds `ranges`
  * pairs [lo, hi), ascending and disjoint; each part of FULL and DELTA is

@@ -82,6 +82,7 @@ _HIP_SIGNATURES = {
     "sparkalog_hip_store_range_rows": (c_int, [c_void_p, c_uint32, c_int, c_void_p, c_uint32, c_void_p]),
     "sparkalog_hip_store_merge_rows": (c_int, [c_void_p, c_uint32, c_int, c_void_p, c_void_p, c_uint64, c_int]),
     "sparkalog_hip_store_remove_range": (c_int, [c_void_p, c_uint32, c_uint32, c_uint32]),
+    "sparkalog_hip_store_difference": (c_int, [c_void_p, c_uint32, c_uint32, c_uint32, u64p]),
     "sparkalog_hip_store_remove_ranges": (c_int, [c_void_p, c_uint32, u32p, c_uint32]),
     "sparkalog_hip_last_hip_error": (c_int, []),
     "sparkalog_hip_status_string": (ctypes.c_char_p, [c_int]),

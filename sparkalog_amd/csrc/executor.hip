@@ -1126,6 +1126,54 @@ extern "C" int sparkalog_hip_store_insert_static(sparkalog_hip_store* s, uint32_
     SPK_GUARD_END
 }
 
+extern "C" int sparkalog_hip_store_difference(sparkalog_hip_store* s, uint32_t target, uint32_t left, uint32_t right,
+                                              uint64_t* rows) {
+    if (!s) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    if (!find(s, left) || !find(s, right)) return SPARKALOG_HIP_ERR_MISSING_RELATION;
+    if (target == left || target == right) return SPARKALOG_HIP_ERR_DUPLICATE_RELATION;
+    SPK_GUARD_BEGIN
+    // the target's slot first: creating it may move every relation state
+    RelationState* ts = slot(s, target);
+    RelationState* ls = find(s, left);
+    RelationState* rs = find(s, right);
+    // both inputs as one sorted unique array each
+    if (!ls->runs.empty()) settle_runs(s, *ls, true);
+    if (!rs->runs.empty()) settle_runs(s, *rs, true);
+    const uint64_t nl = ls->full.rows, nr = rs->full.rows;
+    const uint32_t left_max = ls->max_value;
+    ts->full.keys.reserve(nl ? nl : 1);
+    ts->full.rows = 0;
+    ts->full.touched();
+    ts->present = true;
+    ts->recursive = false;
+    ts->max_value = left_max;
+    s->counters_dev.reserve(sizeof(Counters) / sizeof(uint64_t));
+    Counters* cdev = reinterpret_cast<Counters*>(s->counters_dev.ptr);
+    AntiRight rights{};
+    rights.count = 0;
+    if (nr) {
+        rights.in[0] = PairInput{rs->full.keys.ptr, nullptr, nullptr, nullptr};
+        rights.rows[0] = nr;
+        rights.count = 1;
+    }
+    const size_t tb = anti_join_temp_bytes(nl, rights.count);
+    void* temp = s->scratch(tb);
+    PairInput a{ls->full.keys.ptr, nullptr, nullptr, nullptr};
+    PairOutput o{ts->full.keys.ptr, nullptr, nullptr};
+    const int ev = s->begin_op();
+    check(sorted_anti_join_multi(a, nl, rights, o, nullptr, &cdev->merge_rows, temp, tb, s->stream,
+                                 s->kernel_timer(ev)));
+    s->end_op(ev, kOpAntiJoin, 8.0 * static_cast<double>(nl + nr));
+    uint64_t n = 0;
+    check(hipMemcpyAsync(&n, &cdev->merge_rows, sizeof(n), hipMemcpyDeviceToHost, s->stream));
+    check(hipStreamSynchronize(s->stream));
+    if (s->profiling) s->collect_ops();
+    ts->full.rows = n;
+    if (rows) *rows = n;
+    return SPARKALOG_HIP_OK;
+    SPK_GUARD_END
+}
+
 extern "C" int sparkalog_hip_store_insert_recursive(sparkalog_hip_store* s, uint32_t id, const uint32_t* full0,
                                                     const uint32_t* full1, uint64_t full_rows, const uint32_t* delta0,
                                                     const uint32_t* delta1, uint64_t delta_rows, int on_device) {

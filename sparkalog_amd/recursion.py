@@ -173,6 +173,13 @@ class RelationStore:
         c0, c1, n, dev, _keep = _columns(relation)
         _check(self._lib.sparkalog_hip_store_merge_rows(self._h, int(id), which, c0, c1, n, dev))
 
+    def difference(self, target: RelationId, left: RelationId, right: RelationId) -> int:
+        """target := FULL(left) \\ FULL(right) as a static relation (a
+        non-recursive rule with stratified negation); returns its rows."""
+        n = ctypes.c_uint64(0)
+        _check(self._lib.sparkalog_hip_store_difference(self._h, int(target), int(left), int(right), ctypes.byref(n)))
+        return n.value
+
     def remove_range(self, id: RelationId, first_lo: int, first_hi: int) -> None:
         _check(self._lib.sparkalog_hip_store_remove_range(self._h, int(id), first_lo, first_hi))
 

@@ -202,3 +202,24 @@ def test_tuning_does_not_change_results(merge_ratio, segmented, indexed, expecte
     assert np.array_equal(full[0], bfs["closure"][0]) and np.array_equal(full[1], bfs["closure"][1])
     assert summ.iterations == e["max_dist"]
     assert summ.total_new_rows == e["path_rows"] - e["edges"]
+
+
+def test_program_with_negation_matches_general_path(expected, stream):
+    # path = TC(edge), nonlocal = path \ edge: the general CPU path's program
+    from sparkalog_amd import program
+
+    e = expected["random_dag_1000_4000"]
+    edges = workload.random_dag()
+    store = RelationStore(stream)
+    store.insert_static(R(0), edges)
+    store.insert_recursive(R(1), edges, edges)
+    summ = program.run_program(program.transitive_closure_nonlocal(R(1), R(0), R(2)), store)
+    g = oracle.general_tc(*edges)
+    nl = store.full(R(2))
+    assert np.array_equal(nl[0], g["nonlocal"][0]) and np.array_equal(nl[1], g["nonlocal"][1])
+    assert summ.derived_rows[2] == e["nonlocal_rows"]
+    assert summ.scc[0].iterations == g["iterations"]
+    # an empty negative side keeps every row; an empty positive side gives none
+    store.insert_static(R(3), (np.empty(0, np.uint32), np.empty(0, np.uint32)))
+    assert store.difference(R(4), R(0), R(3)) == edges[0].size
+    assert store.difference(R(5), R(3), R(0)) == 0
