@@ -33,6 +33,7 @@ constexpr int kMergeItems = 8;
 constexpr int kMergeTile = kBlock * kMergeItems;     // 2048 merged rows
 constexpr int kAntiTile = 2048;                      // left rows per tile
 constexpr int kAntiStage = 2048;                     // right rows staged in LDS per chunk
+constexpr int kAntiCompactTile = 4096;               // rows per compaction tile of the anti-join
 // Above this many right rows per 2048-row left tile, reading the range costs
 // more HBM traffic than ~log2(span) probes per left row.
 constexpr size_t kAntiStreamLimit = 1u << 18;
@@ -667,32 +668,25 @@ __global__ void anti_partition_kernel(PairInput left, size_t nl, AntiRight right
     }
 }
 
-// left \ (right_0 ∪ right_1 ∪ ...), keeping left order. Layout: wave w owns
-// tile rows [w*512, w*512+512); item i of lane j is row w*512 + i*64 + j, so
-// every load and the ballot compaction are coalesced and order-preserving.
-// Each right input's candidate range for the tile comes from the partition
-// pass (or, for a tile that is not sorted, from a block-cooperative search);
-// ranges are streamed through LDS once, or probed in place when huge.
+// left \ (right_0 ∪ right_1 ∪ ...), first pass: which left rows some right
+// input holds. Layout: wave w owns tile rows [w*512, w*512+512); item i of
+// lane j is row w*512 + i*64 + j, so every load is coalesced and an item's
+// result is one ballot word. Each right input's candidate range for the tile
+// comes from the partition pass (or, for a tile that is not sorted, from a
+// block-cooperative search); ranges are streamed through LDS once, or probed
+// in place when huge. Tiles are independent: the time a tile takes depends
+// on its ranges, and a single pass with look-back would keep finished tiles
+// waiting for slower predecessors (measured: over a third of the time).
 __global__ __launch_bounds__(kBlock) void anti_join_kernel(PairInput left, size_t nl, AntiRight rights,
-                                                           const size_t* ranges, PairOutput out, uint32_t* count32,
-                                                           uint64_t* count64, uint64_t* status, Header* header) {
+                                                           const size_t* ranges, uint64_t* found_words) {
     constexpr int kItems = kAntiTile / kBlock;  // 8
     constexpr int kWaveRows = kItems * kWave;   // 512
     __shared__ uint64_t stage[kAntiStage];
     __shared__ uint32_t wave_totals[kWavesPerBlock][kItems];
     __shared__ uint64_t red_min[kWavesPerBlock], red_max[kWavesPerBlock];
-    __shared__ uint64_t offset_slot;
-    __shared__ uint32_t tile_slot;
     nl = actual_rows(left, nl);
-    const uint32_t tile = next_tile(&header->counters[0], &tile_slot);
+    const uint32_t tile = blockIdx.x;
     const size_t base = static_cast<size_t>(tile) * kAntiTile;
-    if (nl == 0) {
-        if (tile == 0 && threadIdx.x == 0) {
-            if (count32) *count32 = 0;
-            if (count64) *count64 = 0;
-        }
-        return;
-    }
     if (base >= nl) return;
     const uint32_t lane = lane_id(), wave = wave_id();
     const size_t rows = nl - base < static_cast<size_t>(kAntiTile) ? nl - base : kAntiTile;
@@ -776,39 +770,71 @@ __global__ __launch_bounds__(kBlock) void anti_join_kernel(PairInput left, size_
             }
         }
     }
-    // order-preserving compaction of the rows not found
-    uint64_t masks[kItems];
-    uint32_t wave_count = 0;
+    // found rows as one 64-bit word per item: bit j of word
+    // (base + wave*512 + i*64) / 64 is row base + wave*512 + i*64 + j
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
-        const size_t row = wave * kWaveRows + i * kWave + lane;
-        masks[i] = ballot(row < rows && !((found >> i) & 1u));
+        const uint64_t m = ballot(((found >> i) & 1u) != 0);
+        const size_t row0 = base + wave * kWaveRows + i * kWave;
+        if (lane == 0 && row0 < nl) found_words[row0 / kWave] = m;
+    }
+}
+
+// Order-preserving compaction of the left rows whose found bit is clear, at
+// offsets from decoupled look-back (the same work for every tile, so no tile
+// waits long for another).
+__global__ __launch_bounds__(kBlock) void anti_compact_kernel(PairInput left, size_t nl, const uint64_t* found_words,
+                                                              PairOutput out, uint32_t* count32, uint64_t* count64,
+                                                              uint64_t* status, Header* header) {
+    constexpr int kItems = kAntiCompactTile / kBlock;  // 16
+    constexpr int kWaveRows = kItems * kWave;          // 1024
+    __shared__ uint32_t wave_totals[kWavesPerBlock][kItems];
+    __shared__ uint64_t offset_slot;
+    __shared__ uint32_t tile_slot;
+    nl = actual_rows(left, nl);
+    const uint32_t tile = next_tile(&header->counters[0], &tile_slot);
+    const size_t base = static_cast<size_t>(tile) * kAntiCompactTile;
+    if (nl == 0) {
+        if (tile == 0 && threadIdx.x == 0) {
+            if (count32) *count32 = 0;
+            if (count64) *count64 = 0;
+        }
+        return;
+    }
+    if (base >= nl) return;
+    const uint32_t lane = lane_id(), wave = wave_id();
+    const uint32_t rows = static_cast<uint32_t>(nl - base < static_cast<size_t>(kAntiCompactTile) ? nl - base
+                                                                                               : kAntiCompactTile);
+    uint64_t k[kItems], masks[kItems];
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        const uint32_t r = wave * kWaveRows + i * kWave + lane;
+        const size_t row0 = base + wave * kWaveRows + i * kWave;  // multiple of 64
+        const uint64_t words = row0 < nl ? found_words[row0 / kWave] : ~0ull;
+        k[i] = r < rows ? load_key(left, base + r) : 0;
+        masks[i] = ballot(r < rows && !((words >> lane) & 1ull));
         if (lane == 0) wave_totals[wave][i] = static_cast<uint32_t>(__popcll(masks[i]));
-        wave_count += static_cast<uint32_t>(__popcll(masks[i]));
     }
     __syncthreads();
     uint32_t wave_start = 0, tile_total = 0;
-    for (int w = 0; w < kWavesPerBlock; ++w) {
-        uint32_t c = 0;
-        for (int i = 0; i < kItems; ++i) c += wave_totals[w][i];
-        if (w < static_cast<int>(wave)) wave_start += c;
-        tile_total += c;
-    }
-    (void)wave_count;
+    for (int w = 0; w < kWavesPerBlock; ++w)
+        for (int i = 0; i < kItems; ++i) {
+            if (w < static_cast<int>(wave)) wave_start += wave_totals[w][i];
+            tile_total += wave_totals[w][i];
+        }
     if (wave == 0) {
         const uint64_t excl = wave_lookback(status, tile, tile_total, 0, &header->error);
         if (lane == 0) offset_slot = excl;
     }
+    __syncthreads();
+    const uint64_t offset = offset_slot;
     uint32_t pos = wave_start;
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
-        if ((masks[i] >> lane) & 1ull) stage[pos + lanes_below(masks[i])] = k[i];
+        if ((masks[i] >> lane) & 1ull) store_key(out, offset + pos + lanes_below(masks[i]), k[i]);
         pos += static_cast<uint32_t>(__popcll(masks[i]));
     }
-    __syncthreads();
-    const uint64_t offset = offset_slot;
-    for (uint32_t i = threadIdx.x; i < tile_total; i += kBlock) store_key(out, offset + i, stage[i]);
-    if (tile == tiles_of(nl, kAntiTile) - 1 && threadIdx.x == 0) {
+    if (base + rows == nl && threadIdx.x == 0) {
         if (count32) *count32 = static_cast<uint32_t>(offset + tile_total);
         if (count64) *count64 = offset + tile_total;
     }
@@ -1178,7 +1204,9 @@ hipError_t merge_disjoint(const uint64_t* left, size_t left_rows, const uint64_t
 
 size_t anti_join_temp_bytes(size_t left_rows, uint32_t right_inputs) {
     const size_t tiles = tiles_of(left_rows, kAntiTile);
-    return compact_temp(tiles) + align_up(tiles * (right_inputs ? right_inputs : 1) * 2 * sizeof(size_t), 256);
+    return compact_temp(tiles_of(left_rows, kAntiCompactTile)) +
+           align_up(tiles * (right_inputs ? right_inputs : 1) * 2 * sizeof(size_t), 256) +
+           align_up(tiles * (kAntiTile / kWave) * sizeof(uint64_t), 256);
 }
 
 hipError_t sorted_anti_join_multi(PairInput left, size_t left_rows, const AntiRight& rights, PairOutput out,
@@ -1191,19 +1219,25 @@ hipError_t sorted_anti_join_multi(PairInput left, size_t left_rows, const AntiRi
     }
     if (rights.count > static_cast<uint32_t>(kMaxAntiRights)) return hipErrorInvalidValue;
     const size_t tiles = tiles_of(left_rows, kAntiTile);
+    const size_t compact_tiles = tiles_of(left_rows, kAntiCompactTile);
     if (temp_bytes < anti_join_temp_bytes(left_rows, rights.count)) return hipErrorInvalidValue;
-    hipError_t st = prepare_compact(temp, tiles, stream);
+    hipError_t st = prepare_compact(temp, compact_tiles, stream);
     if (st != hipSuccess) return st;
-    size_t* ranges = reinterpret_cast<size_t*>(static_cast<char*>(temp) + compact_temp(tiles));
+    char* t = static_cast<char*>(temp) + compact_temp(compact_tiles);
+    size_t* ranges = reinterpret_cast<size_t*>(t);
+    uint64_t* found_words = reinterpret_cast<uint64_t*>(
+        t + align_up(tiles * (rights.count ? rights.count : 1) * 2 * sizeof(size_t), 256));
     if (rights.count) {
         anti_partition_kernel<<<grid_for(tiles * rights.count, kBlock, 4096), kBlock, 0, stream>>>(
             left, left_rows, rights, tiles, ranges);
     }
     timer_start(timer, stream);
-    anti_join_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(
-        left, left_rows, rights, ranges, out, count32, count64,
-        reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp));
+    anti_join_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(left, left_rows, rights, ranges,
+                                                                         found_words);
     timer_stop(timer, stream);
+    anti_compact_kernel<<<static_cast<unsigned>(compact_tiles), kBlock, 0, stream>>>(
+        left, left_rows, found_words, out, count32, count64,
+        reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp));
     return hipGetLastError();
 }
 
