@@ -192,9 +192,11 @@ __device__ __forceinline__ uint32_t column_of(uint64_t key, uint32_t col) {
 // wave w owns tile rows [w*512, w*512+512), item i of lane j is row
 // w*512 + i*64 + j, so loads are coalesced and the row-order prefix is one
 // wave scan per item plus one block step.
+// Join offsets of the delta rows (exclusive scan of their index matches)
+// and, for the emit, each row's first matching index row.
 __global__ __launch_bounds__(kBlock) void dense_join_count_kernel(const uint64_t* delta, size_t n, uint32_t key_col,
                                                                   const uint32_t* starts, uint32_t n_keys,
-                                                                  uint64_t* offsets, uint64_t* total,
+                                                                  uint64_t* offsets, uint32_t* match, uint64_t* total,
                                                                   uint64_t* status, uint32_t* counter,
                                                                   uint32_t* error) {
     constexpr int kItems = 8;
@@ -214,7 +216,12 @@ __global__ __launch_bounds__(kBlock) void dense_join_count_kernel(const uint64_t
         uint64_t cnt = 0;
         if (row < n) {
             const uint32_t key = column_of(delta[row], key_col);
-            if (key < n_keys) cnt = starts[key + 1] - starts[key];
+            uint32_t m0 = 0;
+            if (key < n_keys) {
+                m0 = starts[key];
+                cnt = starts[key + 1] - m0;
+            }
+            match[row] = m0;
         }
         const uint64_t incl = wave_inclusive_sum<uint64_t>(cnt);
         c[i] = running + incl - cnt;
@@ -281,10 +288,12 @@ __global__ void emit_partition_kernel(const uint64_t* offsets, size_t n, size_t 
 // Output-balanced emit of the dense join, writing compressed sort keys. Each
 // tile stages its delta rows in LDS, marks where each row's outputs begin,
 // and a block max-scan gives every output position its row.
+// match_base[row] = starts[key] of each delta row (written by the count),
+// so the emit reads it in row order instead of gathering the index's starts.
 __global__ __launch_bounds__(kBlock) void dense_join_emit_kernel(const uint64_t* delta, size_t n,
-                                                                 const uint32_t* starts, uint32_t n_keys,
                                                                  const uint64_t* right_sorted, const uint64_t* offsets,
-                                                                 size_t total, const size_t* row_start, EmitSpec spec,
+                                                                 const uint32_t* match_base, size_t total,
+                                                                 const size_t* row_start, EmitSpec spec,
                                                                  uint64_t* out) {
     constexpr int kPer = kEmitTile / kBlock;  // 8 output positions per thread
     __shared__ uint64_t row_offset[kEmitStage];
@@ -306,9 +315,9 @@ __global__ __launch_bounds__(kBlock) void dense_join_emit_kernel(const uint64_t*
             for (uint32_t i = threadIdx.x; i < span; i += kBlock) {
                 const uint64_t dk = delta[l0 + i];
                 const uint64_t off = offsets[l0 + i];
-                const uint32_t key = column_of(dk, spec.key_col);
-                const uint32_t m0 = key < n_keys ? starts[key] : 0;
-                const uint32_t cnt = key < n_keys ? starts[key + 1] - m0 : 0;
+                const uint32_t m0 = match_base[l0 + i];
+                const uint64_t next = l0 + i + 1 < n ? offsets[l0 + i + 1] : total;
+                const uint64_t cnt = next - off;
                 row_offset[i] = off;
                 row_key[i] = dk;
                 row_match[i] = m0;
@@ -363,7 +372,7 @@ __global__ __launch_bounds__(kBlock) void dense_join_emit_kernel(const uint64_t*
                 }
                 dk = delta[lo - 1];
                 row_start_off = offsets[lo - 1];
-                match = starts[column_of(dk, spec.key_col)];
+                match = match_base[lo - 1];
             }
             const uint64_t r = right_sorted[match + static_cast<uint32_t>(o - row_start_off)];
             // right_sorted rows are (key, other): column right_key is the high half
@@ -464,6 +473,7 @@ struct RuleRuntime {
     CachedIndex index;
     DevBuf<uint64_t> status;    // look-back words of the join count
     DevBuf<uint64_t> offsets;   // join offsets per delta row
+    DevBuf<uint32_t> match;     // first matching index row per delta row
     DevBuf<size_t> row_start;   // first delta row of each emit tile
     DevBuf<uint64_t> cand_a, cand_b, cand_c;
     DevBuf<unsigned char> seg_meta;  // tile bounds and look-back words of the segmented distinct
@@ -1433,6 +1443,7 @@ void session_step(sparkalog_hip_store* s) {
             continue;
         }
         rt.offsets.reserve(L);
+        rt.match.reserve(L);
         const size_t n_tiles = tiles(L, kBlock * 8);
         rt.status.reserve(n_tiles);
         check(fill_u64(rt.status.ptr, n_tiles, 0, s->stream));
@@ -1440,7 +1451,7 @@ void session_step(sparkalog_hip_store* s) {
         timer_start(s->kernel_timer(ev), s->stream);
         dense_join_count_kernel<<<static_cast<unsigned>(n_tiles), kBlock, 0, s->stream>>>(
             ds.delta.keys.ptr, L, rt.plan.left_key, rt.index.starts.ptr, rt.index.n_keys, rt.offsets.ptr,
-            &cdev->join_total[r], rt.status.ptr, &lb_counters[r], lb_error);
+            rt.match.ptr, &cdev->join_total[r], rt.status.ptr, &lb_counters[r], lb_error);
         timer_stop(s->kernel_timer(ev), s->stream);
         check(hipGetLastError());
         s->end_op(ev, kOpJoinCount, 8.0 * L);
@@ -1478,9 +1489,10 @@ void session_step(sparkalog_hip_store* s) {
         emit_partition_kernel<<<grid_for(emit_tiles + 1), kBlock, 0, s->stream>>>(rt.offsets.ptr, rt.delta_rows, M,
                                                                                rt.row_start.ptr, emit_tiles);
         timer_start(s->kernel_timer(ev), s->stream);
-        dense_join_emit_kernel<<<emit_grid, kBlock, 0, s->stream>>>(
-            ds.delta.keys.ptr, rt.delta_rows, rt.index.starts.ptr, rt.index.n_keys, rt.index.sorted.ptr,
-            rt.offsets.ptr, M, rt.row_start.ptr, spec, rt.cand_a.ptr);
+        dense_join_emit_kernel<<<emit_grid, kBlock, 0, s->stream>>>(ds.delta.keys.ptr, rt.delta_rows,
+                                                                     rt.index.sorted.ptr, rt.offsets.ptr,
+                                                                     rt.match.ptr, M, rt.row_start.ptr, spec,
+                                                                     rt.cand_a.ptr);
         timer_stop(s->kernel_timer(ev), s->stream);
         check(hipGetLastError());
         s->end_op(ev, kOpJoinEmit, 8.0 * rt.index.rows + 8.0 * M);
