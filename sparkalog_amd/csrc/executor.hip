@@ -167,6 +167,12 @@ __global__ void gather_index_keys_kernel(const uint64_t* rows, size_t n, uint32_
 }
 
 // starts[v] = first sorted row with key >= v, for v in [0, n_keys].
+__global__ void low_halves_kernel(const uint64_t* in, size_t n, uint32_t* out) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        out[i] = static_cast<uint32_t>(in[i]);
+}
+
 __global__ void dense_starts_kernel(const uint64_t* sorted, size_t n, uint32_t* starts, size_t n_keys) {
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
     for (size_t v = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; v <= n_keys; v += stride) {
@@ -291,7 +297,7 @@ __global__ void emit_partition_kernel(const uint64_t* offsets, size_t n, size_t 
 // match_base[row] = starts[key] of each delta row (written by the count),
 // so the emit reads it in row order instead of gathering the index's starts.
 __global__ __launch_bounds__(kBlock) void dense_join_emit_kernel(const uint64_t* delta, size_t n,
-                                                                 const uint64_t* right_sorted, const uint64_t* offsets,
+                                                                 const uint32_t* right_other, const uint64_t* offsets,
                                                                  const uint32_t* match_base, size_t total,
                                                                  const size_t* row_start, EmitSpec spec,
                                                                  uint64_t* out) {
@@ -374,15 +380,16 @@ __global__ __launch_bounds__(kBlock) void dense_join_emit_kernel(const uint64_t*
                 row_start_off = offsets[lo - 1];
                 match = match_base[lo - 1];
             }
-            const uint64_t r = right_sorted[match + static_cast<uint32_t>(o - row_start_off)];
-            // right_sorted rows are (key, other): column right_key is the high half
+            const uint32_t r_other = right_other[match + static_cast<uint32_t>(o - row_start_off)];
+            // the right row is (join key, other); its join key equals the
+            // left row's key column
             uint32_t v[2];
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
                 if (spec.side[c] == 0)
                     v[c] = column_of(dk, spec.col[c]);
                 else
-                    v[c] = spec.col[c] == spec.right_key ? static_cast<uint32_t>(r >> 32) : static_cast<uint32_t>(r);
+                    v[c] = spec.col[c] == spec.right_key ? column_of(dk, spec.key_col) : r_other;
             }
             out[o] = (static_cast<uint64_t>(v[spec.order[0]]) << spec.shift) | v[spec.order[1]];
         }
@@ -463,6 +470,7 @@ struct CachedIndex {
     uint32_t relation = 0, column = 0;
     uint64_t version = 0;
     DevBuf<uint64_t> sorted;  // (key, other) rows sorted
+    DevBuf<uint32_t> other;   // the other column of the sorted rows (the key is the join key)
     DevBuf<uint32_t> starts;  // dense CSR over key values [0, n_keys]
     uint32_t n_keys = 0;
     uint64_t rows = 0;
@@ -767,6 +775,8 @@ void build_index(sparkalog_hip_store* s, RuleRuntime& rt, RelationState& right, 
     }
     dense_starts_kernel<<<grid_for(static_cast<size_t>(n_keys) + 1), kBlock, 0, s->stream>>>(ix.sorted.ptr, n,
                                                                                           ix.starts.ptr, n_keys);
+    ix.other.reserve(n ? n : 1);
+    if (n) low_halves_kernel<<<grid_for(n), kBlock, 0, s->stream>>>(ix.sorted.ptr, n, ix.other.ptr);
     check(hipGetLastError());
     check(hipStreamSynchronize(s->stream));
     ix.valid = true;
@@ -1490,7 +1500,7 @@ void session_step(sparkalog_hip_store* s) {
                                                                                rt.row_start.ptr, emit_tiles);
         timer_start(s->kernel_timer(ev), s->stream);
         dense_join_emit_kernel<<<emit_grid, kBlock, 0, s->stream>>>(ds.delta.keys.ptr, rt.delta_rows,
-                                                                     rt.index.sorted.ptr, rt.offsets.ptr,
+                                                                     rt.index.other.ptr, rt.offsets.ptr,
                                                                      rt.match.ptr, M, rt.row_start.ptr, spec,
                                                                      rt.cand_a.ptr);
         timer_stop(s->kernel_timer(ev), s->stream);
