@@ -482,6 +482,7 @@ struct RuleRuntime {
     DevBuf<uint64_t> status;    // look-back words of the join count
     DevBuf<uint64_t> offsets;   // join offsets per delta row
     DevBuf<uint32_t> match;     // first matching index row per delta row
+    DevBuf<uint64_t> found;     // candidates already in FULL (one bit each)
     DevBuf<size_t> row_start;   // first delta row of each emit tile
     DevBuf<uint64_t> cand_a, cand_b, cand_c;
     DevBuf<unsigned char> seg_meta;  // tile bounds and look-back words of the segmented distinct
@@ -501,6 +502,7 @@ struct TargetRuntime {
     DevBuf<uint64_t> new_run;      // NEWT's second copy, becomes a run of FULL
     uint64_t contrib_bound = 0;
     int anti_slot = -1;            // profiling event of this iteration's anti-join
+    bool fused = false;            // this iteration's distinct wrote NEWT (candidates in FULL dropped first)
     std::vector<int> union_slots;  // profiling events of this iteration's contribution unions
 };
 
@@ -1438,6 +1440,10 @@ void session_step(sparkalog_hip_store* s) {
     const bool use_x = s->x_anti && vmax < (1u << 24);
 
     clear_words_kernel<<<1, 256, 0, s->stream>>>(s->misc_dev.ptr, 256);
+    for (auto& t : targets) {
+        t.fused = false;
+        t.anti_slot = -1;
+    }
     // ---- RuleRuntime::evaluate, join count (generic.rs:197-241) ----
     for (uint32_t r = 0; r < n_rules; ++r) {
         RuleRuntime& rt = rts[r];
@@ -1506,17 +1512,59 @@ void session_step(sparkalog_hip_store* s) {
         timer_stop(s->kernel_timer(ev), s->stream);
         check(hipGetLastError());
         s->end_op(ev, kOpJoinEmit, 8.0 * rt.index.rows + 8.0 * M);
-        ev = s->begin_op();
-        rt.distinct_slot = ev;
         // The join writes candidates in DELTA order; when the distinct's major
         // column is DELTA's first column they are grouped by it already.
         const bool grouped = spec.side[spec.order[0]] == 0 && spec.col[spec.order[0]] == 0;
+        // A target fed by this rule alone takes NEWT = distinct(candidates)
+        // \ FULL as distinct(candidates \ FULL): candidates already in FULL
+        // are found by probing FULL's runs and dropped before the sort, and
+        // the distinct writes NEWT and FULL's new run itself.
+        int fuse = -1;
+        if (grouped && s->segmented_distinct && !use_x)
+            for (size_t ti = 0; ti < targets.size(); ++ti)
+                if (targets[ti].rules.size() == 1 && targets[ti].rules[0] == r) fuse = static_cast<int>(ti);
+        const uint64_t* found = nullptr;
+        if (fuse >= 0) {
+            TargetRuntime& t = targets[static_cast<size_t>(fuse)];
+            RelationState& ts = *find(s, t.target);
+            if (ts.runs.size() >= kMaxRuns) settle_runs(s, ts, true);
+            AntiRight rights{};
+            rights.count = 0;
+            double right_rows = 0;
+            auto add_right = [&](const DevRelation& x) {
+                if (!x.rows) return;
+                rights.in[rights.count] = PairInput{x.keys.ptr, nullptr, nullptr, nullptr};
+                rights.rows[rights.count] = x.rows;
+                ++rights.count;
+                right_rows += static_cast<double>(x.rows);
+            };
+            add_right(ts.full);
+            for (const auto& x : ts.runs) add_right(x);
+            if (rights.count) {
+                rt.found.reserve(probe_found_words(M));
+                const size_t ptb = probe_grouped_temp_bytes(M, rights.count);
+                void* ptemp = s->scratch(ptb);
+                const int pev = s->begin_op();
+                check(probe_grouped(rt.cand_a.ptr, nullptr, M, rt.shift, rights, rt.found.ptr, ptemp, ptb,
+                                    s->stream, s->kernel_timer(pev)));
+                s->end_op(pev, kOpAntiJoin, 8.0 * M + 8.0 * right_rows);
+                t.anti_slot = pev;
+                found = rt.found.ptr;
+            }
+            algo_bytes += 8.0 * right_rows;
+            t.newt.reserve(M);
+            t.new_run = s->take(M);
+            t.fused = true;
+            t.contrib_bound = M;
+        }
+        ev = s->begin_op();
+        rt.distinct_slot = ev;
         if (grouped && s->segmented_distinct) {
             const size_t mb = segmented_distinct_meta_bytes(M);
             rt.seg_meta.reserve(mb);
             uint64_t side_rows = 0;
             check(segmented_distinct_prepare(rt.cand_a.ptr, M, rt.shift, rt.cand_b.ptr, rt.seg_meta.ptr, mb,
-                                             &side_rows, s->stream));
+                                             &side_rows, s->stream, found));
             size_t tb = 0;
             void* temp = nullptr;
             if (side_rows) {
@@ -1525,10 +1573,15 @@ void session_step(sparkalog_hip_store* s) {
                 temp = s->scratch(tb);
             }
             PairOutput out{rt.distinct_out.ptr, nullptr, nullptr};
+            uint64_t* count = &cdev->distinct_rows[r];
+            if (fuse >= 0) {
+                TargetRuntime& t = targets[static_cast<size_t>(fuse)];
+                out = PairOutput{t.newt.ptr, nullptr, nullptr, t.new_run.ptr};
+                count = &cdev->newt_rows[fuse];
+            }
             check(segmented_distinct_finish(rt.cand_a.ptr, M, rt.shift, known_compressed, rt.cand_b.ptr,
-                                            side_rows ? rt.cand_c.ptr : nullptr, side_rows, out,
-                                            &cdev->distinct_rows[r], rt.seg_meta.ptr, mb, temp, tb, s->stream,
-                                            s->kernel_timer(ev)));
+                                            side_rows ? rt.cand_c.ptr : nullptr, side_rows, out, count,
+                                            rt.seg_meta.ptr, mb, temp, tb, s->stream, s->kernel_timer(ev), found));
         } else {
             const size_t tb = distinct_temp_bytes(M);
             void* temp = s->scratch(tb);
@@ -1546,6 +1599,7 @@ void session_step(sparkalog_hip_store* s) {
     // ---- TargetRuntime::combine (generic.rs:299-358) ----
     for (auto& t : targets) {
         t.union_slots.clear();
+        if (t.fused) continue;  // NEWT written by the distinct
         RelationState& ts = *find(s, t.target);
         const uint64_t* contrib = nullptr;
         const uint64_t* contrib_rows = nullptr;
@@ -1595,7 +1649,6 @@ void session_step(sparkalog_hip_store* s) {
         };
         add_right(ts.full);
         for (const auto& r : ts.runs) add_right(r);
-        t.anti_slot = -1;
         bool parts_fit = true;  // index entries are u32 row counts
         if (ts.full.rows > 0xFFFFFFFFull) parts_fit = false;
         for (const auto& r : ts.runs) parts_fit = parts_fit && r.rows <= 0xFFFFFFFFull;
@@ -1646,15 +1699,25 @@ void session_step(sparkalog_hip_store* s) {
     check(hipStreamSynchronize(s->stream));
     if (err) throw StatusError{SPARKALOG_HIP_ERR_DEVICE};
     // per-iteration algorithmic bytes of the operators run (pairs are 8 bytes)
+    auto fused_target = [&](uint32_t r) -> int {
+        for (size_t ti = 0; ti < targets.size(); ++ti)
+            if (targets[ti].fused && targets[ti].rules[0] == r) return static_cast<int>(ti);
+        return -1;
+    };
     for (uint32_t r = 0; r < n_rules; ++r) {
         const double L = static_cast<double>(rts[r].delta_rows), M = static_cast<double>(rts[r].join_rows);
-        const double U = rts[r].join_rows ? static_cast<double>(chost.distinct_rows[r]) : 0.0;
+        const int fr = fused_target(r);
+        // a fused distinct writes NEWT (twice: DELTA and FULL's new run)
+        const double U = !rts[r].join_rows ? 0.0
+                         : fr >= 0 ? 2.0 * static_cast<double>(chost.newt_rows[fr])
+                                   : static_cast<double>(chost.distinct_rows[r]);
         const double R = static_cast<double>(rts[r].index.rows);
         if (L > 0) algo_bytes += 8 * L + 8 * R + 8 * M + (M > 0 ? 8 * M + 8 * U : 0.0);
         s->add_op_bytes(rts[r].distinct_slot, 8 * U);
     }
     for (uint32_t i = 0; i < n_relations; ++i) {
         TargetRuntime& t = targets[i];
+        if (t.fused) continue;  // counted with the rule's distinct and probe
         const uint64_t C = t.rules.size() > 1 ? chost.contrib_rows[t.rules.back()]
                                               : (t.contrib_bound ? chost.distinct_rows[t.rules[0]] : 0);
         // unions of the rules' contributions: 8 bytes per input and output row

@@ -646,8 +646,17 @@ __device__ __forceinline__ size_t lower_bound_key(const PairInput& r, size_t lo,
 // For every left tile and every right input: the right rows that can match a
 // sorted tile, [lower_bound(first key), upper_bound(last key)). One thread per
 // (tile, right input).
+// Left key i as a pair (first << 32 | second). With minor_bits != 0 the left
+// input holds compressed keys (first << minor_bits | second) of candidate
+// rows grouped by their first column.
+__device__ __forceinline__ uint64_t left_pair(const PairInput& in, size_t i, uint32_t minor_bits) {
+    const uint64_t k = load_key(in, i);
+    if (minor_bits == 0) return k;
+    return ((k >> minor_bits) << 32) | (k & ((1ull << minor_bits) - 1));
+}
+
 __global__ void anti_partition_kernel(PairInput left, size_t nl, AntiRight rights, size_t n_tiles,
-                                      size_t* ranges) {
+                                      size_t* ranges, uint32_t minor_bits) {
     nl = actual_rows(left, nl);
     const size_t tiles = tiles_of(nl, kAntiTile) < n_tiles ? tiles_of(nl, kAntiTile) : n_tiles;
     const size_t jobs = tiles * rights.count;
@@ -659,7 +668,11 @@ __global__ void anti_partition_kernel(PairInput left, size_t nl, AntiRight right
         const size_t nr = rights.rows[r];
         const size_t first = t * kAntiTile;
         const size_t last = (first + kAntiTile < nl ? first + kAntiTile : nl) - 1;
-        const uint64_t lo_key = load_key(left, first), hi_key = load_key(left, last);
+        uint64_t lo_key = left_pair(left, first, minor_bits), hi_key = left_pair(left, last, minor_bits);
+        if (minor_bits) {  // grouped tile: every pair of its first and last first values
+            lo_key &= ~0xFFFFFFFFull;
+            hi_key |= 0xFFFFFFFFull;
+        }
         const size_t lo = lower_bound_key(right, 0, nr, lo_key);
         size_t hi = lower_bound_key(right, lo, nr, hi_key);
         if (hi < nr && load_key(right, hi) == hi_key) ++hi;
@@ -678,7 +691,8 @@ __global__ void anti_partition_kernel(PairInput left, size_t nl, AntiRight right
 // on its ranges, and a single pass with look-back would keep finished tiles
 // waiting for slower predecessors (measured: over a third of the time).
 __global__ __launch_bounds__(kBlock) void anti_join_kernel(PairInput left, size_t nl, AntiRight rights,
-                                                           const size_t* ranges, uint64_t* found_words) {
+                                                           const size_t* ranges, uint64_t* found_words,
+                                                           uint32_t minor_bits) {
     constexpr int kItems = kAntiTile / kBlock;  // 8
     constexpr int kWaveRows = kItems * kWave;   // 512
     __shared__ uint64_t stage[kAntiStage];
@@ -690,13 +704,13 @@ __global__ __launch_bounds__(kBlock) void anti_join_kernel(PairInput left, size_
     if (base >= nl) return;
     const uint32_t lane = lane_id(), wave = wave_id();
     const size_t rows = nl - base < static_cast<size_t>(kAntiTile) ? nl - base : kAntiTile;
-    const uint64_t first_key = load_key(left, base), last_key = load_key(left, base + rows - 1);
+    const uint64_t first_key = left_pair(left, base, minor_bits), last_key = left_pair(left, base + rows - 1, minor_bits);
     uint64_t k[kItems];
     uint64_t kmin = ~0ull, kmax = 0;
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         const size_t r = wave * kWaveRows + i * kWave + lane;
-        k[i] = r < rows ? load_key(left, base + r) : 0;
+        k[i] = r < rows ? left_pair(left, base + r, minor_bits) : 0;
         if (r < rows) {
             kmin = k[i] < kmin ? k[i] : kmin;
             kmax = k[i] > kmax ? k[i] : kmax;
@@ -720,7 +734,8 @@ __global__ __launch_bounds__(kBlock) void anti_join_kernel(PairInput left, size_
     }
     // a sorted tile may use the precomputed ranges; any other order gets its
     // own (pure membership semantics for every input order)
-    const bool sorted_tile = ranges != nullptr && lo_key == first_key && hi_key == last_key;
+    // (ranges of a grouped tile cover all of its first values, in any order)
+    const bool sorted_tile = ranges != nullptr && (minor_bits != 0 || (lo_key == first_key && hi_key == last_key));
     uint32_t found = 0;  // bit i: item i matched
     for (uint32_t r = 0; r < rights.count; ++r) {
         const PairInput& right = rights.in[r];
@@ -1229,15 +1244,42 @@ hipError_t sorted_anti_join_multi(PairInput left, size_t left_rows, const AntiRi
         t + align_up(tiles * (rights.count ? rights.count : 1) * 2 * sizeof(size_t), 256));
     if (rights.count) {
         anti_partition_kernel<<<grid_for(tiles * rights.count, kBlock, 4096), kBlock, 0, stream>>>(
-            left, left_rows, rights, tiles, ranges);
+            left, left_rows, rights, tiles, ranges, 0u);
     }
     timer_start(timer, stream);
     anti_join_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(left, left_rows, rights, ranges,
-                                                                         found_words);
+                                                                         found_words, 0u);
     timer_stop(timer, stream);
     anti_compact_kernel<<<static_cast<unsigned>(compact_tiles), kBlock, 0, stream>>>(
         left, left_rows, found_words, out, count32, count64,
         reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp));
+    return hipGetLastError();
+}
+
+size_t probe_grouped_temp_bytes(size_t rows, uint32_t right_inputs) {
+    return align_up(tiles_of(rows, kAntiTile) * (right_inputs ? right_inputs : 1) * 2 * sizeof(size_t), 256);
+}
+
+size_t probe_found_words(size_t rows) { return tiles_of(rows, kAntiTile) * (kAntiTile / kWave); }
+
+hipError_t probe_grouped(const uint64_t* keys, const uint64_t* rows_dev, size_t bound, uint32_t minor_bits,
+                         const AntiRight& rights, uint64_t* found_words, void* temp, size_t temp_bytes,
+                         hipStream_t stream, const KernelTimer* timer) {
+    if (bound == 0) return hipSuccess;
+    if (minor_bits == 0 || minor_bits > 32 || rights.count > static_cast<uint32_t>(kMaxAntiRights) ||
+        temp_bytes < probe_grouped_temp_bytes(bound, rights.count))
+        return hipErrorInvalidValue;
+    const size_t tiles = tiles_of(bound, kAntiTile);
+    const PairInput left{keys, nullptr, nullptr, rows_dev};
+    size_t* ranges = static_cast<size_t*>(temp);
+    if (rights.count) {
+        anti_partition_kernel<<<grid_for(tiles * rights.count, kBlock, 4096), kBlock, 0, stream>>>(
+            left, bound, rights, tiles, ranges, minor_bits);
+    }
+    timer_start(timer, stream);
+    anti_join_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(left, bound, rights, ranges, found_words,
+                                                                         minor_bits);
+    timer_stop(timer, stream);
     return hipGetLastError();
 }
 

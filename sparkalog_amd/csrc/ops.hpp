@@ -82,13 +82,18 @@ hipError_t unique_sorted(const uint64_t* keys, const uint64_t* rows_dev, size_t 
 // row count on the host; finish sorts those (side/side_b, sort_temp sized by
 // distinct_temp_bytes(side_rows); unused when side_rows is 0) and every tile.
 // `meta` (segmented_distinct_meta_bytes(rows)) must persist between phases.
+// With `found` (probe_grouped's words over the same keys), keys whose found
+// bit is set are dropped first: the output is then (keys \ right inputs),
+// sorted and unique.
 size_t segmented_distinct_meta_bytes(size_t rows);
 hipError_t segmented_distinct_prepare(const uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t* side,
-                                      void* meta, size_t meta_bytes, uint64_t* side_rows, hipStream_t stream);
+                                      void* meta, size_t meta_bytes, uint64_t* side_rows, hipStream_t stream,
+                                      const uint64_t* found = nullptr);
 hipError_t segmented_distinct_finish(const uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t side_varying,
                                      uint64_t* side_a, uint64_t* side_b, uint64_t side_rows, PairOutput out,
                                      uint64_t* unique_rows, void* meta, size_t meta_bytes, void* sort_temp,
-                                     size_t sort_bytes, hipStream_t stream, const KernelTimer* timer = nullptr);
+                                     size_t sort_bytes, hipStream_t stream, const KernelTimer* timer = nullptr,
+                                     const uint64_t* found = nullptr);
 
 // ---- sorted set operators ----
 size_t union_temp_bytes(size_t left_rows, size_t right_rows);
@@ -113,6 +118,17 @@ hipError_t sorted_anti_join(PairInput left, size_t left_rows, PairInput right, s
 hipError_t sorted_anti_join_multi(PairInput left, size_t left_rows, const AntiRight& rights, PairOutput out,
                                   uint32_t* count32, uint64_t* count64, void* temp, size_t temp_bytes,
                                   hipStream_t stream, const KernelTimer* timer = nullptr);
+
+// Which candidate rows the right inputs hold. keys are compressed candidate
+// rows (first << minor_bits | second, minor_bits in 1..32) grouped by their
+// first column (count *rows_dev <= bound); bit j of found_words[i] is set
+// when row 64*i + j is in some right input (probe_found_words(bound) words,
+// written up to the last row).
+size_t probe_grouped_temp_bytes(size_t rows, uint32_t right_inputs);
+size_t probe_found_words(size_t rows);
+hipError_t probe_grouped(const uint64_t* keys, const uint64_t* rows_dev, size_t bound, uint32_t minor_bits,
+                         const AntiRight& rights, uint64_t* found_words, void* temp, size_t temp_bytes,
+                         hipStream_t stream, const KernelTimer* timer = nullptr);
 
 // ---- first-column index of sorted packed runs, and the anti-join using it ----
 // xstart (n_x + 1 u32 entries) of a sorted packed relation with fewer than

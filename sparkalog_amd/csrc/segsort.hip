@@ -99,19 +99,36 @@ __global__ void seg_partition_kernel(const uint64_t* keys, size_t rows, uint32_t
     }
 }
 
+__device__ __forceinline__ bool dropped(const uint64_t* found, size_t i) {
+    return found != nullptr && ((found[i >> 6] >> (i & 63)) & 1ull);
+}
+
 // Copy the keys of tiles larger than kSegCap to the side buffer (any order:
-// they are sorted globally afterwards).
+// they are sorted globally afterwards), except keys whose found bit is set.
 __global__ __launch_bounds__(kBlock) void seg_gather_kernel(const uint64_t* keys, const size_t* bounds, size_t tiles,
-                                                            uint64_t* side, SegHeader* h) {
+                                                            uint64_t* side, SegHeader* h, const uint64_t* found) {
     __shared__ uint64_t slot;
+    __shared__ uint32_t scan_lds[kWavesPerBlock];
     for (size_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         const size_t b0 = bounds[t], b1 = bounds[t + 1];
         if (b1 - b0 <= static_cast<size_t>(kSegCap)) continue;  // block-uniform
+        uint32_t mine = 0;
+        for (size_t i = b0 + threadIdx.x; i < b1; i += kBlock) mine += dropped(found, i) ? 0u : 1u;
+        uint32_t total = 0;
+        (void)block_exclusive_sum<uint32_t>(mine, &total, scan_lds);
         if (threadIdx.x == 0) slot = atomicAdd(reinterpret_cast<unsigned long long*>(&h->side_rows),
-                                               static_cast<unsigned long long>(b1 - b0));
+                                               static_cast<unsigned long long>(total));
         __syncthreads();
         const uint64_t base = slot;
-        for (size_t i = b0 + threadIdx.x; i < b1; i += kBlock) side[base + (i - b0)] = keys[i];
+        uint64_t written = 0;
+        for (size_t c = b0; c < b1; c += kBlock) {
+            const size_t i = c + threadIdx.x;
+            const uint32_t keep = i < b1 && !dropped(found, i) ? 1u : 0u;
+            uint32_t chunk = 0;
+            const uint32_t at = block_exclusive_sum<uint32_t>(keep, &chunk, scan_lds);
+            if (keep) side[base + written + at] = keys[i];
+            written += chunk;
+        }
         __syncthreads();
     }
 }
@@ -123,10 +140,14 @@ template <typename K>
 __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(const uint64_t* keys, size_t rows, uint32_t minor_bits,
                                                           const size_t* bounds, const uint64_t* side_sorted,
                                                           PairOutput out, uint64_t* unique_rows, uint64_t* status,
-                                                          SegHeader* h) {
+                                                          SegHeader* h, const uint64_t* found) {
     constexpr int kKeyBits = static_cast<int>(sizeof(K) * 8);
-    __shared__ K tile_keys[kSegCap];
-    __shared__ uint32_t major_of[kSegCap];
+    // sort keys and run majors; before ranking, the same bytes hold the
+    // tile's surviving raw keys (u64) when some keys are dropped
+    __shared__ uint64_t seg_buf[(kSegCap * (sizeof(K) + sizeof(uint32_t)) + 7) / 8];
+    static_assert(sizeof(seg_buf) >= kSegCap * sizeof(uint64_t), "raw keys fit");
+    K* const tile_keys = reinterpret_cast<K*>(seg_buf);
+    uint32_t* const major_of = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(seg_buf) + kSegCap * sizeof(K));
     __shared__ uint32_t wave_count[kSegWaves][kSegRadix];
     __shared__ uint32_t item_totals[kSegWaves][kSegItems];
     __shared__ uint64_t keep_lds[kSegWaves][kSegItems];  // ballot masks of kept keys
@@ -143,9 +164,9 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(const uint64_t* key
     const uint64_t minor_mask = (1ull << minor_bits) - 1;
     const bool big = size > static_cast<size_t>(kSegCap);
     // each wave owns a contiguous share of the tile: items = ceil(size / 512)
-    const int items = static_cast<int>((size + kSegBlock - 1) / kSegBlock);
-    const uint32_t wave_len = static_cast<uint32_t>(items) * kWave;
-    const uint32_t n = static_cast<uint32_t>(size);
+    int items = static_cast<int>((size + kSegBlock - 1) / kSegBlock);
+    uint32_t wave_len = static_cast<uint32_t>(items) * kWave;
+    uint32_t n = static_cast<uint32_t>(size);
 
     K k[kSegItems];
     uint32_t count = 0;        // unique keys of this tile
@@ -154,6 +175,60 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(const uint64_t* key
         // load; k = (run starts at or before the position within this wave)
         // << minor_bits | minor; majors of run starts are staged in LDS (in
         // tile_keys, free until the first pass) at their wave-local rank
+        uint64_t rawv[kSegItems];
+        uint64_t prev_raw = 0;  // the key before this wave's share (lane 0 of item 0)
+        if (found == nullptr) {
+#pragma unroll
+            for (int i = 0; i < kSegItems; ++i) {
+                if (i >= items) continue;
+                const uint32_t p = wave * wave_len + i * kWave + lane;
+                rawv[i] = p < n ? keys[b0 + p] : 0;
+            }
+            const uint32_t p0 = wave * wave_len;
+            if (p0 > 0 && p0 < n) prev_raw = keys[b0 + p0 - 1];
+        } else {
+            // keep only the keys whose found bit is clear, in order
+            uint32_t pos = 0;
+#pragma unroll
+            for (int i = 0; i < kSegItems; ++i) {
+                if (i >= items) continue;
+                const uint32_t p = wave * wave_len + i * kWave + lane;
+                const bool valid = p < n;
+                rawv[i] = valid ? keys[b0 + p] : 0;
+                const uint64_t m = ballot(valid && !dropped(found, b0 + p));
+                if (lane == 0) {
+                    keep_lds[wave][i] = m;
+                    item_totals[wave][i] = static_cast<uint32_t>(__popcll(m));
+                }
+            }
+            __syncthreads();
+            uint32_t survivors = 0;
+            for (int w = 0; w < kSegWaves; ++w)
+                for (int i = 0; i < items; ++i) {
+                    if (w < static_cast<int>(wave)) pos += item_totals[w][i];
+                    survivors += item_totals[w][i];
+                }
+#pragma unroll
+            for (int i = 0; i < kSegItems; ++i) {
+                if (i >= items) continue;
+                const uint64_t m = keep_lds[wave][i];
+                if ((m >> lane) & 1ull) seg_buf[pos + lanes_below(m)] = rawv[i];
+                pos += static_cast<uint32_t>(__popcll(m));
+            }
+            __syncthreads();
+            n = survivors;
+            items = static_cast<int>((n + kSegBlock - 1) / kSegBlock);
+            wave_len = static_cast<uint32_t>(items) * kWave;
+#pragma unroll
+            for (int i = 0; i < kSegItems; ++i) {
+                if (i >= items) continue;
+                const uint32_t p = wave * wave_len + i * kWave + lane;
+                rawv[i] = p < n ? seg_buf[p] : 0;
+            }
+            const uint32_t p0 = wave * wave_len;
+            if (p0 > 0 && p0 < n) prev_raw = seg_buf[p0 - 1];
+            __syncthreads();  // seg_buf now holds the sort keys and run majors
+        }
         uint32_t* staged = reinterpret_cast<uint32_t*>(tile_keys) + wave * (kSegCap / kSegWaves);
         uint32_t starts_before = 0;  // run starts in earlier items of this wave
         uint32_t last_major = 0;     // major of lane 63 of the previous item
@@ -162,7 +237,7 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(const uint64_t* key
             if (i >= items) continue;
             const uint32_t p = wave * wave_len + i * kWave + lane;
             const bool valid = p < n;
-            const uint64_t raw = valid ? keys[b0 + p] : 0;
+            const uint64_t raw = rawv[i];
             const uint32_t major = static_cast<uint32_t>(raw >> minor_bits);
             const uint32_t up = __shfl_up(major, 1, kWave);
             bool start = false;
@@ -174,7 +249,7 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(const uint64_t* key
                 else if (i > 0)
                     start = last_major != major;
                 else
-                    start = static_cast<uint32_t>(keys[b0 + p - 1] >> minor_bits) != major;
+                    start = static_cast<uint32_t>(prev_raw >> minor_bits) != major;
             }
             last_major = __shfl(major, kWave - 1, kWave);
             const uint64_t m = ballot(start);
@@ -352,7 +427,8 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(const uint64_t* key
 size_t segmented_distinct_meta_bytes(size_t rows) { return seg_layout(rows).total; }
 
 hipError_t segmented_distinct_prepare(const uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t* side,
-                                      void* meta, size_t meta_bytes, uint64_t* side_rows, hipStream_t stream) {
+                                      void* meta, size_t meta_bytes, uint64_t* side_rows, hipStream_t stream,
+                                      const uint64_t* found) {
     *side_rows = 0;
     if (rows == 0) return hipSuccess;
     const SegLayout l = seg_layout(rows);
@@ -365,7 +441,7 @@ hipError_t segmented_distinct_prepare(const uint64_t* keys, size_t rows, uint32_
     seg_init_kernel<<<small_grid, kBlock, 0, stream>>>(h, status, l.tiles);
     seg_partition_kernel<<<small_grid, kBlock, 0, stream>>>(keys, rows, minor_bits, bounds, l.tiles);
     seg_gather_kernel<<<static_cast<unsigned>(l.tiles < 1024 ? l.tiles : 1024), kBlock, 0, stream>>>(
-        keys, bounds, l.tiles, side, h);
+        keys, bounds, l.tiles, side, h, found);
     hipError_t st = hipGetLastError();
     // runs longer than a tile (usually none): one 8-byte read sizes their sort
     if (st == hipSuccess) st = hipMemcpyAsync(side_rows, &h->side_rows, sizeof(uint64_t), hipMemcpyDeviceToHost, stream);
@@ -376,7 +452,8 @@ hipError_t segmented_distinct_prepare(const uint64_t* keys, size_t rows, uint32_
 hipError_t segmented_distinct_finish(const uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t side_varying,
                                      uint64_t* side_a, uint64_t* side_b, uint64_t side_rows, PairOutput out,
                                      uint64_t* unique_rows, void* meta, size_t meta_bytes, void* sort_temp,
-                                     size_t sort_bytes, hipStream_t stream, const KernelTimer* timer) {
+                                     size_t sort_bytes, hipStream_t stream, const KernelTimer* timer,
+                                     const uint64_t* found) {
     if (rows == 0) return set_u64(unique_rows, 0, stream);
     const SegLayout l = seg_layout(rows);
     if (meta_bytes < l.total) return hipErrorInvalidValue;
@@ -401,10 +478,10 @@ hipError_t segmented_distinct_finish(const uint64_t* keys, size_t rows, uint32_t
     // rank bits: at most log2(kSegCap) = 12 (a run per key)
     if (minor_bits + 12 <= 32)
         seg_tile_kernel<uint32_t><<<static_cast<unsigned>(l.tiles), kSegBlock, 0, stream>>>(
-            keys, rows, minor_bits, bounds, side_unique_out, out, unique_rows, status, h);
+            keys, rows, minor_bits, bounds, side_unique_out, out, unique_rows, status, h, found);
     else
         seg_tile_kernel<uint64_t><<<static_cast<unsigned>(l.tiles), kSegBlock, 0, stream>>>(
-            keys, rows, minor_bits, bounds, side_unique_out, out, unique_rows, status, h);
+            keys, rows, minor_bits, bounds, side_unique_out, out, unique_rows, status, h, found);
     timer_stop(timer, stream);
     return hipGetLastError();
 }
