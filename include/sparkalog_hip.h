@@ -230,9 +230,11 @@ int sparkalog_hip_executor_run(sparkalog_hip_store* store, const uint32_t* relat
 
 /* Per-operator profiling: when enabled, every operator of a run is bracketed
  * by HIP events on the store's stream. op: 0 join count, 1 join emit,
- * 2 distinct, 3 anti-join, 4 union of rule contributions, 5 merge of FULL's
- * runs, 6 index build. Reports launches, summed device milliseconds and
- * summed algorithmic bytes (8 bytes per pair read or written). */
+ * 2 distinct, 3 anti-join (probe of FULL), 4 anti-join of few candidates
+ * through FULL's first-column index, 5 union of rule contributions, 6 merge
+ * of FULL's runs, 7 index build (sparkalog_hip_store_op_kernel_stats names
+ * them). Reports launches, summed device milliseconds and summed algorithmic
+ * bytes (8 bytes per pair read or written). */
 int sparkalog_hip_store_set_profiling(sparkalog_hip_store* store, int enabled);
 /* Executor tuning (no counterpart in the reference, whose union rewrites all
  * of FULL each iteration): FULL is kept as size-tiered sorted runs, and the

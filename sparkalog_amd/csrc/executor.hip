@@ -139,11 +139,17 @@ struct OpStat {
     double kernel_ms = 0.0;
 };
 
-enum OpKind { kOpJoinCount = 0, kOpJoinEmit, kOpDistinct, kOpAntiJoin, kOpUnion, kOpMergeRuns, kOpIndex, kOpCount };
-const char* const kOpNames[kOpCount] = {"join_count", "join_emit", "distinct", "anti_join",
+enum OpKind {
+    kOpJoinCount = 0, kOpJoinEmit, kOpDistinct, kOpAntiJoin, kOpProbeSparse, kOpUnion, kOpMergeRuns, kOpIndex, kOpCount
+};
+const char* const kOpNames[kOpCount] = {"join_count", "join_emit", "distinct", "anti_join", "anti_join_sparse",
                                         "union", "merge_runs", "index"};
 const char* const kOpKernels[kOpCount] = {"dense_join_count_kernel", "dense_join_emit_kernel", "seg_tile_kernel",
-                                          "anti_join_kernel", "union_kernel", "merge_disjoint_kernel", ""};
+                                          "anti_join_kernel", "probe_sparse_kernel", "union_kernel",
+                                          "merge_disjoint_kernel", ""};
+// Candidates up to this many are probed one thread each through the
+// first-column index of FULL's runs (few candidates, long FULL ranges).
+constexpr uint64_t kSparseProbeRows = 1ull << 21;
 
 // Events of one profiled operator launch.
 struct OpEvents {
@@ -1542,13 +1548,34 @@ void session_step(sparkalog_hip_store* s) {
             for (const auto& x : ts.runs) add_right(x);
             if (rights.count) {
                 rt.found.reserve(probe_found_words(M));
-                const size_t ptb = probe_grouped_temp_bytes(M, rights.count);
-                void* ptemp = s->scratch(ptb);
-                const int pev = s->begin_op();
-                check(probe_grouped(rt.cand_a.ptr, nullptr, M, rt.shift, rights, rt.found.ptr, ptemp, ptb,
-                                    s->stream, s->kernel_timer(pev)));
-                s->end_op(pev, kOpAntiJoin, 8.0 * M + 8.0 * right_rows);
-                t.anti_slot = pev;
+                bool parts_fit = ts.full.rows <= 0xFFFFFFFFull;
+                for (const auto& x : ts.runs) parts_fit = parts_fit && x.rows <= 0xFFFFFFFFull;
+                if (M <= kSparseProbeRows && vmax < (1u << 24) && parts_fit) {
+                    XRights xr{};
+                    xr.count = 0;
+                    auto add_part = [&](DevRelation& x) {
+                        if (!x.rows) return;
+                        ensure_xidx(s, x, n_x);
+                        xr.keys[xr.count] = x.keys.ptr;
+                        xr.xstart[xr.count] = x.xidx.ptr;
+                        ++xr.count;
+                    };
+                    add_part(ts.full);
+                    for (auto& x : ts.runs) add_part(x);
+                    const int pev = s->begin_op();
+                    check(probe_sparse(rt.cand_a.ptr, M, rt.shift, xr, n_x, rt.found.ptr, s->stream,
+                                       s->kernel_timer(pev)));
+                    s->end_op(pev, kOpProbeSparse, 8.0 * M + 8.0 * right_rows);
+                    t.anti_slot = pev;
+                } else {
+                    const size_t ptb = probe_grouped_temp_bytes(M, rights.count);
+                    void* ptemp = s->scratch(ptb);
+                    const int pev = s->begin_op();
+                    check(probe_grouped(rt.cand_a.ptr, nullptr, M, rt.shift, rights, rt.found.ptr, ptemp, ptb,
+                                        s->stream, s->kernel_timer(pev)));
+                    s->end_op(pev, kOpAntiJoin, 8.0 * M + 8.0 * right_rows);
+                    t.anti_slot = pev;
+                }
                 found = rt.found.ptr;
             }
             algo_bytes += 8.0 * right_rows;
@@ -1753,7 +1780,9 @@ void session_step(sparkalog_hip_store* s) {
             DevRelation run;
             run.keys = std::move(t.new_run);
             run.rows = N;
-            if (use_x && N <= 0xFFFFFFFFull) ensure_xidx(s, run, n_x);
+            // indexed while the anti-join uses the index (or probes sparsely)
+            if ((use_x || (vmax < (1u << 24) && N <= kSparseProbeRows)) && N <= 0xFFFFFFFFull)
+                ensure_xidx(s, run, n_x);
             ts.runs.push_back(std::move(run));
         } else {
             s->give(std::move(t.new_run));

@@ -144,6 +144,11 @@ struct XRights {
     const uint32_t* xstart[kMaxAntiRights];
     uint32_t count;
 };
+// probe_grouped for few candidates: one thread per candidate, searching
+// only its first value's segment of every run (rights' first-column indexes,
+// first values below n_x); found_words as probe_grouped's.
+hipError_t probe_sparse(const uint64_t* keys, size_t rows, uint32_t minor_bits, const XRights& rights, uint32_t n_x,
+                        uint64_t* found_words, hipStream_t stream, const KernelTimer* timer = nullptr);
 // left \ (rights' union) for sorted unique packed left rows (count *left_rows
 // <= bound) whose first values are below n_x; writes out.packed and
 // out.packed_copy (out.first must be null) and the row count to *count64.

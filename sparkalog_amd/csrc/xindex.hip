@@ -37,7 +37,7 @@ constexpr int kXPairItems = kXPairCap / kBlock;
 constexpr int kCTile = 2048;                 // rows per compaction tile
 constexpr int kCItems = kCTile / kBlock;
 constexpr int kCWaveRows = kCItems * kWave;
-constexpr uint32_t kXBuildSpan = 4096;       // first values per index-build block
+constexpr uint32_t kXBuildSpan = 256;        // first values per index-build block
 
 struct XHeader {
     uint32_t error;
@@ -377,6 +377,54 @@ __global__ __launch_bounds__(kBlock) void xanti_compact_kernel(const uint64_t* l
     if (base + rows == nl && threadIdx.x == 0) *count64 = offset + tile_total;
 }
 
+// Sparse probe: one thread per compressed candidate (first << minor_bits
+// | second); for every run, the index gives the candidate's segment (rows of
+// its first value) and a branch-free binary search looks for the pair there.
+// The searches of all runs advance in lockstep, so one step's loads are in
+// flight together. Found candidates set their bit in found_words (one word
+// per 64 candidates, written by the wave's first lane).
+__global__ __launch_bounds__(kBlock) void probe_sparse_kernel(const uint64_t* keys, size_t rows, uint32_t minor_bits,
+                                                              XRights rights, uint32_t n_x, uint64_t* found_words) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const bool valid = i < rows;
+    const uint64_t raw = valid ? keys[i] : 0;
+    const uint32_t x = static_cast<uint32_t>(raw >> minor_bits);
+    const uint64_t key = (static_cast<uint64_t>(x) << 32) | (raw & ((1ull << minor_bits) - 1));
+    uint32_t lo[kMaxAntiRights], len[kMaxAntiRights];
+    uint32_t longest = 0;
+#pragma unroll
+    for (int q = 0; q < kMaxAntiRights; ++q) {
+        lo[q] = len[q] = 0;
+        if (valid && q < static_cast<int>(rights.count) && x < n_x) {
+            lo[q] = rights.xstart[q][x];
+            len[q] = rights.xstart[q][x + 1] - lo[q];
+            longest = len[q] > longest ? len[q] : longest;
+        }
+    }
+    // pos[q] = rows of run q's segment below the key
+    uint32_t pos[kMaxAntiRights];
+#pragma unroll
+    for (int q = 0; q < kMaxAntiRights; ++q) pos[q] = 0;
+    uint32_t step = 1;
+    while (step * 2 <= longest) step *= 2;
+    for (; longest && step; step >>= 1) {
+        uint64_t probe[kMaxAntiRights];
+#pragma unroll
+        for (int q = 0; q < kMaxAntiRights; ++q)
+            probe[q] = pos[q] + step <= len[q] ? rights.keys[q][lo[q] + pos[q] + step - 1] : ~0ull;
+#pragma unroll
+        for (int q = 0; q < kMaxAntiRights; ++q)
+            if (pos[q] + step <= len[q] && probe[q] < key) pos[q] += step;
+    }
+    bool hit = false;
+#pragma unroll
+    for (int q = 0; q < kMaxAntiRights; ++q)
+        if (pos[q] < len[q] && rights.keys[q][lo[q] + pos[q]] == key) hit = true;
+    const uint64_t m = ballot(valid && hit);
+    const size_t row0 = i - lane_id();
+    if (lane_id() == 0 && row0 < rows) found_words[row0 / kWave] = m;
+}
+
 size_t xanti_tiles(size_t bound) { return (bound + kXTile - 1) / kXTile; }
 size_t xcompact_tiles(size_t bound) { return (bound + kCTile - 1) / kCTile; }
 
@@ -400,6 +448,19 @@ size_t anti_join_x_temp_bytes(size_t bound) {
     const size_t b = bound ? bound : 1;
     return 256 + (xcompact_tiles(b) * sizeof(uint64_t) + 255) / 256 * 256 +
            (xanti_tiles(b) * (kXTile / 32) * sizeof(uint32_t) + 255) / 256 * 256;
+}
+
+hipError_t probe_sparse(const uint64_t* keys, size_t rows, uint32_t minor_bits, const XRights& rights, uint32_t n_x,
+                        uint64_t* found_words, hipStream_t stream, const KernelTimer* timer) {
+    if (rows == 0) return hipSuccess;
+    if (minor_bits == 0 || minor_bits > 32 || rights.count > static_cast<uint32_t>(kMaxAntiRights))
+        return hipErrorInvalidValue;
+    const size_t blocks = (rows + kBlock - 1) / kBlock;
+    timer_start(timer, stream);
+    probe_sparse_kernel<<<static_cast<unsigned>(blocks), kBlock, 0, stream>>>(keys, rows, minor_bits, rights, n_x,
+                                                                             found_words);
+    timer_stop(timer, stream);
+    return hipGetLastError();
 }
 
 hipError_t anti_join_x(const uint64_t* left, const uint64_t* left_rows, size_t bound, const XRights& rights,
