@@ -41,6 +41,8 @@ def parse_args():
                    help="tc: also time one fixpoint from host arrays to the closure back in host memory (PCIe)")
     p.add_argument("--indexed-anti-join", action="store_true",
                    help="tc: anti-join FULL through its first-column index (results are identical)")
+    p.add_argument("--sparse-probe-rows", type=int, default=1 << 21,
+                   help="tc: candidate count up to which FULL is probed one candidate per thread")
     p.add_argument("--cpu-sample-nodes", type=int, default=1 << 16)
     p.add_argument("--mode", choices=["tc", "ops", "batched", "served", "copy"], default="tc",
                    help="tc: the headline fixpoint; ops: operator microbench at 2^k rows; "
@@ -237,7 +239,8 @@ def main():
     d_dst = torch.from_numpy(dst.view(np.int32)).cuda()
     stream = HipStream()
     store = RelationStore(stream)
-    store.set_tuning(merge_ratio=args.merge_ratio, indexed_anti_join=args.indexed_anti_join)
+    store.set_tuning(merge_ratio=args.merge_ratio, indexed_anti_join=args.indexed_anti_join,
+                     sparse_probe_rows=args.sparse_probe_rows)
     R = plans.RelationId
     plan = plans.transitive_closure_scc(R(1), R(0))
 

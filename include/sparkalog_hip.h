@@ -243,10 +243,11 @@ int sparkalog_hip_store_set_profiling(sparkalog_hip_store* store, int enabled);
  * 1) sorts candidates grouped by their first column per group in LDS instead
  * of a global radix sort; indexed_anti_join (default 0) gives FULL's runs a
  * first-column index (while first values are below 2^24) so the anti-join
- * reads only the rows of first values it holds. Results do not depend on
- * any of them. */
+ * reads only the rows of first values it holds; iterations with at most
+ * sparse_probe_rows candidates (default 2^21) check them against FULL one
+ * thread each through that index. Results do not depend on any of them. */
 int sparkalog_hip_store_set_tuning(sparkalog_hip_store* store, uint32_t merge_ratio, int segmented_distinct,
-                                   int indexed_anti_join);
+                                   int indexed_anti_join, uint64_t sparse_probe_rows);
 int sparkalog_hip_store_op_stats(sparkalog_hip_store* store, int op, uint64_t* launches, double* ms,
                                  double* bytes);
 /* The operator's name and main kernel, and that kernel's own launches and

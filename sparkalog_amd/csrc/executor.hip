@@ -147,8 +147,9 @@ const char* const kOpNames[kOpCount] = {"join_count", "join_emit", "distinct", "
 const char* const kOpKernels[kOpCount] = {"dense_join_count_kernel", "dense_join_emit_kernel", "seg_tile_kernel",
                                           "anti_join_kernel", "probe_sparse_kernel", "union_kernel",
                                           "merge_disjoint_kernel", ""};
-// Candidates up to this many are probed one thread each through the
-// first-column index of FULL's runs (few candidates, long FULL ranges).
+// Candidates up to this many (by default) are probed one thread each
+// through the first-column index of FULL's runs (few candidates, long FULL
+// ranges).
 constexpr uint64_t kSparseProbeRows = 1ull << 21;
 
 // Events of one profiled operator launch.
@@ -582,6 +583,7 @@ struct sparkalog_hip_store {
         if (b.ptr) xpool.push_back(std::move(b));
     }
     bool x_anti = false;  // anti-join FULL through its first-column index when it fits
+    uint64_t sparse_probe_rows = spk::kSparseProbeRows;
     // small persistent device buffers for range probes and statistics (a
     // fresh allocation would synchronize the whole device when freed)
     spk::DevBuf<uint64_t> small_dev, probe_dev;
@@ -1289,8 +1291,9 @@ extern "C" int sparkalog_hip_store_digest(sparkalog_hip_store* s, uint32_t id, u
 }
 
 extern "C" int sparkalog_hip_store_set_tuning(sparkalog_hip_store* s, uint32_t merge_ratio, int segmented_distinct,
-                                              int indexed_anti_join) {
+                                              int indexed_anti_join, uint64_t sparse_probe_rows) {
     if (!s || merge_ratio < 2) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    s->sparse_probe_rows = sparse_probe_rows;
     s->merge_ratio = merge_ratio;
     s->segmented_distinct = segmented_distinct != 0;
     s->x_anti = indexed_anti_join != 0;
@@ -1550,7 +1553,7 @@ void session_step(sparkalog_hip_store* s) {
                 rt.found.reserve(probe_found_words(M));
                 bool parts_fit = ts.full.rows <= 0xFFFFFFFFull;
                 for (const auto& x : ts.runs) parts_fit = parts_fit && x.rows <= 0xFFFFFFFFull;
-                if (M <= kSparseProbeRows && vmax < (1u << 24) && parts_fit) {
+                if (M <= s->sparse_probe_rows && vmax < (1u << 24) && parts_fit) {
                     XRights xr{};
                     xr.count = 0;
                     auto add_part = [&](DevRelation& x) {
@@ -1781,7 +1784,7 @@ void session_step(sparkalog_hip_store* s) {
             run.keys = std::move(t.new_run);
             run.rows = N;
             // indexed while the anti-join uses the index (or probes sparsely)
-            if ((use_x || (vmax < (1u << 24) && N <= kSparseProbeRows)) && N <= 0xFFFFFFFFull)
+            if ((use_x || (vmax < (1u << 24) && N <= s->sparse_probe_rows)) && N <= 0xFFFFFFFFull)
                 ensure_xidx(s, run, n_x);
             ts.runs.push_back(std::move(run));
         } else {

@@ -187,13 +187,15 @@ def test_ungrouped_projection_plan(stream):
     assert summ.iterations == rsum.iterations and summ.total_new_rows == rsum.total_new_rows
 
 
-@pytest.mark.parametrize("merge_ratio,segmented,indexed", [(2, True, True), (4, False, True), (4, True, False),
-                                                           (16, False, False)])
-def test_tuning_does_not_change_results(merge_ratio, segmented, indexed, expected, stream):
+@pytest.mark.parametrize("merge_ratio,segmented,indexed,sparse", [(2, True, True, 1 << 21), (4, False, True, 0),
+                                                                  (4, True, False, 0), (4, True, False, 1 << 40),
+                                                                  (16, False, False, 1 << 21)])
+def test_tuning_does_not_change_results(merge_ratio, segmented, indexed, sparse, expected, stream):
     e = expected["component_graph_16384"]
     s, d, _ = workload.component_graph(nodes=1 << 14)
     store = RelationStore(stream)
-    store.set_tuning(merge_ratio=merge_ratio, segmented_distinct=segmented, indexed_anti_join=indexed)
+    store.set_tuning(merge_ratio=merge_ratio, segmented_distinct=segmented, indexed_anti_join=indexed,
+                     sparse_probe_rows=sparse)
     store.insert_static(R(0), (s, d))
     store.insert_recursive(R(1), (s, d), (s, d))
     summ = RecursiveExecutor.compile(plans.transitive_closure_scc(R(1), R(0)), store).run(store, 10_000)
@@ -223,3 +225,21 @@ def test_program_with_negation_matches_general_path(expected, stream):
     store.insert_static(R(3), (np.empty(0, np.uint32), np.empty(0, np.uint32)))
     assert store.difference(R(4), R(0), R(3)) == edges[0].size
     assert store.difference(R(5), R(3), R(0)) == 0
+
+
+def test_wide_ids(stream):
+    # node ids spread over [0, 2^28): 64-bit sort keys in the tile sort, no
+    # first-column index (ids >= 2^24), the densest join index still allowed
+    s, d, _ = workload.component_graph(nodes=1 << 12)
+    rng = np.random.default_rng(0x1D5)
+    ids = np.sort(rng.choice(1 << 28, size=1 << 12, replace=False)).astype(np.uint32)
+    rng.shuffle(ids)
+    ws, wd = ids[s], ids[d]
+    key = np.unique((ws.astype(np.uint64) << 32) | wd)
+    edges = ((key >> 32).astype(np.uint32), (key & 0xFFFFFFFF).astype(np.uint32))
+    store, summ = _tc(edges, stream)
+    bfs = oracle.tc_bfs(s, d, 1 << 12, closure=True)
+    exp = np.unique((ids[bfs["closure"][0]].astype(np.uint64) << 32) | ids[bfs["closure"][1]])
+    full = store.full(R(1))
+    assert np.array_equal((full[0].astype(np.uint64) << 32) | full[1], exp)
+    assert summ.total_new_rows == exp.size - key.size
