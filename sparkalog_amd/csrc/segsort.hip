@@ -20,7 +20,7 @@
 namespace spk {
 namespace {
 
-constexpr int kSegTile = 2048;                // nominal tile
+constexpr int kSegTile = 3072;                // nominal tile
 constexpr int kSegCap = 4096;                 // largest tile sorted in LDS
 constexpr int kSegBlock = 512;                // threads of a tile block (8 waves)
 constexpr int kSegWaves = kSegBlock / kWave;

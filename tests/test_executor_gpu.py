@@ -168,6 +168,26 @@ def test_long_candidate_runs(stream):
     assert summ.iterations == rsum.iterations and summ.total_new_rows == rsum.total_new_rows
 
 
+def test_mixed_run_lengths(stream):
+    # sources with 200..1,500 first-layer successors, each with 4 successors
+    # of its own: candidate runs of 800..6,000 rows, on both sides of the
+    # tile size and of the largest tile sorted in LDS, packed back to back
+    rng = np.random.default_rng(0x7A1)
+    fans = rng.permutation(np.repeat([200, 700, 768, 1000, 1023, 1024, 1025, 1100, 1500], 6))
+    e = set()
+    for src, fan in enumerate(fans):
+        mids = 1000 + rng.choice(20_000, size=int(fan), replace=False)
+        e.update((src, int(m)) for m in mids)
+    for m in range(1000, 21_000):
+        e.update((m, 30_000 + int(t)) for t in rng.choice(8000, size=4, replace=False))
+    edges = cols(sorted(e))
+    store, summ = _tc(edges, stream)
+    ref, rsum = _oracle_run({0: edges}, {1: (edges, edges)}, [rule(1, 1, 0)], [1])
+    full, expect = store.full(R(1)), ref.get(1)
+    assert np.array_equal(full[0], expect[0]) and np.array_equal(full[1], expect[1])
+    assert summ.iterations == rsum.iterations and summ.total_new_rows == rsum.total_new_rows
+
+
 def test_ungrouped_projection_plan(stream):
     # output (edge.1, path.0): the distinct's major column is not DELTA's first
     # column, so candidates take the general radix-sort path
