@@ -249,13 +249,7 @@ __global__ __launch_bounds__(kBlock) void sort_pass_kernel(uint64_t* keys_a, uin
         const size_t idx = tile_base + wave * (kSortItems * kWave) + i * kWave + lane;
         const bool valid = idx < rows;
         const uint32_t digit = static_cast<uint32_t>(keys[i] >> shift) & (kRadix - 1);
-        uint64_t peers = ballot(valid);
-#pragma unroll
-        for (int b = 0; b < kRadixBits; ++b) {
-            const bool bit = (digit >> b) & 1u;
-            const uint64_t m = ballot(bit);
-            peers &= bit ? m : ~m;
-        }
+        const uint64_t peers = match_digit<kRadixBits>(digit, valid);
         const uint32_t before = valid ? wave_count[wave][digit] : 0;
         const uint32_t below = lanes_below(peers);
         ranks[i] = before + below;

@@ -11,7 +11,8 @@
 //   2. runs longer than a tile go, gathered, through the global onesweep sort
 //      (rare: a run is one source's candidates in one iteration);
 //   3. every tile sorts its keys in LDS by (local run rank, minor) with stable
-//      8-bit LSD passes over the varying digits only, drops duplicates, and
+//      LSD passes over the bit range where its keys differ (digits of at most
+//      9 bits: 24..27 varying bits take three passes), drops duplicates, and
 //      writes the expanded pairs (major << 32 | minor) at an offset found by
 //      decoupled look-back.
 #include "ops.hpp"
@@ -26,7 +27,8 @@ constexpr int kSegBlock = 512;                // threads of a tile block (8 wave
 constexpr int kSegWaves = kSegBlock / kWave;
 constexpr int kSegItems = kSegCap / kSegBlock;  // 8 keys per thread
 constexpr int kSegWaveRows = kSegItems * kWave;
-constexpr int kSegRadix = 256;
+constexpr int kSegDigitBits = 9;              // widest digit of a tile pass
+constexpr int kSegRadix = 1 << kSegDigitBits;
 
 __device__ __forceinline__ void store_pair(const PairOutput& out, size_t i, uint64_t k) {
     if (out.packed) out.packed[i] = k;
@@ -141,7 +143,6 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(const uint64_t* key
                                                           const size_t* bounds, const uint64_t* side_sorted,
                                                           PairOutput out, uint64_t* unique_rows, uint64_t* status,
                                                           SegHeader* h, const uint64_t* found) {
-    constexpr int kKeyBits = static_cast<int>(sizeof(K) * 8);
     // sort keys and run majors; before ranking, the same bytes hold the
     // tile's surviving raw keys (u64) when some keys are dropped
     __shared__ uint64_t seg_buf[(kSegCap * (sizeof(K) + sizeof(uint32_t)) + 7) / 8];
@@ -296,10 +297,21 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(const uint64_t* key
             }
             vary = n ? (o ^ a) : 0;
         }
-        // stable LSD passes in LDS over the varying 8-bit digits
-        for (int shift = 0; shift < kKeyBits; shift += 8) {
-            if (!((vary >> shift) & 0xFF)) continue;  // block-uniform
-            for (int i = threadIdx.x; i < kSegWaves * kSegRadix; i += kSegBlock) (&wave_count[0][0])[i] = 0;
+        // stable LSD passes in LDS over the varying bit range [lb, lb + width)
+        // of the keys, in equal digits of at most kSegDigitBits bits
+        int lb = 0, width = 0;
+        if (vary) {
+            lb = __builtin_ctzll(static_cast<uint64_t>(vary));
+            width = 64 - __builtin_clzll(static_cast<uint64_t>(vary)) - lb;
+        }
+        const int passes = (width + kSegDigitBits - 1) / kSegDigitBits;
+        const int step = passes ? (width + passes - 1) / passes : 0;
+        for (int pass = 0; pass < passes; ++pass) {  // block-uniform
+            const int shift = lb + pass * step;
+            const int bits = step < lb + width - shift ? step : lb + width - shift;
+            const uint32_t dmask = (1u << bits) - 1u;
+            for (uint32_t i = threadIdx.x; i < (static_cast<uint32_t>(kSegWaves) << bits); i += kSegBlock)
+                wave_count[i >> bits][i & dmask] = 0;
             __syncthreads();
             uint32_t ranks[kSegItems];
 #pragma unroll
@@ -307,24 +319,18 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(const uint64_t* key
                 if (i >= items) continue;
                 const uint32_t p = wave * wave_len + i * kWave + lane;
                 const bool valid = p < n;
-                const uint32_t digit = static_cast<uint32_t>(k[i] >> shift) & 0xFFu;
-                uint64_t peers = ballot(valid);
-#pragma unroll
-                for (int b = 0; b < 8; ++b) {
-                    const bool bit = (digit >> b) & 1u;
-                    const uint64_t mm = ballot(bit);
-                    peers &= bit ? mm : ~mm;
-                }
+                const uint32_t digit = static_cast<uint32_t>(k[i] >> shift) & dmask;
+                const uint64_t peers = match_digit<kSegDigitBits>(digit, valid, bits);
                 const uint32_t before = valid ? wave_count[wave][digit] : 0;
                 const uint32_t below = lanes_below(peers);
                 ranks[i] = before + below;
                 if (valid && below == 0) wave_count[wave][digit] = before + static_cast<uint32_t>(__popcll(peers));
             }
             __syncthreads();
-            // digit d's start per wave: the first 256 threads own one digit each
+            // digit d's start per wave: thread d owns digit d
             const uint32_t d = threadIdx.x;
             uint32_t tc = 0;
-            if (d < kSegRadix) {
+            if (d <= dmask) {
 #pragma unroll
                 for (int w = 0; w < kSegWaves; ++w) {
                     const uint32_t c = wave_count[w][d];
@@ -334,7 +340,7 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(const uint64_t* key
             }
             uint32_t total_unused;
             const uint32_t local_start = block_exclusive_sum<uint32_t, kSegWaves>(tc, &total_unused, scan_lds);
-            if (d < kSegRadix) {
+            if (d <= dmask) {
 #pragma unroll
                 for (int w = 0; w < kSegWaves; ++w) wave_count[w][d] += local_start;
             }
@@ -344,7 +350,7 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(const uint64_t* key
                 if (i >= items) continue;
                 const uint32_t p = wave * wave_len + i * kWave + lane;
                 if (p < n) {
-                    const uint32_t digit = static_cast<uint32_t>(k[i] >> shift) & 0xFFu;
+                    const uint32_t digit = static_cast<uint32_t>(k[i] >> shift) & dmask;
                     tile_keys[wave_count[wave][digit] + ranks[i]] = k[i];
                 }
             }
