@@ -105,12 +105,25 @@ struct DevRelation {
     uint32_t xidx_n = 0;
     const uint64_t* xidx_keys = nullptr;
     uint64_t xidx_rows = 0;
-    void touched() { xidx_n = 0; }
+    // second column of a FULL part (u32 per row), valid while keys/rows are
+    // those it was written for
+    DevBuf<uint32_t> sec;
+    const uint64_t* sec_keys = nullptr;
+    uint64_t sec_rows = 0;
+    void touched() {
+        xidx_n = 0;
+        sec_keys = nullptr;
+    }
     bool xidx_valid(uint32_t n_x) const { return xidx_n == n_x && xidx_keys == keys.ptr && xidx_rows == rows; }
     void xidx_set(uint32_t n_x) {
         xidx_n = n_x;
         xidx_keys = keys.ptr;
         xidx_rows = rows;
+    }
+    bool sec_valid() const { return sec_keys != nullptr && sec_keys == keys.ptr && sec_rows == rows; }
+    void sec_set() {
+        sec_keys = keys.ptr;
+        sec_rows = rows;
     }
 };
 
@@ -507,6 +520,7 @@ struct TargetRuntime {
     DevBuf<uint64_t> newt;
     DevBuf<uint64_t> full_union;
     DevBuf<uint64_t> new_run;      // NEWT's second copy, becomes a run of FULL
+    DevBuf<uint32_t> new_sec;      // that run's second column (when FULL is probed by it)
     uint64_t contrib_bound = 0;
     int anti_slot = -1;            // profiling event of this iteration's anti-join
     bool fused = false;            // this iteration's distinct wrote NEWT (candidates in FULL dropped first)
@@ -581,6 +595,29 @@ struct sparkalog_hip_store {
     }
     void give_x(spk::DevBuf<uint32_t>&& b) {
         if (b.ptr) xpool.push_back(std::move(b));
+    }
+    // second-column buffers of FULL's parts (sized like runs: best fit)
+    std::vector<spk::DevBuf<uint32_t>> secpool;
+    spk::DevBuf<uint32_t> take_sec(size_t n) {
+        size_t best = secpool.size();
+        for (size_t i = 0; i < secpool.size(); ++i)
+            if (secpool[i].cap >= n && (best == secpool.size() || secpool[i].cap < secpool[best].cap)) best = i;
+        spk::DevBuf<uint32_t> b;
+        if (best < secpool.size()) {
+            b = std::move(secpool[best]);
+            secpool.erase(secpool.begin() + static_cast<long>(best));
+        } else {
+            for (auto& pb : secpool)
+                if (pb.cap < n) pb.release();
+            secpool.erase(std::remove_if(secpool.begin(), secpool.end(),
+                                         [](const spk::DevBuf<uint32_t>& x) { return x.ptr == nullptr; }),
+                          secpool.end());
+            b.reserve(n ? n : 1);
+        }
+        return b;
+    }
+    void give_sec(spk::DevBuf<uint32_t>&& b) {
+        if (b.ptr) secpool.push_back(std::move(b));
     }
     bool x_anti = false;  // anti-join FULL through its first-column index when it fits
     uint64_t sparse_probe_rows = spk::kSparseProbeRows;
@@ -834,6 +871,17 @@ void ensure_xidx(sparkalog_hip_store* s, DevRelation& r, uint32_t n_x) {
     r.xidx_set(n_x);
 }
 
+// Write a FULL part's second column unless it is current.
+void ensure_sec(sparkalog_hip_store* s, DevRelation& r) {
+    if (r.sec_valid()) return;
+    if (r.sec.cap < r.rows + kSecPad) {
+        s->give_sec(std::move(r.sec));
+        r.sec = s->take_sec(r.rows + kSecPad);
+    }
+    check(second_column(r.keys.ptr, r.rows, r.sec.ptr, s->stream));
+    r.sec_set();
+}
+
 double settle_runs(sparkalog_hip_store* s, RelationState& ts, bool all) {
     double bytes = 0.0;
     while (!ts.runs.empty()) {
@@ -845,36 +893,45 @@ double settle_runs(sparkalog_hip_store* s, RelationState& ts, bool all) {
         const uint64_t n = prev.rows + last.rows;
         if (prev.rows == 0) {
             const uint32_t last_nx = last.xidx_n && last.xidx_valid(last.xidx_n) ? last.xidx_n : 0;
+            const bool last_sec = last.sec_valid();
             std::swap(prev.keys, last.keys);
             std::swap(prev.xidx, last.xidx);
+            std::swap(prev.sec, last.sec);
             prev.rows = last.rows;
-            if (last_nx)
-                prev.xidx_set(last_nx);
-            else
-                prev.touched();
+            prev.touched();
+            if (last_nx) prev.xidx_set(last_nx);
+            if (last_sec) prev.sec_set();
         } else if (last.rows) {
             // the merged run's index is the sum of the two indexes
             const uint32_t nx = prev.xidx_n;
             const bool add = nx && prev.xidx_valid(nx) && last.xidx_valid(nx);
+            // the merged run keeps a second column while its parts had one
+            const bool sec = prev.sec_valid() && last.sec_valid();
             DevBuf<uint64_t> out = s->take(n);
+            DevBuf<uint32_t> out_sec;
+            if (sec) out_sec = s->take_sec(n + kSecPad);
             const size_t tb = merge_disjoint_temp_bytes(prev.rows, last.rows);
             void* temp = s->scratch(tb);
             const int ev = s->begin_op();
             check(merge_disjoint(prev.keys.ptr, prev.rows, last.keys.ptr, last.rows, out.ptr, temp, tb, s->stream,
-                                 s->kernel_timer(ev)));
-            s->end_op(ev, kOpMergeRuns, 16.0 * n);
+                                 s->kernel_timer(ev), sec ? out_sec.ptr : nullptr));
+            s->end_op(ev, kOpMergeRuns, (sec ? 20.0 : 16.0) * n);
             if (add) check(xindex_add(prev.xidx.ptr, last.xidx.ptr, nx, s->stream));
             s->give(std::move(prev.keys));
             prev.keys = std::move(out);
             prev.rows = n;
-            if (add)
-                prev.xidx_set(nx);
-            else
-                prev.touched();
-            bytes += 16.0 * n;
+            prev.touched();
+            if (add) prev.xidx_set(nx);
+            if (sec) {
+                s->give_sec(std::move(prev.sec));
+                prev.sec = std::move(out_sec);
+                prev.sec_set();
+            }
+            bytes += (sec ? 20.0 : 16.0) * n;
         }
         s->give(std::move(last.keys));
         s->give_x(std::move(last.xidx));
+        s->give_sec(std::move(last.sec));
         ts.runs.pop_back();
     }
     return bytes;
@@ -1231,6 +1288,7 @@ extern "C" int sparkalog_hip_store_clear(sparkalog_hip_store* s) {
         for (auto& r : st.runs) {
             s->give(std::move(r.keys));
             s->give_x(std::move(r.xidx));
+            s->give_sec(std::move(r.sec));
         }
         st.runs.clear();
     }
@@ -1549,11 +1607,15 @@ void session_step(sparkalog_hip_store* s) {
             };
             add_right(ts.full);
             for (const auto& x : ts.runs) add_right(x);
+            bool column_probe = false;
             if (rights.count) {
                 rt.found.reserve(probe_found_words(M));
                 bool parts_fit = ts.full.rows <= 0xFFFFFFFFull;
                 for (const auto& x : ts.runs) parts_fit = parts_fit && x.rows <= 0xFFFFFFFFull;
-                if (M <= s->sparse_probe_rows && vmax < (1u << 24) && parts_fit) {
+                // FULL's parts are probed through their first-column indexes
+                // while first values stay below 2^24
+                const bool indexed = vmax < (1u << 24) && parts_fit;
+                if (M <= s->sparse_probe_rows && indexed) {
                     XRights xr{};
                     xr.count = 0;
                     auto add_part = [&](DevRelation& x) {
@@ -1570,20 +1632,45 @@ void session_step(sparkalog_hip_store* s) {
                                        s->kernel_timer(pev)));
                     s->end_op(pev, kOpProbeSparse, 8.0 * M + 8.0 * right_rows);
                     t.anti_slot = pev;
+                } else if (indexed) {
+                    // stream only the second columns of FULL's parts (4 bytes a row)
+                    XSecRights xr{};
+                    xr.count = 0;
+                    auto add_part = [&](DevRelation& x) {
+                        if (!x.rows) return;
+                        ensure_xidx(s, x, n_x);
+                        ensure_sec(s, x);
+                        xr.second[xr.count] = x.sec.ptr;
+                        xr.xstart[xr.count] = x.xidx.ptr;
+                        ++xr.count;
+                    };
+                    add_part(ts.full);
+                    for (auto& x : ts.runs) add_part(x);
+                    const int pev = s->begin_op();
+                    check(probe_xsec(rt.cand_a.ptr, M, rt.shift, xr, n_x, rt.found.ptr, s->stream,
+                                     s->kernel_timer(pev)));
+                    s->stats[kOpAntiJoin].kernel = "probe_xsec_kernel";
+                    s->end_op(pev, kOpAntiJoin, 8.0 * M + 4.0 * right_rows);
+                    t.anti_slot = pev;
+                    column_probe = true;
                 } else {
                     const size_t ptb = probe_grouped_temp_bytes(M, rights.count);
                     void* ptemp = s->scratch(ptb);
                     const int pev = s->begin_op();
                     check(probe_grouped(rt.cand_a.ptr, nullptr, M, rt.shift, rights, rt.found.ptr, ptemp, ptb,
                                         s->stream, s->kernel_timer(pev)));
+                    s->stats[kOpAntiJoin].kernel = "anti_join_kernel";
                     s->end_op(pev, kOpAntiJoin, 8.0 * M + 8.0 * right_rows);
                     t.anti_slot = pev;
                 }
                 found = rt.found.ptr;
             }
-            algo_bytes += 8.0 * right_rows;
+            algo_bytes += (column_probe ? 4.0 : 8.0) * right_rows;
             t.newt.reserve(M);
             t.new_run = s->take(M);
+            // the new run carries its second column (and, when applied, its
+            // first-column index) while FULL can be probed by columns
+            if (vmax < (1u << 24)) t.new_sec = s->take_sec(M + kSecPad);
             t.fused = true;
             t.contrib_bound = M;
         }
@@ -1606,7 +1693,7 @@ void session_step(sparkalog_hip_store* s) {
             uint64_t* count = &cdev->distinct_rows[r];
             if (fuse >= 0) {
                 TargetRuntime& t = targets[static_cast<size_t>(fuse)];
-                out = PairOutput{t.newt.ptr, nullptr, nullptr, t.new_run.ptr};
+                out = PairOutput{t.newt.ptr, nullptr, nullptr, t.new_run.ptr, t.new_sec.ptr};
                 count = &cdev->newt_rows[fuse];
             }
             check(segmented_distinct_finish(rt.cand_a.ptr, M, rt.shift, known_compressed, rt.cand_b.ptr,
@@ -1783,12 +1870,19 @@ void session_step(sparkalog_hip_store* s) {
             DevRelation run;
             run.keys = std::move(t.new_run);
             run.rows = N;
-            // indexed while the anti-join uses the index (or probes sparsely)
-            if ((use_x || (vmax < (1u << 24) && N <= s->sparse_probe_rows)) && N <= 0xFFFFFFFFull)
+            const bool with_sec = t.new_sec.ptr != nullptr;
+            if (with_sec) {
+                run.sec = std::move(t.new_sec);
+                run.sec_set();
+            }
+            // indexed while the anti-join uses the index (probing sparsely or
+            // by columns), so merged runs get their index by addition
+            if ((use_x || with_sec || (vmax < (1u << 24) && N <= s->sparse_probe_rows)) && N <= 0xFFFFFFFFull)
                 ensure_xidx(s, run, n_x);
             ts.runs.push_back(std::move(run));
         } else {
             s->give(std::move(t.new_run));
+            s->give_sec(std::move(t.new_sec));
         }
         algo_bytes += settle_runs(s, ts, false);
         std::swap(ts.delta, ts.newt);

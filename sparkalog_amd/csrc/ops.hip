@@ -470,7 +470,8 @@ __device__ __forceinline__ uint32_t merge_path_lds(const uint64_t* a, uint32_t n
 // for another. Tile t covers merged rows [t*kMergeTile, (t+1)*kMergeTile),
 // whose split between the inputs comes from the partition pass.
 __global__ __launch_bounds__(kBlock) void merge_disjoint_kernel(const uint64_t* a, size_t na, const uint64_t* b,
-                                                                size_t nb, uint64_t* out, const size_t* part) {
+                                                                size_t nb, uint64_t* out, uint32_t* out_second,
+                                                                const size_t* part) {
     __shared__ uint64_t lds[kMergeTile];
     const size_t tile = blockIdx.x;
     const size_t total = na + nb;
@@ -508,6 +509,8 @@ __global__ __launch_bounds__(kBlock) void merge_disjoint_kernel(const uint64_t* 
         if (diag + i < n) lds[diag + i] = v[i];
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < n; i += kBlock) out[d0 + i] = lds[i];
+    if (out_second)
+        for (uint32_t i = threadIdx.x; i < n; i += kBlock) out_second[d0 + i] = static_cast<uint32_t>(lds[i]);
 }
 
 __global__ void merge_partition_kernel(PairInput left, size_t nl, PairInput right, size_t nr, size_t* part,
@@ -1195,7 +1198,8 @@ size_t merge_disjoint_temp_bytes(size_t left_rows, size_t right_rows) {
 }
 
 hipError_t merge_disjoint(const uint64_t* left, size_t left_rows, const uint64_t* right, size_t right_rows,
-                          uint64_t* out, void* temp, size_t temp_bytes, hipStream_t stream, const KernelTimer* timer) {
+                          uint64_t* out, void* temp, size_t temp_bytes, hipStream_t stream, const KernelTimer* timer,
+                          uint32_t* out_second) {
     const size_t total = left_rows + right_rows;
     if (total == 0) return hipSuccess;
     if (temp_bytes < merge_disjoint_temp_bytes(left_rows, right_rows)) return hipErrorInvalidValue;
@@ -1206,7 +1210,7 @@ hipError_t merge_disjoint(const uint64_t* left, size_t left_rows, const uint64_t
                                                                                     part, tiles + 1);
     timer_start(timer, stream);
     merge_disjoint_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(left, left_rows, right, right_rows,
-                                                                               out, part);
+                                                                               out, out_second, part);
     timer_stop(timer, stream);
     return hipGetLastError();
 }

@@ -42,6 +42,9 @@ struct PairOutput {
     uint32_t* first;    // written when non-null
     uint32_t* second;
     uint64_t* packed_copy = nullptr;  // second packed copy, written when non-null
+    // second column of the rows (u32), written when non-null; only the
+    // segmented distinct writes it
+    uint32_t* second_copy = nullptr;
 };
 
 // ---- radix sort + unique (distinct) ----
@@ -102,11 +105,12 @@ hipError_t sorted_union(PairInput left, size_t left_rows, PairInput right, size_
                         hipStream_t stream, const KernelTimer* timer = nullptr);
 
 // Merge of two sorted packed arrays that share no key (FULL's runs):
-// out[0, left_rows + right_rows) in order.
+// out[0, left_rows + right_rows) in order, and the second column of the
+// merged rows in out_second when it is non-null.
 size_t merge_disjoint_temp_bytes(size_t left_rows, size_t right_rows);
 hipError_t merge_disjoint(const uint64_t* left, size_t left_rows, const uint64_t* right, size_t right_rows,
                           uint64_t* out, void* temp, size_t temp_bytes, hipStream_t stream,
-                          const KernelTimer* timer = nullptr);
+                          const KernelTimer* timer = nullptr, uint32_t* out_second = nullptr);
 
 size_t anti_join_temp_bytes(size_t left_rows, uint32_t right_inputs = 1);
 // left \ right; `count32`/`count64` receive the output row count (either may
@@ -149,6 +153,25 @@ struct XRights {
 // first values below n_x); found_words as probe_grouped's.
 hipError_t probe_sparse(const uint64_t* keys, size_t rows, uint32_t minor_bits, const XRights& rights, uint32_t n_x,
                         uint64_t* found_words, hipStream_t stream, const KernelTimer* timer = nullptr);
+// FULL's runs by first-column index and second column only: the rows of run
+// q with first value x have second values second[q][xstart[q][x],
+// xstart[q][x+1]), ascending (4 bytes per row instead of 8).
+struct XSecRights {
+    const uint32_t* second[kMaxAntiRights];
+    const uint32_t* xstart[kMaxAntiRights];
+    uint32_t count;
+};
+// probe_grouped over FULL's second columns: for candidate rows grouped by
+// their first value in ascending order (compressed keys, first value below
+// n_x), sets bit i of found_words when candidate i is in some run. Each tile
+// streams, per run, the second values of its first-value range through LDS.
+// Every second column passed to probe_xsec is allocated with kSecPad rows
+// past its last row (read, never used).
+constexpr size_t kSecPad = 4;
+hipError_t probe_xsec(const uint64_t* keys, size_t rows, uint32_t minor_bits, const XSecRights& rights,
+                      uint32_t n_x, uint64_t* found_words, hipStream_t stream, const KernelTimer* timer = nullptr);
+// second[i] = low 32 bits of keys[i] (the second column of packed rows).
+hipError_t second_column(const uint64_t* keys, size_t rows, uint32_t* second, hipStream_t stream);
 // left \ (rights' union) for sorted unique packed left rows (count *left_rows
 // <= bound) whose first values are below n_x; writes out.packed and
 // out.packed_copy (out.first must be null) and the row count to *count64.

@@ -35,19 +35,17 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
-// Valid lanes whose digit (the low `bits` bits of `digit`, bits <= MaxBits,
-// the same in every lane) equals this lane's; the stable rank of a lane
-// within its digit is lanes_below of the result. One ballot per digit bit;
-// sb is all ones where this lane's bit is set, so ~(ballot ^ sb) keeps the
-// lanes that agree with it on that bit, one 32-bit bit op per half instead
-// of a 64-bit select.
-template <int MaxBits>
-__device__ __forceinline__ uint64_t match_digit(uint32_t digit, bool valid, int bits = MaxBits) {
+// Valid lanes whose `Bits`-bit digit equals this lane's (the stable rank of a
+// lane within its digit is lanes_below of the result). One ballot per digit
+// bit; sb is all ones where this lane's bit is set, so ~(ballot ^ sb) keeps
+// the lanes that agree with it on that bit: one 32-bit bit op per half
+// (v_bitop3) instead of a 64-bit select.
+template <int Bits>
+__device__ __forceinline__ uint64_t match_digit(uint32_t digit, bool valid) {
     const uint64_t v = ballot(valid);
     uint32_t lo = static_cast<uint32_t>(v), hi = static_cast<uint32_t>(v >> 32);
 #pragma unroll
-    for (int b = 0; b < MaxBits; ++b) {
-        if (b >= bits) break;  // wave-uniform
+    for (int b = 0; b < Bits; ++b) {
         const uint32_t sb = static_cast<uint32_t>(__builtin_amdgcn_sbfe(static_cast<int32_t>(digit), b, 1));
         const uint64_t m = ballot(sb != 0);
         lo &= ~(static_cast<uint32_t>(m) ^ sb);

@@ -11,8 +11,8 @@
 //   2. runs longer than a tile go, gathered, through the global onesweep sort
 //      (rare: a run is one source's candidates in one iteration);
 //   3. every tile sorts its keys in LDS by (local run rank, minor) with stable
-//      LSD passes over the bit range where its keys differ (digits of at most
-//      9 bits: 24..27 varying bits take three passes), drops duplicates, and
+//      LSD passes in 9-bit digits over the bit range where its keys differ
+//      (up to 27 varying bits take three passes), drops duplicates, and
 //      writes the expanded pairs (major << 32 | minor) at an offset found by
 //      decoupled look-back.
 #include "ops.hpp"
@@ -27,12 +27,13 @@ constexpr int kSegBlock = 512;                // threads of a tile block (8 wave
 constexpr int kSegWaves = kSegBlock / kWave;
 constexpr int kSegItems = kSegCap / kSegBlock;  // 8 keys per thread
 constexpr int kSegWaveRows = kSegItems * kWave;
-constexpr int kSegDigitBits = 9;              // widest digit of a tile pass
+constexpr int kSegDigitBits = 9;              // digit of a tile pass
 constexpr int kSegRadix = 1 << kSegDigitBits;
 
 __device__ __forceinline__ void store_pair(const PairOutput& out, size_t i, uint64_t k) {
     if (out.packed) out.packed[i] = k;
     if (out.packed_copy) out.packed_copy[i] = k;
+    if (out.second_copy) out.second_copy[i] = static_cast<uint32_t>(k);
     if (out.first) {
         out.first[i] = static_cast<uint32_t>(k >> 32);
         out.second[i] = static_cast<uint32_t>(k);
@@ -297,40 +298,52 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(const uint64_t* key
             }
             vary = n ? (o ^ a) : 0;
         }
-        // stable LSD passes in LDS over the varying bit range [lb, lb + width)
-        // of the keys, in equal digits of at most kSegDigitBits bits
+        // stable LSD passes in LDS over the bit range [lb, lb + width) where the
+        // keys differ, in 9-bit digits (a last digit reaching past the range
+        // only adds bits that are equal in every key of the tile)
         int lb = 0, width = 0;
         if (vary) {
             lb = __builtin_ctzll(static_cast<uint64_t>(vary));
             width = 64 - __builtin_clzll(static_cast<uint64_t>(vary)) - lb;
         }
         const int passes = (width + kSegDigitBits - 1) / kSegDigitBits;
-        const int step = passes ? (width + passes - 1) / passes : 0;
         for (int pass = 0; pass < passes; ++pass) {  // block-uniform
-            const int shift = lb + pass * step;
-            const int bits = step < lb + width - shift ? step : lb + width - shift;
-            const uint32_t dmask = (1u << bits) - 1u;
-            for (uint32_t i = threadIdx.x; i < (static_cast<uint32_t>(kSegWaves) << bits); i += kSegBlock)
-                wave_count[i >> bits][i & dmask] = 0;
+            const int shift = lb + pass * kSegDigitBits;
+            for (int i = threadIdx.x; i < kSegWaves * kSegRadix; i += kSegBlock) (&wave_count[0][0])[i] = 0;
             __syncthreads();
             uint32_t ranks[kSegItems];
+            if (pass == 0) {
+                // the first pass need not be stable (the order it starts from
+                // carries no information): a rank within (wave, digit) from an
+                // LDS atomic replaces the ballot match
 #pragma unroll
-            for (int i = 0; i < kSegItems; ++i) {
-                if (i >= items) continue;
-                const uint32_t p = wave * wave_len + i * kWave + lane;
-                const bool valid = p < n;
-                const uint32_t digit = static_cast<uint32_t>(k[i] >> shift) & dmask;
-                const uint64_t peers = match_digit<kSegDigitBits>(digit, valid, bits);
-                const uint32_t before = valid ? wave_count[wave][digit] : 0;
-                const uint32_t below = lanes_below(peers);
-                ranks[i] = before + below;
-                if (valid && below == 0) wave_count[wave][digit] = before + static_cast<uint32_t>(__popcll(peers));
+                for (int i = 0; i < kSegItems; ++i) {
+                    if (i >= items) continue;
+                    const uint32_t p = wave * wave_len + i * kWave + lane;
+                    const uint32_t digit = static_cast<uint32_t>(k[i] >> shift) & (kSegRadix - 1);
+                    if (p < n) ranks[i] = atomicAdd(&wave_count[wave][digit], 1u);
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < kSegItems; ++i) {
+                    if (i >= items) continue;
+                    const uint32_t p = wave * wave_len + i * kWave + lane;
+                    const bool valid = p < n;
+                    const uint32_t digit = static_cast<uint32_t>(k[i] >> shift) & (kSegRadix - 1);
+                    const uint64_t peers = match_digit<kSegDigitBits>(digit, valid);
+                    const uint32_t before = wave_count[wave][digit];  // any lane: digit < kSegRadix
+                    const uint32_t below = lanes_below(peers);
+                    ranks[i] = before + below;
+                    if (valid && below == 0)
+                        wave_count[wave][digit] = before + static_cast<uint32_t>(__popcll(peers));
+                }
             }
             __syncthreads();
             // digit d's start per wave: thread d owns digit d
+            static_assert(kSegRadix == kSegBlock, "one digit per thread");
             const uint32_t d = threadIdx.x;
             uint32_t tc = 0;
-            if (d <= dmask) {
+            {
 #pragma unroll
                 for (int w = 0; w < kSegWaves; ++w) {
                     const uint32_t c = wave_count[w][d];
@@ -340,7 +353,7 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(const uint64_t* key
             }
             uint32_t total_unused;
             const uint32_t local_start = block_exclusive_sum<uint32_t, kSegWaves>(tc, &total_unused, scan_lds);
-            if (d <= dmask) {
+            {
 #pragma unroll
                 for (int w = 0; w < kSegWaves; ++w) wave_count[w][d] += local_start;
             }
@@ -350,7 +363,7 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(const uint64_t* key
                 if (i >= items) continue;
                 const uint32_t p = wave * wave_len + i * kWave + lane;
                 if (p < n) {
-                    const uint32_t digit = static_cast<uint32_t>(k[i] >> shift) & dmask;
+                    const uint32_t digit = static_cast<uint32_t>(k[i] >> shift) & (kSegRadix - 1);
                     tile_keys[wave_count[wave][digit] + ranks[i]] = k[i];
                 }
             }
@@ -358,8 +371,9 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(const uint64_t* key
 #pragma unroll
             for (int i = 0; i < kSegItems; ++i) {
                 if (i >= items) continue;
-                const uint32_t p = wave * wave_len + i * kWave + lane;
-                if (p < n) k[i] = tile_keys[p];
+                // p < items * 512 <= kSegCap for every lane; lanes past n
+                // read keys nobody uses
+                k[i] = tile_keys[wave * wave_len + i * kWave + lane];
             }
             __syncthreads();
         }

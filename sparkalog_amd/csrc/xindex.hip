@@ -425,6 +425,157 @@ __global__ __launch_bounds__(kBlock) void probe_sparse_kernel(const uint64_t* ke
     if (lane_id() == 0 && row0 < rows) found_words[row0 / kWave] = m;
 }
 
+// Candidates in FULL, by second columns. Layout as the grouped probe: wave w
+// owns tile rows [w*512, w*512+512), item i of lane j is row w*512 + i*64 + j,
+// so an item's result is one ballot word. Per run, the tile's first-value
+// range [x_lo, x_hi] covers rows [xstart[x_lo], xstart[x_hi + 1]) of the run;
+// their second values stream through LDS in chunks of kPStage, and a
+// candidate (x, y) searches y only in the part of its own segment
+// [xstart[x], xstart[x+1]) that the chunk holds (the tile's index entries
+// are staged in LDS). A range longer than kPStreamLimit (a tile of few
+// first values with huge segments), or a first-value range wider than
+// kPXCap - 1, is searched in place instead.
+constexpr int kPTile = 2048;                   // candidates per tile
+constexpr int kPItems = kPTile / kBlock;       // 8 per thread
+constexpr int kPWaveRows = kPItems * kWave;    // 512 per wave
+constexpr int kPStage = 4096;                  // second values per LDS chunk (16 KB)
+constexpr int kPStageVecs = kPStage / 4 / kBlock;  // 16-byte loads per thread and chunk
+constexpr size_t kPStreamLimit = size_t{1} << 19;
+constexpr int kPXCap = 769;                    // index entries of a tile's first-value range kept in LDS
+
+__global__ __launch_bounds__(kBlock) void probe_xsec_kernel(const uint64_t* keys, size_t rows, uint32_t minor_bits,
+                                                            XSecRights rights, uint64_t* found_words) {
+    __shared__ __align__(16) uint32_t stage[kPStage];
+    __shared__ uint32_t xs_lds[kPXCap];  // the tile's index entries of one run
+    __shared__ uint32_t red_lo[kWavesPerBlock], red_hi[kWavesPerBlock];
+    const size_t base = static_cast<size_t>(blockIdx.x) * kPTile;
+    if (base >= rows) return;
+    const uint32_t lane = lane_id(), wave = wave_id();
+    const size_t n = rows - base < static_cast<size_t>(kPTile) ? rows - base : kPTile;
+    const uint64_t minor_mask = (1ull << minor_bits) - 1;
+    uint32_t x[kPItems], y[kPItems];
+    uint32_t valid = 0;  // bit i: item i is a row of the tile
+    uint32_t xmin = 0xFFFFFFFFu, xmax = 0;
+#pragma unroll
+    for (int i = 0; i < kPItems; ++i) {
+        const size_t r = wave * kPWaveRows + i * kWave + lane;
+        x[i] = y[i] = 0;
+        if (r < n) {
+            const uint64_t k = keys[base + r];
+            x[i] = static_cast<uint32_t>(k >> minor_bits);
+            y[i] = static_cast<uint32_t>(k & minor_mask);
+            valid |= 1u << i;
+            xmin = x[i] < xmin ? x[i] : xmin;
+            xmax = x[i] > xmax ? x[i] : xmax;
+        }
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const uint32_t omin = __shfl_xor(xmin, off, kWave), omax = __shfl_xor(xmax, off, kWave);
+        xmin = omin < xmin ? omin : xmin;
+        xmax = omax > xmax ? omax : xmax;
+    }
+    if (lane == 0) {
+        red_lo[wave] = xmin;
+        red_hi[wave] = xmax;
+    }
+    __syncthreads();
+    uint32_t x_lo = 0xFFFFFFFFu, x_hi = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w) {
+        x_lo = red_lo[w] < x_lo ? red_lo[w] : x_lo;
+        x_hi = red_hi[w] > x_hi ? red_hi[w] : x_hi;
+    }
+    uint32_t found = 0;  // bit i: item i is in some run
+    const uint32_t span_x = x_hi - x_lo + 2;  // index entries xstart[x_lo .. x_hi + 1]
+    const bool x_in_lds = span_x <= static_cast<uint32_t>(kPXCap);  // block-uniform
+    for (uint32_t q = 0; q < rights.count; ++q) {
+        const uint32_t* xs = rights.xstart[q];
+        const uint32_t* sec = rights.second[q];
+        if (x_in_lds) {
+            for (uint32_t j = threadIdx.x; j < span_x; j += kBlock) xs_lds[j] = xs[x_lo + j];
+            __syncthreads();
+        }
+        // the tile's rows of run q; item i's segment is [xstart[x_i], xstart[x_i + 1])
+        const size_t r0 = x_in_lds ? xs_lds[0] : xs[x_lo];
+        const size_t r1 = x_in_lds ? xs_lds[span_x - 1] : xs[x_hi + 1];
+        if (x_in_lds && r1 - r0 <= kPStreamLimit) {
+            // chunks start at multiples of 4 rows and are read 16 bytes a
+            // lane; the last vector may cover up to 3 rows past r1, inside
+            // the buffer's kSecPad rows of padding
+            for (size_t c0 = r0 & ~size_t{3}; c0 < r1; c0 += kPStage) {
+                const size_t c1 = c0 + kPStage < r1 ? c0 + kPStage : r1;
+                const uint32_t nvec = static_cast<uint32_t>((c1 - c0 + 3) / 4);
+                const uint4* chunk = reinterpret_cast<const uint4*>(sec + c0);
+                uint4* stage4 = reinterpret_cast<uint4*>(stage);
+#pragma unroll
+                for (int u = 0; u < kPStageVecs; ++u) {
+                    const uint32_t j = u * kBlock + threadIdx.x;
+                    if (j < nvec) stage4[j] = chunk[j];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int i = 0; i < kPItems; ++i) {
+                    if (!(((valid & ~found) >> i) & 1u)) continue;
+                    const uint32_t lo = xs_lds[x[i] - x_lo], hi = xs_lds[x[i] - x_lo + 1];
+                    if (lo >= c1 || hi <= c0) continue;
+                    uint32_t a = static_cast<uint32_t>((lo > c0 ? lo : c0) - c0);
+                    const uint32_t e = static_cast<uint32_t>((hi < c1 ? hi : c1) - c0);
+                    // the chunk's slice of the segment is ascending: search
+                    // only the one slice whose values span y
+                    if (y[i] < stage[a] || y[i] > stage[e - 1]) continue;
+                    uint32_t b = e;
+                    while (a < b) {
+                        const uint32_t mid = (a + b) >> 1;
+                        if (stage[mid] < y[i])
+                            a = mid + 1;
+                        else
+                            b = mid;
+                    }
+                    if (a < e && stage[a] == y[i]) found |= 1u << i;
+                }
+                __syncthreads();
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < kPItems; ++i) {
+                if (!(((valid & ~found) >> i) & 1u)) continue;
+                // x < 2^24: 32-bit byte offsets from the uniform base. The
+                // empty asm keeps the offsets from being hoisted out of the
+                // run loop, where they would hold registers through the
+                // streaming path too.
+                uint32_t off = x[i] << 2;
+                asm volatile("" : "+v"(off));
+                const char* xs_bytes = reinterpret_cast<const char*>(xs);
+                const uint32_t lo = *reinterpret_cast<const uint32_t*>(xs_bytes + off);
+                const uint32_t hi = *reinterpret_cast<const uint32_t*>(xs_bytes + off + 4u);
+                uint32_t a = lo, b = hi;
+                while (a < b) {
+                    const uint32_t mid = a + ((b - a) >> 1);
+                    if (sec[mid] < y[i])
+                        a = mid + 1;
+                    else
+                        b = mid;
+                }
+                if (a < hi && sec[a] == y[i]) found |= 1u << i;
+            }
+        }
+        __syncthreads();  // xs_lds is rewritten for the next run
+    }
+    // bit j of word (base + wave*512 + i*64) / 64 is row base + wave*512 + i*64 + j
+#pragma unroll
+    for (int i = 0; i < kPItems; ++i) {
+        const uint64_t m = ballot(((found >> i) & 1u) != 0);
+        const size_t row0 = base + wave * kPWaveRows + i * kWave;
+        if (lane == 0 && row0 < rows) found_words[row0 / kWave] = m;
+    }
+}
+
+__global__ void second_column_kernel(const uint64_t* keys, size_t rows, uint32_t* second) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < rows; i += stride)
+        second[i] = static_cast<uint32_t>(keys[i]);
+}
+
 size_t xanti_tiles(size_t bound) { return (bound + kXTile - 1) / kXTile; }
 size_t xcompact_tiles(size_t bound) { return (bound + kCTile - 1) / kCTile; }
 
@@ -460,6 +611,27 @@ hipError_t probe_sparse(const uint64_t* keys, size_t rows, uint32_t minor_bits, 
     probe_sparse_kernel<<<static_cast<unsigned>(blocks), kBlock, 0, stream>>>(keys, rows, minor_bits, rights, n_x,
                                                                              found_words);
     timer_stop(timer, stream);
+    return hipGetLastError();
+}
+
+hipError_t probe_xsec(const uint64_t* keys, size_t rows, uint32_t minor_bits, const XSecRights& rights,
+                      uint32_t n_x, uint64_t* found_words, hipStream_t stream, const KernelTimer* timer) {
+    if (rows == 0) return hipSuccess;
+    if (minor_bits == 0 || minor_bits > 32 || rights.count > static_cast<uint32_t>(kMaxAntiRights) || n_x == 0)
+        return hipErrorInvalidValue;
+    const size_t tiles = (rows + kPTile - 1) / kPTile;
+    timer_start(timer, stream);
+    probe_xsec_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(keys, rows, minor_bits, rights,
+                                                                          found_words);
+    timer_stop(timer, stream);
+    return hipGetLastError();
+}
+
+hipError_t second_column(const uint64_t* keys, size_t rows, uint32_t* second, hipStream_t stream) {
+    if (rows == 0) return hipSuccess;
+    const size_t blocks = (rows + kBlock * 4 - 1) / (kBlock * 4);
+    second_column_kernel<<<static_cast<unsigned>(blocks < 8192 ? blocks : 8192), kBlock, 0, stream>>>(keys, rows,
+                                                                                                     second);
     return hipGetLastError();
 }
 

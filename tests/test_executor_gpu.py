@@ -85,8 +85,10 @@ def test_iteration_limit(stream):
     assert err.value.kind == "IterationLimit"
 
 
-def _tc(edges, stream, max_iterations=10_000):
+def _tc(edges, stream, max_iterations=10_000, sparse_probe_rows=None):
     store = RelationStore(stream)
+    if sparse_probe_rows is not None:
+        store.set_tuning(sparse_probe_rows=sparse_probe_rows)
     store.insert_static(R(0), edges)
     store.insert_recursive(R(1), edges, edges)
     summary = RecursiveExecutor.compile(plans.transitive_closure_scc(R(1), R(0)), store).run(store, max_iterations)
@@ -168,10 +170,12 @@ def test_long_candidate_runs(stream):
     assert summ.iterations == rsum.iterations and summ.total_new_rows == rsum.total_new_rows
 
 
-def test_mixed_run_lengths(stream):
+@pytest.mark.parametrize("sparse_probe_rows", [None, 0])
+def test_mixed_run_lengths(sparse_probe_rows, stream):
     # sources with 200..1,500 first-layer successors, each with 4 successors
     # of its own: candidate runs of 800..6,000 rows, on both sides of the
     # tile size and of the largest tile sorted in LDS, packed back to back
+    # (sparse_probe_rows=0: every iteration probes FULL by its columns)
     rng = np.random.default_rng(0x7A1)
     fans = rng.permutation(np.repeat([200, 700, 768, 1000, 1023, 1024, 1025, 1100, 1500], 6))
     e = set()
@@ -181,7 +185,28 @@ def test_mixed_run_lengths(stream):
     for m in range(1000, 21_000):
         e.update((m, 30_000 + int(t)) for t in rng.choice(8000, size=4, replace=False))
     edges = cols(sorted(e))
-    store, summ = _tc(edges, stream)
+    store, summ = _tc(edges, stream, sparse_probe_rows=sparse_probe_rows)
+    ref, rsum = _oracle_run({0: edges}, {1: (edges, edges)}, [rule(1, 1, 0)], [1])
+    full, expect = store.full(R(1)), ref.get(1)
+    assert np.array_equal(full[0], expect[0]) and np.array_equal(full[1], expect[1])
+    assert summ.iterations == rsum.iterations and summ.total_new_rows == rsum.total_new_rows
+
+
+@pytest.mark.parametrize("sparse_probe_rows", [None, 0])
+def test_huge_segment_probe(sparse_probe_rows, stream):
+    # node 0 reaches 600,000 nodes in one step: FULL's segment of first value
+    # 0 is longer than a tile streams (2^19 rows), so the candidates of 0 in
+    # the next iteration are searched in place; the other first values
+    # (the layer-1 nodes, one successor each) are probed through LDS
+    n1 = 600_000
+    layer1 = np.arange(1, n1 + 1, dtype=np.uint32)
+    layer2 = (n1 + 1 + (layer1 % 1000)).astype(np.uint32)
+    tail = np.array([n1 + 1, n1 + 1001, n1 + 1002], dtype=np.uint32)  # two more steps
+    src = np.concatenate([np.zeros(n1, np.uint32), layer1, tail[:-1]])
+    dst = np.concatenate([layer1, layer2, tail[1:]])
+    key = np.unique((src.astype(np.uint64) << 32) | dst)
+    edges = ((key >> 32).astype(np.uint32), (key & 0xFFFFFFFF).astype(np.uint32))
+    store, summ = _tc(edges, stream, sparse_probe_rows=sparse_probe_rows)
     ref, rsum = _oracle_run({0: edges}, {1: (edges, edges)}, [rule(1, 1, 0)], [1])
     full, expect = store.full(R(1)), ref.get(1)
     assert np.array_equal(full[0], expect[0]) and np.array_equal(full[1], expect[1])
