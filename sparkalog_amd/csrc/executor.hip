@@ -213,31 +213,30 @@ __device__ __forceinline__ uint32_t column_of(uint64_t key, uint32_t col) {
     return col == 0 ? static_cast<uint32_t>(key >> 32) : static_cast<uint32_t>(key);
 }
 
-// Join count over a dense CSR: per-row match counts -> exclusive offsets
-// (u64) with decoupled look-back; the last tile writes the total. Layout:
-// wave w owns tile rows [w*512, w*512+512), item i of lane j is row
+// Join count over a dense CSR, in tiles of kCountTile delta rows: each
+// row's first matching index row (for the emit), its exclusive offset within
+// its tile (u64), and the tile's total in tile_sums. join_scan_kernel turns
+// the totals into tile bases: the offset of row r is
+// local[r] + base[r / kCountTile]. Tiles never wait for one another (a
+// single-pass scan with look-back kept ~1,000 tiles waiting on aggregates).
+// Layout: wave w owns tile rows [w*512, w*512+512), item i of lane j is row
 // w*512 + i*64 + j, so loads are coalesced and the row-order prefix is one
 // wave scan per item plus one block step.
-// Join offsets of the delta rows (exclusive scan of their index matches)
-// and, for the emit, each row's first matching index row.
+constexpr int kCountItems = 8;
+constexpr int kCountTile = kBlock * kCountItems;
+
 __global__ __launch_bounds__(kBlock) void dense_join_count_kernel(const uint64_t* delta, size_t n, uint32_t key_col,
                                                                   const uint32_t* starts, uint32_t n_keys,
-                                                                  uint64_t* offsets, uint32_t* match, uint64_t* total,
-                                                                  uint64_t* status, uint32_t* counter,
-                                                                  uint32_t* error) {
-    constexpr int kItems = 8;
-    constexpr int kTile = kBlock * kItems;
-    constexpr int kWaveRows = kItems * kWave;
+                                                                  uint64_t* local, uint32_t* match,
+                                                                  uint64_t* tile_sums) {
+    constexpr int kWaveRows = kCountItems * kWave;
     __shared__ uint64_t wave_totals[kWavesPerBlock];
-    __shared__ uint64_t offset_slot;
-    __shared__ uint32_t tile_slot;
-    const uint32_t tile = next_tile(counter, &tile_slot);
-    const size_t base = static_cast<size_t>(tile) * kTile;
+    const size_t base = static_cast<size_t>(blockIdx.x) * kCountTile;
     const uint32_t lane = lane_id(), wave = wave_id();
-    uint64_t c[kItems];
+    uint64_t c[kCountItems];
     uint64_t running = 0;  // wave-local prefix in row order
 #pragma unroll
-    for (int i = 0; i < kItems; ++i) {
+    for (int i = 0; i < kCountItems; ++i) {
         const size_t row = base + wave * kWaveRows + i * kWave + lane;
         uint64_t cnt = 0;
         if (row < n) {
@@ -261,18 +260,37 @@ __global__ __launch_bounds__(kBlock) void dense_join_count_kernel(const uint64_t
         if (w < static_cast<int>(wave)) wave_base += wave_totals[w];
         tile_sum += wave_totals[w];
     }
-    if (wave == 0) {
-        const uint64_t excl = wave_lookback(status, tile, tile_sum, 0, error);
-        if (lane == 0) offset_slot = excl;
-    }
-    __syncthreads();
-    const uint64_t offset = offset_slot + wave_base;
 #pragma unroll
-    for (int i = 0; i < kItems; ++i) {
+    for (int i = 0; i < kCountItems; ++i) {
         const size_t row = base + wave * kWaveRows + i * kWave + lane;
-        if (row < n) offsets[row] = offset + c[i];
+        if (row < n) local[row] = wave_base + c[i];
     }
-    if (tile == tiles(n, kTile) - 1 && threadIdx.x == 0) *total = offset_slot + tile_sum;
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = tile_sum;
+}
+
+// Exclusive scan, in place, of the count's tile totals (one block; thread t
+// owns a contiguous segment), and the join total.
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void join_scan_kernel(uint64_t* sums, size_t n, uint64_t* total) {
+    __shared__ uint64_t scan_lds[kScanThreads / kWave];
+    const size_t per = (n + kScanThreads - 1) / kScanThreads;
+    const size_t b = static_cast<size_t>(threadIdx.x) * per;
+    const size_t e = b + per < n ? b + per : n;
+    uint64_t mine = 0;
+#pragma unroll 8
+    for (size_t j = b; j < e; ++j) mine += sums[j];
+    uint64_t all = 0;
+    uint64_t run = block_exclusive_sum<uint64_t, kScanThreads / kWave>(mine, &all, scan_lds);
+    for (size_t j = b; j < e; ++j) {
+        const uint64_t v = sums[j];
+        sums[j] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0) *total = all;
+}
+
+__device__ __forceinline__ uint64_t join_offset(const uint64_t* local, const uint64_t* base, size_t row) {
+    return local[row] + base[row / kCountTile];
 }
 
 struct EmitSpec {
@@ -284,13 +302,13 @@ struct EmitSpec {
     uint32_t shift;            // 32, or fewer bits to compress the sort key
 };
 
-constexpr int kEmitTile = 2048;   // output rows per tile
-constexpr int kEmitStage = 2048;  // delta rows staged per tile
+constexpr int kEmitTile = 1024;   // output rows per tile
+constexpr int kEmitStage = 1024;  // delta rows staged per tile (24 KB of LDS: 6 blocks a CU)
 
 // row_start[t] = the delta row that contains output t*kEmitTile (one thread
 // per tile; the last entry is n).
-__global__ void emit_partition_kernel(const uint64_t* offsets, size_t n, size_t total, size_t* row_start,
-                                      size_t n_tiles) {
+__global__ void emit_partition_kernel(const uint64_t* local, const uint64_t* base, size_t n, size_t total,
+                                      size_t* row_start, size_t n_tiles) {
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
     for (size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; t <= n_tiles; t += stride) {
         if (t == n_tiles) {
@@ -301,7 +319,7 @@ __global__ void emit_partition_kernel(const uint64_t* offsets, size_t n, size_t 
         size_t lo = 0, hi = n;
         while (lo < hi) {
             const size_t mid = (lo + hi) >> 1;
-            if (offsets[mid] <= o)
+            if (join_offset(local, base, mid) <= o)
                 lo = mid + 1;
             else
                 hi = mid;
@@ -317,8 +335,9 @@ __global__ void emit_partition_kernel(const uint64_t* offsets, size_t n, size_t 
 // match_base[row] = starts[key] of each delta row (written by the count),
 // so the emit reads it in row order instead of gathering the index's starts.
 __global__ __launch_bounds__(kBlock) void dense_join_emit_kernel(const uint64_t* delta, size_t n,
-                                                                 const uint32_t* right_other, const uint64_t* offsets,
-                                                                 const uint32_t* match_base, size_t total,
+                                                                 const uint32_t* right_other, const uint64_t* local,
+                                                                 const uint64_t* tile_base, const uint32_t* match_base,
+                                                                 size_t total,
                                                                  const size_t* row_start, EmitSpec spec,
                                                                  uint64_t* out) {
     constexpr int kPer = kEmitTile / kBlock;  // 8 output positions per thread
@@ -340,9 +359,9 @@ __global__ __launch_bounds__(kBlock) void dense_join_emit_kernel(const uint64_t*
             __syncthreads();
             for (uint32_t i = threadIdx.x; i < span; i += kBlock) {
                 const uint64_t dk = delta[l0 + i];
-                const uint64_t off = offsets[l0 + i];
+                const uint64_t off = join_offset(local, tile_base, l0 + i);
                 const uint32_t m0 = match_base[l0 + i];
-                const uint64_t next = l0 + i + 1 < n ? offsets[l0 + i + 1] : total;
+                const uint64_t next = l0 + i + 1 < n ? join_offset(local, tile_base, l0 + i + 1) : total;
                 const uint64_t cnt = next - off;
                 row_offset[i] = off;
                 row_key[i] = dk;
@@ -351,7 +370,7 @@ __global__ __launch_bounds__(kBlock) void dense_join_emit_kernel(const uint64_t*
                 if (i > 0 && cnt > 0 && off < o1) row_of[off - o0] = i;
             }
             __syncthreads();
-            // inclusive max-scan of row_of: thread owns positions [8t, 8t+8)
+            // inclusive max-scan of row_of: thread t owns positions [kPer*t, kPer*t + kPer)
             uint32_t local[kPer];
             uint32_t m = 0;
 #pragma unroll
@@ -391,13 +410,13 @@ __global__ __launch_bounds__(kBlock) void dense_join_emit_kernel(const uint64_t*
                 size_t lo = l0, hi = l1;
                 while (lo < hi) {
                     const size_t mid = (lo + hi) >> 1;
-                    if (offsets[mid] <= o)
+                    if (join_offset(local, tile_base, mid) <= o)
                         lo = mid + 1;
                     else
                         hi = mid;
                 }
                 dk = delta[lo - 1];
-                row_start_off = offsets[lo - 1];
+                row_start_off = join_offset(local, tile_base, lo - 1);
                 match = match_base[lo - 1];
             }
             const uint32_t r_other = right_other[match + static_cast<uint32_t>(o - row_start_off)];
@@ -499,8 +518,8 @@ struct CachedIndex {
 struct RuleRuntime {
     sparkalog_hip_rule plan{};
     CachedIndex index;
-    DevBuf<uint64_t> status;    // look-back words of the join count
-    DevBuf<uint64_t> offsets;   // join offsets per delta row
+    DevBuf<uint64_t> tile_base; // join count: tile totals, then their exclusive prefix
+    DevBuf<uint64_t> offsets;   // join offsets per delta row within its count tile
     DevBuf<uint32_t> match;     // first matching index row per delta row
     DevBuf<uint64_t> found;     // candidates already in FULL (one bit each)
     DevBuf<size_t> row_start;   // first delta row of each emit tile
@@ -1486,8 +1505,7 @@ void session_step(sparkalog_hip_store* s) {
     const uint32_t* relations = ss.relations.data();
     Counters* cdev = reinterpret_cast<Counters*>(s->counters_dev.ptr);
     Counters chost;
-    uint32_t* lb_counters = s->misc_dev.ptr;  // [0..127] look-back counters, [128] error
-    uint32_t* lb_error = s->misc_dev.ptr + 128;
+    uint32_t* lb_error = s->misc_dev.ptr + 128;  // device error flag of this step's kernels
     uint64_t& iterations = ss.iterations;
     uint64_t& total_new = ss.total_new;
     double& algo_bytes = ss.algo_bytes;
@@ -1527,15 +1545,15 @@ void session_step(sparkalog_hip_store* s) {
         }
         rt.offsets.reserve(L);
         rt.match.reserve(L);
-        const size_t n_tiles = tiles(L, kBlock * 8);
-        rt.status.reserve(n_tiles);
-        check(fill_u64(rt.status.ptr, n_tiles, 0, s->stream));
+        const size_t n_tiles = tiles(L, kCountTile);
+        rt.tile_base.reserve(n_tiles);
         const int ev = s->begin_op();
         timer_start(s->kernel_timer(ev), s->stream);
         dense_join_count_kernel<<<static_cast<unsigned>(n_tiles), kBlock, 0, s->stream>>>(
             ds.delta.keys.ptr, L, rt.plan.left_key, rt.index.starts.ptr, rt.index.n_keys, rt.offsets.ptr,
-            rt.match.ptr, &cdev->join_total[r], rt.status.ptr, &lb_counters[r], lb_error);
+            rt.match.ptr, rt.tile_base.ptr);
         timer_stop(s->kernel_timer(ev), s->stream);
+        join_scan_kernel<<<1, kScanThreads, 0, s->stream>>>(rt.tile_base.ptr, n_tiles, &cdev->join_total[r]);
         check(hipGetLastError());
         s->end_op(ev, kOpJoinCount, 8.0 * L);
     }
@@ -1569,12 +1587,13 @@ void session_step(sparkalog_hip_store* s) {
         const uint64_t emit_tiles = tiles(M, kEmitTile);
         const unsigned emit_grid = static_cast<unsigned>(emit_tiles < 65536 ? emit_tiles : 65536);
         rt.row_start.reserve(emit_tiles + 1);
-        emit_partition_kernel<<<grid_for(emit_tiles + 1), kBlock, 0, s->stream>>>(rt.offsets.ptr, rt.delta_rows, M,
-                                                                               rt.row_start.ptr, emit_tiles);
+        emit_partition_kernel<<<grid_for(emit_tiles + 1), kBlock, 0, s->stream>>>(
+            rt.offsets.ptr, rt.tile_base.ptr, rt.delta_rows, M, rt.row_start.ptr, emit_tiles);
         timer_start(s->kernel_timer(ev), s->stream);
         dense_join_emit_kernel<<<emit_grid, kBlock, 0, s->stream>>>(ds.delta.keys.ptr, rt.delta_rows,
                                                                      rt.index.other.ptr, rt.offsets.ptr,
-                                                                     rt.match.ptr, M, rt.row_start.ptr, spec,
+                                                                     rt.tile_base.ptr, rt.match.ptr, M,
+                                                                     rt.row_start.ptr, spec,
                                                                      rt.cand_a.ptr);
         timer_stop(s->kernel_timer(ev), s->stream);
         check(hipGetLastError());
