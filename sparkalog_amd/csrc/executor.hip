@@ -586,12 +586,9 @@ struct sparkalog_hip_store {
             b = std::move(pool[best]);
             pool.erase(pool.begin() + static_cast<long>(best));
         } else {
-            // runs only grow: pooled buffers smaller than this request are
-            // unlikely to be reused, so give their memory back first
-            for (auto& pb : pool)
-                if (pb.cap < n) pb.release();
-            pool.erase(std::remove_if(pool.begin(), pool.end(), [](const spk::DevBuf<uint64_t>& x) { return x.ptr == nullptr; }),
-                       pool.end());
+            // pooled buffers stay pooled: freeing them here (a device-wide
+            // synchronization and hipFree each) would make the next fixpoint
+            // allocate again, in the middle of its iterations
             b.reserve(n ? n : 1);
         }
         return b;
@@ -626,12 +623,7 @@ struct sparkalog_hip_store {
             b = std::move(secpool[best]);
             secpool.erase(secpool.begin() + static_cast<long>(best));
         } else {
-            for (auto& pb : secpool)
-                if (pb.cap < n) pb.release();
-            secpool.erase(std::remove_if(secpool.begin(), secpool.end(),
-                                         [](const spk::DevBuf<uint32_t>& x) { return x.ptr == nullptr; }),
-                          secpool.end());
-            b.reserve(n ? n : 1);
+            b.reserve(n ? n : 1);  // as take(): nothing is freed mid-run
         }
         return b;
     }
