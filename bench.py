@@ -50,7 +50,7 @@ def parse_args():
                         "continuous batching vs sequential; copy: plain device copy of --copy-bytes "
                         "(the known-size reference for HBM byte counters)")
     p.add_argument("--copy-bytes", type=int, default=1 << 30, help="copy mode: bytes copied per launch")
-    p.add_argument("--budget-pairs", type=float, default=4.5e9, help="batched/served: HBM budget in closure pairs")
+    p.add_argument("--budget-pairs", type=float, default=3e9, help="batched/served: HBM budget in closure pairs")
     p.add_argument("--requests", type=int, default=200, help="served: stream length")
     p.add_argument("--utilization", type=float, default=0.7, help="served: offered load of the plain path")
     p.add_argument("--plain-stride", type=int, default=1, help="batched: time the plain path on every k-th instance")

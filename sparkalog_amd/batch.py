@@ -84,7 +84,7 @@ def _wave_edges(src, dst, offsets, members, bits):
             np.concatenate(parts_d) if parts_d else np.empty(0, np.uint32))
 
 
-def run_batched(src, dst, offsets, node_counts, store: RelationStore, budget_pairs: float = 4.5e9,
+def run_batched(src, dst, offsets, node_counts, store: RelationStore, budget_pairs: float = 3e9,
                 max_iterations: int = 100_000) -> BatchResult:
     n = len(node_counts)
     edge_counts = np.diff(np.asarray(offsets, dtype=np.int64))

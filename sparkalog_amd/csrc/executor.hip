@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -43,6 +44,17 @@ inline void check(hipError_t e) {
 struct StatusError {
     int code;
 };
+
+// Stores keep released buffers pooled: a fresh multi-GB hipMalloc costs
+// tens of milliseconds per GB here. When hipMalloc fails, g_relief frees
+// pooled buffers of every store (smallest first) until `bytes` are free and
+// the allocation is retried; it returns false when no pooled buffer is left.
+bool (*g_relief)(size_t bytes) = nullptr;
+// guards every store's pools and the store registry
+inline std::mutex& pool_mutex() {
+    static std::mutex m;
+    return m;
+}
 
 __host__ __device__ inline size_t tiles(size_t n, size_t t) { return (n + t - 1) / t; }
 
@@ -86,6 +98,10 @@ struct DevBuf {
         release();
         void* p = nullptr;
         hipError_t e = hipMalloc(&p, c * sizeof(T));
+        while (e != hipSuccess && g_relief && g_relief(c * sizeof(T))) {
+            (void)hipGetLastError();
+            e = hipMalloc(&p, c * sizeof(T));
+        }
         if (e != hipSuccess) {
             (void)hipGetLastError();
             g_last_hip_error = static_cast<int>(e);
@@ -578,57 +594,70 @@ struct sparkalog_hip_store {
     bool segmented_distinct = true;             // sort grouped candidates per run in LDS
 
     spk::DevBuf<uint64_t> take(size_t n) {
-        size_t best = pool.size();
-        for (size_t i = 0; i < pool.size(); ++i)
-            if (pool[i].cap >= n && (best == pool.size() || pool[i].cap < pool[best].cap)) best = i;
         spk::DevBuf<uint64_t> b;
-        if (best < pool.size()) {
-            b = std::move(pool[best]);
-            pool.erase(pool.begin() + static_cast<long>(best));
-        } else {
-            // pooled buffers stay pooled: freeing them here (a device-wide
-            // synchronization and hipFree each) would make the next fixpoint
-            // allocate again, in the middle of its iterations
-            b.reserve(n ? n : 1);
+        {
+            std::lock_guard<std::mutex> lock(spk::pool_mutex());
+            size_t best = pool.size();
+            for (size_t i = 0; i < pool.size(); ++i)
+                if (pool[i].cap >= n && (best == pool.size() || pool[i].cap < pool[best].cap)) best = i;
+            if (best < pool.size()) {
+                b = std::move(pool[best]);
+                pool.erase(pool.begin() + static_cast<long>(best));
+            }
         }
+        // a miss allocates; pooled buffers stay pooled (freeing them here, a
+        // device-wide synchronization and hipFree each, would make the next
+        // fixpoint allocate again in the middle of its iterations)
+        if (!b.ptr) b.reserve(n ? n : 1);
         return b;
     }
     void give(spk::DevBuf<uint64_t>&& b) {
-        if (b.ptr) pool.push_back(std::move(b));
+        if (!b.ptr) return;
+        std::lock_guard<std::mutex> lock(spk::pool_mutex());
+        pool.push_back(std::move(b));
     }
     // first-column index buffers (all n_x + 1 entries within a fixpoint)
     std::vector<spk::DevBuf<uint32_t>> xpool;
     spk::DevBuf<uint32_t> take_x(size_t n) {
-        for (size_t i = 0; i < xpool.size(); ++i)
-            if (xpool[i].cap >= n) {
-                spk::DevBuf<uint32_t> b = std::move(xpool[i]);
-                xpool.erase(xpool.begin() + static_cast<long>(i));
-                return b;
-            }
         spk::DevBuf<uint32_t> b;
-        b.reserve(n);
+        {
+            std::lock_guard<std::mutex> lock(spk::pool_mutex());
+            for (size_t i = 0; i < xpool.size(); ++i)
+                if (xpool[i].cap >= n) {
+                    b = std::move(xpool[i]);
+                    xpool.erase(xpool.begin() + static_cast<long>(i));
+                    break;
+                }
+        }
+        if (!b.ptr) b.reserve(n);
         return b;
     }
     void give_x(spk::DevBuf<uint32_t>&& b) {
-        if (b.ptr) xpool.push_back(std::move(b));
+        if (!b.ptr) return;
+        std::lock_guard<std::mutex> lock(spk::pool_mutex());
+        xpool.push_back(std::move(b));
     }
     // second-column buffers of FULL's parts (sized like runs: best fit)
     std::vector<spk::DevBuf<uint32_t>> secpool;
     spk::DevBuf<uint32_t> take_sec(size_t n) {
-        size_t best = secpool.size();
-        for (size_t i = 0; i < secpool.size(); ++i)
-            if (secpool[i].cap >= n && (best == secpool.size() || secpool[i].cap < secpool[best].cap)) best = i;
         spk::DevBuf<uint32_t> b;
-        if (best < secpool.size()) {
-            b = std::move(secpool[best]);
-            secpool.erase(secpool.begin() + static_cast<long>(best));
-        } else {
-            b.reserve(n ? n : 1);  // as take(): nothing is freed mid-run
+        {
+            std::lock_guard<std::mutex> lock(spk::pool_mutex());
+            size_t best = secpool.size();
+            for (size_t i = 0; i < secpool.size(); ++i)
+                if (secpool[i].cap >= n && (best == secpool.size() || secpool[i].cap < secpool[best].cap)) best = i;
+            if (best < secpool.size()) {
+                b = std::move(secpool[best]);
+                secpool.erase(secpool.begin() + static_cast<long>(best));
+            }
         }
+        if (!b.ptr) b.reserve(n ? n : 1);  // as take()
         return b;
     }
     void give_sec(spk::DevBuf<uint32_t>&& b) {
-        if (b.ptr) secpool.push_back(std::move(b));
+        if (!b.ptr) return;
+        std::lock_guard<std::mutex> lock(spk::pool_mutex());
+        secpool.push_back(std::move(b));
     }
     bool x_anti = false;  // anti-join FULL through its first-column index when it fits
     uint64_t sparse_probe_rows = spk::kSparseProbeRows;
@@ -1172,7 +1201,55 @@ extern "C" const char* sparkalog_hip_status_string(int status) {
     }
 }
 
+// Registry of live stores, for out-of-memory relief across stores (served
+// lanes keep one store each, on their own threads).
+static std::vector<sparkalog_hip_store*>& stores() {
+    static std::vector<sparkalog_hip_store*> v;
+    return v;
+}
+
+// g_relief: move pooled buffers of every store out of the pools, smallest
+// first, until `bytes` are covered, then free them outside the lock (each
+// free synchronizes the device first, so no queued kernel loses a buffer).
+static bool relieve_pools(size_t bytes) {
+    std::vector<spk::DevBuf<uint64_t>> free64;
+    std::vector<spk::DevBuf<uint32_t>> free32;
+    {
+        std::lock_guard<std::mutex> lock(spk::pool_mutex());
+        struct Pooled {
+            size_t bytes;
+            spk::DevBuf<uint64_t>* b64;
+            spk::DevBuf<uint32_t>* b32;
+        };
+        std::vector<Pooled> all;
+        for (auto* st : stores()) {
+            for (auto& b : st->pool) all.push_back({b.cap * sizeof(uint64_t), &b, nullptr});
+            for (auto& b : st->secpool) all.push_back({b.cap * sizeof(uint32_t), nullptr, &b});
+            for (auto& b : st->xpool) all.push_back({b.cap * sizeof(uint32_t), nullptr, &b});
+        }
+        std::sort(all.begin(), all.end(), [](const Pooled& a, const Pooled& b) { return a.bytes < b.bytes; });
+        size_t got = 0;
+        for (auto& e : all) {
+            if (got >= bytes) break;
+            got += e.bytes;
+            if (e.b64)
+                free64.push_back(std::move(*e.b64));
+            else
+                free32.push_back(std::move(*e.b32));
+        }
+        auto empty64 = [](const spk::DevBuf<uint64_t>& x) { return x.ptr == nullptr; };
+        auto empty32 = [](const spk::DevBuf<uint32_t>& x) { return x.ptr == nullptr; };
+        for (auto* st : stores()) {
+            st->pool.erase(std::remove_if(st->pool.begin(), st->pool.end(), empty64), st->pool.end());
+            st->secpool.erase(std::remove_if(st->secpool.begin(), st->secpool.end(), empty32), st->secpool.end());
+            st->xpool.erase(std::remove_if(st->xpool.begin(), st->xpool.end(), empty32), st->xpool.end());
+        }
+    }
+    return !free64.empty() || !free32.empty();  // freed here, by the destructors
+}
+
 #define SPK_GUARD_BEGIN try {
+
 #define SPK_GUARD_END                                    \
     }                                                    \
     catch (const StatusError& e) {                       \
@@ -1190,6 +1267,11 @@ extern "C" int sparkalog_hip_store_create(sparkalog_hip_store** out, void* strea
     SPK_GUARD_BEGIN
     auto* s = new sparkalog_hip_store();
     s->stream = static_cast<hipStream_t>(stream);
+    {
+        std::lock_guard<std::mutex> lock(spk::pool_mutex());
+        stores().push_back(s);
+        spk::g_relief = relieve_pools;
+    }
     for (int i = 0; i < kOpCount; ++i) {
         s->stats[i].name = kOpNames[i];
         s->stats[i].kernel = kOpKernels[i];
@@ -1209,6 +1291,11 @@ extern "C" void sparkalog_hip_store_destroy(sparkalog_hip_store* store) {
     if (!store) return;
     (void)hipStreamSynchronize(store->stream);
     destroy_session(store);
+    {
+        std::lock_guard<std::mutex> lock(spk::pool_mutex());
+        auto& v = stores();
+        v.erase(std::remove(v.begin(), v.end(), store), v.end());
+    }
     delete store;
 }
 
@@ -1560,7 +1647,6 @@ void session_step(sparkalog_hip_store* s) {
         RelationState& ds = *find(s, rt.plan.delta_input);
         rt.cand_a.reserve(M ? M : 1);
         rt.cand_b.reserve(M ? M : 1);
-        rt.distinct_out.reserve(M ? M : 1);
         if (M == 0) {
             check(set_u64(&cdev->distinct_rows[r], 0, s->stream));
             continue;
@@ -1657,8 +1743,10 @@ void session_step(sparkalog_hip_store* s) {
                     };
                     add_part(ts.full);
                     for (auto& x : ts.runs) add_part(x);
+                    const size_t xtb = probe_xsec_temp_bytes(M);
+                    void* xtemp = s->scratch(xtb);
                     const int pev = s->begin_op();
-                    check(probe_xsec(rt.cand_a.ptr, M, rt.shift, xr, n_x, rt.found.ptr, s->stream,
+                    check(probe_xsec(rt.cand_a.ptr, M, rt.shift, xr, n_x, rt.found.ptr, xtemp, xtb, s->stream,
                                      s->kernel_timer(pev)));
                     s->stats[kOpAntiJoin].kernel = "probe_xsec_kernel";
                     s->end_op(pev, kOpAntiJoin, 8.0 * M + 4.0 * right_rows);
@@ -1700,6 +1788,8 @@ void session_step(sparkalog_hip_store* s) {
                 tb = distinct_temp_bytes(side_rows);
                 temp = s->scratch(tb);
             }
+            // a fused distinct writes NEWT and the new run instead
+            if (fuse < 0) rt.distinct_out.reserve(M);
             PairOutput out{rt.distinct_out.ptr, nullptr, nullptr};
             uint64_t* count = &cdev->distinct_rows[r];
             if (fuse >= 0) {
@@ -1714,6 +1804,7 @@ void session_step(sparkalog_hip_store* s) {
             const size_t tb = distinct_temp_bytes(M);
             void* temp = s->scratch(tb);
             PairInput in{rt.cand_a.ptr, nullptr, nullptr, nullptr};
+            rt.distinct_out.reserve(M);
             PairOutput out{rt.distinct_out.ptr, nullptr, nullptr};
             check(distinct(in, M, rt.cand_a.ptr, rt.cand_b.ptr, out, &cdev->distinct_rows[r], temp, tb,
                            known_compressed, s->stream));

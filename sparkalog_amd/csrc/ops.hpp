@@ -168,8 +168,10 @@ struct XSecRights {
 // Every second column passed to probe_xsec is allocated with kSecPad rows
 // past its last row (read, never used).
 constexpr size_t kSecPad = 4;
+size_t probe_xsec_temp_bytes(size_t rows);
 hipError_t probe_xsec(const uint64_t* keys, size_t rows, uint32_t minor_bits, const XSecRights& rights,
-                      uint32_t n_x, uint64_t* found_words, hipStream_t stream, const KernelTimer* timer = nullptr);
+                      uint32_t n_x, uint64_t* found_words, void* temp, size_t temp_bytes, hipStream_t stream,
+                      const KernelTimer* timer = nullptr);
 // second[i] = low 32 bits of keys[i] (the second column of packed rows).
 hipError_t second_column(const uint64_t* keys, size_t rows, uint32_t* second, hipStream_t stream);
 // left \ (rights' union) for sorted unique packed left rows (count *left_rows

@@ -427,47 +427,72 @@ __global__ __launch_bounds__(kBlock) void probe_sparse_kernel(const uint64_t* ke
 
 // Candidates in FULL, by second columns. Layout as the grouped probe: wave w
 // owns tile rows [w*512, w*512+512), item i of lane j is row w*512 + i*64 + j,
-// so an item's result is one ballot word. Per run, the tile's first-value
-// range [x_lo, x_hi] covers rows [xstart[x_lo], xstart[x_hi + 1]) of the run;
-// their second values stream through LDS in chunks of kPStage, and a
-// candidate (x, y) searches y only in the part of its own segment
-// [xstart[x], xstart[x+1]) that the chunk holds (the tile's index entries
-// are staged in LDS). A range longer than kPStreamLimit (a tile of few
-// first values with huge segments), or a first-value range wider than
-// kPXCap - 1, is searched in place instead.
+// so an item's result is one ballot word. Per run of FULL, the segment
+// bounds [xstart[x], xstart[x+1]) of the tile's first values go to LDS: the
+// index entries of the whole first-value range [x_lo, x_hi + 1] when it is
+// at most kPSeg entries (one coalesced read), else the two entries of each
+// group of equal first values (runs in row order). The tile's rows of that
+// run, [xstart[x_lo], xstart[x_hi + 1]), stream through LDS in chunks of
+// kPStage second values, and a candidate (x, y) searches y only in the part
+// of its own segment that the chunk holds. A tile with a wide range and more
+// than kPSeg / 2 groups (fewer than 4 candidates per first value), or whose
+// rows of some run exceed kPStreamLimit (few first values with huge
+// segments), is listed for probe_xsec_deferred_kernel instead, which searches
+// every run in place (keeping that path out of this kernel keeps it at 64
+// VGPRs).
 constexpr int kPTile = 2048;                   // candidates per tile
 constexpr int kPItems = kPTile / kBlock;       // 8 per thread
 constexpr int kPWaveRows = kPItems * kWave;    // 512 per wave
 constexpr int kPStage = 4096;                  // second values per LDS chunk (16 KB)
 constexpr int kPStageVecs = kPStage / 4 / kBlock;  // 16-byte loads per thread and chunk
 constexpr size_t kPStreamLimit = size_t{1} << 19;
-constexpr int kPXCap = 769;                    // index entries of a tile's first-value range kept in LDS
+constexpr int kPSeg = 512;                     // LDS index entries of a tile's segment bounds (19.6 KB of LDS: 8 blocks a CU)
 
 __global__ __launch_bounds__(kBlock) void probe_xsec_kernel(const uint64_t* keys, size_t rows, uint32_t minor_bits,
-                                                            XSecRights rights, uint64_t* found_words) {
+                                                            XSecRights rights, uint64_t* found_words,
+                                                            uint32_t* deferred) {
     __shared__ __align__(16) uint32_t stage[kPStage];
-    __shared__ uint32_t xs_lds[kPXCap];  // the tile's index entries of one run
-    __shared__ uint32_t red_lo[kWavesPerBlock], red_hi[kWavesPerBlock];
+    __shared__ uint32_t gx[kPSeg / 2];  // first value of each group (group mode)
+    __shared__ uint32_t seg[kPSeg];     // segment bounds of the tile's first values in the current run
+    __shared__ uint32_t red_lo[kWavesPerBlock], red_hi[kWavesPerBlock], wave_groups[kWavesPerBlock];
+    __shared__ uint32_t run_lo[kMaxAntiRights], run_hi[kMaxAntiRights];  // the tile's rows of each run
     const size_t base = static_cast<size_t>(blockIdx.x) * kPTile;
     if (base >= rows) return;
     const uint32_t lane = lane_id(), wave = wave_id();
     const size_t n = rows - base < static_cast<size_t>(kPTile) ? rows - base : kPTile;
     const uint64_t minor_mask = (1ull << minor_bits) - 1;
-    uint32_t x[kPItems], y[kPItems];
+    // g[i]: the item's first value, then k with its segment at seg[k], seg[k + 1]
+    uint32_t g[kPItems], y[kPItems];
     uint32_t valid = 0;  // bit i: item i is a row of the tile
     uint32_t xmin = 0xFFFFFFFFu, xmax = 0;
 #pragma unroll
     for (int i = 0; i < kPItems; ++i) {
         const size_t r = wave * kPWaveRows + i * kWave + lane;
-        x[i] = y[i] = 0;
+        g[i] = y[i] = 0;
         if (r < n) {
             const uint64_t k = keys[base + r];
-            x[i] = static_cast<uint32_t>(k >> minor_bits);
+            g[i] = static_cast<uint32_t>(k >> minor_bits);
             y[i] = static_cast<uint32_t>(k & minor_mask);
             valid |= 1u << i;
-            xmin = x[i] < xmin ? x[i] : xmin;
-            xmax = x[i] > xmax ? x[i] : xmax;
+            xmin = g[i] < xmin ? g[i] : xmin;
+            xmax = g[i] > xmax ? g[i] : xmax;
         }
+    }
+    // group starts: a wave's first row, and rows whose first value differs
+    // from the row before (a first value split over two waves gets two
+    // groups with the same segment, which is harmless)
+    uint32_t started = 0, starts_before = 0;
+#pragma unroll
+    for (int i = 0; i < kPItems; ++i) {
+        const size_t r = wave * kPWaveRows + i * kWave + lane;
+        uint32_t prev = __shfl_up(g[i], 1, kWave);
+        if (i > 0) {
+            const uint32_t last = __shfl(g[i > 0 ? i - 1 : 0], kWave - 1, kWave);
+            if (lane == 0) prev = last;
+        }
+        const bool start = r < n && ((i == 0 && lane == 0) || prev != g[i]);
+        starts_before += static_cast<uint32_t>(__popcll(ballot(start)));
+        started |= (start ? 1u : 0u) << i;
     }
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) {
@@ -478,27 +503,67 @@ __global__ __launch_bounds__(kBlock) void probe_xsec_kernel(const uint64_t* keys
     if (lane == 0) {
         red_lo[wave] = xmin;
         red_hi[wave] = xmax;
+        wave_groups[wave] = starts_before;
     }
     __syncthreads();
-    uint32_t x_lo = 0xFFFFFFFFu, x_hi = 0;
+    uint32_t x_lo = 0xFFFFFFFFu, x_hi = 0, n_groups = 0, wave_off = 0;
     for (int w = 0; w < kWavesPerBlock; ++w) {
         x_lo = red_lo[w] < x_lo ? red_lo[w] : x_lo;
         x_hi = red_hi[w] > x_hi ? red_hi[w] : x_hi;
+        if (w < static_cast<int>(wave)) wave_off += wave_groups[w];
+        n_groups += wave_groups[w];
+    }
+    // the tile's rows of every run, looked up at once (thread q: run q)
+    if (threadIdx.x < rights.count) {
+        run_lo[threadIdx.x] = rights.xstart[threadIdx.x][x_lo];
+        run_hi[threadIdx.x] = rights.xstart[threadIdx.x][x_hi + 1];
+    }
+    __syncthreads();
+    // dense: the range's index entries fit; else one pair per group (block-uniform)
+    const uint32_t span_x = x_hi - x_lo + 2;
+    const bool dense = span_x <= static_cast<uint32_t>(kPSeg);
+    // every run's range must stream, and the bounds must fit (block-uniform)
+    bool stream = dense || n_groups <= static_cast<uint32_t>(kPSeg / 2);
+    for (uint32_t q = 0; q < rights.count; ++q)
+        stream = stream && static_cast<size_t>(run_hi[q] - run_lo[q]) <= kPStreamLimit;
+    if (!stream) {
+        // deferred[0] counts the listed tiles, deferred[1 ..] lists them
+        if (threadIdx.x == 0) deferred[1 + atomicAdd(&deferred[0], 1u)] = blockIdx.x;
+        return;
+    }
+    if (dense) {
+#pragma unroll
+        for (int i = 0; i < kPItems; ++i) g[i] -= x_lo;
+    } else {
+        // group of a row = group starts at or before it, minus one
+        uint32_t before = wave_off;
+#pragma unroll
+        for (int i = 0; i < kPItems; ++i) {
+            const bool start = (started >> i) & 1u;
+            const uint64_t m = ballot(start);
+            const uint32_t grp = before + lanes_below(m) + (start ? 1u : 0u) - 1u;
+            before += static_cast<uint32_t>(__popcll(m));
+            if (!((valid >> i) & 1u)) continue;
+            if (start) gx[grp] = g[i];
+            g[i] = 2 * grp;
+        }
+        __syncthreads();
     }
     uint32_t found = 0;  // bit i: item i is in some run
-    const uint32_t span_x = x_hi - x_lo + 2;  // index entries xstart[x_lo .. x_hi + 1]
-    const bool x_in_lds = span_x <= static_cast<uint32_t>(kPXCap);  // block-uniform
     for (uint32_t q = 0; q < rights.count; ++q) {
         const uint32_t* xs = rights.xstart[q];
         const uint32_t* sec = rights.second[q];
-        if (x_in_lds) {
-            for (uint32_t j = threadIdx.x; j < span_x; j += kBlock) xs_lds[j] = xs[x_lo + j];
-            __syncthreads();
+        if (dense) {
+            for (uint32_t j = threadIdx.x; j < span_x; j += kBlock) seg[j] = xs[x_lo + j];
+        } else {
+            for (uint32_t j = threadIdx.x; j < n_groups; j += kBlock) {
+                seg[2 * j] = xs[gx[j]];
+                seg[2 * j + 1] = xs[gx[j] + 1];
+            }
         }
-        // the tile's rows of run q; item i's segment is [xstart[x_i], xstart[x_i + 1])
-        const size_t r0 = x_in_lds ? xs_lds[0] : xs[x_lo];
-        const size_t r1 = x_in_lds ? xs_lds[span_x - 1] : xs[x_hi + 1];
-        if (x_in_lds && r1 - r0 <= kPStreamLimit) {
+        __syncthreads();
+        const size_t r0 = run_lo[q], r1 = run_hi[q];
+        {
             // chunks start at multiples of 4 rows and are read 16 bytes a
             // lane; the last vector may cover up to 3 rows past r1, inside
             // the buffer's kSecPad rows of padding
@@ -516,7 +581,7 @@ __global__ __launch_bounds__(kBlock) void probe_xsec_kernel(const uint64_t* keys
 #pragma unroll
                 for (int i = 0; i < kPItems; ++i) {
                     if (!(((valid & ~found) >> i) & 1u)) continue;
-                    const uint32_t lo = xs_lds[x[i] - x_lo], hi = xs_lds[x[i] - x_lo + 1];
+                    const uint32_t lo = seg[g[i]], hi = seg[g[i] + 1];
                     if (lo >= c1 || hi <= c0) continue;
                     uint32_t a = static_cast<uint32_t>((lo > c0 ? lo : c0) - c0);
                     const uint32_t e = static_cast<uint32_t>((hi < c1 ? hi : c1) - c0);
@@ -535,31 +600,8 @@ __global__ __launch_bounds__(kBlock) void probe_xsec_kernel(const uint64_t* keys
                 }
                 __syncthreads();
             }
-        } else {
-#pragma unroll
-            for (int i = 0; i < kPItems; ++i) {
-                if (!(((valid & ~found) >> i) & 1u)) continue;
-                // x < 2^24: 32-bit byte offsets from the uniform base. The
-                // empty asm keeps the offsets from being hoisted out of the
-                // run loop, where they would hold registers through the
-                // streaming path too.
-                uint32_t off = x[i] << 2;
-                asm volatile("" : "+v"(off));
-                const char* xs_bytes = reinterpret_cast<const char*>(xs);
-                const uint32_t lo = *reinterpret_cast<const uint32_t*>(xs_bytes + off);
-                const uint32_t hi = *reinterpret_cast<const uint32_t*>(xs_bytes + off + 4u);
-                uint32_t a = lo, b = hi;
-                while (a < b) {
-                    const uint32_t mid = a + ((b - a) >> 1);
-                    if (sec[mid] < y[i])
-                        a = mid + 1;
-                    else
-                        b = mid;
-                }
-                if (a < hi && sec[a] == y[i]) found |= 1u << i;
-            }
         }
-        __syncthreads();  // xs_lds is rewritten for the next run
+        __syncthreads();  // seg is rewritten for the next run
     }
     // bit j of word (base + wave*512 + i*64) / 64 is row base + wave*512 + i*64 + j
 #pragma unroll
@@ -569,6 +611,62 @@ __global__ __launch_bounds__(kBlock) void probe_xsec_kernel(const uint64_t* keys
         if (lane == 0 && row0 < rows) found_words[row0 / kWave] = m;
     }
 }
+
+// The listed tiles of probe_xsec_kernel: one thread per candidate, a
+// lockstep binary search over every run's segment of its first value (one
+// step of all runs in flight at a time). Persistent blocks of kBlock
+// candidates walk the list (kPTile / kBlock blocks per tile).
+__global__ __launch_bounds__(kBlock) void probe_xsec_deferred_kernel(const uint64_t* keys, size_t rows,
+                                                                     uint32_t minor_bits, XSecRights rights,
+                                                                     uint64_t* found_words,
+                                                                     const uint32_t* deferred) {
+    constexpr uint32_t kPer = kPTile / kBlock;
+    const uint64_t minor_mask = (1ull << minor_bits) - 1;
+    const uint32_t listed = deferred[0];
+    for (uint32_t w = blockIdx.x; w < listed * kPer; w += gridDim.x) {
+        const size_t tile = deferred[1 + w / kPer];
+        const size_t i = tile * kPTile + static_cast<size_t>(w % kPer) * kBlock + threadIdx.x;
+        const bool valid = i < rows;
+        const uint64_t k = valid ? keys[i] : 0;
+        const uint32_t x = static_cast<uint32_t>(k >> minor_bits);
+        const uint32_t y = static_cast<uint32_t>(k & minor_mask);
+        uint32_t lo[kMaxAntiRights], len[kMaxAntiRights];
+        uint32_t longest = 0;
+#pragma unroll
+        for (int q = 0; q < kMaxAntiRights; ++q) {
+            lo[q] = len[q] = 0;
+            if (valid && q < static_cast<int>(rights.count)) {
+                lo[q] = rights.xstart[q][x];
+                len[q] = rights.xstart[q][x + 1] - lo[q];
+                longest = len[q] > longest ? len[q] : longest;
+            }
+        }
+        // pos[q] = values of run q's segment below y
+        uint32_t pos[kMaxAntiRights];
+#pragma unroll
+        for (int q = 0; q < kMaxAntiRights; ++q) pos[q] = 0;
+        uint32_t step = 1;
+        while (step * 2 <= longest) step *= 2;
+        for (; longest && step; step >>= 1) {
+            uint32_t probe[kMaxAntiRights];
+#pragma unroll
+            for (int q = 0; q < kMaxAntiRights; ++q)
+                probe[q] = pos[q] + step <= len[q] ? rights.second[q][lo[q] + pos[q] + step - 1] : 0xFFFFFFFFu;
+#pragma unroll
+            for (int q = 0; q < kMaxAntiRights; ++q)
+                if (pos[q] + step <= len[q] && probe[q] < y) pos[q] += step;
+        }
+        bool hit = false;
+#pragma unroll
+        for (int q = 0; q < kMaxAntiRights; ++q)
+            if (pos[q] < len[q] && rights.second[q][lo[q] + pos[q]] == y) hit = true;
+        const uint64_t m = ballot(valid && hit);
+        const size_t row0 = i - lane_id();
+        if (lane_id() == 0 && row0 < rows) found_words[row0 / kWave] = m;
+    }
+}
+
+__global__ void xsec_init_kernel(uint32_t* deferred) { deferred[0] = 0; }
 
 __global__ void second_column_kernel(const uint64_t* keys, size_t rows, uint32_t* second) {
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
@@ -614,16 +712,28 @@ hipError_t probe_sparse(const uint64_t* keys, size_t rows, uint32_t minor_bits, 
     return hipGetLastError();
 }
 
+size_t probe_xsec_temp_bytes(size_t rows) {
+    return ((rows + kPTile - 1) / kPTile + 1) * sizeof(uint32_t);
+}
+
 hipError_t probe_xsec(const uint64_t* keys, size_t rows, uint32_t minor_bits, const XSecRights& rights,
-                      uint32_t n_x, uint64_t* found_words, hipStream_t stream, const KernelTimer* timer) {
+                      uint32_t n_x, uint64_t* found_words, void* temp, size_t temp_bytes, hipStream_t stream,
+                      const KernelTimer* timer) {
     if (rows == 0) return hipSuccess;
-    if (minor_bits == 0 || minor_bits > 32 || rights.count > static_cast<uint32_t>(kMaxAntiRights) || n_x == 0)
+    if (minor_bits == 0 || minor_bits > 32 || rights.count > static_cast<uint32_t>(kMaxAntiRights) || n_x == 0 ||
+        temp_bytes < probe_xsec_temp_bytes(rows))
         return hipErrorInvalidValue;
     const size_t tiles = (rows + kPTile - 1) / kPTile;
+    uint32_t* deferred = static_cast<uint32_t*>(temp);
+    xsec_init_kernel<<<1, 1, 0, stream>>>(deferred);
     timer_start(timer, stream);
     probe_xsec_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(keys, rows, minor_bits, rights,
-                                                                          found_words);
+                                                                          found_words, deferred);
     timer_stop(timer, stream);
+    const size_t per = kPTile / kBlock;
+    const size_t grid = tiles * per < 2048 ? tiles * per : 2048;
+    probe_xsec_deferred_kernel<<<static_cast<unsigned>(grid), kBlock, 0, stream>>>(keys, rows, minor_bits, rights,
+                                                                                   found_words, deferred);
     return hipGetLastError();
 }
 
