@@ -1704,7 +1704,7 @@ void session_step(sparkalog_hip_store* s) {
             };
             add_right(ts.full);
             for (const auto& x : ts.runs) add_right(x);
-            bool column_probe = false;
+            bool column_probe = false, sparse_probe = false;
             if (rights.count) {
                 rt.found.reserve(probe_found_words(M));
                 bool parts_fit = ts.full.rows <= 0xFFFFFFFFull;
@@ -1727,8 +1727,11 @@ void session_step(sparkalog_hip_store* s) {
                     const int pev = s->begin_op();
                     check(probe_sparse(rt.cand_a.ptr, M, rt.shift, xr, n_x, rt.found.ptr, s->stream,
                                        s->kernel_timer(pev)));
-                    s->end_op(pev, kOpProbeSparse, 8.0 * M + 8.0 * right_rows);
+                    // reads the candidates and, per candidate, a few rows of
+                    // each run (not counted): not FULL as a whole
+                    s->end_op(pev, kOpProbeSparse, 8.0 * M);
                     t.anti_slot = pev;
+                    sparse_probe = true;
                 } else if (indexed) {
                     // stream only the second columns of FULL's parts (4 bytes a row)
                     XSecRights xr{};
@@ -1764,7 +1767,7 @@ void session_step(sparkalog_hip_store* s) {
                 }
                 found = rt.found.ptr;
             }
-            algo_bytes += (column_probe ? 4.0 : 8.0) * right_rows;
+            algo_bytes += sparse_probe ? 8.0 * M : (column_probe ? 4.0 : 8.0) * right_rows + 8.0 * M;
             t.newt.reserve(M);
             t.new_run = s->take(M);
             // the new run carries its second column (and, when applied, its
