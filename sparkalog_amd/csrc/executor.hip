@@ -124,6 +124,9 @@ struct DevRelation {
     // second column of a FULL part (u32 per row), valid while keys/rows are
     // those it was written for
     DevBuf<uint32_t> sec;
+    // a run of FULL held only as first-column index + second column (keys
+    // empty): inside a fixpoint, runs of a fused target are kept this way
+    bool csr = false;
     const uint64_t* sec_keys = nullptr;
     uint64_t sec_rows = 0;
     void touched() {
@@ -136,7 +139,7 @@ struct DevRelation {
         xidx_keys = keys.ptr;
         xidx_rows = rows;
     }
-    bool sec_valid() const { return sec_keys != nullptr && sec_keys == keys.ptr && sec_rows == rows; }
+    bool sec_valid() const { return csr || (sec_keys != nullptr && sec_keys == keys.ptr && sec_rows == rows); }
     void sec_set() {
         sec_keys = keys.ptr;
         sec_rows = rows;
@@ -556,6 +559,7 @@ struct TargetRuntime {
     DevBuf<uint64_t> full_union;
     DevBuf<uint64_t> new_run;      // NEWT's second copy, becomes a run of FULL
     DevBuf<uint32_t> new_sec;      // that run's second column (when FULL is probed by it)
+    bool csr_run = false;          // this iteration's new run is kept as index + second column only
     uint64_t contrib_bound = 0;
     int anti_slot = -1;            // profiling event of this iteration's anti-join
     bool fused = false;            // this iteration's distinct wrote NEWT (candidates in FULL dropped first)
@@ -835,13 +839,14 @@ void download(sparkalog_hip_store* s, const DevRelation& src, uint32_t* c0, uint
 bool identity(const uint32_t c[2]) { return c[0] == 0 && c[1] == 1; }
 
 double settle_runs(sparkalog_hip_store* s, RelationState& ts, bool all);
+void settle_all(sparkalog_hip_store* s, RelationState& st);
 
 void build_index(sparkalog_hip_store* s, RuleRuntime& rt, RelationState& right, uint32_t relation) {
     const uint32_t col = rt.plan.right_key;
     CachedIndex& ix = rt.index;
     if (ix.valid && ix.relation == relation && ix.version == right.version && ix.column == col) return;
     // a recursive right input is indexed as one array
-    if (!right.runs.empty()) settle_runs(s, right, true);
+    settle_all(s, right);
     const int ev = s->begin_op();
     const uint64_t n = right.full.rows;
     if (right.max_value >= (1u << 28)) throw StatusError{SPARKALOG_HIP_ERR_UNSUPPORTED_PLAN};
@@ -903,6 +908,18 @@ hipError_t copy_keys(const uint64_t* in, const uint64_t* rows_dev, size_t bound,
 // Build a FULL part's first-column index unless it is current.
 void ensure_xidx(sparkalog_hip_store* s, DevRelation& r, uint32_t n_x) {
     if (r.xidx_valid(n_x)) return;
+    if (r.csr) {
+        // a part held as index + second column has no keys to build from:
+        // its index grows with n_x (first values only grow within a store)
+        const uint32_t old_n = r.xidx_n;
+        if (n_x < old_n || r.rows > 0xFFFFFFFFull) throw StatusError{SPARKALOG_HIP_ERR_INVALID_ARGUMENT};
+        DevBuf<uint32_t> grown = s->take_x(static_cast<size_t>(n_x) + 1);
+        check(xindex_extend(r.xidx.ptr, old_n, grown.ptr, n_x, static_cast<uint32_t>(r.rows), s->stream));
+        s->give_x(std::move(r.xidx));
+        r.xidx = std::move(grown);
+        r.xidx_set(n_x);
+        return;
+    }
     if (r.xidx.cap < static_cast<size_t>(n_x) + 1) {
         s->give_x(std::move(r.xidx));
         r.xidx = s->take_x(static_cast<size_t>(n_x) + 1);
@@ -922,6 +939,46 @@ void ensure_sec(sparkalog_hip_store* s, DevRelation& r) {
     r.sec_set();
 }
 
+// First-column index size for the store's current values: every stored row
+// has both columns below it.
+uint32_t store_n_x(const sparkalog_hip_store* s) {
+    uint32_t vmax = 0;
+    for (const auto& st : s->rel)
+        if (st.present) vmax = st.max_value > vmax ? st.max_value : vmax;
+    return vmax + 1;
+}
+
+// Give a FULL part held as index + second column its packed keys again.
+void ensure_packed(sparkalog_hip_store* s, DevRelation& r) {
+    if (!r.csr) return;
+    const uint32_t nx = r.xidx_n;
+    DevBuf<uint64_t> keys = s->take(r.rows ? r.rows : 1);
+    if (r.rows) {
+        const size_t tb = run_merge_temp_bytes(r.rows);
+        void* temp = s->scratch(tb);
+        check(run_merge(r.xidx.ptr, r.sec.ptr, r.rows, nullptr, nullptr, 0, nx, nullptr, keys.ptr, temp, tb,
+                        s->stream));
+    }
+    s->give(std::move(r.keys));
+    r.keys = std::move(keys);
+    r.csr = false;
+    r.touched();
+    r.xidx_set(nx);
+    r.sec_set();
+}
+
+void packed_parts(sparkalog_hip_store* s, RelationState& st) {
+    ensure_packed(s, st.full);
+    for (auto& r : st.runs) ensure_packed(s, r);
+}
+
+// Merge FULL's newest runs while the older run is smaller than merge_ratio
+// times the newer (or always, when `all`, which leaves FULL as one packed
+// array). Inputs are disjoint sorted unique arrays, so a merge's output size
+// is the sum of its inputs' sizes. When either part is held as index +
+// second column, the merge reads and writes only second columns (8 B a row
+// instead of 20); the last merge of `all` also writes packed keys. Returns
+// the algorithmic bytes moved.
 double settle_runs(sparkalog_hip_store* s, RelationState& ts, bool all) {
     double bytes = 0.0;
     while (!ts.runs.empty()) {
@@ -933,48 +990,98 @@ double settle_runs(sparkalog_hip_store* s, RelationState& ts, bool all) {
         const uint64_t n = prev.rows + last.rows;
         if (prev.rows == 0) {
             const uint32_t last_nx = last.xidx_n && last.xidx_valid(last.xidx_n) ? last.xidx_n : 0;
-            const bool last_sec = last.sec_valid();
+            const bool last_sec = last.sec_valid(), last_csr = last.csr;
             std::swap(prev.keys, last.keys);
             std::swap(prev.xidx, last.xidx);
             std::swap(prev.sec, last.sec);
             prev.rows = last.rows;
             prev.touched();
+            prev.csr = last_csr;
+            last.csr = false;
             if (last_nx) prev.xidx_set(last_nx);
-            if (last_sec) prev.sec_set();
+            if (last_sec && !last_csr) prev.sec_set();
         } else if (last.rows) {
-            // the merged run's index is the sum of the two indexes
-            const uint32_t nx = prev.xidx_n;
-            const bool add = nx && prev.xidx_valid(nx) && last.xidx_valid(nx);
-            // the merged run keeps a second column while its parts had one
-            const bool sec = prev.sec_valid() && last.sec_valid();
-            DevBuf<uint64_t> out = s->take(n);
-            DevBuf<uint32_t> out_sec;
-            if (sec) out_sec = s->take_sec(n + kSecPad);
-            const size_t tb = merge_disjoint_temp_bytes(prev.rows, last.rows);
-            void* temp = s->scratch(tb);
-            const int ev = s->begin_op();
-            check(merge_disjoint(prev.keys.ptr, prev.rows, last.keys.ptr, last.rows, out.ptr, temp, tb, s->stream,
-                                 s->kernel_timer(ev), sec ? out_sec.ptr : nullptr));
-            s->end_op(ev, kOpMergeRuns, (sec ? 20.0 : 16.0) * n);
-            if (add) check(xindex_add(prev.xidx.ptr, last.xidx.ptr, nx, s->stream));
-            s->give(std::move(prev.keys));
-            prev.keys = std::move(out);
-            prev.rows = n;
-            prev.touched();
-            if (add) prev.xidx_set(nx);
-            if (sec) {
+            // an index covering every part: the store's current first values
+            // (never below the index of a part held as index + second column)
+            uint32_t nx = store_n_x(s);
+            if (prev.csr && prev.xidx_n > nx) nx = prev.xidx_n;
+            if (last.csr && last.xidx_n > nx) nx = last.xidx_n;
+            // index entries are u32 row counts: a merged part must stay below 2^32 rows
+            const bool cols = (prev.csr || last.csr) && nx && n <= 0xFFFFFFFFull;
+            if (cols) {
+                ensure_xidx(s, prev, nx);
+                ensure_sec(s, prev);
+                ensure_xidx(s, last, nx);
+                ensure_sec(s, last);
+                const bool packed_out = all && ts.runs.size() == 1;
+                DevBuf<uint32_t> out_sec = s->take_sec(n + kSecPad);
+                DevBuf<uint64_t> out_keys;
+                if (packed_out) out_keys = s->take(n);
+                const size_t tb = run_merge_temp_bytes(n);
+                void* temp = s->scratch(tb);
+                const int ev = s->begin_op();
+                check(run_merge(prev.xidx.ptr, prev.sec.ptr, prev.rows, last.xidx.ptr, last.sec.ptr, last.rows, nx,
+                                out_sec.ptr, packed_out ? out_keys.ptr : nullptr, temp, tb, s->stream,
+                                s->kernel_timer(ev)));
+                s->stats[kOpMergeRuns].kernel = "run_merge_kernel";
+                const double mb = (packed_out ? 16.0 : 8.0) * static_cast<double>(n);
+                s->end_op(ev, kOpMergeRuns, mb);
+                check(xindex_add(prev.xidx.ptr, last.xidx.ptr, nx, s->stream));
+                s->give(std::move(prev.keys));
+                prev.keys = std::move(out_keys);
                 s->give_sec(std::move(prev.sec));
                 prev.sec = std::move(out_sec);
-                prev.sec_set();
+                prev.rows = n;
+                prev.csr = !packed_out;
+                prev.touched();
+                prev.xidx_set(nx);
+                if (packed_out) prev.sec_set();
+                bytes += mb;
+            } else {
+                ensure_packed(s, prev);
+                ensure_packed(s, last);
+                // the merged run's index is the sum of the two indexes
+                const uint32_t px = prev.xidx_n;
+                const bool add = px && prev.xidx_valid(px) && last.xidx_valid(px) && n <= 0xFFFFFFFFull;
+                // the merged run keeps a second column while its parts had one
+                const bool sec = prev.sec_valid() && last.sec_valid();
+                DevBuf<uint64_t> out = s->take(n);
+                DevBuf<uint32_t> out_sec;
+                if (sec) out_sec = s->take_sec(n + kSecPad);
+                const size_t tb = merge_disjoint_temp_bytes(prev.rows, last.rows);
+                void* temp = s->scratch(tb);
+                const int ev = s->begin_op();
+                check(merge_disjoint(prev.keys.ptr, prev.rows, last.keys.ptr, last.rows, out.ptr, temp, tb,
+                                     s->stream, s->kernel_timer(ev), sec ? out_sec.ptr : nullptr));
+                s->stats[kOpMergeRuns].kernel = "merge_disjoint_kernel";
+                s->end_op(ev, kOpMergeRuns, (sec ? 20.0 : 16.0) * n);
+                if (add) check(xindex_add(prev.xidx.ptr, last.xidx.ptr, px, s->stream));
+                s->give(std::move(prev.keys));
+                prev.keys = std::move(out);
+                prev.rows = n;
+                prev.touched();
+                if (add) prev.xidx_set(px);
+                if (sec) {
+                    s->give_sec(std::move(prev.sec));
+                    prev.sec = std::move(out_sec);
+                    prev.sec_set();
+                }
+                bytes += (sec ? 20.0 : 16.0) * n;
             }
-            bytes += (sec ? 20.0 : 16.0) * n;
         }
         s->give(std::move(last.keys));
         s->give_x(std::move(last.xidx));
         s->give_sec(std::move(last.sec));
+        last.csr = false;
         ts.runs.pop_back();
     }
+    if (all) ensure_packed(s, ts.full);
     return bytes;
+}
+
+// FULL as one packed array (every run merged in).
+void settle_all(sparkalog_hip_store* s, RelationState& st) {
+    if (!st.runs.empty() || st.full.csr) settle_runs(s, st, true);
 }
 
 
@@ -1322,8 +1429,8 @@ extern "C" int sparkalog_hip_store_difference(sparkalog_hip_store* s, uint32_t t
     RelationState* ls = find(s, left);
     RelationState* rs = find(s, right);
     // both inputs as one sorted unique array each
-    if (!ls->runs.empty()) settle_runs(s, *ls, true);
-    if (!rs->runs.empty()) settle_runs(s, *rs, true);
+    settle_all(s, *ls);
+    settle_all(s, *rs);
     const uint64_t nl = ls->full.rows, nr = rs->full.rows;
     const uint32_t left_max = ls->max_value;
     ts->full.keys.reserve(nl ? nl : 1);
@@ -1382,6 +1489,7 @@ extern "C" int sparkalog_hip_store_clear(sparkalog_hip_store* s) {
         st.present = false;
         st.recursive = false;
         st.full.rows = st.delta.rows = st.newt.rows = 0;
+        st.full.csr = false;
         st.full.touched();
         for (auto& r : st.runs) {
             s->give(std::move(r.keys));
@@ -1411,7 +1519,7 @@ extern "C" int sparkalog_hip_store_read(sparkalog_hip_store* s, uint32_t id, int
     if (!st) return SPARKALOG_HIP_ERR_MISSING_RELATION;
     if (which == 1 && !st->recursive) return SPARKALOG_HIP_ERR_NOT_RECURSIVE;
     SPK_GUARD_BEGIN
-    if (which == 0 && !st->runs.empty()) settle_runs(s, *st, true);
+    if (which == 0) settle_all(s, *st);
     const DevRelation& r = which == 0 ? st->full : st->delta;
     if (r.rows > capacity || (r.rows && (!c0 || !c1))) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
     download(s, r, c0, c1, to_device);
@@ -1425,7 +1533,7 @@ extern "C" int sparkalog_hip_store_digest(sparkalog_hip_store* s, uint32_t id, u
     RelationState* st = find(s, id);
     if (!st) return SPARKALOG_HIP_ERR_MISSING_RELATION;
     SPK_GUARD_BEGIN
-    if (!st->runs.empty()) settle_runs(s, *st, true);
+    settle_all(s, *st);
     s->misc_dev.reserve(64);
     clear_words_kernel<<<1, 64, 0, s->stream>>>(s->misc_dev.ptr, 4);
     const uint64_t n = st->full.rows;
@@ -1606,6 +1714,7 @@ void session_step(sparkalog_hip_store* s) {
     clear_words_kernel<<<1, 256, 0, s->stream>>>(s->misc_dev.ptr, 256);
     for (auto& t : targets) {
         t.fused = false;
+        t.csr_run = false;
         t.anti_slot = -1;
     }
     // ---- RuleRuntime::evaluate, join count (generic.rs:197-241) ----
@@ -1713,20 +1822,22 @@ void session_step(sparkalog_hip_store* s) {
                 // while first values stay below 2^24
                 const bool indexed = vmax < (1u << 24) && parts_fit;
                 if (M <= s->sparse_probe_rows && indexed) {
-                    XRights xr{};
+                    XSecRights xr{};
                     xr.count = 0;
                     auto add_part = [&](DevRelation& x) {
                         if (!x.rows) return;
                         ensure_xidx(s, x, n_x);
-                        xr.keys[xr.count] = x.keys.ptr;
+                        ensure_sec(s, x);
+                        xr.second[xr.count] = x.sec.ptr;
                         xr.xstart[xr.count] = x.xidx.ptr;
                         ++xr.count;
                     };
                     add_part(ts.full);
                     for (auto& x : ts.runs) add_part(x);
                     const int pev = s->begin_op();
-                    check(probe_sparse(rt.cand_a.ptr, M, rt.shift, xr, n_x, rt.found.ptr, s->stream,
-                                       s->kernel_timer(pev)));
+                    check(probe_sparse_sec(rt.cand_a.ptr, M, rt.shift, xr, n_x, rt.found.ptr, s->stream,
+                                           s->kernel_timer(pev)));
+                    s->stats[kOpProbeSparse].kernel = "probe_sparse_sec_kernel";
                     // reads the candidates and, per candidate, a few rows of
                     // each run (not counted): not FULL as a whole
                     s->end_op(pev, kOpProbeSparse, 8.0 * M);
@@ -1756,6 +1867,13 @@ void session_step(sparkalog_hip_store* s) {
                     t.anti_slot = pev;
                     column_probe = true;
                 } else {
+                    // packed keys of every part (a part held as index +
+                    // second column is expanded first)
+                    packed_parts(s, ts);
+                    rights.count = 0;
+                    right_rows = 0;
+                    add_right(ts.full);
+                    for (const auto& x : ts.runs) add_right(x);
                     const size_t ptb = probe_grouped_temp_bytes(M, rights.count);
                     void* ptemp = s->scratch(ptb);
                     const int pev = s->begin_op();
@@ -1769,10 +1887,14 @@ void session_step(sparkalog_hip_store* s) {
             }
             algo_bytes += sparse_probe ? 8.0 * M : (column_probe ? 4.0 : 8.0) * right_rows + 8.0 * M;
             t.newt.reserve(M);
-            t.new_run = s->take(M);
-            // the new run carries its second column (and, when applied, its
-            // first-column index) while FULL can be probed by columns
-            if (vmax < (1u << 24)) t.new_sec = s->take_sec(M + kSecPad);
+            // while FULL can be probed by columns the new run is kept as its
+            // first-column index (built from NEWT, the same rows) and second
+            // column only; else as packed keys (NEWT's second copy)
+            t.csr_run = vmax < (1u << 24);
+            if (t.csr_run)
+                t.new_sec = s->take_sec(M + kSecPad);
+            else
+                t.new_run = s->take(M);
             t.fused = true;
             t.contrib_bound = M;
         }
@@ -1797,7 +1919,7 @@ void session_step(sparkalog_hip_store* s) {
             uint64_t* count = &cdev->distinct_rows[r];
             if (fuse >= 0) {
                 TargetRuntime& t = targets[static_cast<size_t>(fuse)];
-                out = PairOutput{t.newt.ptr, nullptr, nullptr, t.new_run.ptr, t.new_sec.ptr};
+                out = PairOutput{t.newt.ptr, nullptr, nullptr, t.csr_run ? nullptr : t.new_run.ptr, t.new_sec.ptr};
                 count = &cdev->newt_rows[fuse];
             }
             check(segmented_distinct_finish(rt.cand_a.ptr, M, rt.shift, known_compressed, rt.cand_b.ptr,
@@ -1859,6 +1981,7 @@ void session_step(sparkalog_hip_store* s) {
         t.newt.reserve(bound ? bound : 1);
         t.new_run = s->take(bound ? bound : 1);
         if (ts.runs.size() >= kMaxRuns) settle_runs(s, ts, true);
+        packed_parts(s, ts);  // this anti-join reads FULL's packed keys
         AntiRight rights{};
         rights.count = 0;
         double right_rows = 0;
@@ -1973,17 +2096,28 @@ void session_step(sparkalog_hip_store* s) {
         ts.newt.rows = N;
         if (N) {
             DevRelation run;
-            run.keys = std::move(t.new_run);
             run.rows = N;
-            const bool with_sec = t.new_sec.ptr != nullptr;
-            if (with_sec) {
+            if (t.csr_run) {
+                // first-column index from NEWT (the same rows), second column
+                // from the distinct
+                run.csr = true;
                 run.sec = std::move(t.new_sec);
-                run.sec_set();
+                run.xidx = s->take_x(static_cast<size_t>(n_x) + 1);
+                check(xindex_build(ts.newt.keys.ptr, nullptr, N, n_x, run.xidx.ptr, s->stream));
+                run.xidx_set(n_x);
+            } else {
+                run.keys = std::move(t.new_run);
+                const bool with_sec = t.new_sec.ptr != nullptr;
+                if (with_sec) {
+                    run.sec = std::move(t.new_sec);
+                    run.sec_set();
+                }
+                // indexed while the anti-join uses the index (probing
+                // sparsely or by columns), so merged runs get their index by
+                // addition
+                if ((use_x || with_sec || (vmax < (1u << 24) && N <= s->sparse_probe_rows)) && N <= 0xFFFFFFFFull)
+                    ensure_xidx(s, run, n_x);
             }
-            // indexed while the anti-join uses the index (probing sparsely or
-            // by columns), so merged runs get their index by addition
-            if ((use_x || with_sec || (vmax < (1u << 24) && N <= s->sparse_probe_rows)) && N <= 0xFFFFFFFFull)
-                ensure_xidx(s, run, n_x);
             ts.runs.push_back(std::move(run));
         } else {
             s->give(std::move(t.new_run));
@@ -2082,7 +2216,7 @@ extern "C" int sparkalog_hip_store_instance_stats(sparkalog_hip_store* s, uint32
     RelationState* st = find(s, id);
     if (!st) return SPARKALOG_HIP_ERR_MISSING_RELATION;
     SPK_GUARD_BEGIN
-    if (!st->runs.empty()) settle_runs(s, *st, true);
+    settle_all(s, *st);
     unsigned long long* acc =
         reinterpret_cast<unsigned long long*>(s->probes(2 * static_cast<size_t>(instances) + 2));
     check(fill_u64(reinterpret_cast<uint64_t*>(acc), 2 * static_cast<size_t>(instances), 0, s->stream));
@@ -2111,6 +2245,7 @@ extern "C" int sparkalog_hip_store_range_stats(sparkalog_hip_store* s, uint32_t 
     if (which == 1) {
         parts.push_back(&st->delta);
     } else {
+        packed_parts(s, *st);
         parts.push_back(&st->full);
         for (const auto& r : st->runs) parts.push_back(&r);
     }
@@ -2150,6 +2285,7 @@ extern "C" int sparkalog_hip_store_merge_rows(sparkalog_hip_store* s, uint32_t i
         st->runs.push_back(std::move(incoming));
         st->version += 1;
     } else {
+        ensure_packed(s, st->full);
         union_into(s, st->full, incoming.keys.ptr, incoming.rows);
         st->version += 1;
     }
@@ -2203,6 +2339,7 @@ extern "C" int sparkalog_hip_store_remove_ranges(sparkalog_hip_store* s, uint32_
         r.rows = n;
         r.touched();
     };
+    packed_parts(s, *st);
     strip(st->full);
     for (auto& r : st->runs) strip(r);
     if (st->recursive) strip(st->delta);
@@ -2221,6 +2358,7 @@ extern "C" int sparkalog_hip_store_remove_range(sparkalog_hip_store* s, uint32_t
         const auto range = first_range(s, r, first_lo, first_hi);
         remove_rows(s, r, range.first, range.second);
     };
+    packed_parts(s, *st);
     strip(st->full);
     for (auto& r : st->runs) strip(r);
     if (st->recursive) strip(st->delta);
@@ -2242,6 +2380,7 @@ extern "C" int sparkalog_hip_store_range_rows(sparkalog_hip_store* s, uint32_t i
     if (which == 1) {
         parts.push_back(&st->delta);
     } else {
+        packed_parts(s, *st);
         parts.push_back(&st->full);
         for (const auto& r : st->runs) parts.push_back(&r);
     }

@@ -140,6 +140,10 @@ hipError_t probe_grouped(const uint64_t* keys, const uint64_t* rows_dev, size_t 
 // column < x. The row count is *rows_dev (<= bound) when rows_dev is non-null.
 hipError_t xindex_build(const uint64_t* keys, const uint64_t* rows_dev, size_t bound, uint32_t n_x, uint32_t* xstart,
                         hipStream_t stream);
+// to = from with n_x + 1 entries: a run's index for a larger n_x (its rows
+// all have first values below old_n, so the new entries equal `rows`).
+hipError_t xindex_extend(const uint32_t* from, uint32_t old_n, uint32_t* to, uint32_t n_x, uint32_t rows,
+                         hipStream_t stream);
 // into[x] += other[x]: the index of the merge of two disjoint runs.
 hipError_t xindex_add(uint32_t* into, const uint32_t* other, uint32_t n_x, hipStream_t stream);
 
@@ -172,6 +176,18 @@ size_t probe_xsec_temp_bytes(size_t rows);
 hipError_t probe_xsec(const uint64_t* keys, size_t rows, uint32_t minor_bits, const XSecRights& rights,
                       uint32_t n_x, uint64_t* found_words, void* temp, size_t temp_bytes, hipStream_t stream,
                       const KernelTimer* timer = nullptr);
+// probe_sparse over second columns (XSecRights), one thread per candidate.
+hipError_t probe_sparse_sec(const uint64_t* keys, size_t rows, uint32_t minor_bits, const XSecRights& rights,
+                            uint32_t n_x, uint64_t* found_words, hipStream_t stream,
+                            const KernelTimer* timer = nullptr);
+// Merge of two disjoint runs given by first-column index (n_x + 1 entries)
+// and second column: writes the merged second column and/or the merged
+// packed keys (either may be null, not both). The merged index is
+// xa + xb (xindex_add). With nb == 0 it expands run A to packed keys.
+size_t run_merge_temp_bytes(size_t rows);
+hipError_t run_merge(const uint32_t* xa, const uint32_t* sa, size_t na, const uint32_t* xb, const uint32_t* sb,
+                     size_t nb, uint32_t n_x, uint32_t* out_second, uint64_t* out_packed, void* temp,
+                     size_t temp_bytes, hipStream_t stream, const KernelTimer* timer = nullptr);
 // second[i] = low 32 bits of keys[i] (the second column of packed rows).
 hipError_t second_column(const uint64_t* keys, size_t rows, uint32_t* second, hipStream_t stream);
 // left \ (rights' union) for sorted unique packed left rows (count *left_rows

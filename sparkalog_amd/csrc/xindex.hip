@@ -612,10 +612,45 @@ __global__ __launch_bounds__(kBlock) void probe_xsec_kernel(const uint64_t* keys
     }
 }
 
-// The listed tiles of probe_xsec_kernel: one thread per candidate, a
-// lockstep binary search over every run's segment of its first value (one
-// step of all runs in flight at a time). Persistent blocks of kBlock
-// candidates walk the list (kPTile / kBlock blocks per tile).
+// Whether (x, y) is in some run: a lockstep binary search over every run's
+// segment of first value x (one step of all runs in flight at a time).
+__device__ __forceinline__ bool in_runs(uint32_t x, uint32_t y, const XSecRights& rights) {
+    uint32_t lo[kMaxAntiRights], len[kMaxAntiRights];
+    uint32_t longest = 0;
+#pragma unroll
+    for (int q = 0; q < kMaxAntiRights; ++q) {
+        lo[q] = len[q] = 0;
+        if (q < static_cast<int>(rights.count)) {
+            lo[q] = rights.xstart[q][x];
+            len[q] = rights.xstart[q][x + 1] - lo[q];
+            longest = len[q] > longest ? len[q] : longest;
+        }
+    }
+    // pos[q] = values of run q's segment below y
+    uint32_t pos[kMaxAntiRights];
+#pragma unroll
+    for (int q = 0; q < kMaxAntiRights; ++q) pos[q] = 0;
+    uint32_t step = 1;
+    while (step * 2 <= longest) step *= 2;
+    for (; longest && step; step >>= 1) {
+        uint32_t probe[kMaxAntiRights];
+#pragma unroll
+        for (int q = 0; q < kMaxAntiRights; ++q)
+            probe[q] = pos[q] + step <= len[q] ? rights.second[q][lo[q] + pos[q] + step - 1] : 0xFFFFFFFFu;
+#pragma unroll
+        for (int q = 0; q < kMaxAntiRights; ++q)
+            if (pos[q] + step <= len[q] && probe[q] < y) pos[q] += step;
+    }
+    bool hit = false;
+#pragma unroll
+    for (int q = 0; q < kMaxAntiRights; ++q)
+        if (pos[q] < len[q] && rights.second[q][lo[q] + pos[q]] == y) hit = true;
+    return hit;
+}
+
+// The listed tiles of probe_xsec_kernel, one thread per candidate.
+// Persistent blocks of kBlock candidates walk the list (kPTile / kBlock
+// blocks per tile).
 __global__ __launch_bounds__(kBlock) void probe_xsec_deferred_kernel(const uint64_t* keys, size_t rows,
                                                                      uint32_t minor_bits, XSecRights rights,
                                                                      uint64_t* found_words,
@@ -628,45 +663,172 @@ __global__ __launch_bounds__(kBlock) void probe_xsec_deferred_kernel(const uint6
         const size_t i = tile * kPTile + static_cast<size_t>(w % kPer) * kBlock + threadIdx.x;
         const bool valid = i < rows;
         const uint64_t k = valid ? keys[i] : 0;
-        const uint32_t x = static_cast<uint32_t>(k >> minor_bits);
-        const uint32_t y = static_cast<uint32_t>(k & minor_mask);
-        uint32_t lo[kMaxAntiRights], len[kMaxAntiRights];
-        uint32_t longest = 0;
-#pragma unroll
-        for (int q = 0; q < kMaxAntiRights; ++q) {
-            lo[q] = len[q] = 0;
-            if (valid && q < static_cast<int>(rights.count)) {
-                lo[q] = rights.xstart[q][x];
-                len[q] = rights.xstart[q][x + 1] - lo[q];
-                longest = len[q] > longest ? len[q] : longest;
-            }
-        }
-        // pos[q] = values of run q's segment below y
-        uint32_t pos[kMaxAntiRights];
-#pragma unroll
-        for (int q = 0; q < kMaxAntiRights; ++q) pos[q] = 0;
-        uint32_t step = 1;
-        while (step * 2 <= longest) step *= 2;
-        for (; longest && step; step >>= 1) {
-            uint32_t probe[kMaxAntiRights];
-#pragma unroll
-            for (int q = 0; q < kMaxAntiRights; ++q)
-                probe[q] = pos[q] + step <= len[q] ? rights.second[q][lo[q] + pos[q] + step - 1] : 0xFFFFFFFFu;
-#pragma unroll
-            for (int q = 0; q < kMaxAntiRights; ++q)
-                if (pos[q] + step <= len[q] && probe[q] < y) pos[q] += step;
-        }
-        bool hit = false;
-#pragma unroll
-        for (int q = 0; q < kMaxAntiRights; ++q)
-            if (pos[q] < len[q] && rights.second[q][lo[q] + pos[q]] == y) hit = true;
-        const uint64_t m = ballot(valid && hit);
+        const bool hit = valid && in_runs(static_cast<uint32_t>(k >> minor_bits),
+                                          static_cast<uint32_t>(k & minor_mask), rights);
+        const uint64_t m = ballot(hit);
         const size_t row0 = i - lane_id();
         if (lane_id() == 0 && row0 < rows) found_words[row0 / kWave] = m;
     }
 }
 
+// probe_xsec for few candidates: one thread per candidate, every run
+// searched in place.
+__global__ __launch_bounds__(kBlock) void probe_sparse_sec_kernel(const uint64_t* keys, size_t rows,
+                                                                  uint32_t minor_bits, XSecRights rights,
+                                                                  uint64_t* found_words) {
+    const uint64_t minor_mask = (1ull << minor_bits) - 1;
+    const size_t i = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const bool valid = i < rows;
+    const uint64_t k = valid ? keys[i] : 0;
+    const bool hit = valid && in_runs(static_cast<uint32_t>(k >> minor_bits),
+                                      static_cast<uint32_t>(k & minor_mask), rights);
+    const uint64_t m = ballot(hit);
+    const size_t row0 = i - lane_id();
+    if (lane_id() == 0 && row0 < rows) found_words[row0 / kWave] = m;
+}
+
+// ---- merges of runs held as (first-column index, second column) ----
+// C = A ∪ B for disjoint runs: C's index is xa + xb (xindex_add), and C's
+// second column lists the merged rows in (first, second) order. Output tile
+// t covers merged rows [t*kRTile, (t+1)*kRTile). Its first value bx[t] is
+// the x whose merged segment [xa[x] + xb[x], xa[x+1] + xb[x+1]) holds row
+// t*kRTile (written by the thread of that x), and its split pa[t] between A
+// and B is a merge path inside the two segments of bx[t]. The tile rebuilds
+// the keys (x << 32 | second) of its A and B rows in LDS, x from the index
+// entries of its first-value range, and merges them as merge_disjoint does.
+// With B empty it expands a run to packed keys.
+constexpr int kRTile = 2048;                 // merged rows per tile
+constexpr int kRItems = kRTile / kBlock;     // 8 per thread
+constexpr uint32_t kRSpan = 256;             // index entries of a tile's first-value range kept in LDS (18 KB: 8 blocks a CU)
+
+__global__ void run_bound_x_kernel(const uint32_t* xa, const uint32_t* xb, uint32_t n_x, uint32_t* bx) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t x = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; x < n_x; x += stride) {
+        const uint64_t c0 = static_cast<uint64_t>(xa[x]) + (xb ? xb[x] : 0u);
+        const uint64_t c1 = static_cast<uint64_t>(xa[x + 1]) + (xb ? xb[x + 1] : 0u);
+        for (uint64_t t = (c0 + kRTile - 1) / kRTile; t * kRTile < c1; ++t) bx[t] = static_cast<uint32_t>(x);
+    }
+}
+
+__global__ void run_partition_kernel(const uint32_t* xa, const uint32_t* sa, size_t na, const uint32_t* xb,
+                                     const uint32_t* sb, const uint32_t* bx, size_t tiles, size_t* pa) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; t <= tiles; t += stride) {
+        if (t == tiles) {
+            pa[t] = na;
+            continue;
+        }
+        const uint32_t x = bx[t];
+        const uint64_t a0 = xa[x], b0 = xb ? xb[x] : 0u;
+        const uint64_t la = xa[x + 1] - a0, lb = xb ? xb[x + 1] - b0 : 0u;
+        const uint64_t j = static_cast<uint64_t>(t) * kRTile - a0 - b0;  // merged rows of x before the boundary
+        uint64_t lo = j > lb ? j - lb : 0, hi = j < la ? j : la;
+        while (lo < hi) {  // the rows of A among the first j of x's merged rows (no value in common)
+            const uint64_t mid = (lo + hi) >> 1;
+            if (!(sb[b0 + j - 1 - mid] < sa[a0 + mid]))
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        pa[t] = a0 + lo;
+    }
+}
+
+__device__ __forceinline__ uint32_t merge_path_keys(const uint64_t* a, uint32_t na, const uint64_t* b, uint32_t nb,
+                                                    uint32_t diag) {
+    uint32_t lo = diag > nb ? diag - nb : 0;
+    uint32_t hi = diag < na ? diag : na;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (!(b[diag - 1 - mid] < a[mid]))
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void run_merge_kernel(const uint32_t* xa, const uint32_t* sa, size_t na,
+                                                           const uint32_t* xb, const uint32_t* sb, size_t nb,
+                                                           uint32_t n_x, const uint32_t* bx, const size_t* pa,
+                                                           size_t tiles, uint32_t* out_second, uint64_t* out_packed) {
+    __shared__ uint64_t lds[kRTile];
+    __shared__ uint32_t xa_l[kRSpan], xb_l[kRSpan];
+    const size_t t = blockIdx.x;
+    const size_t total = na + nb;
+    const size_t d0 = t * kRTile;
+    if (d0 >= total) return;
+    const size_t d1 = d0 + kRTile < total ? d0 + kRTile : total;
+    const size_t a0 = pa[t], a1 = pa[t + 1];
+    const size_t b0 = d0 - a0, b1 = d1 - a1;
+    const uint32_t n_a = static_cast<uint32_t>(a1 - a0), n_b = static_cast<uint32_t>(b1 - b0);
+    // every row of the tile has its first value in [x_lo, x_hi]
+    const uint32_t x_lo = bx[t];
+    const uint32_t x_hi = t + 1 < tiles ? bx[t + 1] : n_x - 1;
+    const uint32_t span = x_hi - x_lo + 2;  // entries x_lo .. x_hi + 1
+    const bool staged = span <= kRSpan;     // block-uniform
+    if (staged) {
+        for (uint32_t j = threadIdx.x; j < span; j += kBlock) {
+            xa_l[j] = xa[x_lo + j];
+            xb_l[j] = xb ? xb[x_lo + j] : 0u;
+        }
+    }
+    __syncthreads();
+    // first value of a row: the largest x in [x_lo, x_hi] whose segment
+    // starts at or before it (a tile's range is a few first values in the
+    // large merges; rows are strided so that LDS accesses stay conflict-free)
+    auto first_of = [&](const uint32_t* xs, const uint32_t* xs_l, size_t row) -> uint32_t {
+        uint32_t lo = 0, hi = x_hi - x_lo;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if ((staged ? xs_l[mid] : xs[x_lo + mid]) <= row)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        return x_lo + lo;
+    };
+    for (uint32_t i = threadIdx.x; i < n_a; i += kBlock)
+        lds[i] = (static_cast<uint64_t>(first_of(xa, xa_l, a0 + i)) << 32) | sa[a0 + i];
+    for (uint32_t i = threadIdx.x; i < n_b; i += kBlock)
+        lds[n_a + i] = (static_cast<uint64_t>(first_of(xb, xb_l, b0 + i)) << 32) | sb[b0 + i];
+    __syncthreads();
+    const uint32_t n = n_a + n_b;
+    const uint32_t diag = threadIdx.x * kRItems < n ? threadIdx.x * kRItems : n;
+    uint32_t ia = merge_path_keys(lds, n_a, lds + n_a, n_b, diag);
+    uint32_t ib = diag - ia;
+    uint64_t v[kRItems];
+#pragma unroll
+    for (int i = 0; i < kRItems; ++i) {
+        const bool take_a = ib >= n_b || (ia < n_a && lds[ia] <= lds[n_a + ib]);
+        v[i] = 0;
+        if (diag + i < n) {
+            v[i] = take_a ? lds[ia] : lds[n_a + ib];
+            if (take_a)
+                ++ia;
+            else
+                ++ib;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kRItems; ++i)
+        if (diag + i < n) lds[diag + i] = v[i];
+    __syncthreads();
+    if (out_second)
+        for (uint32_t i = threadIdx.x; i < n; i += kBlock) out_second[d0 + i] = static_cast<uint32_t>(lds[i]);
+    if (out_packed)
+        for (uint32_t i = threadIdx.x; i < n; i += kBlock) out_packed[d0 + i] = lds[i];
+}
+
 __global__ void xsec_init_kernel(uint32_t* deferred) { deferred[0] = 0; }
+
+__global__ void xindex_extend_kernel(uint32_t* xstart, uint32_t old_n, uint32_t n_x, uint32_t rows) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t x = static_cast<size_t>(old_n) + 1 + static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+         x <= n_x; x += stride)
+        xstart[x] = rows;
+}
 
 __global__ void second_column_kernel(const uint64_t* keys, size_t rows, uint32_t* second) {
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
@@ -681,12 +843,14 @@ size_t xcompact_tiles(size_t bound) { return (bound + kCTile - 1) / kCTile; }
 
 hipError_t xindex_build(const uint64_t* keys, const uint64_t* rows_dev, size_t bound, uint32_t n_x, uint32_t* xstart,
                         hipStream_t stream) {
+    if (!xstart || (bound && !keys)) return hipErrorInvalidValue;
     const size_t blocks = (static_cast<size_t>(n_x) + 1 + kXBuildSpan - 1) / kXBuildSpan;
     xindex_build_kernel<<<static_cast<unsigned>(blocks), kBlock, 0, stream>>>(keys, rows_dev, bound, n_x, xstart);
     return hipGetLastError();
 }
 
 hipError_t xindex_add(uint32_t* into, const uint32_t* other, uint32_t n_x, hipStream_t stream) {
+    if (!into || !other) return hipErrorInvalidValue;
     const size_t n = static_cast<size_t>(n_x) + 1;
     const size_t blocks = (n + kBlock * 4 - 1) / (kBlock * 4);
     xindex_add_kernel<<<static_cast<unsigned>(blocks < 8192 ? blocks : 8192), kBlock, 0, stream>>>(into, other, n);
@@ -712,6 +876,26 @@ hipError_t probe_sparse(const uint64_t* keys, size_t rows, uint32_t minor_bits, 
     return hipGetLastError();
 }
 
+hipError_t xindex_extend(const uint32_t* from, uint32_t old_n, uint32_t* to, uint32_t n_x, uint32_t rows,
+                         hipStream_t stream) {
+    if (!from || !to || n_x < old_n) return hipErrorInvalidValue;
+    hipError_t e = hipMemcpyAsync(to, from, (static_cast<size_t>(old_n) + 1) * sizeof(uint32_t),
+                                  hipMemcpyDeviceToDevice, stream);
+    if (e != hipSuccess || n_x == old_n) return e;
+    const size_t n = static_cast<size_t>(n_x - old_n);
+    const size_t blocks = (n + kBlock - 1) / kBlock;
+    xindex_extend_kernel<<<static_cast<unsigned>(blocks < 4096 ? blocks : 4096), kBlock, 0, stream>>>(to, old_n, n_x,
+                                                                                                   rows);
+    return hipGetLastError();
+}
+
+static bool xsec_rights_ok(const XSecRights& rights) {
+    if (rights.count > static_cast<uint32_t>(kMaxAntiRights)) return false;
+    for (uint32_t q = 0; q < rights.count; ++q)
+        if (!rights.second[q] || !rights.xstart[q]) return false;
+    return true;
+}
+
 size_t probe_xsec_temp_bytes(size_t rows) {
     return ((rows + kPTile - 1) / kPTile + 1) * sizeof(uint32_t);
 }
@@ -720,7 +904,7 @@ hipError_t probe_xsec(const uint64_t* keys, size_t rows, uint32_t minor_bits, co
                       uint32_t n_x, uint64_t* found_words, void* temp, size_t temp_bytes, hipStream_t stream,
                       const KernelTimer* timer) {
     if (rows == 0) return hipSuccess;
-    if (minor_bits == 0 || minor_bits > 32 || rights.count > static_cast<uint32_t>(kMaxAntiRights) || n_x == 0 ||
+    if (minor_bits == 0 || minor_bits > 32 || !xsec_rights_ok(rights) || n_x == 0 || !keys || !found_words ||
         temp_bytes < probe_xsec_temp_bytes(rows))
         return hipErrorInvalidValue;
     const size_t tiles = (rows + kPTile - 1) / kPTile;
@@ -737,8 +921,51 @@ hipError_t probe_xsec(const uint64_t* keys, size_t rows, uint32_t minor_bits, co
     return hipGetLastError();
 }
 
+hipError_t probe_sparse_sec(const uint64_t* keys, size_t rows, uint32_t minor_bits, const XSecRights& rights,
+                            uint32_t n_x, uint64_t* found_words, hipStream_t stream, const KernelTimer* timer) {
+    if (rows == 0) return hipSuccess;
+    if (minor_bits == 0 || minor_bits > 32 || !xsec_rights_ok(rights) || n_x == 0 || !keys || !found_words)
+        return hipErrorInvalidValue;
+    timer_start(timer, stream);
+    probe_sparse_sec_kernel<<<static_cast<unsigned>((rows + kBlock - 1) / kBlock), kBlock, 0, stream>>>(
+        keys, rows, minor_bits, rights, found_words);
+    timer_stop(timer, stream);
+    return hipGetLastError();
+}
+
+size_t run_merge_temp_bytes(size_t rows) {
+    const size_t tiles = (rows + kRTile - 1) / kRTile;
+    return (tiles * sizeof(uint32_t) + 255) / 256 * 256 + (tiles + 1) * sizeof(size_t);
+}
+
+hipError_t run_merge(const uint32_t* xa, const uint32_t* sa, size_t na, const uint32_t* xb, const uint32_t* sb,
+                     size_t nb, uint32_t n_x, uint32_t* out_second, uint64_t* out_packed, void* temp,
+                     size_t temp_bytes, hipStream_t stream, const KernelTimer* timer) {
+    const size_t total = na + nb;
+    if (total == 0) return hipSuccess;
+    if (n_x == 0 || !xa || (na && !sa) || (nb && (!xb || !sb)) || temp_bytes < run_merge_temp_bytes(total) ||
+        (!out_second && !out_packed))
+        return hipErrorInvalidValue;
+    if (!nb) xb = sb = nullptr;
+    const size_t tiles = (total + kRTile - 1) / kRTile;
+    uint32_t* bx = static_cast<uint32_t*>(temp);
+    size_t* pa = reinterpret_cast<size_t*>(static_cast<char*>(temp) + (tiles * sizeof(uint32_t) + 255) / 256 * 256);
+    const size_t xblocks = (static_cast<size_t>(n_x) + kBlock - 1) / kBlock;
+    run_bound_x_kernel<<<static_cast<unsigned>(xblocks < 8192 ? xblocks : 8192), kBlock, 0, stream>>>(xa, xb, n_x,
+                                                                                                    bx);
+    const size_t pblocks = (tiles + 1 + kBlock - 1) / kBlock;
+    run_partition_kernel<<<static_cast<unsigned>(pblocks < 4096 ? pblocks : 4096), kBlock, 0, stream>>>(
+        xa, sa, na, xb, sb, bx, tiles, pa);
+    timer_start(timer, stream);
+    run_merge_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(xa, sa, na, xb, sb, nb, n_x, bx, pa, tiles,
+                                                                          out_second, out_packed);
+    timer_stop(timer, stream);
+    return hipGetLastError();
+}
+
 hipError_t second_column(const uint64_t* keys, size_t rows, uint32_t* second, hipStream_t stream) {
     if (rows == 0) return hipSuccess;
+    if (!keys || !second) return hipErrorInvalidValue;
     const size_t blocks = (rows + kBlock * 4 - 1) / (kBlock * 4);
     second_column_kernel<<<static_cast<unsigned>(blocks < 8192 ? blocks : 8192), kBlock, 0, stream>>>(keys, rows,
                                                                                                      second);
