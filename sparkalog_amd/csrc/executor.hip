@@ -560,6 +560,7 @@ struct TargetRuntime {
     DevBuf<uint64_t> new_run;      // NEWT's second copy, becomes a run of FULL
     DevBuf<uint32_t> new_sec;      // that run's second column (when FULL is probed by it)
     bool csr_run = false;          // this iteration's new run is kept as index + second column only
+    DevBuf<uint32_t> new_xidx;     // its first-column index, written by the distinct
     uint64_t contrib_bound = 0;
     int anti_slot = -1;            // profiling event of this iteration's anti-join
     bool fused = false;            // this iteration's distinct wrote NEWT (candidates in FULL dropped first)
@@ -1891,10 +1892,13 @@ void session_step(sparkalog_hip_store* s) {
             // first-column index (built from NEWT, the same rows) and second
             // column only; else as packed keys (NEWT's second copy)
             t.csr_run = vmax < (1u << 24);
-            if (t.csr_run)
+            if (t.csr_run) {
                 t.new_sec = s->take_sec(M + kSecPad);
-            else
+                t.new_xidx = s->take_x(static_cast<size_t>(n_x) + 1);
+                check(xindex_clear(t.new_xidx.ptr, n_x, s->stream));
+            } else {
                 t.new_run = s->take(M);
+            }
             t.fused = true;
             t.contrib_bound = M;
         }
@@ -1922,9 +1926,19 @@ void session_step(sparkalog_hip_store* s) {
                 out = PairOutput{t.newt.ptr, nullptr, nullptr, t.csr_run ? nullptr : t.new_run.ptr, t.new_sec.ptr};
                 count = &cdev->newt_rows[fuse];
             }
+            uint32_t* xidx_out = nullptr;
+            if (fuse >= 0 && targets[static_cast<size_t>(fuse)].csr_run)
+                xidx_out = targets[static_cast<size_t>(fuse)].new_xidx.ptr;
             check(segmented_distinct_finish(rt.cand_a.ptr, M, rt.shift, known_compressed, rt.cand_b.ptr,
                                             side_rows ? rt.cand_c.ptr : nullptr, side_rows, out, count,
-                                            rt.seg_meta.ptr, mb, temp, tb, s->stream, s->kernel_timer(ev), found));
+                                            rt.seg_meta.ptr, mb, temp, tb, s->stream, s->kernel_timer(ev), found,
+                                            xidx_out));
+            if (xidx_out) {
+                // the new run's index: entries of first values without new rows
+                const size_t ftb = xindex_fill_temp_bytes(n_x);
+                void* ftemp = s->scratch(ftb);
+                check(xindex_fill(xidx_out, n_x, count, ftemp, ftb, s->stream));
+            }
         } else {
             const size_t tb = distinct_temp_bytes(M);
             void* temp = s->scratch(tb);
@@ -2052,9 +2066,12 @@ void session_step(sparkalog_hip_store* s) {
     for (uint32_t r = 0; r < n_rules; ++r) {
         const double L = static_cast<double>(rts[r].delta_rows), M = static_cast<double>(rts[r].join_rows);
         const int fr = fused_target(r);
-        // a fused distinct writes NEWT (twice: DELTA and FULL's new run)
+        // a fused distinct writes NEWT and FULL's new run: a packed copy
+        // (8 B a row), or only its second column (4 B) when the run is kept
+        // as index + second column
         const double U = !rts[r].join_rows ? 0.0
-                         : fr >= 0 ? 2.0 * static_cast<double>(chost.newt_rows[fr])
+                         : fr >= 0 ? (targets[static_cast<size_t>(fr)].csr_run ? 1.5 : 2.0) *
+                                         static_cast<double>(chost.newt_rows[fr])
                                    : static_cast<double>(chost.distinct_rows[r]);
         const double R = static_cast<double>(rts[r].index.rows);
         if (L > 0) algo_bytes += 8 * L + 8 * R + 8 * M + (M > 0 ? 8 * M + 8 * U : 0.0);
@@ -2098,12 +2115,10 @@ void session_step(sparkalog_hip_store* s) {
             DevRelation run;
             run.rows = N;
             if (t.csr_run) {
-                // first-column index from NEWT (the same rows), second column
-                // from the distinct
+                // first-column index and second column, both written with NEWT
                 run.csr = true;
                 run.sec = std::move(t.new_sec);
-                run.xidx = s->take_x(static_cast<size_t>(n_x) + 1);
-                check(xindex_build(ts.newt.keys.ptr, nullptr, N, n_x, run.xidx.ptr, s->stream));
+                run.xidx = std::move(t.new_xidx);
                 run.xidx_set(n_x);
             } else {
                 run.keys = std::move(t.new_run);
@@ -2122,6 +2137,7 @@ void session_step(sparkalog_hip_store* s) {
         } else {
             s->give(std::move(t.new_run));
             s->give_sec(std::move(t.new_sec));
+            s->give_x(std::move(t.new_xidx));
         }
         algo_bytes += settle_runs(s, ts, false);
         std::swap(ts.delta, ts.newt);

@@ -96,7 +96,7 @@ hipError_t segmented_distinct_finish(const uint64_t* keys, size_t rows, uint32_t
                                      uint64_t* side_a, uint64_t* side_b, uint64_t side_rows, PairOutput out,
                                      uint64_t* unique_rows, void* meta, size_t meta_bytes, void* sort_temp,
                                      size_t sort_bytes, hipStream_t stream, const KernelTimer* timer = nullptr,
-                                     const uint64_t* found = nullptr);
+                                     const uint64_t* found = nullptr, uint32_t* xidx_out = nullptr);
 
 // ---- sorted set operators ----
 size_t union_temp_bytes(size_t left_rows, size_t right_rows);
@@ -140,6 +140,15 @@ hipError_t probe_grouped(const uint64_t* keys, const uint64_t* rows_dev, size_t 
 // column < x. The row count is *rows_dev (<= bound) when rows_dev is non-null.
 hipError_t xindex_build(const uint64_t* keys, const uint64_t* rows_dev, size_t bound, uint32_t n_x, uint32_t* xstart,
                         hipStream_t stream);
+// A first-column index from the entries the segmented distinct wrote
+// (xidx_out: the output position of each first value's first row, every
+// other entry 0xFFFFFFFF, set by xindex_clear): entry n_x becomes *rows_dev
+// and every unset entry the next set one (a suffix minimum). temp holds
+// xindex_fill_temp_bytes(n_x).
+hipError_t xindex_clear(uint32_t* xstart, uint32_t n_x, hipStream_t stream);
+size_t xindex_fill_temp_bytes(uint32_t n_x);
+hipError_t xindex_fill(uint32_t* xstart, uint32_t n_x, const uint64_t* rows_dev, void* temp, size_t temp_bytes,
+                       hipStream_t stream);
 // to = from with n_x + 1 entries: a run's index for a larger n_x (its rows
 // all have first values below old_n, so the new entries equal `rows`).
 hipError_t xindex_extend(const uint32_t* from, uint32_t old_n, uint32_t* to, uint32_t n_x, uint32_t rows,

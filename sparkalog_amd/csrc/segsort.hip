@@ -143,7 +143,7 @@ template <typename K>
 __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(const uint64_t* keys, size_t rows, uint32_t minor_bits,
                                                           const size_t* bounds, const uint64_t* side_sorted,
                                                           PairOutput out, uint64_t* unique_rows, uint64_t* status,
-                                                          SegHeader* h, const uint64_t* found) {
+                                                          SegHeader* h, const uint64_t* found, uint32_t* xidx_out) {
     // sort keys and run majors; before ranking, the same bytes hold the
     // tile's surviving raw keys (u64) when some keys are dropped
     __shared__ uint64_t seg_buf[(kSegCap * (sizeof(K) + sizeof(uint32_t)) + 7) / 8];
@@ -427,9 +427,15 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(const uint64_t* key
             const uint64_t kept = keep_lds[wave][i];
             if ((kept >> lane) & 1ull) {
                 const uint32_t rank = static_cast<uint32_t>(k[i] >> minor_bits);
-                const uint64_t pair = (static_cast<uint64_t>(major_of[rank]) << 32) |
-                                      (static_cast<uint64_t>(k[i]) & minor_mask);
-                store_pair(out, offset + pos + lanes_below(kept), pair);
+                const uint32_t major = major_of[rank];
+                const uint64_t pair = (static_cast<uint64_t>(major) << 32) | (static_cast<uint64_t>(k[i]) & minor_mask);
+                const uint64_t at = offset + pos + lanes_below(kept);
+                store_pair(out, at, pair);
+                // the first output row of its first value starts that value's
+                // segment of the output's first-column index
+                const uint32_t p = wave * wave_len + i * kWave + lane;
+                if (xidx_out && (p == 0 || static_cast<uint32_t>(tile_keys[p - 1] >> minor_bits) != rank))
+                    xidx_out[major] = static_cast<uint32_t>(at);
             }
             pos += static_cast<uint32_t>(__popcll(kept));
         }
@@ -437,6 +443,8 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(const uint64_t* key
         for (uint32_t i = threadIdx.x; i < count; i += kSegBlock) {
             const uint64_t kk = side_sorted[side0 + i];
             store_pair(out, offset + i, ((kk >> minor_bits) << 32) | (kk & minor_mask));
+            if (xidx_out && (i == 0 || (side_sorted[side0 + i - 1] >> minor_bits) != (kk >> minor_bits)))
+                xidx_out[kk >> minor_bits] = static_cast<uint32_t>(offset + i);
         }
     }
     if (tile == tiles - 1 && threadIdx.x == 0) *unique_rows = offset + count;
@@ -473,7 +481,7 @@ hipError_t segmented_distinct_finish(const uint64_t* keys, size_t rows, uint32_t
                                      uint64_t* side_a, uint64_t* side_b, uint64_t side_rows, PairOutput out,
                                      uint64_t* unique_rows, void* meta, size_t meta_bytes, void* sort_temp,
                                      size_t sort_bytes, hipStream_t stream, const KernelTimer* timer,
-                                     const uint64_t* found) {
+                                     const uint64_t* found, uint32_t* xidx_out) {
     if (rows == 0) return set_u64(unique_rows, 0, stream);
     const SegLayout l = seg_layout(rows);
     if (meta_bytes < l.total) return hipErrorInvalidValue;
@@ -498,10 +506,10 @@ hipError_t segmented_distinct_finish(const uint64_t* keys, size_t rows, uint32_t
     // rank bits: at most log2(kSegCap) = 12 (a run per key)
     if (minor_bits + 12 <= 32)
         seg_tile_kernel<uint32_t><<<static_cast<unsigned>(l.tiles), kSegBlock, 0, stream>>>(
-            keys, rows, minor_bits, bounds, side_unique_out, out, unique_rows, status, h, found);
+            keys, rows, minor_bits, bounds, side_unique_out, out, unique_rows, status, h, found, xidx_out);
     else
         seg_tile_kernel<uint64_t><<<static_cast<unsigned>(l.tiles), kSegBlock, 0, stream>>>(
-            keys, rows, minor_bits, bounds, side_unique_out, out, unique_rows, status, h, found);
+            keys, rows, minor_bits, bounds, side_unique_out, out, unique_rows, status, h, found, xidx_out);
     timer_stop(timer, stream);
     return hipGetLastError();
 }

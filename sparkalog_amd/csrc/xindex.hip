@@ -823,6 +823,80 @@ __global__ __launch_bounds__(kBlock) void run_merge_kernel(const uint32_t* xa, c
 
 __global__ void xsec_init_kernel(uint32_t* deferred) { deferred[0] = 0; }
 
+constexpr uint32_t kFillChunk = 4096;  // index entries per block of the suffix-minimum fill
+
+__global__ void xindex_clear_kernel(uint32_t* xstart, size_t n) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        xstart[i] = 0xFFFFFFFFu;
+}
+
+// pass 1: entry n_x = the row count; each chunk's minimum
+__global__ __launch_bounds__(kBlock) void xfill_chunk_min_kernel(uint32_t* xstart, uint32_t n_x,
+                                                                 const uint64_t* rows_dev, uint32_t* chunk_min) {
+    __shared__ uint32_t red[kWavesPerBlock];
+    const size_t b0 = static_cast<size_t>(blockIdx.x) * kFillChunk;
+    const size_t n = static_cast<size_t>(n_x) + 1;
+    const size_t b1 = b0 + kFillChunk < n ? b0 + kFillChunk : n;
+    if (b1 == n && threadIdx.x == 0) xstart[n_x] = static_cast<uint32_t>(*rows_dev);
+    uint32_t m = 0xFFFFFFFFu;
+    for (size_t i = b0 + threadIdx.x; i < b1; i += kBlock) {
+        const uint32_t v = i == n_x ? static_cast<uint32_t>(*rows_dev) : xstart[i];
+        m = v < m ? v : m;
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const uint32_t o = __shfl_xor(m, off, kWave);
+        m = o < m ? o : m;
+    }
+    if (lane_id() == 0) red[wave_id()] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kWavesPerBlock; ++w) m = red[w] < m ? red[w] : m;
+        chunk_min[blockIdx.x] = m < red[0] ? m : red[0];
+    }
+}
+
+// pass 2: every entry = the minimum of itself and every entry after it
+__global__ __launch_bounds__(kBlock) void xfill_apply_kernel(uint32_t* xstart, uint32_t n_x, const uint32_t* chunk_min,
+                                                             uint32_t chunks) {
+    constexpr uint32_t kPer = kFillChunk / kBlock;  // 16 consecutive entries per thread
+    __shared__ uint32_t thread_min[kBlock];
+    __shared__ uint32_t carry_slot;
+    const size_t b0 = static_cast<size_t>(blockIdx.x) * kFillChunk;
+    const size_t n = static_cast<size_t>(n_x) + 1;
+    // carry: the minimum of every later chunk
+    uint32_t c = 0xFFFFFFFFu;
+    for (uint32_t j = blockIdx.x + 1 + threadIdx.x; j < chunks; j += kBlock) c = chunk_min[j] < c ? chunk_min[j] : c;
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const uint32_t o = __shfl_xor(c, off, kWave);
+        c = o < c ? o : c;
+    }
+    if (threadIdx.x == 0) carry_slot = 0xFFFFFFFFu;
+    __syncthreads();
+    if (lane_id() == 0) atomicMin(&carry_slot, c);
+    // this thread's entries, their suffix minimum within the thread
+    uint32_t v[kPer];
+    uint32_t m = 0xFFFFFFFFu;
+#pragma unroll
+    for (int q = kPer - 1; q >= 0; --q) {
+        const size_t i = b0 + threadIdx.x * kPer + q;
+        v[q] = i < n ? xstart[i] : 0xFFFFFFFFu;
+        m = v[q] < m ? v[q] : m;
+        v[q] = m;
+    }
+    thread_min[threadIdx.x] = m;
+    __syncthreads();
+    uint32_t after = carry_slot;  // entries of later threads and chunks
+    for (uint32_t t = threadIdx.x + 1; t < kBlock; ++t) after = thread_min[t] < after ? thread_min[t] : after;
+#pragma unroll
+    for (int q = 0; q < static_cast<int>(kPer); ++q) {
+        const size_t i = b0 + threadIdx.x * kPer + q;
+        if (i < n) xstart[i] = v[q] < after ? v[q] : after;
+    }
+}
+
 __global__ void xindex_extend_kernel(uint32_t* xstart, uint32_t old_n, uint32_t n_x, uint32_t rows) {
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
     for (size_t x = static_cast<size_t>(old_n) + 1 + static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -886,6 +960,28 @@ hipError_t xindex_extend(const uint32_t* from, uint32_t old_n, uint32_t* to, uin
     const size_t blocks = (n + kBlock - 1) / kBlock;
     xindex_extend_kernel<<<static_cast<unsigned>(blocks < 4096 ? blocks : 4096), kBlock, 0, stream>>>(to, old_n, n_x,
                                                                                                    rows);
+    return hipGetLastError();
+}
+
+hipError_t xindex_clear(uint32_t* xstart, uint32_t n_x, hipStream_t stream) {
+    if (!xstart) return hipErrorInvalidValue;
+    const size_t n = static_cast<size_t>(n_x) + 1;
+    const size_t blocks = (n + kBlock - 1) / kBlock;
+    xindex_clear_kernel<<<static_cast<unsigned>(blocks < 4096 ? blocks : 4096), kBlock, 0, stream>>>(xstart, n);
+    return hipGetLastError();
+}
+
+size_t xindex_fill_temp_bytes(uint32_t n_x) {
+    return ((static_cast<size_t>(n_x) + kFillChunk) / kFillChunk) * sizeof(uint32_t);
+}
+
+hipError_t xindex_fill(uint32_t* xstart, uint32_t n_x, const uint64_t* rows_dev, void* temp, size_t temp_bytes,
+                       hipStream_t stream) {
+    if (!xstart || !rows_dev || temp_bytes < xindex_fill_temp_bytes(n_x)) return hipErrorInvalidValue;
+    const uint32_t chunks = static_cast<uint32_t>((static_cast<size_t>(n_x) + kFillChunk) / kFillChunk);
+    uint32_t* chunk_min = static_cast<uint32_t*>(temp);
+    xfill_chunk_min_kernel<<<chunks, kBlock, 0, stream>>>(xstart, n_x, rows_dev, chunk_min);
+    xfill_apply_kernel<<<chunks, kBlock, 0, stream>>>(xstart, n_x, chunk_min, chunks);
     return hipGetLastError();
 }
 
