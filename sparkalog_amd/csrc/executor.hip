@@ -252,21 +252,36 @@ __global__ __launch_bounds__(kBlock) void dense_join_count_kernel(const uint64_t
     __shared__ uint64_t wave_totals[kWavesPerBlock];
     const size_t base = static_cast<size_t>(blockIdx.x) * kCountTile;
     const uint32_t lane = lane_id(), wave = wave_id();
+    // loads in two batches (keys, then index entries), so that a wave has
+    // all of them in flight at once instead of one dependent pair per item;
+    // indexes are clamped (n >= 1, n_keys >= 1) so no load sits in a branch
+    uint64_t dk[kCountItems];
+#pragma unroll
+    for (int i = 0; i < kCountItems; ++i) {
+        const size_t row = base + wave * kWaveRows + i * kWave + lane;
+        dk[i] = delta[row < n ? row : n - 1];
+    }
+    uint32_t m0[kCountItems], m1[kCountItems];
+#pragma unroll
+    for (int i = 0; i < kCountItems; ++i) {
+        const size_t row = base + wave * kWaveRows + i * kWave + lane;
+        const uint32_t key = column_of(dk[i], key_col);
+        const bool hit = row < n && key < n_keys;
+        const uint32_t k = hit ? key : 0u;
+        const uint32_t a = starts[k], b = starts[k + 1];
+        m0[i] = hit ? a : 0u;
+        m1[i] = hit ? b : 0u;
+    }
+#pragma unroll
+    for (int i = 0; i < kCountItems; ++i) {
+        const size_t row = base + wave * kWaveRows + i * kWave + lane;
+        if (row < n) match[row] = m0[i];
+    }
     uint64_t c[kCountItems];
     uint64_t running = 0;  // wave-local prefix in row order
 #pragma unroll
     for (int i = 0; i < kCountItems; ++i) {
-        const size_t row = base + wave * kWaveRows + i * kWave + lane;
-        uint64_t cnt = 0;
-        if (row < n) {
-            const uint32_t key = column_of(delta[row], key_col);
-            uint32_t m0 = 0;
-            if (key < n_keys) {
-                m0 = starts[key];
-                cnt = starts[key + 1] - m0;
-            }
-            match[row] = m0;
-        }
+        const uint64_t cnt = m1[i] - m0[i];
         const uint64_t incl = wave_inclusive_sum<uint64_t>(cnt);
         c[i] = running + incl - cnt;
         running += __shfl(incl, kWave - 1, kWave);

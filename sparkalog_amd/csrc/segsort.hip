@@ -179,12 +179,26 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(const uint64_t* key
         // tile_keys, free until the first pass) at their wave-local rank
         uint64_t rawv[kSegItems];
         uint64_t prev_raw = 0;  // the key before this wave's share (lane 0 of item 0)
+        // every load of the wave's share is issued before any is used
+        // (positions clamped to the tile: no load sits in a branch of its own)
+        const uint32_t last = n ? n - 1 : 0;
+        uint64_t fw[kSegItems];  // found word of each key
+        if (n > 0) {
+#pragma unroll
+            for (int i = 0; i < kSegItems; ++i) {
+                if (i >= items) continue;
+                const uint32_t p = wave * wave_len + i * kWave + lane;
+                const size_t g = b0 + (p < n ? p : last);
+                rawv[i] = keys[g];
+                if (found != nullptr) fw[i] = found[g >> 6];
+            }
+        }
         if (found == nullptr) {
 #pragma unroll
             for (int i = 0; i < kSegItems; ++i) {
                 if (i >= items) continue;
                 const uint32_t p = wave * wave_len + i * kWave + lane;
-                rawv[i] = p < n ? keys[b0 + p] : 0;
+                if (p >= n) rawv[i] = 0;
             }
             const uint32_t p0 = wave * wave_len;
             if (p0 > 0 && p0 < n) prev_raw = keys[b0 + p0 - 1];
@@ -196,8 +210,8 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(const uint64_t* key
                 if (i >= items) continue;
                 const uint32_t p = wave * wave_len + i * kWave + lane;
                 const bool valid = p < n;
-                rawv[i] = valid ? keys[b0 + p] : 0;
-                const uint64_t m = ballot(valid && !dropped(found, b0 + p));
+                const size_t g = b0 + p;
+                const uint64_t m = ballot(valid && !((fw[i] >> (g & 63)) & 1ull));
                 if (lane == 0) {
                     keep_lds[wave][i] = m;
                     item_totals[wave][i] = static_cast<uint32_t>(__popcll(m));

@@ -213,6 +213,27 @@ def test_huge_segment_probe(sparse_probe_rows, stream):
     assert summ.iterations == rsum.iterations and summ.total_new_rows == rsum.total_new_rows
 
 
+def test_high_degree_join_keys(stream):
+    # join keys of degree 250-1,200 next to keys of Poisson(1.5) degree: the
+    # join's per-row match counts and the output-balanced emit under skew
+    rng = np.random.default_rng(7)
+    n = 1500
+    deg = rng.poisson(1.5, n)
+    deg[[16, 31, 32, 40, 97, 111, 1023]] = [254, 255, 256, 1200, 254, 250, 300]
+    deg[rng.choice(np.arange(128, n), 20, replace=False)] = rng.integers(240, 270, 20)
+    src = np.repeat(np.arange(n, dtype=np.uint32), deg)
+    dst = np.concatenate([rng.choice(n, d, replace=False) for d in deg]).astype(np.uint32)
+    key = np.unique((src.astype(np.uint64) << 32) | dst)
+    edges = ((key >> 32).astype(np.uint32), (key & 0xFFFFFFFF).astype(np.uint32))
+    degrees = np.bincount(edges[0], minlength=n)
+    assert degrees[31] == 255 and degrees[97] == 254 and degrees[40] == 1200
+    store, summ = _tc(edges, stream)
+    ref, rsum = _oracle_run({0: edges}, {1: (edges, edges)}, [rule(1, 1, 0)], [1])
+    full, expect = store.full(R(1)), ref.get(1)
+    assert np.array_equal(full[0], expect[0]) and np.array_equal(full[1], expect[1])
+    assert summ.iterations == rsum.iterations and summ.total_new_rows == rsum.total_new_rows
+
+
 def test_ungrouped_projection_plan(stream):
     # output (edge.1, path.0): the distinct's major column is not DELTA's first
     # column, so candidates take the general radix-sort path
