@@ -465,17 +465,33 @@ __global__ __launch_bounds__(kBlock) void probe_xsec_kernel(const uint64_t* keys
     uint32_t g[kPItems], y[kPItems];
     uint32_t valid = 0;  // bit i: item i is a row of the tile
     uint32_t xmin = 0xFFFFFFFFu, xmax = 0;
+    // key loads in pairs, each pair issued before it is used (rows clamped
+    // to the tile, n >= 1; 32-bit offsets from the tile's base); larger
+    // batches take more than 64 VGPRs and cost a block per CU
+    constexpr int kHalf = 2;
+    const uint64_t* tile_keys = keys + base;
+    const uint32_t n_last = static_cast<uint32_t>(n - 1);
 #pragma unroll
-    for (int i = 0; i < kPItems; ++i) {
-        const size_t r = wave * kPWaveRows + i * kWave + lane;
-        g[i] = y[i] = 0;
-        if (r < n) {
-            const uint64_t k = keys[base + r];
-            g[i] = static_cast<uint32_t>(k >> minor_bits);
-            y[i] = static_cast<uint32_t>(k & minor_mask);
-            valid |= 1u << i;
-            xmin = g[i] < xmin ? g[i] : xmin;
-            xmax = g[i] > xmax ? g[i] : xmax;
+    for (int h = 0; h < kPItems; h += kHalf) {
+        uint64_t kv[kHalf];
+#pragma unroll
+        for (int j = 0; j < kHalf; ++j) {
+            const uint32_t r = wave * kPWaveRows + (h + j) * kWave + lane;
+            kv[j] = tile_keys[r < n_last ? r : n_last];
+        }
+#pragma unroll
+        for (int j = 0; j < kHalf; ++j) {
+            const int i = h + j;
+            const size_t r = wave * kPWaveRows + i * kWave + lane;
+            g[i] = y[i] = 0;
+            if (r < n) {
+                const uint64_t k = kv[j];
+                g[i] = static_cast<uint32_t>(k >> minor_bits);
+                y[i] = static_cast<uint32_t>(k & minor_mask);
+                valid |= 1u << i;
+                xmin = g[i] < xmin ? g[i] : xmin;
+                xmax = g[i] > xmax ? g[i] : xmax;
+            }
         }
     }
     // group starts: a wave's first row, and rows whose first value differs
@@ -788,12 +804,27 @@ __global__ __launch_bounds__(kBlock) void run_merge_kernel(const uint32_t* xa, c
         }
         return x_lo + lo;
     };
-    for (uint32_t i = threadIdx.x; i < n_a; i += kBlock)
-        lds[i] = (static_cast<uint64_t>(first_of(xa, xa_l, a0 + i)) << 32) | sa[a0 + i];
-    for (uint32_t i = threadIdx.x; i < n_b; i += kBlock)
-        lds[n_a + i] = (static_cast<uint64_t>(first_of(xb, xb_l, b0 + i)) << 32) | sb[b0 + i];
-    __syncthreads();
+    // the tile's second values (a's rows, then b's) are all loaded before
+    // any is used: position p of the tile is item p / kBlock of thread
+    // p % kBlock (positions clamped to the tile, which holds >= 1 row)
     const uint32_t n = n_a + n_b;
+    const uint32_t* sbb = sb ? sb : sa;
+    uint32_t sv[kRTile / kBlock];
+#pragma unroll
+    for (int q = 0; q < kRTile / kBlock; ++q) {
+        const uint32_t p = threadIdx.x + q * kBlock;
+        const uint32_t pc = p < n ? p : n - 1;
+        sv[q] = pc < n_a ? sa[a0 + pc] : sbb[b0 + (pc - n_a)];
+    }
+#pragma unroll
+    for (int q = 0; q < kRTile / kBlock; ++q) {
+        const uint32_t p = threadIdx.x + q * kBlock;
+        if (p < n_a)
+            lds[p] = (static_cast<uint64_t>(first_of(xa, xa_l, a0 + p)) << 32) | sv[q];
+        else if (p < n)
+            lds[p] = (static_cast<uint64_t>(first_of(xb, xb_l, b0 + (p - n_a))) << 32) | sv[q];
+    }
+    __syncthreads();
     const uint32_t diag = threadIdx.x * kRItems < n ? threadIdx.x * kRItems : n;
     uint32_t ia = merge_path_keys(lds, n_a, lds + n_a, n_b, diag);
     uint32_t ib = diag - ia;
