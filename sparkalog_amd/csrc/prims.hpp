@@ -91,6 +91,15 @@ __device__ __forceinline__ T block_exclusive_sum(T v, T* total, T* lds) {
     return wave_offset + inclusive - v;
 }
 
+// 16 bytes per lane from global memory straight into LDS (gfx950
+// global_load_lds_dwordx4), with no VGPRs holding the data: lane i of the
+// wave writes dst_wave[i], where dst_wave (wave-uniform) is the wave's slot
+// of 64 vectors. The compiler waits for the load before the next barrier.
+__device__ __forceinline__ void load_lds16(const uint4* src, uint4* dst_wave) {
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                     (__attribute__((address_space(3))) void*)dst_wave, 16, 0, 0);
+}
+
 __device__ __forceinline__ uint64_t load_status(const uint64_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

@@ -588,10 +588,12 @@ __global__ __launch_bounds__(kBlock) void probe_xsec_kernel(const uint64_t* keys
                 const uint32_t nvec = static_cast<uint32_t>((c1 - c0 + 3) / 4);
                 const uint4* chunk = reinterpret_cast<const uint4*>(sec + c0);
                 uint4* stage4 = reinterpret_cast<uint4*>(stage);
+                // straight into LDS, every vector of the chunk in flight at
+                // once (lanes past nvec repeat the last vector)
 #pragma unroll
                 for (int u = 0; u < kPStageVecs; ++u) {
                     const uint32_t j = u * kBlock + threadIdx.x;
-                    if (j < nvec) stage4[j] = chunk[j];
+                    load_lds16(chunk + (j < nvec ? j : nvec - 1), stage4 + u * kBlock + wave * kWave);
                 }
                 __syncthreads();
 #pragma unroll
