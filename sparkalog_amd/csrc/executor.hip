@@ -157,6 +157,9 @@ struct RelationState {
     std::vector<DevRelation> runs;
     uint64_t version = 0;
     uint32_t max_value = 0;  // max over both columns of every row ever stored
+    // a step's run merges are issued by the next step, behind its join
+    // count, so they overlap the host's wait for the join sizes
+    bool settle_pending = false;
 };
 
 // Per-op timing (hipEvents) and algorithmic bytes when profiling is on: the
@@ -603,6 +606,11 @@ struct sparkalog_hip_store {
     spk::DevBuf<uint32_t> misc_dev;       // lookback counters / error words
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
+    // pinned host copies of the step's counters (read back once per phase)
+    // and the event the host waits on for the join sizes
+    spk::Counters* counters_host = nullptr;
+    uint64_t* words_host = nullptr;  // [0]: device error flag, [1]: oversized-run rows of the distinct
+    hipEvent_t counters_ready = nullptr;
     bool profiling = false;
     spk::OpStat stats[spk::kOpCount];
     std::vector<spk::OpEvents> event_pool;
@@ -696,8 +704,21 @@ struct sparkalog_hip_store {
     std::vector<spk::TargetRuntime> ws_targets;
     std::vector<spk::RuleRuntime> ws_rules;
 
+    spk::Counters& host_counters() {
+        if (!counters_host) {
+            void* p = nullptr;
+            spk::check(hipHostMalloc(&p, sizeof(spk::Counters) + 64, hipHostMallocDefault));
+            counters_host = static_cast<spk::Counters*>(p);
+            words_host = reinterpret_cast<uint64_t*>(static_cast<char*>(p) + sizeof(spk::Counters));
+            spk::check(hipEventCreateWithFlags(&counters_ready, hipEventDisableTiming));
+        }
+        return *counters_host;
+    }
+
     ~sparkalog_hip_store() {
         if (pinned) (void)hipHostFree(pinned);
+        if (counters_host) (void)hipHostFree(counters_host);
+        if (counters_ready) (void)hipEventDestroy(counters_ready);
         for (auto& e : event_pool) {
             (void)hipEventDestroy(e.begin);
             (void)hipEventDestroy(e.end);
@@ -1707,7 +1728,7 @@ void session_step(sparkalog_hip_store* s) {
     const uint32_t n_relations = static_cast<uint32_t>(ss.relations.size());
     const uint32_t* relations = ss.relations.data();
     Counters* cdev = reinterpret_cast<Counters*>(s->counters_dev.ptr);
-    Counters chost;
+    Counters& chost = s->host_counters();
     uint32_t* lb_error = s->misc_dev.ptr + 128;  // device error flag of this step's kernels
     uint64_t& iterations = ss.iterations;
     uint64_t& total_new = ss.total_new;
@@ -1762,7 +1783,18 @@ void session_step(sparkalog_hip_store* s) {
         s->end_op(ev, kOpJoinCount, 8.0 * L);
     }
     check(hipMemcpyAsync(&chost, cdev, sizeof(Counters), hipMemcpyDeviceToHost, s->stream));
-    check(hipStreamSynchronize(s->stream));
+    check(hipEventRecord(s->counters_ready, s->stream));
+    // the previous step's run merges: queued behind the join count (which
+    // reads DELTA and the index, not FULL), they keep the device busy while
+    // the host waits for the join sizes; the probe below sees FULL settled
+    for (uint32_t i = 0; i < n_relations; ++i) {
+        RelationState& ts = *find(s, relations[i]);
+        if (ts.settle_pending) {
+            ts.settle_pending = false;
+            algo_bytes += settle_runs(s, ts, false);
+        }
+    }
+    check(hipEventSynchronize(s->counters_ready));
     // ---- join emit + distinct (generic.rs:231-249) ----
     for (uint32_t r = 0; r < n_rules; ++r) {
         RuleRuntime& rt = rts[r];
@@ -1922,7 +1954,8 @@ void session_step(sparkalog_hip_store* s) {
         if (grouped && s->segmented_distinct) {
             const size_t mb = segmented_distinct_meta_bytes(M);
             rt.seg_meta.reserve(mb);
-            uint64_t side_rows = 0;
+            uint64_t& side_rows = s->words_host[1];
+            side_rows = 0;
             check(segmented_distinct_prepare(rt.cand_a.ptr, M, rt.shift, rt.cand_b.ptr, rt.seg_meta.ptr, mb,
                                              &side_rows, s->stream, found));
             size_t tb = 0;
@@ -2068,10 +2101,11 @@ void session_step(sparkalog_hip_store* s) {
         }
     }
     check(hipMemcpyAsync(&chost, cdev, sizeof(Counters), hipMemcpyDeviceToHost, s->stream));
-    uint32_t err = 0;
-    check(hipMemcpyAsync(&err, lb_error, sizeof(err), hipMemcpyDeviceToHost, s->stream));
+    uint32_t* err_host = reinterpret_cast<uint32_t*>(&s->words_host[0]);
+    *err_host = 0;
+    check(hipMemcpyAsync(err_host, lb_error, sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
     check(hipStreamSynchronize(s->stream));
-    if (err) throw StatusError{SPARKALOG_HIP_ERR_DEVICE};
+    if (*err_host) throw StatusError{SPARKALOG_HIP_ERR_DEVICE};
     // per-iteration algorithmic bytes of the operators run (pairs are 8 bytes)
     auto fused_target = [&](uint32_t r) -> int {
         for (size_t ti = 0; ti < targets.size(); ++ti)
@@ -2154,7 +2188,11 @@ void session_step(sparkalog_hip_store* s) {
             s->give_sec(std::move(t.new_sec));
             s->give_x(std::move(t.new_xidx));
         }
-        algo_bytes += settle_runs(s, ts, false);
+        // merges wait for the next step unless FULL holds too many runs
+        if (ts.runs.size() > kMaxRuns)
+            algo_bytes += settle_runs(s, ts, false);
+        else
+            ts.settle_pending = !ts.runs.empty();
         std::swap(ts.delta, ts.newt);
         ts.version += 1;
         total_new += ts.delta.rows;
