@@ -280,7 +280,11 @@ def main():
     traffic = None
     if PROFILE_SUMMARY.exists():
         prof = json.loads(PROFILE_SUMMARY.read_text())
-        traffic = prof.get("traffic_bytes_per_launch", {}).get(dom["kernel"])
+        # a kernel chain ("a+b+c", timed as one) moves the sum of its kernels' bytes
+        per_kernel = prof.get("traffic_bytes_per_launch", {})
+        parts = dom["kernel"].split("+")
+        if all(k in per_kernel for k in parts):
+            traffic = sum(per_kernel[k] for k in parts)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom["kernel"],
                 "operator": dom_op, "launches_per_step": launches // args.steps,

@@ -179,7 +179,10 @@ enum OpKind {
 };
 const char* const kOpNames[kOpCount] = {"join_count", "join_emit", "distinct", "anti_join", "anti_join_sparse",
                                         "union", "merge_runs", "index"};
-const char* const kOpKernels[kOpCount] = {"dense_join_count_kernel", "dense_join_emit_kernel", "seg_tile_kernel",
+// (the segmented distinct's events span its three kernels: tile sort, count
+// scan and copy to the offsets)
+const char* const kOpKernels[kOpCount] = {"dense_join_count_kernel", "dense_join_emit_kernel",
+                                          "seg_tile_kernel+seg_scan_kernel+seg_write_kernel",
                                           "anti_join_kernel", "probe_sparse_kernel", "union_kernel",
                                           "merge_disjoint_kernel", ""};
 // Candidates up to this many (by default) are probed one thread each
@@ -305,25 +308,12 @@ __global__ __launch_bounds__(kBlock) void dense_join_count_kernel(const uint64_t
     if (threadIdx.x == 0) tile_sums[blockIdx.x] = tile_sum;
 }
 
-// Exclusive scan, in place, of the count's tile totals (one block; thread t
-// owns a contiguous segment), and the join total.
+// Exclusive scan, in place, of the count's tile totals (one block, n >= 1),
+// and the join total.
 constexpr int kScanThreads = 1024;
 __global__ __launch_bounds__(kScanThreads) void join_scan_kernel(uint64_t* sums, size_t n, uint64_t* total) {
     __shared__ uint64_t scan_lds[kScanThreads / kWave];
-    const size_t per = (n + kScanThreads - 1) / kScanThreads;
-    const size_t b = static_cast<size_t>(threadIdx.x) * per;
-    const size_t e = b + per < n ? b + per : n;
-    uint64_t mine = 0;
-#pragma unroll 8
-    for (size_t j = b; j < e; ++j) mine += sums[j];
-    uint64_t all = 0;
-    uint64_t run = block_exclusive_sum<uint64_t, kScanThreads / kWave>(mine, &all, scan_lds);
-    for (size_t j = b; j < e; ++j) {
-        const uint64_t v = sums[j];
-        sums[j] = run;
-        run += v;
-    }
-    if (threadIdx.x == 0) *total = all;
+    block_scan_inplace<kScanThreads, 4>(sums, n, total, scan_lds);
 }
 
 __device__ __forceinline__ uint64_t join_offset(const uint64_t* local, const uint64_t* base, size_t row) {

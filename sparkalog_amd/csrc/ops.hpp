@@ -83,7 +83,8 @@ hipError_t unique_sorted(const uint64_t* keys, const uint64_t* rows_dev, size_t 
 // (major << 32 | minor). Two phases: prepare partitions the input into tiles
 // aligned to runs, copies runs longer than a tile to `side` and returns their
 // row count on the host; finish sorts those (side/side_b, sort_temp sized by
-// distinct_temp_bytes(side_rows); unused when side_rows is 0) and every tile.
+// distinct_temp_bytes(side_rows); unused when side_rows is 0) and every tile,
+// overwriting `keys` (each tile's unique rows are staged over its input).
 // `meta` (segmented_distinct_meta_bytes(rows)) must persist between phases.
 // With `found` (probe_grouped's words over the same keys), keys whose found
 // bit is set are dropped first: the output is then (keys \ right inputs),
@@ -92,7 +93,7 @@ size_t segmented_distinct_meta_bytes(size_t rows);
 hipError_t segmented_distinct_prepare(const uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t* side,
                                       void* meta, size_t meta_bytes, uint64_t* side_rows, hipStream_t stream,
                                       const uint64_t* found = nullptr);
-hipError_t segmented_distinct_finish(const uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t side_varying,
+hipError_t segmented_distinct_finish(uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t side_varying,
                                      uint64_t* side_a, uint64_t* side_b, uint64_t side_rows, PairOutput out,
                                      uint64_t* unique_rows, void* meta, size_t meta_bytes, void* sort_temp,
                                      size_t sort_bytes, hipStream_t stream, const KernelTimer* timer = nullptr,

@@ -91,6 +91,37 @@ __device__ __forceinline__ T block_exclusive_sum(T v, T* total, T* lds) {
     return wave_offset + inclusive - v;
 }
 
+// Exclusive scan in place of n >= 1 u64 values by one block of kThreads
+// threads; *total = their sum. Chunks of kThreads * kItems consecutive values
+// (thread t owns items [t * kItems, t * kItems + kItems) of a chunk) are each
+// loaded at once (indexes clamped) and scanned with one block step, so a
+// thread waits for memory once per chunk rather than once per value.
+template <int kThreads, int kItems>
+__device__ __forceinline__ void block_scan_inplace(uint64_t* v, size_t n, uint64_t* total, uint64_t* lds) {
+    uint64_t carry = 0;
+    for (size_t c0 = 0; c0 < n; c0 += static_cast<size_t>(kThreads) * kItems) {
+        uint64_t x[kItems];
+        uint64_t mine = 0;
+#pragma unroll
+        for (int q = 0; q < kItems; ++q) {
+            const size_t idx = c0 + static_cast<size_t>(threadIdx.x) * kItems + q;
+            const uint64_t val = v[idx < n ? idx : n - 1];
+            x[q] = idx < n ? val : 0;
+            mine += x[q];
+        }
+        uint64_t all = 0;
+        uint64_t run = carry + block_exclusive_sum<uint64_t, kThreads / kWave>(mine, &all, lds);
+#pragma unroll
+        for (int q = 0; q < kItems; ++q) {
+            const size_t idx = c0 + static_cast<size_t>(threadIdx.x) * kItems + q;
+            if (idx < n) v[idx] = run;
+            run += x[q];
+        }
+        carry += all;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
 // 16 bytes per lane from global memory straight into LDS (gfx950
 // global_load_lds_dwordx4), with no VGPRs holding the data: lane i of the
 // wave writes dst_wave[i], where dst_wave (wave-uniform) is the wave's slot
