@@ -772,6 +772,7 @@ __global__ __launch_bounds__(kBlock) void run_merge_kernel(const uint32_t* xa, c
                                                            size_t tiles, uint32_t* out_second, uint64_t* out_packed) {
     __shared__ uint64_t lds[kRTile];
     __shared__ uint32_t xa_l[kRSpan], xb_l[kRSpan];
+    __shared__ uint32_t scan_max[kWavesPerBlock];
     const size_t t = blockIdx.x;
     const size_t total = na + nb;
     const size_t d0 = t * kRTile;
@@ -780,36 +781,10 @@ __global__ __launch_bounds__(kBlock) void run_merge_kernel(const uint32_t* xa, c
     const size_t a0 = pa[t], a1 = pa[t + 1];
     const size_t b0 = d0 - a0, b1 = d1 - a1;
     const uint32_t n_a = static_cast<uint32_t>(a1 - a0), n_b = static_cast<uint32_t>(b1 - b0);
-    // every row of the tile has its first value in [x_lo, x_hi]
-    const uint32_t x_lo = bx[t];
-    const uint32_t x_hi = t + 1 < tiles ? bx[t + 1] : n_x - 1;
-    const uint32_t span = x_hi - x_lo + 2;  // entries x_lo .. x_hi + 1
-    const bool staged = span <= kRSpan;     // block-uniform
-    if (staged) {
-        for (uint32_t j = threadIdx.x; j < span; j += kBlock) {
-            xa_l[j] = xa[x_lo + j];
-            xb_l[j] = xb ? xb[x_lo + j] : 0u;
-        }
-    }
-    __syncthreads();
-    // first value of a row: the largest x in [x_lo, x_hi] whose segment
-    // starts at or before it (a tile's range is a few first values in the
-    // large merges; rows are strided so that LDS accesses stay conflict-free)
-    auto first_of = [&](const uint32_t* xs, const uint32_t* xs_l, size_t row) -> uint32_t {
-        uint32_t lo = 0, hi = x_hi - x_lo;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1) >> 1;
-            if ((staged ? xs_l[mid] : xs[x_lo + mid]) <= row)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
-        return x_lo + lo;
-    };
-    // the tile's second values (a's rows, then b's) are all loaded before
-    // any is used: position p of the tile is item p / kBlock of thread
-    // p % kBlock (positions clamped to the tile, which holds >= 1 row)
     const uint32_t n = n_a + n_b;
+    // the tile's second values (a's rows, then b's) are all loaded first:
+    // position p of the tile is item p / kBlock of thread p % kBlock
+    // (positions clamped to the tile, which holds >= 1 row)
     const uint32_t* sbb = sb ? sb : sa;
     uint32_t sv[kRTile / kBlock];
 #pragma unroll
@@ -818,13 +793,92 @@ __global__ __launch_bounds__(kBlock) void run_merge_kernel(const uint32_t* xa, c
         const uint32_t pc = p < n ? p : n - 1;
         sv[q] = pc < n_a ? sa[a0 + pc] : sbb[b0 + (pc - n_a)];
     }
+    // every row of the tile has its first value in [x_lo, x_hi]
+    const uint32_t x_lo = bx[t];
+    const uint32_t x_hi = t + 1 < tiles ? bx[t + 1] : n_x - 1;
+    const uint32_t span = x_hi - x_lo + 2;  // entries x_lo .. x_hi + 1
+    const bool staged = span <= kRSpan;     // block-uniform
+    uint32_t xr[kRTile / kBlock];           // first value of each of the thread's rows
+    if (staged) {
+        // Each first value marks the tile position where its rows start, as
+        // (part << 8 | x - x_lo) with part 1 for b's rows (so b's marks
+        // dominate a's); a block max-scan then gives every position its
+        // first value. The marks live in lds (u16) before the keys do.
+        uint16_t* const mark = reinterpret_cast<uint16_t*>(lds);
+        static_assert(kRTile * sizeof(uint16_t) == kBlock * sizeof(uint4), "one vector of marks per thread");
+        static_assert(kRSpan <= 256, "x - x_lo fits a byte");
+        for (uint32_t j = threadIdx.x; j < span; j += kBlock) {
+            xa_l[j] = xa[x_lo + j];
+            xb_l[j] = xb ? xb[x_lo + j] : 0u;
+        }
+        reinterpret_cast<uint4*>(lds)[threadIdx.x] = make_uint4(0u, 0u, 0u, 0u);
+        __syncthreads();
+        // x's rows start at position clamp(xs[x] - base) of their part; when
+        // several first values start at one position (empty segments, or
+        // segments begun before the tile) the largest marks it
+        auto at = [](uint32_t v, size_t base, uint32_t len) -> uint32_t {
+            return v <= base ? 0u : (v - base < len ? static_cast<uint32_t>(v - base) : len);
+        };
+        for (uint32_t j = threadIdx.x; j + 1 < span; j += kBlock) {
+            const uint32_t pa0 = at(xa_l[j], a0, n_a), pa1 = at(xa_l[j + 1], a0, n_a);
+            if (pa0 < pa1) mark[pa0] = static_cast<uint16_t>(j);
+            if (xb) {
+                const uint32_t pb0 = at(xb_l[j], b0, n_b), pb1 = at(xb_l[j + 1], b0, n_b);
+                if (pb0 < pb1) mark[n_a + pb0] = static_cast<uint16_t>(0x100u | j);
+            }
+        }
+        __syncthreads();
+        // inclusive max-scan: thread t owns positions [8t, 8t + 8)
+        uint4 w = reinterpret_cast<const uint4*>(lds)[threadIdx.x];
+        uint32_t m[8] = {w.x & 0xFFFFu, w.x >> 16, w.y & 0xFFFFu, w.y >> 16,
+                         w.z & 0xFFFFu, w.z >> 16, w.w & 0xFFFFu, w.w >> 16};
+#pragma unroll
+        for (int i = 1; i < 8; ++i) m[i] = m[i] > m[i - 1] ? m[i] : m[i - 1];
+        uint32_t incl = m[7];
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const uint32_t o = __shfl_up(incl, off, kWave);
+            if (static_cast<int>(lane_id()) >= off) incl = o > incl ? o : incl;
+        }
+        if (lane_id() == kWave - 1) scan_max[wave_id()] = incl;
+        __syncthreads();
+        uint32_t prior = __shfl_up(incl, 1, kWave);
+        if (lane_id() == 0) prior = 0;
+        for (int wv = 0; wv < static_cast<int>(wave_id()); ++wv) prior = scan_max[wv] > prior ? scan_max[wv] : prior;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) m[i] = m[i] > prior ? m[i] : prior;
+        w = make_uint4(m[0] | (m[1] << 16), m[2] | (m[3] << 16), m[4] | (m[5] << 16), m[6] | (m[7] << 16));
+        reinterpret_cast<uint4*>(lds)[threadIdx.x] = w;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kRTile / kBlock; ++q) {
+            const uint32_t p = threadIdx.x + q * kBlock;
+            xr[q] = x_lo + (mark[p < n ? p : 0] & 0xFFu);
+        }
+        __syncthreads();  // lds holds keys from here on
+    } else {
+        // a wide first-value range: binary search over the indexes in place
+        auto first_of = [&](const uint32_t* xs, size_t row) -> uint32_t {
+            uint32_t lo = 0, hi = x_hi - x_lo;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (xs[x_lo + mid] <= row)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            return x_lo + lo;
+        };
+#pragma unroll
+        for (int q = 0; q < kRTile / kBlock; ++q) {
+            const uint32_t p = threadIdx.x + q * kBlock;
+            xr[q] = p < n_a ? first_of(xa, a0 + p) : (p < n ? first_of(xb, b0 + (p - n_a)) : 0u);
+        }
+    }
 #pragma unroll
     for (int q = 0; q < kRTile / kBlock; ++q) {
         const uint32_t p = threadIdx.x + q * kBlock;
-        if (p < n_a)
-            lds[p] = (static_cast<uint64_t>(first_of(xa, xa_l, a0 + p)) << 32) | sv[q];
-        else if (p < n)
-            lds[p] = (static_cast<uint64_t>(first_of(xb, xb_l, b0 + (p - n_a))) << 32) | sv[q];
+        if (p < n) lds[p] = (static_cast<uint64_t>(xr[q]) << 32) | sv[q];
     }
     __syncthreads();
     const uint32_t diag = threadIdx.x * kRItems < n ? threadIdx.x * kRItems : n;
