@@ -155,8 +155,7 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(uint64_t* keys, siz
     K* const tile_keys = reinterpret_cast<K*>(seg_buf);
     uint32_t* const major_of = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(seg_buf) + kSegCap * sizeof(K));
     __shared__ uint32_t wave_count[kSegWaves][kSegRadix];
-    __shared__ uint32_t item_totals[kSegWaves][kSegItems];
-    __shared__ uint64_t keep_lds[kSegWaves][kSegItems];  // ballot masks of kept keys
+    __shared__ uint32_t wave_tot[kSegWaves];  // per wave: kept keys, run starts, unique keys
     __shared__ uint32_t scan_lds[kSegWaves];
     __shared__ K red_or[kSegWaves], red_and[kSegWaves];
     __shared__ uint32_t tile_slot;
@@ -176,6 +175,8 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(uint64_t* keys, siz
     K k[kSegItems];
     uint32_t count = 0;        // unique keys of this tile
     size_t side0 = 0;          // big tile: its slice of the side output
+    uint64_t kept[kSegItems];  // ballot masks of the unique keys (wave-uniform)
+    uint32_t wave_start = 0;   // unique keys in earlier waves
     if (!big) {
         // load; k = (run starts at or before the position within this wave)
         // << minor_bits | minor; majors of run starts are staged in LDS (in
@@ -206,31 +207,34 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(uint64_t* keys, siz
             const uint32_t p0 = wave * wave_len;
             if (p0 > 0 && p0 < n) prev_raw = keys[b0 + p0 - 1];
         } else {
-            // keep only the keys whose found bit is clear, in order
-            uint32_t pos = 0;
+            // keep only the keys whose found bit is clear, in order; the
+            // ballot masks stay in (scalar) registers, and only the waves'
+            // totals go through LDS
+            uint64_t keep[kSegItems];
+            uint32_t mine = 0;
 #pragma unroll
             for (int i = 0; i < kSegItems; ++i) {
+                keep[i] = 0;
                 if (i >= items) continue;
                 const uint32_t p = wave * wave_len + i * kWave + lane;
                 const bool valid = p < n;
                 const size_t g = b0 + p;
-                const uint64_t m = ballot(valid && !((fw[i] >> (g & 63)) & 1ull));
-                if (lane == 0) {
-                    keep_lds[wave][i] = m;
-                    item_totals[wave][i] = static_cast<uint32_t>(__popcll(m));
-                }
+                keep[i] = ballot(valid && !((fw[i] >> (g & 63)) & 1ull));
+                mine += static_cast<uint32_t>(__popcll(keep[i]));
             }
+            if (lane == 0) wave_tot[wave] = mine;
             __syncthreads();
-            uint32_t survivors = 0;
-            for (int w = 0; w < kSegWaves; ++w)
-                for (int i = 0; i < items; ++i) {
-                    if (w < static_cast<int>(wave)) pos += item_totals[w][i];
-                    survivors += item_totals[w][i];
-                }
+            uint32_t pos = 0, survivors = 0;
+#pragma unroll
+            for (int w = 0; w < kSegWaves; ++w) {
+                const uint32_t c = wave_tot[w];
+                if (w < static_cast<int>(wave)) pos += c;
+                survivors += c;
+            }
 #pragma unroll
             for (int i = 0; i < kSegItems; ++i) {
                 if (i >= items) continue;
-                const uint64_t m = keep_lds[wave][i];
+                const uint64_t m = keep[i];
                 if ((m >> lane) & 1ull) seg_buf[pos + lanes_below(m)] = rawv[i];
                 pos += static_cast<uint32_t>(__popcll(m));
             }
@@ -277,10 +281,10 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(uint64_t* keys, siz
             k[i] = (static_cast<K>(local) << minor_bits) | static_cast<K>(raw & minor_mask);
             starts_before += static_cast<uint32_t>(__popcll(m));
         }
-        if (lane == 0) item_totals[wave][0] = starts_before;
+        if (lane == 0) wave_tot[wave] = starts_before;
         __syncthreads();
         uint32_t wave_base = 0;
-        for (int w = 0; w < static_cast<int>(wave); ++w) wave_base += item_totals[w][0];
+        for (int w = 0; w < static_cast<int>(wave); ++w) wave_base += wave_tot[w];
         for (uint32_t j = lane; j < starts_before; j += kWave) major_of[wave_base + j] = staged[j];
         // rank = wave_base + local - 1 (unsigned wrap-around is exact)
         const K rank_add = static_cast<K>(static_cast<K>(static_cast<K>(wave_base) - static_cast<K>(1)) << minor_bits);
@@ -402,21 +406,25 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(uint64_t* keys, siz
             if (p < n) tile_keys[p] = k[i];
         }
         __syncthreads();
+        uint32_t mine = 0;
 #pragma unroll
         for (int i = 0; i < kSegItems; ++i) {
+            kept[i] = 0;
             if (i >= items) continue;
             const uint32_t p = wave * wave_len + i * kWave + lane;
             const bool valid = p < n;
             const K prev = p > 0 && valid ? tile_keys[p - 1] : static_cast<K>(~k[i]);
-            const uint64_t kept = ballot(valid && (p == 0 || prev != k[i]));
-            if (lane == 0) {
-                keep_lds[wave][i] = kept;
-                item_totals[wave][i] = static_cast<uint32_t>(__popcll(kept));
-            }
+            kept[i] = ballot(valid && (p == 0 || prev != k[i]));
+            mine += static_cast<uint32_t>(__popcll(kept[i]));
         }
+        if (lane == 0) wave_tot[wave] = mine;
         __syncthreads();
-        for (int w = 0; w < kSegWaves; ++w)
-            for (int i = 0; i < items; ++i) count += item_totals[w][i];
+#pragma unroll
+        for (int w = 0; w < kSegWaves; ++w) {
+            const uint32_t c = wave_tot[w];
+            if (w < static_cast<int>(wave)) wave_start += c;
+            count += c;
+        }
     } else {
         // oversized tile: its unique keys are a slice of the globally sorted side output
         const uint64_t first_major = keys[b0] >> minor_bits;
@@ -432,21 +440,18 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(uint64_t* keys, siz
     // seg_write_kernel moves them to their offsets once every count is known
     if (threadIdx.x == 0) counts[tile] = count;
     if (!big) {
-        uint32_t wave_start = 0;
-        for (int w = 0; w < static_cast<int>(wave); ++w)
-            for (int i = 0; i < items; ++i) wave_start += item_totals[w][i];
         uint32_t pos = wave_start;
 #pragma unroll
         for (int i = 0; i < kSegItems; ++i) {
             if (i >= items) continue;
-            const uint64_t kept = keep_lds[wave][i];
-            if ((kept >> lane) & 1ull) {
+            const uint64_t kept_i = kept[i];
+            if ((kept_i >> lane) & 1ull) {
                 const uint32_t rank = static_cast<uint32_t>(k[i] >> minor_bits);
                 const uint32_t major = major_of[rank];
-                keys[b0 + pos + lanes_below(kept)] =
+                keys[b0 + pos + lanes_below(kept_i)] =
                     (static_cast<uint64_t>(major) << 32) | (static_cast<uint64_t>(k[i]) & minor_mask);
             }
-            pos += static_cast<uint32_t>(__popcll(kept));
+            pos += static_cast<uint32_t>(__popcll(kept_i));
         }
     }
 }
