@@ -1117,40 +1117,73 @@ void settle_all(sparkalog_hip_store* s, RelationState& st) {
 // batched and served fixpoints keep many independent instances in one
 // relation, instance = first column >> instance bits)
 
-__global__ void instance_stats_kernel(const uint64_t* keys, size_t n, uint32_t bits, uint32_t instances,
-                                      unsigned long long* rows, unsigned long long* digests) {
-    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
-    const size_t gtid = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const size_t trips = (n + stride - 1) / stride;
+// Rows and order-independent digest per instance over sorted keys. Each wave
+// walks its own contiguous range of rows (instances are contiguous in sorted
+// keys) and keeps the current instance's sums in registers, adding them to the
+// instance's counters when the instance changes: an atomic per instance and
+// wave instead of one per 64 rows, all of an instance's waves on one counter.
+// A batch of 64 rows spanning two instances adds per row.
+constexpr uint32_t kStatsBatch = 8;                             // row batches loaded at once
+constexpr uint32_t kStatsWaveRows = 8 * kStatsBatch * kWave;    // rows per wave
+__global__ __launch_bounds__(kBlock) void instance_stats_kernel(const uint64_t* keys, size_t n, uint32_t bits,
+                                                                uint32_t instances, unsigned long long* rows,
+                                                                unsigned long long* digests) {
+    const size_t wave_g = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+    const size_t r0 = wave_g * kStatsWaveRows;
+    if (r0 >= n) return;  // wave-uniform
+    const size_t r1 = r0 + kStatsWaveRows < n ? r0 + kStatsWaveRows : n;
+    const uint32_t lane = lane_id();
     const uint32_t mask = bits >= 32 ? 0xFFFFFFFFu : ((1u << bits) - 1u);
-    for (size_t t = 0; t < trips; ++t) {
-        const size_t i = t * stride + gtid;
-        const bool valid = i < n;
-        uint32_t inst = 0xFFFFFFFFu;
-        unsigned long long c = 0;
-        if (valid) {
-            const uint64_t k = keys[i];
-            const uint32_t x = static_cast<uint32_t>(k >> 32), z = static_cast<uint32_t>(k);
-            inst = bits >= 32 ? 0 : x >> bits;
+    uint32_t cur = 0xFFFFFFFFu;     // instance being summed (wave-uniform)
+    unsigned long long acc = 0;     // this lane's digest share of it
+    unsigned long long acc_rows = 0;
+    auto flush = [&]() {
+        const unsigned long long sum = wave_sum<unsigned long long>(acc);
+        if (lane == 0 && cur < instances && acc_rows) {
+            atomicAdd(&digests[cur], sum);
+            atomicAdd(&rows[cur], acc_rows);
+        }
+        acc = 0;
+        acc_rows = 0;
+    };
+    for (size_t b = r0; b < r1; b += static_cast<size_t>(kStatsBatch) * kWave) {
+        uint64_t kk[kStatsBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kStatsBatch; ++j) {
+            const size_t i = b + j * kWave + lane;
+            kk[j] = keys[i < r1 ? i : r1 - 1];
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kStatsBatch; ++j) {
+            const size_t i = b + j * kWave + lane;
+            const bool valid = i < r1;
+            const uint32_t x = static_cast<uint32_t>(kk[j] >> 32), z = static_cast<uint32_t>(kk[j]);
+            const uint32_t inst = valid ? (bits >= 32 ? 0u : x >> bits) : 0xFFFFFFFFu;
             uint64_t zz = ((static_cast<uint64_t>(x & mask) << 32) | (z & mask)) + 0x9E3779B97F4A7C15ull;
             zz = (zz ^ (zz >> 30)) * 0xBF58476D1CE4E5B9ull;
             zz = (zz ^ (zz >> 27)) * 0x94D049BB133111EBull;
-            c = zz ^ (zz >> 31);
-        }
-        const uint32_t lead = __builtin_amdgcn_readfirstlane(inst);
-        const uint64_t valid_mask = ballot(valid);
-        if (valid_mask == 0) continue;
-        if (ballot(valid && inst == lead) == valid_mask) {
-            const unsigned long long sum = wave_sum<unsigned long long>(c);
-            if (lane_id() == 0 && lead < instances) {
-                atomicAdd(&digests[lead], sum);
-                atomicAdd(&rows[lead], static_cast<unsigned long long>(__popcll(valid_mask)));
+            const unsigned long long c = valid ? (zz ^ (zz >> 31)) : 0ull;
+            const uint64_t valid_mask = ballot(valid);
+            if (valid_mask == 0) continue;  // wave-uniform
+            const uint32_t lead = __builtin_amdgcn_readfirstlane(inst);
+            if (ballot(valid && inst == lead) == valid_mask) {
+                if (lead != cur) {
+                    flush();
+                    cur = lead;
+                }
+                acc += c;
+                acc_rows += static_cast<unsigned long long>(__popcll(valid_mask));
+            } else {
+                flush();
+                cur = 0xFFFFFFFFu;
+                if (valid && inst < instances) {
+                    atomicAdd(&digests[inst], c);
+                    atomicAdd(&rows[inst], 1ull);
+                }
             }
-        } else if (valid && inst < instances) {
-            atomicAdd(&digests[inst], c);
-            atomicAdd(&rows[inst], 1ull);
         }
     }
+    flush();
 }
 
 // lower bounds of a set of keys (one thread each)
@@ -2279,9 +2312,12 @@ extern "C" int sparkalog_hip_store_instance_stats(sparkalog_hip_store* s, uint32
     unsigned long long* acc =
         reinterpret_cast<unsigned long long*>(s->probes(2 * static_cast<size_t>(instances) + 2));
     check(fill_u64(reinterpret_cast<uint64_t*>(acc), 2 * static_cast<size_t>(instances), 0, s->stream));
-    if (st->full.rows && instances)
-        instance_stats_kernel<<<grid_for(st->full.rows), kBlock, 0, s->stream>>>(
+    if (st->full.rows && instances) {
+        const size_t waves = (st->full.rows + kStatsWaveRows - 1) / kStatsWaveRows;
+        const size_t blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+        instance_stats_kernel<<<static_cast<unsigned>(blocks), kBlock, 0, s->stream>>>(
             st->full.keys.ptr, st->full.rows, instance_bits, instances, acc, acc + instances);
+    }
     check(hipGetLastError());
     if (instances) {
         check(hipMemcpyAsync(rows, acc, instances * sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));

@@ -158,8 +158,8 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(uint64_t* keys, siz
     __shared__ uint32_t wave_tot[kSegWaves];  // per wave: kept keys, run starts, unique keys
     __shared__ uint32_t scan_lds[kSegWaves];
     __shared__ K red_or[kSegWaves], red_and[kSegWaves];
-    __shared__ uint32_t tile_slot;
-    const uint32_t tile = next_tile(&h->counter, &tile_slot);
+    // tiles are independent (no look-back), so dispatch order needs no ticket
+    const uint32_t tile = blockIdx.x;
     const size_t tiles = seg_tiles(rows);
     if (tile >= tiles) return;
     const uint32_t lane = lane_id(), wave = wave_id();
