@@ -142,10 +142,39 @@ __global__ __launch_bounds__(kBlock) void seg_gather_kernel(const uint64_t* keys
 }
 
 // One tile (whole runs, at most kSegCap keys) per block. K is the in-LDS
+// block_search (first index in [lo, hi) where `before` turns false, for a
+// predicate true on a prefix) for the 512-thread tile kernel, reducing
+// through `red` (kSegWaves u32 of LDS the caller has free) instead of
+// __syncthreads_count, which brings LDS of its own.
+template <typename Pred>
+__device__ __forceinline__ size_t tile_search(size_t lo, size_t hi, Pred before, uint32_t* red) {
+    while (lo < hi) {
+        const size_t span = hi - lo;
+        const size_t stride = span <= static_cast<size_t>(kSegBlock) ? 1 : (span + kSegBlock - 1) / kSegBlock;
+        const size_t probe = lo + static_cast<size_t>(threadIdx.x) * stride;
+        const uint64_t m = ballot(probe < hi && before(probe));
+        if (lane_id() == 0) red[wave_id()] = static_cast<uint32_t>(__popcll(m));
+        __syncthreads();
+        uint32_t count = 0;
+#pragma unroll
+        for (int w = 0; w < kSegWaves; ++w) count += red[w];
+        __syncthreads();
+        if (count == 0) return lo;
+        if (stride == 1) return lo + count;
+        // answer in (probe[count - 1], probe[count]]
+        const size_t last_true = lo + static_cast<size_t>(count - 1) * stride;
+        const size_t next = lo + static_cast<size_t>(count) * stride;
+        lo = last_true + 1;
+        if (next < hi) hi = next;
+    }
+    return lo;
+}
+
 // sort key (local run rank << minor_bits | minor): u32 when rank and minor
 // fit in 32 bits (minor_bits <= 20), else u64.
 template <typename K>
-__global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(uint64_t* keys, size_t rows, uint32_t minor_bits,
+__global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) void seg_tile_kernel(
+    uint64_t* keys, size_t rows, uint32_t minor_bits,
                                                           const size_t* bounds, const uint64_t* side_sorted,
                                                           uint64_t* counts, SegHeader* h, const uint64_t* found) {
     // sort keys and run majors; before ranking, the same bytes hold the
@@ -154,10 +183,18 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(uint64_t* keys, siz
     static_assert(sizeof(seg_buf) >= kSegCap * sizeof(uint64_t), "raw keys fit");
     K* const tile_keys = reinterpret_cast<K*>(seg_buf);
     uint32_t* const major_of = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(seg_buf) + kSegCap * sizeof(K));
-    __shared__ uint32_t wave_count[kSegWaves][kSegRadix];
-    __shared__ uint32_t wave_tot[kSegWaves];  // per wave: kept keys, run starts, unique keys
-    __shared__ uint32_t scan_lds[kSegWaves];
-    __shared__ K red_or[kSegWaves], red_and[kSegWaves];
+    // digit counts per wave (a wave ranks at most 512 keys: u16), 40 KB in
+    // all for u32 sort keys, so 4 blocks fit a CU. The small arrays alias
+    // space idle while they are used: per-wave totals and the OR/AND of the
+    // keys live in wave_count outside the passes, the scan's scratch in
+    // tile_keys during a pass's digit scan (the keys are in registers then).
+    __shared__ uint16_t wave_count[kSegWaves][kSegRadix];
+    uint32_t* const wave_tot = reinterpret_cast<uint32_t*>(&wave_count[0][0]);  // kept keys, run starts, unique keys
+    K* const red_or = reinterpret_cast<K*>(&wave_count[0][0]) + 2 * kSegWaves;
+    K* const red_and = red_or + kSegWaves;
+    uint32_t* const scan_lds = reinterpret_cast<uint32_t*>(tile_keys);
+    static_assert(sizeof(uint32_t) * kSegWaves + 2 * sizeof(K) * kSegWaves + 2 * sizeof(K) * kSegWaves <=
+                      sizeof(uint16_t) * kSegWaves * kSegRadix, "aliases fit");
     // tiles are independent (no look-back), so dispatch order needs no ticket
     const uint32_t tile = blockIdx.x;
     const size_t tiles = seg_tiles(rows);
@@ -186,7 +223,7 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(uint64_t* keys, siz
         // every load of the wave's share is issued before any is used
         // (positions clamped to the tile: no load sits in a branch of its own)
         const uint32_t last = n ? n - 1 : 0;
-        uint64_t fw[kSegItems];  // found word of each key
+        uint32_t fw[kSegItems];  // the 32-bit half of the found bitmap holding each key's bit
         if (n > 0) {
 #pragma unroll
             for (int i = 0; i < kSegItems; ++i) {
@@ -194,7 +231,7 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(uint64_t* keys, siz
                 const uint32_t p = wave * wave_len + i * kWave + lane;
                 const size_t g = b0 + (p < n ? p : last);
                 rawv[i] = keys[g];
-                if (found != nullptr) fw[i] = found[g >> 6];
+                if (found != nullptr) fw[i] = reinterpret_cast<const uint32_t*>(found)[g >> 5];
             }
         }
         if (found == nullptr) {
@@ -219,7 +256,7 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(uint64_t* keys, siz
                 const uint32_t p = wave * wave_len + i * kWave + lane;
                 const bool valid = p < n;
                 const size_t g = b0 + p;
-                keep[i] = ballot(valid && !((fw[i] >> (g & 63)) & 1ull));
+                keep[i] = ballot(valid && !((fw[i] >> (g & 31)) & 1u));
                 mine += static_cast<uint32_t>(__popcll(keep[i]));
             }
             if (lane == 0) wave_tot[wave] = mine;
@@ -328,9 +365,11 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(uint64_t* keys, siz
             width = 64 - __builtin_clzll(static_cast<uint64_t>(vary)) - lb;
         }
         const int passes = (width + kSegDigitBits - 1) / kSegDigitBits;
+        __syncthreads();  // red_or / red_and (in wave_count) read by every thread
         for (int pass = 0; pass < passes; ++pass) {  // block-uniform
             const int shift = lb + pass * kSegDigitBits;
-            for (int i = threadIdx.x; i < kSegWaves * kSegRadix; i += kSegBlock) (&wave_count[0][0])[i] = 0;
+            for (int i = threadIdx.x; i < kSegWaves * kSegRadix / 2; i += kSegBlock)
+                reinterpret_cast<uint32_t*>(&wave_count[0][0])[i] = 0;
             __syncthreads();
             uint32_t ranks[kSegItems];
             if (pass == 0) {
@@ -342,7 +381,12 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(uint64_t* keys, siz
                     if (i >= items) continue;
                     const uint32_t p = wave * wave_len + i * kWave + lane;
                     const uint32_t digit = static_cast<uint32_t>(k[i] >> shift) & (kSegRadix - 1);
-                    if (p < n) ranks[i] = atomicAdd(&wave_count[wave][digit], 1u);
+                    if (p < n) {
+                        // two u16 counts per u32 word
+                        uint32_t* const word = reinterpret_cast<uint32_t*>(&wave_count[wave][0]) + (digit >> 1);
+                        const uint32_t sh = 16u * (digit & 1u);
+                        ranks[i] = (atomicAdd(word, 1u << sh) >> sh) & 0xFFFFu;
+                    }
                 }
             } else {
 #pragma unroll
@@ -356,7 +400,7 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(uint64_t* keys, siz
                     const uint32_t below = lanes_below(peers);
                     ranks[i] = before + below;
                     if (valid && below == 0)
-                        wave_count[wave][digit] = before + static_cast<uint32_t>(__popcll(peers));
+                        wave_count[wave][digit] = static_cast<uint16_t>(before + static_cast<uint32_t>(__popcll(peers)));
                 }
             }
             __syncthreads();
@@ -368,7 +412,7 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(uint64_t* keys, siz
 #pragma unroll
                 for (int w = 0; w < kSegWaves; ++w) {
                     const uint32_t c = wave_count[w][d];
-                    wave_count[w][d] = tc;
+                    wave_count[w][d] = static_cast<uint16_t>(tc);
                     tc += c;
                 }
             }
@@ -376,7 +420,8 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(uint64_t* keys, siz
             const uint32_t local_start = block_exclusive_sum<uint32_t, kSegWaves>(tc, &total_unused, scan_lds);
             {
 #pragma unroll
-                for (int w = 0; w < kSegWaves; ++w) wave_count[w][d] += local_start;
+                for (int w = 0; w < kSegWaves; ++w)
+                    wave_count[w][d] = static_cast<uint16_t>(wave_count[w][d] + local_start);
             }
             __syncthreads();
 #pragma unroll
@@ -430,9 +475,10 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_kernel(uint64_t* keys, siz
         const uint64_t first_major = keys[b0] >> minor_bits;
         const uint64_t last_major = keys[b1 - 1] >> minor_bits;
         const size_t n_side = h->side_unique;
-        side0 = block_search(0, n_side, [&](size_t i) { return (side_sorted[i] >> minor_bits) < first_major; });
-        const size_t side1 =
-            block_search(side0, n_side, [&](size_t i) { return (side_sorted[i] >> minor_bits) <= last_major; });
+        side0 = tile_search(
+            0, n_side, [&](size_t i) { return (side_sorted[i] >> minor_bits) < first_major; }, wave_tot);
+        const size_t side1 = tile_search(
+            side0, n_side, [&](size_t i) { return (side_sorted[i] >> minor_bits) <= last_major; }, wave_tot);
         count = static_cast<uint32_t>(side1 - side0);
     }
     // the tile's unique rows go back over its own input slice (every load of
