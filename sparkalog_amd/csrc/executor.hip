@@ -368,10 +368,14 @@ __global__ __launch_bounds__(kBlock) void dense_join_emit_kernel(const uint64_t*
                                                                  const size_t* row_start, EmitSpec spec,
                                                                  uint64_t* out) {
     constexpr int kPer = kEmitTile / kBlock;  // 8 output positions per thread
-    __shared__ uint64_t row_offset[kEmitStage];
+    // 18 KB: 8 blocks a CU. A row's first output is kept relative to the
+    // tile's first (u32, wrapping: differences are exact below 2^32, and a
+    // row has fewer outputs than that); a position's row fits 16 bits.
+    __shared__ uint32_t row_offset[kEmitStage];
     __shared__ uint64_t row_key[kEmitStage];
     __shared__ uint32_t row_match[kEmitStage];
-    __shared__ uint32_t row_of[kEmitTile];
+    __shared__ uint16_t row_of[kEmitTile];
+    static_assert(kEmitStage <= 65536, "a staged row index fits 16 bits");
     __shared__ uint32_t scan_lds[kWavesPerBlock];
     const uint64_t n_tiles = (total + kEmitTile - 1) / kEmitTile;
     for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
@@ -390,11 +394,11 @@ __global__ __launch_bounds__(kBlock) void dense_join_emit_kernel(const uint64_t*
                 const uint32_t m0 = match_base[l0 + i];
                 const uint64_t next = l0 + i + 1 < n ? join_offset(local, tile_base, l0 + i + 1) : total;
                 const uint64_t cnt = next - off;
-                row_offset[i] = off;
+                row_offset[i] = static_cast<uint32_t>(off - o0);
                 row_key[i] = dk;
                 row_match[i] = m0;
                 // first output position of row i inside the tile (row 0 owns o0)
-                if (i > 0 && cnt > 0 && off < o1) row_of[off - o0] = i;
+                if (i > 0 && cnt > 0 && off < o1) row_of[off - o0] = static_cast<uint16_t>(i);
             }
             __syncthreads();
             // inclusive max-scan of row_of: thread t owns positions [kPer*t, kPer*t + kPer)
@@ -421,17 +425,17 @@ __global__ __launch_bounds__(kBlock) void dense_join_emit_kernel(const uint64_t*
 #pragma unroll
             for (int q = 0; q < kPer; ++q) {
                 const uint32_t v = local[q] > prior ? local[q] : prior;
-                row_of[threadIdx.x * kPer + q] = v;
+                row_of[threadIdx.x * kPer + q] = static_cast<uint16_t>(v);
             }
             __syncthreads();
         }
         for (uint64_t o = o0 + threadIdx.x; o < o1; o += kBlock) {
-            uint64_t dk, row_start_off;
-            uint32_t match;
+            uint64_t dk;
+            uint32_t match, within;  // the row's first matching index row, and o's place among its outputs
             if (staged) {
                 const uint32_t i = row_of[o - o0];
                 dk = row_key[i];
-                row_start_off = row_offset[i];
+                within = static_cast<uint32_t>(o - o0) - row_offset[i];
                 match = row_match[i];
             } else {
                 size_t lo = l0, hi = l1;
@@ -443,10 +447,10 @@ __global__ __launch_bounds__(kBlock) void dense_join_emit_kernel(const uint64_t*
                         hi = mid;
                 }
                 dk = delta[lo - 1];
-                row_start_off = join_offset(local, tile_base, lo - 1);
+                within = static_cast<uint32_t>(o - join_offset(local, tile_base, lo - 1));
                 match = match_base[lo - 1];
             }
-            const uint32_t r_other = right_other[match + static_cast<uint32_t>(o - row_start_off)];
+            const uint32_t r_other = right_other[match + within];
             // the right row is (join key, other); its join key equals the
             // left row's key column
             uint32_t v[2];
