@@ -234,6 +234,21 @@ __global__ void dense_starts_kernel(const uint64_t* sorted, size_t n, uint32_t* 
     }
 }
 
+__global__ void max_degree_kernel(const uint32_t* starts, size_t n_keys, uint32_t* out) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    uint32_t m = 0;
+    for (size_t v = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n_keys; v += stride) {
+        const uint32_t d = starts[v + 1] - starts[v];
+        m = d > m ? d : m;
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const uint32_t o = __shfl_xor(m, off, kWave);
+        m = o > m ? o : m;
+    }
+    if (lane_id() == 0 && m) atomicMax(out, m);
+}
+
 __device__ __forceinline__ uint32_t column_of(uint64_t key, uint32_t col) {
     return col == 0 ? static_cast<uint32_t>(key >> 32) : static_cast<uint32_t>(key);
 }
@@ -250,10 +265,16 @@ __device__ __forceinline__ uint32_t column_of(uint64_t key, uint32_t col) {
 constexpr int kCountItems = 8;
 constexpr int kCountTile = kBlock * kCountItems;
 
+template <typename O>
 __global__ __launch_bounds__(kBlock) void dense_join_count_kernel(const uint64_t* delta, size_t n, uint32_t key_col,
                                                                   const uint32_t* starts, uint32_t n_keys,
-                                                                  uint64_t* local, uint32_t* match,
-                                                                  uint64_t* tile_sums) {
+                                                                  O* local, uint32_t* match,
+                                                                  uint64_t* tile_sums, uint32_t* clear,
+                                                                  uint32_t clear_words) {
+    // the step's status words, cleared by the first block (no kernel before
+    // the count in a step reads them)
+    if (blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < clear_words; i += kBlock) clear[i] = 0;
     constexpr int kWaveRows = kCountItems * kWave;
     __shared__ uint64_t wave_totals[kWavesPerBlock];
     const size_t base = static_cast<size_t>(blockIdx.x) * kCountTile;
@@ -303,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void dense_join_count_kernel(const uint64_t
 #pragma unroll
     for (int i = 0; i < kCountItems; ++i) {
         const size_t row = base + wave * kWaveRows + i * kWave + lane;
-        if (row < n) local[row] = wave_base + c[i];
+        if (row < n) local[row] = static_cast<O>(wave_base + c[i]);
     }
     if (threadIdx.x == 0) tile_sums[blockIdx.x] = tile_sum;
 }
@@ -316,8 +337,12 @@ __global__ __launch_bounds__(kScanThreads) void join_scan_kernel(uint64_t* sums,
     block_scan_inplace<kScanThreads, 4>(sums, n, total, scan_lds);
 }
 
-__device__ __forceinline__ uint64_t join_offset(const uint64_t* local, const uint64_t* base, size_t row) {
-    return local[row] + base[row / kCountTile];
+// A delta row's first output: its offset within its count tile (u32 when
+// every tile's total fits: the index's largest degree times kCountTile is
+// below 2^32, else u64) plus the tile's base.
+template <typename O>
+__device__ __forceinline__ uint64_t join_offset(const O* local, const uint64_t* base, size_t row) {
+    return static_cast<uint64_t>(local[row]) + base[row / kCountTile];
 }
 
 struct EmitSpec {
@@ -330,11 +355,12 @@ struct EmitSpec {
 };
 
 constexpr int kEmitTile = 1024;   // output rows per tile
-constexpr int kEmitStage = 1024;  // delta rows staged per tile (24 KB of LDS: 6 blocks a CU)
+constexpr int kEmitStage = 1024;  // delta rows staged per tile
 
 // row_start[t] = the delta row that contains output t*kEmitTile (one thread
 // per tile; the last entry is n).
-__global__ void emit_partition_kernel(const uint64_t* local, const uint64_t* base, size_t n, size_t total,
+template <typename O>
+__global__ void emit_partition_kernel(const O* local, const uint64_t* base, size_t n, size_t total,
                                       size_t* row_start, size_t n_tiles) {
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
     for (size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; t <= n_tiles; t += stride) {
@@ -361,8 +387,9 @@ __global__ void emit_partition_kernel(const uint64_t* local, const uint64_t* bas
 // and a block max-scan gives every output position its row.
 // match_base[row] = starts[key] of each delta row (written by the count),
 // so the emit reads it in row order instead of gathering the index's starts.
+template <typename O>
 __global__ __launch_bounds__(kBlock) void dense_join_emit_kernel(const uint64_t* delta, size_t n,
-                                                                 const uint32_t* right_other, const uint64_t* local,
+                                                                 const uint32_t* right_other, const O* local,
                                                                  const uint64_t* tile_base, const uint32_t* match_base,
                                                                  size_t total,
                                                                  const size_t* row_start, EmitSpec spec,
@@ -544,13 +571,15 @@ struct CachedIndex {
     DevBuf<uint32_t> starts;  // dense CSR over key values [0, n_keys]
     uint32_t n_keys = 0;
     uint64_t rows = 0;
+    uint32_t max_degree = 0;  // most rows of one key
 };
 
 struct RuleRuntime {
     sparkalog_hip_rule plan{};
     CachedIndex index;
     DevBuf<uint64_t> tile_base; // join count: tile totals, then their exclusive prefix
-    DevBuf<uint64_t> offsets;   // join offsets per delta row within its count tile
+    DevBuf<uint64_t> offsets;   // join offsets per delta row within its count tile (u32 when narrow)
+    bool narrow = false;        // offsets held as u32
     DevBuf<uint32_t> match;     // first matching index row per delta row
     DevBuf<uint64_t> found;     // candidates already in FULL (one bit each)
     DevBuf<size_t> row_start;   // first delta row of each emit tile
@@ -603,7 +632,7 @@ struct sparkalog_hip_store {
     // pinned host copies of the step's counters (read back once per phase)
     // and the event the host waits on for the join sizes
     spk::Counters* counters_host = nullptr;
-    uint64_t* words_host = nullptr;  // [0]: device error flag, [1]: oversized-run rows of the distinct
+    uint64_t* words_host = nullptr;  // [0]: device error flag, [1]: oversized-run rows of the distinct, [2]: index degree
     hipEvent_t counters_ready = nullptr;
     bool profiling = false;
     spk::OpStat stats[spk::kOpCount];
@@ -900,8 +929,15 @@ void build_index(sparkalog_hip_store* s, RuleRuntime& rt, RelationState& right, 
                                                                                           ix.starts.ptr, n_keys);
     ix.other.reserve(n ? n : 1);
     if (n) low_halves_kernel<<<grid_for(n), kBlock, 0, s->stream>>>(ix.sorted.ptr, n, ix.other.ptr);
+    s->host_counters();
+    uint32_t* degree_dev = reinterpret_cast<uint32_t*>(s->small16());
+    clear_words_kernel<<<1, 64, 0, s->stream>>>(degree_dev, 1);
+    max_degree_kernel<<<grid_for(n_keys), kBlock, 0, s->stream>>>(ix.starts.ptr, n_keys, degree_dev);
+    uint32_t& degree_host = *reinterpret_cast<uint32_t*>(&s->words_host[2]);
     check(hipGetLastError());
+    check(hipMemcpyAsync(&degree_host, degree_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
     check(hipStreamSynchronize(s->stream));
+    ix.max_degree = degree_host;
     ix.valid = true;
     ix.relation = relation;
     ix.version = right.version;
@@ -1775,7 +1811,9 @@ void session_step(sparkalog_hip_store* s) {
     const uint32_t n_x = vmax + 1;
     const bool use_x = s->x_anti && vmax < (1u << 24);
 
-    clear_words_kernel<<<1, 256, 0, s->stream>>>(s->misc_dev.ptr, 256);
+    // the step's status words are cleared by its first join count (or below
+    // when no rule has DELTA rows)
+    bool status_cleared = false;
     for (auto& t : targets) {
         t.fused = false;
         t.csr_run = false;
@@ -1800,15 +1838,26 @@ void session_step(sparkalog_hip_store* s) {
         const size_t n_tiles = tiles(L, kCountTile);
         rt.tile_base.reserve(n_tiles);
         const int ev = s->begin_op();
+        // tile-local offsets as u32 while every count tile's total fits
+        rt.narrow = static_cast<uint64_t>(rt.index.max_degree) * kCountTile < (1ull << 32);
+        const uint32_t clear_words = status_cleared ? 0u : 256u;
+        status_cleared = true;
         timer_start(s->kernel_timer(ev), s->stream);
-        dense_join_count_kernel<<<static_cast<unsigned>(n_tiles), kBlock, 0, s->stream>>>(
-            ds.delta.keys.ptr, L, rt.plan.left_key, rt.index.starts.ptr, rt.index.n_keys, rt.offsets.ptr,
-            rt.match.ptr, rt.tile_base.ptr);
+        if (rt.narrow)
+            dense_join_count_kernel<uint32_t><<<static_cast<unsigned>(n_tiles), kBlock, 0, s->stream>>>(
+                ds.delta.keys.ptr, L, rt.plan.left_key, rt.index.starts.ptr, rt.index.n_keys,
+                reinterpret_cast<uint32_t*>(rt.offsets.ptr), rt.match.ptr, rt.tile_base.ptr, s->misc_dev.ptr,
+                clear_words);
+        else
+            dense_join_count_kernel<uint64_t><<<static_cast<unsigned>(n_tiles), kBlock, 0, s->stream>>>(
+                ds.delta.keys.ptr, L, rt.plan.left_key, rt.index.starts.ptr, rt.index.n_keys, rt.offsets.ptr,
+                rt.match.ptr, rt.tile_base.ptr, s->misc_dev.ptr, clear_words);
         timer_stop(s->kernel_timer(ev), s->stream);
         join_scan_kernel<<<1, kScanThreads, 0, s->stream>>>(rt.tile_base.ptr, n_tiles, &cdev->join_total[r]);
         check(hipGetLastError());
         s->end_op(ev, kOpJoinCount, 8.0 * L);
     }
+    if (!status_cleared) clear_words_kernel<<<1, 256, 0, s->stream>>>(s->misc_dev.ptr, 256);
     check(hipMemcpyAsync(&chost, cdev, sizeof(Counters), hipMemcpyDeviceToHost, s->stream));
     check(hipEventRecord(s->counters_ready, s->stream));
     // the previous step's run merges: queued behind the join count (which
@@ -1849,14 +1898,22 @@ void session_step(sparkalog_hip_store* s) {
         const uint64_t emit_tiles = tiles(M, kEmitTile);
         const unsigned emit_grid = static_cast<unsigned>(emit_tiles < 65536 ? emit_tiles : 65536);
         rt.row_start.reserve(emit_tiles + 1);
-        emit_partition_kernel<<<grid_for(emit_tiles + 1), kBlock, 0, s->stream>>>(
-            rt.offsets.ptr, rt.tile_base.ptr, rt.delta_rows, M, rt.row_start.ptr, emit_tiles);
+        const uint32_t* local32 = reinterpret_cast<const uint32_t*>(rt.offsets.ptr);
+        if (rt.narrow)
+            emit_partition_kernel<uint32_t><<<grid_for(emit_tiles + 1), kBlock, 0, s->stream>>>(
+                local32, rt.tile_base.ptr, rt.delta_rows, M, rt.row_start.ptr, emit_tiles);
+        else
+            emit_partition_kernel<uint64_t><<<grid_for(emit_tiles + 1), kBlock, 0, s->stream>>>(
+                rt.offsets.ptr, rt.tile_base.ptr, rt.delta_rows, M, rt.row_start.ptr, emit_tiles);
         timer_start(s->kernel_timer(ev), s->stream);
-        dense_join_emit_kernel<<<emit_grid, kBlock, 0, s->stream>>>(ds.delta.keys.ptr, rt.delta_rows,
-                                                                     rt.index.other.ptr, rt.offsets.ptr,
-                                                                     rt.tile_base.ptr, rt.match.ptr, M,
-                                                                     rt.row_start.ptr, spec,
-                                                                     rt.cand_a.ptr);
+        if (rt.narrow)
+            dense_join_emit_kernel<uint32_t><<<emit_grid, kBlock, 0, s->stream>>>(
+                ds.delta.keys.ptr, rt.delta_rows, rt.index.other.ptr, local32, rt.tile_base.ptr, rt.match.ptr, M,
+                rt.row_start.ptr, spec, rt.cand_a.ptr);
+        else
+            dense_join_emit_kernel<uint64_t><<<emit_grid, kBlock, 0, s->stream>>>(
+                ds.delta.keys.ptr, rt.delta_rows, rt.index.other.ptr, rt.offsets.ptr, rt.tile_base.ptr,
+                rt.match.ptr, M, rt.row_start.ptr, spec, rt.cand_a.ptr);
         timer_stop(s->kernel_timer(ev), s->stream);
         check(hipGetLastError());
         s->end_op(ev, kOpJoinEmit, 8.0 * rt.index.rows + 8.0 * M);

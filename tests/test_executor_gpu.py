@@ -234,6 +234,22 @@ def test_high_degree_join_keys(stream):
     assert summ.iterations == rsum.iterations and summ.total_new_rows == rsum.total_new_rows
 
 
+def test_join_key_past_narrow_offsets(stream):
+    # a key with 2^21 + 8 matches: a count tile of 2,048 rows could then total
+    # 2^32 or more, so the join keeps its per-tile offsets as u64
+    n = (1 << 21) + 8
+    src = np.concatenate([np.zeros(n, np.uint32), np.array([n + 1, n + 2], np.uint32)])
+    dst = np.concatenate([np.arange(1, n + 1, dtype=np.uint32), np.array([0, n + 1], np.uint32)])
+    key = np.unique((src.astype(np.uint64) << 32) | dst)
+    edges = ((key >> 32).astype(np.uint32), (key & 0xFFFFFFFF).astype(np.uint32))
+    store, summ = _tc(edges, stream)
+    ref, rsum = _oracle_run({0: edges}, {1: (edges, edges)}, [rule(1, 1, 0)], [1])
+    full, expect = store.full(R(1)), ref.get(1)
+    assert full[0].size == 3 * n + 3
+    assert np.array_equal(full[0], expect[0]) and np.array_equal(full[1], expect[1])
+    assert summ.iterations == rsum.iterations and summ.total_new_rows == rsum.total_new_rows
+
+
 def test_ungrouped_projection_plan(stream):
     # output (edge.1, path.0): the distinct's major column is not DELTA's first
     # column, so candidates take the general radix-sort path
