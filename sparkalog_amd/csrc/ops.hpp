@@ -88,7 +88,9 @@ hipError_t unique_sorted(const uint64_t* keys, const uint64_t* rows_dev, size_t 
 // `meta` (segmented_distinct_meta_bytes(rows)) must persist between phases.
 // With `found` (probe_grouped's words over the same keys), keys whose found
 // bit is set are dropped first: the output is then (keys \ right inputs),
-// sorted and unique.
+// sorted and unique. With `xidx_out` (all entries 0xFFFFFFFF, from
+// xindex_clear), finish also marks the first row of each first value in the
+// output's first-column index (completed by xindex_fill).
 size_t segmented_distinct_meta_bytes(size_t rows);
 hipError_t segmented_distinct_prepare(const uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t* side,
                                       void* meta, size_t meta_bytes, uint64_t* side_rows, hipStream_t stream,

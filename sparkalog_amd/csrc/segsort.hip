@@ -72,22 +72,18 @@ SegLayout seg_layout(size_t rows) {
     return l;
 }
 
-__global__ void seg_init_kernel(SegHeader* h, uint64_t* status, size_t tiles) {
+// bounds[t] = start of the run containing position t*kSegTile (or rows);
+// the first thread also resets the header (the per-tile counts need no
+// reset: every tile writes its own).
+__global__ void seg_partition_kernel(const uint64_t* keys, size_t rows, uint32_t minor_bits, size_t* bounds,
+                                     size_t tiles, SegHeader* h) {
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
-    const size_t gtid = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (gtid == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         h->error = 0;
         h->counter = 0;
         h->side_rows = 0;
         h->side_unique = 0;
     }
-    for (size_t i = gtid; i < tiles; i += stride) status[i] = 0;
-}
-
-// bounds[t] = start of the run containing position t*kSegTile (or rows).
-__global__ void seg_partition_kernel(const uint64_t* keys, size_t rows, uint32_t minor_bits, size_t* bounds,
-                                     size_t tiles) {
-    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
     for (size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; t <= tiles; t += stride) {
         const size_t p = t * kSegTile;
         if (p >= rows) {
@@ -577,11 +573,9 @@ hipError_t segmented_distinct_prepare(const uint64_t* keys, size_t rows, uint32_
     if (meta_bytes < l.total || minor_bits == 0 || minor_bits > 32) return hipErrorInvalidValue;
     char* base = static_cast<char*>(meta);
     SegHeader* h = reinterpret_cast<SegHeader*>(base);
-    uint64_t* status = reinterpret_cast<uint64_t*>(base + l.status_off);
     size_t* bounds = reinterpret_cast<size_t*>(base + l.bounds_off);
     const unsigned small_grid = static_cast<unsigned>(l.tiles / kBlock + 1 < 4096 ? l.tiles / kBlock + 1 : 4096);
-    seg_init_kernel<<<small_grid, kBlock, 0, stream>>>(h, status, l.tiles);
-    seg_partition_kernel<<<small_grid, kBlock, 0, stream>>>(keys, rows, minor_bits, bounds, l.tiles);
+    seg_partition_kernel<<<small_grid, kBlock, 0, stream>>>(keys, rows, minor_bits, bounds, l.tiles, h);
     seg_gather_kernel<<<static_cast<unsigned>(l.tiles < 1024 ? l.tiles : 1024), kBlock, 0, stream>>>(
         keys, bounds, l.tiles, side, h, found);
     hipError_t st = hipGetLastError();

@@ -918,18 +918,28 @@ __global__ void xindex_clear_kernel(uint32_t* xstart, size_t n) {
         xstart[i] = 0xFFFFFFFFu;
 }
 
-// pass 1: entry n_x = the row count; each chunk's minimum
+// pass 1: entry n_x = the row count; each chunk's minimum. Loads are
+// clamped into the index and all issued before any is used.
 __global__ __launch_bounds__(kBlock) void xfill_chunk_min_kernel(uint32_t* xstart, uint32_t n_x,
                                                                  const uint64_t* rows_dev, uint32_t* chunk_min) {
+    constexpr uint32_t kPer = kFillChunk / kBlock;  // entries per thread, strided by the block
     __shared__ uint32_t red[kWavesPerBlock];
     const size_t b0 = static_cast<size_t>(blockIdx.x) * kFillChunk;
     const size_t n = static_cast<size_t>(n_x) + 1;
-    const size_t b1 = b0 + kFillChunk < n ? b0 + kFillChunk : n;
-    if (b1 == n && threadIdx.x == 0) xstart[n_x] = static_cast<uint32_t>(*rows_dev);
+    const uint32_t total = static_cast<uint32_t>(*rows_dev);
+    if (b0 + kFillChunk >= n && threadIdx.x == 0) xstart[n_x] = total;
+    uint32_t v[kPer];
+#pragma unroll
+    for (uint32_t q = 0; q < kPer; ++q) {
+        const size_t i = b0 + q * kBlock + threadIdx.x;
+        v[q] = xstart[i < n_x ? i : n_x - 1];  // n_x >= 1
+    }
     uint32_t m = 0xFFFFFFFFu;
-    for (size_t i = b0 + threadIdx.x; i < b1; i += kBlock) {
-        const uint32_t v = i == n_x ? static_cast<uint32_t>(*rows_dev) : xstart[i];
-        m = v < m ? v : m;
+#pragma unroll
+    for (uint32_t q = 0; q < kPer; ++q) {
+        const size_t i = b0 + q * kBlock + threadIdx.x;
+        const uint32_t x = i < n_x ? v[q] : (i == n_x ? total : 0xFFFFFFFFu);
+        m = x < m ? x : m;
     }
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) {
@@ -944,14 +954,17 @@ __global__ __launch_bounds__(kBlock) void xfill_chunk_min_kernel(uint32_t* xstar
     }
 }
 
-// pass 2: every entry = the minimum of itself and every entry after it
+// pass 2: every entry = the minimum of itself and every entry after it.
+// Thread t owns kPer consecutive entries; the minimum over later threads
+// comes from a wave suffix scan and one value per later wave.
 __global__ __launch_bounds__(kBlock) void xfill_apply_kernel(uint32_t* xstart, uint32_t n_x, const uint32_t* chunk_min,
                                                              uint32_t chunks) {
     constexpr uint32_t kPer = kFillChunk / kBlock;  // 16 consecutive entries per thread
-    __shared__ uint32_t thread_min[kBlock];
+    __shared__ uint32_t wave_min[kWavesPerBlock];
     __shared__ uint32_t carry_slot;
     const size_t b0 = static_cast<size_t>(blockIdx.x) * kFillChunk;
     const size_t n = static_cast<size_t>(n_x) + 1;
+    const uint32_t lane = lane_id(), wave = wave_id();
     // carry: the minimum of every later chunk
     uint32_t c = 0xFFFFFFFFu;
     for (uint32_t j = blockIdx.x + 1 + threadIdx.x; j < chunks; j += kBlock) c = chunk_min[j] < c ? chunk_min[j] : c;
@@ -962,21 +975,36 @@ __global__ __launch_bounds__(kBlock) void xfill_apply_kernel(uint32_t* xstart, u
     }
     if (threadIdx.x == 0) carry_slot = 0xFFFFFFFFu;
     __syncthreads();
-    if (lane_id() == 0) atomicMin(&carry_slot, c);
-    // this thread's entries, their suffix minimum within the thread
+    if (lane == 0) atomicMin(&carry_slot, c);
+    // this thread's entries (loads clamped, all issued first), their suffix
+    // minimum within the thread
     uint32_t v[kPer];
+#pragma unroll
+    for (uint32_t q = 0; q < kPer; ++q) {
+        const size_t i = b0 + threadIdx.x * kPer + q;
+        v[q] = xstart[i < n ? i : n - 1];
+    }
     uint32_t m = 0xFFFFFFFFu;
 #pragma unroll
     for (int q = kPer - 1; q >= 0; --q) {
         const size_t i = b0 + threadIdx.x * kPer + q;
-        v[q] = i < n ? xstart[i] : 0xFFFFFFFFu;
-        m = v[q] < m ? v[q] : m;
+        const uint32_t x = i < n ? v[q] : 0xFFFFFFFFu;
+        m = x < m ? x : m;
         v[q] = m;
     }
-    thread_min[threadIdx.x] = m;
+    // minimum over this and later lanes of the wave
+    uint32_t incl = m;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t o = __shfl_down(incl, off, kWave);
+        if (lane + off < static_cast<uint32_t>(kWave)) incl = o < incl ? o : incl;
+    }
+    if (lane == 0) wave_min[wave] = incl;
+    uint32_t after = __shfl_down(incl, 1, kWave);  // later lanes of this wave
+    if (lane == kWave - 1) after = 0xFFFFFFFFu;
     __syncthreads();
-    uint32_t after = carry_slot;  // entries of later threads and chunks
-    for (uint32_t t = threadIdx.x + 1; t < kBlock; ++t) after = thread_min[t] < after ? thread_min[t] : after;
+    for (int w = static_cast<int>(wave) + 1; w < kWavesPerBlock; ++w) after = wave_min[w] < after ? wave_min[w] : after;
+    after = carry_slot < after ? carry_slot : after;
 #pragma unroll
     for (int q = 0; q < static_cast<int>(kPer); ++q) {
         const size_t i = b0 + threadIdx.x * kPer + q;
