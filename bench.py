@@ -37,12 +37,12 @@ def parse_args():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--nodes", type=int, default=1 << 20)
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--merge-ratio", type=int, default=4, help="tc: FULL run-merge ratio (results are identical)")
+    p.add_argument("--merge-ratio", type=int, default=3, help="tc: FULL run-merge ratio (results are identical)")
     p.add_argument("--host-io", action="store_true",
                    help="tc: also time one fixpoint from host arrays to the closure back in host memory (PCIe)")
     p.add_argument("--indexed-anti-join", action="store_true",
                    help="tc: anti-join FULL through its first-column index (results are identical)")
-    p.add_argument("--sparse-probe-rows", type=int, default=1 << 21,
+    p.add_argument("--sparse-probe-rows", type=int, default=1 << 22,
                    help="tc: candidate count up to which FULL is probed one candidate per thread")
     p.add_argument("--cpu-sample-nodes", type=int, default=1 << 16)
     p.add_argument("--mode", choices=["tc", "ops", "batched", "served", "copy"], default="tc",

@@ -239,12 +239,12 @@ int sparkalog_hip_store_set_profiling(sparkalog_hip_store* store, int enabled);
 /* Executor tuning (no counterpart in the reference, whose union rewrites all
  * of FULL each iteration): FULL is kept as size-tiered sorted runs, and the
  * newest run is merged into the one before while that one holds fewer than
- * merge_ratio (>= 2, default 4) times its rows; segmented_distinct (default
+ * merge_ratio (>= 2, default 3) times its rows; segmented_distinct (default
  * 1) sorts candidates grouped by their first column per group in LDS instead
  * of a global radix sort; indexed_anti_join (default 0) gives FULL's runs a
  * first-column index (while first values are below 2^24) so the anti-join
  * reads only the rows of first values it holds; iterations with at most
- * sparse_probe_rows candidates (default 2^21) check them against FULL one
+ * sparse_probe_rows candidates (default 2^22) check them against FULL one
  * thread each through that index. Results do not depend on any of them. */
 int sparkalog_hip_store_set_tuning(sparkalog_hip_store* store, uint32_t merge_ratio, int segmented_distinct,
                                    int indexed_anti_join, uint64_t sparse_probe_rows);

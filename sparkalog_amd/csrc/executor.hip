@@ -188,7 +188,7 @@ const char* const kOpKernels[kOpCount] = {"dense_join_count_kernel", "dense_join
 // Candidates up to this many (by default) are probed one thread each
 // through the first-column index of FULL's runs (few candidates, long FULL
 // ranges).
-constexpr uint64_t kSparseProbeRows = 1ull << 21;
+constexpr uint64_t kSparseProbeRows = 1ull << 22;
 
 // Events of one profiled operator launch.
 struct OpEvents {
@@ -640,7 +640,7 @@ struct sparkalog_hip_store {
     size_t events_used = 0;
     uint64_t generation = 0;                    // bumped by every insert; versions derive from it
     std::vector<spk::DevBuf<uint64_t>> pool;    // released run buffers, reused by later merges
-    uint32_t merge_ratio = 4;                   // merge two runs when the older is < ratio x the newer
+    uint32_t merge_ratio = 3;                   // merge two runs when the older is < ratio x the newer
     void* session = nullptr;                    // spk::Session of the running fixpoint
     bool segmented_distinct = true;             // sort grouped candidates per run in LDS
 

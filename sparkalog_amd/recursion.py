@@ -192,8 +192,8 @@ class RelationStore:
         _check(self._lib.sparkalog_hip_store_remove_ranges(self._h, int(id), b.ctypes.data_as(_native.u32p),
                                                            b.size // 2))
 
-    def set_tuning(self, merge_ratio: int = 4, segmented_distinct: bool = True, indexed_anti_join: bool = False,
-                   sparse_probe_rows: int = 1 << 21) -> None:
+    def set_tuning(self, merge_ratio: int = 3, segmented_distinct: bool = True, indexed_anti_join: bool = False,
+                   sparse_probe_rows: int = 1 << 22) -> None:
         """Run-merge ratio of FULL's sorted runs, the grouped (per-run LDS)
         distinct, the first-column-indexed anti-join and the candidate count
         up to which FULL is probed one candidate per thread; results are
