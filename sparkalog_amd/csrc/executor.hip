@@ -369,15 +369,30 @@ __global__ void emit_partition_kernel(const O* local, const uint64_t* base, size
             continue;
         }
         const uint64_t o = static_cast<uint64_t>(t) * kEmitTile;
-        size_t lo = 0, hi = n;
+        // the last count tile whose first row starts at or before o (base[0]
+        // is 0), then the last row of it that does: two short searches, the
+        // first over a few thousand bases that stay in cache
+        const size_t n_ct = (n + kCountTile - 1) / kCountTile;
+        size_t lo = 0, hi = n_ct;
         while (lo < hi) {
             const size_t mid = (lo + hi) >> 1;
-            if (join_offset(local, base, mid) <= o)
+            if (base[mid] <= o)
                 lo = mid + 1;
             else
                 hi = mid;
         }
-        row_start[t] = lo - 1;
+        const size_t ct = lo - 1;
+        const uint64_t b = base[ct];
+        // a count tile's first row has local offset 0
+        size_t a = ct * kCountTile + 1, e = ct * kCountTile + kCountTile < n ? ct * kCountTile + kCountTile : n;
+        while (a < e) {
+            const size_t mid = (a + e) >> 1;
+            if (b + static_cast<uint64_t>(local[mid]) <= o)
+                a = mid + 1;
+            else
+                e = mid;
+        }
+        row_start[t] = a - 1;
     }
     (void)total;
 }
