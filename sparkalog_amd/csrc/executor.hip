@@ -409,7 +409,7 @@ __global__ __launch_bounds__(kBlock) void dense_join_emit_kernel(const uint64_t*
                                                                  size_t total,
                                                                  const size_t* row_start, EmitSpec spec,
                                                                  uint64_t* out) {
-    constexpr int kPer = kEmitTile / kBlock;  // 8 output positions per thread
+    constexpr int kPer = kEmitTile / kBlock;  // 4 output positions per thread
     // 18 KB: 8 blocks a CU. A row's first output is kept relative to the
     // tile's first (u32, wrapping: differences are exact below 2^32, and a
     // row has fewer outputs than that); a position's row fits 16 bits.
@@ -599,7 +599,7 @@ struct RuleRuntime {
     DevBuf<uint64_t> found;     // candidates already in FULL (one bit each)
     DevBuf<size_t> row_start;   // first delta row of each emit tile
     DevBuf<uint64_t> cand_a, cand_b, cand_c;
-    DevBuf<unsigned char> seg_meta;  // tile bounds and look-back words of the segmented distinct
+    DevBuf<unsigned char> seg_meta;  // header, tile counts and bounds of the segmented distinct
     DevBuf<uint64_t> distinct_out;
     uint64_t join_rows = 0;     // M (host, after the first sync)
     int distinct_slot = -1;     // profiling event of this iteration's distinct
@@ -641,7 +641,7 @@ struct sparkalog_hip_store {
     std::vector<spk::RelationState> rel;
     spk::DevBuf<unsigned char> temp;
     spk::DevBuf<uint64_t> counters_dev;   // Counters on device
-    spk::DevBuf<uint32_t> misc_dev;       // lookback counters / error words
+    spk::DevBuf<uint32_t> misc_dev;       // small counters: upload maxima, digests, the step's error word
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
     // pinned host copies of the step's counters (read back once per phase)
