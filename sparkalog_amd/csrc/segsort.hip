@@ -10,11 +10,11 @@
 //      that contains them (one thread per boundary), so no run crosses a tile;
 //   2. runs longer than a tile go, gathered, through the global onesweep sort
 //      (rare: a run is one source's candidates in one iteration);
-//   3. every tile sorts its keys in LDS by (local run rank, minor) with stable
-//      LSD passes in 9-bit digits over the bit range where its keys differ
-//      (up to 27 varying bits take three passes), drops duplicates, and
-//      writes the expanded pairs (major << 32 | minor) back over its own
-//      input slice, with its unique count;
+//   3. every tile sorts its keys in LDS by (local run rank, minor) with LSD
+//      passes over the bit range where its keys differ (a 12-bit digit, then
+//      stable 9-bit digits: up to 30 varying bits take three passes), drops
+//      duplicates, and writes the expanded pairs (major << 32 | minor) back
+//      over its own input slice, with its unique count;
 //   4. a one-block scan turns the counts into offsets, and a copy moves each
 //      tile's rows to its offset. A single pass with decoupled look-back
 //      kept tiles waiting for slower predecessors for about 38% of their
@@ -33,6 +33,8 @@ constexpr int kSegItems = kSegCap / kSegBlock;  // 8 keys per thread
 constexpr int kSegWaveRows = kSegItems * kWave;
 constexpr int kSegDigitBits = 9;              // digit of a tile pass
 constexpr int kSegRadix = 1 << kSegDigitBits;
+constexpr int kSegFirstBits = 12;             // digit of the first (unstable) pass
+constexpr int kSegFirstRadix = 1 << kSegFirstBits;
 
 __device__ __forceinline__ void store_pair(const PairOutput& out, size_t i, uint64_t k) {
     if (out.packed) out.packed[i] = k;
@@ -184,13 +186,14 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) 
     // space idle while they are used: per-wave totals and the OR/AND of the
     // keys live in wave_count outside the passes, the scan's scratch in
     // tile_keys during a pass's digit scan (the keys are in registers then).
-    __shared__ uint16_t wave_count[kSegWaves][kSegRadix];
+    __shared__ __attribute__((aligned(16))) uint16_t wave_count[kSegWaves][kSegRadix];
     uint32_t* const wave_tot = reinterpret_cast<uint32_t*>(&wave_count[0][0]);  // kept keys, run starts, unique keys
     K* const red_or = reinterpret_cast<K*>(&wave_count[0][0]) + 2 * kSegWaves;
     K* const red_and = red_or + kSegWaves;
     uint32_t* const scan_lds = reinterpret_cast<uint32_t*>(tile_keys);
     static_assert(sizeof(uint32_t) * kSegWaves + 2 * sizeof(K) * kSegWaves + 2 * sizeof(K) * kSegWaves <=
                       sizeof(uint16_t) * kSegWaves * kSegRadix, "aliases fit");
+    static_assert(sizeof(uint16_t) * kSegFirstRadix == sizeof(wave_count), "first-pass counts fit");
     // tiles are independent (no look-back), so dispatch order needs no ticket
     const uint32_t tile = blockIdx.x;
     const size_t tiles = seg_tiles(rows);
@@ -352,39 +355,74 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) 
             }
             vary = n ? (o ^ a) : 0;
         }
-        // stable LSD passes in LDS over the bit range [lb, lb + width) where the
-        // keys differ, in 9-bit digits (a last digit reaching past the range
-        // only adds bits that are equal in every key of the tile)
+        // LSD passes in LDS over the bit range [lb, lb + width) where the keys
+        // differ (a last digit reaching past the range only adds bits that
+        // are equal in every key of the tile). The first pass need not be
+        // stable (the order it starts from carries no information): it ranks
+        // with block-wide LDS atomics on a 12-bit digit, whose 4,096 block
+        // counts take the same 8 KB as the later passes' per-wave counts; the
+        // later passes are stable, on 9-bit digits. Up to 21 varying bits
+        // take two passes, up to 30 three.
         int lb = 0, width = 0;
         if (vary) {
             lb = __builtin_ctzll(static_cast<uint64_t>(vary));
             width = 64 - __builtin_clzll(static_cast<uint64_t>(vary)) - lb;
         }
-        const int passes = (width + kSegDigitBits - 1) / kSegDigitBits;
+        const int passes =
+            width == 0 ? 0
+                       : 1 + (width > kSegFirstBits ? (width - kSegFirstBits + kSegDigitBits - 1) / kSegDigitBits : 0);
         __syncthreads();  // red_or / red_and (in wave_count) read by every thread
+        uint32_t* const words = reinterpret_cast<uint32_t*>(&wave_count[0][0]);  // two u16 counts a word
         for (int pass = 0; pass < passes; ++pass) {  // block-uniform
-            const int shift = lb + pass * kSegDigitBits;
-            for (int i = threadIdx.x; i < kSegWaves * kSegRadix / 2; i += kSegBlock)
-                reinterpret_cast<uint32_t*>(&wave_count[0][0])[i] = 0;
+            for (int i = threadIdx.x; i < kSegWaves * kSegRadix / 2; i += kSegBlock) words[i] = 0;
             __syncthreads();
             uint32_t ranks[kSegItems];
             if (pass == 0) {
-                // the first pass need not be stable (the order it starts from
-                // carries no information): a rank within (wave, digit) from an
-                // LDS atomic replaces the ballot match
+                // rank within the digit over the whole block, from an LDS atomic
 #pragma unroll
                 for (int i = 0; i < kSegItems; ++i) {
                     if (i >= items) continue;
                     const uint32_t p = wave * wave_len + i * kWave + lane;
-                    const uint32_t digit = static_cast<uint32_t>(k[i] >> shift) & (kSegRadix - 1);
+                    const uint32_t digit = static_cast<uint32_t>(k[i] >> lb) & (kSegFirstRadix - 1);
                     if (p < n) {
-                        // two u16 counts per u32 word
-                        uint32_t* const word = reinterpret_cast<uint32_t*>(&wave_count[wave][0]) + (digit >> 1);
                         const uint32_t sh = 16u * (digit & 1u);
-                        ranks[i] = (atomicAdd(word, 1u << sh) >> sh) & 0xFFFFu;
+                        ranks[i] = (atomicAdd(&words[digit >> 1], 1u << sh) >> sh) & 0xFFFFu;
+                    }
+                }
+                __syncthreads();
+                // thread t owns digits [8t, 8t + 8): their counts become
+                // exclusive starts in place (every count and start is at most
+                // kSegCap, so the u16 halves never carry into each other)
+                static_assert(kSegFirstRadix == 8 * kSegBlock, "eight digits per thread");
+                uint4 q = reinterpret_cast<uint4*>(words)[threadIdx.x];
+                const uint32_t pairs = q.x + q.y + q.z + q.w;
+                uint32_t total_unused;
+                uint32_t run = block_exclusive_sum<uint32_t, kSegWaves>((pairs & 0xFFFFu) + (pairs >> 16),
+                                                                        &total_unused, scan_lds);
+                const auto to_starts = [&run](uint32_t w) {
+                    const uint32_t lo = w & 0xFFFFu;
+                    const uint32_t out = run | ((run + lo) << 16);
+                    run += lo + (w >> 16);
+                    return out;
+                };
+                q.x = to_starts(q.x);
+                q.y = to_starts(q.y);
+                q.z = to_starts(q.z);
+                q.w = to_starts(q.w);
+                reinterpret_cast<uint4*>(words)[threadIdx.x] = q;
+                __syncthreads();
+#pragma unroll
+                for (int i = 0; i < kSegItems; ++i) {
+                    if (i >= items) continue;
+                    const uint32_t p = wave * wave_len + i * kWave + lane;
+                    if (p < n) {
+                        const uint32_t digit = static_cast<uint32_t>(k[i] >> lb) & (kSegFirstRadix - 1);
+                        const uint32_t at = (words[digit >> 1] >> (16u * (digit & 1u))) & 0xFFFFu;
+                        tile_keys[at + ranks[i]] = k[i];
                     }
                 }
             } else {
+                const int shift = lb + kSegFirstBits + (pass - 1) * kSegDigitBits;
 #pragma unroll
                 for (int i = 0; i < kSegItems; ++i) {
                     if (i >= items) continue;
@@ -398,35 +436,35 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) 
                     if (valid && below == 0)
                         wave_count[wave][digit] = static_cast<uint16_t>(before + static_cast<uint32_t>(__popcll(peers)));
                 }
-            }
-            __syncthreads();
-            // digit d's start per wave: thread d owns digit d
-            static_assert(kSegRadix == kSegBlock, "one digit per thread");
-            const uint32_t d = threadIdx.x;
-            uint32_t tc = 0;
-            {
+                __syncthreads();
+                // digit d's start per wave: thread d owns digit d
+                static_assert(kSegRadix == kSegBlock, "one digit per thread");
+                const uint32_t d = threadIdx.x;
+                uint32_t tc = 0;
+                {
 #pragma unroll
-                for (int w = 0; w < kSegWaves; ++w) {
-                    const uint32_t c = wave_count[w][d];
-                    wave_count[w][d] = static_cast<uint16_t>(tc);
-                    tc += c;
+                    for (int w = 0; w < kSegWaves; ++w) {
+                        const uint32_t c = wave_count[w][d];
+                        wave_count[w][d] = static_cast<uint16_t>(tc);
+                        tc += c;
+                    }
                 }
-            }
-            uint32_t total_unused;
-            const uint32_t local_start = block_exclusive_sum<uint32_t, kSegWaves>(tc, &total_unused, scan_lds);
-            {
+                uint32_t total_unused;
+                const uint32_t local_start = block_exclusive_sum<uint32_t, kSegWaves>(tc, &total_unused, scan_lds);
+                {
 #pragma unroll
-                for (int w = 0; w < kSegWaves; ++w)
-                    wave_count[w][d] = static_cast<uint16_t>(wave_count[w][d] + local_start);
-            }
-            __syncthreads();
+                    for (int w = 0; w < kSegWaves; ++w)
+                        wave_count[w][d] = static_cast<uint16_t>(wave_count[w][d] + local_start);
+                }
+                __syncthreads();
 #pragma unroll
-            for (int i = 0; i < kSegItems; ++i) {
-                if (i >= items) continue;
-                const uint32_t p = wave * wave_len + i * kWave + lane;
-                if (p < n) {
-                    const uint32_t digit = static_cast<uint32_t>(k[i] >> shift) & (kSegRadix - 1);
-                    tile_keys[wave_count[wave][digit] + ranks[i]] = k[i];
+                for (int i = 0; i < kSegItems; ++i) {
+                    if (i >= items) continue;
+                    const uint32_t p = wave * wave_len + i * kWave + lane;
+                    if (p < n) {
+                        const uint32_t digit = static_cast<uint32_t>(k[i] >> shift) & (kSegRadix - 1);
+                        tile_keys[wave_count[wave][digit] + ranks[i]] = k[i];
+                    }
                 }
             }
             __syncthreads();
