@@ -902,10 +902,13 @@ __global__ __launch_bounds__(kBlock) void run_merge_kernel(const uint32_t* xa, c
     for (int i = 0; i < kRItems; ++i)
         if (diag + i < n) lds[diag + i] = v[i];
     __syncthreads();
+    // the merged run is read again only in later steps: streaming stores
+    // (nontemporal) leave L2 to the index entries
     if (out_second)
-        for (uint32_t i = threadIdx.x; i < n; i += kBlock) out_second[d0 + i] = static_cast<uint32_t>(lds[i]);
+        for (uint32_t i = threadIdx.x; i < n; i += kBlock)
+            __builtin_nontemporal_store(static_cast<uint32_t>(lds[i]), &out_second[d0 + i]);
     if (out_packed)
-        for (uint32_t i = threadIdx.x; i < n; i += kBlock) out_packed[d0 + i] = lds[i];
+        for (uint32_t i = threadIdx.x; i < n; i += kBlock) __builtin_nontemporal_store(lds[i], &out_packed[d0 + i]);
 }
 
 __global__ void xsec_init_kernel(uint32_t* deferred) { deferred[0] = 0; }
