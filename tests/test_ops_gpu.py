@@ -195,6 +195,28 @@ def test_join_zipf_keys(stream):
     _join_case((left[1], left[0]), right, 1, 0, ((0, 0), (1, 1)), stream)
 
 
+@pytest.mark.parametrize("spread", ["uniform", "clustered"])
+def test_join_long_key_column(spread, stream):
+    """Index key columns far longer than the join's 2,048-entry LDS sample:
+    keys uniform over 2^32 (the interpolated window holds the key) and keys
+    in four tight clusters (the window misses and the whole bracket is
+    searched); probes hit, miss by one, miss at random and sit at 0 and
+    2^32 - 1."""
+    rng = np.random.default_rng(700)
+    n = 300_000
+    if spread == "uniform":
+        rk = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+    else:
+        rk = (rng.integers(0, 4, n, dtype=np.uint32) * np.uint32(1_000_000_000)
+              + rng.integers(0, 400_000, n, dtype=np.uint32)).astype(np.uint32)
+    right = (rk, rng.integers(0, 1 << 31, n, dtype=np.uint32))
+    lk = np.concatenate([rk[rng.integers(0, n, 150_000)], (rk[:50_000].astype(np.uint64) + 1).astype(np.uint32),
+                         rng.integers(0, 1 << 32, 50_000, dtype=np.uint64).astype(np.uint32),
+                         np.array([0, 0xFFFFFFFF], np.uint32)])
+    left = (rng.integers(0, 1 << 31, lk.size, dtype=np.uint32), lk)
+    _join_case(left, right, 1, 0, ((0, 0), (1, 1)), stream)
+
+
 @pytest.mark.parametrize("n", [1, 4095, 4096, 4097, 3_000_001])
 def test_filter_random(n, stream):
     rng = np.random.default_rng(n)
