@@ -83,6 +83,24 @@ sparkalog_hip_error sparkalog_hip_join_u32_count(const uint32_t* left_keys, size
                                                  uint64_t* total, void* temporary, size_t temporary_bytes,
                                                  void* stream);
 
+/* No counterpart in the reference: sparkalog_hip_join_u32_count for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). Same arguments and results
+ * as sparkalog_hip_join_u32_count without `temporary`: its scratch (look-back
+ * header and status words) is a static per-device array of
+ * SPARKALOG_HIP_NO_ALLOC_SCRATCH_BYTES, so it allocates nothing (no hipMalloc,
+ * hipFree or allocating library call) and issues no hipMemsetAsync. At most
+ * SPARKALOG_HIP_NO_ALLOC_MAX_LEFT_ROWS left rows; larger inputs return
+ * hipErrorInvalidValue before any device work. Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (other streams, other
+ * threads) race and must be serialized by the caller. */
+#define SPARKALOG_HIP_NO_ALLOC_SCRATCH_BYTES (1u << 20)
+#define SPARKALOG_HIP_NO_ALLOC_MAX_LEFT_ROWS 268369920u /* (1 MiB - 256 B) / 8 B per 2,048-row tile */
+sparkalog_hip_error sparkalog_hip_join_u32_count_no_alloc(const uint32_t* left_keys, size_t left_rows,
+                                                          const uint32_t* index_keys, const uint32_t* index_starts,
+                                                          size_t unique_keys, uint64_t* counts, uint64_t* offsets,
+                                                          uint64_t* total, void* stream);
+
 /* column_ops.cu:401-440: writes output rows offsets[l] .. offsets[l]+counts[l]
  * of left row l in index order; side 0 takes projection[l], side 1 takes
  * projection[index_rows[match]]. */

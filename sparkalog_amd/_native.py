@@ -50,6 +50,7 @@ _HIP_SIGNATURES = {
     "sparkalog_hip_copy_u64": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
     "sparkalog_hip_join_u32_temporary_bytes": (c_int, [c_void_p, c_void_p, c_size_t, POINTER(c_size_t), c_void_p]),
     "sparkalog_hip_join_u32_count": (c_int, [c_void_p, c_size_t, c_void_p, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "sparkalog_hip_join_u32_count_no_alloc": (c_int, [c_void_p, c_size_t, c_void_p, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p, c_void_p]),
     "sparkalog_hip_join_u32_emit": (c_int, [c_void_p, c_size_t, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_uint32, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p, c_void_p]),
     "sparkalog_hip_distinct_u32_temporary_bytes": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, POINTER(c_size_t), c_void_p]),
     "sparkalog_hip_distinct_u32": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),

@@ -2,8 +2,18 @@
 // validation and empty-input behaviour follow the reference's entry points
 // in crates/sparkalog-execution/native/column_ops.cu:231-775; the work is
 // done by the spk:: launchers in ops.hip.
+#include <mutex>
+#include <vector>
+
 #include "ops.hpp"
 #include "sparkalog_hip.h"
+
+// Scratch of sparkalog_hip_join_u32_count_no_alloc: the join count's
+// look-back header and one status word per 2,048 left rows. Each device
+// holds its own copy until the process ends. Host code must never pass
+// `scratch` itself to a kernel (it names a host-side stand-in); it uses the
+// device address that hipGetSymbolAddress returned for the current device.
+static __device__ __align__(256) unsigned char scratch[SPARKALOG_HIP_NO_ALLOC_SCRATCH_BYTES];
 
 namespace {
 
@@ -13,6 +23,28 @@ inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 // hipMemsetAsync, so that callers may capture these calls in a HIP graph.
 hipError_t zero_u32(uint32_t* p, void* stream) { return spk::set_u32(p, 0, as_stream(stream)); }
 hipError_t zero_u64(uint64_t* p, void* stream) { return spk::set_u64(p, 0, as_stream(stream)); }
+
+// Device address of `scratch` per device, looked up once per device (with
+// that device current) on the first no-alloc call made there.
+std::mutex g_scratch_mutex;
+std::vector<void*> g_scratch_ptr;
+
+hipError_t scratch_of_current_device(void** out) {
+    int device = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (static_cast<size_t>(device) >= g_scratch_ptr.size()) g_scratch_ptr.resize(static_cast<size_t>(device) + 1, nullptr);
+    if (g_scratch_ptr[device] == nullptr) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (p == nullptr) return hipErrorInvalidSymbol;
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
 
 }  // namespace
 
@@ -85,6 +117,29 @@ extern "C" sparkalog_hip_error sparkalog_hip_join_u32_count(const uint32_t* left
         return hipErrorInvalidValue;
     return spk::join_count(left_keys, left_rows, index_keys, index_starts, unique_keys, counts, offsets, total,
                            temporary, temporary_bytes, as_stream(stream));
+}
+
+extern "C" sparkalog_hip_error sparkalog_hip_join_u32_count_no_alloc(const uint32_t* left_keys, size_t left_rows,
+                                                                     const uint32_t* index_keys,
+                                                                     const uint32_t* index_starts, size_t unique_keys,
+                                                                     uint64_t* counts, uint64_t* offsets,
+                                                                     uint64_t* total, void* stream) {
+    if (total == nullptr) return hipErrorInvalidValue;
+    // sizes past the static scratch are refused before any device work
+    if (left_rows > SPARKALOG_HIP_NO_ALLOC_MAX_LEFT_ROWS) return hipErrorInvalidValue;
+    const size_t need = spk::join_temp_bytes(left_rows);
+    if (need > SPARKALOG_HIP_NO_ALLOC_SCRATCH_BYTES) return hipErrorInvalidValue;
+    if (left_rows == 0) return zero_u64(total, stream);
+    if (left_keys == nullptr || index_keys == nullptr || index_starts == nullptr || counts == nullptr ||
+        offsets == nullptr)
+        return hipErrorInvalidValue;
+    void* temp = nullptr;
+    const hipError_t e = scratch_of_current_device(&temp);
+    if (e != hipSuccess) return e;
+    // join_count clears the header and the status words it uses with kernels
+    // (no allocation, no hipMemsetAsync)
+    return spk::join_count(left_keys, left_rows, index_keys, index_starts, unique_keys, counts, offsets, total, temp,
+                           need, as_stream(stream));
 }
 
 extern "C" sparkalog_hip_error sparkalog_hip_join_u32_emit(const uint32_t* left_keys, size_t left_rows,

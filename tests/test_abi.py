@@ -49,6 +49,11 @@ def test_host_side_argument_validation_needs_no_gpu():
     assert lib.sparkalog_hip_filter_u32_temporary_bytes(None, None, None, 10, ctypes.byref(size), None) != 0
     assert lib.sparkalog_hip_distinct_u32(None, None, 4, None, None, None, None, None, None, 0, None) != 0
     assert lib.sparkalog_hip_fill_mod_u32(None, 4, 0, None) != 0
+    # the no-alloc join count refuses inputs past its static scratch first
+    dummy_total = ctypes.c_void_p(16)
+    assert lib.sparkalog_hip_join_u32_count_no_alloc(None, 268369921, None, None, 0, None, None, dummy_total,
+                                                     None) != 0
+    assert lib.sparkalog_hip_join_u32_count_no_alloc(None, 8, None, None, 0, None, None, None, None) != 0
     # temporary-size queries are pure host arithmetic
     dummy = ctypes.c_void_p(16)
     assert lib.sparkalog_hip_union_u32_temporary_bytes(None, 100, None, 50, None, None, dummy, ctypes.byref(size), None) == 0
