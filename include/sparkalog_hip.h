@@ -14,7 +14,12 @@
  *   - an empty input writes a zero count on the stream and succeeds;
  *   - a null pointer with a non-zero length is hipErrorInvalidValue;
  *   - counts are written on the device (u32 for filter and anti-join, u64 for
- *     distinct, union and the join total).
+ *     distinct, union and the join total);
+ *   - single-pass kernels find their output offsets by waiting on earlier
+ *     tiles with a bounded wait; if such a wait ever gives up (a scheduling
+ *     failure of the device, never seen in practice), the call's count is
+ *     written as all ones (UINT32_MAX / UINT64_MAX) after its kernels, so
+ *     the caller can tell it from a result.
  * Scratch arrays the reference needed for its CUB pipeline (filter flags,
  * anti-join flags/selection, union packed copies, distinct packed/scratch)
  * keep their parameters; the kernels here use them only where noted.
@@ -266,6 +271,16 @@ int sparkalog_hip_store_set_profiling(sparkalog_hip_store* store, int enabled);
  * thread each through that index. Results do not depend on any of them. */
 int sparkalog_hip_store_set_tuning(sparkalog_hip_store* store, uint32_t merge_ratio, int segmented_distinct,
                                    int indexed_anti_join, uint64_t sparse_probe_rows);
+/* Named executor options (no counterpart in the reference); results do not
+ * depend on any of them. Unknown names or values return
+ * SPARKALOG_HIP_ERR_INVALID_ARGUMENT.
+ *   "join_index": layout of the join's right-relation index (the reference's
+ *     U32RangeIndex, crates/sparkalog-storage/src/lib.rs:345-423): 0 = chosen
+ *     per relation (default: a dense start array over key values while the
+ *     largest value is below 2^28 and the array is at most about 4 entries per
+ *     row beyond 2^24; else the unique keys through a hash table, which
+ *     indexes any u32 value), 1 = dense where the values allow it, 2 = hashed. */
+int sparkalog_hip_store_set_option(sparkalog_hip_store* store, const char* name, uint64_t value);
 int sparkalog_hip_store_op_stats(sparkalog_hip_store* store, int op, uint64_t* launches, double* ms,
                                  double* bytes);
 /* The operator's name and main kernel, and that kernel's own launches and

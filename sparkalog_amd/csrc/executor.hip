@@ -253,6 +253,86 @@ __device__ __forceinline__ uint32_t column_of(uint64_t key, uint32_t col) {
     return col == 0 ? static_cast<uint32_t>(key >> 32) : static_cast<uint32_t>(key);
 }
 
+// ---- hashed join index ----
+// A right relation whose key values are sparse (any u32 value, or far more
+// possible values than rows) is indexed by its unique keys instead of a dense
+// start array over every value: rows sorted by (key, other) as for the dense
+// index, starts[r] = the first row of the r-th unique key (K + 1 entries),
+// and an open-addressing table of (key << 32 | r) slots, linear probing,
+// Fibonacci hashing, at most half full (so every probe sequence meets an
+// empty slot).
+constexpr uint64_t kEmptySlot = ~0ull;
+
+__host__ __device__ __forceinline__ uint64_t hash_slot(uint32_t key, uint32_t bits) {
+    return (static_cast<uint64_t>(key) * 0x9E3779B97F4A7C15ull) >> (64 - bits);
+}
+
+// The join index a count kernel reads: dense (starts over key values
+// [0, n_keys]) when table is null, else hashed (starts over unique keys).
+struct KeyIndex {
+    const uint32_t* starts;
+    uint32_t n_keys;
+    const uint64_t* table;
+    uint32_t bits;
+};
+
+constexpr int kHeadItems = 8;  // consecutive sorted rows per thread
+constexpr int kHeadTile = kBlock * kHeadItems;
+
+// Unique-key heads of n >= 1 sorted index rows (key in the high half). With
+// write == 0 each block stores its head count in tile_counts[block]; with
+// write == 1 (tile_counts then holding their exclusive prefix) each head row
+// goes to starts[tile_counts[block] + its rank].
+__global__ __launch_bounds__(kBlock) void key_heads_kernel(const uint64_t* sorted, size_t n, uint64_t* tile_counts,
+                                                           uint32_t* starts, int write) {
+    __shared__ uint64_t lds[kWavesPerBlock];
+    const size_t first = static_cast<size_t>(blockIdx.x) * kHeadTile + static_cast<size_t>(threadIdx.x) * kHeadItems;
+    bool have_prev = first > 0 && first - 1 < n;
+    uint32_t prev = have_prev ? static_cast<uint32_t>(sorted[first - 1] >> 32) : 0u;
+    uint32_t heads = 0;
+#pragma unroll
+    for (int i = 0; i < kHeadItems; ++i) {
+        const size_t idx = first + i;
+        if (idx < n) {
+            const uint32_t key = static_cast<uint32_t>(sorted[idx] >> 32);
+            if (!have_prev || key != prev) heads |= 1u << i;
+            prev = key;
+            have_prev = true;
+        }
+    }
+    uint64_t block_total = 0;
+    const uint64_t excl = block_exclusive_sum<uint64_t>(static_cast<uint64_t>(__popc(heads)), &block_total, lds);
+    if (!write) {
+        if (threadIdx.x == 0) tile_counts[blockIdx.x] = block_total;
+        return;
+    }
+    uint64_t pos = tile_counts[blockIdx.x] + excl;
+#pragma unroll
+    for (int i = 0; i < kHeadItems; ++i)
+        if ((heads >> i) & 1u) starts[pos++] = static_cast<uint32_t>(first + i);
+}
+
+// starts[K] = n, K = *unique_keys (device).
+__global__ void close_starts_kernel(uint32_t* starts, const uint64_t* unique_keys, size_t n) {
+    starts[*unique_keys] = static_cast<uint32_t>(n);
+}
+
+// Each unique key r into the table (keys are distinct, so no slot is taken
+// twice; the table has more slots than keys, so every loop ends).
+__global__ void hash_insert_kernel(const uint64_t* sorted, const uint32_t* starts, uint32_t unique_keys,
+                                   unsigned long long* table, uint32_t bits) {
+    const uint64_t mask = (bits >= 64 ? ~0ull : (1ull << bits) - 1);
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t r = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < unique_keys; r += stride) {
+        const uint32_t key = static_cast<uint32_t>(sorted[starts[r]] >> 32);
+        const unsigned long long v = (static_cast<unsigned long long>(key) << 32) | static_cast<uint32_t>(r);
+        uint64_t h = hash_slot(key, bits);
+        while (atomicCAS(&table[h], static_cast<unsigned long long>(kEmptySlot), v) !=
+               static_cast<unsigned long long>(kEmptySlot))
+            h = (h + 1) & mask;
+    }
+}
+
 // Join count over a dense CSR, in tiles of kCountTile delta rows: each
 // row's first matching index row (for the emit), its exclusive offset within
 // its tile (u64), and the tile's total in tile_sums. join_scan_kernel turns
@@ -265,12 +345,12 @@ __device__ __forceinline__ uint32_t column_of(uint64_t key, uint32_t col) {
 constexpr int kCountItems = 8;
 constexpr int kCountTile = kBlock * kCountItems;
 
-template <typename O>
+template <typename O, bool Hashed>
 __global__ __launch_bounds__(kBlock) void dense_join_count_kernel(const uint64_t* delta, size_t n, uint32_t key_col,
-                                                                  const uint32_t* starts, uint32_t n_keys,
-                                                                  O* local, uint32_t* match,
+                                                                  KeyIndex ix, O* local, uint32_t* match,
                                                                   uint64_t* tile_sums, uint32_t* clear,
                                                                   uint32_t clear_words) {
+    const uint32_t* starts = ix.starts;
     // the step's status words, cleared by the first block (no kernel before
     // the count in a step reads them)
     if (blockIdx.x == 0)
@@ -292,15 +372,44 @@ __global__ __launch_bounds__(kBlock) void dense_join_count_kernel(const uint64_t
         dk[i] = __builtin_nontemporal_load(&delta[row < n ? row : n - 1]);
     }
     uint32_t m0[kCountItems], m1[kCountItems];
+    if constexpr (!Hashed) {
 #pragma unroll
-    for (int i = 0; i < kCountItems; ++i) {
-        const size_t row = base + wave * kWaveRows + i * kWave + lane;
-        const uint32_t key = column_of(dk[i], key_col);
-        const bool hit = row < n && key < n_keys;
-        const uint32_t k = hit ? key : 0u;
-        const uint32_t a = starts[k], b = starts[k + 1];
-        m0[i] = hit ? a : 0u;
-        m1[i] = hit ? b : 0u;
+        for (int i = 0; i < kCountItems; ++i) {
+            const size_t row = base + wave * kWaveRows + i * kWave + lane;
+            const uint32_t key = column_of(dk[i], key_col);
+            const bool hit = row < n && key < ix.n_keys;
+            const uint32_t k = hit ? key : 0u;
+            const uint32_t a = starts[k], b = starts[k + 1];
+            m0[i] = hit ? a : 0u;
+            m1[i] = hit ? b : 0u;
+        }
+    } else {
+        // first probes of all items in flight together, then the few that
+        // met another key walk on (the table is at most half full)
+        const uint64_t mask = (1ull << ix.bits) - 1;
+        uint64_t h[kCountItems], sl[kCountItems];
+#pragma unroll
+        for (int i = 0; i < kCountItems; ++i) {
+            h[i] = hash_slot(column_of(dk[i], key_col), ix.bits);
+            sl[i] = ix.table[h[i]];
+        }
+#pragma unroll
+        for (int i = 0; i < kCountItems; ++i) {
+            const uint32_t key = column_of(dk[i], key_col);
+            while (sl[i] != kEmptySlot && static_cast<uint32_t>(sl[i] >> 32) != key) {
+                h[i] = (h[i] + 1) & mask;
+                sl[i] = ix.table[h[i]];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kCountItems; ++i) {
+            const size_t row = base + wave * kWaveRows + i * kWave + lane;
+            const bool hit = row < n && sl[i] != kEmptySlot;
+            const uint32_t r = hit ? static_cast<uint32_t>(sl[i]) : 0u;
+            const uint32_t a = starts[r], b = starts[r + 1];
+            m0[i] = hit ? a : 0u;
+            m1[i] = hit ? b : 0u;
+        }
     }
 #pragma unroll
     for (int i = 0; i < kCountItems; ++i) {
@@ -589,11 +698,20 @@ struct CachedIndex {
     uint64_t version = 0;
     DevBuf<uint64_t> sorted;  // (key, other) rows sorted
     DevBuf<uint32_t> other;   // the other column of the sorted rows (the key is the join key)
-    DevBuf<uint32_t> starts;  // dense CSR over key values [0, n_keys]
+    DevBuf<uint32_t> starts;  // dense: CSR over key values [0, n_keys]; hashed: over the n_keys unique keys
     uint32_t n_keys = 0;
     uint64_t rows = 0;
     uint32_t max_degree = 0;  // most rows of one key
+    bool hashed = false;      // sparse key values: unique keys found through `table`
+    DevBuf<uint64_t> table;   // (key << 32 | unique-key rank) slots, kEmptySlot when free
+    uint32_t bits = 0;        // table size is 2^bits
+    KeyIndex view() const { return KeyIndex{starts.ptr, n_keys, hashed ? table.ptr : nullptr, bits}; }
 };
+
+// Join index layout: chosen from the right relation's values (dense while
+// its largest value is below 2^28 and the start array is at most about four
+// entries per row beyond 2^24), or forced by sparkalog_hip_store_set_join_index.
+enum JoinIndexMode { kIndexAuto = 0, kIndexDense = 1, kIndexHashed = 2 };
 
 struct RuleRuntime {
     sparkalog_hip_rule plan{};
@@ -653,7 +771,9 @@ struct sparkalog_hip_store {
     // pinned host copies of the step's counters (read back once per phase)
     // and the event the host waits on for the join sizes
     spk::Counters* counters_host = nullptr;
-    uint64_t* words_host = nullptr;  // [0]: device error flag, [1]: oversized-run rows of the distinct, [2]: index degree
+    // [0]: device error flag, [1]: oversized-run rows of the distinct, [2]: index degree, [3]: a call's error word,
+    // [4]: unique keys of a hashed index
+    uint64_t* words_host = nullptr;
     hipEvent_t counters_ready = nullptr;
     bool profiling = false;
     spk::OpStat stats[spk::kOpCount];
@@ -732,6 +852,7 @@ struct sparkalog_hip_store {
         secpool.push_back(std::move(b));
     }
     bool x_anti = false;  // anti-join FULL through its first-column index when it fits
+    int join_index_mode = spk::kIndexAuto;
     uint64_t sparse_probe_rows = spk::kSparseProbeRows;
     // small persistent device buffers for range probes and statistics (a
     // fresh allocation would synchronize the whole device when freed)
@@ -789,6 +910,25 @@ struct sparkalog_hip_store {
     }
 
     // Profiling brackets around one operator.
+    // Device word for the look-back reports of one call that synchronizes
+    // before it returns: cleared on the stream, then routed to every launcher
+    // of this thread by an ErrorSink; check_call_error reads it after that
+    // synchronization.
+    uint32_t* call_error_word() {
+        misc_dev.reserve(256);
+        uint32_t* w = misc_dev.ptr + 130;
+        spk::check(spk::set_u32(w, 0, stream));
+        return w;
+    }
+    void check_call_error(const uint32_t* w) {
+        host_counters();
+        uint32_t* h = reinterpret_cast<uint32_t*>(&words_host[3]);
+        *h = 0;
+        spk::check(hipMemcpyAsync(h, w, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        spk::check(hipStreamSynchronize(stream));
+        if (*h) throw spk::StatusError{SPARKALOG_HIP_ERR_DEVICE};
+    }
+
     int begin_op() {
         if (!profiling) return -1;
         if (events_used == event_pool.size()) {
@@ -930,10 +1070,17 @@ void build_index(sparkalog_hip_store* s, RuleRuntime& rt, RelationState& right, 
     settle_all(s, right);
     const int ev = s->begin_op();
     const uint64_t n = right.full.rows;
-    if (right.max_value >= (1u << 28)) throw StatusError{SPARKALOG_HIP_ERR_UNSUPPORTED_PLAN};
-    const uint32_t n_keys = right.max_value + 1;
+    // dense while the start array stays near the relation's own size; any
+    // u32 value otherwise (storage/src/lib.rs:357-393 indexes every key)
+    const uint64_t dense_entries = static_cast<uint64_t>(right.max_value) + 1;
+    bool hashed = n > 0 && (right.max_value >= (1u << 28) || dense_entries > 4 * n + (1ull << 24));
+    if (s->join_index_mode == kIndexDense && right.max_value < (1u << 28)) hashed = false;
+    if (s->join_index_mode == kIndexHashed && n > 0) hashed = true;
+    // look-back reports of the index sort (the step's own word is cleared
+    // after the index is built)
+    uint32_t* err = s->call_error_word();
+    ErrorSink sink(err);
     ix.sorted.reserve(n ? n : 1);
-    ix.starts.reserve(static_cast<size_t>(n_keys) + 1);
     DevBuf<uint64_t> tmp = s->take(n ? n : 1);
     if (n) {
         gather_index_keys_kernel<<<grid_for(n), kBlock, 0, s->stream>>>(right.full.keys.ptr, n, col, ix.sorted.ptr);
@@ -946,24 +1093,62 @@ void build_index(sparkalog_hip_store* s, RuleRuntime& rt, RelationState& right, 
         check(radix_sort(in, n, ix.sorted.ptr, tmp.ptr, temp, distinct_temp_bytes(n), known, s->stream, &sorted));
         if (sorted != ix.sorted.ptr) std::swap(ix.sorted, tmp);
     }
-    dense_starts_kernel<<<grid_for(static_cast<size_t>(n_keys) + 1), kBlock, 0, s->stream>>>(ix.sorted.ptr, n,
-                                                                                          ix.starts.ptr, n_keys);
+    uint64_t n_keys = 0;
+    s->host_counters();
+    if (hashed) {
+        // unique keys: head counts per tile, their exclusive scan (and K),
+        // then each head's row into starts
+        const size_t head_tiles = tiles(n, kHeadTile);
+        DevBuf<uint64_t> counts = s->take(head_tiles + 1);
+        uint64_t* k_dev = s->small16() + 4;
+        key_heads_kernel<<<static_cast<unsigned>(head_tiles), kBlock, 0, s->stream>>>(ix.sorted.ptr, n, counts.ptr,
+                                                                                     nullptr, 0);
+        join_scan_kernel<<<1, kScanThreads, 0, s->stream>>>(counts.ptr, head_tiles, k_dev);
+        check(hipGetLastError());
+        uint64_t& k_host = s->words_host[4];
+        k_host = 0;
+        check(hipMemcpyAsync(&k_host, k_dev, sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
+        check(hipStreamSynchronize(s->stream));
+        n_keys = k_host;
+        if (n_keys == 0 || n_keys > n) throw StatusError{SPARKALOG_HIP_ERR_DEVICE};
+        ix.starts.reserve(static_cast<size_t>(n_keys) + 1);
+        key_heads_kernel<<<static_cast<unsigned>(head_tiles), kBlock, 0, s->stream>>>(ix.sorted.ptr, n, counts.ptr,
+                                                                                     ix.starts.ptr, 1);
+        close_starts_kernel<<<1, 1, 0, s->stream>>>(ix.starts.ptr, k_dev, n);
+        s->give(std::move(counts));
+        // 2^bits >= 2K slots: at most half full
+        uint32_t bits = 4;
+        while ((1ull << bits) < 2 * n_keys) ++bits;
+        ix.table.reserve(static_cast<size_t>(1) << bits);
+        check(fill_u64(ix.table.ptr, static_cast<size_t>(1) << bits, kEmptySlot, s->stream));
+        hash_insert_kernel<<<grid_for(n_keys), kBlock, 0, s->stream>>>(
+            ix.sorted.ptr, ix.starts.ptr, static_cast<uint32_t>(n_keys),
+            reinterpret_cast<unsigned long long*>(ix.table.ptr), bits);
+        check(hipGetLastError());
+        ix.bits = bits;
+    } else {
+        n_keys = n ? dense_entries : 1;  // an empty relation: one key with no rows
+        ix.starts.reserve(static_cast<size_t>(n_keys) + 1);
+        dense_starts_kernel<<<grid_for(static_cast<size_t>(n_keys) + 1), kBlock, 0, s->stream>>>(ix.sorted.ptr, n,
+                                                                                              ix.starts.ptr, n_keys);
+        ix.bits = 0;
+    }
+    ix.hashed = hashed;
     ix.other.reserve(n ? n : 1);
     if (n) low_halves_kernel<<<grid_for(n), kBlock, 0, s->stream>>>(ix.sorted.ptr, n, ix.other.ptr);
-    s->host_counters();
     uint32_t* degree_dev = reinterpret_cast<uint32_t*>(s->small16());
     clear_words_kernel<<<1, 64, 0, s->stream>>>(degree_dev, 1);
     max_degree_kernel<<<grid_for(n_keys), kBlock, 0, s->stream>>>(ix.starts.ptr, n_keys, degree_dev);
     uint32_t& degree_host = *reinterpret_cast<uint32_t*>(&s->words_host[2]);
     check(hipGetLastError());
     check(hipMemcpyAsync(&degree_host, degree_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
-    check(hipStreamSynchronize(s->stream));
+    s->check_call_error(err);  // synchronizes
     ix.max_degree = degree_host;
     ix.valid = true;
     ix.relation = relation;
     ix.version = right.version;
     ix.column = col;
-    ix.n_keys = n_keys;
+    ix.n_keys = static_cast<uint32_t>(n_keys);
     ix.rows = n;
     s->give(std::move(tmp));
     s->end_op(ev, kOpIndex, 16.0 * n);
@@ -1327,11 +1512,15 @@ void union_into(sparkalog_hip_store* s, DevRelation& dst, const uint64_t* keys, 
     Counters* cdev = reinterpret_cast<Counters*>(s->counters_dev.ptr);
     PairInput a{dst.keys.ptr, nullptr, nullptr, nullptr};
     PairInput b{keys, nullptr, nullptr, nullptr};
-    check(sorted_union(a, dst.rows, b, rows, PairOutput{out.ptr, nullptr, nullptr}, &cdev->merge_rows, temp, tb,
-                       s->stream));
+    uint32_t* err = s->call_error_word();
+    {
+        ErrorSink sink(err);
+        check(sorted_union(a, dst.rows, b, rows, PairOutput{out.ptr, nullptr, nullptr}, &cdev->merge_rows, temp, tb,
+                           s->stream));
+    }
     uint64_t n = 0;
     check(hipMemcpyAsync(&n, &cdev->merge_rows, sizeof(n), hipMemcpyDeviceToHost, s->stream));
-    check(hipStreamSynchronize(s->stream));
+    s->check_call_error(err);
     s->give(std::move(dst.keys));
     dst.keys = std::move(out);
     dst.rows = n;
@@ -1573,13 +1762,17 @@ extern "C" int sparkalog_hip_store_difference(sparkalog_hip_store* s, uint32_t t
     void* temp = s->scratch(tb);
     PairInput a{ls->full.keys.ptr, nullptr, nullptr, nullptr};
     PairOutput o{ts->full.keys.ptr, nullptr, nullptr};
+    uint32_t* err = s->call_error_word();
     const int ev = s->begin_op();
-    check(sorted_anti_join_multi(a, nl, rights, o, nullptr, &cdev->merge_rows, temp, tb, s->stream,
-                                 s->kernel_timer(ev)));
+    {
+        ErrorSink sink(err);
+        check(sorted_anti_join_multi(a, nl, rights, o, nullptr, &cdev->merge_rows, temp, tb, s->stream,
+                                     s->kernel_timer(ev)));
+    }
     s->end_op(ev, kOpAntiJoin, 8.0 * static_cast<double>(nl + nr));
     uint64_t n = 0;
     check(hipMemcpyAsync(&n, &cdev->merge_rows, sizeof(n), hipMemcpyDeviceToHost, s->stream));
-    check(hipStreamSynchronize(s->stream));
+    s->check_call_error(err);
     if (s->profiling) s->collect_ops();
     ts->full.rows = n;
     if (rows) *rows = n;
@@ -1683,6 +1876,19 @@ extern "C" int sparkalog_hip_store_set_tuning(sparkalog_hip_store* s, uint32_t m
     s->segmented_distinct = segmented_distinct != 0;
     s->x_anti = indexed_anti_join != 0;
     return SPARKALOG_HIP_OK;
+}
+
+extern "C" int sparkalog_hip_store_set_option(sparkalog_hip_store* s, const char* name, uint64_t value) {
+    if (!s || !name) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    const std::string key(name);
+    if (key == "join_index") {
+        if (value > 2) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+        s->join_index_mode = static_cast<int>(value);
+        // cached indexes are rebuilt in the new layout
+        for (auto& rt : s->ws_rules) rt.index.valid = false;
+        return SPARKALOG_HIP_OK;
+    }
+    return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
 }
 
 extern "C" int sparkalog_hip_store_set_profiling(sparkalog_hip_store* s, int enabled) {
@@ -1814,6 +2020,10 @@ void session_step(sparkalog_hip_store* s) {
     Counters* cdev = reinterpret_cast<Counters*>(s->counters_dev.ptr);
     Counters& chost = s->host_counters();
     uint32_t* lb_error = s->misc_dev.ptr + 128;  // device error flag of this step's kernels
+    // every look-back launcher of the step reports into it (the index build
+    // below keeps a word of its own, since the step's word is cleared by the
+    // first join count)
+    ErrorSink step_sink(lb_error);
     uint64_t& iterations = ss.iterations;
     uint64_t& total_new = ss.total_new;
     double& algo_bytes = ss.algo_bytes;
@@ -1864,15 +2074,25 @@ void session_step(sparkalog_hip_store* s) {
         const uint32_t clear_words = status_cleared ? 0u : 256u;
         status_cleared = true;
         timer_start(s->kernel_timer(ev), s->stream);
-        if (rt.narrow)
-            dense_join_count_kernel<uint32_t><<<static_cast<unsigned>(n_tiles), kBlock, 0, s->stream>>>(
-                ds.delta.keys.ptr, L, rt.plan.left_key, rt.index.starts.ptr, rt.index.n_keys,
-                reinterpret_cast<uint32_t*>(rt.offsets.ptr), rt.match.ptr, rt.tile_base.ptr, s->misc_dev.ptr,
+        const KeyIndex kix = rt.index.view();
+        uint32_t* local32 = reinterpret_cast<uint32_t*>(rt.offsets.ptr);
+        const unsigned count_grid = static_cast<unsigned>(n_tiles);
+        if (rt.narrow && !rt.index.hashed)
+            dense_join_count_kernel<uint32_t, false><<<count_grid, kBlock, 0, s->stream>>>(
+                ds.delta.keys.ptr, L, rt.plan.left_key, kix, local32, rt.match.ptr, rt.tile_base.ptr, s->misc_dev.ptr,
                 clear_words);
+        else if (rt.narrow)
+            dense_join_count_kernel<uint32_t, true><<<count_grid, kBlock, 0, s->stream>>>(
+                ds.delta.keys.ptr, L, rt.plan.left_key, kix, local32, rt.match.ptr, rt.tile_base.ptr, s->misc_dev.ptr,
+                clear_words);
+        else if (!rt.index.hashed)
+            dense_join_count_kernel<uint64_t, false><<<count_grid, kBlock, 0, s->stream>>>(
+                ds.delta.keys.ptr, L, rt.plan.left_key, kix, rt.offsets.ptr, rt.match.ptr, rt.tile_base.ptr,
+                s->misc_dev.ptr, clear_words);
         else
-            dense_join_count_kernel<uint64_t><<<static_cast<unsigned>(n_tiles), kBlock, 0, s->stream>>>(
-                ds.delta.keys.ptr, L, rt.plan.left_key, rt.index.starts.ptr, rt.index.n_keys, rt.offsets.ptr,
-                rt.match.ptr, rt.tile_base.ptr, s->misc_dev.ptr, clear_words);
+            dense_join_count_kernel<uint64_t, true><<<count_grid, kBlock, 0, s->stream>>>(
+                ds.delta.keys.ptr, L, rt.plan.left_key, kix, rt.offsets.ptr, rt.match.ptr, rt.tile_base.ptr,
+                s->misc_dev.ptr, clear_words);
         timer_stop(s->kernel_timer(ev), s->stream);
         join_scan_kernel<<<1, kScanThreads, 0, s->stream>>>(rt.tile_base.ptr, n_tiles, &cdev->join_total[r]);
         check(hipGetLastError());

@@ -20,6 +20,15 @@
 #include "prims.hpp"
 
 namespace spk {
+
+namespace {
+thread_local uint32_t* g_error_sink = nullptr;
+}  // namespace
+
+uint32_t* error_sink() { return g_error_sink; }
+ErrorSink::ErrorSink(uint32_t* word) : previous_(g_error_sink) { g_error_sink = word; }
+ErrorSink::~ErrorSink() { g_error_sink = previous_; }
+
 namespace {
 
 constexpr int kSortItems = 16;
@@ -207,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void sort_digit_scan_kernel(SortDesc* desc)
 // The next active pass's status region is cleared slice by slice.
 __global__ __launch_bounds__(kBlock) void sort_pass_kernel(uint64_t* keys_a, uint64_t* keys_b, size_t rows,
                                                            SortDesc* desc, int pass, uint64_t* status_r0,
-                                                           uint64_t* status_r1) {
+                                                           uint64_t* status_r1, uint32_t* error) {
     __shared__ uint64_t tile_keys[kSortTile];
     __shared__ uint32_t wave_count[kWavesPerBlock][kRadix];
     __shared__ uint64_t digit_base[kRadix];  // global start - local start
@@ -284,7 +293,7 @@ __global__ __launch_bounds__(kBlock) void sort_pass_kernel(uint64_t* keys_a, uin
             const uint64_t flag = w >> 62;
             if (flag == 0) {
                 if (++spins >= kSpinLimit) {
-                    atomicOr(&desc->header.error, kErrLookback);
+                    atomicOr(error, kErrLookback);
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
@@ -330,7 +339,8 @@ __global__ __launch_bounds__(kBlock) void sort_pass_kernel(uint64_t* keys_a, uin
 __global__ __launch_bounds__(kBlock) void unique_kernel(const uint64_t* keys_a, const uint64_t* keys_b,
                                                         const SortDesc* sort_desc, int sorted_in_b,
                                                         size_t rows, const uint64_t* rows_dev, PairOutput out,
-                                                        uint64_t* unique_rows, uint64_t* status, Header* header) {
+                                                        uint64_t* unique_rows, uint64_t* status, Header* header,
+                                                        uint32_t* error) {
     __shared__ uint64_t stage[kCompactTile];
     __shared__ uint32_t scan_lds[kWavesPerBlock];
     __shared__ uint64_t offset_slot;
@@ -371,7 +381,7 @@ __global__ __launch_bounds__(kBlock) void unique_kernel(const uint64_t* keys_a, 
     uint32_t tile_total;
     const uint32_t local = block_exclusive_sum<uint32_t>(count, &tile_total, scan_lds);
     if (wave_id() == 0) {
-        const uint64_t excl = wave_lookback(status, tile, tile_total, 0, &header->error);
+        const uint64_t excl = wave_lookback(status, tile, tile_total, 0, error);
         if (lane_id() == 0) offset_slot = excl;
     }
     // stage kept keys in order
@@ -388,6 +398,21 @@ __global__ __launch_bounds__(kBlock) void unique_kernel(const uint64_t* keys_a, 
 
 __global__ void set_u64_kernel(uint64_t* p, uint64_t v) { *p = v; }
 __global__ void set_u32_kernel(uint32_t* p, uint32_t v) { *p = v; }
+
+// After an operator's kernels (stream order: every tile has finished): a
+// look-back report turns the output count into all ones.
+__global__ void report_error_kernel(const uint32_t* error, uint32_t* count32, uint64_t* count64) {
+    if (*error == 0) return;
+    if (count32) *count32 = 0xFFFFFFFFu;
+    if (count64) *count64 = ~0ull;
+}
+
+// The look-back error word of a launch: the thread's sink, else the launch's
+// own header (reported into the count afterwards by report_error).
+uint32_t* error_word(Header* header) {
+    uint32_t* sink = error_sink();
+    return sink ? sink : &header->error;
+}
 
 struct SortLayout {
     size_t desc_off, status0_off, status1_off, unique_status_off, total;
@@ -411,7 +436,7 @@ SortLayout sort_layout(size_t rows) {
 }
 
 hipError_t run_sort(PairInput in, size_t rows, uint64_t* keys_a, uint64_t* keys_b, void* temp,
-                    const SortLayout& l, uint64_t known_varying, hipStream_t stream) {
+                    const SortLayout& l, uint64_t known_varying, hipStream_t stream, uint32_t* error) {
     char* base = static_cast<char*>(temp);
     SortDesc* desc = reinterpret_cast<SortDesc*>(base + l.desc_off);
     uint64_t* s0 = reinterpret_cast<uint64_t*>(base + l.status0_off);
@@ -425,7 +450,7 @@ hipError_t run_sort(PairInput in, size_t rows, uint64_t* keys_a, uint64_t* keys_
     for (int p = 0; p < kPasses; ++p) {
         if (known_varying && !((known_varying >> (p * kRadixBits)) & (kRadix - 1))) continue;
         sort_pass_kernel<<<static_cast<unsigned>(l.sort_tiles), kBlock, 0, stream>>>(keys_a, keys_b, rows, desc,
-                                                                                    p, s0, s1);
+                                                                                    p, s0, s1, error);
     }
     return hipGetLastError();
 }
@@ -527,7 +552,7 @@ __global__ void merge_partition_kernel(PairInput left, size_t nl, PairInput righ
 
 __global__ __launch_bounds__(kBlock) void union_kernel(PairInput left, size_t nl, PairInput right, size_t nr,
                                                        PairOutput out, uint64_t* unique_rows, const size_t* part,
-                                                       uint64_t* status, Header* header) {
+                                                       uint64_t* status, Header* header, uint32_t* error) {
     __shared__ uint64_t lds[kMergeTile + 2];
     __shared__ uint32_t scan_lds[kWavesPerBlock];
     __shared__ uint64_t offset_slot;
@@ -612,7 +637,7 @@ __global__ __launch_bounds__(kBlock) void union_kernel(PairInput left, size_t nl
     uint32_t tile_total;
     const uint32_t local = block_exclusive_sum<uint32_t>(count, &tile_total, scan_lds);
     if (wave_id() == 0) {
-        const uint64_t excl = wave_lookback(status, tile, tile_total, 0, &header->error);
+        const uint64_t excl = wave_lookback(status, tile, tile_total, 0, error);
         if (lane_id() == 0) offset_slot = excl;
     }
     // compact in LDS (reads of lds[] above are complete after the scan's barriers)
@@ -797,7 +822,7 @@ __global__ __launch_bounds__(kBlock) void anti_join_kernel(PairInput left, size_
 // waits long for another).
 __global__ __launch_bounds__(kBlock) void anti_compact_kernel(PairInput left, size_t nl, const uint64_t* found_words,
                                                               PairOutput out, uint32_t* count32, uint64_t* count64,
-                                                              uint64_t* status, Header* header) {
+                                                              uint64_t* status, Header* header, uint32_t* error) {
     constexpr int kItems = kAntiCompactTile / kBlock;  // 16
     constexpr int kWaveRows = kItems * kWave;          // 1024
     __shared__ uint32_t wave_totals[kWavesPerBlock][kItems];
@@ -835,7 +860,7 @@ __global__ __launch_bounds__(kBlock) void anti_compact_kernel(PairInput left, si
             tile_total += wave_totals[w][i];
         }
     if (wave == 0) {
-        const uint64_t excl = wave_lookback(status, tile, tile_total, 0, &header->error);
+        const uint64_t excl = wave_lookback(status, tile, tile_total, 0, error);
         if (lane == 0) offset_slot = excl;
     }
     __syncthreads();
@@ -931,7 +956,8 @@ __device__ __forceinline__ int64_t find_index_key_sampled(const uint32_t* keys, 
 __global__ __launch_bounds__(kBlock) void join_count_kernel(const uint32_t* left_keys, size_t nl,
                                                             const uint32_t* index_keys, const uint32_t* index_starts,
                                                             size_t k, uint64_t* counts, uint64_t* offsets,
-                                                            uint64_t* total, uint64_t* status, Header* header) {
+                                                            uint64_t* total, uint64_t* status, Header* header,
+                                                            uint32_t* error) {
     constexpr int kItems = 8;
     constexpr int kTile = kBlock * kItems;
     __shared__ uint64_t scan_lds[kWavesPerBlock];
@@ -965,7 +991,7 @@ __global__ __launch_bounds__(kBlock) void join_count_kernel(const uint32_t* left
     }
     sum = running;
     if (wave_id() == 0) {
-        const uint64_t excl = wave_lookback(status, tile, sum, 0, &header->error);
+        const uint64_t excl = wave_lookback(status, tile, sum, 0, error);
         if (lane_id() == 0) offset_slot = excl;
     }
     __syncthreads();
@@ -1074,7 +1100,7 @@ __device__ __forceinline__ bool compare_u32(uint32_t value, uint32_t op, uint32_
 
 __global__ __launch_bounds__(kBlock) void filter_kernel(const uint32_t* input, size_t n, uint32_t op,
                                                         uint32_t operand, uint32_t* out_rows, uint32_t* count,
-                                                        uint64_t* status, Header* header) {
+                                                        uint64_t* status, Header* header, uint32_t* error) {
     __shared__ uint32_t stage[kCompactTile];
     __shared__ uint32_t wave_totals[kWavesPerBlock][kCompactItems];
     __shared__ uint64_t offset_slot;
@@ -1104,7 +1130,7 @@ __global__ __launch_bounds__(kBlock) void filter_kernel(const uint32_t* input, s
         tile_total += s;
     }
     if (wave == 0) {
-        const uint64_t excl = wave_lookback(status, tile, tile_total, 0, &header->error);
+        const uint64_t excl = wave_lookback(status, tile, tile_total, 0, error);
         if (lane == 0) offset_slot = excl;
     }
     uint32_t pos = wave_start;
@@ -1176,6 +1202,12 @@ hipError_t prepare_compact(void* temp, size_t tiles, hipStream_t stream) {
 // ========================================================================
 // launchers
 
+hipError_t report_error(const uint32_t* error, uint32_t* count32, uint64_t* count64, hipStream_t stream) {
+    if (error_sink()) return hipSuccess;  // the sink's owner checks it
+    report_error_kernel<<<1, 1, 0, stream>>>(error, count32, count64);
+    return hipGetLastError();
+}
+
 size_t distinct_temp_bytes(size_t rows) {
     const SortLayout l = sort_layout(rows);
     return kTempHeader + l.total;
@@ -1187,7 +1219,11 @@ hipError_t radix_sort(PairInput in, size_t rows, uint64_t* keys_a, uint64_t* key
     if (temp_bytes < kTempHeader + l.total) return hipErrorInvalidValue;
     if (rows == 0) return hipSuccess;
     char* sort_temp = static_cast<char*>(temp) + kTempHeader;
-    hipError_t st = run_sort(in, rows, keys_a, keys_b, sort_temp, l, known_varying, stream);
+    // a sort has no output count to mark: without a sink its reports stay in
+    // the header (the sort is only called by owners of a sink)
+    Header* header = static_cast<Header*>(temp);
+    if (!error_sink()) clear_header_kernel<<<1, 64, 0, stream>>>(header);
+    hipError_t st = run_sort(in, rows, keys_a, keys_b, sort_temp, l, known_varying, stream, error_word(header));
     if (sorted_host && known_varying) {
         int active = 0;
         for (int p = 0; p < kPasses; ++p)
@@ -1207,17 +1243,22 @@ hipError_t distinct(PairInput in, size_t rows, uint64_t* keys_a, uint64_t* keys_
     const SortLayout l = sort_layout(rows);
     if (temp_bytes < kTempHeader + l.total) return hipErrorInvalidValue;
     char* sort_temp = static_cast<char*>(temp) + kTempHeader;
-    hipError_t st = run_sort(in, rows, keys_a, keys_b, sort_temp, l, known_varying, stream);
-    if (st != hipSuccess) return st;
+    // the header (tile counter and error word) is cleared once, before the
+    // sort, so the sort's reports reach the unique count's check
     Header* header = static_cast<Header*>(temp);
-    uint64_t* ustatus = reinterpret_cast<uint64_t*>(sort_temp + l.unique_status_off);
     clear_header_kernel<<<1, 64, 0, stream>>>(header);
+    uint32_t* error = error_word(header);
+    hipError_t st = run_sort(in, rows, keys_a, keys_b, sort_temp, l, known_varying, stream, error);
+    if (st != hipSuccess) return st;
+    uint64_t* ustatus = reinterpret_cast<uint64_t*>(sort_temp + l.unique_status_off);
     st = clear_u64(ustatus, l.unique_tiles, stream);
     if (st != hipSuccess) return st;
     const SortDesc* desc = reinterpret_cast<const SortDesc*>(sort_temp + l.desc_off);
     unique_kernel<<<static_cast<unsigned>(l.unique_tiles), kBlock, 0, stream>>>(keys_a, keys_b, desc, 0, rows, nullptr,
-                                                                              out, unique_rows, ustatus, header);
-    return hipGetLastError();
+                                                                              out, unique_rows, ustatus, header, error);
+    st = hipGetLastError();
+    if (st != hipSuccess) return st;
+    return report_error(error, nullptr, unique_rows, stream);
 }
 
 hipError_t unique_sorted(const uint64_t* keys, const uint64_t* rows_dev, size_t bound, PairOutput out,
@@ -1227,10 +1268,13 @@ hipError_t unique_sorted(const uint64_t* keys, const uint64_t* rows_dev, size_t 
     if (temp_bytes < compact_temp(tiles)) return hipErrorInvalidValue;
     hipError_t st = prepare_compact(temp, tiles, stream);
     if (st != hipSuccess) return st;
+    uint32_t* error = error_word(static_cast<Header*>(temp));
     unique_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(
         keys, keys, nullptr, 0, bound, rows_dev, out, unique_rows,
-        reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp));
-    return hipGetLastError();
+        reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp), error);
+    st = hipGetLastError();
+    if (st != hipSuccess) return st;
+    return report_error(error, nullptr, unique_rows, stream);
 }
 
 size_t union_temp_bytes(size_t left_rows, size_t right_rows) {
@@ -1253,12 +1297,15 @@ hipError_t sorted_union(PairInput left, size_t left_rows, PairInput right, size_
     size_t* part = reinterpret_cast<size_t*>(static_cast<char*>(temp) + compact_temp(tiles));
     merge_partition_kernel<<<grid_for(tiles + 1, kBlock, 4096), kBlock, 0, stream>>>(left, left_rows, right,
                                                                                     right_rows, part, tiles + 1);
+    uint32_t* error = error_word(static_cast<Header*>(temp));
     timer_start(timer, stream);
     union_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(
         left, left_rows, right, right_rows, out, unique_rows, part,
-        reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp));
+        reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp), error);
     timer_stop(timer, stream);
-    return hipGetLastError();
+    st = hipGetLastError();
+    if (st != hipSuccess) return st;
+    return report_error(error, nullptr, unique_rows, stream);
 }
 
 size_t merge_disjoint_temp_bytes(size_t left_rows, size_t right_rows) {
@@ -1316,10 +1363,13 @@ hipError_t sorted_anti_join_multi(PairInput left, size_t left_rows, const AntiRi
     anti_join_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(left, left_rows, rights, ranges,
                                                                          found_words, 0u);
     timer_stop(timer, stream);
+    uint32_t* error = error_word(static_cast<Header*>(temp));
     anti_compact_kernel<<<static_cast<unsigned>(compact_tiles), kBlock, 0, stream>>>(
         left, left_rows, found_words, out, count32, count64,
-        reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp));
-    return hipGetLastError();
+        reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp), error);
+    st = hipGetLastError();
+    if (st != hipSuccess) return st;
+    return report_error(error, count32, count64, stream);
 }
 
 size_t probe_grouped_temp_bytes(size_t rows, uint32_t right_inputs) {
@@ -1372,10 +1422,13 @@ hipError_t join_count(const uint32_t* left_keys, size_t left_rows, const uint32_
     if (temp_bytes < compact_temp(tiles)) return hipErrorInvalidValue;
     hipError_t st = prepare_compact(temp, tiles, stream);
     if (st != hipSuccess) return st;
+    uint32_t* error = error_word(static_cast<Header*>(temp));
     join_count_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(
         left_keys, left_rows, index_keys, index_starts, unique_keys, counts, offsets, total,
-        reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp));
-    return hipGetLastError();
+        reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp), error);
+    st = hipGetLastError();
+    if (st != hipSuccess) return st;
+    return report_error(error, nullptr, total, stream);
 }
 
 hipError_t join_emit(const uint32_t* left_keys, size_t left_rows, const uint32_t* index_keys,
@@ -1401,10 +1454,13 @@ hipError_t filter(const uint32_t* input, size_t rows, uint32_t op, uint32_t oper
     if (temp_bytes < compact_temp(tiles)) return hipErrorInvalidValue;
     hipError_t st = prepare_compact(temp, tiles, stream);
     if (st != hipSuccess) return st;
+    uint32_t* error = error_word(static_cast<Header*>(temp));
     filter_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(
         input, rows, op, operand, out_rows, count,
-        reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp));
-    return hipGetLastError();
+        reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp), error);
+    st = hipGetLastError();
+    if (st != hipSuccess) return st;
+    return report_error(error, count, nullptr, stream);
 }
 
 hipError_t add_one_i32(int32_t* data, size_t len, hipStream_t stream) {

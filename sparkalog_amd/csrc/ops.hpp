@@ -241,4 +241,27 @@ hipError_t set_u64(uint64_t* out, uint64_t value, hipStream_t stream);
 // overflow bits); every launcher reserves it at offset 0 of `temp`.
 constexpr size_t kTempHeader = 256;
 
+// Where look-back kernels report a bounded wait that gave up (kErrLookback).
+// While an ErrorSink lives on a host thread, every launcher called on that
+// thread passes the sink's device word to its look-back kernels, and the
+// owner of the word checks it after its next synchronization. With no sink,
+// a launcher reports into its temporary header and then overwrites the
+// operator's output count with all ones (UINT32_MAX or UINT64_MAX), which is
+// how the operator ABI makes such a failure visible to its caller.
+uint32_t* error_sink();
+class ErrorSink {
+public:
+    explicit ErrorSink(uint32_t* word);
+    ~ErrorSink();
+    ErrorSink(const ErrorSink&) = delete;
+    ErrorSink& operator=(const ErrorSink&) = delete;
+
+private:
+    uint32_t* previous_;
+};
+// With no sink on this thread: after an operator's kernels (stream order),
+// marks its output count all ones when `error` holds a report. A no-op under
+// a sink, whose owner checks the word instead.
+hipError_t report_error(const uint32_t* error, uint32_t* count32, uint64_t* count64, hipStream_t stream);
+
 }  // namespace spk

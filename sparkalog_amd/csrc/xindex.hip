@@ -323,7 +323,7 @@ __global__ __launch_bounds__(kBlock) void anti_join_x_kernel(const uint64_t* lef
 __global__ __launch_bounds__(kBlock) void xanti_compact_kernel(const uint64_t* left, const uint64_t* left_rows,
                                                                size_t bound, const uint32_t* found_bits,
                                                                PairOutput out, uint64_t* count64, uint64_t* status,
-                                                               XHeader* h) {
+                                                               XHeader* h, uint32_t* error) {
     __shared__ uint32_t item_cnt[kWavesPerBlock][kCItems];
     __shared__ uint64_t offset_slot;
     __shared__ uint32_t tile_slot;
@@ -359,7 +359,7 @@ __global__ __launch_bounds__(kBlock) void xanti_compact_kernel(const uint64_t* l
             tile_total += item_cnt[w][i];
         }
     if (wave == 0) {
-        const uint64_t excl = wave_lookback(status, tile, tile_total, 0, &h->error);
+        const uint64_t excl = wave_lookback(status, tile, tile_total, 0, error);
         if (lane == 0) offset_slot = excl;
     }
     __syncthreads();
@@ -1205,9 +1205,12 @@ hipError_t anti_join_x(const uint64_t* left, const uint64_t* left_rows, size_t b
     anti_join_x_kernel<<<static_cast<unsigned>(probe_tiles), kBlock, 0, stream>>>(left, left_rows, bound, rights,
                                                                                   n_x, found_bits);
     timer_stop(timer, stream);
+    uint32_t* error = error_sink() ? error_sink() : &h->error;
     xanti_compact_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(left, left_rows, bound, found_bits, out,
-                                                                             count64, status, h);
-    return hipGetLastError();
+                                                                             count64, status, h, error);
+    const hipError_t st = hipGetLastError();
+    if (st != hipSuccess) return st;
+    return report_error(error, nullptr, count64, stream);
 }
 
 }  // namespace spk

@@ -57,6 +57,24 @@ def _check(operation: str, code: int) -> None:
         raise HipError(operation, code)
 
 
+class DeviceWaitAbandoned(ExecutionError):
+    """A kernel's bounded wait for another tile gave up: the ABI then writes
+    an all-ones output count (no counterpart in the reference, whose CUB
+    calls have no such wait)."""
+
+    def __init__(self, operation: str):
+        super().__init__(f"{operation}: a device tile wait gave up (all-ones output count)")
+
+
+def _device_count(operation: str, t: "torch.Tensor", width: int) -> int:
+    """The device-written output count of an operator (u32 or u64)."""
+    mask = (1 << width) - 1
+    value = int(t.item()) & mask
+    if value == mask:
+        raise DeviceWaitAbandoned(operation)
+    return value
+
+
 def _check_row_count(rows: int) -> None:
     # check_row_count, crates/sparkalog-execution/src/lib.rs:325-331
     if rows > U32_MAX:
@@ -157,7 +175,7 @@ def filter_hip(column: torch.Tensor, predicate: U32Predicate, stream: HipStream)
            lib.sparkalog_hip_filter_u32(_ptr(column), rows, predicate.op, predicate.value, None, _ptr(out),
                                         _ptr(count), _ptr(temp), temp.numel(), stream.raw))
     stream.synchronize()
-    return out[: int(count.item())]
+    return out[: _device_count("sparkalog_hip_filter_u32", count, 32)]
 
 
 @dataclass
@@ -207,7 +225,7 @@ def join_hip(left: Relation, right: Relation, right_index: RangeIndex, plan: Bin
                                             _ptr(counts), _ptr(offsets), _ptr(total), _ptr(temp), temp.numel(),
                                             stream.raw))
     stream.synchronize()
-    m = int(total.item())
+    m = _device_count("sparkalog_hip_join_u32_count", total, 64)
     out0 = _empty(m, torch.int32, dev)
     out1 = _empty(m, torch.int32, dev)
     if m:
@@ -242,7 +260,7 @@ def distinct_hip(input: Relation, plan: BinaryDistinct, stream: HipStream) -> Re
            lib.sparkalog_hip_distinct_u32(_ptr(a), _ptr(b), n, _ptr(packed), _ptr(scratch), _ptr(count), _ptr(out0),
                                           _ptr(out1), _ptr(temp), temp.numel(), stream.raw))
     stream.synchronize()
-    u = int(count.item())
+    u = _device_count("sparkalog_hip_distinct_u32", count, 64)
     return Relation((out0[:u], out1[:u]))
 
 
@@ -267,7 +285,7 @@ def anti_join_hip(left: Relation, right: Relation, plan: SortedBinaryAntiJoin, s
                                            _ptr(count), _ptr(out0), _ptr(out1), _ptr(temp), temp.numel(),
                                            stream.raw))
     stream.synchronize()
-    o = int(count.item()) & U32_MAX
+    o = _device_count("sparkalog_hip_anti_join_u32", count, 32)
     return Relation((out0[:o], out1[:o]))
 
 
@@ -293,5 +311,5 @@ def union_hip(left: Relation, right: Relation, plan: SortedBinaryUnion, stream: 
                                        None, None, _ptr(count), _ptr(out0), _ptr(out1), _ptr(temp), temp.numel(),
                                        stream.raw))
     stream.synchronize()
-    u = int(count.item())
+    u = _device_count("sparkalog_hip_union_u32", count, 64)
     return Relation((out0[:u], out1[:u]))

@@ -201,6 +201,17 @@ class RelationStore:
         _check(self._lib.sparkalog_hip_store_set_tuning(self._h, merge_ratio, int(segmented_distinct),
                                                         int(indexed_anti_join), sparse_probe_rows))
 
+    JOIN_INDEX = {"auto": 0, "dense": 1, "hashed": 2}
+
+    def set_option(self, name: str, value: int) -> None:
+        """A named executor option (sparkalog_hip_store_set_option); results
+        do not depend on any of them."""
+        _check(self._lib.sparkalog_hip_store_set_option(self._h, name.encode(), int(value)), name)
+
+    def set_join_index(self, layout: str) -> None:
+        """Layout of the join index: "auto", "dense" or "hashed"."""
+        self.set_option("join_index", self.JOIN_INDEX[layout])
+
     def set_profiling(self, enabled: bool) -> None:
         _check(self._lib.sparkalog_hip_store_set_profiling(self._h, int(enabled)))
 

@@ -70,6 +70,29 @@ def batch_graphs(seed: int = C4_SEED, instances: int = 4096, n_min: int = 16, n_
             _take(c, instances, np.uint32))
 
 
+def wide_labels(nodes: int, seed: int = 0xF011) -> np.ndarray:
+    """Distinct u32 labels for node indexes 0..nodes-1 (nodes >= 2) spread over
+    the whole u32 range: 0 and 0xFFFFFFFF always among them, the rest drawn
+    uniformly, in random order. Relabelling a graph keeps its closure (as a
+    set of pairs) up to the same relabelling."""
+    if nodes < 2:
+        raise ValueError("wide_labels needs at least two nodes")
+    rng = np.random.default_rng(seed)
+    vals = np.unique(rng.integers(1, 0xFFFFFFFF, size=nodes + nodes // 4 + 16, dtype=np.uint64))
+    rng.shuffle(vals)
+    if vals.size < nodes - 2:
+        raise ValueError("too few distinct labels drawn")
+    ids = np.concatenate([np.array([0, 0xFFFFFFFF], dtype=np.uint64), vals[: nodes - 2]]).astype(np.uint32)
+    rng.shuffle(ids)
+    return ids
+
+
+def relabel(src, dst, labels) -> tuple[np.ndarray, np.ndarray]:
+    """Edges (labels[src], labels[dst]) as sorted unique pairs."""
+    key = np.unique((labels[src].astype(np.uint64) << np.uint64(32)) | labels[dst].astype(np.uint64))
+    return (key >> np.uint64(32)).astype(np.uint32), (key & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+
+
 KEY_UNIFORM = 0
 KEY_ZIPF = 1
 

@@ -60,6 +60,21 @@ def test_host_side_argument_validation_needs_no_gpu():
     assert size.value > 0
 
 
+def test_all_ones_count_is_a_device_error():
+    # the ABI's report of an abandoned device wait reaches the caller as an
+    # exception, never as a row count
+    import torch
+
+    from sparkalog_amd import execution as ex
+
+    assert ex._device_count("op", torch.tensor([5], dtype=torch.int64), 64) == 5
+    assert ex._device_count("op", torch.tensor([7], dtype=torch.int32), 32) == 7
+    with pytest.raises(ex.DeviceWaitAbandoned):
+        ex._device_count("op", torch.tensor([-1], dtype=torch.int64), 64)
+    with pytest.raises(ex.DeviceWaitAbandoned):
+        ex._device_count("op", torch.tensor([-1], dtype=torch.int32), 32)
+
+
 def test_status_strings():
     lib = _native.hip()
     assert lib.sparkalog_hip_status_string(4).decode().startswith("recursive SCC did not converge")

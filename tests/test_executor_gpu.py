@@ -309,6 +309,74 @@ def test_program_with_negation_matches_general_path(expected, stream):
     assert store.difference(R(5), R(3), R(0)) == 0
 
 
+def _wide_expected(s, d, nodes, labels):
+    bfs = oracle.tc_bfs(s, d, nodes, closure=True)
+    a, b = bfs["closure"]
+    return np.sort((labels[a].astype(np.uint64) << np.uint64(32)) | labels[b].astype(np.uint64)), bfs
+
+
+@pytest.mark.parametrize("layout", ["auto", "hashed"])
+def test_ids_over_the_whole_u32_range(layout, stream):
+    # labels spread over [0, 2^32), 0 and 0xFFFFFFFF among them: the join
+    # index holds the unique keys through its hash table (the reference
+    # indexes any u32 key, storage/src/lib.rs:357-393); exact arrays against
+    # BFS and the oracle's semi-naive executor
+    nodes = 1 << 12
+    s, d, _ = workload.component_graph(nodes=nodes)
+    labels = workload.wide_labels(nodes)
+    edges = workload.relabel(s, d, labels)
+    store = RelationStore(stream)
+    store.set_join_index(layout)
+    store.insert_static(R(0), edges)
+    store.insert_recursive(R(1), edges, edges)
+    summ = RecursiveExecutor.compile(plans.transitive_closure_scc(R(1), R(0)), store).run(store, 10_000)
+    exp, bfs = _wide_expected(s, d, nodes, labels)
+    full = store.full(R(1))
+    assert np.array_equal((full[0].astype(np.uint64) << np.uint64(32)) | full[1], exp)
+    assert summ.iterations == bfs["max_dist"]
+    assert summ.total_new_rows == exp.size - edges[0].size
+    ref = oracle.Store()
+    ref.insert_static(0, edges)
+    ref.insert_recursive(1, edges, edges)
+    rc, ref_summary = ref.run([1], [oracle.rule_from_plan(plans.transitive_closure_scc(R(1), R(0)).rules[0])],
+                              10_000, 4)
+    assert rc == 0
+    o = ref.get(1)
+    assert np.array_equal(full[0], o[0]) and np.array_equal(full[1], o[1])
+    assert summ.iterations == ref_summary.iterations
+
+
+def test_ids_over_the_whole_u32_range_2p17(expected, stream):
+    # the 2^17-node component graph relabelled into the full u32 range:
+    # rows, order-independent digest, sortedness and iterations against the
+    # committed BFS fixture
+    e = expected["component_graph_131072_wide"]
+    s, d, _ = workload.component_graph(nodes=1 << 17)
+    edges = workload.relabel(s, d, workload.wide_labels(1 << 17))
+    assert workload.pair_digest(*edges) == e["edge_digest"]
+    store, summ = _tc(edges, stream)
+    digest, sorted_unique, rows = store.digest(R(1))
+    assert rows == e["path_rows"] and digest == e["path_digest"] and sorted_unique
+    assert summ.iterations == e["max_dist"]
+    assert summ.total_new_rows == e["path_rows"] - e["edges"]
+
+
+@pytest.mark.parametrize("layout", ["dense", "hashed"])
+def test_join_index_layouts_agree(layout, expected, stream):
+    # the 2^14-node graph with either index layout forced: exact arrays
+    e = expected["component_graph_16384"]
+    s, d, _ = workload.component_graph(nodes=1 << 14)
+    store = RelationStore(stream)
+    store.set_join_index(layout)
+    store.insert_static(R(0), (s, d))
+    store.insert_recursive(R(1), (s, d), (s, d))
+    summ = RecursiveExecutor.compile(plans.transitive_closure_scc(R(1), R(0)), store).run(store, 10_000)
+    full = store.full(R(1))
+    bfs = oracle.tc_bfs(s, d, 1 << 14, closure=True)
+    assert np.array_equal(full[0], bfs["closure"][0]) and np.array_equal(full[1], bfs["closure"][1])
+    assert summ.iterations == e["max_dist"]
+
+
 def test_wide_ids(stream):
     # node ids spread over [0, 2^28): 64-bit sort keys in the tile sort, no
     # first-column index (ids >= 2^24), the densest join index still allowed

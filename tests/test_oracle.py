@@ -241,6 +241,36 @@ def test_seminaive_matches_bfs(seed):
     assert s.total_new_rows == bfs["pairs"] - len(e)
 
 
+def _wide_closure(s, d, nodes, labels):
+    """BFS closure in node-index space, relabelled and sorted."""
+    bfs = oracle.tc_bfs(s, d, nodes, closure=True)
+    a, b = bfs["closure"]
+    key = np.sort((labels[a].astype(np.uint64) << np.uint64(32)) | labels[b].astype(np.uint64))
+    return (key >> np.uint64(32)).astype(np.uint32), (key & np.uint64(0xFFFFFFFF)).astype(np.uint32), bfs
+
+
+def test_seminaive_over_the_whole_u32_range():
+    # ids 0 and 0xFFFFFFFF among labels spread over [0, 2^32): the
+    # restatement indexes any u32 key (storage/src/lib.rs:357-393)
+    s, d, _ = workload.component_graph(nodes=1 << 11)
+    labels = workload.wide_labels(1 << 11)
+    assert labels.min() == 0 and labels.max() == 0xFFFFFFFF and np.unique(labels).size == labels.size
+    edges = workload.relabel(s, d, labels)
+    full, summ = _seminaive(edges, threads=2)
+    a, b, bfs = _wide_closure(s, d, 1 << 11, labels)
+    assert np.array_equal(full[0], a) and np.array_equal(full[1], b)
+    assert summ.iterations == bfs["max_dist"]
+
+
+def test_wide_fixture_matches_relabelling(expected):
+    e = expected["component_graph_131072_wide"]
+    base = expected["component_graph_131072"]
+    s, d, _ = workload.component_graph(nodes=1 << 17)
+    ws, wd = workload.relabel(s, d, workload.wide_labels(1 << 17))
+    assert workload.pair_digest(ws, wd) == e["edge_digest"] and ws.size == e["edges"] == base["edges"]
+    assert e["path_rows"] == base["path_rows"] and e["max_dist"] == base["max_dist"]
+
+
 def test_c1_fixture_three_ways(expected):
     e = expected["random_dag_1000_4000"]
     s, d = workload.random_dag()
