@@ -37,6 +37,10 @@ def parse_args():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--nodes", type=int, default=1 << 20)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-batched", action="store_true", help="tc: leave out the batched configuration")
+    p.add_argument("--no-served", action="store_true", help="tc: leave out the served configuration")
+    p.add_argument("--bench-plain-stride", type=int, default=8,
+                   help="tc: the batched summary times the plain path on every k-th instance")
     p.add_argument("--merge-ratio", type=int, default=3, help="tc: FULL run-merge ratio (results are identical)")
     p.add_argument("--host-io", action="store_true",
                    help="tc: also time one fixpoint from host arrays to the closure back in host memory (PCIe)")
@@ -44,8 +48,15 @@ def parse_args():
                    help="tc: anti-join FULL through its first-column index (results are identical)")
     p.add_argument("--sparse-probe-rows", type=int, default=1 << 22,
                    help="tc: candidate count up to which FULL is probed one candidate per thread")
-    p.add_argument("--cpu-sample-nodes", type=int, default=1 << 16)
-    p.add_argument("--mode", choices=["tc", "ops", "batched", "served", "copy"], default="tc",
+    p.add_argument("--cpu-sample-nodes", type=int, default=1 << 16,
+                   help="CPU baseline: nodes of the parallel oracle's component-graph draw")
+    p.add_argument("--cpu-serial-nodes", type=int, default=1 << 14,
+                   help="CPU baseline: nodes of the serial oracle's component-graph draw")
+    p.add_argument("--crossover-ops", default="filter,join,distinct,anti_join,union",
+                   help="crossover mode: operators")
+    p.add_argument("--crossover-min-log2", type=int, default=8, help="crossover mode: smallest log2 row count")
+    p.add_argument("--crossover-max-log2", type=int, default=24, help="crossover mode: largest log2 row count")
+    p.add_argument("--mode", choices=["tc", "ops", "batched", "served", "copy", "crossover"], default="tc",
                    help="tc: the headline fixpoint; ops: operator microbench at 2^k rows; "
                         "batched: 4,096 instances in one loop vs one run each; served: Poisson stream, "
                         "continuous batching vs sequential; copy: plain device copy of --copy-bytes "
@@ -62,33 +73,108 @@ def parse_args():
     p.add_argument("--sizes", default="24,26,28", help="ops mode: log2 row counts")
     p.add_argument("--ops", default="distinct,union,anti_join,join", help="ops mode: operators")
     p.add_argument("--samples", type=int, default=5, help="ops mode: timed launches per case")
+    p.add_argument("--cpu-max-log2", type=int, default=24,
+                   help="ops mode: largest case checked against (and timed with) the oracle on this host; "
+                        "larger ones are checked against the committed oracle fixtures")
     return p.parse_args()
 
 
+def _cpu_threads() -> int:
+    return len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+
+
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def oracle_operator(name, host, threads):
+    """CPU leg of the operator microbench: the oracle's serial (threads = 1)
+    or parallel (Rayon-style chunks) operator on the case's host inputs. Its
+    output is also what the GPU result must equal before any timing is kept
+    (distinct-crossover.rs:227-234)."""
+    import oracle
+
+    if name == "distinct":
+        return oracle.distinct(*host["input"], threads=threads)
+    if name == "union":
+        return oracle.union(host["left"], host["right"], threads=threads)
+    if name == "anti_join":
+        return oracle.anti_join(host["left"], host["right"], threads=threads)
+    return oracle.join(host["left"], host["right"], 1, 0, ((0, 0), (1, 1)), threads=threads)
+
+
 def run_ops(args) -> None:
-    """Operator drop-in microbench: one JSON line per (operator, rows, keys)."""
+    """Operator drop-in microbench: one JSON line per (operator, rows, keys).
+    Every case is checked before it is timed: against the oracle run on this
+    host (which is also timed, as the case's CPU baseline) up to
+    2^--cpu-max-log2 rows, and above that against the digests the oracle
+    wrote into tests/golden/op_expected.json."""
+    import ctypes
+
+    import numpy as np
     import torch
 
     from sparkalog_amd import opbench, workload
 
     torch.cuda.set_device(0)
+    threads = _cpu_threads()
+    fixtures_path = ROOT / "tests" / "golden" / "op_expected.json"
+    fixtures = json.loads(fixtures_path.read_text()) if fixtures_path.exists() else {}
+    modes = {"uniform": workload.KEY_UNIFORM, "zipf1.1": workload.KEY_ZIPF}
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     for name in args.ops.split(","):
-        modes = [workload.KEY_UNIFORM, workload.KEY_ZIPF] if name in ("distinct", "join") else [workload.KEY_UNIFORM]
         for k in [int(x) for x in args.sizes.split(",")]:
-            for mode in modes:
-                op = opbench.make(name, 1 << k, mode)
+            for mode_name, mode in modes.items():
+                host = opbench.host_inputs(name, 1 << k, mode)
+                op = opbench.make(name, 1 << k, mode, host=host)
+                op.launch(stream)
+                torch.cuda.synchronize()
+                got = op.result()
+                check = {}
+                if k <= args.cpu_max_log2:
+                    t0 = time.perf_counter()
+                    exp = oracle_operator(name, host, threads)
+                    check = {"checked_against": "oracle", "cpu_parallel_ms": round(1000 * (time.perf_counter() - t0), 3),
+                             "cpu_threads": threads}
+                    ok = got[0].size == exp[0].size and np.array_equal(got[0], exp[0]) and \
+                        np.array_equal(got[1], exp[1])
+                    del exp
+                else:
+                    e = fixtures.get(f"{name}_2p{k}_{mode_name}")
+                    if e is None:
+                        raise SystemExit(f"ops: no fixture for {name} 2^{k} {mode_name}; run "
+                                         "tests/golden/make_op_fixtures.py")
+                    ok = got[0].size == e["out_rows"] and workload.pair_digest(*got) == e["digest"]
+                    if name == "join":
+                        ok = ok and workload.ordered_pair_digest(*got) == e["ordered_digest"]
+                    else:
+                        key = (got[0].astype(np.uint64) << np.uint64(32)) | got[1]
+                        ok = ok and bool(np.all(key[1:] > key[:-1]))
+                    check = {"checked_against": "oracle fixture (rows, digests, order)"}
+                del got
+                if not ok:
+                    print(json.dumps({"op": name, "rows": 1 << k, "keys": mode_name, "error": "result differs"}),
+                          flush=True)
+                    raise SystemExit(1)
                 r = opbench.time_op(op, args.samples)
-                r.update({"keys": "zipf1.1" if mode == workload.KEY_ZIPF else "uniform",
-                          "frac_of_hbm_peak": round(r["gbs"] / HBM_PEAK_GBS, 4)})
+                r.update({"keys": mode_name, "frac_of_hbm_peak": round(r["gbs"] / HBM_PEAK_GBS, 4), "checked": True})
+                r.update(check)
                 print(json.dumps(r), flush=True)
-                del op
+                del op, host
                 torch.cuda.empty_cache()
 
 
-def run_batched_mode(args) -> None:
-    """Batched fixpoints: 4,096 instances (nodes log-uniform on [16, 16384], Poisson
-    out-degree mean uniform on [1.2, 3.0]) in one semi-naive loop per HBM
-    wave, against one executor run per instance."""
+def batched_summary(budget_pairs: float, plain_stride: int, full_log: bool = False) -> dict:
+    """Batched fixpoints: 4,096 instances (nodes log-uniform on [16, 16384],
+    Poisson out-degree mean uniform on [1.2, 3.0]) in one semi-naive loop per
+    HBM wave, against one executor run per instance (every plain_stride-th
+    instance timed, the total scaled from them)."""
     import numpy as np
     import torch
 
@@ -102,33 +188,43 @@ def run_batched_mode(args) -> None:
     # a cold pass first: its waves grow the store's workspaces and pools
     # (multi-GB allocations), then the measured pass on warm buffers, as the
     # plain path after it runs
-    cold = batch.run_batched(s, d, offs, counts, store, budget_pairs=args.budget_pairs)
-    res = batch.run_batched(s, d, offs, counts, store, budget_pairs=args.budget_pairs)
+    cold = batch.run_batched(s, d, offs, counts, store, budget_pairs=budget_pairs)
+    res = batch.run_batched(s, d, offs, counts, store, budget_pairs=budget_pairs)
     if not (np.array_equal(cold.rows, res.rows) and np.array_equal(cold.digests, res.digests)):
         raise SystemExit("batched passes disagree")
-    picks = list(range(0, counts.size, args.plain_stride))
+    picks = list(range(0, counts.size, plain_stride))
     plain = batch.run_plain(s, d, offs, counts, store, instances=picks)
     same = bool(np.array_equal(plain.rows[picks], res.rows[picks]) and
                 np.array_equal(plain.digests[picks], res.digests[picks]))
     scale = counts.size / len(picks)
     tuples = int(res.rows.sum())
-    print(json.dumps({
-        "mode": "batched", "instances": int(counts.size), "edges": int(s.size), "result_tuples": tuples,
-        "waves": len(res.waves), "batched_s": round(res.seconds, 3), "batched_s_cold": round(cold.seconds, 3),
-        "batched_tuples_per_s": round(tuples / res.seconds, 1),
-        "plain_s_measured": round(plain.seconds, 3), "plain_instances_measured": len(picks),
-        "plain_s_estimated_all": round(plain.seconds * scale, 3),
-        "speedup_vs_plain": round(plain.seconds * scale / res.seconds, 3),
-        "plain_results_equal": same, "max_iterations": int(res.iterations.max()), "wave_log": res.waves}),
-        flush=True)
+    out = {"instances": int(counts.size), "edges": int(s.size), "result_tuples": tuples,
+           "waves": len(res.waves), "seconds": round(res.seconds, 3), "seconds_cold": round(cold.seconds, 3),
+           "tuples_per_s": round(tuples / res.seconds, 1),
+           "plain_seconds_measured": round(plain.seconds, 3), "plain_instances_measured": len(picks),
+           "plain_seconds_estimated_all": round(plain.seconds * scale, 3),
+           "plain_tuples_per_s": round(tuples / (plain.seconds * scale), 1),
+           "speedup_vs_plain": round(plain.seconds * scale / res.seconds, 3),
+           "results_equal_plain": same, "max_iterations": int(res.iterations.max())}
+    if full_log:
+        out["wave_log"] = res.waves
+    del store
+    torch.cuda.empty_cache()
+    return out
 
 
-def run_served_mode(args) -> None:
+def run_batched_mode(args) -> None:
+    r = batched_summary(args.budget_pairs, args.plain_stride, full_log=True)
+    r["mode"] = "batched"
+    print(json.dumps(r), flush=True)
+
+
+def served_summary(budget_pairs: float, requests: int, utilization: float, lane_bounds: str, lane_slots: str,
+                   one_lane: bool = True) -> dict:
     """Served fixpoints: a Poisson stream of TC requests (edges log-uniform
-    on [2^8, 2^20]); continuous batching in size-class lanes (and in one
-    lane) against sequential plain execution of the same stream, at an
-    arrival rate giving the plain path the requested utilisation."""
-    import numpy as np
+    on [2^8, 2^20]) at an arrival rate giving the plain path the requested
+    utilisation; continuous batching in size-class lanes (and, optionally,
+    in one lane) against sequential plain execution of the same stream."""
     import torch
 
     from sparkalog_amd import serving
@@ -142,22 +238,133 @@ def run_served_mode(args) -> None:
     serving.serve_plain(calib[:2], store)  # warm-up
     done, secs = serving.serve_plain(calib, store)
     mean_service = secs / len(calib)
-    rate = args.utilization / mean_service
-    reqs = serving.make_requests(args.requests, rate_per_s=rate)
-    bounds = [float(b) for b in args.lane_bounds.split(",")] if args.lane_bounds else []
+    rate = utilization / mean_service
+    reqs = serving.make_requests(requests, rate_per_s=rate)
+    bounds = [float(b) for b in lane_bounds.split(",")] if lane_bounds else []
     lanes = [RelationStore(HipStream(torch.cuda.Stream())) for _ in range(len(bounds) + 1)]
-    slots = [int(x) for x in args.lane_slots.split(",")] if args.lane_slots else None
-    serving.serve_lanes(reqs[:6], lanes, bounds, budget_pairs=args.budget_pairs, lane_slots=slots)  # warm-up
-    mixed, t_mixed = serving.serve_lanes(reqs, lanes, bounds, budget_pairs=args.budget_pairs, lane_slots=slots)
-    one, t_one = serving.serve_batched(reqs, store, budget_pairs=args.budget_pairs)
+    slots = [int(x) for x in lane_slots.split(",")] if lane_slots else None
+    serving.serve_lanes(reqs[:6], lanes, bounds, budget_pairs=budget_pairs, lane_slots=slots)  # warm-up
+    mixed, t_mixed = serving.serve_lanes(reqs, lanes, bounds, budget_pairs=budget_pairs, lane_slots=slots)
     alone, t_alone = serving.serve_plain(reqs, store)
-    equal = all((m.rows, m.digest) == (a.rows, a.digest) for m, a in zip(mixed, alone)) and \
-        all((m.rows, m.digest) == (a.rows, a.digest) for m, a in zip(one, alone))
-    print(json.dumps({"mode": "served", "rate_per_s": round(rate, 3), "mean_plain_service_ms": round(1000 * mean_service, 3),
-                      "lane_bounds_pairs": bounds, "lane_slots": slots,
-                      "batched_lanes": serving.summarize(mixed, t_mixed),
-                      "batched_one_lane": serving.summarize(one, t_one), "plain": serving.summarize(alone, t_alone),
-                      "results_equal_alone_vs_mix": equal}), flush=True)
+    equal = all((m.rows, m.digest) == (a.rows, a.digest) for m, a in zip(mixed, alone))
+    out = {"utilization": utilization, "rate_per_s": round(rate, 3), "mean_plain_service_ms": round(1000 * mean_service, 3),
+           "lane_bounds_pairs": bounds, "lane_slots": slots,
+           "lanes": serving.summarize(mixed, t_mixed), "plain": serving.summarize(alone, t_alone)}
+    if one_lane:
+        one, t_one = serving.serve_batched(reqs, store, budget_pairs=budget_pairs)
+        equal = equal and all((m.rows, m.digest) == (a.rows, a.digest) for m, a in zip(one, alone))
+        out["one_lane"] = serving.summarize(one, t_one)
+    out["results_equal_alone_vs_mix"] = equal
+    del store, lanes
+    torch.cuda.empty_cache()
+    return out
+
+
+def run_served_mode(args) -> None:
+    r = served_summary(args.budget_pairs, args.requests, args.utilization, args.lane_bounds, args.lane_slots)
+    r["mode"] = "served"
+    print(json.dumps(r), flush=True)
+
+
+def _median(xs):
+    xs = sorted(xs)
+    return xs[(len(xs) - 1) // 2]
+
+
+def run_crossover(args) -> None:
+    """Placement crossover (the methodology of the reference's
+    crates/sparkalog-cli/src/bin/*-crossover.rs): for each operator and row
+    count, the oracle's serial and parallel CPU operators (the reference's
+    CPU code restated; index built outside the timing), the HIP operator
+    through the C ABI with its input resident in HBM (wall time of the call
+    and its synchronization, as the reference times `*_cuda` + wait), and
+    the PCIe copies of the input in either direction (pinned host memory).
+    Every GPU result is checked against the oracle first. Prints one JSON
+    line per case and a final line with the policies derived for MI355X
+    (sparkalog_amd.placement.derive)."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    import oracle
+    from sparkalog_amd import opbench, placement, workload
+
+    torch.cuda.set_device(0)
+    threads = min(_cpu_threads(), CPU_SHARE)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    table = {}
+    for name in args.crossover_ops.split(","):
+        rows_list = [1 << k for k in range(args.crossover_min_log2, args.crossover_max_log2 + 1, 2)]
+        t = {k: [] for k in ("rows", "cpu_serial_ms", "cpu_parallel_ms", "gpu_ms", "h2d_ms", "d2h_ms")}
+        for n in rows_list:
+            host = opbench.host_inputs(name, n, workload.KEY_UNIFORM)
+            op = opbench.make(name, n, workload.KEY_UNIFORM, host=host)
+            if name == "join":
+                op.sync_after_count = True  # join_cuda reads the total before the emit (lib.rs:1608)
+                index = oracle.range_index(host["right"][0])
+
+            def cpu(th):
+                if name == "filter":
+                    return oracle.filter_rows(host["values"], op.OP_GE, op.OPERAND, threads=th)
+                if name == "join":
+                    return oracle.join(host["left"], host["right"], 1, 0, ((0, 0), (1, 1)), threads=th, index=index)
+                return oracle_operator(name, host, th)
+
+            op.launch(stream)
+            torch.cuda.synchronize()
+            got = op.result()
+            exp = cpu(threads)
+            ok = np.array_equal(got, exp) if name == "filter" else \
+                (np.array_equal(got[0], exp[0]) and np.array_equal(got[1], exp[1]))
+            if not ok:
+                print(json.dumps({"op": name, "rows": n, "error": "result differs"}), flush=True)
+                raise SystemExit(1)
+            samples = args.samples if n <= (1 << 20) else 3
+            ser, par, gpu, h2d, d2h = [], [], [], [], []
+            for _ in range(samples):
+                t0 = time.perf_counter()
+                cpu(1)
+                ser.append(1000 * (time.perf_counter() - t0))
+                t0 = time.perf_counter()
+                cpu(threads)
+                par.append(1000 * (time.perf_counter() - t0))
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                op.launch(stream)
+                torch.cuda.synchronize()
+                gpu.append(1000 * (time.perf_counter() - t0))
+            # the input's bytes across PCIe, from and to pinned host memory
+            nbytes = int(opbench.input_bytes(name, host))
+            pinned = torch.empty(max(nbytes // 4, 1), dtype=torch.int32, pin_memory=True)
+            devbuf = torch.empty(max(nbytes // 4, 1), dtype=torch.int32, device="cuda")
+            for _ in range(samples):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                devbuf.copy_(pinned, non_blocking=True)
+                torch.cuda.synchronize()
+                h2d.append(1000 * (time.perf_counter() - t0))
+                t0 = time.perf_counter()
+                pinned.copy_(devbuf, non_blocking=True)
+                torch.cuda.synchronize()
+                d2h.append(1000 * (time.perf_counter() - t0))
+            measure = n + (op.right_rows if name == "union" else 0)
+            row = {"op": name, "rows": n, "policy_rows": measure, "checked": True,
+                   "cpu_serial_ms": round(_median(ser), 4), "cpu_parallel_ms": round(_median(par), 4),
+                   "cpu_threads": threads, "gpu_ms": round(_median(gpu), 4), "h2d_ms": round(_median(h2d), 4),
+                   "d2h_ms": round(_median(d2h), 4), "input_bytes": nbytes}
+            print(json.dumps(row), flush=True)
+            t["rows"].append(measure)
+            for k in ("cpu_serial_ms", "cpu_parallel_ms", "gpu_ms", "h2d_ms", "d2h_ms"):
+                t[k].append(row[k])
+            del op, host, got, exp, pinned, devbuf
+            torch.cuda.empty_cache()
+        table[name] = t
+    derived = placement.derive(table)
+    print(json.dumps({"mode": "crossover", "cpu_threads": threads, "cpu_model": _cpu_model(),
+                      "nproc": os.cpu_count(),
+                      "derived_mi355x": {k: vars(v) for k, v in derived.items()},
+                      "gb10": {k: vars(v) for k, v in placement.MEASURED_GB10.items()}}), flush=True)
 
 
 def run_copy_mode(args) -> None:
@@ -190,14 +397,18 @@ def run_copy_mode(args) -> None:
           flush=True)
 
 
-def cpu_baseline(sample_nodes: int) -> dict:
-    """The oracle's parallel semi-naive executor (Rayon-style chunking) on a
-    smaller draw of the same component-graph distribution."""
+# The box's CPU share for one GPU: worker pools are sized to it whatever
+# nproc reports there.
+CPU_SHARE = 16
+
+
+def _oracle_tc(nodes: int, threads: int) -> tuple[float, int, int, int]:
+    """One oracle semi-naive fixpoint of the component graph of `nodes`
+    nodes: (seconds, edges, result tuples, iterations)."""
     import oracle
     from sparkalog_amd import plans, workload
 
-    threads = min(16, os.cpu_count() or 1)
-    s, d, _ = workload.component_graph(nodes=sample_nodes)
+    s, d, _ = workload.component_graph(nodes=nodes)
     store = oracle.Store()
     store.insert_static(0, (s, d))
     store.insert_recursive(1, (s, d), (s, d))
@@ -208,10 +419,27 @@ def cpu_baseline(sample_nodes: int) -> dict:
     dt = time.perf_counter() - t0
     if rc != 0:
         raise RuntimeError(f"oracle run failed with {rc}")
-    rows = store.get(1)[0].size
-    return {"value": rows / dt, "unit": "tuples/s", "cores": threads, "kind": "port",
-            "sample": f"component graph of {sample_nodes} nodes (same generator, seed 0xC2; {s.size} edges, "
-                      f"{rows} result tuples, {summary.iterations} iterations) in {dt:.2f} s"}
+    return dt, int(s.size), int(store.get(1)[0].size), int(summary.iterations)
+
+
+def cpu_baseline(sample_nodes: int, serial_nodes: int) -> dict:
+    """The oracle's semi-naive executor (the reference's CPU operators and
+    RecursiveExecutor restated in C) on this host: parallel (Rayon-style
+    chunks, one worker per core of the GPU's CPU share) on a draw of
+    `sample_nodes` nodes of the same component-graph distribution, and serial
+    on `serial_nodes` nodes. The full 2^20-node graph (1.03e9 tuples) would
+    take minutes on either, so bounded samples stand in for it."""
+    nproc = os.cpu_count() or 1
+    threads = min(_cpu_threads(), CPU_SHARE)
+    dt, edges, rows, iters = _oracle_tc(sample_nodes, threads)
+    sdt, sedges, srows, siters = _oracle_tc(serial_nodes, 1)
+    return {"value": round(rows / dt, 1), "unit": "tuples/s", "cores": threads, "kind": "port",
+            "sample": f"parallel: component graph of {sample_nodes} nodes (same generator, seed 0xC2; {edges} "
+                      f"edges, {rows} result tuples, {iters} iterations) in {dt:.2f} s on {threads} threads",
+            "serial": {"value": round(srows / sdt, 1), "unit": "tuples/s", "cores": 1,
+                       "sample": f"component graph of {serial_nodes} nodes ({sedges} edges, {srows} result "
+                                 f"tuples, {siters} iterations) in {sdt:.2f} s"},
+            "host": {"nproc": nproc, "affinity": _cpu_threads(), "cpu_share_used": threads, "model": _cpu_model()}}
 
 
 def main():
@@ -231,6 +459,9 @@ def main():
         return
     if args.mode == "copy":
         run_copy_mode(args)
+        return
+    if args.mode == "crossover":
+        run_crossover(args)
         return
     import numpy as np
     import torch
@@ -349,8 +580,16 @@ def main():
                            "readback_s": round(t2 - t1, 4), "rows": int(host_full[0].size),
                            "tuples_per_s_pcie_inclusive": round(host_full[0].size / (t2 - t0), 1)}
         del host_full
+    if not args.no_batched:
+        # the batched configuration (4,096 instances) beside the plain path
+        del store
+        torch.cuda.empty_cache()
+        line["batched"] = batched_summary(args.budget_pairs, args.bench_plain_stride)
+    if not args.no_served:
+        line["served"] = served_summary(args.budget_pairs, args.requests, args.utilization, args.lane_bounds,
+                                        args.lane_slots, one_lane=False)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(args.cpu_sample_nodes)
+        line["cpu_baseline"] = cpu_baseline(args.cpu_sample_nodes, args.cpu_serial_nodes)
     print(json.dumps(line), flush=True)
 
 

@@ -86,10 +86,13 @@ void sparkalog_wl_distinct_input(uint64_t seed, uint64_t rows, double unique_fra
 
 /*
  * Two sorted unique relations of `rows` pairs each that share
- * round(overlap*rows) pairs: a pool of 2*rows - shared unique pairs is drawn,
- * shuffled, split, and each side sorted.
+ * round(overlap*rows) pairs: a pool of 2*rows - shared unique pairs (first
+ * column uniform on [0, key_space) for key_mode 0, Zipf(zipf_s) ranks for
+ * key_mode 1; second column uniform) is drawn, shuffled, split, and each side
+ * sorted.
  */
-void sparkalog_wl_overlapping_sets(uint64_t seed, uint64_t rows, double overlap,
+void sparkalog_wl_overlapping_sets(uint64_t seed, uint64_t rows, double overlap, int key_mode,
+                                   uint32_t key_space, double zipf_s,
                                    uint32_t* left_first, uint32_t* left_second,
                                    uint32_t* right_first, uint32_t* right_second);
 

@@ -64,7 +64,7 @@ def lib() -> ctypes.CDLL:
         L.oracle_range_index.restype = c_uint64
         L.oracle_range_index.argtypes = [u32, c_uint64, u32, u32, u32]
         L.oracle_join_count.restype = c_uint64
-        L.oracle_join_count.argtypes = [u32, c_uint64, u32, u32, c_uint64]
+        L.oracle_join_count.argtypes = [u32, c_uint64, u32, u32, c_uint64, c_int]
         L.oracle_join.restype = c_uint64
         L.oracle_join.argtypes = [u32, c_uint64, u32, u32, u32, c_uint64, u32, c_uint32, u32, c_uint32, u32, u32, c_int]
         L.oracle_filter.restype = c_uint64
@@ -131,14 +131,17 @@ def range_index(values):
     return keys[:k], starts[: k + 1], rows[: v.size]
 
 
-def join(left, right, left_key: int, right_key: int, output, threads: int = 1):
-    """output = ((side, column), (side, column)); side 0 left, 1 right."""
+def join(left, right, left_key: int, right_key: int, output, threads: int = 1, index=None):
+    """output = ((side, column), (side, column)); side 0 left, 1 right.
+    `index` = range_index(right[right_key]) when already built (the
+    reference's crossover harness builds it outside the timed join,
+    join-crossover.rs:127-133)."""
     left = (_u32(left[0]), _u32(left[1]))
     right = (_u32(right[0]), _u32(right[1]))
-    keys, starts, rows = range_index(right[right_key])
+    keys, starts, rows = index if index is not None else range_index(right[right_key])
     lk = left[left_key]
     L = lib()
-    m = L.oracle_join_count(_p(lk), lk.size, _p(keys), starts.ctypes.data_as(c_void_p), keys.size)
+    m = L.oracle_join_count(_p(lk), lk.size, _p(keys), starts.ctypes.data_as(c_void_p), keys.size, threads)
     o0, o1 = np.empty(max(m, 1), np.uint32), np.empty(max(m, 1), np.uint32)
     p = [(left if side == 0 else right)[col] for side, col in output]
     L.oracle_join(_p(lk), lk.size, _p(keys), starts.ctypes.data_as(c_void_p), _p(rows), keys.size, _p(p[0]),
@@ -146,10 +149,10 @@ def join(left, right, left_key: int, right_key: int, output, threads: int = 1):
     return o0[:m], o1[:m]
 
 
-def filter_rows(values, op: int, operand: int):
+def filter_rows(values, op: int, operand: int, threads: int = 1):
     v = _u32(values)
     out = np.empty(max(v.size, 1), np.uint32)
-    n = lib().oracle_filter(_p(v), v.size, op, operand, _p(out), 1)
+    n = lib().oracle_filter(_p(v), v.size, op, operand, _p(out), threads)
     return out[:n]
 
 

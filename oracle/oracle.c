@@ -415,14 +415,46 @@ static int64_t find_key(const uint32_t* keys, uint64_t k, uint32_t key) {
     return (low < k && keys[low] == key) ? (int64_t)low : -1;
 }
 
-uint64_t oracle_join_count(const uint32_t* left_keys, uint64_t l, const uint32_t* index_keys,
-                           const uint32_t* index_starts, uint64_t k) {
+/* count_join_rows, execution/src/lib.rs:1309-1337: per-row match counts
+ * (in parallel chunks when threads > 1, as the par_iter of :1317-1320), then
+ * a serial exclusive prefix into offsets (when non-null); returns the total. */
+typedef struct {
+    const uint32_t *left_keys, *keys, *starts;
+    uint64_t l, k, chunk;
+    uint64_t* counts;
+} count_ctx;
+
+static void join_count_worker(void* p, int w, int ws) {
+    count_ctx* c = (count_ctx*)p;
+    uint64_t chunks = div_ceil(c->l, c->chunk);
+    for (uint64_t ch = (uint64_t)w; ch < chunks; ch += (uint64_t)ws) {
+        uint64_t b = ch * c->chunk, e = b + c->chunk < c->l ? b + c->chunk : c->l;
+        for (uint64_t i = b; i < e; ++i) {
+            int64_t idx = find_key(c->keys, c->k, c->left_keys[i]);
+            c->counts[i] = idx >= 0 ? (uint64_t)(c->starts[idx + 1] - c->starts[idx]) : 0;
+        }
+    }
+}
+
+static uint64_t count_join_rows(const uint32_t* left_keys, uint64_t l, const uint32_t* index_keys,
+                                const uint32_t* index_starts, uint64_t k, uint64_t* offsets, int threads) {
+    if (l == 0) return 0;
+    uint64_t* counts = (uint64_t*)malloc(l * sizeof(uint64_t));
+    int ws = clamp_workers(threads, l);
+    count_ctx c = {left_keys, index_keys, index_starts, l, k, div_ceil(l, (uint64_t)ws), counts};
+    run_workers(ws, join_count_worker, &c);
     uint64_t total = 0;
     for (uint64_t i = 0; i < l; ++i) {
-        int64_t idx = find_key(index_keys, k, left_keys[i]);
-        if (idx >= 0) total += (uint64_t)(index_starts[idx + 1] - index_starts[idx]);
+        if (offsets) offsets[i] = total;
+        total += counts[i];
     }
+    free(counts);
     return total;
+}
+
+uint64_t oracle_join_count(const uint32_t* left_keys, uint64_t l, const uint32_t* index_keys,
+                           const uint32_t* index_starts, uint64_t k, int threads) {
+    return count_join_rows(left_keys, l, index_keys, index_starts, k, NULL, threads);
 }
 
 typedef struct {
@@ -456,14 +488,8 @@ uint64_t oracle_join(const uint32_t* left_keys, uint64_t l, const uint32_t* inde
                      const uint32_t* projection0, uint32_t side0, const uint32_t* projection1,
                      uint32_t side1, uint32_t* out0, uint32_t* out1, int threads) {
     if (l == 0) return 0;
-    /* count_join_rows, execution/src/lib.rs:1309-1337 */
     uint64_t* offsets = (uint64_t*)malloc(l * sizeof(uint64_t));
-    uint64_t total = 0;
-    for (uint64_t i = 0; i < l; ++i) {
-        offsets[i] = total;
-        int64_t idx = find_key(index_keys, k, left_keys[i]);
-        if (idx >= 0) total += (uint64_t)(index_starts[idx + 1] - index_starts[idx]);
-    }
+    uint64_t total = count_join_rows(left_keys, l, index_keys, index_starts, k, offsets, threads);
     int ws = clamp_workers(threads, l);
     join_ctx c = {left_keys, index_keys, index_starts, index_rows, projection0, projection1,
                   side0,     side1,      l,            k,          div_ceil(l, (uint64_t)ws),
@@ -681,7 +707,7 @@ static int rule_evaluate(rule_rt* rt, oracle_store* s, int threads) {
     const uint32_t* proj[2];
     for (int i = 0; i < 2; ++i)
         if (pick(p->out_input[i] == 0 ? delta : right, p->out_column[i], &proj[i])) return ORACLE_MISSING_COLUMN;
-    uint64_t m = oracle_join_count(left_key, delta->len, rt->index.keys, rt->index.starts, rt->index.k);
+    uint64_t m = oracle_join_count(left_key, delta->len, rt->index.keys, rt->index.starts, rt->index.k, threads);
     if (orel_reserve(&rt->candidates, m)) return ORACLE_NO_MEMORY;
     rt->candidates.len = oracle_join(left_key, delta->len, rt->index.keys, rt->index.starts,
                                      rt->index.rows, rt->index.k, proj[0], p->out_input[0], proj[1],

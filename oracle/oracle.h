@@ -58,9 +58,10 @@ uint64_t oracle_union(const uint32_t* lf, const uint32_t* ls, uint64_t l,
 uint64_t oracle_range_index(const uint32_t* values, uint64_t n, uint32_t* keys,
                             uint32_t* starts, uint32_t* rows);
 
-/* execution/src/lib.rs:1315-1337 + :1340-1367: the join's exact output size. */
+/* execution/src/lib.rs:1309-1337 (count_join_rows): the join's exact output
+ * size, per-row counts in parallel chunks when threads > 1. */
 uint64_t oracle_join_count(const uint32_t* left_keys, uint64_t l, const uint32_t* index_keys,
-                           const uint32_t* index_starts, uint64_t k);
+                           const uint32_t* index_starts, uint64_t k, int threads);
 
 /*
  * execution/src/lib.rs:1340-1367 (serial) and :1370-1423 (parallel).

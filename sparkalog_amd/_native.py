@@ -98,7 +98,8 @@ _WORKLOAD_SIGNATURES = {
     "sparkalog_wl_batch_graphs": (c_uint64, [c_uint64, c_uint32, c_uint32, c_uint32, c_double, c_double, POINTER(u32p), POINTER(u32p), POINTER(u64p), POINTER(u32p)]),
     "sparkalog_wl_pairs": (None, [c_uint64, c_uint64, c_int, c_uint32, c_double, c_void_p, c_void_p]),
     "sparkalog_wl_distinct_input": (None, [c_uint64, c_uint64, c_double, c_int, c_uint32, c_double, c_void_p, c_void_p]),
-    "sparkalog_wl_overlapping_sets": (None, [c_uint64, c_uint64, c_double, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "sparkalog_wl_overlapping_sets": (None, [c_uint64, c_uint64, c_double, c_int, c_uint32, c_double, c_void_p,
+                                             c_void_p, c_void_p, c_void_p]),
     "sparkalog_wl_pair_digest": (c_uint64, [c_void_p, c_void_p, c_uint64]),
     "sparkalog_wl_packed_digest": (c_uint64, [c_void_p, c_uint64]),
     "sparkalog_wl_mix64": (c_uint64, [c_uint64]),

@@ -55,7 +55,7 @@ def build_workload() -> Path:
     OUT.mkdir(exist_ok=True)
     target = OUT / "libsparkalog_workload.so"
     tmp = OUT / "libsparkalog_workload.so.tmp"
-    _run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", f"-I{INCLUDE}",
+    _run(["g++", "-O2", "-std=c++17", "-fopenmp", "-shared", "-fPIC", f"-I{INCLUDE}",
           str(CSRC / "workload.cpp"), "-o", str(tmp), "-lpthread"])
     tmp.replace(target)
     return target

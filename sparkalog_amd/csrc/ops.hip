@@ -1017,8 +1017,6 @@ __global__ __launch_bounds__(kBlock) void join_emit_kernel(const uint32_t* left_
     __shared__ size_t bounds[2];
     __shared__ uint64_t total_slot;
     __shared__ uint32_t key_sample[kKeySample];
-    const size_t stride = key_sample_stride(k);
-    if (k > 0) load_key_sample(index_keys, k, stride, key_sample);  // block-uniform
     if (threadIdx.x == 0) {
         const int64_t idx = find_index_key(index_keys, k, left_keys[nl - 1]);
         const uint64_t last = idx >= 0 ? index_starts[idx + 1] - index_starts[idx] : 0;
@@ -1027,6 +1025,11 @@ __global__ __launch_bounds__(kBlock) void join_emit_kernel(const uint32_t* left_
     __syncthreads();
     const uint64_t total = total_slot;
     const uint64_t tiles = (total + kEmitTile - 1) / kEmitTile;
+    // a block with no output tile leaves before loading the key sample (the
+    // test is block-uniform, so no barrier below is left half-reached)
+    if (blockIdx.x >= tiles) return;
+    const size_t stride = key_sample_stride(k);
+    if (k > 0) load_key_sample(index_keys, k, stride, key_sample);  // block-uniform
     for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         const uint64_t o0 = tile * kEmitTile;
         const uint64_t o1 = o0 + kEmitTile < total ? o0 + kEmitTile : total;

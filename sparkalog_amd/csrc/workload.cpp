@@ -5,6 +5,7 @@
 #include "sparkalog_workload.h"
 
 #include <algorithm>
+#include <parallel/algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -171,7 +172,7 @@ std::vector<uint64_t> draw_unique_pairs(uint64_t seed, uint64_t count, int key_m
                                         uint32_t key_space, double zipf_s) {
     std::vector<uint64_t> pool(count);
     draw_pairs(seed, count, key_mode, key_space, zipf_s, pool.data());
-    std::sort(pool.begin(), pool.end());
+    __gnu_parallel::sort(pool.begin(), pool.end());
     pool.erase(std::unique(pool.begin(), pool.end()), pool.end());
     uint64_t refill = finalize(seed ^ 0xF111ull);
     ZipfSampler zipf(zipf_s > 0 ? zipf_s : 1.1, static_cast<double>(key_space));
@@ -184,7 +185,7 @@ std::vector<uint64_t> draw_unique_pairs(uint64_t seed, uint64_t count, int key_m
             e = pack(key, static_cast<uint32_t>(splitmix(refill) >> 32));
         }
         pool.insert(pool.end(), extra.begin(), extra.end());
-        std::sort(pool.begin(), pool.end());
+        __gnu_parallel::sort(pool.begin(), pool.end());
         pool.erase(std::unique(pool.begin(), pool.end()), pool.end());
     }
     return pool;
@@ -195,6 +196,23 @@ void fisher_yates(uint64_t& st, uint64_t* data, uint64_t n) {
         const uint64_t j = bounded64(st, i);
         std::swap(data[i - 1], data[j]);
     }
+}
+
+// A random permutation for operator inputs of up to 2^28 rows, in parallel:
+// position i gets the tag finalize(seed ^ i * kGolden) (distinct for every i,
+// as both maps are bijections of u64), and the rows are sorted by tag.
+void keyed_shuffle(uint64_t seed, uint64_t* data, uint64_t n) {
+    std::vector<std::pair<uint64_t, uint64_t>> tagged(n);
+    const uint64_t chunks = (n + kChunkRows - 1) / kChunkRows;
+    parallel_chunks(chunks, [&](uint64_t c) {
+        const uint64_t end = std::min(n, (c + 1) * kChunkRows);
+        for (uint64_t i = c * kChunkRows; i < end; ++i) tagged[i] = {finalize(seed ^ (i * kGolden)), data[i]};
+    });
+    __gnu_parallel::sort(tagged.begin(), tagged.end());
+    parallel_chunks(chunks, [&](uint64_t c) {
+        const uint64_t end = std::min(n, (c + 1) * kChunkRows);
+        for (uint64_t i = c * kChunkRows; i < end; ++i) data[i] = tagged[i].second;
+    });
 }
 
 void split_pairs(const uint64_t* packed, uint64_t rows, uint32_t* first, uint32_t* second) {
@@ -287,7 +305,7 @@ extern "C" uint64_t sparkalog_wl_component_graph(uint64_t seed, uint32_t nodes, 
     for (auto& e : edges) {
         e = pack(static_cast<uint32_t>(perm[e >> 32]), static_cast<uint32_t>(perm[e & 0xFFFFFFFFull]));
     }
-    std::sort(edges.begin(), edges.end());
+    __gnu_parallel::sort(edges.begin(), edges.end());
     edges.erase(std::unique(edges.begin(), edges.end()), edges.end());
     std::vector<uint32_t> a(edges.size()), b(edges.size());
     split_pairs(edges.data(), edges.size(), a.data(), b.data());
@@ -358,28 +376,45 @@ extern "C" void sparkalog_wl_distinct_input(uint64_t seed, uint64_t rows, double
     pool.resize(rows);
     uint64_t st = finalize(seed ^ 0xD0B1Eull);
     for (uint64_t r = unique; r < rows; ++r) pool[r] = pool[bounded64(st, unique)];
-    fisher_yates(st, pool.data(), rows);
+    keyed_shuffle(finalize(st), pool.data(), rows);
     split_pairs(pool.data(), rows, first, second);
 }
 
-extern "C" void sparkalog_wl_overlapping_sets(uint64_t seed, uint64_t rows, double overlap,
+extern "C" void sparkalog_wl_overlapping_sets(uint64_t seed, uint64_t rows, double overlap, int key_mode,
+                                              uint32_t key_space, double zipf_s,
                                               uint32_t* left_first, uint32_t* left_second,
                                               uint32_t* right_first, uint32_t* right_second) {
     if (rows == 0) return;
     uint64_t shared = static_cast<uint64_t>(std::llround(overlap * static_cast<double>(rows)));
     if (shared > rows) shared = rows;
     const uint64_t total = 2 * rows - shared;
-    std::vector<uint64_t> pool = draw_unique_pairs(seed, total, 0, 1u << 24, 0.0);
+    const std::vector<uint64_t> pool = draw_unique_pairs(seed, total, key_mode, key_space, zipf_s);
+    // One pass over the sorted pool assigns every pair to both sides, the
+    // left only or the right only, with probabilities proportional to the
+    // quotas still open (sequential selection sampling, Knuth's Algorithm S):
+    // exact counts, a uniformly random split, and both sides come out sorted.
     uint64_t st = finalize(seed ^ 0x0E71Aull);
-    fisher_yates(st, pool.data(), total);
-    // pool[0, shared) goes to both sides; the rest is split evenly.
-    std::vector<uint64_t> left(pool.begin(), pool.begin() + rows);
-    std::vector<uint64_t> right(pool.begin(), pool.begin() + shared);
-    right.insert(right.end(), pool.begin() + rows, pool.end());
-    std::sort(left.begin(), left.end());
-    std::sort(right.begin(), right.end());
-    split_pairs(left.data(), rows, left_first, left_second);
-    split_pairs(right.data(), rows, right_first, right_second);
+    uint64_t open_shared = shared, open_left = rows - shared, open_right = rows - shared;
+    uint64_t nl = 0, nr = 0;
+    for (uint64_t i = 0; i < total; ++i) {
+        const uint64_t u = bounded64(st, open_shared + open_left + open_right);
+        const uint64_t k = pool[i];
+        if (u < open_shared) {
+            --open_shared;
+            left_first[nl] = static_cast<uint32_t>(k >> 32);
+            left_second[nl++] = static_cast<uint32_t>(k);
+            right_first[nr] = static_cast<uint32_t>(k >> 32);
+            right_second[nr++] = static_cast<uint32_t>(k);
+        } else if (u < open_shared + open_left) {
+            --open_left;
+            left_first[nl] = static_cast<uint32_t>(k >> 32);
+            left_second[nl++] = static_cast<uint32_t>(k);
+        } else {
+            --open_right;
+            right_first[nr] = static_cast<uint32_t>(k >> 32);
+            right_second[nr++] = static_cast<uint32_t>(k);
+        }
+    }
 }
 
 extern "C" uint64_t sparkalog_wl_pair_digest(const uint32_t* first, const uint32_t* second,

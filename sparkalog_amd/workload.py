@@ -116,10 +116,12 @@ def distinct_input(seed: int, rows: int, unique_fraction: float = 0.7, key_mode:
     return a, b
 
 
-def overlapping_sets(seed: int, rows: int, overlap: float = 0.5):
+def overlapping_sets(seed: int, rows: int, overlap: float = 0.5, key_mode: int = KEY_UNIFORM,
+                     key_space: int = 1 << 24, zipf_s: float = 1.1):
     lib = _native.workload()
     arrs = [np.empty(rows, dtype=np.uint32) for _ in range(4)]
-    lib.sparkalog_wl_overlapping_sets(seed, rows, overlap, *[x.ctypes.data_as(ctypes.c_void_p) for x in arrs])
+    lib.sparkalog_wl_overlapping_sets(seed, rows, overlap, key_mode, key_space, zipf_s,
+                                      *[x.ctypes.data_as(ctypes.c_void_p) for x in arrs])
     return (arrs[0], arrs[1]), (arrs[2], arrs[3])
 
 
@@ -128,6 +130,26 @@ def pair_digest(first, second) -> int:
     b = np.ascontiguousarray(second, dtype=np.uint32)
     return _native.workload().sparkalog_wl_pair_digest(a.ctypes.data_as(ctypes.c_void_p),
                                                        b.ctypes.data_as(ctypes.c_void_p), a.size)
+
+
+def ordered_pair_digest(first, second, chunk: int = 1 << 24) -> int:
+    """Order-dependent digest of a pair sequence: the sum over rows i of
+    mix64(((a_i << 32) | b_i) + (i + 1) * 0x9E3779B97F4A7C15) mod 2^64 (mix64
+    as in pair_digest), so a row moved to another position changes it."""
+    a = np.ascontiguousarray(first, dtype=np.uint32)
+    b = np.ascontiguousarray(second, dtype=np.uint32)
+    m1, m2 = np.uint64(0xBF58476D1CE4E5B9), np.uint64(0x94D049BB133111EB)
+    golden = np.uint64(0x9E3779B97F4A7C15)
+    total = np.uint64(0)
+    with np.errstate(over="ignore"):
+        for s in range(0, a.size, chunk):
+            n = min(chunk, a.size - s)
+            z = ((a[s:s + n].astype(np.uint64) << np.uint64(32)) | b[s:s + n].astype(np.uint64)) + \
+                np.arange(s + 1, s + 1 + n, dtype=np.uint64) * golden
+            z = (z ^ (z >> np.uint64(30))) * m1
+            z = (z ^ (z >> np.uint64(27))) * m2
+            total = total + np.sum(z ^ (z >> np.uint64(31)), dtype=np.uint64)
+    return int(total)
 
 
 def mix64(key: int) -> int:

@@ -39,3 +39,37 @@ def random_pairs(rng, n, hi):
 def sorted_unique(columns):
     return tuple(np.asarray(x, dtype=np.uint32) for x in zip(*sorted(set(pairs(columns))))) \
         if len(columns[0]) else (np.empty(0, np.uint32), np.empty(0, np.uint32))
+
+
+def ordered_digest(first, second):
+    """Order-dependent digest of a pair sequence (workload.ordered_pair_digest)."""
+    from sparkalog_amd import workload
+
+    return workload.ordered_pair_digest(first, second)
+
+
+def strictly_sorted(first, second) -> bool:
+    k = packed((first, second))
+    return bool(np.all(k[1:] > k[:-1]))
+
+
+def oracle_operator(name, host, threads):
+    """The oracle's result of one operator microbench case
+    (sparkalog_amd.opbench.host_inputs)."""
+    import oracle
+
+    if name == "distinct":
+        return oracle.distinct(*host["input"], threads=threads)
+    if name == "union":
+        return oracle.union(host["left"], host["right"], threads=threads)
+    if name == "anti_join":
+        return oracle.anti_join(host["left"], host["right"], threads=threads)
+    return oracle.join(host["left"], host["right"], 1, 0, ((0, 0), (1, 1)), threads=threads)
+
+
+def operator_input_digest(name, host):
+    from sparkalog_amd import workload
+
+    if name == "distinct":
+        return int(workload.pair_digest(*host["input"]))
+    return int(sum(workload.pair_digest(*p) for p in (host["left"], host["right"])) % (1 << 64))

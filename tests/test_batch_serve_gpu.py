@@ -90,6 +90,25 @@ def test_served_lanes_equal_alone(stream):
         assert (m.rows, m.digest) == (a.rows, a.digest), req.rid
 
 
+def test_served_requests_up_to_2p20_edges(stream):
+    # the served stream's full edge range: eight requests with edges
+    # log-uniform on [2^8, 2^20] (seed 0x5C21 draws 424 to 1,011,046 edges),
+    # arriving faster than they are served, in three lanes: every result
+    # alone = in the mix = BFS
+    import torch
+    from sparkalog_amd.execution import HipStream
+
+    reqs = serving.make_requests(8, rate_per_s=40.0, seed=0x5C21, min_log2_edges=8, max_log2_edges=20)
+    assert max(r.src.size for r in reqs) > 1_000_000
+    stores = [RelationStore(HipStream(torch.cuda.Stream())) for _ in range(3)]
+    mixed, _ = serving.serve_lanes(reqs, stores, bounds=[1e6, 3e7], budget_pairs=6e9, max_slots=64)
+    alone, _ = serving.serve_plain(reqs, RelationStore(stream))
+    assert [r.rid for r in mixed] == list(range(len(reqs)))
+    for m, a, req in zip(mixed, alone, reqs):
+        bfs = oracle.tc_bfs(req.src, req.dst, req.nodes, threads=8)
+        assert (m.rows, m.digest) == (a.rows, a.digest) == (bfs["pairs"], bfs["digest"]), req.rid
+
+
 def test_remove_ranges_matches_filter(stream):
     rng = np.random.default_rng(0x77)
     a = rng.integers(0, 4000, 20000).astype(np.uint32)

@@ -50,3 +50,26 @@ def test_operator_inputs():
     z, _ = workload.pairs(7, 200000, workload.KEY_ZIPF)
     counts = np.bincount(z[z < 1000], minlength=1000)
     assert counts[0] > counts[10] > counts[500]          # Zipf: rank 1 is the most frequent
+
+
+def test_ordered_pair_digest():
+    rng = np.random.default_rng(3)
+    a = rng.integers(0, 1 << 32, 1000, dtype=np.uint64).astype(np.uint32)
+    b = rng.integers(0, 1 << 32, 1000, dtype=np.uint64).astype(np.uint32)
+    d = workload.ordered_pair_digest(a, b)
+    # one row: the same term as the order-independent digest's
+    assert workload.ordered_pair_digest(a[:1], b[:1]) == workload.pair_digest(a[:1], b[:1])
+    # two rows swapped: a different digest; chunking does not matter
+    a2, b2 = a.copy(), b.copy()
+    a2[[10, 500]], b2[[10, 500]] = a2[[500, 10]], b2[[500, 10]]
+    assert workload.ordered_pair_digest(a2, b2) != d
+    assert workload.ordered_pair_digest(a, b, chunk=7) == d
+
+
+def test_overlapping_sets_with_zipf_keys():
+    (lf, ls), (rf, rs) = workload.overlapping_sets(8, 40000, 0.5, workload.KEY_ZIPF)
+    lk, rk = packed((lf, ls)), packed((rf, rs))
+    assert np.all(lk[1:] > lk[:-1]) and np.all(rk[1:] > rk[:-1])
+    assert np.intersect1d(lk, rk).size == 20000
+    counts = np.bincount(lf[lf < 100], minlength=100)
+    assert counts[0] > counts[50]
