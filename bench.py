@@ -64,7 +64,10 @@ def parse_args():
     p.add_argument("--copy-bytes", type=int, default=1 << 30, help="copy mode: bytes copied per launch")
     p.add_argument("--budget-pairs", type=float, default=3e9, help="batched/served: HBM budget in closure pairs")
     p.add_argument("--requests", type=int, default=200, help="served: stream length")
-    p.add_argument("--utilization", type=float, default=0.7, help="served: offered load of the plain path")
+    p.add_argument("--utilization", default="0.7,1.0,1.5",
+                   help="served mode: offered loads of the plain path (comma list)")
+    p.add_argument("--bench-utilization", default="0.7,1.0",
+                   help="tc: offered loads of the served summary in the bench line")
     p.add_argument("--plain-stride", type=int, default=1, help="batched: time the plain path on every k-th instance")
     p.add_argument("--lane-bounds", default="1e6,3e7",
                    help="served: closure-size bounds (pairs) between continuous-batching lanes")
@@ -219,11 +222,12 @@ def run_batched_mode(args) -> None:
     print(json.dumps(r), flush=True)
 
 
-def served_summary(budget_pairs: float, requests: int, utilization: float, lane_bounds: str, lane_slots: str,
+def served_summary(budget_pairs: float, requests: int, utilizations: str, lane_bounds: str, lane_slots: str,
                    one_lane: bool = True) -> dict:
     """Served fixpoints: a Poisson stream of TC requests (edges log-uniform
-    on [2^8, 2^20]) at an arrival rate giving the plain path the requested
-    utilisation; continuous batching in size-class lanes (and, optionally,
+    on [2^8, 2^20]) at arrival rates giving the plain path each requested
+    utilisation (its mean service time measured on the same requests served
+    back to back); continuous batching in size-class lanes (and, optionally,
     in one lane) against sequential plain execution of the same stream."""
     import torch
 
@@ -233,27 +237,32 @@ def served_summary(budget_pairs: float, requests: int, utilization: float, lane_
 
     torch.cuda.set_device(0)
     store = RelationStore(HipStream())
-    # service time of the plain path on a calibration stream (all at t=0)
-    calib = serving.make_requests(24, rate_per_s=1e9, seed=0xCA1)
-    serving.serve_plain(calib[:2], store)  # warm-up
-    done, secs = serving.serve_plain(calib, store)
-    mean_service = secs / len(calib)
-    rate = utilization / mean_service
-    reqs = serving.make_requests(requests, rate_per_s=rate)
+    # the plain path's service time on these very requests (same seed: the
+    # sizes do not depend on the rate), all present at the start
+    back_to_back = serving.make_requests(requests, rate_per_s=1e9)
+    serving.serve_plain(back_to_back[:4], store)  # warm-up
+    _, secs = serving.serve_plain(back_to_back, store)
+    mean_service = secs / requests
     bounds = [float(b) for b in lane_bounds.split(",")] if lane_bounds else []
     lanes = [RelationStore(HipStream(torch.cuda.Stream())) for _ in range(len(bounds) + 1)]
     slots = [int(x) for x in lane_slots.split(",")] if lane_slots else None
-    serving.serve_lanes(reqs[:6], lanes, bounds, budget_pairs=budget_pairs, lane_slots=slots)  # warm-up
-    mixed, t_mixed = serving.serve_lanes(reqs, lanes, bounds, budget_pairs=budget_pairs, lane_slots=slots)
-    alone, t_alone = serving.serve_plain(reqs, store)
-    equal = all((m.rows, m.digest) == (a.rows, a.digest) for m, a in zip(mixed, alone))
-    out = {"utilization": utilization, "rate_per_s": round(rate, 3), "mean_plain_service_ms": round(1000 * mean_service, 3),
-           "lane_bounds_pairs": bounds, "lane_slots": slots,
-           "lanes": serving.summarize(mixed, t_mixed), "plain": serving.summarize(alone, t_alone)}
-    if one_lane:
-        one, t_one = serving.serve_batched(reqs, store, budget_pairs=budget_pairs)
-        equal = equal and all((m.rows, m.digest) == (a.rows, a.digest) for m, a in zip(one, alone))
-        out["one_lane"] = serving.summarize(one, t_one)
+    serving.serve_lanes(back_to_back[:6], lanes, bounds, budget_pairs=budget_pairs, lane_slots=slots)  # warm-up
+    out = {"requests": requests, "mean_plain_service_ms": round(1000 * mean_service, 3),
+           "lane_bounds_pairs": bounds, "lane_slots": slots, "loads": []}
+    equal = True
+    for u in [float(x) for x in utilizations.split(",")]:
+        rate = u / mean_service
+        reqs = serving.make_requests(requests, rate_per_s=rate)
+        mixed, t_mixed = serving.serve_lanes(reqs, lanes, bounds, budget_pairs=budget_pairs, lane_slots=slots)
+        alone, t_alone = serving.serve_plain(reqs, store)
+        equal = equal and all((m.rows, m.digest) == (a.rows, a.digest) for m, a in zip(mixed, alone))
+        load = {"utilization": u, "rate_per_s": round(rate, 3), "lanes": serving.summarize(mixed, t_mixed),
+                "plain": serving.summarize(alone, t_alone)}
+        if one_lane:
+            one, t_one = serving.serve_batched(reqs, store, budget_pairs=budget_pairs)
+            equal = equal and all((m.rows, m.digest) == (a.rows, a.digest) for m, a in zip(one, alone))
+            load["one_lane"] = serving.summarize(one, t_one)
+        out["loads"].append(load)
     out["results_equal_alone_vs_mix"] = equal
     del store, lanes
     torch.cuda.empty_cache()
@@ -586,7 +595,7 @@ def main():
         torch.cuda.empty_cache()
         line["batched"] = batched_summary(args.budget_pairs, args.bench_plain_stride)
     if not args.no_served:
-        line["served"] = served_summary(args.budget_pairs, args.requests, args.utilization, args.lane_bounds,
+        line["served"] = served_summary(args.budget_pairs, args.requests, args.bench_utilization, args.lane_bounds,
                                         args.lane_slots, one_lane=False)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample_nodes, args.cpu_serial_nodes)

@@ -308,6 +308,12 @@ int sparkalog_hip_session_end(sparkalog_hip_store* store, sparkalog_hip_summary*
 int sparkalog_hip_set_instance_tracking(sparkalog_hip_store* store, uint32_t instance_bits, uint32_t instances);
 int sparkalog_hip_instance_iterations(sparkalog_hip_store* store, uint32_t* last_new, uint32_t instances);
 
+/* Per instance: DELTA rows after the last session_step, counted on the
+ * device from that step's NEWT and read back with its counters, so a caller
+ * learns which instances have converged without another synchronization
+ * (host copy only; valid after a step of a session begun with tracking). */
+int sparkalog_hip_instance_delta_rows(sparkalog_hip_store* store, uint64_t* rows, uint32_t instances);
+
 /* Per instance: FULL rows and digest of its local pairs. */
 int sparkalog_hip_store_instance_stats(sparkalog_hip_store* store, uint32_t id, uint32_t instance_bits,
                                        uint32_t instances, uint64_t* rows, uint64_t* digests);

@@ -514,12 +514,63 @@ static int predicate_matches(uint32_t op, uint32_t operand, uint32_t value) {
     }
 }
 
+/* filter_cpu_parallel, execution/src/lib.rs:1705-1767: flags per chunk in
+ * parallel (chunk = ceil(rows / workers)), per-chunk counts serially, then
+ * each chunk writes its row ids at its offset in parallel. */
+typedef struct {
+    const uint32_t* input;
+    uint32_t op, operand;
+    uint64_t n, chunk;
+    uint32_t* flags;
+    const uint64_t* offsets;
+    uint32_t* out_rows;
+} filter_ctx;
+
+static void filter_flag_worker(void* p, int w, int ws) {
+    filter_ctx* c = (filter_ctx*)p;
+    uint64_t chunks = div_ceil(c->n, c->chunk);
+    for (uint64_t ch = (uint64_t)w; ch < chunks; ch += (uint64_t)ws) {
+        uint64_t b = ch * c->chunk, e = b + c->chunk < c->n ? b + c->chunk : c->n;
+        for (uint64_t i = b; i < e; ++i) c->flags[i] = (uint32_t)predicate_matches(c->op, c->operand, c->input[i]);
+    }
+}
+
+static void filter_write_worker(void* p, int w, int ws) {
+    filter_ctx* c = (filter_ctx*)p;
+    uint64_t chunks = div_ceil(c->n, c->chunk);
+    for (uint64_t ch = (uint64_t)w; ch < chunks; ch += (uint64_t)ws) {
+        uint64_t b = ch * c->chunk, e = b + c->chunk < c->n ? b + c->chunk : c->n;
+        uint64_t out = c->offsets[ch];
+        for (uint64_t i = b; i < e; ++i)
+            if (c->flags[i]) c->out_rows[out++] = (uint32_t)i;
+    }
+}
+
 uint64_t oracle_filter(const uint32_t* input, uint64_t n, uint32_t op, uint32_t operand,
                        uint32_t* out_rows, int threads) {
-    (void)threads; /* the parallel variant yields the same ordered row ids */
+    int ws = clamp_workers(threads, n);
+    if (n == 0) return 0;
+    if (ws <= 1) {
+        /* filter_cpu_serial, execution/src/lib.rs:1681-1702 */
+        uint64_t selected = 0;
+        for (uint64_t row = 0; row < n; ++row)
+            if (predicate_matches(op, operand, input[row])) out_rows[selected++] = (uint32_t)row;
+        return selected;
+    }
+    uint64_t chunk = div_ceil(n, (uint64_t)ws), chunks = div_ceil(n, chunk);
+    uint32_t* flags = (uint32_t*)malloc(n * sizeof(uint32_t));
+    uint64_t* offsets = (uint64_t*)malloc(chunks * sizeof(uint64_t));
+    filter_ctx c = {input, op, operand, n, chunk, flags, offsets, out_rows};
+    run_workers(ws, filter_flag_worker, &c);
     uint64_t selected = 0;
-    for (uint64_t row = 0; row < n; ++row)
-        if (predicate_matches(op, operand, input[row])) out_rows[selected++] = (uint32_t)row;
+    for (uint64_t ch = 0; ch < chunks; ++ch) {
+        offsets[ch] = selected;
+        uint64_t b = ch * chunk, e = b + chunk < n ? b + chunk : n;
+        for (uint64_t i = b; i < e; ++i) selected += flags[i];
+    }
+    run_workers(ws, filter_write_worker, &c);
+    free(flags);
+    free(offsets);
     return selected;
 }
 

@@ -79,6 +79,7 @@ _HIP_SIGNATURES = {
     "sparkalog_hip_session_end": (c_int, [c_void_p, POINTER(Summary)]),
     "sparkalog_hip_set_instance_tracking": (c_int, [c_void_p, c_uint32, c_uint32]),
     "sparkalog_hip_instance_iterations": (c_int, [c_void_p, c_void_p, c_uint32]),
+    "sparkalog_hip_instance_delta_rows": (c_int, [c_void_p, c_void_p, c_uint32]),
     "sparkalog_hip_store_instance_stats": (c_int, [c_void_p, c_uint32, c_uint32, c_uint32, c_void_p, c_void_p]),
     "sparkalog_hip_store_range_stats": (c_int, [c_void_p, c_uint32, c_int, c_uint32, c_uint32, c_uint32, u64p, u64p]),
     "sparkalog_hip_store_range_rows": (c_int, [c_void_p, c_uint32, c_int, c_void_p, c_uint32, c_void_p]),

@@ -1950,7 +1950,39 @@ struct Session {
     // per-instance tracking: instance of a row = first >> instance_bits
     uint32_t instance_bits = 0, instances = 0;
     DevBuf<uint32_t> last_new;  // per instance: 1 + last iteration with new rows
+    // per instance: DELTA rows after the last step, counted on the device
+    // from NEWT and read back with the step's counters (pinned)
+    DevBuf<uint64_t> inst_delta;
+    uint64_t* inst_delta_host = nullptr;
+    uint32_t inst_delta_cap = 0;
+    bool inst_delta_valid = false;
 };
+
+// Rows of each instance in sorted packed keys (count *rows_dev <= bound):
+// thread j adds the rows whose first column >> bits == j to out[j].
+__global__ void instance_rows_kernel(const uint64_t* keys, const uint64_t* rows_dev, size_t bound, uint32_t bits,
+                                     uint32_t instances, unsigned long long* out) {
+    const size_t n = *rows_dev < bound ? *rows_dev : bound;
+    const auto lower = [&](uint64_t j) -> size_t {
+        const uint64_t first = j << bits;
+        if (first > 0xFFFFFFFFull) return n;
+        const uint64_t target = first << 32;
+        size_t lo = 0, hi = n;
+        while (lo < hi) {
+            const size_t mid = (lo + hi) >> 1;
+            if (keys[mid] < target)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        return lo;
+    };
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t j = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < instances; j += stride) {
+        const size_t c = lower(j + 1) - lower(j);
+        if (c) atomicAdd(&out[j], static_cast<unsigned long long>(c));
+    }
+}
 
 Session& session_of(sparkalog_hip_store* s) {
     if (!s->session) s->session = new Session();
@@ -1962,6 +1994,7 @@ void destroy_session(sparkalog_hip_store* s) {
     Session* ss = static_cast<Session*>(s->session);
     if (ss->t_begin) (void)hipEventDestroy(ss->t_begin);
     if (ss->t_end) (void)hipEventDestroy(ss->t_end);
+    if (ss->inst_delta_host) (void)hipHostFree(ss->inst_delta_host);
     delete ss;
     s->session = nullptr;
 }
@@ -1997,7 +2030,17 @@ int session_begin(sparkalog_hip_store* s, const uint32_t* relations, uint32_t n_
         ss.last_new.reserve(ss.instances ? ss.instances : 1);
         clear_words_kernel<<<1, 1024, 0, s->stream>>>(ss.last_new.ptr, ss.instances);
         check(hipGetLastError());
+        ss.inst_delta.reserve(ss.instances ? ss.instances : 1);
+        if (ss.inst_delta_cap < ss.instances) {
+            if (ss.inst_delta_host) (void)hipHostFree(ss.inst_delta_host);
+            ss.inst_delta_host = nullptr;
+            void* p = nullptr;
+            check(hipHostMalloc(&p, sizeof(uint64_t) * ss.instances, hipHostMallocDefault));
+            ss.inst_delta_host = static_cast<uint64_t*>(p);
+            ss.inst_delta_cap = ss.instances;
+        }
     }
+    ss.inst_delta_valid = false;
     ss.active = true;
     return SPARKALOG_HIP_OK;
 }
@@ -2425,6 +2468,22 @@ void session_step(sparkalog_hip_store* s) {
             algo_bytes += 8.0 * right_rows;
         }
     }
+    // per instance: its NEWT rows (the next DELTA), read back with the counters
+    if (ss.instance_bits && ss.instances) {
+        clear_words_kernel<<<1, 1024, 0, s->stream>>>(reinterpret_cast<uint32_t*>(ss.inst_delta.ptr),
+                                                      2 * static_cast<size_t>(ss.instances));
+        for (uint32_t i = 0; i < n_relations; ++i) {
+            TargetRuntime& t = targets[i];
+            if (!t.newt.ptr) continue;
+            instance_rows_kernel<<<grid_for(ss.instances), kBlock, 0, s->stream>>>(
+                t.newt.ptr, &cdev->newt_rows[i], t.newt.cap, ss.instance_bits, ss.instances,
+                reinterpret_cast<unsigned long long*>(ss.inst_delta.ptr));
+        }
+        check(hipGetLastError());
+        check(hipMemcpyAsync(ss.inst_delta_host, ss.inst_delta.ptr, sizeof(uint64_t) * ss.instances,
+                             hipMemcpyDeviceToHost, s->stream));
+        ss.inst_delta_valid = true;
+    }
     check(hipMemcpyAsync(&chost, cdev, sizeof(Counters), hipMemcpyDeviceToHost, s->stream));
     uint32_t* err_host = reinterpret_cast<uint32_t*>(&s->words_host[0]);
     *err_host = 0;
@@ -2807,6 +2866,16 @@ extern "C" int sparkalog_hip_set_instance_tracking(sparkalog_hip_store* s, uint3
     Session& ss = session_of(s);
     ss.instance_bits = instances ? instance_bits : 0;
     ss.instances = instances;
+    return SPARKALOG_HIP_OK;
+}
+
+extern "C" int sparkalog_hip_instance_delta_rows(sparkalog_hip_store* s, uint64_t* rows, uint32_t instances) {
+    if (!s || (instances && !rows)) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    Session& ss = session_of(s);
+    if (!ss.instance_bits || instances > ss.instances || !ss.inst_delta_valid)
+        return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    // written by the last session_step, which synchronized after the copy
+    std::memcpy(rows, ss.inst_delta_host, sizeof(uint64_t) * instances);
     return SPARKALOG_HIP_OK;
 }
 

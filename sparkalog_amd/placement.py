@@ -167,6 +167,14 @@ def derive(table: dict) -> dict:
     return out
 
 
-# Derived by `derive` from `bench.py --mode crossover` on one MI355X and
-# its host's CPU share (16 cores); the measured table is in DESIGN.md.
-MEASURED_MI355X: dict = {}
+# Derived by `derive` from `bench.py --mode crossover` on one MI355X and its
+# host's CPU share (16 cores of an AMD EPYC), synthetic inputs of 2^8..2^24
+# rows; where two runs differed (the filter's GPU threshold: 2^14 and 2^16),
+# the larger value is kept.
+MEASURED_MI355X = {
+    "filter": FilterPlacementPolicy(65_536, 65_536, 4_194_304),
+    "join": JoinPlacementPolicy(256, 4_096),
+    "distinct": DistinctPlacementPolicy(16_384, 16_384, 1_048_576),
+    "anti_join": AntiJoinPlacementPolicy(16_384, 16_384, 262_144, 262_144),
+    "union": UnionPlacementPolicy(32_768, 8_192, 2_097_152),
+}

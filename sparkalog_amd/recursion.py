@@ -145,6 +145,13 @@ class RelationStore:
         _check(self._lib.sparkalog_hip_instance_iterations(self._h, out.ctypes.data_as(ctypes.c_void_p), instances))
         return out[:instances]
 
+    def instance_delta_rows(self, instances: int) -> np.ndarray:
+        """Per instance, DELTA rows after the last step (read back with the
+        step's counters: no device work here)."""
+        out = np.zeros(instances, np.uint64)
+        _check(self._lib.sparkalog_hip_instance_delta_rows(self._h, out.ctypes.data_as(ctypes.c_void_p), instances))
+        return out
+
     def instance_stats(self, id: RelationId, instance_bits: int, instances: int) -> tuple[np.ndarray, np.ndarray]:
         rows = np.zeros(max(instances, 1), dtype=np.uint64)
         digests = np.zeros(max(instances, 1), dtype=np.uint64)

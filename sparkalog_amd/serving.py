@@ -129,6 +129,9 @@ def _lane(requests: list[Request], store: RelationStore, t0: float, budget_pairs
     store.insert_static(EDGE, empty)
     store.insert_recursive(PATH, empty, empty)
     ex = RecursiveExecutor.compile(plans.transitive_closure_scc(PATH, EDGE), store)
+    # slot s is instance s: each step reads back every slot's DELTA rows with
+    # its own counters, so convergence needs no extra synchronization
+    store.set_instance_tracking(bits, max_slots)
     ex.begin(store)
     free = list(range(max_slots))[::-1]
     active = {}  # slot -> (request, admitted time, estimate)
@@ -147,36 +150,40 @@ def _lane(requests: list[Request], store: RelationStore, t0: float, budget_pairs
             free.extend(dead)
             dead.clear()
             dead_pairs = 0
-        # admission between steps, in arrival order
+        # admission between steps, in arrival order: every request that fits
+        # joins at once (one merge per relation part for all of them; slots
+        # ascend with arrival, so their rows concatenate in sorted order)
+        admitted = []
         while pending and pending[0].arrival <= now and free:
             req = pending[0]
             est = batch.estimate_closure(req.nodes, req.src.size)
-            if active and used + est > budget_pairs:
+            if (active or admitted) and used + est > budget_pairs:
                 break
             pending.pop(0)
-            slot = free.pop()
-            base = np.uint32(slot << bits)
-            gs, gd = req.src + base, req.dst + base
+            admitted.append((free.pop(), req, est))
+            used += est
+        if admitted:
+            admitted.sort(key=lambda a: a[0])
+            gs = np.concatenate([req.src + np.uint32(slot << bits) for slot, req, _ in admitted])
+            gd = np.concatenate([req.dst + np.uint32(slot << bits) for slot, req, _ in admitted])
             store.merge_rows(EDGE, 0, (gs, gd))
             store.merge_rows(PATH, 0, (gs, gd))
             store.merge_rows(PATH, 1, (gs, gd))
-            active[slot] = (req, now, est)
-            used += est
+            for slot, req, est in admitted:
+                active[slot] = (req, now, est)
         if not active:
             _wait_until(t0, pending[0].arrival)
             continue
         ex.step(store)
-        # retire converged instances (DELTA rows of every active slot in one call)
-        slots = list(active)
-        bounds = np.array([[s << bits, (s + 1) << bits] for s in slots], dtype=np.uint32).reshape(-1)
-        left = store.range_rows(PATH, 1, bounds)
-        for slot, delta_rows in zip(slots, left):
-            if delta_rows:
+        # retire converged instances (the step's per-instance DELTA rows)
+        left = store.instance_delta_rows(max_slots)
+        for slot in list(active):
+            if left[slot]:
                 continue
             lo, hi = slot << bits, (slot + 1) << bits
-            req, admitted, est = active.pop(slot)
+            req, admitted_at, est = active.pop(slot)
             rows, digest = store.range_stats(PATH, 0, lo, hi, lo)
-            results.append(Served(req.rid, req.arrival, admitted, time.perf_counter() - t0, rows, digest))
+            results.append(Served(req.rid, req.arrival, admitted_at, time.perf_counter() - t0, rows, digest))
             # its rows stay until the next batched removal: with an empty
             # DELTA they take part in no join, and no new instance uses the
             # slot before they are gone

@@ -166,6 +166,18 @@ def test_join_matches_nested_loop_order(seed, threads):
         assert pairs(oracle.join(left, right, left_key, right_key, output, threads=threads)) == expect
 
 
+def test_parallel_filter_matches_serial():
+    # filter_cpu_parallel (lib.rs:1705-1767): chunked flags, per-chunk offsets
+    rng = np.random.default_rng(21)
+    v = rng.integers(0, 100, 100_003, dtype=np.uint32)
+    # op codes 0..5 = Eq, Ne, Lt, Le, Gt, Ge (lib.rs:1769-1778)
+    masks = [v == 50, v != 50, v < 50, v <= 50, v > 50, v >= 50]
+    for op in range(6):
+        serial = oracle.filter_rows(v, op, 50, threads=1)
+        assert np.array_equal(oracle.filter_rows(v, op, 50, threads=7), serial)
+        assert np.array_equal(serial, np.flatnonzero(masks[op]).astype(np.uint32))
+
+
 def test_filter_matches_predicates():
     rng = np.random.default_rng(7)
     v = rng.integers(0, 100, size=5000, dtype=np.uint32)
