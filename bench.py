@@ -73,6 +73,8 @@ def parse_args():
                    help="served: closure-size bounds (pairs) between continuous-batching lanes")
     p.add_argument("--lane-slots", default="256,256,2",
                    help="served: instances in flight per lane (comma list; empty = 256 each)")
+    p.add_argument("--graph", type=int, default=1,
+                   help="batched/served: replay single-rule iterations from captured HIP graphs (1) or not (0)")
     p.add_argument("--sizes", default="24,26,28", help="ops mode: log2 row counts")
     p.add_argument("--ops", default="distinct,union,anti_join,join", help="ops mode: operators")
     p.add_argument("--samples", type=int, default=5, help="ops mode: timed launches per case")
@@ -173,7 +175,7 @@ def run_ops(args) -> None:
                 torch.cuda.empty_cache()
 
 
-def batched_summary(budget_pairs: float, plain_stride: int, full_log: bool = False) -> dict:
+def batched_summary(budget_pairs: float, plain_stride: int, full_log: bool = False, graph: bool = False) -> dict:
     """Batched fixpoints: 4,096 instances (nodes log-uniform on [16, 16384],
     Poisson out-degree mean uniform on [1.2, 3.0]) in one semi-naive loop per
     HBM wave, against one executor run per instance (every plain_stride-th
@@ -188,6 +190,7 @@ def batched_summary(budget_pairs: float, plain_stride: int, full_log: bool = Fal
     torch.cuda.set_device(0)
     s, d, offs, counts = workload.batch_graphs()
     store = RelationStore(HipStream())
+    store.set_graph(graph)
     # a cold pass first: its waves grow the store's workspaces and pools
     # (multi-GB allocations), then the measured pass on warm buffers, as the
     # plain path after it runs
@@ -201,7 +204,7 @@ def batched_summary(budget_pairs: float, plain_stride: int, full_log: bool = Fal
                 np.array_equal(plain.digests[picks], res.digests[picks]))
     scale = counts.size / len(picks)
     tuples = int(res.rows.sum())
-    out = {"instances": int(counts.size), "edges": int(s.size), "result_tuples": tuples,
+    out = {"instances": int(counts.size), "graph_replay": graph, "edges": int(s.size), "result_tuples": tuples,
            "waves": len(res.waves), "seconds": round(res.seconds, 3), "seconds_cold": round(cold.seconds, 3),
            "tuples_per_s": round(tuples / res.seconds, 1),
            "plain_seconds_measured": round(plain.seconds, 3), "plain_instances_measured": len(picks),
@@ -217,13 +220,13 @@ def batched_summary(budget_pairs: float, plain_stride: int, full_log: bool = Fal
 
 
 def run_batched_mode(args) -> None:
-    r = batched_summary(args.budget_pairs, args.plain_stride, full_log=True)
+    r = batched_summary(args.budget_pairs, args.plain_stride, full_log=True, graph=bool(args.graph))
     r["mode"] = "batched"
     print(json.dumps(r), flush=True)
 
 
 def served_summary(budget_pairs: float, requests: int, utilizations: str, lane_bounds: str, lane_slots: str,
-                   one_lane: bool = True) -> dict:
+                   one_lane: bool = True, graph: bool = False) -> dict:
     """Served fixpoints: a Poisson stream of TC requests (edges log-uniform
     on [2^8, 2^20]) at arrival rates giving the plain path each requested
     utilisation (its mean service time measured on the same requests served
@@ -237,6 +240,7 @@ def served_summary(budget_pairs: float, requests: int, utilizations: str, lane_b
 
     torch.cuda.set_device(0)
     store = RelationStore(HipStream())
+    store.set_graph(graph)
     # the plain path's service time on these very requests (same seed: the
     # sizes do not depend on the rate), all present at the start
     back_to_back = serving.make_requests(requests, rate_per_s=1e9)
@@ -245,9 +249,11 @@ def served_summary(budget_pairs: float, requests: int, utilizations: str, lane_b
     mean_service = secs / requests
     bounds = [float(b) for b in lane_bounds.split(",")] if lane_bounds else []
     lanes = [RelationStore(HipStream(torch.cuda.Stream())) for _ in range(len(bounds) + 1)]
+    for lane in lanes:
+        lane.set_graph(graph)
     slots = [int(x) for x in lane_slots.split(",")] if lane_slots else None
     serving.serve_lanes(back_to_back[:6], lanes, bounds, budget_pairs=budget_pairs, lane_slots=slots)  # warm-up
-    out = {"requests": requests, "mean_plain_service_ms": round(1000 * mean_service, 3),
+    out = {"requests": requests, "graph_replay": graph, "mean_plain_service_ms": round(1000 * mean_service, 3),
            "lane_bounds_pairs": bounds, "lane_slots": slots, "loads": []}
     equal = True
     for u in [float(x) for x in utilizations.split(",")]:
@@ -270,7 +276,8 @@ def served_summary(budget_pairs: float, requests: int, utilizations: str, lane_b
 
 
 def run_served_mode(args) -> None:
-    r = served_summary(args.budget_pairs, args.requests, args.utilization, args.lane_bounds, args.lane_slots)
+    r = served_summary(args.budget_pairs, args.requests, args.utilization, args.lane_bounds, args.lane_slots,
+                       graph=bool(args.graph))
     r["mode"] = "served"
     print(json.dumps(r), flush=True)
 
@@ -593,10 +600,10 @@ def main():
         # the batched configuration (4,096 instances) beside the plain path
         del store
         torch.cuda.empty_cache()
-        line["batched"] = batched_summary(args.budget_pairs, args.bench_plain_stride)
+        line["batched"] = batched_summary(args.budget_pairs, args.bench_plain_stride, graph=bool(args.graph))
     if not args.no_served:
         line["served"] = served_summary(args.budget_pairs, args.requests, args.bench_utilization, args.lane_bounds,
-                                        args.lane_slots, one_lane=False)
+                                        args.lane_slots, one_lane=False, graph=bool(args.graph))
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample_nodes, args.cpu_serial_nodes)
     print(json.dumps(line), flush=True)

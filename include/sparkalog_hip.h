@@ -279,8 +279,19 @@ int sparkalog_hip_store_set_tuning(sparkalog_hip_store* store, uint32_t merge_ra
  *     per relation (default: a dense start array over key values while the
  *     largest value is below 2^28 and the array is at most about 4 entries per
  *     row beyond 2^24; else the unique keys through a hash table, which
- *     indexes any u32 value), 1 = dense where the values allow it, 2 = hashed. */
+ *     indexes any u32 value), 1 = dense where the values allow it, 2 = hashed.
+ *   "graph": 1 = replay each semi-naive iteration of a fixpoint with one rule
+ *     over a static right relation (transitive closure's shape) from a
+ *     captured HIP graph: FULL stays one sorted array, every count stays on
+ *     the device, and the host synchronizes once per iteration (other plans
+ *     run the regular executor); 0 = off (default).
+ *   "graph_max_rows": FULL capacity (rows, 2^16..2^31, default 2^25) past
+ *     which a graph-mode fixpoint continues in the regular executor, whose
+ *     size-tiered FULL is not rewritten every iteration. */
 int sparkalog_hip_store_set_option(sparkalog_hip_store* store, const char* name, uint64_t value);
+/* Graphs captured and graph replays of graph-mode iterations since the store
+ * was created (no counterpart in the reference). */
+int sparkalog_hip_store_graph_stats(sparkalog_hip_store* store, uint64_t* captures, uint64_t* replays);
 int sparkalog_hip_store_op_stats(sparkalog_hip_store* store, int op, uint64_t* launches, double* ms,
                                  double* bytes);
 /* The operator's name and main kernel, and that kernel's own launches and

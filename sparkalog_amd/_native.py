@@ -69,6 +69,7 @@ _HIP_SIGNATURES = {
     "sparkalog_hip_store_set_profiling": (c_int, [c_void_p, c_int]),
     "sparkalog_hip_store_set_tuning": (c_int, [c_void_p, c_uint32, c_int, c_int, c_uint64]),
     "sparkalog_hip_store_set_option": (c_int, [c_void_p, ctypes.c_char_p, c_uint64]),
+    "sparkalog_hip_store_graph_stats": (c_int, [c_void_p, u64p, u64p]),
     "sparkalog_hip_store_op_stats": (c_int, [c_void_p, c_int, u64p, POINTER(c_double), POINTER(c_double)]),
     "sparkalog_hip_store_op_kernel_stats": (c_int, [c_void_p, c_int, POINTER(ctypes.c_char_p), POINTER(ctypes.c_char_p),
                                                      u64p, POINTER(c_double)]),

@@ -56,6 +56,14 @@ inline std::mutex& pool_mutex() {
     return m;
 }
 
+// Held while a graph is captured, and around every call that synchronizes
+// the whole device (a free does): synchronizing a stream that another
+// thread is capturing would invalidate that capture.
+inline std::recursive_mutex& device_mutex() {
+    static std::recursive_mutex m;
+    return m;
+}
+
 __host__ __device__ inline size_t tiles(size_t n, size_t t) { return (n + t - 1) / t; }
 
 inline uint64_t bit_mask_for(uint64_t max_value) {
@@ -84,6 +92,7 @@ struct DevBuf {
     ~DevBuf() { release(); }
     void release() {
         if (ptr) {
+            std::lock_guard<std::recursive_mutex> lock(device_mutex());
             // kernels queued on any stream may still use the buffer
             (void)hipDeviceSynchronize();
             (void)hipFree(ptr);
@@ -96,6 +105,7 @@ struct DevBuf {
         size_t c = cap ? cap : 1024;
         while (c < n) c += c / 2 + 1;
         release();
+        std::lock_guard<std::recursive_mutex> lock(device_mutex());
         void* p = nullptr;
         hipError_t e = hipMalloc(&p, c * sizeof(T));
         while (e != hipSuccess && g_relief && g_relief(c * sizeof(T))) {
@@ -345,11 +355,14 @@ __global__ void hash_insert_kernel(const uint64_t* sorted, const uint32_t* start
 constexpr int kCountItems = 8;
 constexpr int kCountTile = kBlock * kCountItems;
 
+// With n_dev, the row count is *n_dev (at most n, which then only sizes the
+// launch) and tiles past it report a zero total.
 template <typename O, bool Hashed>
 __global__ __launch_bounds__(kBlock) void dense_join_count_kernel(const uint64_t* delta, size_t n, uint32_t key_col,
                                                                   KeyIndex ix, O* local, uint32_t* match,
                                                                   uint64_t* tile_sums, uint32_t* clear,
-                                                                  uint32_t clear_words) {
+                                                                  uint32_t clear_words,
+                                                                  const uint64_t* n_dev = nullptr) {
     const uint32_t* starts = ix.starts;
     // the step's status words, cleared by the first block (no kernel before
     // the count in a step reads them)
@@ -358,6 +371,11 @@ __global__ __launch_bounds__(kBlock) void dense_join_count_kernel(const uint64_t
     constexpr int kWaveRows = kCountItems * kWave;
     __shared__ uint64_t wave_totals[kWavesPerBlock];
     const size_t base = static_cast<size_t>(blockIdx.x) * kCountTile;
+    if (n_dev) n = *n_dev < n ? static_cast<size_t>(*n_dev) : n;
+    if (base >= n) {  // block-uniform: a tile past the rows (never with n_dev null)
+        if (threadIdx.x == 0) tile_sums[blockIdx.x] = 0;
+        return;
+    }
     const uint32_t lane = lane_id(), wave = wave_id();
     // loads in two batches (keys, then index entries), so that a wave has
     // all of them in flight at once instead of one dependent pair per item;
@@ -471,12 +489,19 @@ constexpr int kEmitStage = 1024;  // delta rows staged per tile
 
 // row_start[t] = the delta row that contains output t*kEmitTile (one thread
 // per tile; the last entry is n).
+// With n_dev / total_dev the counts are read from the device (n, total and
+// n_tiles then only bound them and the launch).
 template <typename O>
 __global__ void emit_partition_kernel(const O* local, const uint64_t* base, size_t n, size_t total,
-                                      size_t* row_start, size_t n_tiles) {
+                                      size_t* row_start, size_t n_tiles, const uint64_t* n_dev = nullptr,
+                                      const uint64_t* total_dev = nullptr) {
+    if (n_dev) n = *n_dev < n ? static_cast<size_t>(*n_dev) : n;
+    if (total_dev) total = *total_dev < total ? static_cast<size_t>(*total_dev) : total;
+    const size_t tiles_now = (total + kEmitTile - 1) / kEmitTile;
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
     for (size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; t <= n_tiles; t += stride) {
-        if (t == n_tiles) {
+        if (t > tiles_now) continue;
+        if (t == tiles_now) {
             row_start[t] = n;
             continue;
         }
@@ -523,7 +548,10 @@ __global__ __launch_bounds__(kBlock) void dense_join_emit_kernel(const uint64_t*
                                                                  const uint64_t* tile_base, const uint32_t* match_base,
                                                                  size_t total,
                                                                  const size_t* row_start, EmitSpec spec,
-                                                                 uint64_t* out) {
+                                                                 uint64_t* out, const uint64_t* n_dev = nullptr,
+                                                                 const uint64_t* total_dev = nullptr) {
+    if (n_dev) n = *n_dev < n ? static_cast<size_t>(*n_dev) : n;
+    if (total_dev) total = *total_dev < total ? static_cast<size_t>(*total_dev) : total;
     constexpr int kPer = kEmitTile / kBlock;  // 4 output positions per thread
     // 18 KB: 8 blocks a CU. A row's first output is kept relative to the
     // tile's first (u32, wrapping: differences are exact below 2^32, and a
@@ -747,6 +775,74 @@ struct TargetRuntime {
     std::vector<int> union_slots;  // profiling events of this iteration's contribution unions
 };
 
+// Graph-replayed iterations (sparkalog_hip_store_set_option "graph"): for a
+// single rule over a static right relation (transitive closure's shape),
+// FULL is one sorted array and every count lives on the device, so one
+// iteration is a fixed sequence of launches over fixed buffers, captured
+// once as a HIP graph and replayed; the host synchronizes once per
+// iteration, to read the counts and the overflow flags. Device counter words:
+enum GraphWord {
+    kGwFull0 = 0, kGwFull1 = 1,    // FULL rows in buffer slot 0 / 1
+    kGwDelta0 = 2, kGwDelta1 = 3,  // DELTA rows in buffer slot 0 / 1
+    kGwJoin = 4,                   // join output rows (M)
+    kGwJoinUsed = 5,               // M, or 0 when it overflows the candidate buffers
+    kGwUnique = 6,                 // distinct candidates (U)
+    kGwUniqueUsed = 7,             // U, or 0 after any overflow
+    kGwFlags = 8,                  // bit 0: M > candidate capacity, bit 1: FULL + U > FULL capacity
+    kGwError = 9,                  // look-back reports (low 32 bits)
+    kGwDone = 10,                  // iterations completed
+    kGwNext = 11,                  // the iteration's number + 1 (per-instance tracking)
+    kGwWords = 16
+};
+
+struct GraphKey {
+    const void* ptr[24];
+    uint64_t val[12];
+    bool operator==(const GraphKey& o) const {
+        return std::memcmp(ptr, o.ptr, sizeof(ptr)) == 0 && std::memcmp(val, o.val, sizeof(val)) == 0;
+    }
+};
+
+struct GraphRun {
+    uint64_t cap_full = 0, cap_cand = 0, cap_delta = 0;
+    DevBuf<uint64_t> full_next, delta_next;  // the buffers the next iteration writes
+    DevBuf<uint64_t> cand_a, cand_b, uniq;
+    DevBuf<uint64_t> offsets, tile_base;
+    DevBuf<uint32_t> match;
+    DevBuf<size_t> row_start;
+    DevBuf<unsigned char> sort_temp, merge_temp;
+    DevBuf<uint64_t> words;   // kGwWords device counters
+    uint64_t* host = nullptr; // pinned mirror of `words`, one block per slot parity
+    int slot = 0;             // which words slot holds the current FULL/DELTA rows
+    // state seen after the last graph iteration: anything else means the
+    // relations changed in between, and the device row counts are rewritten
+    const void* seen_full = nullptr;
+    const void* seen_delta = nullptr;
+    uint64_t seen_full_rows = ~0ull, seen_delta_rows = ~0ull;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    GraphKey key[2];
+    // graphs are captured and launched on a stream of their own (the
+    // store's stream may be the null stream, which cannot be captured),
+    // ordered after the store's stream by an event
+    hipStream_t stream = nullptr;
+    hipEvent_t ready = nullptr;
+    int next_evict = 0;
+    uint64_t captures = 0, replays = 0;
+    void drop_graphs() {
+        for (auto& e : exec)
+            if (e) {
+                (void)hipGraphExecDestroy(e);
+                e = nullptr;
+            }
+    }
+    ~GraphRun() {
+        drop_graphs();
+        if (host) (void)hipHostFree(host);
+        if (ready) (void)hipEventDestroy(ready);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
 // Small device counters read back once per phase.
 struct Counters {
     uint64_t join_total[64];
@@ -772,7 +868,7 @@ struct sparkalog_hip_store {
     // and the event the host waits on for the join sizes
     spk::Counters* counters_host = nullptr;
     // [0]: device error flag, [1]: oversized-run rows of the distinct, [2]: index degree, [3]: a call's error word,
-    // [4]: unique keys of a hashed index
+    // [4]: unique keys of a hashed index, [5]: output rows of a merge
     uint64_t* words_host = nullptr;
     hipEvent_t counters_ready = nullptr;
     bool profiling = false;
@@ -853,6 +949,9 @@ struct sparkalog_hip_store {
     }
     bool x_anti = false;  // anti-join FULL through its first-column index when it fits
     int join_index_mode = spk::kIndexAuto;
+    int graph_mode = 0;   // replay captured iterations where the plan allows
+    uint64_t graph_max_rows = 1ull << 25;  // larger FULLs leave graph mode
+    spk::GraphRun graph;
     uint64_t sparse_probe_rows = spk::kSparseProbeRows;
     // small persistent device buffers for range probes and statistics (a
     // fresh allocation would synchronize the whole device when freed)
@@ -872,6 +971,7 @@ struct sparkalog_hip_store {
     spk::Counters& host_counters() {
         if (!counters_host) {
             void* p = nullptr;
+            std::lock_guard<std::recursive_mutex> lock(spk::device_mutex());
             spk::check(hipHostMalloc(&p, sizeof(spk::Counters) + 64, hipHostMallocDefault));
             counters_host = static_cast<spk::Counters*>(p);
             words_host = reinterpret_cast<uint64_t*>(static_cast<char*>(p) + sizeof(spk::Counters));
@@ -894,6 +994,7 @@ struct sparkalog_hip_store {
 
     void* staging(size_t bytes) {
         if (bytes > pinned_bytes) {
+            std::lock_guard<std::recursive_mutex> lock(spk::device_mutex());
             if (pinned) (void)hipHostFree(pinned);
             pinned = nullptr;
             size_t b = pinned_bytes ? pinned_bytes : (1u << 20);
@@ -1518,7 +1619,9 @@ void union_into(sparkalog_hip_store* s, DevRelation& dst, const uint64_t* keys, 
         check(sorted_union(a, dst.rows, b, rows, PairOutput{out.ptr, nullptr, nullptr}, &cdev->merge_rows, temp, tb,
                            s->stream));
     }
-    uint64_t n = 0;
+    s->host_counters();
+    uint64_t& n = s->words_host[5];
+    n = 0;
     check(hipMemcpyAsync(&n, &cdev->merge_rows, sizeof(n), hipMemcpyDeviceToHost, s->stream));
     s->check_call_error(err);
     s->give(std::move(dst.keys));
@@ -1878,9 +1981,26 @@ extern "C" int sparkalog_hip_store_set_tuning(sparkalog_hip_store* s, uint32_t m
     return SPARKALOG_HIP_OK;
 }
 
+extern "C" int sparkalog_hip_store_graph_stats(sparkalog_hip_store* s, uint64_t* captures, uint64_t* replays) {
+    if (!s) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    if (captures) *captures = s->graph.captures;
+    if (replays) *replays = s->graph.replays;
+    return SPARKALOG_HIP_OK;
+}
+
 extern "C" int sparkalog_hip_store_set_option(sparkalog_hip_store* s, const char* name, uint64_t value) {
     if (!s || !name) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
     const std::string key(name);
+    if (key == "graph") {
+        if (value > 1) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+        s->graph_mode = static_cast<int>(value);
+        return SPARKALOG_HIP_OK;
+    }
+    if (key == "graph_max_rows") {
+        if (value < (1ull << 16) || value > (1ull << 31)) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+        s->graph_max_rows = value;
+        return SPARKALOG_HIP_OK;
+    }
     if (key == "join_index") {
         if (value > 2) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
         s->join_index_mode = static_cast<int>(value);
@@ -1942,6 +2062,7 @@ namespace {
 // split into steps so that a caller may change relations between steps).
 struct Session {
     bool active = false;
+    bool graph = false;  // iterations replayed from captured graphs (graph_step)
     std::vector<uint32_t> relations;
     std::vector<sparkalog_hip_rule> rules;
     uint64_t iterations = 0, total_new = 0;
@@ -2000,7 +2121,10 @@ void destroy_session(sparkalog_hip_store* s) {
 }
 
 __global__ void track_instances_kernel(const uint64_t* keys, size_t n, uint32_t bits, uint32_t instances,
-                                       uint32_t value, uint32_t* last_new) {
+                                       uint32_t value, uint32_t* last_new, const uint64_t* n_dev = nullptr,
+                                       const uint64_t* value_dev = nullptr) {
+    if (n_dev) n = *n_dev < n ? static_cast<size_t>(*n_dev) : n;
+    if (value_dev) value = static_cast<uint32_t>(*value_dev);
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
     for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
         const uint32_t inst = static_cast<uint32_t>(keys[i] >> 32) >> bits;
@@ -2016,6 +2140,14 @@ int session_begin(sparkalog_hip_store* s, const uint32_t* relations, uint32_t n_
     if (compiled != SPARKALOG_HIP_OK) return compiled;
     ss.relations.assign(relations, relations + n_relations);
     ss.rules.assign(rules, rules + n_rules);
+    // graph-replayed iterations serve one rule whose right input is static
+    ss.graph = false;
+    if (s->graph_mode && n_relations == 1 && n_rules == 1 && rules[0].target == relations[0] &&
+        rules[0].delta_input == relations[0] && rules[0].right_input != relations[0]) {
+        const RelationState* right = find(s, rules[0].right_input);
+        ss.graph = right && !right->recursive;
+    }
+    s->graph.seen_full = s->graph.seen_delta = nullptr;  // device row counts rewritten at the first step
     ss.iterations = ss.total_new = 0;
     ss.algo_bytes = 0.0;
     s->events_used = 0;
@@ -2032,6 +2164,7 @@ int session_begin(sparkalog_hip_store* s, const uint32_t* relations, uint32_t n_
         check(hipGetLastError());
         ss.inst_delta.reserve(ss.instances ? ss.instances : 1);
         if (ss.inst_delta_cap < ss.instances) {
+            std::lock_guard<std::recursive_mutex> lock(device_mutex());
             if (ss.inst_delta_host) (void)hipHostFree(ss.inst_delta_host);
             ss.inst_delta_host = nullptr;
             void* p = nullptr;
@@ -2051,10 +2184,16 @@ uint64_t pending_delta(sparkalog_hip_store* s) {
     return pending;
 }
 
+void graph_step(sparkalog_hip_store* s);
+
 // One semi-naive iteration: evaluate every rule, combine per target, apply
 // per target (generic.rs:481-512).
 void session_step(sparkalog_hip_store* s) {
     Session& ss = session_of(s);
+    if (ss.graph) {
+        graph_step(s);
+        return;
+    }
     std::vector<TargetRuntime>& targets = s->ws_targets;
     std::vector<RuleRuntime>& rts = s->ws_rules;
     const uint32_t n_rules = static_cast<uint32_t>(ss.rules.size());
@@ -2592,6 +2731,331 @@ void session_step(sparkalog_hip_store* s) {
         check(hipGetLastError());
     }
     ++iterations;
+}
+
+// ---- graph-replayed iterations ----
+
+__global__ void graph_begin_kernel(uint64_t* w) {
+    w[kGwFlags] = 0;
+    w[kGwError] = 0;
+    w[kGwNext] = w[kGwDone] + 1;
+}
+
+// M beyond the candidate buffers: flag it and run the rest on no rows.
+__global__ void graph_check_join_kernel(uint64_t* w, uint64_t cap) {
+    const uint64_t m = w[kGwJoin];
+    if (m > cap) {
+        w[kGwFlags] |= 1;
+        w[kGwJoinUsed] = 0;
+    } else {
+        w[kGwJoinUsed] = m;
+    }
+}
+
+// FULL + U beyond the FULL buffers (or an earlier overflow): no rows merge.
+__global__ void graph_check_full_kernel(uint64_t* w, int slot, uint64_t cap) {
+    const uint64_t f = w[kGwFull0 + slot], u = w[kGwUnique];
+    if (w[kGwFlags] & 1) {
+        w[kGwUniqueUsed] = 0;
+    } else if (f + u > cap) {
+        w[kGwFlags] |= 2;
+        w[kGwUniqueUsed] = 0;
+    } else {
+        w[kGwUniqueUsed] = u;
+    }
+}
+
+__global__ void graph_end_kernel(uint64_t* w) {
+    if (w[kGwFlags] == 0) w[kGwDone] = w[kGwNext];
+}
+
+__global__ void graph_set_rows_kernel(uint64_t* w, int slot, uint64_t full_rows, uint64_t delta_rows,
+                                      uint64_t done) {
+    w[kGwFull0 + slot] = full_rows;
+    w[kGwDelta0 + slot] = delta_rows;
+    w[kGwDone] = done;
+}
+
+inline uint64_t pow2_at_least(uint64_t n) {
+    uint64_t c = 1;
+    while (c < n) c <<= 1;
+    return c;
+}
+
+// A buffer with at least `cap` entries holding the first `rows` of `b`.
+void grow_keep(sparkalog_hip_store* s, DevBuf<uint64_t>& b, uint64_t cap, uint64_t rows) {
+    if (b.ptr && b.cap >= cap) return;
+    DevBuf<uint64_t> nb = s->take(cap);
+    if (rows) check(hipMemcpyAsync(nb.ptr, b.ptr, rows * sizeof(uint64_t), hipMemcpyDeviceToDevice, s->stream));
+    s->give(std::move(b));
+    b = std::move(nb);
+}
+
+// One semi-naive iteration (generic.rs:481-512) for a single rule over a
+// static right relation, from a captured HIP graph: join count, scan,
+// output-balanced emit, global sort + unique of the candidates, and one merge
+// giving FULL' = FULL ∪ U and NEWT = U \ FULL; counts stay on the device and
+// the host synchronizes once, to read them. An overflow of the candidate or
+// FULL buffers (flagged on the device, with no rows written) grows them, and
+// the iteration runs again; nothing of the current FULL or DELTA changes
+// until an iteration succeeds, since the graph writes the other buffers.
+void graph_step(sparkalog_hip_store* s) {
+    Session& ss = session_of(s);
+    GraphRun& g = s->graph;
+    RuleRuntime& rt = s->ws_rules[0];
+    const sparkalog_hip_rule& plan = rt.plan;
+    RelationState& ts = *find(s, ss.relations[0]);
+    RelationState& rs = *find(s, plan.right_input);
+    build_index(s, rt, rs, plan.right_input);
+    if (!ts.runs.empty() || ts.full.csr) settle_all(s, ts);  // FULL as one packed array
+    ts.settle_pending = false;
+    uint32_t vmax = 0;
+    for (const auto& st : s->rel)
+        if (st.present) vmax = st.max_value > vmax ? st.max_value : vmax;
+    const uint64_t vmask = bit_mask_for(vmax);
+    uint32_t vbits = 0;
+    while ((1ull << vbits) <= vmask && vbits < 32) ++vbits;
+    if (vbits == 0) vbits = 1;
+    const uint32_t shift = vbits <= 24 ? vbits : 32;
+    const uint64_t known = shift == 32 ? ((vmask << 32) | vmask) : ((vmask << shift) | vmask);
+    const uint64_t L = ts.delta.rows, F = ts.full.rows;
+    if (F + 2 > 0xFFFFFFFFull) throw StatusError{SPARKALOG_HIP_ERR_TOO_MANY_ROWS};
+    g.words.reserve(kGwWords);
+    if (!g.host) {
+        std::lock_guard<std::recursive_mutex> lock(device_mutex());
+        void* p = nullptr;
+        check(hipHostMalloc(&p, 2 * kGwWords * sizeof(uint64_t), hipHostMallocDefault));
+        g.host = static_cast<uint64_t*>(p);
+    }
+    rt.narrow = static_cast<uint64_t>(rt.index.max_degree) * kCountTile < (1ull << 32);
+    if (!g.stream) {
+        std::lock_guard<std::recursive_mutex> lock(device_mutex());
+        check(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+        check(hipEventCreateWithFlags(&g.ready, hipEventDisableTiming));
+    }
+    const hipStream_t gs = g.stream;
+    for (int attempt = 0;; ++attempt) {
+        if (attempt > 4) throw StatusError{SPARKALOG_HIP_ERR_DEVICE};
+        // a FULL past graph_max_rows is rewritten every iteration at a cost the
+        // regular executor's size-tiered runs avoid: the rest of the fixpoint
+        // runs there (FULL and DELTA are single sorted arrays, as it expects)
+        if (std::max(g.cap_full, pow2_at_least(2 * F + (1ull << 16))) > s->graph_max_rows) {
+            ss.graph = false;
+            session_step(s);
+            return;
+        }
+        // capacities (powers of two, never shrinking within a store)
+        g.cap_full = std::max(g.cap_full, pow2_at_least(2 * F + (1ull << 16)));
+        g.cap_cand = std::max(g.cap_cand, pow2_at_least(4 * L + (1ull << 14)));
+        if (g.cap_full > 0xFFFFFFFFull || g.cap_cand > 0xFFFFFFFFull) throw StatusError{SPARKALOG_HIP_ERR_TOO_MANY_ROWS};
+        g.cap_delta = std::max(g.cap_cand, L ? L : 1);
+        grow_keep(s, ts.full.keys, g.cap_full, F);
+        grow_keep(s, ts.delta.keys, g.cap_delta, L);
+        ts.full.touched();
+        ts.delta.touched();
+        g.full_next.reserve(g.cap_full);
+        g.delta_next.reserve(g.cap_delta);
+        g.cand_a.reserve(g.cap_cand);
+        g.cand_b.reserve(g.cap_cand);
+        g.uniq.reserve(g.cap_cand);
+        g.offsets.reserve(g.cap_delta);
+        g.match.reserve(g.cap_delta);
+        const size_t count_tiles = tiles(g.cap_delta, kCountTile);
+        g.tile_base.reserve(count_tiles);
+        const size_t emit_tiles = tiles(g.cap_cand, kEmitTile);
+        g.row_start.reserve(emit_tiles + 1);
+        const size_t sort_bytes = distinct_temp_bytes(g.cap_cand);
+        g.sort_temp.reserve(sort_bytes);
+        const size_t merge_bytes = merge_new_temp_bytes(g.cap_full, g.cap_cand);
+        g.merge_temp.reserve(merge_bytes);
+        const bool tracking = ss.instance_bits && ss.instances;
+        // rows the device does not know yet (first step, or relations changed)
+        if (g.seen_full != ts.full.keys.ptr || g.seen_delta != ts.delta.keys.ptr || g.seen_full_rows != F ||
+            g.seen_delta_rows != L) {
+            graph_set_rows_kernel<<<1, 1, 0, s->stream>>>(g.words.ptr, g.slot, F, L, ss.iterations);
+            check(hipGetLastError());
+            g.seen_full = ts.full.keys.ptr;
+            g.seen_delta = ts.delta.keys.ptr;
+            g.seen_full_rows = F;
+            g.seen_delta_rows = L;
+        }
+        GraphKey key{};
+        const void* ptrs[] = {ts.full.keys.ptr, g.full_next.ptr, ts.delta.keys.ptr, g.delta_next.ptr, g.cand_a.ptr,
+                              g.cand_b.ptr, g.uniq.ptr, g.offsets.ptr, g.tile_base.ptr, g.match.ptr, g.row_start.ptr,
+                              g.sort_temp.ptr, g.merge_temp.ptr, g.words.ptr, rt.index.starts.ptr,
+                              rt.index.table.ptr, rt.index.other.ptr, tracking ? ss.inst_delta.ptr : nullptr,
+                              tracking ? ss.last_new.ptr : nullptr, tracking ? ss.inst_delta_host : nullptr};
+        static_assert(sizeof(ptrs) <= sizeof(key.ptr), "graph key holds every pointer");
+        std::memcpy(key.ptr, ptrs, sizeof(ptrs));
+        const uint64_t vals[] = {g.cap_full, g.cap_cand, g.cap_delta, static_cast<uint64_t>(g.slot), shift,
+                                 static_cast<uint64_t>(rt.narrow) | (static_cast<uint64_t>(rt.index.hashed) << 1),
+                                 rt.index.bits | (static_cast<uint64_t>(rt.index.n_keys) << 8), known,
+                                 tracking ? (static_cast<uint64_t>(ss.instance_bits) << 32 | ss.instances) : 0,
+                                 static_cast<uint64_t>(plan.left_key) | (plan.right_key << 2) |
+                                     (plan.out_input[0] << 4) | (plan.out_input[1] << 6) | (plan.out_column[0] << 8) |
+                                     (plan.out_column[1] << 10) | (plan.distinct_columns[0] << 12) |
+                                     (plan.distinct_columns[1] << 14)};
+        static_assert(sizeof(vals) <= sizeof(key.val), "graph key holds every launch value");
+        std::memcpy(key.val, vals, sizeof(vals));
+        int which = -1;
+        for (int k = 0; k < 2; ++k)
+            if (g.exec[k] && g.key[k] == key) which = k;
+        if (which < 0) {
+            which = g.next_evict;
+            g.next_evict ^= 1;
+            if (g.exec[which]) {
+                check(hipGraphExecDestroy(g.exec[which]));
+                g.exec[which] = nullptr;
+            }
+            // capture one iteration's launches (no allocation, no memset,
+            // no synchronization inside)
+            const int cur = g.slot, nxt = g.slot ^ 1;
+            uint64_t* w = g.words.ptr;
+            std::lock_guard<std::recursive_mutex> capture_lock(device_mutex());
+            check(hipStreamBeginCapture(gs, hipStreamCaptureModeThreadLocal));
+            try {
+                ErrorSink sink(reinterpret_cast<uint32_t*>(&w[kGwError]));
+                graph_begin_kernel<<<1, 1, 0, gs>>>(w);
+                const KeyIndex kix = rt.index.view();
+                const unsigned count_grid = static_cast<unsigned>(count_tiles);
+                uint32_t* local32 = reinterpret_cast<uint32_t*>(g.offsets.ptr);
+                const uint64_t* n_dev = &w[kGwDelta0 + cur];
+                if (rt.narrow && !rt.index.hashed)
+                    dense_join_count_kernel<uint32_t, false><<<count_grid, kBlock, 0, gs>>>(
+                        ts.delta.keys.ptr, g.cap_delta, plan.left_key, kix, local32, g.match.ptr, g.tile_base.ptr,
+                        nullptr, 0, n_dev);
+                else if (rt.narrow)
+                    dense_join_count_kernel<uint32_t, true><<<count_grid, kBlock, 0, gs>>>(
+                        ts.delta.keys.ptr, g.cap_delta, plan.left_key, kix, local32, g.match.ptr, g.tile_base.ptr,
+                        nullptr, 0, n_dev);
+                else if (!rt.index.hashed)
+                    dense_join_count_kernel<uint64_t, false><<<count_grid, kBlock, 0, gs>>>(
+                        ts.delta.keys.ptr, g.cap_delta, plan.left_key, kix, g.offsets.ptr, g.match.ptr,
+                        g.tile_base.ptr, nullptr, 0, n_dev);
+                else
+                    dense_join_count_kernel<uint64_t, true><<<count_grid, kBlock, 0, gs>>>(
+                        ts.delta.keys.ptr, g.cap_delta, plan.left_key, kix, g.offsets.ptr, g.match.ptr,
+                        g.tile_base.ptr, nullptr, 0, n_dev);
+                join_scan_kernel<<<1, kScanThreads, 0, gs>>>(g.tile_base.ptr, count_tiles, &w[kGwJoin]);
+                graph_check_join_kernel<<<1, 1, 0, gs>>>(w, g.cap_cand);
+                EmitSpec spec;
+                spec.key_col = plan.left_key;
+                spec.right_key = plan.right_key;
+                spec.side[0] = plan.out_input[0];
+                spec.side[1] = plan.out_input[1];
+                spec.col[0] = plan.out_column[0];
+                spec.col[1] = plan.out_column[1];
+                spec.order[0] = plan.distinct_columns[0];
+                spec.order[1] = plan.distinct_columns[1];
+                spec.shift = shift;
+                const unsigned emit_grid = static_cast<unsigned>(emit_tiles < 65536 ? emit_tiles : 65536);
+                const uint64_t* m_dev = &w[kGwJoinUsed];
+                if (rt.narrow) {
+                    emit_partition_kernel<uint32_t><<<grid_for(emit_tiles + 1), kBlock, 0, gs>>>(
+                        local32, g.tile_base.ptr, g.cap_delta, g.cap_cand, g.row_start.ptr, emit_tiles, n_dev, m_dev);
+                    dense_join_emit_kernel<uint32_t><<<emit_grid, kBlock, 0, gs>>>(
+                        ts.delta.keys.ptr, g.cap_delta, rt.index.other.ptr, local32, g.tile_base.ptr, g.match.ptr,
+                        g.cap_cand, g.row_start.ptr, spec, g.cand_a.ptr, n_dev, m_dev);
+                } else {
+                    emit_partition_kernel<uint64_t><<<grid_for(emit_tiles + 1), kBlock, 0, gs>>>(
+                        g.offsets.ptr, g.tile_base.ptr, g.cap_delta, g.cap_cand, g.row_start.ptr, emit_tiles, n_dev,
+                        m_dev);
+                    dense_join_emit_kernel<uint64_t><<<emit_grid, kBlock, 0, gs>>>(
+                        ts.delta.keys.ptr, g.cap_delta, rt.index.other.ptr, g.offsets.ptr, g.tile_base.ptr,
+                        g.match.ptr, g.cap_cand, g.row_start.ptr, spec, g.cand_a.ptr, n_dev, m_dev);
+                }
+                PairInput in{g.cand_a.ptr, nullptr, nullptr, m_dev};
+                PairOutput out{g.uniq.ptr, nullptr, nullptr};
+                check(distinct(in, g.cap_cand, g.cand_a.ptr, g.cand_b.ptr, out, &w[kGwUnique], g.sort_temp.ptr,
+                               sort_bytes, known, gs));
+                if (shift != 32)
+                    expand_keys_kernel<<<grid_for(g.cap_cand), kBlock, 0, gs>>>(g.uniq.ptr, &w[kGwUnique],
+                                                                                       g.cap_cand, shift);
+                graph_check_full_kernel<<<1, 1, 0, gs>>>(w, cur, g.cap_full);
+                check(merge_new(ts.full.keys.ptr, &w[kGwFull0 + cur], g.cap_full, g.uniq.ptr, &w[kGwUniqueUsed],
+                                g.cap_cand, g.full_next.ptr, &w[kGwFull0 + nxt], g.delta_next.ptr,
+                                &w[kGwDelta0 + nxt], g.merge_temp.ptr, merge_bytes, gs));
+                if (tracking) {
+                    clear_words_kernel<<<1, 1024, 0, gs>>>(reinterpret_cast<uint32_t*>(ss.inst_delta.ptr),
+                                                                  2 * static_cast<size_t>(ss.instances));
+                    instance_rows_kernel<<<grid_for(ss.instances), kBlock, 0, gs>>>(
+                        g.delta_next.ptr, &w[kGwDelta0 + nxt], g.cap_delta, ss.instance_bits, ss.instances,
+                        reinterpret_cast<unsigned long long*>(ss.inst_delta.ptr));
+                    track_instances_kernel<<<grid_for(g.cap_delta), kBlock, 0, gs>>>(
+                        g.delta_next.ptr, g.cap_delta, ss.instance_bits, ss.instances, 0, ss.last_new.ptr,
+                        &w[kGwDelta0 + nxt], &w[kGwNext]);
+                    check(hipMemcpyAsync(ss.inst_delta_host, ss.inst_delta.ptr, sizeof(uint64_t) * ss.instances,
+                                         hipMemcpyDeviceToHost, gs));
+                }
+                graph_end_kernel<<<1, 1, 0, gs>>>(w);
+                check(hipGetLastError());
+                check(hipMemcpyAsync(g.host + kGwWords * cur, w, kGwWords * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                     gs));
+            } catch (...) {
+                hipGraph_t broken = nullptr;
+                (void)hipStreamEndCapture(gs, &broken);
+                if (broken) (void)hipGraphDestroy(broken);
+                (void)hipGetLastError();
+                throw;
+            }
+            hipGraph_t graph = nullptr;
+            check(hipStreamEndCapture(gs, &graph));
+            const hipError_t inst = hipGraphInstantiate(&g.exec[which], graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            check(inst);
+            g.key[which] = key;
+            ++g.captures;
+        } else {
+            ++g.replays;
+        }
+        // after everything queued on the store's stream (uploads, the index,
+        // the rows just set)
+        check(hipEventRecord(g.ready, s->stream));
+        check(hipStreamWaitEvent(gs, g.ready, 0));
+        check(hipGraphLaunch(g.exec[which], gs));
+        check(hipStreamSynchronize(gs));
+        const uint64_t* h = g.host + kGwWords * g.slot;
+        if (static_cast<uint32_t>(h[kGwError])) throw StatusError{SPARKALOG_HIP_ERR_DEVICE};
+        const uint64_t flags = h[kGwFlags];
+        if (flags) {
+            // grow to what the device counted, and run the iteration again
+            // (in the regular executor when FULL would pass graph_max_rows)
+            if (flags & 1) g.cap_cand = std::max(g.cap_cand, pow2_at_least(h[kGwJoin] + h[kGwJoin] / 4 + 1));
+            if (flags & 2) {
+                const uint64_t need = pow2_at_least(F + h[kGwUnique] + (F + h[kGwUnique]) / 4 + 1);
+                if (need > s->graph_max_rows) {
+                    ss.graph = false;
+                    session_step(s);
+                    return;
+                }
+                g.cap_full = std::max(g.cap_full, need);
+            }
+            continue;
+        }
+        const int nxt = g.slot ^ 1;
+        const uint64_t M = h[kGwJoin], U = h[kGwUnique];
+        const uint64_t F2 = h[kGwFull0 + nxt], N = h[kGwDelta0 + nxt];
+        if (F2 != F + N) throw StatusError{SPARKALOG_HIP_ERR_DEVICE};
+        std::swap(ts.full.keys, g.full_next);
+        std::swap(ts.delta.keys, g.delta_next);
+        ts.full.rows = F2;
+        ts.delta.rows = N;
+        ts.newt.rows = 0;
+        ts.full.touched();
+        ts.delta.touched();
+        ts.version += 1;
+        g.slot = nxt;
+        g.seen_full = ts.full.keys.ptr;
+        g.seen_delta = ts.delta.keys.ptr;
+        g.seen_full_rows = F2;
+        g.seen_delta_rows = N;
+        ss.total_new += N;
+        ss.inst_delta_valid = tracking;
+        const double R = static_cast<double>(rt.index.rows);
+        ss.algo_bytes += 8.0 * L + 8.0 * R + 16.0 * M + 8.0 * U + 8.0 * (F + U) + 8.0 * (F2 + N);
+        break;
+    }
+    ++ss.iterations;
 }
 
 int session_end(sparkalog_hip_store* s, sparkalog_hip_summary* summary, int rc) {

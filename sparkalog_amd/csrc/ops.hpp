@@ -210,6 +210,20 @@ hipError_t anti_join_x(const uint64_t* left, const uint64_t* left_rows, size_t b
                        uint32_t n_x, PairOutput out, uint64_t* count64, void* temp, size_t temp_bytes,
                        hipStream_t stream, const KernelTimer* timer = nullptr);
 
+// ---- one pass of anti-join and union for the graph-replayed iteration ----
+// For FULL (sorted unique packed pairs, count *full_rows <= full_cap) and the
+// distinct candidates (sorted unique, count *cand_rows <= cand_cap): writes
+// FULL ∪ candidates to full_out (count *full_out_rows) and candidates \ FULL
+// to newt_out (count *newt_rows), in order. Every count is read on the
+// device and every launch is sized by the capacities, so the calls can be
+// captured in a HIP graph; temp holds merge_new_temp_bytes(full_cap,
+// cand_cap).
+size_t merge_new_temp_bytes(size_t full_cap, size_t cand_cap);
+hipError_t merge_new(const uint64_t* full, const uint64_t* full_rows, size_t full_cap, const uint64_t* cand,
+                     const uint64_t* cand_rows, size_t cand_cap, uint64_t* full_out, uint64_t* full_out_rows,
+                     uint64_t* newt_out, uint64_t* newt_rows, void* temp, size_t temp_bytes, hipStream_t stream,
+                     const KernelTimer* timer = nullptr);
+
 // ---- indexed join (U32RangeIndex layout) ----
 size_t join_temp_bytes(size_t left_rows);
 hipError_t join_count(const uint32_t* left_keys, size_t left_rows, const uint32_t* index_keys,

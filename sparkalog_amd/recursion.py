@@ -219,6 +219,18 @@ class RelationStore:
         """Layout of the join index: "auto", "dense" or "hashed"."""
         self.set_option("join_index", self.JOIN_INDEX[layout])
 
+    def set_graph(self, enabled: bool) -> None:
+        """Replay each iteration of single-rule fixpoints from a captured HIP
+        graph (one host synchronization per iteration)."""
+        self.set_option("graph", int(bool(enabled)))
+
+    def graph_stats(self) -> tuple[int, int]:
+        """(graphs captured, graph replays) so far."""
+        c = ctypes.c_uint64(0)
+        r = ctypes.c_uint64(0)
+        _check(self._lib.sparkalog_hip_store_graph_stats(self._h, ctypes.byref(c), ctypes.byref(r)))
+        return c.value, r.value
+
     def set_profiling(self, enabled: bool) -> None:
         _check(self._lib.sparkalog_hip_store_set_profiling(self._h, int(enabled)))
 
