@@ -41,13 +41,22 @@ def parse_args():
     p.add_argument("--no-served", action="store_true", help="tc: leave out the served configuration")
     p.add_argument("--bench-plain-stride", type=int, default=8,
                    help="tc: the batched summary times the plain path on every k-th instance")
-    p.add_argument("--merge-ratio", type=int, default=3, help="tc: FULL run-merge ratio (results are identical)")
+    p.add_argument("--merge-ratio", type=int, default=None,
+                   help="tc: FULL run-merge ratio (default: the tuning file's; results are identical)")
     p.add_argument("--host-io", action="store_true",
                    help="tc: also time one fixpoint from host arrays to the closure back in host memory (PCIe)")
-    p.add_argument("--indexed-anti-join", action="store_true",
-                   help="tc: anti-join FULL through its first-column index (results are identical)")
-    p.add_argument("--sparse-probe-rows", type=int, default=1 << 22,
-                   help="tc: candidate count up to which FULL is probed one candidate per thread")
+    p.add_argument("--indexed-anti-join", type=int, default=None,
+                   help="tc: anti-join FULL through its first-column index, 0/1 (default: the tuning file's)")
+    p.add_argument("--sparse-probe-rows", type=int, default=None,
+                   help="tc: candidate count up to which FULL is probed one candidate per thread "
+                        "(default: the tuning file's)")
+    p.add_argument("--tuning", default="file",
+                   help="tuning file applied to every store ('file' = the package's or SPARKALOG_HIP_TUNING, "
+                        "'none' = built-in defaults, or a path)")
+    p.add_argument("--tuning-out", default="gpurun_out/tuning_gfx950.json", help="tune mode: file written")
+    p.add_argument("--tune-profiles", default="closure,batched", help="tune mode: profiles tuned")
+    p.add_argument("--no-default-compare", action="store_true",
+                   help="tc: skip timing the built-in default tuning beside the tuned one")
     p.add_argument("--cpu-sample-nodes", type=int, default=1 << 16,
                    help="CPU baseline: nodes of the parallel oracle's component-graph draw")
     p.add_argument("--cpu-serial-nodes", type=int, default=1 << 14,
@@ -56,11 +65,12 @@ def parse_args():
                    help="crossover mode: operators")
     p.add_argument("--crossover-min-log2", type=int, default=8, help="crossover mode: smallest log2 row count")
     p.add_argument("--crossover-max-log2", type=int, default=24, help="crossover mode: largest log2 row count")
-    p.add_argument("--mode", choices=["tc", "ops", "batched", "served", "copy", "crossover"], default="tc",
+    p.add_argument("--mode", choices=["tc", "ops", "batched", "served", "copy", "crossover", "tune"], default="tc",
                    help="tc: the headline fixpoint; ops: operator microbench at 2^k rows; "
                         "batched: 4,096 instances in one loop vs one run each; served: Poisson stream, "
                         "continuous batching vs sequential; copy: plain device copy of --copy-bytes "
-                        "(the known-size reference for HBM byte counters)")
+                        "(the known-size reference for HBM byte counters); tune: run the autotuner and "
+                        "write --tuning-out")
     p.add_argument("--copy-bytes", type=int, default=1 << 30, help="copy mode: bytes copied per launch")
     p.add_argument("--budget-pairs", type=float, default=3e9, help="batched/served: HBM budget in closure pairs")
     p.add_argument("--requests", type=int, default=200, help="served: stream length")
@@ -73,8 +83,9 @@ def parse_args():
                    help="served: closure-size bounds (pairs) between continuous-batching lanes")
     p.add_argument("--lane-slots", default="256,256,2",
                    help="served: instances in flight per lane (comma list; empty = 256 each)")
-    p.add_argument("--graph", type=int, default=1,
-                   help="batched/served: replay single-rule iterations from captured HIP graphs (1) or not (0)")
+    p.add_argument("--graph", type=int, default=-1,
+                   help="batched/served: replay single-rule iterations from captured HIP graphs (1) or not (0); "
+                        "-1 = the tuning file's choice")
     p.add_argument("--sizes", default="24,26,28", help="ops mode: log2 row counts")
     p.add_argument("--ops", default="distinct,union,anti_join,join", help="ops mode: operators")
     p.add_argument("--samples", type=int, default=5, help="ops mode: timed launches per case")
@@ -82,6 +93,15 @@ def parse_args():
                    help="ops mode: largest case checked against (and timed with) the oracle on this host; "
                         "larger ones are checked against the committed oracle fixtures")
     return p.parse_args()
+
+
+def _tuning_arg(args):
+    """RelationStore's `tuning` argument from --tuning."""
+    return {"file": "file", "none": None}.get(args.tuning, args.tuning)
+
+
+def _graph_arg(args):
+    return None if args.graph < 0 else bool(args.graph)
 
 
 def _cpu_threads() -> int:
@@ -175,7 +195,8 @@ def run_ops(args) -> None:
                 torch.cuda.empty_cache()
 
 
-def batched_summary(budget_pairs: float, plain_stride: int, full_log: bool = False, graph: bool = False) -> dict:
+def batched_summary(budget_pairs: float, plain_stride: int, full_log: bool = False, graph=None,
+                    tuning="file") -> dict:
     """Batched fixpoints: 4,096 instances (nodes log-uniform on [16, 16384],
     Poisson out-degree mean uniform on [1.2, 3.0]) in one semi-naive loop per
     HBM wave, against one executor run per instance (every plain_stride-th
@@ -189,8 +210,10 @@ def batched_summary(budget_pairs: float, plain_stride: int, full_log: bool = Fal
 
     torch.cuda.set_device(0)
     s, d, offs, counts = workload.batch_graphs()
-    store = RelationStore(HipStream())
-    store.set_graph(graph)
+    store = RelationStore(HipStream(), tuning=tuning, profile="batched")
+    if graph is not None:
+        store.set_graph(graph)
+    graph = bool(store.tuning.get("graph", 0)) if graph is None else graph
     # a cold pass first: its waves grow the store's workspaces and pools
     # (multi-GB allocations), then the measured pass on warm buffers, as the
     # plain path after it runs
@@ -220,13 +243,14 @@ def batched_summary(budget_pairs: float, plain_stride: int, full_log: bool = Fal
 
 
 def run_batched_mode(args) -> None:
-    r = batched_summary(args.budget_pairs, args.plain_stride, full_log=True, graph=bool(args.graph))
+    r = batched_summary(args.budget_pairs, args.plain_stride, full_log=True, graph=_graph_arg(args),
+                        tuning=_tuning_arg(args))
     r["mode"] = "batched"
     print(json.dumps(r), flush=True)
 
 
 def served_summary(budget_pairs: float, requests: int, utilizations: str, lane_bounds: str, lane_slots: str,
-                   one_lane: bool = True, graph: bool = False) -> dict:
+                   one_lane: bool = True, graph=None, tuning="file") -> dict:
     """Served fixpoints: a Poisson stream of TC requests (edges log-uniform
     on [2^8, 2^20]) at arrival rates giving the plain path each requested
     utilisation (its mean service time measured on the same requests served
@@ -239,8 +263,10 @@ def served_summary(budget_pairs: float, requests: int, utilizations: str, lane_b
     from sparkalog_amd.recursion import RelationStore
 
     torch.cuda.set_device(0)
-    store = RelationStore(HipStream())
-    store.set_graph(graph)
+    store = RelationStore(HipStream(), tuning=tuning, profile="batched")
+    if graph is not None:
+        store.set_graph(graph)
+    graph = bool(store.tuning.get("graph", 0)) if graph is None else graph
     # the plain path's service time on these very requests (same seed: the
     # sizes do not depend on the rate), all present at the start
     back_to_back = serving.make_requests(requests, rate_per_s=1e9)
@@ -248,7 +274,8 @@ def served_summary(budget_pairs: float, requests: int, utilizations: str, lane_b
     _, secs = serving.serve_plain(back_to_back, store)
     mean_service = secs / requests
     bounds = [float(b) for b in lane_bounds.split(",")] if lane_bounds else []
-    lanes = [RelationStore(HipStream(torch.cuda.Stream())) for _ in range(len(bounds) + 1)]
+    lanes = [RelationStore(HipStream(torch.cuda.Stream()), tuning=tuning, profile="batched")
+             for _ in range(len(bounds) + 1)]
     for lane in lanes:
         lane.set_graph(graph)
     slots = [int(x) for x in lane_slots.split(",")] if lane_slots else None
@@ -277,7 +304,7 @@ def served_summary(budget_pairs: float, requests: int, utilizations: str, lane_b
 
 def run_served_mode(args) -> None:
     r = served_summary(args.budget_pairs, args.requests, args.utilization, args.lane_bounds, args.lane_slots,
-                       graph=bool(args.graph))
+                       graph=_graph_arg(args), tuning=_tuning_arg(args))
     r["mode"] = "served"
     print(json.dumps(r), flush=True)
 
@@ -383,6 +410,26 @@ def run_crossover(args) -> None:
                       "gb10": {k: vars(v) for k, v in placement.MEASURED_GB10.items()}}), flush=True)
 
 
+def run_tune_mode(args) -> None:
+    """Autotuner (sparkalog_amd.autotune): coordinate descent over the
+    executor's knobs on each profile's calibration draw; writes the tuning
+    file and prints one JSON line per profile (the sweep's trace stays in
+    the file)."""
+    import torch
+
+    from sparkalog_amd import autotune
+
+    torch.cuda.set_device(0)
+    profiles = tuple(p for p in args.tune_profiles.split(",") if p)
+    result = autotune.tune(profiles, log=lambda m: print(m, flush=True))
+    out = Path(args.tuning_out)
+    out.parent.mkdir(parents=True, exist_ok=True)
+    autotune.write(result, out)
+    for name, prof in result["profiles"].items():
+        print(json.dumps({"mode": "tune", "profile": name, "file": str(out),
+                          **{k: v for k, v in prof.items() if k != "trace"}}), flush=True)
+
+
 def run_copy_mode(args) -> None:
     """The library's plain device copy of --copy-bytes, launched --steps
     times and timed with HIP events on its stream: the measured HBM copy
@@ -479,10 +526,13 @@ def main():
     if args.mode == "crossover":
         run_crossover(args)
         return
+    if args.mode == "tune":
+        run_tune_mode(args)
+        return
     import numpy as np
     import torch
 
-    from sparkalog_amd import plans, workload
+    from sparkalog_amd import autotune, plans, workload
     from sparkalog_amd.execution import HipStream
     from sparkalog_amd.recursion import RecursiveExecutor, RelationStore
 
@@ -492,18 +542,21 @@ def main():
     d_src = torch.from_numpy(src.view(np.int32)).cuda()
     d_dst = torch.from_numpy(dst.view(np.int32)).cuda()
     stream = HipStream()
-    store = RelationStore(stream)
-    store.set_tuning(merge_ratio=args.merge_ratio, indexed_anti_join=args.indexed_anti_join,
-                     sparse_probe_rows=args.sparse_probe_rows)
+    store = RelationStore(stream, tuning=_tuning_arg(args))
+    overrides = {k: v for k, v in (("merge_ratio", args.merge_ratio), ("indexed_anti_join", args.indexed_anti_join),
+                                   ("sparse_probe_rows", args.sparse_probe_rows)) if v is not None}
+    autotune.apply(store, overrides)
+    tuned_values = {**store.tuning, **overrides}
     R = plans.RelationId
     plan = plans.transitive_closure_scc(R(1), R(0))
 
-    def step(profile: bool):
-        store.clear()
-        store.insert_static(R(0), (d_src, d_dst))
-        store.insert_recursive(R(1), (d_src, d_dst), (d_src, d_dst))
-        ex = RecursiveExecutor.compile(plan, store)
-        return ex.run(store, 10_000)
+    def step(profile: bool, on=None):
+        st = store if on is None else on
+        st.clear()
+        st.insert_static(R(0), (d_src, d_dst))
+        st.insert_recursive(R(1), (d_src, d_dst), (d_src, d_dst))
+        ex = RecursiveExecutor.compile(plan, st)
+        return ex.run(st, 10_000)
 
     for _ in range(args.warmup):
         step(False)
@@ -579,6 +632,25 @@ def main():
                 for k, v in ops.items() if v["launches"]},
         "roofline": roofline,
     }
+    tuning_file = autotune.default_file() if args.tuning == "file" else (None if args.tuning == "none"
+                                                                          else Path(args.tuning))
+    line["tuning"] = {"file": tuning_file.name if tuning_file else None, "profile": "closure",
+                      "values": tuned_values}
+    if tuned_values and not args.no_default_compare:
+        # the same fixpoint under the built-in defaults, on a store of its own
+        base = RelationStore(stream, tuning=None)
+        step(False, base)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        base_summ = [step(False, base) for _ in range(max(2, args.steps))]
+        torch.cuda.synchronize()
+        base_ms = 1000.0 * (time.perf_counter() - t0) / len(base_summ)
+        line["tuning"].update({
+            "default_ms_per_step": round(base_ms, 3), "tuned_ms_per_step": round(ms_per_step, 3),
+            "speedup_vs_default": round(base_ms / ms_per_step, 4),
+            "results_equal_default": base_summ[-1].relation_rows == summaries[-1].relation_rows and
+            base.digest(R(1)) == store.digest(R(1))})
+        base.close()
     if args.host_io:
         # edges uploaded from host arrays (pinned staging), fixpoint, closure
         # read back to host memory: the rate a caller handing over host
@@ -600,10 +672,12 @@ def main():
         # the batched configuration (4,096 instances) beside the plain path
         del store
         torch.cuda.empty_cache()
-        line["batched"] = batched_summary(args.budget_pairs, args.bench_plain_stride, graph=bool(args.graph))
+        line["batched"] = batched_summary(args.budget_pairs, args.bench_plain_stride, graph=_graph_arg(args),
+                                          tuning=_tuning_arg(args))
     if not args.no_served:
         line["served"] = served_summary(args.budget_pairs, args.requests, args.bench_utilization, args.lane_bounds,
-                                        args.lane_slots, one_lane=False, graph=bool(args.graph))
+                                        args.lane_slots, one_lane=False, graph=_graph_arg(args),
+                                        tuning=_tuning_arg(args))
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample_nodes, args.cpu_serial_nodes)
     print(json.dumps(line), flush=True)

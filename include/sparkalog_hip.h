@@ -161,6 +161,45 @@ sparkalog_hip_error sparkalog_hip_union_u32(const uint32_t* left_first, const ui
                                             uint32_t* output_second, void* temporary, size_t temporary_bytes,
                                             void* stream);
 
+/* ---- device text ingest ----
+ * load_binary_u32 (crates/sparkalog-storage/src/io.rs:72-141) on the device:
+ * a whitespace-separated text of two u32 fields per line, already in device
+ * memory (`text` 16-byte aligned, `len` bytes), parsed into two device
+ * columns in file order. Two phases, as the reference reads the file twice
+ * (io.rs:75-83): sparkalog_hip_text_lines writes the line count (u64,
+ * device), which bounds the rows and sizes the columns and the temporary of
+ * sparkalog_hip_load_binary_u32; that call writes *rows (u64, device) and
+ * *error (u64, device): SPARKALOG_HIP_LOAD_OK, or the reference's first
+ * error (a Git LFS pointer line anywhere, io.rs:97-99; else the first line
+ * with other than two fields, io.rs:120-125, or a field that is not a u32,
+ * io.rs:126-127, first field before second). Lines are split at '\n' (a
+ * trailing '\r' is whitespace), trimmed, and skipped when empty or starting
+ * with '#' or '%' (io.rs:86-89); fields are split at ASCII whitespace and
+ * parsed as Rust's str::parse::<u32> does (an optional '+', then decimal
+ * digits, at most 4294967295). Differences: non-ASCII whitespace does not
+ * separate fields, and bytes that are not UTF-8 are reported as a bad field
+ * rather than an I/O error. Both calls are asynchronous on `stream`. */
+#define SPARKALOG_HIP_LOAD_OK UINT64_MAX
+#define SPARKALOG_HIP_LOAD_ERR_CAPACITY 1ull /* the text has more lines than `lines` */
+#define SPARKALOG_HIP_LOAD_ERR_LFS_POINTER 0ull
+#define SPARKALOG_HIP_LOAD_ERR_FIELD_COUNT ((1ull << 62) | 1ull)
+#define SPARKALOG_HIP_LOAD_ERR_U32_FIRST ((1ull << 62) | 2ull)
+#define SPARKALOG_HIP_LOAD_ERR_U32_SECOND ((1ull << 62) | 3ull)
+/* An error code other than OK and CAPACITY is KIND | (line << 2), line
+ * 1-based as in the reference's messages. */
+#define SPARKALOG_HIP_LOAD_ERROR_KIND(code) ((code) & ((3ull << 62) | 3ull))
+#define SPARKALOG_HIP_LOAD_ERROR_LINE(code) (((code) >> 2) & ((1ull << 60) - 1))
+sparkalog_hip_error sparkalog_hip_text_lines_temporary_bytes(size_t len, size_t* temporary_bytes);
+sparkalog_hip_error sparkalog_hip_text_lines(const char* text, size_t len, uint64_t* lines, void* temporary,
+                                             size_t temporary_bytes, void* stream);
+sparkalog_hip_error sparkalog_hip_load_binary_u32_temporary_bytes(size_t len, uint64_t lines,
+                                                                  size_t* temporary_bytes);
+/* `first` and `second` hold at least `lines` entries (the count from
+ * sparkalog_hip_text_lines). */
+sparkalog_hip_error sparkalog_hip_load_binary_u32(const char* text, size_t len, uint64_t lines, uint32_t* first,
+                                                  uint32_t* second, uint64_t* rows, uint64_t* error,
+                                                  void* temporary, size_t temporary_bytes, void* stream);
+
 /* ------------------------------------------------------------------ */
 /* Part 2: device-resident relation store and semi-naive executor      */
 
@@ -287,11 +326,22 @@ int sparkalog_hip_store_set_tuning(sparkalog_hip_store* store, uint32_t merge_ra
  *     run the regular executor); 0 = off (default).
  *   "graph_max_rows": FULL capacity (rows, 2^16..2^31, default 2^25) past
  *     which a graph-mode fixpoint continues in the regular executor, whose
- *     size-tiered FULL is not rewritten every iteration. */
+ *     size-tiered FULL is not rewritten every iteration.
+ *   "distinct_tile": nominal tile of the segmented distinct in keys (512..4096,
+ *     a multiple of 64; default 3072): tiles are cut every distinct_tile keys
+ *     and moved back to run starts, and each is sorted in LDS.
+ *   "merge_ratio", "segmented_distinct", "indexed_anti_join",
+ *     "sparse_probe_rows": the knobs of sparkalog_hip_store_set_tuning, one at
+ *     a time (merge_ratio 2..64, the two switches 0/1). */
 int sparkalog_hip_store_set_option(sparkalog_hip_store* store, const char* name, uint64_t value);
 /* Graphs captured and graph replays of graph-mode iterations since the store
  * was created (no counterpart in the reference). */
 int sparkalog_hip_store_graph_stats(sparkalog_hip_store* store, uint64_t* captures, uint64_t* replays);
+/* Of those captures, how many updated an executable graph in place (the same
+ * launches over other buffers or sizes) rather than instantiating a new one,
+ * and the host milliseconds spent capturing and instantiating or updating
+ * (no counterpart in the reference). */
+int sparkalog_hip_store_graph_capture_stats(sparkalog_hip_store* store, uint64_t* updates, double* capture_ms);
 int sparkalog_hip_store_op_stats(sparkalog_hip_store* store, int op, uint64_t* launches, double* ms,
                                  double* bytes);
 /* The operator's name and main kernel, and that kernel's own launches and

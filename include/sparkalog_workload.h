@@ -106,6 +106,20 @@ uint64_t sparkalog_wl_packed_digest(const uint64_t* packed, uint64_t rows);
 /* The mixing function of the digest (the SplitMix64 finaliser). */
 uint64_t sparkalog_wl_mix64(uint64_t key);
 
+/*
+ * Text of `rows` binary u32 facts in the reference loader's format
+ * (crates/sparkalog-storage/src/io.rs:72-141), with the values written to
+ * first/second (caller buffers of `rows` entries) as the ground truth of a
+ * parse. Values are uniform on [0, key_space) (0 = the whole u32 range).
+ * style bits: 1 = mixed whitespace (tabs, runs, leading/trailing), 2 = a
+ * comment or blank line before about one row in 64, 4 = CRLF on about half
+ * the lines, 8 = '+' signs and leading zeros, 16 = no newline after the
+ * last row. *text is malloc'd (release with sparkalog_wl_free); returns its
+ * length, or UINT64_MAX on allocation failure.
+ */
+uint64_t sparkalog_wl_edge_text(uint64_t seed, uint64_t rows, uint32_t key_space, uint32_t style,
+                                char** text, uint32_t* first, uint32_t* second);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,7 +16,8 @@ HERE = Path(__file__).resolve().parent
 LIB = HERE / "_build" / "liboracle.so"
 
 __all__ = ["Rule", "Summary", "BfsResult", "lib", "build", "distinct", "anti_join", "union", "range_index",
-           "join", "filter_rows", "tc_bfs", "general_tc", "Store", "rule_from_plan", "ORACLE_ITERATION_LIMIT"]
+           "join", "filter_rows", "tc_bfs", "general_tc", "Store", "rule_from_plan", "ORACLE_ITERATION_LIMIT",
+           "load_binary_u32", "LOAD_ERRORS"]
 
 ORACLE_ITERATION_LIMIT = 4
 
@@ -86,8 +87,35 @@ def lib() -> ctypes.CDLL:
         L.oracle_general_tc.restype = c_uint64
         L.oracle_general_tc.argtypes = [u32, u32, c_uint64, POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64),
                                         u32, u32, u32, u32, c_uint64]
+        L.oracle_load_binary_u32.restype = c_uint64
+        L.oracle_load_binary_u32.argtypes = [ctypes.c_char_p, c_uint64, u32, u32, c_uint64, POINTER(LoadError)]
         _lib = L
     return _lib
+
+
+class LoadError(ctypes.Structure):
+    _fields_ = [("kind", c_int), ("line", c_uint64), ("column", c_uint32), ("value_offset", c_uint64),
+                ("value_len", c_uint64)]
+
+
+LOAD_ERRORS = {1: "GitLfsPointer", 2: "InvalidFieldCount", 3: "InvalidU32", 4: "Capacity"}
+
+
+def load_binary_u32(text: bytes):
+    """(first, second, None) or (None, None, error dict) for an in-memory
+    text (storage/src/io.rs:72-141)."""
+    cap = text.count(b"\n") + 1
+    first = np.empty(cap, np.uint32)
+    second = np.empty(cap, np.uint32)
+    err = LoadError()
+    n = lib().oracle_load_binary_u32(text, len(text), _p(first), _p(second), cap, ctypes.byref(err))
+    if err.kind:
+        e = {"kind": LOAD_ERRORS[err.kind], "line": err.line}
+        if err.kind == 3:
+            e["column"] = err.column
+            e["value"] = text[err.value_offset:err.value_offset + err.value_len].decode("utf-8", "replace")
+        return None, None, e
+    return first[:n], second[:n], None
 
 
 def _u32(x) -> np.ndarray:

@@ -1152,3 +1152,87 @@ uint64_t oracle_general_tc(const uint32_t* src, const uint32_t* dst, uint64_t ed
     free(nonlocal.v);
     return path_rows;
 }
+
+/* ---- text loader (storage/src/io.rs:72-141) ---- */
+
+static int ascii_ws(unsigned char c) { return c == ' ' || (c >= 9 && c <= 13); }
+
+/* One line [s, e) after str::trim; 0 when it is not a data line (io.rs:86-89). */
+static int data_line_span(const char* t, uint64_t* s, uint64_t* e) {
+    while (*s < *e && ascii_ws((unsigned char)t[*s])) ++*s;
+    while (*e > *s && ascii_ws((unsigned char)t[*e - 1])) --*e;
+    return *s < *e && t[*s] != '#' && t[*s] != '%';
+}
+
+/* str::parse::<u32>: an optional '+', then at least one digit, no overflow. */
+static int rust_parse_u32(const char* t, uint64_t s, uint64_t e, uint32_t* out) {
+    uint64_t v = 0;
+    if (s < e && t[s] == '+') ++s;
+    if (s == e) return 0;
+    for (; s < e; ++s) {
+        if (t[s] < '0' || t[s] > '9') return 0;
+        v = v * 10 + (uint64_t)(t[s] - '0');
+        if (v > 0xFFFFFFFFull) return 0;
+    }
+    *out = (uint32_t)v;
+    return 1;
+}
+
+uint64_t oracle_load_binary_u32(const char* text, uint64_t len, uint32_t* first, uint32_t* second,
+                                uint64_t capacity, oracle_load_error* error) {
+    static const char lfs[] = "version https://git-lfs.github.com/spec/v1";
+    const uint64_t lfs_len = sizeof(lfs) - 1;
+    memset(error, 0, sizeof(*error));
+    /* count_rows (io.rs:91-103): a pointer line anywhere fails the load */
+    uint64_t line = 0;
+    for (uint64_t pos = 0; pos < len; ++line) {
+        uint64_t end = pos;
+        while (end < len && text[end] != '\n') ++end;
+        uint64_t s = pos, e = end;
+        if (data_line_span(text, &s, &e) && e - s == lfs_len && memcmp(text + s, lfs, lfs_len) == 0) {
+            error->kind = ORACLE_LOAD_LFS_POINTER;
+            error->line = line + 1;
+            return 0;
+        }
+        pos = end + 1;
+    }
+    /* populate_columns (io.rs:105-131) */
+    uint64_t rows = 0;
+    line = 0;
+    for (uint64_t pos = 0; pos < len; ++line) {
+        uint64_t end = pos;
+        while (end < len && text[end] != '\n') ++end;
+        uint64_t s = pos, e = end;
+        pos = end + 1;
+        if (!data_line_span(text, &s, &e)) continue;
+        uint64_t a_end = s;
+        while (a_end < e && !ascii_ws((unsigned char)text[a_end])) ++a_end;
+        uint64_t b = a_end;
+        while (b < e && ascii_ws((unsigned char)text[b])) ++b;
+        uint64_t b_end = b;
+        while (b_end < e && !ascii_ws((unsigned char)text[b_end])) ++b_end;
+        if (b == e || b_end != e) {
+            error->kind = ORACLE_LOAD_FIELD_COUNT;
+            error->line = line + 1;
+            return 0;
+        }
+        uint32_t x, y;
+        if (!rust_parse_u32(text, s, a_end, &x) || !rust_parse_u32(text, b, b_end, &y)) {
+            const int col1 = !rust_parse_u32(text, s, a_end, &x);
+            error->kind = ORACLE_LOAD_INVALID_U32;
+            error->line = line + 1;
+            error->column = col1 ? 1 : 2;
+            error->value_offset = col1 ? s : b;
+            error->value_len = col1 ? a_end - s : b_end - b;
+            return 0;
+        }
+        if (rows >= capacity) {
+            error->kind = ORACLE_LOAD_CAPACITY;
+            return 0;
+        }
+        first[rows] = x;
+        second[rows] = y;
+        ++rows;
+    }
+    return rows;
+}

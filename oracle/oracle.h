@@ -78,6 +78,32 @@ uint64_t oracle_join(const uint32_t* left_keys, uint64_t l, const uint32_t* inde
 uint64_t oracle_filter(const uint32_t* input, uint64_t n, uint32_t op, uint32_t operand,
                        uint32_t* out_rows, int threads);
 
+/* ---- text loader: storage/src/io.rs:72-141 ---- */
+
+enum {
+    ORACLE_LOAD_OK = 0,
+    ORACLE_LOAD_LFS_POINTER = 1,   /* io.rs:97-99 (checked over every line first) */
+    ORACLE_LOAD_FIELD_COUNT = 2,   /* io.rs:120-125 */
+    ORACLE_LOAD_INVALID_U32 = 3,   /* io.rs:136-141 */
+    ORACLE_LOAD_CAPACITY = 4       /* more rows than `capacity` */
+};
+
+typedef struct {
+    int kind;
+    uint64_t line;         /* 1-based, as the reference reports it */
+    uint32_t column;       /* 1 or 2 for ORACLE_LOAD_INVALID_U32 */
+    uint64_t value_offset; /* the offending field, for ORACLE_LOAD_INVALID_U32 */
+    uint64_t value_len;
+} oracle_load_error;
+
+/* load_binary_u32 over an in-memory text: lines split at '\n', trimmed of
+ * ASCII whitespace, blank and '#'/'%' lines skipped (io.rs:86-89), two
+ * whitespace-separated fields parsed as Rust's str::parse::<u32> (an optional
+ * '+', decimal digits, no overflow). Returns the rows written to
+ * first/second, or 0 with *error set. */
+uint64_t oracle_load_binary_u32(const char* text, uint64_t len, uint32_t* first, uint32_t* second,
+                                uint64_t capacity, oracle_load_error* error);
+
 /* ---- semi-naive executor: recursion/src/generic.rs:16-534 ---- */
 
 typedef struct {

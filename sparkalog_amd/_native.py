@@ -58,6 +58,10 @@ _HIP_SIGNATURES = {
     "sparkalog_hip_anti_join_u32": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
     "sparkalog_hip_union_u32_temporary_bytes": (c_int, [c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p, POINTER(c_size_t), c_void_p]),
     "sparkalog_hip_union_u32": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "sparkalog_hip_text_lines_temporary_bytes": (c_int, [c_size_t, POINTER(c_size_t)]),
+    "sparkalog_hip_text_lines": (c_int, [c_void_p, c_size_t, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "sparkalog_hip_load_binary_u32_temporary_bytes": (c_int, [c_size_t, c_uint64, POINTER(c_size_t)]),
+    "sparkalog_hip_load_binary_u32": (c_int, [c_void_p, c_size_t, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
     "sparkalog_hip_store_create": (c_int, [POINTER(c_void_p), c_void_p]),
     "sparkalog_hip_store_destroy": (None, [c_void_p]),
     "sparkalog_hip_store_insert_static": (c_int, [c_void_p, c_uint32, c_void_p, c_void_p, c_uint64, c_int]),
@@ -70,6 +74,7 @@ _HIP_SIGNATURES = {
     "sparkalog_hip_store_set_tuning": (c_int, [c_void_p, c_uint32, c_int, c_int, c_uint64]),
     "sparkalog_hip_store_set_option": (c_int, [c_void_p, ctypes.c_char_p, c_uint64]),
     "sparkalog_hip_store_graph_stats": (c_int, [c_void_p, u64p, u64p]),
+    "sparkalog_hip_store_graph_capture_stats": (c_int, [c_void_p, u64p, ctypes.POINTER(ctypes.c_double)]),
     "sparkalog_hip_store_op_stats": (c_int, [c_void_p, c_int, u64p, POINTER(c_double), POINTER(c_double)]),
     "sparkalog_hip_store_op_kernel_stats": (c_int, [c_void_p, c_int, POINTER(ctypes.c_char_p), POINTER(ctypes.c_char_p),
                                                      u64p, POINTER(c_double)]),
@@ -96,6 +101,7 @@ _WORKLOAD_SIGNATURES = {
     "sparkalog_wl_splitmix64": (c_uint64, [u64p]),
     "sparkalog_wl_free": (None, [c_void_p]),
     "sparkalog_wl_random_dag": (c_int, [c_uint64, c_uint32, c_uint64, c_void_p, c_void_p]),
+    "sparkalog_wl_edge_text": (c_uint64, [c_uint64, c_uint64, c_uint32, c_uint32, POINTER(c_void_p), u32p, u32p]),
     "sparkalog_wl_component_graph": (c_uint64, [c_uint64, c_uint32, c_double, c_uint32, c_uint32, c_double, POINTER(u32p), POINTER(u32p), u32p]),
     "sparkalog_wl_batch_graphs": (c_uint64, [c_uint64, c_uint32, c_uint32, c_uint32, c_double, c_double, POINTER(u32p), POINTER(u32p), POINTER(u64p), POINTER(u32p)]),
     "sparkalog_wl_pairs": (None, [c_uint64, c_uint64, c_int, c_uint32, c_double, c_void_p, c_void_p]),

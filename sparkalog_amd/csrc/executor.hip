@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -828,6 +829,8 @@ struct GraphRun {
     hipEvent_t ready = nullptr;
     int next_evict = 0;
     uint64_t captures = 0, replays = 0;
+    uint64_t updates = 0;     // captures applied to an existing executable graph
+    double capture_ms = 0;    // host time spent capturing and instantiating
     void drop_graphs() {
         for (auto& e : exec)
             if (e) {
@@ -953,6 +956,7 @@ struct sparkalog_hip_store {
     uint64_t graph_max_rows = 1ull << 25;  // larger FULLs leave graph mode
     spk::GraphRun graph;
     uint64_t sparse_probe_rows = spk::kSparseProbeRows;
+    uint32_t seg_tile = spk::kSegTileDefault;  // nominal tile of the segmented distinct
     // small persistent device buffers for range probes and statistics (a
     // fresh allocation would synchronize the whole device when freed)
     spk::DevBuf<uint64_t> small_dev, probe_dev;
@@ -1988,6 +1992,13 @@ extern "C" int sparkalog_hip_store_graph_stats(sparkalog_hip_store* s, uint64_t*
     return SPARKALOG_HIP_OK;
 }
 
+extern "C" int sparkalog_hip_store_graph_capture_stats(sparkalog_hip_store* s, uint64_t* updates, double* capture_ms) {
+    if (!s) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+    if (updates) *updates = s->graph.updates;
+    if (capture_ms) *capture_ms = s->graph.capture_ms;
+    return SPARKALOG_HIP_OK;
+}
+
 extern "C" int sparkalog_hip_store_set_option(sparkalog_hip_store* s, const char* name, uint64_t value) {
     if (!s || !name) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
     const std::string key(name);
@@ -2006,6 +2017,26 @@ extern "C" int sparkalog_hip_store_set_option(sparkalog_hip_store* s, const char
         s->join_index_mode = static_cast<int>(value);
         // cached indexes are rebuilt in the new layout
         for (auto& rt : s->ws_rules) rt.index.valid = false;
+        return SPARKALOG_HIP_OK;
+    }
+    if (key == "distinct_tile") {
+        if (value < 512 || value > 4096 || value % 64 != 0) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+        s->seg_tile = static_cast<uint32_t>(value);
+        return SPARKALOG_HIP_OK;
+    }
+    // the set_tuning knobs one at a time
+    if (key == "merge_ratio") {
+        if (value < 2 || value > 64) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+        s->merge_ratio = static_cast<uint32_t>(value);
+        return SPARKALOG_HIP_OK;
+    }
+    if (key == "sparse_probe_rows") {
+        s->sparse_probe_rows = value;
+        return SPARKALOG_HIP_OK;
+    }
+    if (key == "segmented_distinct" || key == "indexed_anti_join") {
+        if (value > 1) return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
+        (key == "segmented_distinct" ? s->segmented_distinct : s->x_anti) = value != 0;
         return SPARKALOG_HIP_OK;
     }
     return SPARKALOG_HIP_ERR_INVALID_ARGUMENT;
@@ -2148,6 +2179,9 @@ int session_begin(sparkalog_hip_store* s, const uint32_t* relations, uint32_t n_
         ss.graph = right && !right->recursive;
     }
     s->graph.seen_full = s->graph.seen_delta = nullptr;  // device row counts rewritten at the first step
+    // launch capacities follow this fixpoint's sizes (the buffers keep their
+    // allocations): a small fixpoint after a large one launches small grids
+    s->graph.cap_full = s->graph.cap_cand = s->graph.cap_delta = 0;
     ss.iterations = ss.total_new = 0;
     ss.algo_bytes = 0.0;
     s->events_used = 0;
@@ -2459,12 +2493,12 @@ void session_step(sparkalog_hip_store* s) {
         ev = s->begin_op();
         rt.distinct_slot = ev;
         if (grouped && s->segmented_distinct) {
-            const size_t mb = segmented_distinct_meta_bytes(M);
+            const size_t mb = segmented_distinct_meta_bytes(M, s->seg_tile);
             rt.seg_meta.reserve(mb);
             uint64_t& side_rows = s->words_host[1];
             side_rows = 0;
             check(segmented_distinct_prepare(rt.cand_a.ptr, M, rt.shift, rt.cand_b.ptr, rt.seg_meta.ptr, mb,
-                                             &side_rows, s->stream, found));
+                                             &side_rows, s->stream, found, s->seg_tile));
             size_t tb = 0;
             void* temp = nullptr;
             if (side_rows) {
@@ -2487,7 +2521,7 @@ void session_step(sparkalog_hip_store* s) {
             check(segmented_distinct_finish(rt.cand_a.ptr, M, rt.shift, known_compressed, rt.cand_b.ptr,
                                             side_rows ? rt.cand_c.ptr : nullptr, side_rows, out, count,
                                             rt.seg_meta.ptr, mb, temp, tb, s->stream, s->kernel_timer(ev), found,
-                                            xidx_out));
+                                            xidx_out, s->seg_tile));
             if (xidx_out) {
                 // the new run's index: entries of first values without new rows
                 const size_t ftb = xindex_fill_temp_bytes(n_x);
@@ -2903,10 +2937,7 @@ void graph_step(sparkalog_hip_store* s) {
         if (which < 0) {
             which = g.next_evict;
             g.next_evict ^= 1;
-            if (g.exec[which]) {
-                check(hipGraphExecDestroy(g.exec[which]));
-                g.exec[which] = nullptr;
-            }
+            const auto capture_t0 = std::chrono::steady_clock::now();
             // capture one iteration's launches (no allocation, no memset,
             // no synchronization inside)
             const int cur = g.slot, nxt = g.slot ^ 1;
@@ -3000,11 +3031,31 @@ void graph_step(sparkalog_hip_store* s) {
             }
             hipGraph_t graph = nullptr;
             check(hipStreamEndCapture(gs, &graph));
-            const hipError_t inst = hipGraphInstantiate(&g.exec[which], graph, nullptr, nullptr, 0);
+            // a graph of the same launches with other pointers or sizes (a new
+            // layout, a new fixpoint) updates the slot's executable graph in
+            // place; one that differs in its launches is instantiated anew
+            bool updated = false;
+            if (g.exec[which]) {
+                hipGraphNode_t error_node = nullptr;
+                hipGraphExecUpdateResult result = hipGraphExecUpdateError;
+                const hipError_t ue = hipGraphExecUpdate(g.exec[which], graph, &error_node, &result);
+                updated = ue == hipSuccess && result == hipGraphExecUpdateSuccess;
+                if (!updated) {
+                    (void)hipGetLastError();
+                    (void)hipGraphExecDestroy(g.exec[which]);
+                    g.exec[which] = nullptr;
+                }
+            }
+            hipError_t inst = hipSuccess;
+            if (!updated) inst = hipGraphInstantiate(&g.exec[which], graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
+            if (inst != hipSuccess) g.exec[which] = nullptr;
             check(inst);
             g.key[which] = key;
             ++g.captures;
+            if (updated) ++g.updates;
+            g.capture_ms +=
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - capture_t0).count();
         } else {
             ++g.replays;
         }

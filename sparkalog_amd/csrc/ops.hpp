@@ -91,15 +91,19 @@ hipError_t unique_sorted(const uint64_t* keys, const uint64_t* rows_dev, size_t 
 // sorted and unique. With `xidx_out` (all entries 0xFFFFFFFF, from
 // xindex_clear), finish also marks the first row of each first value in the
 // output's first-column index (completed by xindex_fill).
-size_t segmented_distinct_meta_bytes(size_t rows);
+// `tile_rows` is the nominal tile (512..4096 keys, 3,072 by default): tile
+// boundaries sit every tile_rows keys before they move back to run starts.
+constexpr uint32_t kSegTileDefault = 3072;
+size_t segmented_distinct_meta_bytes(size_t rows, uint32_t tile_rows = kSegTileDefault);
 hipError_t segmented_distinct_prepare(const uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t* side,
                                       void* meta, size_t meta_bytes, uint64_t* side_rows, hipStream_t stream,
-                                      const uint64_t* found = nullptr);
+                                      const uint64_t* found = nullptr, uint32_t tile_rows = kSegTileDefault);
 hipError_t segmented_distinct_finish(uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t side_varying,
                                      uint64_t* side_a, uint64_t* side_b, uint64_t side_rows, PairOutput out,
                                      uint64_t* unique_rows, void* meta, size_t meta_bytes, void* sort_temp,
                                      size_t sort_bytes, hipStream_t stream, const KernelTimer* timer = nullptr,
-                                     const uint64_t* found = nullptr, uint32_t* xidx_out = nullptr);
+                                     const uint64_t* found = nullptr, uint32_t* xidx_out = nullptr,
+                                     uint32_t tile_rows = kSegTileDefault);
 
 // ---- sorted set operators ----
 size_t union_temp_bytes(size_t left_rows, size_t right_rows);

@@ -25,7 +25,6 @@
 namespace spk {
 namespace {
 
-constexpr int kSegTile = 3072;                // nominal tile
 constexpr int kSegCap = 4096;                 // largest tile sorted in LDS
 constexpr int kSegBlock = 512;                // threads of a tile block (8 waves)
 constexpr int kSegWaves = kSegBlock / kWave;
@@ -46,7 +45,7 @@ __device__ __forceinline__ void store_pair(const PairOutput& out, size_t i, uint
     }
 }
 
-__host__ __device__ inline size_t seg_tiles(size_t n) { return (n + kSegTile - 1) / kSegTile; }
+__host__ __device__ inline size_t seg_tiles(size_t n, uint32_t tile) { return (n + tile - 1) / tile; }
 __host__ __device__ inline size_t seg_align(size_t v) { return (v + 255) / 256 * 256; }
 
 struct SegHeader {
@@ -62,9 +61,9 @@ struct SegLayout {
 
 // Metadata of one segmented distinct: header, per-tile unique counts (then
 // offsets) and tile bounds.
-SegLayout seg_layout(size_t rows) {
+SegLayout seg_layout(size_t rows, uint32_t tile) {
     SegLayout l{};
-    l.tiles = seg_tiles(rows);
+    l.tiles = seg_tiles(rows, tile);
     size_t off = seg_align(sizeof(SegHeader));
     l.status_off = off;
     off += seg_align(l.tiles * sizeof(uint64_t));
@@ -74,11 +73,11 @@ SegLayout seg_layout(size_t rows) {
     return l;
 }
 
-// bounds[t] = start of the run containing position t*kSegTile (or rows);
+// bounds[t] = start of the run containing position t*tile (or rows);
 // the first thread also resets the header (the per-tile counts need no
 // reset: every tile writes its own).
 __global__ void seg_partition_kernel(const uint64_t* keys, size_t rows, uint32_t minor_bits, size_t* bounds,
-                                     size_t tiles, SegHeader* h) {
+                                     size_t tiles, SegHeader* h, uint32_t tile) {
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         h->error = 0;
@@ -87,7 +86,7 @@ __global__ void seg_partition_kernel(const uint64_t* keys, size_t rows, uint32_t
         h->side_unique = 0;
     }
     for (size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; t <= tiles; t += stride) {
-        const size_t p = t * kSegTile;
+        const size_t p = t * tile;
         if (p >= rows) {
             bounds[t] = rows;
             continue;
@@ -174,7 +173,8 @@ template <typename K>
 __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) void seg_tile_kernel(
     uint64_t* keys, size_t rows, uint32_t minor_bits,
                                                           const size_t* bounds, const uint64_t* side_sorted,
-                                                          uint64_t* counts, SegHeader* h, const uint64_t* found) {
+                                                          uint64_t* counts, SegHeader* h, const uint64_t* found,
+                                                          uint32_t tile_rows) {
     // sort keys and run majors; before ranking, the same bytes hold the
     // tile's surviving raw keys (u64) when some keys are dropped
     __shared__ uint64_t seg_buf[(kSegCap * (sizeof(K) + sizeof(uint32_t)) + 7) / 8];
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) 
     static_assert(sizeof(uint16_t) * kSegFirstRadix == sizeof(wave_count), "first-pass counts fit");
     // tiles are independent (no look-back), so dispatch order needs no ticket
     const uint32_t tile = blockIdx.x;
-    const size_t tiles = seg_tiles(rows);
+    const size_t tiles = seg_tiles(rows, tile_rows);
     if (tile >= tiles) return;
     const uint32_t lane = lane_id(), wave = wave_id();
     const size_t b0 = bounds[tile], b1 = bounds[tile + 1];
@@ -600,20 +600,21 @@ __global__ __launch_bounds__(kBlock) void seg_write_kernel(const uint64_t* keys,
 
 }  // namespace
 
-size_t segmented_distinct_meta_bytes(size_t rows) { return seg_layout(rows).total; }
+size_t segmented_distinct_meta_bytes(size_t rows, uint32_t tile_rows) { return seg_layout(rows, tile_rows).total; }
 
 hipError_t segmented_distinct_prepare(const uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t* side,
                                       void* meta, size_t meta_bytes, uint64_t* side_rows, hipStream_t stream,
-                                      const uint64_t* found) {
+                                      const uint64_t* found, uint32_t tile_rows) {
     *side_rows = 0;
     if (rows == 0) return hipSuccess;
-    const SegLayout l = seg_layout(rows);
+    if (tile_rows < 512 || tile_rows > static_cast<uint32_t>(kSegCap)) return hipErrorInvalidValue;
+    const SegLayout l = seg_layout(rows, tile_rows);
     if (meta_bytes < l.total || minor_bits == 0 || minor_bits > 32) return hipErrorInvalidValue;
     char* base = static_cast<char*>(meta);
     SegHeader* h = reinterpret_cast<SegHeader*>(base);
     size_t* bounds = reinterpret_cast<size_t*>(base + l.bounds_off);
     const unsigned small_grid = static_cast<unsigned>(l.tiles / kBlock + 1 < 4096 ? l.tiles / kBlock + 1 : 4096);
-    seg_partition_kernel<<<small_grid, kBlock, 0, stream>>>(keys, rows, minor_bits, bounds, l.tiles, h);
+    seg_partition_kernel<<<small_grid, kBlock, 0, stream>>>(keys, rows, minor_bits, bounds, l.tiles, h, tile_rows);
     seg_gather_kernel<<<static_cast<unsigned>(l.tiles < 1024 ? l.tiles : 1024), kBlock, 0, stream>>>(
         keys, bounds, l.tiles, side, h, found);
     hipError_t st = hipGetLastError();
@@ -627,9 +628,10 @@ hipError_t segmented_distinct_finish(uint64_t* keys, size_t rows, uint32_t minor
                                      uint64_t* side_a, uint64_t* side_b, uint64_t side_rows, PairOutput out,
                                      uint64_t* unique_rows, void* meta, size_t meta_bytes, void* sort_temp,
                                      size_t sort_bytes, hipStream_t stream, const KernelTimer* timer,
-                                     const uint64_t* found, uint32_t* xidx_out) {
+                                     const uint64_t* found, uint32_t* xidx_out, uint32_t tile_rows) {
     if (rows == 0) return set_u64(unique_rows, 0, stream);
-    const SegLayout l = seg_layout(rows);
+    if (tile_rows < 512 || tile_rows > static_cast<uint32_t>(kSegCap)) return hipErrorInvalidValue;
+    const SegLayout l = seg_layout(rows, tile_rows);
     if (meta_bytes < l.total) return hipErrorInvalidValue;
     char* base = static_cast<char*>(meta);
     SegHeader* h = reinterpret_cast<SegHeader*>(base);
@@ -652,10 +654,10 @@ hipError_t segmented_distinct_finish(uint64_t* keys, size_t rows, uint32_t minor
     // rank bits: at most log2(kSegCap) = 12 (a run per key)
     if (minor_bits + 12 <= 32)
         seg_tile_kernel<uint32_t><<<static_cast<unsigned>(l.tiles), kSegBlock, 0, stream>>>(
-            keys, rows, minor_bits, bounds, side_unique_out, status, h, found);
+            keys, rows, minor_bits, bounds, side_unique_out, status, h, found, tile_rows);
     else
         seg_tile_kernel<uint64_t><<<static_cast<unsigned>(l.tiles), kSegBlock, 0, stream>>>(
-            keys, rows, minor_bits, bounds, side_unique_out, status, h, found);
+            keys, rows, minor_bits, bounds, side_unique_out, status, h, found, tile_rows);
     seg_scan_kernel<<<1, kSegScanThreads, 0, stream>>>(status, l.tiles, unique_rows);
     const unsigned write_grid = static_cast<unsigned>(l.tiles < 65536 ? l.tiles : 65536);
     seg_write_kernel<<<write_grid, kBlock, 0, stream>>>(keys, bounds, side_unique_out, status, unique_rows, h,

@@ -429,3 +429,56 @@ extern "C" uint64_t sparkalog_wl_packed_digest(const uint64_t* packed, uint64_t 
     for (uint64_t r = 0; r < rows; ++r) sum += finalize(packed[r] + kGolden);
     return sum;
 }
+
+namespace {
+
+char* put_u32(char* p, uint32_t v) {
+    char buf[10];
+    int n = 0;
+    do {
+        buf[n++] = static_cast<char>('0' + v % 10);
+        v /= 10;
+    } while (v);
+    while (n) *p++ = buf[--n];
+    return p;
+}
+
+}  // namespace
+
+extern "C" uint64_t sparkalog_wl_edge_text(uint64_t seed, uint64_t rows, uint32_t key_space, uint32_t style,
+                                           char** text, uint32_t* first, uint32_t* second) {
+    // at most 80 bytes per row (two 10-digit values with '+' and up to 3
+    // leading zeros, separators, padding, CRLF) plus a comment or blank line
+    const uint64_t cap = rows * 80 + 64;
+    char* out = static_cast<char*>(std::malloc(cap ? cap : 1));
+    if (!out) return UINT64_MAX;
+    uint64_t s = seed;
+    char* p = out;
+    const char* seps[] = {" ", "\t", "  ", " \t "};
+    for (uint64_t r = 0; r < rows; ++r) {
+        const uint64_t bits = splitmix(s);
+        if ((style & 2) && (bits & 63) == 0) {
+            const char* extra[] = {"# comment 1 2 3", "% note", "", "   \t"};
+            const char* e = extra[(bits >> 6) & 3];
+            while (*e) *p++ = *e++;
+            *p++ = '\n';
+        }
+        const uint32_t a = key_space ? bounded(s, key_space) : static_cast<uint32_t>(splitmix(s) >> 32);
+        const uint32_t b = key_space ? bounded(s, key_space) : static_cast<uint32_t>(splitmix(s) >> 32);
+        first[r] = a;
+        second[r] = b;
+        if ((style & 1) && ((bits >> 8) & 7) == 0) *p++ = ' ';
+        if ((style & 8) && ((bits >> 11) & 7) == 0) *p++ = '+';
+        if ((style & 8) && ((bits >> 14) & 7) == 0) *p++ = '0';
+        p = put_u32(p, a);
+        const char* sep = (style & 1) ? seps[(bits >> 17) & 3] : " ";
+        while (*sep) *p++ = *sep++;
+        if ((style & 8) && ((bits >> 19) & 7) == 0) *p++ = '+';
+        p = put_u32(p, b);
+        if ((style & 1) && ((bits >> 22) & 7) == 0) *p++ = '\t';
+        if ((style & 4) && ((bits >> 25) & 1)) *p++ = '\r';
+        if (r + 1 < rows || !(style & 16)) *p++ = '\n';
+    }
+    *text = out;
+    return static_cast<uint64_t>(p - out);
+}

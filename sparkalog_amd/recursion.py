@@ -75,12 +75,22 @@ def _columns(rel):
 class RelationStore:
     """RelationStore (generic.rs:16-127) in device memory."""
 
-    def __init__(self, stream=None):
+    def __init__(self, stream=None, tuning="file", profile: str = "closure"):
+        """`tuning`: "file" = the autotuner's file (autotune.default_file(),
+        if any), None = built-in defaults, or a dict / tuning file path;
+        `profile` names the file's profile ("closure" or "batched"). Results
+        never depend on it."""
+        from . import autotune
+
         self._lib = _native.hip()
         handle = ctypes.c_void_p()
         raw = None if stream is None else getattr(stream, "raw", stream)
         _check(self._lib.sparkalog_hip_store_create(ctypes.byref(handle), raw))
         self._h = handle
+        if isinstance(tuning, str) and tuning == "file":
+            tuning = autotune.default_file()
+        self.tuning = autotune.profile_values(tuning, profile)
+        autotune.apply(self, self.tuning)
 
     def close(self) -> None:
         if self._h:
@@ -230,6 +240,14 @@ class RelationStore:
         r = ctypes.c_uint64(0)
         _check(self._lib.sparkalog_hip_store_graph_stats(self._h, ctypes.byref(c), ctypes.byref(r)))
         return c.value, r.value
+
+    def graph_capture_stats(self) -> tuple[int, float]:
+        """(captures that updated an executable graph in place, host ms spent
+        capturing and instantiating or updating)."""
+        u = ctypes.c_uint64(0)
+        ms = ctypes.c_double(0)
+        _check(self._lib.sparkalog_hip_store_graph_capture_stats(self._h, ctypes.byref(u), ctypes.byref(ms)))
+        return u.value, ms.value
 
     def set_profiling(self, enabled: bool) -> None:
         _check(self._lib.sparkalog_hip_store_set_profiling(self._h, int(enabled)))

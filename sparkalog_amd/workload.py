@@ -125,6 +125,25 @@ def overlapping_sets(seed: int, rows: int, overlap: float = 0.5, key_mode: int =
     return (arrs[0], arrs[1]), (arrs[2], arrs[3])
 
 
+TEXT_MIXED_SPACE, TEXT_COMMENTS, TEXT_CRLF, TEXT_SIGNS, TEXT_NO_FINAL_NEWLINE = 1, 2, 4, 8, 16
+
+
+def edge_text(seed: int, rows: int, key_space: int = 0, style: int = 0):
+    """(text as a uint8 array, first, second): `rows` facts in the loader's
+    text format and the values written (the parse's ground truth)."""
+    lib = _native.workload()
+    first = np.empty(max(rows, 1), np.uint32)
+    second = np.empty(max(rows, 1), np.uint32)
+    ptr = ctypes.c_void_p()
+    n = lib.sparkalog_wl_edge_text(seed, rows, key_space, style, ctypes.byref(ptr),
+                                   first.ctypes.data_as(_native.u32p), second.ctypes.data_as(_native.u32p))
+    if n == 0xFFFFFFFFFFFFFFFF:
+        raise MemoryError("edge text generation failed")
+    text = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_uint8)), shape=(max(n, 1),))[:n].copy()
+    lib.sparkalog_wl_free(ptr)
+    return text, first[:rows], second[:rows]
+
+
 def pair_digest(first, second) -> int:
     a = np.ascontiguousarray(first, dtype=np.uint32)
     b = np.ascontiguousarray(second, dtype=np.uint32)

@@ -1,4 +1,5 @@
 import json
+import os
 import sys
 from pathlib import Path
 
@@ -13,6 +14,9 @@ GOLDEN = ROOT / "tests" / "golden"
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) device; runs through the HIP C ABI")
+    # stores in tests start from the built-in defaults, so each test runs the
+    # executor path it names; tests of tuning files pass them explicitly
+    os.environ.setdefault("SPARKALOG_HIP_TUNING", "")
 
 
 @pytest.fixture(scope="session")
