@@ -161,6 +161,19 @@ sparkalog_hip_error sparkalog_hip_union_u32(const uint32_t* left_first, const ui
                                             uint32_t* output_second, void* temporary, size_t temporary_bytes,
                                             void* stream);
 
+/* ---- device range index ----
+ * U32RangeIndex::build_slice (crates/sparkalog-storage/src/lib.rs:357-393)
+ * on the device: from `rows` values (the join key column of the right
+ * relation), keys[K] sorted unique, starts[K + 1] (starts[K] = rows) and
+ * index_rows[rows] = row ids sorted by (value, row), the layout
+ * sparkalog_hip_join_u32_count / _emit take. *unique_keys (u64, device) = K;
+ * keys and starts hold at least rows and rows + 1 entries. Asynchronous on
+ * `stream`; a look-back wait that gives up writes K as all ones. */
+sparkalog_hip_error sparkalog_hip_range_index_u32_temporary_bytes(size_t rows, size_t* temporary_bytes);
+sparkalog_hip_error sparkalog_hip_range_index_u32(const uint32_t* values, size_t rows, uint32_t* keys,
+                                                  uint32_t* starts, uint32_t* index_rows, uint64_t* unique_keys,
+                                                  void* temporary, size_t temporary_bytes, void* stream);
+
 /* ---- device text ingest ----
  * load_binary_u32 (crates/sparkalog-storage/src/io.rs:72-141) on the device:
  * a whitespace-separated text of two u32 fields per line, already in device

@@ -58,6 +58,8 @@ _HIP_SIGNATURES = {
     "sparkalog_hip_anti_join_u32": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
     "sparkalog_hip_union_u32_temporary_bytes": (c_int, [c_void_p, c_size_t, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p, POINTER(c_size_t), c_void_p]),
     "sparkalog_hip_union_u32": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "sparkalog_hip_range_index_u32_temporary_bytes": (c_int, [c_size_t, POINTER(c_size_t)]),
+    "sparkalog_hip_range_index_u32": (c_int, [c_void_p, c_size_t, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
     "sparkalog_hip_text_lines_temporary_bytes": (c_int, [c_size_t, POINTER(c_size_t)]),
     "sparkalog_hip_text_lines": (c_int, [c_void_p, c_size_t, c_void_p, c_void_p, c_size_t, c_void_p]),
     "sparkalog_hip_load_binary_u32_temporary_bytes": (c_int, [c_size_t, c_uint64, POINTER(c_size_t)]),

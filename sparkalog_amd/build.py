@@ -19,7 +19,7 @@ CSRC = PKG / "csrc"
 OUT = PKG / "_lib"
 INCLUDE = ROOT / "include"
 
-HIP_SOURCES = ["ops.hip", "segsort.hip", "xindex.hip", "compact.hip", "ingest.hip", "capi.hip", "executor.hip"]
+HIP_SOURCES = ["ops.hip", "segsort.hip", "xindex.hip", "compact.hip", "index.hip", "ingest.hip", "capi.hip", "executor.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 # packed-fp32 instruction selection stays off for every HIP translation unit

@@ -194,6 +194,29 @@ class RangeIndex:
             return torch.from_numpy(np.ascontiguousarray(np.asarray(x, dtype=np.uint32)).view(np.int32).copy()).to(device)
         return RangeIndex(t(keys), t(starts), t(rows), source_rows)
 
+    @staticmethod
+    def build(values: torch.Tensor, stream: "HipStream") -> "RangeIndex":
+        """U32RangeIndex::build_slice (storage/src/lib.rs:357-393) on the
+        device, over a u32 column (an int32 device tensor)."""
+        lib = _native.hip()
+        n = int(values.numel())
+        _check_row_count(n)
+        dev = values.device
+        keys = _empty(n, torch.int32, dev)
+        starts = _empty(n + 1, torch.int32, dev)
+        rows = _empty(n, torch.int32, dev)
+        k = torch.zeros(1, dtype=torch.int64, device=dev)
+        tb = ctypes.c_size_t(0)
+        _check("sparkalog_hip_range_index_u32_temporary_bytes",
+               lib.sparkalog_hip_range_index_u32_temporary_bytes(n, ctypes.byref(tb)))
+        temp = _temp(tb.value, dev)
+        _check("sparkalog_hip_range_index_u32",
+               lib.sparkalog_hip_range_index_u32(_ptr(values), n, _ptr(keys), _ptr(starts), _ptr(rows), _ptr(k),
+                                                 _ptr(temp), temp.numel(), stream.raw))
+        stream.synchronize()
+        u = _device_count("sparkalog_hip_range_index_u32", k, 64)
+        return RangeIndex(keys[:u], starts[:u + 1], rows[:n], n)
+
 
 def join_hip(left: Relation, right: Relation, right_index: RangeIndex, plan: BinaryEqualityJoin,
              stream: HipStream) -> Relation:

@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import re
 from pathlib import Path
 
 import numpy as np
@@ -58,7 +59,7 @@ def _field(text: np.ndarray, line: int, column: int) -> str:
     nl = np.flatnonzero(text == 10)
     start = 0 if line == 1 else int(nl[line - 2]) + 1
     end = int(nl[line - 1]) if line - 1 < nl.size else text.size
-    fields = bytes(text[start:end]).split()
+    fields = [f for f in re.split(rb"[ \t\n\x0b\x0c\r]+", bytes(text[start:end])) if f]
     return fields[column - 1].decode("utf-8", "replace") if len(fields) >= column else ""
 
 
