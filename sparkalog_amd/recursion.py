@@ -134,6 +134,18 @@ class RelationStore:
     def full(self, id: RelationId) -> tuple[np.ndarray, np.ndarray]:
         return self._read(id, 0)
 
+    def full_device(self, id: RelationId, device=None):
+        """FULL copied into two new device tensors (int32 holding u32)."""
+        import torch
+
+        n = self.rows(id, 0)
+        dev = device or torch.device("cuda", torch.cuda.current_device())
+        a = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+        b = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+        _check(self._lib.sparkalog_hip_store_read(self._h, int(id), 0, ctypes.c_void_p(a.data_ptr()),
+                                                  ctypes.c_void_p(b.data_ptr()), n, 1))
+        return a[:n], b[:n]
+
     def delta(self, id: RelationId) -> tuple[np.ndarray, np.ndarray]:
         return self._read(id, 1)
 
