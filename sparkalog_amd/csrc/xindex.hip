@@ -770,7 +770,9 @@ __global__ __launch_bounds__(kBlock) void run_merge_kernel(const uint32_t* xa, c
                                                            const uint32_t* xb, const uint32_t* sb, size_t nb,
                                                            uint32_t n_x, const uint32_t* bx, const size_t* pa,
                                                            size_t tiles, uint32_t* out_second, uint64_t* out_packed) {
-    __shared__ uint64_t lds[kRTile];
+    // kRTile keys, then one u64 of padding per 32 for the merged rows'
+    // transposition (see rpad)
+    __shared__ uint64_t lds[kRTile + kRTile / 32];
     __shared__ uint32_t xa_l[kRSpan], xb_l[kRSpan];
     __shared__ uint32_t scan_max[kWavesPerBlock];
     const size_t t = blockIdx.x;
@@ -884,31 +886,43 @@ __global__ __launch_bounds__(kBlock) void run_merge_kernel(const uint32_t* xa, c
     const uint32_t diag = threadIdx.x * kRItems < n ? threadIdx.x * kRItems : n;
     uint32_t ia = merge_path_keys(lds, n_a, lds + n_a, n_b, diag);
     uint32_t ib = diag - ia;
+    // the heads of both sides stay in registers: one LDS read per merged row
+    // (the side just consumed), not two
+    constexpr uint64_t kPast = ~0ull;  // above every key (x < 2^24)
+    uint64_t ha = ia < n_a ? lds[ia] : kPast;
+    uint64_t hb = ib < n_b ? lds[n_a + ib] : kPast;
     uint64_t v[kRItems];
 #pragma unroll
     for (int i = 0; i < kRItems; ++i) {
-        const bool take_a = ib >= n_b || (ia < n_a && lds[ia] <= lds[n_a + ib]);
-        v[i] = 0;
+        const bool take_a = ha <= hb;
+        v[i] = take_a ? ha : hb;
         if (diag + i < n) {
-            v[i] = take_a ? lds[ia] : lds[n_a + ib];
-            if (take_a)
+            if (take_a) {
                 ++ia;
-            else
+                ha = ia < n_a ? lds[ia] : kPast;
+            } else {
                 ++ib;
+                hb = ib < n_b ? lds[n_a + ib] : kPast;
+            }
         }
     }
     __syncthreads();
+    // thread t's rows are [8t, 8t + 8): written unpadded, lanes 64 bytes
+    // apart would meet in a quarter of the banks; one u64 of padding per 32
+    // rows spreads them (rpad), and the coalesced stores read back through it
+    auto rpad = [](uint32_t j) { return j + (j >> 5); };
 #pragma unroll
     for (int i = 0; i < kRItems; ++i)
-        if (diag + i < n) lds[diag + i] = v[i];
+        if (diag + i < n) lds[rpad(diag + i)] = v[i];
     __syncthreads();
     // the merged run is read again only in later steps: streaming stores
     // (nontemporal) leave L2 to the index entries
     if (out_second)
         for (uint32_t i = threadIdx.x; i < n; i += kBlock)
-            __builtin_nontemporal_store(static_cast<uint32_t>(lds[i]), &out_second[d0 + i]);
+            __builtin_nontemporal_store(static_cast<uint32_t>(lds[rpad(i)]), &out_second[d0 + i]);
     if (out_packed)
-        for (uint32_t i = threadIdx.x; i < n; i += kBlock) __builtin_nontemporal_store(lds[i], &out_packed[d0 + i]);
+        for (uint32_t i = threadIdx.x; i < n; i += kBlock)
+            __builtin_nontemporal_store(lds[rpad(i)], &out_packed[d0 + i]);
 }
 
 __global__ void xsec_init_kernel(uint32_t* deferred) { deferred[0] = 0; }
