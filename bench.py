@@ -65,7 +65,9 @@ def parse_args():
                    help="crossover mode: operators")
     p.add_argument("--crossover-min-log2", type=int, default=8, help="crossover mode: smallest log2 row count")
     p.add_argument("--crossover-max-log2", type=int, default=24, help="crossover mode: largest log2 row count")
-    p.add_argument("--mode", choices=["tc", "ops", "batched", "served", "copy", "crossover", "tune"], default="tc",
+    p.add_argument("--chain", type=int, default=2000, help="graph mode: nodes of the chain whose closure is timed")
+    p.add_argument("--mode", choices=["tc", "ops", "batched", "served", "copy", "crossover", "tune", "graph"],
+                   default="tc",
                    help="tc: the headline fixpoint; ops: operator microbench at 2^k rows; "
                         "batched: 4,096 instances in one loop vs one run each; served: Poisson stream, "
                         "continuous batching vs sequential; copy: plain device copy of --copy-bytes "
@@ -76,13 +78,16 @@ def parse_args():
     p.add_argument("--requests", type=int, default=200, help="served: stream length")
     p.add_argument("--utilization", default="0.7,1.0,1.5",
                    help="served mode: offered loads of the plain path (comma list)")
-    p.add_argument("--bench-utilization", default="0.7,1.0",
+    p.add_argument("--bench-utilization", default="0.7,1.0,1.5",
                    help="tc: offered loads of the served summary in the bench line")
     p.add_argument("--plain-stride", type=int, default=1, help="batched: time the plain path on every k-th instance")
-    p.add_argument("--lane-bounds", default="1e6,3e7",
+    p.add_argument("--lane-route", default="edges", choices=["edges", "closure"],
+                   help="served: route requests to lanes by edge count or by estimated closure")
+    p.add_argument("--lane-bounds", default="4096,65536",
                    help="served: closure-size bounds (pairs) between continuous-batching lanes")
-    p.add_argument("--lane-slots", default="256,256,2",
-                   help="served: instances in flight per lane (comma list; empty = 256 each)")
+    p.add_argument("--lane-slots", default="256,16,0",
+                   help="served: instances in flight per lane (comma list; empty = 256 each; 0 = a plain lane, "
+                        "one executor run per request)")
     p.add_argument("--graph", type=int, default=-1,
                    help="batched/served: replay single-rule iterations from captured HIP graphs (1) or not (0); "
                         "-1 = the tuning file's choice")
@@ -249,8 +254,24 @@ def run_batched_mode(args) -> None:
     print(json.dumps(r), flush=True)
 
 
+def _class_latency(results, requests, bounds: list[float]) -> list[dict]:
+    """p50 / p99 latency per size class (edge-count bounds) of a stream."""
+    from sparkalog_amd import serving
+
+    edges = {r.rid: r.src.size for r in requests}
+    out = []
+    for i in range(len(bounds) + 1):
+        lo = bounds[i - 1] if i else 0
+        hi = bounds[i] if i < len(bounds) else float("inf")
+        lat = [1000.0 * (x.done - x.arrival) for x in results if lo <= edges[x.rid] < hi]
+        out.append({"edges_below": None if hi == float("inf") else hi, "requests": len(lat),
+                    "p50_ms": round(serving.percentile(lat, 50), 3) if lat else None,
+                    "p99_ms": round(serving.percentile(lat, 99), 3) if lat else None})
+    return out
+
+
 def served_summary(budget_pairs: float, requests: int, utilizations: str, lane_bounds: str, lane_slots: str,
-                   one_lane: bool = True, graph=None, tuning="file") -> dict:
+                   one_lane: bool = True, graph=None, tuning="file", route: str = "edges") -> dict:
     """Served fixpoints: a Poisson stream of TC requests (edges log-uniform
     on [2^8, 2^20]) at arrival rates giving the plain path each requested
     utilisation (its mean service time measured on the same requests served
@@ -279,18 +300,24 @@ def served_summary(budget_pairs: float, requests: int, utilizations: str, lane_b
     for lane in lanes:
         lane.set_graph(graph)
     slots = [int(x) for x in lane_slots.split(",")] if lane_slots else None
-    serving.serve_lanes(back_to_back[:6], lanes, bounds, budget_pairs=budget_pairs, lane_slots=slots)  # warm-up
+    # warm-up: the whole stream once, so the lanes' stores have grown their
+    # workspaces before anything is timed (as the plain store has)
+    serving.serve_lanes(back_to_back, lanes, bounds, budget_pairs=budget_pairs, lane_slots=slots, route=route)
     out = {"requests": requests, "graph_replay": graph, "mean_plain_service_ms": round(1000 * mean_service, 3),
-           "lane_bounds_pairs": bounds, "lane_slots": slots, "loads": []}
+           "lane_route": route, "lane_bounds": bounds, "lane_slots": slots, "loads": []}
     equal = True
     for u in [float(x) for x in utilizations.split(",")]:
         rate = u / mean_service
         reqs = serving.make_requests(requests, rate_per_s=rate)
-        mixed, t_mixed = serving.serve_lanes(reqs, lanes, bounds, budget_pairs=budget_pairs, lane_slots=slots)
+        mixed, t_mixed = serving.serve_lanes(reqs, lanes, bounds, budget_pairs=budget_pairs, lane_slots=slots,
+                                             route=route)
         alone, t_alone = serving.serve_plain(reqs, store)
         equal = equal and all((m.rows, m.digest) == (a.rows, a.digest) for m, a in zip(mixed, alone))
         load = {"utilization": u, "rate_per_s": round(rate, 3), "lanes": serving.summarize(mixed, t_mixed),
                 "plain": serving.summarize(alone, t_alone)}
+        if route == "edges":
+            load["lanes_by_class"] = _class_latency(mixed, reqs, bounds)
+            load["plain_by_class"] = _class_latency(alone, reqs, bounds)
         if one_lane:
             one, t_one = serving.serve_batched(reqs, store, budget_pairs=budget_pairs)
             equal = equal and all((m.rows, m.digest) == (a.rows, a.digest) for m, a in zip(one, alone))
@@ -304,7 +331,7 @@ def served_summary(budget_pairs: float, requests: int, utilizations: str, lane_b
 
 def run_served_mode(args) -> None:
     r = served_summary(args.budget_pairs, args.requests, args.utilization, args.lane_bounds, args.lane_slots,
-                       graph=_graph_arg(args), tuning=_tuning_arg(args))
+                       graph=_graph_arg(args), tuning=_tuning_arg(args), route=args.lane_route)
     r["mode"] = "served"
     print(json.dumps(r), flush=True)
 
@@ -408,6 +435,59 @@ def run_crossover(args) -> None:
                       "nproc": os.cpu_count(),
                       "derived_mi355x": {k: vars(v) for k, v in derived.items()},
                       "gb10": {k: vars(v) for k, v in placement.MEASURED_GB10.items()}}), flush=True)
+
+
+def run_graph_mode(args) -> None:
+    """Per-iteration HIP graph replay against the stream executor on the
+    closure of a chain of --chain nodes (chain - 1 iterations of one new row
+    per source: launch- and synchronization-bound), and on the 2^14-node
+    component graph; results checked equal."""
+    import numpy as np
+    import torch
+
+    from sparkalog_amd import plans, workload
+    from sparkalog_amd.execution import HipStream
+    from sparkalog_amd.recursion import RecursiveExecutor, RelationStore
+
+    torch.cuda.set_device(0)
+    R = plans.RelationId
+    plan = plans.transitive_closure_scc(R(1), R(0))
+    n = args.chain
+    chain = (np.arange(n - 1, dtype=np.uint32), np.arange(1, n, dtype=np.uint32))
+    s, d, _ = workload.component_graph(nodes=1 << 14)
+    out = {"mode": "graph", "cases": []}
+    for name, edges in ((f"chain_{n}", chain), ("component_graph_16384", (s, d))):
+        case = {"workload": name}
+        for graph in (0, 1):
+            store = RelationStore(HipStream(), tuning=None)
+            store.set_graph(bool(graph))
+
+            def once():
+                store.clear()
+                store.insert_static(R(0), edges)
+                store.insert_recursive(R(1), edges, edges)
+                summ = RecursiveExecutor.compile(plan, store).run(store, 1_000_000)
+                return summ, store.digest(R(1))
+
+            once()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            reps = max(1, args.steps)
+            for _ in range(reps):
+                summ, dig = once()
+            torch.cuda.synchronize()
+            ms = 1000.0 * (time.perf_counter() - t0) / reps
+            caps, reps_g = store.graph_stats()
+            upd, cap_ms = store.graph_capture_stats()
+            case["graph" if graph else "stream"] = {
+                "ms": round(ms, 3), "iterations": summ.iterations, "us_per_iteration": round(1000 * ms / summ.iterations, 2),
+                "rows": dig[2], "digest": dig[0], "captures": caps, "replays": reps_g, "updates": upd,
+                "capture_ms": round(cap_ms, 2)}
+            store.close()
+        case["results_equal"] = (case["graph"]["rows"], case["graph"]["digest"]) == \
+            (case["stream"]["rows"], case["stream"]["digest"])
+        out["cases"].append(case)
+    print(json.dumps(out), flush=True)
 
 
 def run_tune_mode(args) -> None:
@@ -528,6 +608,9 @@ def main():
         return
     if args.mode == "tune":
         run_tune_mode(args)
+        return
+    if args.mode == "graph":
+        run_graph_mode(args)
         return
     import numpy as np
     import torch
@@ -676,7 +759,7 @@ def main():
                                           tuning=_tuning_arg(args))
     if not args.no_served:
         line["served"] = served_summary(args.budget_pairs, args.requests, args.bench_utilization, args.lane_bounds,
-                                        args.lane_slots, one_lane=False, graph=_graph_arg(args),
+                                        args.lane_slots, one_lane=False, graph=_graph_arg(args), route=args.lane_route,
                                         tuning=_tuning_arg(args))
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample_nodes, args.cpu_serial_nodes)

@@ -102,10 +102,11 @@ def _wait_until(t0: float, t: float) -> None:
         time.sleep(delay)
 
 
-def serve_plain(requests: list[Request], store: RelationStore) -> tuple[list[Served], float]:
+def _plain_loop(requests: list[Request], store: RelationStore, t0: float, results: list[Served],
+                stats: dict | None = None) -> None:
+    """One executor run per request, in arrival order, on `store`."""
     plan = plans.transitive_closure_scc(PATH, EDGE)
-    results = []
-    t0 = time.perf_counter()
+    busy = 0.0
     for req in requests:
         _wait_until(t0, req.arrival)
         start = time.perf_counter() - t0
@@ -114,14 +115,52 @@ def serve_plain(requests: list[Request], store: RelationStore) -> tuple[list[Ser
         store.insert_recursive(PATH, (req.src, req.dst), (req.src, req.dst))
         RecursiveExecutor.compile(plan, store).run(store, 1_000_000)
         digest, _, rows = store.digest(PATH)
-        results.append(Served(req.rid, req.arrival, start, time.perf_counter() - t0, rows, digest))
+        done = time.perf_counter() - t0
+        busy += done - start
+        results.append(Served(req.rid, req.arrival, start, done, rows, digest))
+    if stats is not None:
+        stats.update({"plain": True, "busy_s": round(busy, 4)})
+
+
+def serve_plain(requests: list[Request], store: RelationStore) -> tuple[list[Served], float]:
+    results: list[Served] = []
+    t0 = time.perf_counter()
+    _plain_loop(requests, store, t0, results)
     return results, time.perf_counter() - t0
 
 
+class ClosureEstimator:
+    """HBM budgeting estimate of a request's closure. Before any request has
+    finished it is batch.estimate_closure (a random digraph's giant
+    component squared: safe, and far above the closure of graphs made of
+    many small components); afterwards it is `margin` times the largest
+    closure-per-node ratio observed so far times the request's nodes, never
+    above the first estimate."""
+
+    def __init__(self, margin: float = 2.0):
+        self.margin = margin
+        self.ratio = 0.0
+
+    def __call__(self, req: Request) -> float:
+        prior = batch.estimate_closure(req.nodes, req.src.size)
+        if self.ratio <= 0.0:
+            return prior
+        return min(prior, self.margin * self.ratio * max(req.nodes, 1))
+
+    def observe(self, req: Request, rows: int) -> None:
+        if req.nodes:
+            self.ratio = max(self.ratio, rows / req.nodes)
+
+
 def _lane(requests: list[Request], store: RelationStore, t0: float, budget_pairs: float, max_slots: int,
-          bits: int, results: list[Served]) -> None:
+          bits: int, results: list[Served], stats: dict | None = None,
+          estimator: ClosureEstimator | None = None) -> None:
     """One continuous-batching loop over `requests` (arrival order) on
-    `store`'s stream; appends to `results`."""
+    `store`'s stream; appends to `results`. `stats`, when given, receives
+    the loop's step, admission, retirement and removal counts and seconds."""
+    st = {"steps": 0, "step_s": 0.0, "admissions": 0, "admit_s": 0.0, "removals": 0, "remove_s": 0.0,
+          "retire_s": 0.0, "idle_s": 0.0, "max_active": 0}
+    estimate = estimator or ClosureEstimator()
     if (max_slots << bits) > (1 << 28):
         max_slots = (1 << 28) >> bits
     store.clear()
@@ -144,25 +183,29 @@ def _lane(requests: list[Request], store: RelationStore, t0: float, budget_pairs
         # converged instances' rows are removed in batches: all dead slots in
         # one pass over each relation, when slots run out or their rows grow
         if dead and ((not free and pending) or dead_pairs > budget_pairs / 4):
+            tr = time.perf_counter()
             bounds = np.array([[s << bits, (s + 1) << bits] for s in sorted(dead)], dtype=np.uint32).reshape(-1)
             store.remove_ranges(PATH, bounds)
             store.remove_ranges(EDGE, bounds)
             free.extend(dead)
             dead.clear()
             dead_pairs = 0
+            st["removals"] += 1
+            st["remove_s"] += time.perf_counter() - tr
         # admission between steps, in arrival order: every request that fits
         # joins at once (one merge per relation part for all of them; slots
         # ascend with arrival, so their rows concatenate in sorted order)
         admitted = []
         while pending and pending[0].arrival <= now and free:
             req = pending[0]
-            est = batch.estimate_closure(req.nodes, req.src.size)
+            est = estimate(req)
             if (active or admitted) and used + est > budget_pairs:
                 break
             pending.pop(0)
             admitted.append((free.pop(), req, est))
             used += est
         if admitted:
+            ta = time.perf_counter()
             admitted.sort(key=lambda a: a[0])
             gs = np.concatenate([req.src + np.uint32(slot << bits) for slot, req, _ in admitted])
             gd = np.concatenate([req.dst + np.uint32(slot << bits) for slot, req, _ in admitted])
@@ -171,11 +214,20 @@ def _lane(requests: list[Request], store: RelationStore, t0: float, budget_pairs
             store.merge_rows(PATH, 1, (gs, gd))
             for slot, req, est in admitted:
                 active[slot] = (req, now, est)
+            st["admissions"] += 1
+            st["admit_s"] += time.perf_counter() - ta
         if not active:
+            ti = time.perf_counter()
             _wait_until(t0, pending[0].arrival)
+            st["idle_s"] += time.perf_counter() - ti
             continue
+        st["max_active"] = max(st["max_active"], len(active))
+        ts = time.perf_counter()
         ex.step(store)
+        st["steps"] += 1
+        st["step_s"] += time.perf_counter() - ts
         # retire converged instances (the step's per-instance DELTA rows)
+        tr = time.perf_counter()
         left = store.instance_delta_rows(max_slots)
         for slot in list(active):
             if left[slot]:
@@ -184,13 +236,17 @@ def _lane(requests: list[Request], store: RelationStore, t0: float, budget_pairs
             req, admitted_at, est = active.pop(slot)
             rows, digest = store.range_stats(PATH, 0, lo, hi, lo)
             results.append(Served(req.rid, req.arrival, admitted_at, time.perf_counter() - t0, rows, digest))
+            estimate.observe(req, rows)
             # its rows stay until the next batched removal: with an empty
             # DELTA they take part in no join, and no new instance uses the
             # slot before they are gone
             used -= est
             dead.append(slot)
             dead_pairs += rows
+        st["retire_s"] += time.perf_counter() - tr
     ex.end(store)
+    if stats is not None:
+        stats.update({k: round(v, 4) if isinstance(v, float) else v for k, v in st.items()})
 
 
 def _instance_bits(requests: list[Request]) -> int:
@@ -209,33 +265,44 @@ def serve_batched(requests: list[Request], store: RelationStore, budget_pairs: f
 
 def serve_lanes(requests: list[Request], stores: list[RelationStore], bounds: list[float],
                 budget_pairs: float = 6e9, max_slots: int = 256,
-                lane_slots: list[int] | None = None) -> tuple[list[Served], float]:
+                lane_slots: list[int] | None = None, stats: list | None = None,
+                route: str = "closure") -> tuple[list[Served], float]:
     """Continuous batching in size classes: request r goes to the first lane
-    whose bound exceeds its estimated closure (the last lane takes the rest).
+    whose bound exceeds its size - its estimated closure (route "closure")
+    or its edge count (route "edges") - and the last lane takes the rest.
     Every request of a component graph takes about as many steps as its
     diameter, and a batch step costs about as much as its largest instances,
     so small requests get lanes of their own; each lane batches on its own
     store and stream, driven by its own host thread (the native calls release
     the interpreter lock), and the HBM budget is split evenly. lane_slots
     caps the instances in flight per lane (default max_slots each); a cap of
-    1 serves that lane's requests one at a time, in arrival order."""
+    1 serves that lane's requests one at a time, in arrival order, still
+    through the batching loop, and a cap of 0 makes the lane a plain loop
+    (one executor run per request, as serve_plain)."""
     if len(stores) != len(bounds) + 1:
         raise ValueError("serve_lanes: one store per lane (len(bounds) + 1)")
     lanes: list[list[Request]] = [[] for _ in stores]
+    if route not in ("closure", "edges"):
+        raise ValueError(f"serve_lanes: unknown route {route!r}")
     for req in requests:
-        est = batch.estimate_closure(req.nodes, req.src.size)
-        lane = next((i for i, b in enumerate(bounds) if est < b), len(bounds))
+        size = batch.estimate_closure(req.nodes, req.src.size) if route == "closure" else req.src.size
+        lane = next((i for i, b in enumerate(bounds) if size < b), len(bounds))
         lanes[lane].append(req)
     bits = _instance_bits(requests)
     results: list[list[Served]] = [[] for _ in stores]
     errors: list[BaseException] = []
+    lane_stats = [{"requests": len(lanes[i])} for i in range(len(stores))]
     t0 = time.perf_counter()
 
     def run(i: int) -> None:
         try:
             if lanes[i]:
                 slots = lane_slots[i] if lane_slots else max_slots
-                _lane(lanes[i], stores[i], t0, budget_pairs / len(stores), slots, bits, results[i])
+                if slots == 0:
+                    _plain_loop(lanes[i], stores[i], t0, results[i], lane_stats[i])
+                else:
+                    _lane(lanes[i], stores[i], t0, budget_pairs / len(stores), slots, bits, results[i],
+                          lane_stats[i])
         except BaseException as e:  # re-raised on the calling thread
             errors.append(e)
 
@@ -246,5 +313,7 @@ def serve_lanes(requests: list[Request], stores: list[RelationStore], bounds: li
         t.join()
     if errors:
         raise errors[0]
+    if stats is not None:
+        stats.extend(lane_stats)
     merged = sorted((r for lane in results for r in lane), key=lambda r: r.rid)
     return merged, time.perf_counter() - t0

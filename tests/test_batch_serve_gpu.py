@@ -109,6 +109,37 @@ def test_served_requests_up_to_2p20_edges(stream):
         assert (m.rows, m.digest) == (a.rows, a.digest) == (bfs["pairs"], bfs["digest"]), req.rid
 
 
+def test_served_lanes_by_edge_count_with_adaptive_admission(stream):
+    # routed by edge count (the bench's lanes), admission estimates learned
+    # from finished requests: many requests share a batch, results unchanged
+    import torch
+    from sparkalog_amd.execution import HipStream
+
+    reqs = serving.make_requests(40, rate_per_s=2000.0, seed=0x5E, min_log2_edges=8, max_log2_edges=16)
+    stores = [RelationStore(HipStream(torch.cuda.Stream())) for _ in range(3)]
+    stats = []
+    mixed, _ = serving.serve_lanes(reqs, stores, bounds=[2048, 16384], budget_pairs=3e9, lane_slots=[256, 16, 0],
+                                   stats=stats, route="edges")
+    alone, _ = serving.serve_plain(reqs, RelationStore(stream))
+    assert [r.rid for r in mixed] == list(range(len(reqs)))
+    for m, a, req in zip(mixed, alone, reqs):
+        bfs = oracle.tc_bfs(req.src, req.dst, req.nodes, threads=8)
+        assert (m.rows, m.digest) == (a.rows, a.digest) == (bfs["pairs"], bfs["digest"]), req.rid
+    assert sum(st["requests"] for st in stats) == len(reqs)
+    assert max(st.get("max_active", 0) for st in stats) > 1  # requests did share a batch
+    assert stats[2].get("plain")  # the largest class ran as a plain lane
+
+
+def test_closure_estimator_learns_a_bound():
+    est = serving.ClosureEstimator(margin=2.0)
+    req = serving.make_requests(1, rate_per_s=1.0, seed=3, min_log2_edges=12, max_log2_edges=12)[0]
+    prior = est(req)
+    est.observe(req, 1000 * req.nodes)
+    assert est(req) == min(prior, 2.0 * 1000 * req.nodes)
+    est.observe(req, 10 * req.nodes)  # the largest ratio stays
+    assert est(req) == min(prior, 2.0 * 1000 * req.nodes)
+
+
 def test_remove_ranges_matches_filter(stream):
     rng = np.random.default_rng(0x77)
     a = rng.integers(0, 4000, 20000).astype(np.uint32)
