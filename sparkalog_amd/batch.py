@@ -59,11 +59,16 @@ def local_bits_for(node_counts) -> int:
     return max(1, int(np.max(node_counts) - 1).bit_length()) if len(node_counts) else 1
 
 
-def plan_waves(node_counts, edge_counts, budget_pairs: float) -> list[list[int]]:
+def plan_waves(node_counts, edge_counts, budget_pairs: float, max_members: int | None = None) -> list[list[int]]:
+    """Consecutive instances per wave while their estimated closures fit
+    the budget, and at most max_members (default: as many as global ids
+    slot << bits | v fit in u32, bits = local_bits_for(node_counts))."""
+    if max_members is None:
+        max_members = 1 << (32 - local_bits_for(node_counts)) if len(node_counts) else 1
     waves, current, used = [], [], 0.0
     for i, (n, e) in enumerate(zip(node_counts, edge_counts)):
         est = estimate_closure(int(n), int(e))
-        if current and used + est > budget_pairs:
+        if current and (used + est > budget_pairs or len(current) >= max_members):
             waves.append(current)
             current, used = [], 0.0
         current.append(i)

@@ -130,26 +130,36 @@ def serve_plain(requests: list[Request], store: RelationStore) -> tuple[list[Ser
 
 
 class ClosureEstimator:
-    """HBM budgeting estimate of a request's closure. Before any request has
-    finished it is batch.estimate_closure (a random digraph's giant
-    component squared: safe, and far above the closure of graphs made of
-    many small components); afterwards it is `margin` times the largest
-    closure-per-node ratio observed so far times the request's nodes, never
-    above the first estimate."""
+    """HBM budgeting estimate of a request's closure: the random-digraph
+    estimate batch.estimate_closure (its giant component squared: safe, and
+    far above the closure of graphs made of many small components) scaled
+    by what finished requests of about the same size showed. Requests are
+    bucketed by log2 of their node count; a bucket's factor is the largest
+    observed closure / estimate in it, and a request takes `margin` times
+    the largest factor of its bucket and the two beside it, at most 1 (the
+    plain estimate when none of them has finished a request yet)."""
 
     def __init__(self, margin: float = 2.0):
         self.margin = margin
-        self.ratio = 0.0
+        self.factor: dict[int, float] = {}
+
+    @staticmethod
+    def _bucket(req: Request) -> int:
+        return max(int(req.nodes), 1).bit_length()
 
     def __call__(self, req: Request) -> float:
         prior = batch.estimate_closure(req.nodes, req.src.size)
-        if self.ratio <= 0.0:
+        b = self._bucket(req)
+        seen = [self.factor[k] for k in (b - 1, b, b + 1) if k in self.factor]
+        if not seen:
             return prior
-        return min(prior, self.margin * self.ratio * max(req.nodes, 1))
+        return prior * min(1.0, self.margin * max(seen))
 
     def observe(self, req: Request, rows: int) -> None:
-        if req.nodes:
-            self.ratio = max(self.ratio, rows / req.nodes)
+        prior = batch.estimate_closure(req.nodes, req.src.size)
+        if prior > 0:
+            b = self._bucket(req)
+            self.factor[b] = max(self.factor.get(b, 0.0), rows / prior)
 
 
 def _lane(requests: list[Request], store: RelationStore, t0: float, budget_pairs: float, max_slots: int,

@@ -130,16 +130,6 @@ def test_served_lanes_by_edge_count_with_adaptive_admission(stream):
     assert stats[2].get("plain")  # the largest class ran as a plain lane
 
 
-def test_closure_estimator_learns_a_bound():
-    est = serving.ClosureEstimator(margin=2.0)
-    req = serving.make_requests(1, rate_per_s=1.0, seed=3, min_log2_edges=12, max_log2_edges=12)[0]
-    prior = est(req)
-    est.observe(req, 1000 * req.nodes)
-    assert est(req) == min(prior, 2.0 * 1000 * req.nodes)
-    est.observe(req, 10 * req.nodes)  # the largest ratio stays
-    assert est(req) == min(prior, 2.0 * 1000 * req.nodes)
-
-
 def test_remove_ranges_matches_filter(stream):
     rng = np.random.default_rng(0x77)
     a = rng.integers(0, 4000, 20000).astype(np.uint32)

@@ -73,3 +73,22 @@ def test_overlapping_sets_with_zipf_keys():
     assert np.intersect1d(lk, rk).size == 20000
     counts = np.bincount(lf[lf < 100], minlength=100)
     assert counts[0] > counts[50]
+
+
+def test_plan_waves_respects_budget_and_id_space():
+    from sparkalog_amd import batch
+
+    nodes = np.full(100, 1 << 13, np.int64)   # 13 local bits: at most 2^19 members
+    edges = np.full(100, 1 << 14, np.int64)
+    waves = batch.plan_waves(nodes, edges, budget_pairs=1e30)
+    assert waves == [list(range(100))]
+    waves = batch.plan_waves(nodes, edges, budget_pairs=1e30, max_members=7)
+    assert [len(w) for w in waves] == [7] * 14 + [2]
+    assert sum(waves, []) == list(range(100))
+    est = batch.estimate_closure(1 << 13, 1 << 14)
+    waves = batch.plan_waves(nodes, edges, budget_pairs=3.5 * est)
+    assert all(len(w) == 3 for w in waves[:-1]) and sum(waves, []) == list(range(100))
+    # 2^18 local ids leave 2^14 members
+    big = np.full(20000, 1 << 18, np.int64)
+    waves = batch.plan_waves(big, np.ones(20000, np.int64), budget_pairs=1e30)
+    assert max(len(w) for w in waves) == 1 << 14
