@@ -342,3 +342,100 @@ def test_database_loads_files_on_the_device(tmp_path, stream):
     db.run()
     assert db.query("path").rows == [[10, 20], [10, 30], [10, 40], [10, 50], [20, 30], [20, 40], [20, 50],
                                      [30, 40], [30, 50], [40, 50]]
+
+
+# ---- random programs against a naive stratified evaluator ----
+
+def _naive_eval(facts, rules, order):
+    """Set-semantics evaluation of binary rules stratum by stratum (order =
+    lists of predicates, each list a stratum evaluated to its fixpoint).
+    Rules: (head, body) with body a list of (negated, pred, (t0, t1)), terms
+    variable names. Independent of the device pipeline and of the oracle."""
+    rel = {p: set(v) for p, v in facts.items()}
+    for stratum in order:
+        changed = True
+        while changed:
+            changed = False
+            for head, hv, body in rules:
+                if head not in stratum:
+                    continue
+                envs = [{}]
+                for neg, pred, tv in body:
+                    if neg:
+                        continue
+                    nxt = []
+                    for env in envs:
+                        for a, b in rel.get(pred, ()):
+                            e = dict(env)
+                            ok = True
+                            for var, val in zip(tv, (a, b)):
+                                if e.setdefault(var, val) != val:
+                                    ok = False
+                            if ok:
+                                nxt.append(e)
+                    envs = nxt
+                for neg, pred, tv in body:
+                    if neg:
+                        envs = [e for e in envs if (e[tv[0]], e[tv[1]]) not in rel.get(pred, set())]
+                for e in envs:
+                    t = (e[hv[0]], e[hv[1]])
+                    if t not in rel.setdefault(head, set()):
+                        rel[head].add(t)
+                        changed = True
+    return rel
+
+
+def _random_program(rng):
+    """A random stratified program over edb e0, e1 and idb p0..p3 with rule
+    shapes the device pipeline lowers."""
+    facts = {f"e{i}": {(int(a), int(b)) for a, b in rng.integers(0, 12, (int(rng.integers(5, 40)), 2))}
+             for i in range(2)}
+    rules = []
+    defined = ["e0", "e1"]
+    strata = []
+    for k in range(4):
+        p = f"p{k}"
+        shape = rng.choice(["copy", "flip", "join", "neg", "tc", "tc_left"])
+        a, b = rng.choice(defined, 2)
+        if shape == "copy":
+            rules.append((p, ("x", "y"), [(False, a, ("x", "y"))]))
+        elif shape == "flip":
+            rules.append((p, ("y", "x"), [(False, a, ("x", "y"))]))
+        elif shape == "join":
+            rules.append((p, ("x", "z"), [(False, a, ("x", "y")), (False, b, ("y", "z"))]))
+        elif shape == "neg":
+            rules.append((p, ("x", "y"), [(False, a, ("x", "y")), (True, b, ("y", "x") if rng.random() < 0.5
+                                                                              else ("x", "y"))]))
+        elif shape == "tc":
+            rules.append((p, ("x", "y"), [(False, a, ("x", "y"))]))
+            rules.append((p, ("x", "z"), [(False, p, ("x", "y")), (False, b, ("y", "z"))]))
+        else:  # recursion through the left column of the static input
+            rules.append((p, ("x", "y"), [(False, a, ("x", "y"))]))
+            rules.append((p, ("x", "z"), [(False, b, ("x", "y")), (False, p, ("y", "z"))]))
+        if rng.random() < 0.3:  # a second rule for the same predicate
+            rules.append((p, ("x", "y"), [(False, b, ("y", "x"))]))
+        defined.append(p)
+        strata.append([p])
+    return facts, rules, strata
+
+
+def _source(facts, rules):
+    lines = [f"{p}({a}, {b})." for p, ts in facts.items() for a, b in sorted(ts)]
+    for head, hv, body in rules:
+        lits = ", ".join(("!" if neg else "") + f"{pred}({t[0]}, {t[1]})" for neg, pred, t in body)
+        lines.append(f"{head}({hv[0]}, {hv[1]}) :- {lits}.")
+    return "\n".join(lines) + "\n"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(24))
+def test_random_programs_match_naive_evaluation(seed, stream):
+    rng = np.random.default_rng(0xDA7A + seed)
+    facts, rules, strata = _random_program(rng)
+    expected = _naive_eval(facts, rules, strata)
+    db = dl.Database(stream)
+    db.load_program(_source(facts, rules))
+    db.run()
+    for p in ["e0", "e1", "p0", "p1", "p2", "p3"]:
+        got = {tuple(r) for r in db.query(p).rows} if db.catalog.predicates.id(p) in db.active else set()
+        assert got == expected.get(p, set()), (seed, p, _source(facts, rules))
