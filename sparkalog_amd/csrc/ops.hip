@@ -487,6 +487,34 @@ __device__ __forceinline__ uint32_t merge_path_lds(const uint64_t* a, uint32_t n
     return lo;
 }
 
+// The serial merge of a thread's kMergeItems outputs from (ia, ib): both
+// sides' heads stay in registers, so each output reads LDS once (the side it
+// consumed). A goes first on ties. Outputs past n are 0.
+__device__ __forceinline__ void merge_serial(const uint64_t* la, uint32_t na, const uint64_t* lb, uint32_t nb,
+                                             uint32_t ia, uint32_t ib, uint32_t diag, uint32_t n,
+                                             uint64_t (&v)[kMergeItems]) {
+    uint64_t ha = ia < na ? la[ia] : 0, hb = ib < nb ? lb[ib] : 0;
+#pragma unroll
+    for (int i = 0; i < kMergeItems; ++i) {
+        const bool take_a = ib >= nb || (ia < na && ha <= hb);
+        v[i] = 0;
+        if (diag + i < n) {
+            if (take_a) {
+                v[i] = ha;
+                if (++ia < na) ha = la[ia];
+            } else {
+                v[i] = hb;
+                if (++ib < nb) hb = lb[ib];
+            }
+        }
+    }
+}
+
+// LDS index of merged row j in a blocked-to-striped transposition: thread t's
+// rows [8t, 8t + 8) written unpadded would put lanes 64 bytes apart in a
+// quarter of the banks; one u64 of padding per 32 rows spreads them.
+__device__ __forceinline__ uint32_t merge_pad(uint32_t j) { return j + (j >> 5); }
+
 // Merge-path partition of every tile boundary: part[t] = number of left rows
 // among the first t*kMergeTile merged rows. One thread per boundary, so the
 // searches of all tiles overlap instead of stalling each tile's start.
@@ -497,7 +525,7 @@ __device__ __forceinline__ uint32_t merge_path_lds(const uint64_t* a, uint32_t n
 __global__ __launch_bounds__(kBlock) void merge_disjoint_kernel(const uint64_t* a, size_t na, const uint64_t* b,
                                                                 size_t nb, uint64_t* out, uint32_t* out_second,
                                                                 const size_t* part) {
-    __shared__ uint64_t lds[kMergeTile];
+    __shared__ uint64_t lds[kMergeTile + kMergeTile / 32];  // + padding of the transposition (merge_pad)
     const size_t tile = blockIdx.x;
     const size_t total = na + nb;
     const size_t d0 = tile * kMergeTile;
@@ -516,26 +544,16 @@ __global__ __launch_bounds__(kBlock) void merge_disjoint_kernel(const uint64_t* 
     uint32_t ia = merge_path_lds(la, n_a, lb, n_b, diag);
     uint32_t ib = diag - ia;
     uint64_t v[kMergeItems];
-#pragma unroll
-    for (int i = 0; i < kMergeItems; ++i) {
-        const bool take_a = ib >= n_b || (ia < n_a && la[ia] <= lb[ib]);
-        v[i] = 0;
-        if (diag + i < n) {
-            v[i] = take_a ? la[ia] : lb[ib];
-            if (take_a)
-                ++ia;
-            else
-                ++ib;
-        }
-    }
+    merge_serial(la, n_a, lb, n_b, ia, ib, diag, n, v);
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < kMergeItems; ++i)
-        if (diag + i < n) lds[diag + i] = v[i];
+        if (diag + i < n) lds[merge_pad(diag + i)] = v[i];
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += kBlock) out[d0 + i] = lds[i];
+    for (uint32_t i = threadIdx.x; i < n; i += kBlock) out[d0 + i] = lds[merge_pad(i)];
     if (out_second)
-        for (uint32_t i = threadIdx.x; i < n; i += kBlock) out_second[d0 + i] = static_cast<uint32_t>(lds[i]);
+        for (uint32_t i = threadIdx.x; i < n; i += kBlock)
+            out_second[d0 + i] = static_cast<uint32_t>(lds[merge_pad(i)]);
 }
 
 __global__ void merge_partition_kernel(PairInput left, size_t nl, PairInput right, size_t nr, size_t* part,
@@ -553,8 +571,9 @@ __global__ void merge_partition_kernel(PairInput left, size_t nl, PairInput righ
 __global__ __launch_bounds__(kBlock) void union_kernel(PairInput left, size_t nl, PairInput right, size_t nr,
                                                        PairOutput out, uint64_t* unique_rows, const size_t* part,
                                                        uint64_t* status, Header* header, uint32_t* error) {
-    __shared__ uint64_t lds[kMergeTile + 2];
+    __shared__ uint64_t lds[kMergeTile + kMergeTile / 32 + 2];  // + padding of the compaction (merge_pad)
     __shared__ uint32_t scan_lds[kWavesPerBlock];
+    __shared__ uint64_t wave_last[kWavesPerBlock];  // each wave's last merged row
     __shared__ uint64_t offset_slot;
     __shared__ uint32_t tile_slot;
     __shared__ uint64_t prev_slot;
@@ -601,25 +620,14 @@ __global__ __launch_bounds__(kBlock) void union_kernel(PairInput left, size_t nl
     uint32_t ia = merge_path_lds(la, na, lb, nb, diag);
     uint32_t ib = diag - ia;
     uint64_t v[kMergeItems];
-#pragma unroll
-    for (int i = 0; i < kMergeItems; ++i) {
-        const bool take_a = ib >= nb || (ia < na && la[ia] <= lb[ib]);
-        if (diag + i < n) {
-            v[i] = take_a ? la[ia] : lb[ib];
-            if (take_a)
-                ++ia;
-            else
-                ++ib;
-        } else {
-            v[i] = 0;
-        }
-    }
-    __syncthreads();
-    // write merged run to LDS to find each element's predecessor
-#pragma unroll
-    for (int i = 0; i < kMergeItems; ++i)
-        if (diag + i < n) lds[diag + i] = v[i];
-    __syncthreads();
+    merge_serial(la, na, lb, nb, ia, ib, diag, n, v);
+    // each row's predecessor in merged order: the thread's previous row, or
+    // for its first row the previous thread's last (a shuffle inside the
+    // wave, the previous wave's last row across waves)
+    const uint64_t before = __shfl_up(v[kMergeItems - 1], 1, kWave);
+    if (lane_id() == kWave - 1) wave_last[wave_id()] = v[kMergeItems - 1];
+    __syncthreads();  // also: every LDS read of the merge is done
+    const uint64_t first_prev = lane_id() > 0 ? before : (wave_id() > 0 ? wave_last[wave_id() - 1] : 0);
     uint32_t keep = 0;
 #pragma unroll
     for (int i = 0; i < kMergeItems; ++i) {
@@ -629,7 +637,7 @@ __global__ __launch_bounds__(kBlock) void union_kernel(PairInput left, size_t nl
             if (pos == 0)
                 flag = !have_prev_slot || v[i] != prev_slot;
             else
-                flag = v[i] != lds[pos - 1];
+                flag = v[i] != (i > 0 ? v[i - 1] : first_prev);
             keep |= static_cast<uint32_t>(flag) << i;
         }
     }
@@ -644,10 +652,10 @@ __global__ __launch_bounds__(kBlock) void union_kernel(PairInput left, size_t nl
     uint32_t pos = local;
 #pragma unroll
     for (int i = 0; i < kMergeItems; ++i)
-        if ((keep >> i) & 1u) lds[pos++] = v[i];
+        if ((keep >> i) & 1u) lds[merge_pad(pos++)] = v[i];
     __syncthreads();
     const uint64_t offset = offset_slot;
-    for (uint32_t i = threadIdx.x; i < tile_total; i += kBlock) store_key(out, offset + i, lds[i]);
+    for (uint32_t i = threadIdx.x; i < tile_total; i += kBlock) store_key(out, offset + i, lds[merge_pad(i)]);
     if (tile == tiles_of(total, kMergeTile) - 1 && threadIdx.x == 0) *unique_rows = offset + tile_total;
 }
 
