@@ -194,7 +194,8 @@ const char* const kOpNames[kOpCount] = {"join_count", "join_emit", "distinct", "
 // scan and copy to the offsets)
 const char* const kOpKernels[kOpCount] = {"dense_join_count_kernel", "dense_join_emit_kernel",
                                           "seg_tile_kernel+seg_scan_kernel+seg_write_kernel",
-                                          "anti_join_kernel", "probe_sparse_kernel", "union_kernel",
+                                          "anti_join_kernel", "probe_sparse_kernel",
+                                          "union_tile_kernel+group_scan_kernel+union_tile_kernel",
                                           "merge_disjoint_kernel", ""};
 // Candidates up to this many (by default) are probed one thread each
 // through the first-column index of FULL's runs (few candidates, long FULL

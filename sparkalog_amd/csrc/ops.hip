@@ -3,19 +3,24 @@
 // Design (all integer work, HBM-bound, no MFMA):
 //  * distinct  = onesweep LSD radix sort (Adinets & Merrill 2022) over the
 //    varying 8-bit digits of the packed key, with stable wave64 match-ranking
-//    in LDS, followed by a single-pass unique compaction.
+//    in LDS, followed by a two-pass unique compaction.
 //  * union     = merge-path partitioned merge (Green, McColl & Bader 2012)
-//    with fused duplicate drop and single-pass compaction.
+//    with fused duplicate drop, in two passes: tiles count their kept rows,
+//    then merge again and write at their offsets.
 //  * anti-join = left-tile partitioning with a per-tile right range found by
 //    binary search; ranges that fit LDS are staged there, larger ones are
 //    searched in place, so work is O(L log(R/L)) rather than O(L+R) when
-//    R >> L.
+//    R >> L; then a compaction of the kept rows at offsets from their tiles'
+//    counts.
 //  * join      = single-pass count + exclusive scan, then an output-balanced
 //    emit (every block writes an equal slice of the output, whatever the key
 //    skew).
 //  * filter    = ballot compaction of matching row ids.
-// Variable-size outputs use decoupled look-back, so every operator reads
-// its input once.
+// Where tiles of a compaction would wait on each other's counts (decoupled
+// look-back, whose status polls cross XCDs on this chip), the union, the
+// anti-join and the unique compaction count first and write second: a few
+// bytes more read, no tile ever waits. The sort's digit passes and the join
+// count and filter keep a single pass with look-back.
 #include "ops.hpp"
 #include "prims.hpp"
 
@@ -38,11 +43,11 @@ constexpr int kRadix = 1 << kRadixBits;              // 256 digits
 constexpr int kPasses = 8;                           // 64-bit keys
 constexpr int kCompactItems = 16;
 constexpr int kCompactTile = kBlock * kCompactItems; // 4096 rows
+constexpr int kTileGroup = 64;                      // tiles per group of a two-pass compaction
 constexpr int kMergeItems = 8;
 constexpr int kMergeTile = kBlock * kMergeItems;     // 2048 merged rows
 constexpr int kAntiTile = 2048;                      // left rows per tile
 constexpr int kAntiStage = 2048;                     // right rows staged in LDS per chunk
-constexpr int kAntiCompactTile = 4096;               // rows per compaction tile of the anti-join
 // Above this many right rows per 2048-row left tile, reading the range costs
 // more HBM traffic than ~log2(span) probes per left row.
 constexpr size_t kAntiStreamLimit = 1u << 18;
@@ -99,14 +104,20 @@ __device__ __forceinline__ void store_key(const PairOutput& out, size_t i, uint6
 
 // ------------------------------------------------------------------------
 // radix sort
+//
+// LSD radix sort over the varying 8-bit digits of the packed keys, reduce
+// then scan: each digit pass is an upsweep (every chunk of tiles counts its
+// keys per digit), a scan of the counts in (digit, chunk) order, and a
+// downsweep in which a block sorts its chunk's tiles one after another,
+// stably inside each tile in LDS, and writes every digit's keys at the
+// chunk's running offset for that digit. No tile waits for another: a
+// onesweep pass (per-digit decoupled look-back) ran at a third of the copy
+// ceiling here, its look-back chains crossing XCDs.
 
 struct SortDesc {
-    Header header;                       // counters[0..7]: per-pass tile counters
     uint64_t rows;                       // actual row count of this sort
     uint64_t or_mask;
     uint64_t and_mask;
-    uint32_t hist[kPasses][kRadix];
-    uint64_t digit_start[kPasses][kRadix];
 };
 
 __device__ __forceinline__ uint32_t varying_digits(const SortDesc* d) {
@@ -118,80 +129,7 @@ __device__ __forceinline__ uint32_t varying_digits(const SortDesc* d) {
     return m;
 }
 
-// Pass 0 of the sort: copy/pack keys into keys_a, reduce OR/AND of the keys,
-// build the digit histograms of every pass, and clear status region 0.
-// Launched over a fixed grid with a grid-stride loop.
-__global__ __launch_bounds__(kBlock) void sort_histogram_kernel(PairInput in, size_t rows,
-                                                                uint64_t* keys_a, SortDesc* desc,
-                                                                uint64_t known_varying,
-                                                                uint64_t* status0, size_t status_words) {
-    __shared__ uint32_t hist[kPasses][kRadix];
-    __shared__ uint64_t red_or[kWavesPerBlock], red_and[kWavesPerBlock];
-    for (int i = threadIdx.x; i < kPasses * kRadix; i += kBlock) (&hist[0][0])[i] = 0;
-    __syncthreads();
-    const size_t stride = static_cast<size_t>(gridDim.x) * kBlock;
-    const size_t gtid = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x;
-    for (size_t i = gtid; i < status_words; i += stride) status0[i] = 0;
-    rows = desc->rows;
-    uint64_t acc_or = 0, acc_and = ~0ull;
-    // Every lane of a wave runs the same number of trips so that the wave
-    // ballots below see all 64 lanes.
-    const size_t trips = tiles_of(rows, stride);
-    for (size_t t = 0; t < trips; ++t) {
-        const size_t i = t * stride + gtid;
-        const bool valid = i < rows;
-        uint64_t key = 0;
-        if (valid) {
-            key = load_key(in, i);
-            if (!in.packed || in.packed != keys_a) keys_a[i] = key;
-            acc_or |= key;
-            acc_and &= key;
-        }
-        const uint64_t valid_mask = ballot(valid);
-        if (valid_mask == 0) continue;
-#pragma unroll
-        for (int p = 0; p < kPasses; ++p) {
-            const uint32_t digit = static_cast<uint32_t>(key >> (p * kRadixBits)) & (kRadix - 1);
-            const uint32_t lead = __builtin_amdgcn_readfirstlane(digit);
-            const uint64_t same = ballot(valid && digit == lead);
-            if (same == valid_mask) {
-                if (lane_id() == 0) atomicAdd(&hist[p][lead], static_cast<uint32_t>(__popcll(valid_mask)));
-            } else if (valid) {
-                atomicAdd(&hist[p][digit], 1u);
-            }
-        }
-    }
-    // OR/AND reduction
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-        acc_or |= __shfl_xor(acc_or, off, kWave);
-        acc_and &= __shfl_xor(acc_and, off, kWave);
-    }
-    if (lane_id() == 0) {
-        red_or[wave_id()] = acc_or;
-        red_and[wave_id()] = acc_and;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t o = 0, a = ~0ull;
-        for (int w = 0; w < kWavesPerBlock; ++w) {
-            o |= red_or[w];
-            a &= red_and[w];
-        }
-        if (known_varying == 0) {
-            atomicOr(reinterpret_cast<unsigned long long*>(&desc->or_mask), static_cast<unsigned long long>(o));
-            atomicAnd(reinterpret_cast<unsigned long long*>(&desc->and_mask), static_cast<unsigned long long>(a));
-        }
-    }
-    for (int i = threadIdx.x; i < kPasses * kRadix; i += kBlock) {
-        const uint32_t c = (&hist[0][0])[i];
-        if (c) atomicAdd(&(&desc->hist[0][0])[i], c);
-    }
-}
-
 __global__ void sort_init_kernel(SortDesc* desc, uint64_t known_varying, PairInput in, size_t bound) {
-    for (int i = threadIdx.x; i < kPasses * kRadix; i += blockDim.x) (&desc->hist[0][0])[i] = 0;
-    if (threadIdx.x < 16) reinterpret_cast<uint32_t*>(&desc->header)[threadIdx.x] = 0;
     if (threadIdx.x == 0) {
         desc->rows = actual_rows(in, bound);
         desc->or_mask = known_varying;  // 0 when unknown: OR-reduced on device
@@ -199,202 +137,297 @@ __global__ void sort_init_kernel(SortDesc* desc, uint64_t known_varying, PairInp
     }
 }
 
-// Exclusive scan of each pass's 256-digit histogram (one block, 256 threads).
-__global__ __launch_bounds__(kBlock) void sort_digit_scan_kernel(SortDesc* desc) {
-    __shared__ uint64_t lds[kWavesPerBlock];
-    for (int p = 0; p < kPasses; ++p) {
-        uint64_t total;
-        const uint64_t v = desc->hist[p][threadIdx.x];
-        desc->digit_start[p][threadIdx.x] = block_exclusive_sum<uint64_t>(v, &total, lds);
+// With no varying digit every key is equal and no pass runs: the keys are
+// copied to keys_a, where the sorted output of zero passes lies.
+__global__ void sort_constant_copy_kernel(PairInput in, uint64_t* keys_a, const SortDesc* desc) {
+    if (varying_digits(desc) != 0 || in.packed == keys_a) return;
+    const size_t rows = desc->rows;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < rows; i += stride)
+        keys_a[i] = load_key(in, i);
+}
+
+// A pass's source and destination: the first active pass reads the input
+// (packed or two columns), later ones ping-pong keys_b -> keys_a -> keys_b.
+struct PassIo {
+    bool active;
+    bool from_input;
+    const uint64_t* src;
+    uint64_t* dst;
+};
+
+__device__ __forceinline__ PassIo pass_io(const SortDesc* desc, int pass, uint64_t* keys_a, uint64_t* keys_b) {
+    const uint32_t varying = varying_digits(desc);
+    PassIo io{};
+    io.active = (varying >> pass) & 1u;
+    const uint32_t ordinal = __popc(varying & ((1u << pass) - 1u));
+    io.from_input = ordinal == 0;
+    io.src = (ordinal & 1u) ? keys_b : keys_a;
+    io.dst = (ordinal & 1u) ? keys_a : keys_b;
+    return io;
+}
+
+// Upsweep of a digit pass: block c counts the digits of the keys of chunk c
+// (chunk_tiles tiles of kSortTile keys) into counts[d * chunks + c]. Whole
+// tiles are read as 16-byte vectors (key order does not matter to a count)
+// when the columns are 16-byte aligned. With `reduce` (the first pass of a
+// sort whose varying digits are unknown) it also ORs and ANDs every key into
+// the descriptor, and counts whether or not its digit varies.
+__device__ __forceinline__ void count_digit(uint32_t* hist, uint64_t key, int shift, bool valid,
+                                            uint64_t valid_mask) {
+    const uint32_t digit = static_cast<uint32_t>(key >> shift) & (kRadix - 1);
+    const uint32_t lead = __builtin_amdgcn_readfirstlane(digit);
+    const uint64_t same = ballot(valid && digit == lead);
+    if (same == valid_mask) {
+        if (lane_id() == 0) atomicAdd(&hist[lead], static_cast<uint32_t>(__popcll(valid_mask)));
+    } else if (valid) {
+        atomicAdd(&hist[digit], 1u);
     }
 }
 
-// One onesweep pass over digit `pass`. Keys move from src to dst (selected on
-// the device from the count of varying digits below this pass), ranked
-// stably within the tile by wave64 match masks, and placed at the global
-// digit offset found by per-digit decoupled look-back (thread d owns digit d).
-// The next active pass's status region is cleared slice by slice.
-__global__ __launch_bounds__(kBlock) void sort_pass_kernel(uint64_t* keys_a, uint64_t* keys_b, size_t rows,
-                                                           SortDesc* desc, int pass, uint64_t* status_r0,
-                                                           uint64_t* status_r1, uint32_t* error) {
+__global__ __launch_bounds__(kBlock) void sort_upsweep_kernel(PairInput in, uint64_t* keys_a, uint64_t* keys_b,
+                                                              SortDesc* desc, int pass, uint32_t chunk_tiles,
+                                                              uint32_t* counts, int reduce) {
+    __shared__ uint32_t hist[kRadix];
+    __shared__ uint64_t red_or[kWavesPerBlock], red_and[kWavesPerBlock];
+    const PassIo io = pass_io(desc, pass, keys_a, keys_b);
+    if (!reduce && !io.active) return;
+    const bool from_input = reduce || io.from_input;
+    const size_t rows = desc->rows;
+    const uint32_t chunks = gridDim.x;
+    const uint32_t c = blockIdx.x;
+    const int shift = pass * kRadixBits;
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    const size_t c0 = static_cast<size_t>(c) * chunk_tiles * kSortTile;
+    const size_t c1 = c0 + static_cast<size_t>(chunk_tiles) * kSortTile < rows
+                          ? c0 + static_cast<size_t>(chunk_tiles) * kSortTile
+                          : rows;
+    const uint64_t* packed = from_input ? in.packed : io.src;
+    const bool aligned = packed ? (reinterpret_cast<uintptr_t>(packed) & 15u) == 0
+                                : ((reinterpret_cast<uintptr_t>(in.first) | reinterpret_cast<uintptr_t>(in.second)) &
+                                   15u) == 0;
+    uint64_t acc_or = 0, acc_and = ~0ull;
+    for (size_t t0 = c0; t0 < c1; t0 += kSortTile) {
+        uint64_t kv[kSortItems];
+        const bool whole = c1 - t0 >= static_cast<size_t>(kSortTile) && aligned;  // block-uniform
+        if (whole && packed) {
+            // 2,048 vectors of two keys: thread t loads vectors t + 256 q
+            const uint4* v = reinterpret_cast<const uint4*>(packed + t0);
+#pragma unroll
+            for (int q = 0; q < kSortItems / 2; ++q) {
+                const uint4 x = v[q * kBlock + threadIdx.x];
+                kv[2 * q] = (static_cast<uint64_t>(x.y) << 32) | x.x;
+                kv[2 * q + 1] = (static_cast<uint64_t>(x.w) << 32) | x.z;
+            }
+        } else if (whole) {
+            // 1,024 vectors of four values per column
+            const uint4* f = reinterpret_cast<const uint4*>(in.first + t0);
+            const uint4* g = reinterpret_cast<const uint4*>(in.second + t0);
+            uint4 xf[kSortItems / 4], xg[kSortItems / 4];
+#pragma unroll
+            for (int q = 0; q < kSortItems / 4; ++q) {
+                xf[q] = f[q * kBlock + threadIdx.x];
+                xg[q] = g[q * kBlock + threadIdx.x];
+            }
+#pragma unroll
+            for (int q = 0; q < kSortItems / 4; ++q) {
+                kv[4 * q] = pack_pair(xf[q].x, xg[q].x);
+                kv[4 * q + 1] = pack_pair(xf[q].y, xg[q].y);
+                kv[4 * q + 2] = pack_pair(xf[q].z, xg[q].z);
+                kv[4 * q + 3] = pack_pair(xf[q].w, xg[q].w);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < kSortItems; ++i) {
+                const size_t idx = t0 + i * kBlock + threadIdx.x;
+                const size_t at = idx < c1 ? idx : c1 - 1;
+                kv[i] = from_input ? load_key(in, at) : io.src[at];
+            }
+        }
+        const uint32_t valid_rows = whole ? kSortTile : static_cast<uint32_t>(c1 - t0);
+#pragma unroll
+        for (int i = 0; i < kSortItems; ++i) {
+            // item i of every thread: which positions it holds does not
+            // matter, only that the tile's first valid_rows keys are counted
+            const bool valid = whole || i * kBlock + threadIdx.x < valid_rows;
+            const uint64_t valid_mask = ballot(valid);
+            if (valid_mask == 0) continue;
+            if (valid) {
+                acc_or |= kv[i];
+                acc_and &= kv[i];
+            }
+            count_digit(hist, kv[i], shift, valid, valid_mask);
+        }
+    }
+    __syncthreads();
+    counts[static_cast<size_t>(threadIdx.x) * chunks + c] = hist[threadIdx.x];
+    if (reduce) {
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+            acc_or |= __shfl_xor(acc_or, off, kWave);
+            acc_and &= __shfl_xor(acc_and, off, kWave);
+        }
+        if (lane_id() == 0) {
+            red_or[wave_id()] = acc_or;
+            red_and[wave_id()] = acc_and;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && c0 < c1) {
+            uint64_t o = 0, a = ~0ull;
+            for (int w = 0; w < kWavesPerBlock; ++w) {
+                o |= red_or[w];
+                a &= red_and[w];
+            }
+            atomicOr(reinterpret_cast<unsigned long long*>(&desc->or_mask), static_cast<unsigned long long>(o));
+            atomicAnd(reinterpret_cast<unsigned long long*>(&desc->and_mask), static_cast<unsigned long long>(a));
+        }
+    }
+}
+
+// Exclusive scan in place of each digit's chunk counts (block d, at most
+// kBlock * 16 chunks) and the digit's total.
+__global__ __launch_bounds__(kBlock) void sort_count_scan_kernel(const SortDesc* desc, int pass, uint32_t* counts,
+                                                                 uint32_t chunks, uint64_t* digit_total) {
+    __shared__ uint32_t scan_lds[kWavesPerBlock];
+    if (!((varying_digits(desc) >> pass) & 1u)) return;
+    constexpr int kItems = 16;
+    uint32_t* row = counts + static_cast<size_t>(blockIdx.x) * chunks;
+    uint32_t x[kItems];
+    uint32_t mine = 0;
+#pragma unroll
+    for (int q = 0; q < kItems; ++q) {
+        const uint32_t i = threadIdx.x * kItems + q;
+        x[q] = i < chunks ? row[i] : 0u;
+        mine += x[q];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_sum<uint32_t>(mine, &total, scan_lds);
+#pragma unroll
+    for (int q = 0; q < kItems; ++q) {
+        const uint32_t i = threadIdx.x * kItems + q;
+        if (i < chunks) row[i] = run;
+        run += x[q];
+    }
+    if (threadIdx.x == 0) digit_total[blockIdx.x] = total;
+}
+
+// Downsweep of a digit pass: block c sorts the tiles of chunk c in order.
+// Each tile is ranked stably by wave64 match masks (wave w owns keys
+// [w*1024, w*1024+1024) of the tile; item i of lane j is key w*1024 + i*64 +
+// j), placed in LDS in digit order and written out in runs of one digit at
+// the chunk's running offset for that digit. The next tile's keys are
+// loaded while the current one is ranked.
+__global__ __launch_bounds__(kBlock) void sort_downsweep_kernel(PairInput in, uint64_t* keys_a, uint64_t* keys_b,
+                                                                const SortDesc* desc, int pass,
+                                                                uint32_t chunk_tiles, const uint32_t* counts,
+                                                                const uint64_t* digit_total) {
+    // 40 KB in all, so 4 blocks fit a CU: the scans' scratch lives in
+    // tile_keys while it is idle (before the first tile, and while a tile's
+    // keys are in registers)
     __shared__ uint64_t tile_keys[kSortTile];
     __shared__ uint32_t wave_count[kWavesPerBlock][kRadix];
-    __shared__ uint64_t digit_base[kRadix];  // global start - local start
-    __shared__ uint32_t scan_lds[kWavesPerBlock];
-    __shared__ uint32_t tile_slot;
-
-    const uint32_t varying = varying_digits(desc);
-    if (!((varying >> pass) & 1u)) return;  // constant digit: identity pass
-    const uint32_t ordinal = __popc(varying & ((1u << pass) - 1u));
-    const uint64_t* src = (ordinal & 1u) ? keys_b : keys_a;
-    uint64_t* dst = (ordinal & 1u) ? keys_a : keys_b;
-    uint64_t* status = (ordinal & 1u) ? status_r1 : status_r0;
-    uint64_t* next_status = (ordinal & 1u) ? status_r0 : status_r1;
+    __shared__ uint64_t digit_base[kRadix];  // output position of the digit's tile run - its local start
+    __shared__ uint64_t running[kRadix];     // next output position of the digit in this chunk
+    uint32_t* const scan_lds = reinterpret_cast<uint32_t*>(tile_keys);
+    uint64_t* const scan64 = tile_keys;
+    const PassIo io = pass_io(desc, pass, keys_a, keys_b);
+    if (!io.active) return;
+    const size_t rows = desc->rows;
+    const uint32_t chunks = gridDim.x;
+    const uint32_t c = blockIdx.x;
     const int shift = pass * kRadixBits;
-
-    rows = desc->rows;
-    const uint32_t tile = next_tile(&desc->header.counters[pass], &tile_slot);
-    const size_t tile_base = static_cast<size_t>(tile) * kSortTile;
-    if (tile_base >= rows) return;  // launch sized for an upper bound
-    const uint32_t lane = lane_id();
-    const uint32_t wave = wave_id();
-
-    for (int i = threadIdx.x; i < kWavesPerBlock * kRadix; i += kBlock) (&wave_count[0][0])[i] = 0;
-    // clear this tile's slice of the next active pass's status region
-    for (int d = threadIdx.x; d < kRadix; d += kBlock) next_status[static_cast<size_t>(tile) * kRadix + d] = 0;
-    __syncthreads();
-
-    // Load: wave w owns keys [w*1024, w*1024+1024) of the tile; item i of lane
-    // j is key w*1024 + i*64 + j (coalesced 512-byte rows).
+    const uint32_t lane = lane_id(), wave = wave_id();
+    const uint32_t d = threadIdx.x;  // thread d owns digit d
+    {
+        uint64_t total;
+        const uint64_t start = block_exclusive_sum<uint64_t>(digit_total[d], &total, scan64);
+        running[d] = start + counts[static_cast<size_t>(d) * chunks + c];
+    }
+    const size_t c0 = static_cast<size_t>(c) * chunk_tiles * kSortTile;
+    const size_t c1 = c0 + static_cast<size_t>(chunk_tiles) * kSortTile < rows
+                          ? c0 + static_cast<size_t>(chunk_tiles) * kSortTile
+                          : rows;
+    if (c0 >= c1) return;  // block-uniform
     uint64_t keys[kSortItems];
-    uint32_t ranks[kSortItems];
+    const auto load_tile = [&](size_t t0, uint64_t (&kv)[kSortItems]) {
 #pragma unroll
-    for (int i = 0; i < kSortItems; ++i) {
-        const size_t idx = tile_base + wave * (kSortItems * kWave) + i * kWave + lane;
-        keys[i] = idx < rows ? src[idx] : ~0ull;
-    }
-#pragma unroll
-    for (int i = 0; i < kSortItems; ++i) {
-        const size_t idx = tile_base + wave * (kSortItems * kWave) + i * kWave + lane;
-        const bool valid = idx < rows;
-        const uint32_t digit = static_cast<uint32_t>(keys[i] >> shift) & (kRadix - 1);
-        const uint64_t peers = match_digit<kRadixBits>(digit, valid);
-        const uint32_t before = valid ? wave_count[wave][digit] : 0;
-        const uint32_t below = lanes_below(peers);
-        ranks[i] = before + below;
-        if (valid && below == 0) wave_count[wave][digit] = before + static_cast<uint32_t>(__popcll(peers));
-    }
-    __syncthreads();
-
-    // Thread d: tile count of digit d, exclusive over waves, then look-back.
-    const uint32_t d = threadIdx.x;
-    uint32_t tile_count = 0;
-#pragma unroll
-    for (int w = 0; w < kWavesPerBlock; ++w) {
-        const uint32_t c = wave_count[w][d];
-        wave_count[w][d] = tile_count;
-        tile_count += c;
-    }
-    uint32_t tile_total;
-    const uint32_t local_start = block_exclusive_sum<uint32_t>(tile_count, &tile_total, scan_lds);
-
-    // Per-digit look-back (single thread per digit chain).
-    uint64_t* my_status = status + static_cast<size_t>(tile) * kRadix + d;
-    uint64_t exclusive = 0;
-    if (tile == 0) {
-        exclusive = desc->digit_start[pass][d];
-        store_status(my_status, kFlagPrefix | ((exclusive + tile_count) & kValueMask));
-    } else {
-        store_status(my_status, kFlagAggregate | tile_count);
-        int64_t j = static_cast<int64_t>(tile) - 1;
-        uint32_t spins = 0;
-        for (;;) {
-            const uint64_t w = load_status(status + static_cast<size_t>(j) * kRadix + d);
-            const uint64_t flag = w >> 62;
-            if (flag == 0) {
-                if (++spins >= kSpinLimit) {
-                    atomicOr(error, kErrLookback);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-                continue;
-            }
-            exclusive += w & kValueMask;
-            if (flag == 2) break;
-            --j;
-            spins = 0;
+        for (int i = 0; i < kSortItems; ++i) {
+            const size_t idx = t0 + wave * (kSortItems * kWave) + i * kWave + lane;
+            const size_t at = idx < c1 ? idx : c1 - 1;
+            kv[i] = io.from_input ? load_key(in, at) : io.src[at];
         }
-        store_status(my_status, kFlagPrefix | ((exclusive + tile_count) & kValueMask));
-    }
-    digit_base[d] = exclusive - local_start;
-    // local positions: reuse wave_count as local start per (wave, digit)
+    };
+    load_tile(c0, keys);
+    for (size_t t0 = c0; t0 < c1; t0 += kSortTile) {
+        uint64_t next[kSortItems];
+        if (t0 + kSortTile < c1) load_tile(t0 + kSortTile, next);
+        for (int i = threadIdx.x; i < kWavesPerBlock * kRadix; i += kBlock) (&wave_count[0][0])[i] = 0;
+        __syncthreads();
+        uint32_t ranks[kSortItems];
 #pragma unroll
-    for (int w = 0; w < kWavesPerBlock; ++w) wave_count[w][d] += local_start;
-    __syncthreads();
-
-    // Scatter into LDS in tile-sorted order.
-#pragma unroll
-    for (int i = 0; i < kSortItems; ++i) {
-        const size_t idx = tile_base + wave * (kSortItems * kWave) + i * kWave + lane;
-        if (idx < rows) {
+        for (int i = 0; i < kSortItems; ++i) {
+            const size_t idx = t0 + wave * (kSortItems * kWave) + i * kWave + lane;
+            const bool valid = idx < c1;
             const uint32_t digit = static_cast<uint32_t>(keys[i] >> shift) & (kRadix - 1);
-            tile_keys[wave_count[wave][digit] + ranks[i]] = keys[i];
+            const uint64_t peers = match_digit<kRadixBits>(digit, valid);
+            const uint32_t before = valid ? wave_count[wave][digit] : 0;
+            const uint32_t below = lanes_below(peers);
+            ranks[i] = before + below;
+            if (valid && below == 0) wave_count[wave][digit] = before + static_cast<uint32_t>(__popcll(peers));
         }
-    }
-    __syncthreads();
-    // Write out in local order: consecutive threads write consecutive
-    // positions of the same digit run.
-    const uint32_t valid_rows = static_cast<uint32_t>(rows - tile_base < static_cast<size_t>(kSortTile)
-                                                          ? rows - tile_base
-                                                          : kSortTile);
-    for (uint32_t i = threadIdx.x; i < valid_rows; i += kBlock) {
-        const uint64_t k = tile_keys[i];
-        const uint32_t digit = static_cast<uint32_t>(k >> shift) & (kRadix - 1);
-        dst[digit_base[digit] + i] = k;
+        __syncthreads();
+        // thread d: the tile's count of digit d, exclusive over waves
+        uint32_t tile_count = 0;
+#pragma unroll
+        for (int w = 0; w < kWavesPerBlock; ++w) {
+            const uint32_t cnt = wave_count[w][d];
+            wave_count[w][d] = tile_count;
+            tile_count += cnt;
+        }
+        uint32_t tile_total;
+        const uint32_t local_start = block_exclusive_sum<uint32_t>(tile_count, &tile_total, scan_lds);
+        digit_base[d] = running[d] - local_start;
+        running[d] += tile_count;
+#pragma unroll
+        for (int w = 0; w < kWavesPerBlock; ++w) wave_count[w][d] += local_start;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kSortItems; ++i) {
+            const size_t idx = t0 + wave * (kSortItems * kWave) + i * kWave + lane;
+            if (idx < c1) {
+                const uint32_t digit = static_cast<uint32_t>(keys[i] >> shift) & (kRadix - 1);
+                tile_keys[wave_count[wave][digit] + ranks[i]] = keys[i];
+            }
+        }
+        __syncthreads();
+        // consecutive threads write consecutive positions of one digit's run
+        const uint32_t valid_rows = static_cast<uint32_t>(c1 - t0 < static_cast<size_t>(kSortTile) ? c1 - t0
+                                                                                                  : kSortTile);
+        for (uint32_t i = threadIdx.x; i < valid_rows; i += kBlock) {
+            const uint64_t k = tile_keys[i];
+            const uint32_t digit = static_cast<uint32_t>(k >> shift) & (kRadix - 1);
+            io.dst[digit_base[digit] + i] = k;
+        }
+        __syncthreads();  // tile_keys and digit_base are rewritten by the next tile
+#pragma unroll
+        for (int i = 0; i < kSortItems; ++i) keys[i] = next[i];
     }
 }
 
-// Single-pass unique compaction of sorted keys (in keys_a or keys_b as the
-// sort descriptor says). Writes unique keys to `out` in order and the count.
-__global__ __launch_bounds__(kBlock) void unique_kernel(const uint64_t* keys_a, const uint64_t* keys_b,
-                                                        const SortDesc* sort_desc, int sorted_in_b,
-                                                        size_t rows, const uint64_t* rows_dev, PairOutput out,
-                                                        uint64_t* unique_rows, uint64_t* status, Header* header,
-                                                        uint32_t* error) {
-    __shared__ uint64_t stage[kCompactTile];
-    __shared__ uint32_t scan_lds[kWavesPerBlock];
-    __shared__ uint64_t offset_slot;
-    __shared__ uint32_t tile_slot;
-    const uint64_t* keys = keys_a;
-    if (sort_desc) {
-        keys = (__popc(varying_digits(sort_desc)) & 1u) ? keys_b : keys_a;
-        rows = sort_desc->rows;
-    } else if (sorted_in_b) {
-        keys = keys_b;
-    }
-    if (rows_dev) rows = *rows_dev < rows ? *rows_dev : rows;
-    const uint32_t tile = next_tile(&header->counters[0], &tile_slot);
-    const size_t base = static_cast<size_t>(tile) * kCompactTile;
-    if (rows == 0) {
-        if (tile == 0 && threadIdx.x == 0) *unique_rows = 0;
-        return;
-    }
-    if (base >= rows) return;
-    // blocked arrangement: thread t owns rows base + t*16 .. +16
-    uint64_t k[kCompactItems];
-    uint32_t keep = 0;
-    const size_t first = base + static_cast<size_t>(threadIdx.x) * kCompactItems;
-    uint64_t prev = 0;
-    bool have_prev = first > 0 && first - 1 < rows;
-    if (have_prev) prev = keys[first - 1];
-#pragma unroll
-    for (int i = 0; i < kCompactItems; ++i) {
-        const size_t idx = first + i;
-        const bool valid = idx < rows;
-        k[i] = valid ? keys[idx] : 0;
-        const bool flag = valid && (!have_prev || k[i] != prev);
-        keep |= static_cast<uint32_t>(flag) << i;
-        prev = k[i];
-        have_prev = valid;
-    }
-    const uint32_t count = __popc(keep);
-    uint32_t tile_total;
-    const uint32_t local = block_exclusive_sum<uint32_t>(count, &tile_total, scan_lds);
-    if (wave_id() == 0) {
-        const uint64_t excl = wave_lookback(status, tile, tile_total, 0, error);
-        if (lane_id() == 0) offset_slot = excl;
-    }
-    // stage kept keys in order
-    uint32_t pos = local;
-#pragma unroll
-    for (int i = 0; i < kCompactItems; ++i)
-        if ((keep >> i) & 1u) stage[pos++] = k[i];
-    __syncthreads();
-    const uint64_t offset = offset_slot;
-    for (uint32_t i = threadIdx.x; i < tile_total; i += kBlock) store_key(out, offset + i, stage[i]);
-    const size_t tiles = tiles_of(rows, kCompactTile);
-    if (tile == tiles - 1 && threadIdx.x == 0) *unique_rows = offset + tile_total;
-}
+// Unique compaction of sorted keys (in keys_a or keys_b as the sort
+// descriptor says, else in keys_a), both passes of a two-pass compaction
+// (the tile-group counts are defined with the union below): tiles of
+// kCompactTile rows; wave w owns rows [w*1024, w*1024+1024), item i of lane
+// j is row w*1024 + i*64 + j, so every load is coalesced and a row's
+// predecessor is a shuffle away. A row is kept when it differs from the row
+// before it. The count pass writes the tile's kept count and adds it to its
+// group's sum; the write pass writes the kept rows at the tile's offset.
+template <bool kWrite>
+__global__ __launch_bounds__(kBlock) void unique_tile_kernel(const uint64_t* keys_a, const uint64_t* keys_b,
+                                                             const SortDesc* sort_desc, size_t rows,
+                                                             const uint64_t* rows_dev, uint32_t* counts,
+                                                             uint64_t* group_sums, PairOutput out);
 
 __global__ void set_u64_kernel(uint64_t* p, uint64_t v) { *p = v; }
 __global__ void set_u32_kernel(uint32_t* p, uint32_t v) { *p = v; }
@@ -414,43 +447,59 @@ uint32_t* error_word(Header* header) {
     return sink ? sink : &header->error;
 }
 
+// Chunks of a digit pass: at most kSortChunks (the count scan's capacity),
+// each a whole number of tiles.
+constexpr size_t kSortChunks = static_cast<size_t>(kBlock) * 16;
+
 struct SortLayout {
-    size_t desc_off, status0_off, status1_off, unique_status_off, total;
-    size_t sort_tiles, unique_tiles;
+    size_t desc_off, counts_off, totals_off, unique_counts_off, unique_groups_off, total;
+    size_t sort_tiles, chunk_tiles, chunks, unique_tiles, unique_groups;
 };
 
 SortLayout sort_layout(size_t rows) {
     SortLayout l{};
     l.sort_tiles = tiles_of(rows, kSortTile);
+    l.chunk_tiles = l.sort_tiles > kSortChunks ? tiles_of(l.sort_tiles, kSortChunks) : 1;
+    l.chunks = tiles_of(l.sort_tiles, l.chunk_tiles);
     l.unique_tiles = tiles_of(rows, kCompactTile);
+    l.unique_groups = tiles_of(l.unique_tiles, kTileGroup);
     l.desc_off = 0;
     size_t off = align_up(sizeof(SortDesc), 256);
-    l.status0_off = off;
-    off += align_up(l.sort_tiles * kRadix * sizeof(uint64_t), 256);
-    l.status1_off = off;
-    off += align_up(l.sort_tiles * kRadix * sizeof(uint64_t), 256);
-    l.unique_status_off = off;
-    off += align_up(l.unique_tiles * sizeof(uint64_t), 256);
+    l.counts_off = off;
+    off += align_up(kRadix * (l.chunks ? l.chunks : 1) * sizeof(uint32_t), 256);
+    l.totals_off = off;
+    off += align_up(kRadix * sizeof(uint64_t), 256);
+    l.unique_counts_off = off;
+    off += align_up(l.unique_tiles * sizeof(uint32_t), 256);
+    l.unique_groups_off = off;
+    off += align_up((l.unique_groups ? l.unique_groups : 1) * sizeof(uint64_t), 256);
     l.total = off;
     return l;
 }
 
+unsigned grid_for(size_t n, size_t per_block, unsigned cap);
+
+// rows >= 1; in.packed, when set, is not keys_b. The sorted keys end in
+// keys_a or keys_b by the parity of the number of varying digits.
 hipError_t run_sort(PairInput in, size_t rows, uint64_t* keys_a, uint64_t* keys_b, void* temp,
-                    const SortLayout& l, uint64_t known_varying, hipStream_t stream, uint32_t* error) {
+                    const SortLayout& l, uint64_t known_varying, hipStream_t stream) {
     char* base = static_cast<char*>(temp);
     SortDesc* desc = reinterpret_cast<SortDesc*>(base + l.desc_off);
-    uint64_t* s0 = reinterpret_cast<uint64_t*>(base + l.status0_off);
-    uint64_t* s1 = reinterpret_cast<uint64_t*>(base + l.status1_off);
-    sort_init_kernel<<<1, kBlock, 0, stream>>>(desc, known_varying, in, rows);
-    const size_t hist_blocks = tiles_of(rows, kSortTile);
-    const unsigned hist_grid = static_cast<unsigned>(hist_blocks < 2048 ? (hist_blocks ? hist_blocks : 1) : 2048);
-    sort_histogram_kernel<<<hist_grid, kBlock, 0, stream>>>(in, rows, keys_a, desc, known_varying, s0,
-                                                           l.sort_tiles * kRadix);
-    sort_digit_scan_kernel<<<1, kBlock, 0, stream>>>(desc);
+    uint32_t* counts = reinterpret_cast<uint32_t*>(base + l.counts_off);
+    uint64_t* totals = reinterpret_cast<uint64_t*>(base + l.totals_off);
+    sort_init_kernel<<<1, 64, 0, stream>>>(desc, known_varying, in, rows);
+    const unsigned chunks = static_cast<unsigned>(l.chunks);
+    const uint32_t chunk_tiles = static_cast<uint32_t>(l.chunk_tiles);
     for (int p = 0; p < kPasses; ++p) {
         if (known_varying && !((known_varying >> (p * kRadixBits)) & (kRadix - 1))) continue;
-        sort_pass_kernel<<<static_cast<unsigned>(l.sort_tiles), kBlock, 0, stream>>>(keys_a, keys_b, rows, desc,
-                                                                                    p, s0, s1, error);
+        // unknown varying digits: pass 0's upsweep finds them
+        const int reduce = !known_varying && p == 0;
+        sort_upsweep_kernel<<<chunks, kBlock, 0, stream>>>(in, keys_a, keys_b, desc, p, chunk_tiles, counts, reduce);
+        if (reduce)
+            sort_constant_copy_kernel<<<grid_for(rows, kBlock * 4, 2048), kBlock, 0, stream>>>(in, keys_a, desc);
+        sort_count_scan_kernel<<<kRadix, kBlock, 0, stream>>>(desc, p, counts, chunks, totals);
+        sort_downsweep_kernel<<<chunks, kBlock, 0, stream>>>(in, keys_a, keys_b, desc, p, chunk_tiles, counts,
+                                                              totals);
     }
     return hipGetLastError();
 }
@@ -458,13 +507,13 @@ hipError_t run_sort(PairInput in, size_t rows, uint64_t* keys_a, uint64_t* keys_
 // ------------------------------------------------------------------------
 // merge-path union
 
-__device__ __forceinline__ size_t merge_path(const PairInput& a, size_t na, const PairInput& b, size_t nb,
-                                             size_t diag) {
-    size_t lo = diag > nb ? diag - nb : 0;
-    size_t hi = diag < na ? diag : na;
+// Merge-path split of diagonal `diag` (the number of A rows among the first
+// `diag` merged rows; A goes first on ties), searched in [lo, hi], which
+// must hold it.
+__device__ __forceinline__ size_t merge_path_range(const PairInput& a, const PairInput& b, size_t diag, size_t lo,
+                                                   size_t hi) {
     while (lo < hi) {
         const size_t mid = (lo + hi) >> 1;
-        // A goes first on ties.
         if (!(load_key(b, diag - 1 - mid) < load_key(a, mid)))
             lo = mid + 1;
         else
@@ -556,71 +605,234 @@ __global__ __launch_bounds__(kBlock) void merge_disjoint_kernel(const uint64_t* 
             out_second[d0 + i] = static_cast<uint32_t>(lds[merge_pad(i)]);
 }
 
+// Merge-path splits part[t] of the diagonals t*step (t < n_parts). Without
+// `coarse` each split is searched over the whole input; with the splits of
+// every group-th diagonal in `coarse` (a launch of this kernel with step *
+// group), a split is searched only between those of the coarse diagonals
+// around it: the split s of d satisfies i0 <= s <= i1 and D0 - i0 <= d - s
+// <= D1 - i1, and the window's rows are few and close together (at 2^28
+// rows a side, one level took 29 dependent random loads per split and 0.5 ms
+// in all). Also zeroes `zero[0, n_zero)` (the tile-group sums of a two-pass
+// union).
 __global__ void merge_partition_kernel(PairInput left, size_t nl, PairInput right, size_t nr, size_t* part,
-                                       size_t n_parts) {
+                                       size_t n_parts, size_t step, const size_t* coarse, size_t group,
+                                       uint64_t* zero, size_t n_zero) {
     nl = actual_rows(left, nl);
     nr = actual_rows(right, nr);
     const size_t total = nl + nr;
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n_zero; t += stride) zero[t] = 0;
     for (size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n_parts; t += stride) {
-        const size_t d = t * kMergeTile < total ? t * kMergeTile : total;
-        part[t] = merge_path(left, nl, right, nr, d);
+        const size_t d = t * step < total ? t * step : total;
+        size_t lo = d > nr ? d - nr : 0;
+        size_t hi = d < nl ? d : nl;
+        if (coarse) {
+            const size_t g = t / group;
+            if (t % group == 0) {
+                part[t] = coarse[g];
+                continue;
+            }
+            const size_t D0 = g * group * step < total ? g * group * step : total;
+            const size_t D1 = (g + 1) * group * step < total ? (g + 1) * group * step : total;
+            const size_t i0 = coarse[g], i1 = coarse[g + 1];
+            const size_t b0 = D0 - i0, b1 = D1 - i1;  // B rows before the coarse diagonals
+            if (i0 > lo) lo = i0;
+            if (i1 < hi) hi = i1;
+            if (d > b1 && d - b1 > lo) lo = d - b1;
+            if (d - b0 + i0 < hi) hi = d - b0 + i0;  // d >= D0 >= b0
+        }
+        part[t] = merge_path_range(left, right, d, lo, hi);
     }
 }
 
-__global__ __launch_bounds__(kBlock) void union_kernel(PairInput left, size_t nl, PairInput right, size_t nr,
-                                                       PairOutput out, uint64_t* unique_rows, const size_t* part,
-                                                       uint64_t* status, Header* header, uint32_t* error) {
+// Diagonals of a partition searched directly; above this, through a coarse
+// level of every kPartGroup-th diagonal.
+constexpr size_t kPartGroup = 64;
+constexpr size_t kPartDirect = 4 * kPartGroup;
+
+size_t coarse_parts(size_t tiles) { return tiles_of(tiles, kPartGroup) + 1; }
+
+// part[0, tiles] of a merge in kMergeTile-row tiles; `coarse` holds
+// coarse_parts(tiles) entries.
+void merge_partition(PairInput left, size_t nl, PairInput right, size_t nr, size_t* part, size_t tiles,
+                     size_t* coarse, hipStream_t stream, uint64_t* zero = nullptr, size_t n_zero = 0) {
+    const size_t n_parts = tiles + 1;
+    if (n_parts <= kPartDirect) {
+        merge_partition_kernel<<<grid_for(n_parts > n_zero ? n_parts : n_zero, kBlock, 4096), kBlock, 0, stream>>>(
+            left, nl, right, nr, part, n_parts, kMergeTile, nullptr, 1, zero, n_zero);
+        return;
+    }
+    const size_t n_coarse = coarse_parts(tiles);
+    merge_partition_kernel<<<grid_for(n_coarse, 64, 4096), 64, 0, stream>>>(
+        left, nl, right, nr, coarse, n_coarse, static_cast<size_t>(kMergeTile) * kPartGroup, nullptr, 1, zero, n_zero);
+    merge_partition_kernel<<<grid_for(n_parts, kBlock, 4096), kBlock, 0, stream>>>(left, nl, right, nr, part, n_parts,
+                                                                                  kMergeTile, coarse, kPartGroup,
+                                                                                  nullptr, 0);
+}
+
+// Tile counts of the two-pass compactions (union, anti-join): tiles are
+// grouped by kTileGroup; a group's sum is added by its tiles' count pass
+// (one atomic per tile), a one-block scan turns the group sums into
+// offsets, and a tile of the write pass adds the counts of the earlier
+// tiles of its group (one wave load). No tile waits on another: on this
+// chip a status poll crosses XCDs, and tiles waiting for predecessors in a
+// decoupled look-back ran at a third of the copy ceiling.
+
+__global__ __launch_bounds__(1024) void group_scan_kernel(uint64_t* group_sums, size_t n, uint64_t* total64,
+                                                          uint32_t* total32) {
+    __shared__ uint64_t scan_lds[1024 / kWave];
+    __shared__ uint64_t total_slot;
+    block_scan_inplace<1024, 4>(group_sums, n, &total_slot, scan_lds);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (total64) *total64 = total_slot;
+        if (total32) *total32 = static_cast<uint32_t>(total_slot);
+    }
+}
+
+// Output offset of `tile` from the scanned group sums and the counts of the
+// earlier tiles of its group; call from one whole wave.
+__device__ __forceinline__ uint64_t group_tile_offset(const uint64_t* group_prefix, const uint32_t* counts,
+                                                      uint32_t tile) {
+    const uint32_t g0 = tile & ~static_cast<uint32_t>(kTileGroup - 1);
+    const uint32_t lane = lane_id();
+    const uint32_t c = g0 + lane < tile ? counts[g0 + lane] : 0u;
+    return group_prefix[tile / kTileGroup] + wave_sum(static_cast<uint64_t>(c));
+}
+
+template <bool kWrite>
+__global__ __launch_bounds__(kBlock) void unique_tile_kernel(const uint64_t* keys_a, const uint64_t* keys_b,
+                                                             const SortDesc* sort_desc, size_t rows,
+                                                             const uint64_t* rows_dev, uint32_t* counts,
+                                                             uint64_t* group_sums, PairOutput out) {
+    constexpr int kItems = kCompactItems;      // 16
+    constexpr int kWaveRows = kItems * kWave;  // 1024
+    __shared__ uint32_t wave_tot[kWavesPerBlock];
+    __shared__ uint64_t offset_slot;
+    const uint64_t* keys = keys_a;
+    if (sort_desc) {
+        keys = (__popc(varying_digits(sort_desc)) & 1u) ? keys_b : keys_a;
+        rows = sort_desc->rows;
+    }
+    if (rows_dev) rows = *rows_dev < rows ? *rows_dev : rows;
+    const uint32_t tile = blockIdx.x;
+    const size_t base = static_cast<size_t>(tile) * kCompactTile;
+    if (base >= rows) {
+        if (!kWrite && threadIdx.x == 0) counts[tile] = 0;
+        return;
+    }
+    const uint32_t lane = lane_id(), wave = wave_id();
+    const size_t w0 = base + static_cast<size_t>(wave) * kWaveRows;  // the wave's first row
+    // every load is issued before any is used (rows clamped to the input)
+    uint64_t k[kItems];
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        const size_t r = w0 + i * kWave + lane;
+        k[i] = keys[r < rows ? r : rows - 1];
+    }
+    uint64_t before0 = 0;  // the row before the wave's first (lane 0)
+    if (lane == 0 && w0 > 0 && w0 <= rows) before0 = keys[w0 - 1];
+    uint64_t offset = 0;
+    if (kWrite && wave == 0) offset = group_tile_offset(group_sums, counts, tile);
+    uint64_t masks[kItems];
+    uint32_t mine = 0;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        const size_t r = w0 + i * kWave + lane;
+        const uint64_t up = __shfl_up(k[i], 1, kWave);
+        const uint64_t last_of_prev = i > 0 ? __shfl(k[i > 0 ? i - 1 : 0], kWave - 1, kWave) : before0;
+        const uint64_t pred = lane > 0 ? up : last_of_prev;
+        masks[i] = ballot(r < rows && (r == 0 || k[i] != pred));
+        mine += static_cast<uint32_t>(__popcll(masks[i]));
+    }
+    if (lane == 0) wave_tot[wave] = mine;
+    if (kWrite && threadIdx.x == 0) offset_slot = offset;
+    __syncthreads();
+    if (!kWrite) {
+        if (threadIdx.x == 0) {
+            uint32_t c = 0;
+            for (int w = 0; w < kWavesPerBlock; ++w) c += wave_tot[w];
+            counts[tile] = c;
+            atomicAdd(reinterpret_cast<unsigned long long*>(&group_sums[tile / kTileGroup]),
+                      static_cast<unsigned long long>(c));
+        }
+        return;
+    }
+    uint64_t pos = offset_slot;
+    for (int w = 0; w < static_cast<int>(wave); ++w) pos += wave_tot[w];
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        if ((masks[i] >> lane) & 1ull) store_key(out, pos + lanes_below(masks[i]), k[i]);
+        pos += static_cast<uint32_t>(__popcll(masks[i]));
+    }
+}
+
+// Sorted union, both passes: tile t merges merged rows [t*kMergeTile,
+// (t+1)*kMergeTile) (its split between the inputs from the partition pass)
+// and keeps the rows that differ from their predecessor in merged order. The
+// count pass writes the tile's kept count and adds it to its group's sum;
+// the write pass merges again and writes the kept rows at the tile's offset.
+template <bool kWrite>
+__global__ __launch_bounds__(kBlock) void union_tile_kernel(PairInput left, size_t nl, PairInput right, size_t nr,
+                                                            const size_t* part, uint32_t* counts,
+                                                            uint64_t* group_sums, PairOutput out) {
     __shared__ uint64_t lds[kMergeTile + kMergeTile / 32 + 2];  // + padding of the compaction (merge_pad)
     __shared__ uint32_t scan_lds[kWavesPerBlock];
     __shared__ uint64_t wave_last[kWavesPerBlock];  // each wave's last merged row
     __shared__ uint64_t offset_slot;
-    __shared__ uint32_t tile_slot;
     __shared__ uint64_t prev_slot;
-    __shared__ int have_prev_slot;
     nl = actual_rows(left, nl);
     nr = actual_rows(right, nr);
-    const uint32_t tile = next_tile(&header->counters[0], &tile_slot);
+    const uint32_t tile = blockIdx.x;
     const size_t total = nl + nr;
     const size_t d0 = static_cast<size_t>(tile) * kMergeTile;
-    if (total == 0) {
-        if (tile == 0 && threadIdx.x == 0) *unique_rows = 0;
+    if (d0 >= total) {
+        if (!kWrite && threadIdx.x == 0) counts[tile] = 0;
         return;
     }
-    if (d0 >= total) return;
     const size_t d1 = d0 + kMergeTile < total ? d0 + kMergeTile : total;
     const size_t a0 = part[tile], a1 = part[tile + 1];
     const size_t b0 = d0 - a0, b1 = d1 - a1;
     const uint32_t na = static_cast<uint32_t>(a1 - a0), nb = static_cast<uint32_t>(b1 - b0);
-    uint64_t* la = lds;
-    uint64_t* lb = lds + na;
-    for (uint32_t i = threadIdx.x; i < na; i += kBlock) la[i] = load_key(left, a0 + i);
-    for (uint32_t i = threadIdx.x; i < nb; i += kBlock) lb[i] = load_key(right, b0 + i);
-    // element preceding the tile in merged order: the larger of the last A
-    // and last B consumed before it
+    const uint32_t n = na + nb;
+    // every load of the tile is issued before any is used: position j of the
+    // tile's A-then-B slice lands at lds[j] (positions past n reload a valid
+    // row and are dropped); thread 0 also loads the row before the tile in
+    // merged order (the larger of the last A and last B before it), and in
+    // the write pass wave 0 reads the tile's offset
+    uint64_t x[kMergeItems];
+#pragma unroll
+    for (int k = 0; k < kMergeItems; ++k) {
+        const uint32_t j = threadIdx.x + k * kBlock;
+        const uint32_t jj = j < n ? j : 0;
+        x[k] = jj < na ? load_key(left, a0 + jj) : load_key(right, b0 + (jj - na));
+    }
+    uint64_t pa = 0, pb = 0;
     if (threadIdx.x == 0) {
-        uint64_t p = 0;
-        int h = 0;
-        if (a0 > 0) {
-            p = load_key(left, a0 - 1);
-            h = 1;
-        }
-        if (b0 > 0) {
-            const uint64_t q = load_key(right, b0 - 1);
-            if (!h || q > p) p = q;
-            h = 1;
-        }
-        prev_slot = p;
-        have_prev_slot = h;
+        if (a0 > 0) pa = load_key(left, a0 - 1);
+        if (b0 > 0) pb = load_key(right, b0 - 1);
+    }
+    uint64_t offset = 0;
+    if (kWrite && wave_id() == 0) offset = group_tile_offset(group_sums, counts, tile);
+#pragma unroll
+    for (int k = 0; k < kMergeItems; ++k) {
+        const uint32_t j = threadIdx.x + k * kBlock;
+        if (j < n) lds[j] = x[k];
+    }
+    if (threadIdx.x == 0) {
+        // flag of merged row 0: it differs from the row before the tile, if any
+        prev_slot = a0 == 0 ? pb : (b0 == 0 ? pa : (pa > pb ? pa : pb));
+        if (kWrite) offset_slot = offset;
     }
     __syncthreads();
-    // Each thread merges kMergeItems consecutive outputs.
-    const uint32_t n = na + nb;
+    const bool have_prev = d0 > 0;
+    const uint64_t* la = lds;
+    const uint64_t* lb = lds + na;
+    // each thread merges kMergeItems consecutive outputs
     const uint32_t diag = threadIdx.x * kMergeItems < n ? threadIdx.x * kMergeItems : n;
-    uint32_t ia = merge_path_lds(la, na, lb, nb, diag);
-    uint32_t ib = diag - ia;
+    const uint32_t ia = merge_path_lds(la, na, lb, nb, diag);
     uint64_t v[kMergeItems];
-    merge_serial(la, na, lb, nb, ia, ib, diag, n, v);
+    merge_serial(la, na, lb, nb, ia, diag - ia, diag, n, v);
     // each row's predecessor in merged order: the thread's previous row, or
     // for its first row the previous thread's last (a shuffle inside the
     // wave, the previous wave's last row across waves)
@@ -635,18 +847,21 @@ __global__ __launch_bounds__(kBlock) void union_kernel(PairInput left, size_t nl
         if (pos < n) {
             bool flag;
             if (pos == 0)
-                flag = !have_prev_slot || v[i] != prev_slot;
+                flag = !have_prev || v[i] != prev_slot;
             else
                 flag = v[i] != (i > 0 ? v[i - 1] : first_prev);
             keep |= static_cast<uint32_t>(flag) << i;
         }
     }
-    const uint32_t count = __popc(keep);
     uint32_t tile_total;
-    const uint32_t local = block_exclusive_sum<uint32_t>(count, &tile_total, scan_lds);
-    if (wave_id() == 0) {
-        const uint64_t excl = wave_lookback(status, tile, tile_total, 0, error);
-        if (lane_id() == 0) offset_slot = excl;
+    const uint32_t local = block_exclusive_sum<uint32_t>(static_cast<uint32_t>(__popc(keep)), &tile_total, scan_lds);
+    if (!kWrite) {
+        if (threadIdx.x == 0) {
+            counts[tile] = tile_total;
+            atomicAdd(reinterpret_cast<unsigned long long*>(&group_sums[tile / kTileGroup]),
+                      static_cast<unsigned long long>(tile_total));
+        }
+        return;
     }
     // compact in LDS (reads of lds[] above are complete after the scan's barriers)
     uint32_t pos = local;
@@ -654,9 +869,8 @@ __global__ __launch_bounds__(kBlock) void union_kernel(PairInput left, size_t nl
     for (int i = 0; i < kMergeItems; ++i)
         if ((keep >> i) & 1u) lds[merge_pad(pos++)] = v[i];
     __syncthreads();
-    const uint64_t offset = offset_slot;
-    for (uint32_t i = threadIdx.x; i < tile_total; i += kBlock) store_key(out, offset + i, lds[merge_pad(i)]);
-    if (tile == tiles_of(total, kMergeTile) - 1 && threadIdx.x == 0) *unique_rows = offset + tile_total;
+    const uint64_t base = offset_slot;
+    for (uint32_t i = threadIdx.x; i < tile_total; i += kBlock) store_key(out, base + i, lds[merge_pad(i)]);
 }
 
 // ------------------------------------------------------------------------
@@ -720,9 +934,13 @@ __global__ void anti_partition_kernel(PairInput left, size_t nl, AntiRight right
 // in place when huge. Tiles are independent: the time a tile takes depends
 // on its ranges, and a single pass with look-back would keep finished tiles
 // waiting for slower predecessors (measured: over a third of the time).
+// With `tile_kept`, the tile also writes its count of rows no right input
+// holds and adds it to its tile group's sum (the count pass of the two-pass
+// compaction, anti_write_kernel).
 __global__ __launch_bounds__(kBlock) void anti_join_kernel(PairInput left, size_t nl, AntiRight rights,
                                                            const size_t* ranges, uint64_t* found_words,
-                                                           uint32_t minor_bits) {
+                                                           uint32_t minor_bits, uint32_t* tile_kept = nullptr,
+                                                           uint64_t* group_sums = nullptr) {
     constexpr int kItems = kAntiTile / kBlock;  // 8
     constexpr int kWaveRows = kItems * kWave;   // 512
     __shared__ uint64_t stage[kAntiStage];
@@ -731,7 +949,10 @@ __global__ __launch_bounds__(kBlock) void anti_join_kernel(PairInput left, size_
     nl = actual_rows(left, nl);
     const uint32_t tile = blockIdx.x;
     const size_t base = static_cast<size_t>(tile) * kAntiTile;
-    if (base >= nl) return;
+    if (base >= nl) {
+        if (tile_kept && threadIdx.x == 0) tile_kept[tile] = 0;
+        return;
+    }
     const uint32_t lane = lane_id(), wave = wave_id();
     const size_t rows = nl - base < static_cast<size_t>(kAntiTile) ? nl - base : kAntiTile;
     const uint64_t first_key = left_pair(left, base, minor_bits), last_key = left_pair(left, base + rows - 1, minor_bits);
@@ -817,71 +1038,73 @@ __global__ __launch_bounds__(kBlock) void anti_join_kernel(PairInput left, size_
     }
     // found rows as one 64-bit word per item: bit j of word
     // (base + wave*512 + i*64) / 64 is row base + wave*512 + i*64 + j
+    uint32_t found_rows = 0;
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         const uint64_t m = ballot(((found >> i) & 1u) != 0);
         const size_t row0 = base + wave * kWaveRows + i * kWave;
         if (lane == 0 && row0 < nl) found_words[row0 / kWave] = m;
+        found_rows += static_cast<uint32_t>(__popcll(m));
+    }
+    if (tile_kept) {
+        if (lane == 0) wave_totals[wave][0] = found_rows;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t f = 0;
+            for (int w = 0; w < kWavesPerBlock; ++w) f += wave_totals[w][0];
+            const uint32_t kept = static_cast<uint32_t>(rows) - f;
+            tile_kept[tile] = kept;
+            atomicAdd(reinterpret_cast<unsigned long long*>(&group_sums[tile / kTileGroup]),
+                      static_cast<unsigned long long>(kept));
+        }
     }
 }
 
 // Order-preserving compaction of the left rows whose found bit is clear, at
-// offsets from decoupled look-back (the same work for every tile, so no tile
-// waits long for another).
-__global__ __launch_bounds__(kBlock) void anti_compact_kernel(PairInput left, size_t nl, const uint64_t* found_words,
-                                                              PairOutput out, uint32_t* count32, uint64_t* count64,
-                                                              uint64_t* status, Header* header, uint32_t* error) {
-    constexpr int kItems = kAntiCompactTile / kBlock;  // 16
-    constexpr int kWaveRows = kItems * kWave;          // 1024
-    __shared__ uint32_t wave_totals[kWavesPerBlock][kItems];
+// the offset the tile counts give (the write pass of the two-pass
+// compaction): the anti-join kernel's tiles and row layout, so a tile's
+// found bits are eight words.
+__global__ __launch_bounds__(kBlock) void anti_write_kernel(PairInput left, size_t nl, const uint64_t* found_words,
+                                                            const uint32_t* tile_kept, const uint64_t* group_prefix,
+                                                            PairOutput out) {
+    constexpr int kItems = kAntiTile / kBlock;  // 8
+    constexpr int kWaveRows = kItems * kWave;   // 512
+    __shared__ uint32_t wave_tot[kWavesPerBlock];
     __shared__ uint64_t offset_slot;
-    __shared__ uint32_t tile_slot;
     nl = actual_rows(left, nl);
-    const uint32_t tile = next_tile(&header->counters[0], &tile_slot);
-    const size_t base = static_cast<size_t>(tile) * kAntiCompactTile;
-    if (nl == 0) {
-        if (tile == 0 && threadIdx.x == 0) {
-            if (count32) *count32 = 0;
-            if (count64) *count64 = 0;
-        }
-        return;
-    }
+    const uint32_t tile = blockIdx.x;
+    const size_t base = static_cast<size_t>(tile) * kAntiTile;
     if (base >= nl) return;
     const uint32_t lane = lane_id(), wave = wave_id();
-    const uint32_t rows = static_cast<uint32_t>(nl - base < static_cast<size_t>(kAntiCompactTile) ? nl - base
-                                                                                               : kAntiCompactTile);
-    uint64_t k[kItems], masks[kItems];
+    const uint32_t rows = static_cast<uint32_t>(nl - base < static_cast<size_t>(kAntiTile) ? nl - base : kAntiTile);
+    uint64_t k[kItems], words[kItems];
+    // every load is issued before any is used (rows clamped to the tile)
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         const uint32_t r = wave * kWaveRows + i * kWave + lane;
+        k[i] = load_key(left, base + (r < rows ? r : rows - 1));
         const size_t row0 = base + wave * kWaveRows + i * kWave;  // multiple of 64
-        const uint64_t words = row0 < nl ? found_words[row0 / kWave] : ~0ull;
-        k[i] = r < rows ? load_key(left, base + r) : 0;
-        masks[i] = ballot(r < rows && !((words >> lane) & 1ull));
-        if (lane == 0) wave_totals[wave][i] = static_cast<uint32_t>(__popcll(masks[i]));
+        words[i] = row0 < nl ? found_words[row0 / kWave] : ~0ull;
     }
-    __syncthreads();
-    uint32_t wave_start = 0, tile_total = 0;
-    for (int w = 0; w < kWavesPerBlock; ++w)
-        for (int i = 0; i < kItems; ++i) {
-            if (w < static_cast<int>(wave)) wave_start += wave_totals[w][i];
-            tile_total += wave_totals[w][i];
-        }
-    if (wave == 0) {
-        const uint64_t excl = wave_lookback(status, tile, tile_total, 0, error);
-        if (lane == 0) offset_slot = excl;
-    }
-    __syncthreads();
-    const uint64_t offset = offset_slot;
-    uint32_t pos = wave_start;
+    uint64_t offset = 0;
+    if (wave == 0) offset = group_tile_offset(group_prefix, tile_kept, tile);
+    uint64_t masks[kItems];
+    uint32_t mine = 0;
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
-        if ((masks[i] >> lane) & 1ull) store_key(out, offset + pos + lanes_below(masks[i]), k[i]);
-        pos += static_cast<uint32_t>(__popcll(masks[i]));
+        const uint32_t r = wave * kWaveRows + i * kWave + lane;
+        masks[i] = ballot(r < rows && !((words[i] >> lane) & 1ull));
+        mine += static_cast<uint32_t>(__popcll(masks[i]));
     }
-    if (base + rows == nl && threadIdx.x == 0) {
-        if (count32) *count32 = static_cast<uint32_t>(offset + tile_total);
-        if (count64) *count64 = offset + tile_total;
+    if (lane == 0) wave_tot[wave] = mine;
+    if (threadIdx.x == 0) offset_slot = offset;
+    __syncthreads();
+    uint64_t pos = offset_slot;
+    for (int w = 0; w < static_cast<int>(wave); ++w) pos += wave_tot[w];
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        if ((masks[i] >> lane) & 1ull) store_key(out, pos + lanes_below(masks[i]), k[i]);
+        pos += static_cast<uint32_t>(__popcll(masks[i]));
     }
 }
 
@@ -1228,12 +1451,9 @@ hipError_t radix_sort(PairInput in, size_t rows, uint64_t* keys_a, uint64_t* key
     const SortLayout l = sort_layout(rows);
     if (temp_bytes < kTempHeader + l.total) return hipErrorInvalidValue;
     if (rows == 0) return hipSuccess;
+    if (in.packed && in.packed == keys_b) return hipErrorInvalidValue;
     char* sort_temp = static_cast<char*>(temp) + kTempHeader;
-    // a sort has no output count to mark: without a sink its reports stay in
-    // the header (the sort is only called by owners of a sink)
-    Header* header = static_cast<Header*>(temp);
-    if (!error_sink()) clear_header_kernel<<<1, 64, 0, stream>>>(header);
-    hipError_t st = run_sort(in, rows, keys_a, keys_b, sort_temp, l, known_varying, stream, error_word(header));
+    hipError_t st = run_sort(in, rows, keys_a, keys_b, sort_temp, l, known_varying, stream);
     if (sorted_host && known_varying) {
         int active = 0;
         for (int p = 0; p < kPasses; ++p)
@@ -1252,45 +1472,58 @@ hipError_t distinct(PairInput in, size_t rows, uint64_t* keys_a, uint64_t* keys_
     }
     const SortLayout l = sort_layout(rows);
     if (temp_bytes < kTempHeader + l.total) return hipErrorInvalidValue;
+    if (in.packed && in.packed == keys_b) return hipErrorInvalidValue;
     char* sort_temp = static_cast<char*>(temp) + kTempHeader;
-    // the header (tile counter and error word) is cleared once, before the
-    // sort, so the sort's reports reach the unique count's check
-    Header* header = static_cast<Header*>(temp);
-    clear_header_kernel<<<1, 64, 0, stream>>>(header);
-    uint32_t* error = error_word(header);
-    hipError_t st = run_sort(in, rows, keys_a, keys_b, sort_temp, l, known_varying, stream, error);
+    hipError_t st = run_sort(in, rows, keys_a, keys_b, sort_temp, l, known_varying, stream);
     if (st != hipSuccess) return st;
-    uint64_t* ustatus = reinterpret_cast<uint64_t*>(sort_temp + l.unique_status_off);
-    st = clear_u64(ustatus, l.unique_tiles, stream);
+    uint32_t* ucounts = reinterpret_cast<uint32_t*>(sort_temp + l.unique_counts_off);
+    uint64_t* ugroups = reinterpret_cast<uint64_t*>(sort_temp + l.unique_groups_off);
+    st = clear_u64(ugroups, l.unique_groups, stream);
     if (st != hipSuccess) return st;
     const SortDesc* desc = reinterpret_cast<const SortDesc*>(sort_temp + l.desc_off);
-    unique_kernel<<<static_cast<unsigned>(l.unique_tiles), kBlock, 0, stream>>>(keys_a, keys_b, desc, 0, rows, nullptr,
-                                                                              out, unique_rows, ustatus, header, error);
-    st = hipGetLastError();
-    if (st != hipSuccess) return st;
-    return report_error(error, nullptr, unique_rows, stream);
+    const unsigned grid = static_cast<unsigned>(l.unique_tiles);
+    unique_tile_kernel<false><<<grid, kBlock, 0, stream>>>(keys_a, keys_b, desc, rows, nullptr, ucounts, ugroups, out);
+    group_scan_kernel<<<1, 1024, 0, stream>>>(ugroups, l.unique_groups, unique_rows, nullptr);
+    unique_tile_kernel<true><<<grid, kBlock, 0, stream>>>(keys_a, keys_b, desc, rows, nullptr, ucounts, ugroups, out);
+    return hipGetLastError();
 }
 
 hipError_t unique_sorted(const uint64_t* keys, const uint64_t* rows_dev, size_t bound, PairOutput out,
                          uint64_t* unique_rows, void* temp, size_t temp_bytes, hipStream_t stream) {
     if (bound == 0) return set_u64(unique_rows, 0, stream);
-    const size_t tiles = tiles_of(bound, kCompactTile);
-    if (temp_bytes < compact_temp(tiles)) return hipErrorInvalidValue;
-    hipError_t st = prepare_compact(temp, tiles, stream);
+    const SortLayout l = sort_layout(bound);
+    if (temp_bytes < kTempHeader + l.total) return hipErrorInvalidValue;
+    char* sort_temp = static_cast<char*>(temp) + kTempHeader;
+    uint32_t* ucounts = reinterpret_cast<uint32_t*>(sort_temp + l.unique_counts_off);
+    uint64_t* ugroups = reinterpret_cast<uint64_t*>(sort_temp + l.unique_groups_off);
+    hipError_t st = clear_u64(ugroups, l.unique_groups, stream);
     if (st != hipSuccess) return st;
-    uint32_t* error = error_word(static_cast<Header*>(temp));
-    unique_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(
-        keys, keys, nullptr, 0, bound, rows_dev, out, unique_rows,
-        reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp), error);
-    st = hipGetLastError();
-    if (st != hipSuccess) return st;
-    return report_error(error, nullptr, unique_rows, stream);
+    const unsigned grid = static_cast<unsigned>(l.unique_tiles);
+    unique_tile_kernel<false><<<grid, kBlock, 0, stream>>>(keys, keys, nullptr, bound, rows_dev, ucounts, ugroups, out);
+    group_scan_kernel<<<1, 1024, 0, stream>>>(ugroups, l.unique_groups, unique_rows, nullptr);
+    unique_tile_kernel<true><<<grid, kBlock, 0, stream>>>(keys, keys, nullptr, bound, rows_dev, ucounts, ugroups, out);
+    return hipGetLastError();
 }
 
-size_t union_temp_bytes(size_t left_rows, size_t right_rows) {
-    const size_t tiles = tiles_of(left_rows + right_rows, kMergeTile);
-    return compact_temp(tiles) + align_up((tiles + 1) * sizeof(size_t), 256);
+// Scratch of the union: header, merge-path splits, per-tile kept counts and
+// tile-group sums.
+struct UnionLayout {
+    size_t tiles, groups, part_off, coarse_off, counts_off, groups_off, total;
+};
+
+UnionLayout union_layout(size_t left_rows, size_t right_rows) {
+    UnionLayout l{};
+    l.tiles = tiles_of(left_rows + right_rows, kMergeTile);
+    l.groups = tiles_of(l.tiles, kTileGroup);
+    l.part_off = kTempHeader;
+    l.coarse_off = l.part_off + align_up((l.tiles + 1) * sizeof(size_t), 256);
+    l.counts_off = l.coarse_off + align_up(coarse_parts(l.tiles) * sizeof(size_t), 256);
+    l.groups_off = l.counts_off + align_up(l.tiles * sizeof(uint32_t), 256);
+    l.total = l.groups_off + align_up((l.groups ? l.groups : 1) * sizeof(uint64_t), 256);
+    return l;
 }
+
+size_t union_temp_bytes(size_t left_rows, size_t right_rows) { return union_layout(left_rows, right_rows).total; }
 
 hipError_t sorted_union(PairInput left, size_t left_rows, PairInput right, size_t right_rows, PairOutput out,
                         uint64_t* unique_rows, void* temp, size_t temp_bytes, hipStream_t stream,
@@ -1300,26 +1533,28 @@ hipError_t sorted_union(PairInput left, size_t left_rows, PairInput right, size_
         set_u64_kernel<<<1, 1, 0, stream>>>(unique_rows, 0);
         return hipGetLastError();
     }
-    const size_t tiles = tiles_of(total, kMergeTile);
-    if (temp_bytes < union_temp_bytes(left_rows, right_rows)) return hipErrorInvalidValue;
-    hipError_t st = prepare_compact(temp, tiles, stream);
-    if (st != hipSuccess) return st;
-    size_t* part = reinterpret_cast<size_t*>(static_cast<char*>(temp) + compact_temp(tiles));
-    merge_partition_kernel<<<grid_for(tiles + 1, kBlock, 4096), kBlock, 0, stream>>>(left, left_rows, right,
-                                                                                    right_rows, part, tiles + 1);
-    uint32_t* error = error_word(static_cast<Header*>(temp));
+    const UnionLayout l = union_layout(left_rows, right_rows);
+    if (temp_bytes < l.total) return hipErrorInvalidValue;
+    if (l.tiles > 0xFFFFFFFFull) return hipErrorInvalidValue;
+    char* base = static_cast<char*>(temp);
+    size_t* part = reinterpret_cast<size_t*>(base + l.part_off);
+    uint32_t* counts = reinterpret_cast<uint32_t*>(base + l.counts_off);
+    uint64_t* groups = reinterpret_cast<uint64_t*>(base + l.groups_off);
+    merge_partition(left, left_rows, right, right_rows, part, l.tiles, reinterpret_cast<size_t*>(base + l.coarse_off),
+                    stream, groups, l.groups);
     timer_start(timer, stream);
-    union_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(
-        left, left_rows, right, right_rows, out, unique_rows, part,
-        reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp), error);
+    union_tile_kernel<false><<<static_cast<unsigned>(l.tiles), kBlock, 0, stream>>>(left, left_rows, right, right_rows,
+                                                                                   part, counts, groups, out);
+    group_scan_kernel<<<1, 1024, 0, stream>>>(groups, l.groups, unique_rows, nullptr);
+    union_tile_kernel<true><<<static_cast<unsigned>(l.tiles), kBlock, 0, stream>>>(left, left_rows, right, right_rows,
+                                                                                  part, counts, groups, out);
     timer_stop(timer, stream);
-    st = hipGetLastError();
-    if (st != hipSuccess) return st;
-    return report_error(error, nullptr, unique_rows, stream);
+    return hipGetLastError();
 }
 
 size_t merge_disjoint_temp_bytes(size_t left_rows, size_t right_rows) {
-    return align_up((tiles_of(left_rows + right_rows, kMergeTile) + 1) * sizeof(size_t), 256);
+    const size_t tiles = tiles_of(left_rows + right_rows, kMergeTile);
+    return align_up((tiles + 1) * sizeof(size_t), 256) + align_up(coarse_parts(tiles) * sizeof(size_t), 256);
 }
 
 hipError_t merge_disjoint(const uint64_t* left, size_t left_rows, const uint64_t* right, size_t right_rows,
@@ -1331,8 +1566,9 @@ hipError_t merge_disjoint(const uint64_t* left, size_t left_rows, const uint64_t
     const size_t tiles = tiles_of(total, kMergeTile);
     size_t* part = static_cast<size_t*>(temp);
     const PairInput a{left, nullptr, nullptr, nullptr}, b{right, nullptr, nullptr, nullptr};
-    merge_partition_kernel<<<grid_for(tiles + 1, kBlock, 4096), kBlock, 0, stream>>>(a, left_rows, b, right_rows,
-                                                                                    part, tiles + 1);
+    merge_partition(a, left_rows, b, right_rows, part, tiles,
+                    reinterpret_cast<size_t*>(static_cast<char*>(temp) + align_up((tiles + 1) * sizeof(size_t), 256)),
+                    stream);
     timer_start(timer, stream);
     merge_disjoint_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(left, left_rows, right, right_rows,
                                                                                out, out_second, part);
@@ -1340,11 +1576,26 @@ hipError_t merge_disjoint(const uint64_t* left, size_t left_rows, const uint64_t
     return hipGetLastError();
 }
 
+// Scratch of the anti-join: header, right ranges per (tile, right input),
+// found bits, per-tile kept counts and tile-group sums.
+struct AntiLayout {
+    size_t tiles, groups, ranges_off, found_off, kept_off, groups_off, total;
+};
+
+AntiLayout anti_layout(size_t left_rows, uint32_t right_inputs) {
+    AntiLayout l{};
+    l.tiles = tiles_of(left_rows, kAntiTile);
+    l.groups = tiles_of(l.tiles, kTileGroup);
+    l.ranges_off = kTempHeader;
+    l.found_off = l.ranges_off + align_up(l.tiles * (right_inputs ? right_inputs : 1) * 2 * sizeof(size_t), 256);
+    l.kept_off = l.found_off + align_up(l.tiles * (kAntiTile / kWave) * sizeof(uint64_t), 256);
+    l.groups_off = l.kept_off + align_up(l.tiles * sizeof(uint32_t), 256);
+    l.total = l.groups_off + align_up((l.groups ? l.groups : 1) * sizeof(uint64_t), 256);
+    return l;
+}
+
 size_t anti_join_temp_bytes(size_t left_rows, uint32_t right_inputs) {
-    const size_t tiles = tiles_of(left_rows, kAntiTile);
-    return compact_temp(tiles_of(left_rows, kAntiCompactTile)) +
-           align_up(tiles * (right_inputs ? right_inputs : 1) * 2 * sizeof(size_t), 256) +
-           align_up(tiles * (kAntiTile / kWave) * sizeof(uint64_t), 256);
+    return anti_layout(left_rows, right_inputs).total;
 }
 
 hipError_t sorted_anti_join_multi(PairInput left, size_t left_rows, const AntiRight& rights, PairOutput out,
@@ -1356,30 +1607,28 @@ hipError_t sorted_anti_join_multi(PairInput left, size_t left_rows, const AntiRi
         return hipGetLastError();
     }
     if (rights.count > static_cast<uint32_t>(kMaxAntiRights)) return hipErrorInvalidValue;
-    const size_t tiles = tiles_of(left_rows, kAntiTile);
-    const size_t compact_tiles = tiles_of(left_rows, kAntiCompactTile);
-    if (temp_bytes < anti_join_temp_bytes(left_rows, rights.count)) return hipErrorInvalidValue;
-    hipError_t st = prepare_compact(temp, compact_tiles, stream);
+    const AntiLayout l = anti_layout(left_rows, rights.count);
+    if (temp_bytes < l.total) return hipErrorInvalidValue;
+    if (l.tiles > 0xFFFFFFFFull) return hipErrorInvalidValue;
+    char* base = static_cast<char*>(temp);
+    size_t* ranges = reinterpret_cast<size_t*>(base + l.ranges_off);
+    uint64_t* found_words = reinterpret_cast<uint64_t*>(base + l.found_off);
+    uint32_t* kept = reinterpret_cast<uint32_t*>(base + l.kept_off);
+    uint64_t* groups = reinterpret_cast<uint64_t*>(base + l.groups_off);
+    hipError_t st = clear_u64(groups, l.groups, stream);
     if (st != hipSuccess) return st;
-    char* t = static_cast<char*>(temp) + compact_temp(compact_tiles);
-    size_t* ranges = reinterpret_cast<size_t*>(t);
-    uint64_t* found_words = reinterpret_cast<uint64_t*>(
-        t + align_up(tiles * (rights.count ? rights.count : 1) * 2 * sizeof(size_t), 256));
     if (rights.count) {
-        anti_partition_kernel<<<grid_for(tiles * rights.count, kBlock, 4096), kBlock, 0, stream>>>(
-            left, left_rows, rights, tiles, ranges, 0u);
+        anti_partition_kernel<<<grid_for(l.tiles * rights.count, kBlock, 4096), kBlock, 0, stream>>>(
+            left, left_rows, rights, l.tiles, ranges, 0u);
     }
     timer_start(timer, stream);
-    anti_join_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(left, left_rows, rights, ranges,
-                                                                         found_words, 0u);
+    anti_join_kernel<<<static_cast<unsigned>(l.tiles), kBlock, 0, stream>>>(left, left_rows, rights, ranges,
+                                                                           found_words, 0u, kept, groups);
     timer_stop(timer, stream);
-    uint32_t* error = error_word(static_cast<Header*>(temp));
-    anti_compact_kernel<<<static_cast<unsigned>(compact_tiles), kBlock, 0, stream>>>(
-        left, left_rows, found_words, out, count32, count64,
-        reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp), error);
-    st = hipGetLastError();
-    if (st != hipSuccess) return st;
-    return report_error(error, count32, count64, stream);
+    group_scan_kernel<<<1, 1024, 0, stream>>>(groups, l.groups, count64, count32);
+    anti_write_kernel<<<static_cast<unsigned>(l.tiles), kBlock, 0, stream>>>(left, left_rows, found_words, kept,
+                                                                            groups, out);
+    return hipGetLastError();
 }
 
 size_t probe_grouped_temp_bytes(size_t rows, uint32_t right_inputs) {

@@ -74,7 +74,7 @@ struct AntiRight {
 };
 
 // Unique compaction of sorted keys; the row count is *rows_dev (<= bound)
-// when rows_dev is non-null.
+// when rows_dev is non-null. `temp` holds distinct_temp_bytes(bound).
 hipError_t unique_sorted(const uint64_t* keys, const uint64_t* rows_dev, size_t bound, PairOutput out,
                          uint64_t* unique_rows, void* temp, size_t temp_bytes, hipStream_t stream);
 
