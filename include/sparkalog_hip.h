@@ -15,11 +15,13 @@
  *   - a null pointer with a non-zero length is hipErrorInvalidValue;
  *   - counts are written on the device (u32 for filter and anti-join, u64 for
  *     distinct, union and the join total);
- *   - single-pass kernels find their output offsets by waiting on earlier
- *     tiles with a bounded wait; if such a wait ever gives up (a scheduling
- *     failure of the device, never seen in practice), the call's count is
- *     written as all ones (UINT32_MAX / UINT64_MAX) after its kernels, so
- *     the caller can tell it from a result.
+ *   - the join count and the filter find their output offsets in a single
+ *     pass, waiting on earlier tiles with a bounded wait; if such a wait
+ *     ever gives up (a scheduling failure of the device, never seen in
+ *     practice), the call's count is written as all ones (UINT32_MAX /
+ *     UINT64_MAX) after its kernels, so the caller can tell it from a
+ *     result. The distinct, union and anti-join count first and write
+ *     second, and never wait on another tile.
  * Scratch arrays the reference needed for its CUB pipeline (filter flags,
  * anti-join flags/selection, union packed copies, distinct packed/scratch)
  * keep their parameters; the kernels here use them only where noted.
@@ -168,7 +170,7 @@ sparkalog_hip_error sparkalog_hip_union_u32(const uint32_t* left_first, const ui
  * index_rows[rows] = row ids sorted by (value, row), the layout
  * sparkalog_hip_join_u32_count / _emit take. *unique_keys (u64, device) = K;
  * keys and starts hold at least rows and rows + 1 entries. Asynchronous on
- * `stream`; a look-back wait that gives up writes K as all ones. */
+ * `stream`. */
 sparkalog_hip_error sparkalog_hip_range_index_u32_temporary_bytes(size_t rows, size_t* temporary_bytes);
 sparkalog_hip_error sparkalog_hip_range_index_u32(const uint32_t* values, size_t rows, uint32_t* keys,
                                                   uint32_t* starts, uint32_t* index_rows, uint64_t* unique_keys,

@@ -74,17 +74,14 @@ __global__ void index_end_kernel(const uint64_t* unique_keys, size_t n, uint32_t
     starts[*unique_keys] = static_cast<uint32_t>(n);
 }
 
-__global__ void index_clear_kernel(uint32_t* word) { *word = 0; }
 
 struct IndexLayout {
-    size_t error_off, packed_a_off, packed_b_off, tiles_off, sort_off, total;
+    size_t packed_a_off, packed_b_off, tiles_off, sort_off, total;
 };
 
 IndexLayout index_layout(size_t n) {
     IndexLayout l{};
     size_t off = 0;
-    l.error_off = off;
-    off += 256;
     l.packed_a_off = off;
     off += index_align(n * sizeof(uint64_t));
     l.packed_b_off = off;
@@ -112,7 +109,6 @@ hipError_t range_index(const uint32_t* values, size_t n, uint32_t* keys, uint32_
     const IndexLayout l = index_layout(n);
     if (temp_bytes < l.total) return hipErrorInvalidValue;
     char* base = static_cast<char*>(temp);
-    uint32_t* own_error = reinterpret_cast<uint32_t*>(base + l.error_off);
     uint64_t* packed_a = reinterpret_cast<uint64_t*>(base + l.packed_a_off);
     uint64_t* packed_b = reinterpret_cast<uint64_t*>(base + l.packed_b_off);
     uint64_t* tile_keys = reinterpret_cast<uint64_t*>(base + l.tiles_off);
@@ -126,17 +122,9 @@ hipError_t range_index(const uint32_t* values, size_t n, uint32_t* keys, uint32_
     uint64_t row_mask = 0;
     while (row_mask < n - 1) row_mask = (row_mask << 1) | 1;
     const uint64_t varying = (0xFFFFFFFFull << 32) | (row_mask ? row_mask : 1);
-    // the sort's look-back reports go to a caller's sink when there is one,
-    // else to this call's own word, reported into the key count below
-    const bool own = error_sink() == nullptr;
-    hipError_t st = hipSuccess;
     uint64_t* sorted = nullptr;
-    {
-        if (own) index_clear_kernel<<<1, 1, 0, stream>>>(own_error);
-        ErrorSink sink(own ? own_error : error_sink());
-        PairInput in{packed_a, nullptr, nullptr, nullptr};
-        st = radix_sort(in, n, packed_a, packed_b, sort_temp, l.total - l.sort_off, varying, stream, &sorted);
-    }
+    const PairInput in{packed_a, nullptr, nullptr, nullptr};
+    hipError_t st = radix_sort(in, n, packed_a, packed_b, sort_temp, l.total - l.sort_off, varying, stream, &sorted);
     if (st != hipSuccess) return st;
     index_runs_kernel<false><<<static_cast<unsigned>(tiles), kIndexBlock, 0, stream>>>(sorted, n, tile_keys, nullptr,
                                                                                         nullptr, nullptr);
@@ -144,9 +132,7 @@ hipError_t range_index(const uint32_t* values, size_t n, uint32_t* keys, uint32_
     index_runs_kernel<true><<<static_cast<unsigned>(tiles), kIndexBlock, 0, stream>>>(sorted, n, tile_keys, keys,
                                                                                        starts, rows);
     index_end_kernel<<<1, 1, 0, stream>>>(unique_keys, n, starts);
-    st = hipGetLastError();
-    if (st == hipSuccess && own) st = report_error(own_error, nullptr, unique_keys, stream);
-    return st;
+    return hipGetLastError();
 }
 
 }  // namespace spk
