@@ -104,6 +104,28 @@ def test_distinct_operator_input(key_mode, stream):
     assert same(got, oracle.distinct(*a, threads=8))
 
 
+@pytest.mark.parametrize("offset", [1, 2, 3])
+def test_ops_on_unaligned_column_views(offset, stream):
+    # columns that start 4, 8 or 12 bytes into their allocation: whole tiles
+    # are read as 16-byte vectors only when the columns are 16-byte aligned
+    rng = np.random.default_rng(70 + offset)
+    n = 3 * 4096 + 17
+    a = random_pairs(rng, n + offset, 5000)
+    full = dev(a)
+    view = ex.Relation((full.columns[0][offset:], full.columns[1][offset:]))
+    got = host(ex.distinct_hip(view, plans.BinaryDistinct(), stream))
+    assert same(got, oracle.distinct(a[0][offset:], a[1][offset:]))
+    left = oracle.distinct(*random_pairs(rng, n, 3000))
+    right = oracle.distinct(*random_pairs(rng, n, 3000))
+    # the relations one row into a larger allocation
+    pad = lambda c: (np.concatenate([[7], c[0]]).astype(np.uint32), np.concatenate([[9], c[1]]).astype(np.uint32))
+    lp, rp = dev(pad(left)), dev(pad(right))
+    lv = ex.Relation((lp.columns[0][1:], lp.columns[1][1:]))
+    rv = ex.Relation((rp.columns[0][1:], rp.columns[1][1:]))
+    assert same(host(ex.union_hip(lv, rv, plans.SortedBinaryUnion(), stream)), oracle.union(left, right))
+    assert same(host(ex.anti_join_hip(lv, rv, plans.SortedBinaryAntiJoin(), stream)), oracle.anti_join(left, right))
+
+
 def _sorted_set(rng, n, hi):
     return oracle.distinct(*random_pairs(rng, n, hi))
 
