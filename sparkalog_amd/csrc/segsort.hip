@@ -56,17 +56,25 @@ struct SegHeader {
 };
 
 struct SegLayout {
-    size_t status_off, bounds_off, total, tiles;
+    size_t status_off, groups_off, bounds_off, total, tiles, groups;
 };
 
-// Metadata of one segmented distinct: header, per-tile unique counts (then
-// offsets) and tile bounds.
+// Tiles of a group: a tile adds its unique count to its group's sum, a
+// one-block scan turns the group sums into offsets, and the copy of a tile
+// adds the counts of the earlier tiles of its group (one wave load).
+constexpr uint32_t kSegGroup = 64;
+
+// Metadata of one segmented distinct: header, per-tile unique counts, group
+// sums (then group offsets) and tile bounds.
 SegLayout seg_layout(size_t rows, uint32_t tile) {
     SegLayout l{};
     l.tiles = seg_tiles(rows, tile);
+    l.groups = seg_tiles(l.tiles, kSegGroup);
     size_t off = seg_align(sizeof(SegHeader));
     l.status_off = off;
     off += seg_align(l.tiles * sizeof(uint64_t));
+    l.groups_off = off;
+    off += seg_align((l.groups ? l.groups : 1) * sizeof(uint64_t));
     l.bounds_off = off;
     off += seg_align((l.tiles + 1) * sizeof(size_t));
     l.total = off;
@@ -74,11 +82,14 @@ SegLayout seg_layout(size_t rows, uint32_t tile) {
 }
 
 // bounds[t] = start of the run containing position t*tile (or rows);
-// the first thread also resets the header (the per-tile counts need no
-// reset: every tile writes its own).
+// the first thread also resets the header and the grid the tile-group sums
+// (the per-tile counts need no reset: every tile writes its own).
 __global__ void seg_partition_kernel(const uint64_t* keys, size_t rows, uint32_t minor_bits, size_t* bounds,
-                                     size_t tiles, SegHeader* h, uint32_t tile) {
+                                     size_t tiles, SegHeader* h, uint32_t tile, uint64_t* group_sums,
+                                     size_t groups) {
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t g = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < groups; g += stride)
+        group_sums[g] = 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         h->error = 0;
         h->counter = 0;
@@ -174,7 +185,7 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) 
     uint64_t* keys, size_t rows, uint32_t minor_bits,
                                                           const size_t* bounds, const uint64_t* side_sorted,
                                                           uint64_t* counts, SegHeader* h, const uint64_t* found,
-                                                          uint32_t tile_rows) {
+                                                          uint32_t tile_rows, uint64_t* group_sums) {
     // sort keys and run majors; before ranking, the same bytes hold the
     // tile's surviving raw keys (u64) when some keys are dropped
     __shared__ uint64_t seg_buf[(kSegCap * (sizeof(K) + sizeof(uint32_t)) + 7) / 8];
@@ -518,7 +529,11 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) 
     // the tile's unique rows go back over its own input slice (every load of
     // the slice is done: the write needs no other tile), expanded to pairs;
     // seg_write_kernel moves them to their offsets once every count is known
-    if (threadIdx.x == 0) counts[tile] = count;
+    if (threadIdx.x == 0) {
+        counts[tile] = count;
+        atomicAdd(reinterpret_cast<unsigned long long*>(&group_sums[tile / kSegGroup]),
+                  static_cast<unsigned long long>(count));
+    }
     if (!big) {
         uint32_t pos = wave_start;
 #pragma unroll
@@ -536,8 +551,8 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) 
     }
 }
 
-// Exclusive scan, in place, of the tiles' unique counts (one block, n >= 1),
-// and the total.
+// Exclusive scan, in place, of the tile groups' unique counts (one block,
+// n >= 1), and the total.
 constexpr int kSegScanThreads = 1024;
 __global__ __launch_bounds__(kSegScanThreads) void seg_scan_kernel(uint64_t* counts, size_t n, uint64_t* total) {
     __shared__ uint64_t scan_lds[kSegScanThreads / kWave];
@@ -551,14 +566,25 @@ __global__ __launch_bounds__(kSegScanThreads) void seg_scan_kernel(uint64_t* cou
 // chunk are all issued before any is used (positions clamped to the tile).
 constexpr int kSegWriteItems = 8;
 __global__ __launch_bounds__(kBlock) void seg_write_kernel(const uint64_t* keys, const size_t* bounds,
-                                                           const uint64_t* side_sorted, const uint64_t* offsets,
-                                                           const uint64_t* total, const SegHeader* h, size_t tiles,
-                                                           uint32_t minor_bits, PairOutput out, uint32_t* xidx_out) {
+                                                           const uint64_t* side_sorted, const uint64_t* counts,
+                                                           const uint64_t* group_offsets, const SegHeader* h,
+                                                           size_t tiles, uint32_t minor_bits, PairOutput out,
+                                                           uint32_t* xidx_out) {
+    __shared__ uint64_t offset_slot;
     const uint64_t minor_mask = (1ull << minor_bits) - 1;
+    const uint32_t lane = lane_id();
     for (size_t t = blockIdx.x; t < tiles; t += gridDim.x) {
-        const uint64_t o = offsets[t];
-        const uint64_t c = (t + 1 < tiles ? offsets[t + 1] : *total) - o;
+        const uint64_t c = counts[t];
         if (c == 0) continue;  // block-uniform
+        // the tile's offset: its group's, plus the earlier tiles of the group
+        if (wave_id() == 0) {
+            const size_t g0 = t / kSegGroup * kSegGroup;
+            const uint64_t before = g0 + lane < t ? counts[g0 + lane] : 0;
+            const uint64_t o = group_offsets[t / kSegGroup] + wave_sum(before);
+            if (lane == 0) offset_slot = o;
+        }
+        __syncthreads();
+        const uint64_t o = offset_slot;
         const size_t b0 = bounds[t], b1 = bounds[t + 1];
         const bool big = b1 - b0 > static_cast<size_t>(kSegCap);
         const uint64_t* src = keys + b0;
@@ -571,13 +597,19 @@ __global__ __launch_bounds__(kBlock) void seg_write_kernel(const uint64_t* keys,
             src = side_sorted + side0;
         }
         for (uint64_t c0 = 0; c0 < c; c0 += static_cast<uint64_t>(kBlock) * kSegWriteItems) {
+            // a row's predecessor is the previous lane's row; lane 0 loads its own
             uint64_t v[kSegWriteItems], pv[kSegWriteItems];
 #pragma unroll
             for (int q = 0; q < kSegWriteItems; ++q) {
                 const uint64_t i = c0 + threadIdx.x + static_cast<uint64_t>(q) * kBlock;
                 const uint64_t ic = i < c ? i : c - 1;
                 v[q] = src[ic];
-                pv[q] = src[ic > 0 ? ic - 1 : 0];
+                pv[q] = lane == 0 ? src[ic > 0 ? ic - 1 : 0] : 0;
+            }
+#pragma unroll
+            for (int q = 0; q < kSegWriteItems; ++q) {
+                const uint64_t up = __shfl_up(v[q], 1, kWave);
+                if (lane > 0) pv[q] = up;
             }
 #pragma unroll
             for (int q = 0; q < kSegWriteItems; ++q) {
@@ -595,6 +627,7 @@ __global__ __launch_bounds__(kBlock) void seg_write_kernel(const uint64_t* keys,
                     xidx_out[k >> 32] = static_cast<uint32_t>(o + i);
             }
         }
+        __syncthreads();  // offset_slot is rewritten for the block's next tile
     }
 }
 
@@ -614,7 +647,9 @@ hipError_t segmented_distinct_prepare(const uint64_t* keys, size_t rows, uint32_
     SegHeader* h = reinterpret_cast<SegHeader*>(base);
     size_t* bounds = reinterpret_cast<size_t*>(base + l.bounds_off);
     const unsigned small_grid = static_cast<unsigned>(l.tiles / kBlock + 1 < 4096 ? l.tiles / kBlock + 1 : 4096);
-    seg_partition_kernel<<<small_grid, kBlock, 0, stream>>>(keys, rows, minor_bits, bounds, l.tiles, h, tile_rows);
+    uint64_t* groups = reinterpret_cast<uint64_t*>(base + l.groups_off);
+    seg_partition_kernel<<<small_grid, kBlock, 0, stream>>>(keys, rows, minor_bits, bounds, l.tiles, h, tile_rows,
+                                                            groups, l.groups);
     seg_gather_kernel<<<static_cast<unsigned>(l.tiles < 1024 ? l.tiles : 1024), kBlock, 0, stream>>>(
         keys, bounds, l.tiles, side, h, found);
     hipError_t st = hipGetLastError();
@@ -636,6 +671,7 @@ hipError_t segmented_distinct_finish(uint64_t* keys, size_t rows, uint32_t minor
     char* base = static_cast<char*>(meta);
     SegHeader* h = reinterpret_cast<SegHeader*>(base);
     uint64_t* status = reinterpret_cast<uint64_t*>(base + l.status_off);
+    uint64_t* groups = reinterpret_cast<uint64_t*>(base + l.groups_off);
     size_t* bounds = reinterpret_cast<size_t*>(base + l.bounds_off);
     hipError_t st = hipSuccess;
     uint64_t* side_unique_out = side_a;
@@ -654,14 +690,14 @@ hipError_t segmented_distinct_finish(uint64_t* keys, size_t rows, uint32_t minor
     // rank bits: at most log2(kSegCap) = 12 (a run per key)
     if (minor_bits + 12 <= 32)
         seg_tile_kernel<uint32_t><<<static_cast<unsigned>(l.tiles), kSegBlock, 0, stream>>>(
-            keys, rows, minor_bits, bounds, side_unique_out, status, h, found, tile_rows);
+            keys, rows, minor_bits, bounds, side_unique_out, status, h, found, tile_rows, groups);
     else
         seg_tile_kernel<uint64_t><<<static_cast<unsigned>(l.tiles), kSegBlock, 0, stream>>>(
-            keys, rows, minor_bits, bounds, side_unique_out, status, h, found, tile_rows);
-    seg_scan_kernel<<<1, kSegScanThreads, 0, stream>>>(status, l.tiles, unique_rows);
+            keys, rows, minor_bits, bounds, side_unique_out, status, h, found, tile_rows, groups);
+    seg_scan_kernel<<<1, kSegScanThreads, 0, stream>>>(groups, l.groups, unique_rows);
     const unsigned write_grid = static_cast<unsigned>(l.tiles < 65536 ? l.tiles : 65536);
-    seg_write_kernel<<<write_grid, kBlock, 0, stream>>>(keys, bounds, side_unique_out, status, unique_rows, h,
-                                                        l.tiles, minor_bits, out, xidx_out);
+    seg_write_kernel<<<write_grid, kBlock, 0, stream>>>(keys, bounds, side_unique_out, status, groups, h, l.tiles,
+                                                        minor_bits, out, xidx_out);
     timer_stop(timer, stream);
     return hipGetLastError();
 }
