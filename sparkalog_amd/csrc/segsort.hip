@@ -121,31 +121,54 @@ __device__ __forceinline__ bool dropped(const uint64_t* found, size_t i) {
 
 // Copy the keys of tiles larger than kSegCap to the side buffer (any order:
 // they are sorted globally afterwards), except keys whose found bit is set.
+// A block takes a chunk of kGatherItems keys a thread at a time: per item
+// the wave's kept keys are compacted by ballot, the waves' totals are
+// scanned once per chunk, and one atomic on the side count places the chunk
+// (it had scanned the whole block every 256 keys, twice, which made the
+// gather of the batched fixpoint's long runs cost as much as their sort).
+constexpr int kGatherItems = 16;
 __global__ __launch_bounds__(kBlock) void seg_gather_kernel(const uint64_t* keys, const size_t* bounds, size_t tiles,
                                                             uint64_t* side, SegHeader* h, const uint64_t* found) {
     __shared__ uint64_t slot;
-    __shared__ uint32_t scan_lds[kWavesPerBlock];
+    __shared__ uint32_t wave_tot[kWavesPerBlock];
+    const uint32_t lane = lane_id(), wave = wave_id();
     for (size_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         const size_t b0 = bounds[t], b1 = bounds[t + 1];
         if (b1 - b0 <= static_cast<size_t>(kSegCap)) continue;  // block-uniform
-        uint32_t mine = 0;
-        for (size_t i = b0 + threadIdx.x; i < b1; i += kBlock) mine += dropped(found, i) ? 0u : 1u;
-        uint32_t total = 0;
-        (void)block_exclusive_sum<uint32_t>(mine, &total, scan_lds);
-        if (threadIdx.x == 0) slot = atomicAdd(reinterpret_cast<unsigned long long*>(&h->side_rows),
-                                               static_cast<unsigned long long>(total));
-        __syncthreads();
-        const uint64_t base = slot;
-        uint64_t written = 0;
-        for (size_t c = b0; c < b1; c += kBlock) {
-            const size_t i = c + threadIdx.x;
-            const uint32_t keep = i < b1 && !dropped(found, i) ? 1u : 0u;
-            uint32_t chunk = 0;
-            const uint32_t at = block_exclusive_sum<uint32_t>(keep, &chunk, scan_lds);
-            if (keep) side[base + written + at] = keys[i];
-            written += chunk;
+        for (size_t c = b0; c < b1; c += static_cast<size_t>(kBlock) * kGatherItems) {
+            uint64_t k[kGatherItems], m[kGatherItems];
+            uint32_t mine = 0;  // the wave's kept keys (every lane holds the same)
+#pragma unroll
+            for (int q = 0; q < kGatherItems; ++q) {
+                const size_t i = c + static_cast<size_t>(q) * kBlock + threadIdx.x;
+                k[q] = keys[i < b1 ? i : b1 - 1];
+            }
+#pragma unroll
+            for (int q = 0; q < kGatherItems; ++q) {
+                const size_t i = c + static_cast<size_t>(q) * kBlock + threadIdx.x;
+                m[q] = ballot(i < b1 && !dropped(found, i));
+                mine += static_cast<uint32_t>(__popcll(m[q]));
+            }
+            if (lane == 0) wave_tot[wave] = mine;
+            __syncthreads();
+            uint32_t before = 0, total = 0;
+#pragma unroll
+            for (int w = 0; w < kWavesPerBlock; ++w) {
+                if (w < static_cast<int>(wave)) before += wave_tot[w];
+                total += wave_tot[w];
+            }
+            if (threadIdx.x == 0)
+                slot = atomicAdd(reinterpret_cast<unsigned long long*>(&h->side_rows),
+                                 static_cast<unsigned long long>(total));
+            __syncthreads();
+            uint64_t pos = slot + before;
+#pragma unroll
+            for (int q = 0; q < kGatherItems; ++q) {
+                if ((m[q] >> lane) & 1ull) side[pos + lanes_below(m[q])] = k[q];
+                pos += static_cast<uint32_t>(__popcll(m[q]));
+            }
+            __syncthreads();  // wave_tot and slot are rewritten by the next chunk
         }
-        __syncthreads();
     }
 }
 
