@@ -92,3 +92,21 @@ def test_plan_waves_respects_budget_and_id_space():
     big = np.full(20000, 1 << 18, np.int64)
     waves = batch.plan_waves(big, np.ones(20000, np.int64), budget_pairs=1e30)
     assert max(len(w) for w in waves) == 1 << 14
+
+
+def test_plan_waves_in_diameter_order_covers_every_instance_once():
+    from sparkalog_amd import batch
+
+    rng = np.random.default_rng(5)
+    nodes = np.exp(rng.uniform(np.log(16), np.log(16384), 500)).astype(np.int64)
+    edges = (nodes * rng.uniform(1.2, 3.0, 500)).astype(np.int64)
+    order = batch.diameter_order(nodes, edges)
+    assert sorted(order) == list(range(500))
+    est = [batch.estimated_iterations(int(n), int(e)) for n, e in zip(nodes, edges)]
+    assert all(est[a] >= est[b] for a, b in zip(order, order[1:]))
+    budget = 5e8
+    waves = batch.plan_waves(nodes, edges, budget_pairs=budget, order=order)
+    assert sorted(sum(waves, [])) == list(range(500))
+    for w in waves:
+        total = sum(batch.estimate_closure(int(nodes[i]), int(edges[i])) for i in w)
+        assert len(w) == 1 or total <= budget
