@@ -92,7 +92,8 @@ def parse_args():
                    help="batched/served: replay single-rule iterations from captured HIP graphs (1) or not (0); "
                         "-1 = the tuning file's choice")
     p.add_argument("--sizes", default="24,26,28", help="ops mode: log2 row counts")
-    p.add_argument("--ops", default="distinct,union,anti_join,join", help="ops mode: operators")
+    p.add_argument("--ops", default="distinct,union,anti_join,join",
+                   help="ops mode: operators (distinct, union, anti_join, join, filter)")
     p.add_argument("--samples", type=int, default=5, help="ops mode: timed launches per case")
     p.add_argument("--cpu-max-log2", type=int, default=24,
                    help="ops mode: largest case checked against (and timed with) the oracle on this host; "
@@ -136,6 +137,9 @@ def oracle_operator(name, host, threads):
         return oracle.union(host["left"], host["right"], threads=threads)
     if name == "anti_join":
         return oracle.anti_join(host["left"], host["right"], threads=threads)
+    if name == "filter":
+        from sparkalog_amd.opbench import Filter
+        return oracle.filter_rows(host["values"], Filter.OP_GE, Filter.OPERAND, threads=threads)
     return oracle.join(host["left"], host["right"], 1, 0, ((0, 0), (1, 1)), threads=threads)
 
 
@@ -160,14 +164,21 @@ def run_ops(args) -> None:
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     for name in args.ops.split(","):
         for k in [int(x) for x in args.sizes.split(",")]:
-            for mode_name, mode in modes.items():
+            # the filter's input is i % 97 (no key distribution); it is checked
+            # against the oracle at every size
+            for mode_name, mode in (modes.items() if name != "filter" else [("i%97", workload.KEY_UNIFORM)]):
                 host = opbench.host_inputs(name, 1 << k, mode)
                 op = opbench.make(name, 1 << k, mode, host=host)
                 op.launch(stream)
                 torch.cuda.synchronize()
                 got = op.result()
                 check = {}
-                if k <= args.cpu_max_log2:
+                if name == "filter":
+                    exp = oracle_operator(name, host, threads)
+                    check = {"checked_against": "oracle"}
+                    ok = np.array_equal(got, exp)
+                    del exp
+                elif k <= args.cpu_max_log2:
                     t0 = time.perf_counter()
                     exp = oracle_operator(name, host, threads)
                     check = {"checked_against": "oracle", "cpu_parallel_ms": round(1000 * (time.perf_counter() - t0), 3),

@@ -15,13 +15,12 @@
  *   - a null pointer with a non-zero length is hipErrorInvalidValue;
  *   - counts are written on the device (u32 for filter and anti-join, u64 for
  *     distinct, union and the join total);
- *   - the join count and the filter find their output offsets in a single
- *     pass, waiting on earlier tiles with a bounded wait; if such a wait
- *     ever gives up (a scheduling failure of the device, never seen in
- *     practice), the call's count is written as all ones (UINT32_MAX /
- *     UINT64_MAX) after its kernels, so the caller can tell it from a
- *     result. The distinct, union and anti-join count first and write
- *     second, and never wait on another tile.
+ *   - the join count finds its output offsets in a single pass, waiting
+ *     on earlier tiles with a bounded wait; if such a wait ever gives up (a
+ *     scheduling failure of the device, never seen in practice), the total
+ *     is written as all ones (UINT64_MAX) after its kernels, so the caller
+ *     can tell it from a result. The filter, distinct, union and anti-join
+ *     count first and write second, and never wait on another tile.
  * Scratch arrays the reference needed for its CUB pipeline (filter flags,
  * anti-join flags/selection, union packed copies, distinct packed/scratch)
  * keep their parameters; the kernels here use them only where noted.

@@ -15,12 +15,13 @@
 //  * join      = single-pass count + exclusive scan, then an output-balanced
 //    emit (every block writes an equal slice of the output, whatever the key
 //    skew).
-//  * filter    = ballot compaction of matching row ids.
+//  * filter    = ballot compaction of matching row ids, in two passes.
 // Where tiles of a compaction would wait on each other's counts (decoupled
-// look-back, whose status polls cross XCDs on this chip), the union, the
-// anti-join and the unique compaction count first and write second: a few
-// bytes more read, no tile ever waits. The sort's digit passes and the join
-// count and filter keep a single pass with look-back.
+// look-back, whose status polls cross XCDs on this chip), the filter, the
+// union, the anti-join and the unique compaction count first and write
+// second: a few bytes more read, no tile ever waits; the sort's digit passes
+// count per chunk of tiles and scan the counts. Only the join count keeps a
+// single pass with look-back.
 #include "ops.hpp"
 #include "prims.hpp"
 
@@ -1332,52 +1333,63 @@ __device__ __forceinline__ bool compare_u32(uint32_t value, uint32_t op, uint32_
     }
 }
 
-__global__ __launch_bounds__(kBlock) void filter_kernel(const uint32_t* input, size_t n, uint32_t op,
-                                                        uint32_t operand, uint32_t* out_rows, uint32_t* count,
-                                                        uint64_t* status, Header* header, uint32_t* error) {
-    __shared__ uint32_t stage[kCompactTile];
-    __shared__ uint32_t wave_totals[kWavesPerBlock][kCompactItems];
+// Filter, both passes of a two-pass compaction: tiles of kCompactTile rows;
+// wave w owns rows [w*1024, w*1024+1024) of a tile, item i of lane j is row
+// w*1024 + i*64 + j, and ballots rank the matches in row order. The count
+// pass writes the tile's match count and adds it to its group's sum; the
+// write pass writes the matching row ids at the tile's offset.
+template <bool kWrite>
+__global__ __launch_bounds__(kBlock) void filter_tile_kernel(const uint32_t* input, size_t n, uint32_t op,
+                                                             uint32_t operand, uint32_t* out_rows, uint32_t* counts,
+                                                             uint64_t* group_sums) {
+    __shared__ uint32_t wave_tot[kWavesPerBlock];
     __shared__ uint64_t offset_slot;
-    __shared__ uint32_t tile_slot;
-    const uint32_t tile = next_tile(&header->counters[0], &tile_slot);
+    const uint32_t tile = blockIdx.x;
     const size_t base = static_cast<size_t>(tile) * kCompactTile;
+    if (base >= n) {
+        if (!kWrite && threadIdx.x == 0) counts[tile] = 0;
+        return;
+    }
     const uint32_t lane = lane_id(), wave = wave_id();
-    // wave w owns rows [w*1024, w*1024+1024) of the tile; item i of lane j is
-    // row w*1024 + i*64 + j. Ballots rank matches in row order per item.
+    const size_t w0 = base + static_cast<size_t>(wave) * (kCompactItems * kWave);
+    // every load is issued before any is used (rows clamped to the input)
+    uint32_t v[kCompactItems];
+#pragma unroll
+    for (int i = 0; i < kCompactItems; ++i) {
+        const size_t row = w0 + i * kWave + lane;
+        v[i] = input[row < n ? row : n - 1];
+    }
+    uint64_t offset = 0;
+    if (kWrite && wave == 0) offset = group_tile_offset(group_sums, counts, tile);
     uint64_t masks[kCompactItems];
-    uint32_t wave_count = 0;
+    uint32_t mine = 0;
 #pragma unroll
     for (int i = 0; i < kCompactItems; ++i) {
-        const size_t row = base + wave * (kCompactItems * kWave) + i * kWave + lane;
-        const bool hit = row < n && compare_u32(input[row], op, operand);
-        masks[i] = ballot(hit);
-        if (lane == 0) wave_totals[wave][i] = static_cast<uint32_t>(__popcll(masks[i]));
-        wave_count += static_cast<uint32_t>(__popcll(masks[i]));
+        const size_t row = w0 + i * kWave + lane;
+        masks[i] = ballot(row < n && compare_u32(v[i], op, operand));
+        mine += static_cast<uint32_t>(__popcll(masks[i]));
     }
+    if (lane == 0) wave_tot[wave] = mine;
+    if (kWrite && threadIdx.x == 0) offset_slot = offset;
     __syncthreads();
-    // tile-local start of each wave
-    uint32_t wave_start = 0, tile_total = 0;
-    for (int w = 0; w < kWavesPerBlock; ++w) {
-        uint32_t s = 0;
-        for (int i = 0; i < kCompactItems; ++i) s += wave_totals[w][i];
-        if (w < static_cast<int>(wave)) wave_start += s;
-        tile_total += s;
+    if (!kWrite) {
+        if (threadIdx.x == 0) {
+            uint32_t c = 0;
+            for (int w = 0; w < kWavesPerBlock; ++w) c += wave_tot[w];
+            counts[tile] = c;
+            atomicAdd(reinterpret_cast<unsigned long long*>(&group_sums[tile / kTileGroup]),
+                      static_cast<unsigned long long>(c));
+        }
+        return;
     }
-    if (wave == 0) {
-        const uint64_t excl = wave_lookback(status, tile, tile_total, 0, error);
-        if (lane == 0) offset_slot = excl;
-    }
-    uint32_t pos = wave_start;
+    uint64_t pos = offset_slot;
+    for (int w = 0; w < static_cast<int>(wave); ++w) pos += wave_tot[w];
 #pragma unroll
     for (int i = 0; i < kCompactItems; ++i) {
-        const size_t row = base + wave * (kCompactItems * kWave) + i * kWave + lane;
-        if ((masks[i] >> lane) & 1ull) stage[pos + lanes_below(masks[i])] = static_cast<uint32_t>(row);
+        const size_t row = w0 + i * kWave + lane;
+        if ((masks[i] >> lane) & 1ull) out_rows[pos + lanes_below(masks[i])] = static_cast<uint32_t>(row);
         pos += static_cast<uint32_t>(__popcll(masks[i]));
     }
-    __syncthreads();
-    const uint64_t offset = offset_slot;
-    for (uint32_t i = threadIdx.x; i < tile_total; i += kBlock) out_rows[offset + i] = stage[i];
-    if (tile == tiles_of(n, kCompactTile) - 1 && threadIdx.x == 0) *count = static_cast<uint32_t>(offset + tile_total);
 }
 
 // ------------------------------------------------------------------------
@@ -1701,7 +1713,14 @@ hipError_t join_emit(const uint32_t* left_keys, size_t left_rows, const uint32_t
     return hipGetLastError();
 }
 
-size_t filter_temp_bytes(size_t rows) { return compact_temp(tiles_of(rows, kCompactTile)); }
+// Scratch of the filter: header, per-tile match counts and tile-group sums.
+size_t filter_counts_off() { return kTempHeader; }
+size_t filter_groups_off(size_t tiles) { return kTempHeader + align_up(tiles * sizeof(uint32_t), 256); }
+
+size_t filter_temp_bytes(size_t rows) {
+    const size_t tiles = tiles_of(rows, kCompactTile);
+    return filter_groups_off(tiles) + align_up((tiles_of(tiles, kTileGroup) + 1) * sizeof(uint64_t), 256);
+}
 
 hipError_t filter(const uint32_t* input, size_t rows, uint32_t op, uint32_t operand, uint32_t* out_rows,
                   uint32_t* count, void* temp, size_t temp_bytes, hipStream_t stream) {
@@ -1710,16 +1729,18 @@ hipError_t filter(const uint32_t* input, size_t rows, uint32_t op, uint32_t oper
         return hipGetLastError();
     }
     const size_t tiles = tiles_of(rows, kCompactTile);
-    if (temp_bytes < compact_temp(tiles)) return hipErrorInvalidValue;
-    hipError_t st = prepare_compact(temp, tiles, stream);
+    if (temp_bytes < filter_temp_bytes(rows) || tiles > 0xFFFFFFFFull) return hipErrorInvalidValue;
+    char* base = static_cast<char*>(temp);
+    uint32_t* counts = reinterpret_cast<uint32_t*>(base + filter_counts_off());
+    uint64_t* groups = reinterpret_cast<uint64_t*>(base + filter_groups_off(tiles));
+    const size_t n_groups = tiles_of(tiles, kTileGroup);
+    hipError_t st = clear_u64(groups, n_groups, stream);
     if (st != hipSuccess) return st;
-    uint32_t* error = error_word(static_cast<Header*>(temp));
-    filter_kernel<<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(
-        input, rows, op, operand, out_rows, count,
-        reinterpret_cast<uint64_t*>(static_cast<char*>(temp) + kTempHeader), static_cast<Header*>(temp), error);
-    st = hipGetLastError();
-    if (st != hipSuccess) return st;
-    return report_error(error, count, nullptr, stream);
+    const unsigned grid = static_cast<unsigned>(tiles);
+    filter_tile_kernel<false><<<grid, kBlock, 0, stream>>>(input, rows, op, operand, out_rows, counts, groups);
+    group_scan_kernel<<<1, 1024, 0, stream>>>(groups, n_groups, nullptr, count);
+    filter_tile_kernel<true><<<grid, kBlock, 0, stream>>>(input, rows, op, operand, out_rows, counts, groups);
+    return hipGetLastError();
 }
 
 hipError_t add_one_i32(int32_t* data, size_t len, hipStream_t stream) {

@@ -29,7 +29,6 @@ constexpr int kSegCap = 4096;                 // largest tile sorted in LDS
 constexpr int kSegBlock = 512;                // threads of a tile block (8 waves)
 constexpr int kSegWaves = kSegBlock / kWave;
 constexpr int kSegItems = kSegCap / kSegBlock;  // 8 keys per thread
-constexpr int kSegWaveRows = kSegItems * kWave;
 constexpr int kSegDigitBits = 9;              // digit of a tile pass
 constexpr int kSegRadix = 1 << kSegDigitBits;
 constexpr int kSegFirstBits = 12;             // digit of the first (unstable) pass
