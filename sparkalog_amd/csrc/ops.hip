@@ -1246,6 +1246,8 @@ __global__ __launch_bounds__(kBlock) void join_emit_kernel(const uint32_t* left_
                                                            uint32_t* out1) {
     __shared__ uint64_t row_offset[kEmitStage + 1];
     __shared__ uint32_t row_match[kEmitStage];
+    __shared__ uint16_t row_list[kEmitStage];  // the tile's rows with outputs
+    __shared__ uint32_t list_len;
     __shared__ size_t bounds[2];
     __shared__ uint64_t total_slot;
     __shared__ uint32_t key_sample[kKeySample];
@@ -1272,11 +1274,29 @@ __global__ __launch_bounds__(kBlock) void join_emit_kernel(const uint32_t* left_
         const size_t span = l1 - l0;
         const bool staged = span <= static_cast<size_t>(kEmitStage);
         if (staged) {
-            for (uint32_t i = threadIdx.x; i < span; i += kBlock) {
-                const size_t row = l0 + i;
-                row_offset[i] = offsets[row];
+            // only rows with outputs need their key searched (about a third of
+            // uniform probe keys have none): the tile's offsets go to LDS, the
+            // rows whose next offset is larger are listed (any order), and the
+            // block's threads share the list's searches
+            if (threadIdx.x == 0) list_len = 0;
+            for (uint32_t i = threadIdx.x; i < span; i += kBlock) row_offset[i] = offsets[l0 + i];
+            if (threadIdx.x == 0) row_offset[span] = l1 < nl ? offsets[l1] : total;
+            __syncthreads();
+            for (uint32_t c = 0; c < span; c += kBlock) {  // block-uniform trips
+                const uint32_t i = c + threadIdx.x;
+                const bool has = i < span && row_offset[i + 1] > row_offset[i];
+                const uint64_t m = ballot(has);
+                uint32_t at = 0;
+                if (lane_id() == 0 && m) at = atomicAdd(&list_len, static_cast<uint32_t>(__popcll(m)));
+                at = __shfl(at, 0, kWave);
+                if (has) row_list[at + lanes_below(m)] = static_cast<uint16_t>(i);
+            }
+            __syncthreads();
+            const uint32_t listed = list_len;
+            for (uint32_t j = threadIdx.x; j < listed; j += kBlock) {
+                const uint32_t i = row_list[j];
                 const int64_t idx =
-                    k > 0 ? find_index_key_sampled(index_keys, k, left_keys[row], key_sample, stride) : -1;
+                    k > 0 ? find_index_key_sampled(index_keys, k, left_keys[l0 + i], key_sample, stride) : -1;
                 row_match[i] = idx >= 0 ? index_starts[idx] : 0;
             }
         }
