@@ -81,10 +81,12 @@ hipError_t unique_sorted(const uint64_t* keys, const uint64_t* rows_dev, size_t 
 // Distinct of compressed keys (major << minor_bits | minor) whose major part
 // is non-decreasing along the input; writes sorted unique expanded pairs
 // (major << 32 | minor). Two phases: prepare partitions the input into tiles
-// aligned to runs, copies runs longer than a tile to `side` and returns their
-// row count on the host; finish sorts those (side/side_b, sort_temp sized by
-// distinct_temp_bytes(side_rows); unused when side_rows is 0) and every tile,
-// overwriting `keys` (each tile's unique rows are staged over its input).
+// aligned to runs, sorts the tiles of 4,097..16,384 keys (one block each),
+// copies the keys of longer tiles to `side` and returns their row count on
+// the host; finish sorts those (side/side_b, sort_temp sized by
+// distinct_temp_bytes(side_rows); unused when side_rows is 0) and every
+// other tile, overwriting `keys` (each tile's unique rows are staged over
+// its input).
 // `meta` (segmented_distinct_meta_bytes(rows)) must persist between phases.
 // With `found` (probe_grouped's words over the same keys), keys whose found
 // bit is set are dropped first: the output is then (keys \ right inputs),
@@ -95,7 +97,7 @@ hipError_t unique_sorted(const uint64_t* keys, const uint64_t* rows_dev, size_t 
 // boundaries sit every tile_rows keys before they move back to run starts.
 constexpr uint32_t kSegTileDefault = 3072;
 size_t segmented_distinct_meta_bytes(size_t rows, uint32_t tile_rows = kSegTileDefault);
-hipError_t segmented_distinct_prepare(const uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t* side,
+hipError_t segmented_distinct_prepare(uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t* side,
                                       void* meta, size_t meta_bytes, uint64_t* side_rows, hipStream_t stream,
                                       const uint64_t* found = nullptr, uint32_t tile_rows = kSegTileDefault);
 hipError_t segmented_distinct_finish(uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t side_varying,

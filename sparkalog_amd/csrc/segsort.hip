@@ -8,9 +8,11 @@
 // write per key:
 //   1. partition: tile boundaries are moved back to the start of the run
 //      that contains them (one thread per boundary), so no run crosses a tile;
-//   2. runs longer than a tile go, gathered, through the global onesweep sort
-//      (rare: a run is one source's candidates in one iteration);
-//   3. every tile sorts its keys in LDS by (local run rank, minor) with LSD
+//   2. tiles of 4,097..16,384 keys (a long run: one source's candidates in
+//      one iteration) are sorted in LDS by a persistent kernel, one
+//      1,024-thread block a tile; longer tiles go, gathered, through the
+//      global radix sort;
+//   3. every other tile sorts its keys in LDS by (local run rank, minor) with LSD
 //      passes over the bit range where its keys differ (a 12-bit digit, then
 //      stable 9-bit digits: up to 30 varying bits take three passes), drops
 //      duplicates, and writes the expanded pairs (major << 32 | minor) back
@@ -34,6 +36,30 @@ constexpr int kSegRadix = 1 << kSegDigitBits;
 constexpr int kSegFirstBits = 12;             // digit of the first (unstable) pass
 constexpr int kSegFirstRadix = 1 << kSegFirstBits;
 
+// Tiles longer than kSegCap keys and at most kMedCap are sorted in LDS as
+// well, by a persistent kernel of kMedBlock-thread blocks (seg_medium_kernel,
+// one tile a block at a time); longer tiles go through the global sort.
+// Every tile, whatever its length, holds at most as many runs as the nominal
+// tile has keys (the run containing its nominal start, then runs that start
+// before the next nominal boundary), so a run rank fits kSegRankBits bits.
+// A medium tile's sort key is (rank, the minor's bits in the range where the
+// tile's minors differ): 32 bits when that range is at most 20 bits wide;
+// a wider tile goes to the global sort after all.
+constexpr int kMedCap = 16384;
+constexpr int kMedBlock = 1024;
+constexpr int kMedWaves = kMedBlock / kWave;    // 16
+constexpr int kMedItems = kMedCap / kMedBlock;  // 16 keys per thread
+constexpr int kMedDigitBits = 10;               // one digit per thread
+constexpr int kMedRadix = 1 << kMedDigitBits;
+constexpr unsigned kMedGrid = 256;              // one block a CU (LDS bound)
+constexpr uint32_t kSegRankBits = 12;
+static_assert(kSegCap <= (1 << kSegRankBits), "run ranks fit");
+static_assert(kMedRadix == kMedBlock, "one digit per thread");
+
+__host__ __device__ inline bool seg_medium(size_t size) {
+    return size > static_cast<size_t>(kSegCap) && size <= static_cast<size_t>(kMedCap);
+}
+
 __device__ __forceinline__ void store_pair(const PairOutput& out, size_t i, uint64_t k) {
     if (out.packed) out.packed[i] = k;
     if (out.packed_copy) out.packed_copy[i] = k;
@@ -50,12 +76,13 @@ __host__ __device__ inline size_t seg_align(size_t v) { return (v + 255) / 256 *
 struct SegHeader {
     uint32_t error;
     uint32_t counter;
-    uint64_t side_rows;     // keys of oversized runs gathered for the global sort
+    uint64_t side_rows;     // keys gathered for the global sort
     uint64_t side_unique;   // their unique count
+    uint64_t medium_tiles;  // tiles listed for seg_medium_kernel
 };
 
 struct SegLayout {
-    size_t status_off, groups_off, bounds_off, total, tiles, groups;
+    size_t status_off, groups_off, bounds_off, list_off, side_off, total, tiles, groups;
 };
 
 // Tiles of a group: a tile adds its unique count to its group's sum, a
@@ -64,7 +91,8 @@ struct SegLayout {
 constexpr uint32_t kSegGroup = 64;
 
 // Metadata of one segmented distinct: header, per-tile unique counts, group
-// sums (then group offsets) and tile bounds.
+// sums (then group offsets), tile bounds, the list of medium tiles and a
+// flag per tile longer than kSegCap: 1 when its keys went to the side sort.
 SegLayout seg_layout(size_t rows, uint32_t tile) {
     SegLayout l{};
     l.tiles = seg_tiles(rows, tile);
@@ -76,6 +104,10 @@ SegLayout seg_layout(size_t rows, uint32_t tile) {
     off += seg_align((l.groups ? l.groups : 1) * sizeof(uint64_t));
     l.bounds_off = off;
     off += seg_align((l.tiles + 1) * sizeof(size_t));
+    l.list_off = off;
+    off += seg_align((l.tiles ? l.tiles : 1) * sizeof(uint32_t));
+    l.side_off = off;
+    off += seg_align((l.tiles ? l.tiles : 1) * sizeof(uint8_t));
     l.total = off;
     return l;
 }
@@ -94,6 +126,7 @@ __global__ void seg_partition_kernel(const uint64_t* keys, size_t rows, uint32_t
         h->counter = 0;
         h->side_rows = 0;
         h->side_unique = 0;
+        h->medium_tiles = 0;
     }
     for (size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; t <= tiles; t += stride) {
         const size_t p = t * tile;
@@ -118,8 +151,9 @@ __device__ __forceinline__ bool dropped(const uint64_t* found, size_t i) {
     return found != nullptr && ((found[i >> 6] >> (i & 63)) & 1ull);
 }
 
-// Copy the keys of tiles larger than kSegCap to the side buffer (any order:
-// they are sorted globally afterwards), except keys whose found bit is set.
+// Copy the keys of tiles longer than kMedCap to the side buffer (any order:
+// they are sorted globally afterwards), except keys whose found bit is set,
+// and list the medium tiles (any order).
 // A block takes a chunk of kGatherItems keys a thread at a time: per item
 // the wave's kept keys are compacted by ballot, the waves' totals are
 // scanned once per chunk, and one atomic on the side count places the chunk
@@ -127,13 +161,23 @@ __device__ __forceinline__ bool dropped(const uint64_t* found, size_t i) {
 // gather of the batched fixpoint's long runs cost as much as their sort).
 constexpr int kGatherItems = 16;
 __global__ __launch_bounds__(kBlock) void seg_gather_kernel(const uint64_t* keys, const size_t* bounds, size_t tiles,
-                                                            uint64_t* side, SegHeader* h, const uint64_t* found) {
+                                                            uint64_t* side, SegHeader* h, const uint64_t* found,
+                                                            uint32_t* medium_list, uint8_t* side_tile) {
     __shared__ uint64_t slot;
     __shared__ uint32_t wave_tot[kWavesPerBlock];
     const uint32_t lane = lane_id(), wave = wave_id();
     for (size_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         const size_t b0 = bounds[t], b1 = bounds[t + 1];
         if (b1 - b0 <= static_cast<size_t>(kSegCap)) continue;  // block-uniform
+        if (seg_medium(b1 - b0)) {
+            if (threadIdx.x == 0) {
+                const unsigned long long at =
+                    atomicAdd(reinterpret_cast<unsigned long long*>(&h->medium_tiles), 1ull);
+                medium_list[at] = static_cast<uint32_t>(t);
+            }
+            continue;
+        }
+        if (threadIdx.x == 0) side_tile[t] = 1;
         for (size_t c = b0; c < b1; c += static_cast<size_t>(kBlock) * kGatherItems) {
             uint64_t k[kGatherItems], m[kGatherItems];
             uint32_t mine = 0;  // the wave's kept keys (every lane holds the same)
@@ -207,7 +251,8 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) 
     uint64_t* keys, size_t rows, uint32_t minor_bits,
                                                           const size_t* bounds, const uint64_t* side_sorted,
                                                           uint64_t* counts, SegHeader* h, const uint64_t* found,
-                                                          uint32_t tile_rows, uint64_t* group_sums) {
+                                                          uint32_t tile_rows, uint64_t* group_sums,
+                                                          const uint8_t* side_tile) {
     // sort keys and run majors; before ranking, the same bytes hold the
     // tile's surviving raw keys (u64) when some keys are dropped
     __shared__ uint64_t seg_buf[(kSegCap * (sizeof(K) + sizeof(uint32_t)) + 7) / 8];
@@ -236,6 +281,7 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) 
     const size_t size = b1 - b0;
     const uint64_t minor_mask = (1ull << minor_bits) - 1;
     const bool big = size > static_cast<size_t>(kSegCap);
+    if (big && !side_tile[tile]) return;  // seg_medium_kernel sorted it
     // each wave owns a contiguous share of the tile: items = ceil(size / 512)
     int items = static_cast<int>((size + kSegBlock - 1) / kSegBlock);
     uint32_t wave_len = static_cast<uint32_t>(items) * kWave;
@@ -573,6 +619,281 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) 
     }
 }
 
+// Medium tiles (ids listed by seg_gather_kernel), one at a time per block
+// of a persistent grid: drops the keys whose found bit is set, sorts the
+// rest in LDS by (local run rank, minor) with stable LSD passes on 10-bit
+// digits over the bits where they differ, drops duplicates, and writes the
+// expanded pairs back over the tile's input slice with its unique count, as
+// seg_tile_kernel does for tiles of at most kSegCap keys. A tile whose
+// minors differ over more than 32 - kSegRankBits bits copies its kept keys
+// to the side buffer instead and is flagged in side_tile. Wave w owns
+// positions [w * 1024, w * 1024 + 1024) of a tile; item i of lane j is
+// position w * 1024 + i * 64 + j.
+__global__ __launch_bounds__(kMedBlock) void seg_medium_kernel(uint64_t* keys, uint32_t minor_bits,
+                                                              const size_t* bounds, const uint32_t* list,
+                                                              SegHeader* h, uint64_t* side, uint8_t* side_tile,
+                                                              uint64_t* counts, const uint64_t* found,
+                                                              uint64_t* group_sums) {
+    __shared__ uint32_t sk[kMedCap];                    // sort keys
+    __shared__ uint32_t majors[1 << kSegRankBits];      // major of each local run
+    __shared__ uint16_t wcount[kMedWaves][kMedRadix];   // per-wave digit counts, then starts
+    __shared__ uint32_t wtot[kMedWaves], wkeep[kMedWaves], scan_lds[kMedWaves];
+    __shared__ uint32_t minor_or[kMedWaves], minor_and[kMedWaves], red_or[kMedWaves], red_and[kMedWaves];
+    __shared__ uint64_t side_slot;
+    const uint64_t n_tiles = h->medium_tiles;
+    const uint32_t minor_mask = static_cast<uint32_t>((1ull << minor_bits) - 1);
+    uint32_t* const words = reinterpret_cast<uint32_t*>(&wcount[0][0]);
+    for (uint64_t j = blockIdx.x; j < n_tiles; j += gridDim.x) {
+        // the thread's positions are recomputed per tile (an empty asm hides
+        // the lane from loop-invariant code motion, which kept 16 positions
+        // and their addresses live across the whole loop and spilled them)
+        uint32_t lane = lane_id();
+        asm volatile("" : "+v"(lane));
+        const uint32_t wave = wave_id();
+        const uint32_t p_wave = wave * kMedItems * kWave;
+        const uint32_t t = list[j];
+        const size_t b0 = bounds[t];
+        const uint32_t n = static_cast<uint32_t>(bounds[t + 1] - b0);  // in (kSegCap, kMedCap]
+        // loads address the tile from its (uniform) base by 32-bit byte
+        // offsets: one register an address, recomputed rather than kept
+        const char* const tile_keys = reinterpret_cast<const char*>(keys + b0);
+        const auto key_at = [tile_keys, n](uint32_t p) {
+            return *reinterpret_cast<const uint64_t*>(tile_keys + (p < n ? p : n - 1) * 8u);
+        };
+        // every load is issued before any is used (positions clamped to the tile)
+        uint64_t raw[kMedItems];
+        uint32_t fw[kMedItems];  // the 32-bit half of the found bitmap holding each key's bit
+#pragma unroll
+        for (int i = 0; i < kMedItems; ++i) raw[i] = key_at(p_wave + i * kWave + lane);
+        if (found != nullptr) {
+            // word ((b0 & 31) + p) / 32 of the bitmap from the tile's first word
+            const char* const tile_found = reinterpret_cast<const char*>(found) + (b0 >> 5) * 4;
+            const uint32_t first_bit = static_cast<uint32_t>(b0 & 31);
+#pragma unroll
+            for (int i = 0; i < kMedItems; ++i) {
+                const uint32_t p = p_wave + i * kWave + lane;
+                fw[i] = *reinterpret_cast<const uint32_t*>(tile_found + ((first_bit + (p < n ? p : n - 1)) >> 5) * 4u);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < kMedItems; ++i) fw[i] = 0;
+        }
+        // the major before an item's first lane: the key before the wave's
+        // share for item 0, then lane 63's of the previous item
+        uint32_t last_major = 0;
+        if (lane == 0 && p_wave > 0 && p_wave < n) last_major = static_cast<uint32_t>(key_at(p_wave - 1) >> minor_bits);
+        // run starts over every position, kept or not (a key whose major
+        // differs from its predecessor's), the kept keys and their minors' OR/AND
+        uint64_t starts[kMedItems], keep[kMedItems];
+        uint32_t wave_starts = 0, wave_keep = 0, mor = 0, mand = 0xFFFFFFFFu;
+#pragma unroll
+        for (int i = 0; i < kMedItems; ++i) {
+            const uint32_t p = p_wave + i * kWave + lane;
+            const bool valid = p < n;
+            const uint32_t major = static_cast<uint32_t>(raw[i] >> minor_bits);
+            const uint32_t up = __shfl_up(major, 1, kWave);
+            const uint32_t prev = lane > 0 ? up : last_major;
+            last_major = __shfl(major, kWave - 1, kWave);
+            starts[i] = ballot(valid && (p == 0 || prev != major));
+            wave_starts += static_cast<uint32_t>(__popcll(starts[i]));
+            const bool kept = valid && !((fw[i] >> ((b0 + p) & 31)) & 1u);
+            keep[i] = ballot(kept);
+            wave_keep += static_cast<uint32_t>(__popcll(keep[i]));
+            if (kept) {
+                mor |= static_cast<uint32_t>(raw[i]) & minor_mask;
+                mand &= static_cast<uint32_t>(raw[i]) & minor_mask;
+            }
+        }
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+            mor |= __shfl_xor(mor, off, kWave);
+            mand &= __shfl_xor(mand, off, kWave);
+        }
+        if (lane == 0) {
+            wtot[wave] = wave_starts;
+            wkeep[wave] = wave_keep;
+            minor_or[wave] = mor;
+            minor_and[wave] = mand;
+        }
+        __syncthreads();
+        uint32_t rank_base = 0, pos = 0, survivors = 0, mo = 0, ma = 0xFFFFFFFFu;
+#pragma unroll
+        for (int w = 0; w < kMedWaves; ++w) {
+            if (w < static_cast<int>(wave)) {
+                rank_base += wtot[w];
+                pos += wkeep[w];
+            }
+            survivors += wkeep[w];
+            mo |= minor_or[w];
+            ma &= minor_and[w];
+        }
+        // the bit range [mlb, mlb + mw) where the kept minors differ
+        const uint32_t mvary = survivors ? (mo ^ ma) : 0u;
+        const int mlb = mvary ? __builtin_ctz(mvary) : 0;
+        const int mw = mvary ? 32 - __builtin_clz(mvary) - mlb : 0;
+        // the tile's keys again (from cache: it was read just now), so that
+        // they need no registers across the barrier
+#pragma unroll
+        for (int i = 0; i < kMedItems; ++i) raw[i] = key_at(p_wave + i * kWave + lane);
+        if (mw + static_cast<int>(kSegRankBits) > 32) {  // block-uniform
+            // too wide for 32-bit sort keys: the kept keys go to the global sort
+            if (threadIdx.x == 0) {
+                side_slot = atomicAdd(reinterpret_cast<unsigned long long*>(&h->side_rows),
+                                      static_cast<unsigned long long>(survivors));
+                side_tile[t] = 1;
+            }
+            __syncthreads();
+            uint64_t at = side_slot + pos;
+#pragma unroll
+            for (int i = 0; i < kMedItems; ++i) {
+                if ((keep[i] >> lane) & 1ull) side[at + lanes_below(keep[i])] = raw[i];
+                at += static_cast<uint32_t>(__popcll(keep[i]));
+            }
+        } else {
+            const uint32_t field = (1u << mw) - 1u;               // mw <= 20
+            const uint32_t fixed = ma & ~(field << mlb);         // the minor bits every kept key shares
+            // rank = run starts at or before the position, minus one; the
+            // majors go to majors[rank], the kept keys' (rank, field) to sk
+            // in position order
+            uint32_t or_acc = 0, and_acc = 0xFFFFFFFFu;
+#pragma unroll
+            for (int i = 0; i < kMedItems; ++i) {
+                const uint32_t p = p_wave + i * kWave + lane;
+                const bool start = (starts[i] >> lane) & 1ull;
+                const uint32_t rank = rank_base + lanes_below(starts[i]) + (start ? 1u : 0u) - 1u;
+                rank_base += static_cast<uint32_t>(__popcll(starts[i]));
+                if (start && p < n && rank < (1u << kSegRankBits))
+                    majors[rank] = static_cast<uint32_t>(raw[i] >> minor_bits);
+                if ((keep[i] >> lane) & 1ull) {
+                    const uint32_t key = (rank << mw) | ((static_cast<uint32_t>(raw[i]) >> mlb) & field);
+                    sk[pos + lanes_below(keep[i])] = key;
+                    or_acc |= key;
+                    and_acc &= key;
+                }
+                pos += static_cast<uint32_t>(__popcll(keep[i]));
+            }
+#pragma unroll
+            for (int off = kWave / 2; off > 0; off >>= 1) {
+                or_acc |= __shfl_xor(or_acc, off, kWave);
+                and_acc &= __shfl_xor(and_acc, off, kWave);
+            }
+            if (lane == 0) {
+                red_or[wave] = or_acc;
+                red_and[wave] = and_acc;
+            }
+            __syncthreads();
+            uint32_t vary = 0;
+            {
+                uint32_t o = 0, a = 0xFFFFFFFFu;
+#pragma unroll
+                for (int w = 0; w < kMedWaves; ++w) {
+                    o |= red_or[w];
+                    a &= red_and[w];
+                }
+                vary = survivors ? (o ^ a) : 0u;
+            }
+            const int lb = vary ? __builtin_ctz(vary) : 0;
+            const int width = vary ? 32 - __builtin_clz(vary) - lb : 0;
+            const int passes = (width + kMedDigitBits - 1) / kMedDigitBits;
+            // the survivors, shared out again: items = ceil(survivors / 1024) a thread
+            const int items = static_cast<int>((survivors + kMedBlock - 1) / kMedBlock);
+            const uint32_t wave_len = static_cast<uint32_t>(items) * kWave;
+            for (int pass = 0; pass < passes; ++pass) {  // block-uniform
+                const int shift = lb + pass * kMedDigitBits;
+                // (every entry is defined: items past `items` would
+                // otherwise leave undefined values that the compiler keeps)
+                uint32_t k[kMedItems], ranks[kMedItems];
+#pragma unroll
+                for (int i = 0; i < kMedItems; ++i) {
+                    k[i] = 0;
+                    ranks[i] = 0;
+                    if (i >= items) continue;
+                    const uint32_t p = wave * wave_len + i * kWave + lane;
+                    k[i] = p < survivors ? sk[p] : 0u;
+                }
+                for (int i = threadIdx.x; i < kMedWaves * kMedRadix / 2; i += kMedBlock) words[i] = 0;
+                __syncthreads();
+#pragma unroll
+                for (int i = 0; i < kMedItems; ++i) {
+                    if (i >= items) continue;
+                    const uint32_t p = wave * wave_len + i * kWave + lane;
+                    const bool valid = p < survivors;
+                    const uint32_t digit = (k[i] >> shift) & (kMedRadix - 1);
+                    const uint64_t peers = match_digit<kMedDigitBits>(digit, valid);
+                    const uint32_t before = wcount[wave][digit];
+                    const uint32_t below = lanes_below(peers);
+                    ranks[i] = before + below;
+                    if (valid && below == 0) wcount[wave][digit] = static_cast<uint16_t>(before + __popcll(peers));
+                }
+                __syncthreads();
+                // thread d owns digit d: its start per wave, then over the tile
+                const uint32_t d = threadIdx.x;
+                uint32_t tc = 0;
+#pragma unroll
+                for (int w = 0; w < kMedWaves; ++w) {
+                    const uint32_t c = wcount[w][d];
+                    wcount[w][d] = static_cast<uint16_t>(tc);
+                    tc += c;
+                }
+                uint32_t total_unused;
+                const uint32_t local_start = block_exclusive_sum<uint32_t, kMedWaves>(tc, &total_unused, scan_lds);
+#pragma unroll
+                for (int w = 0; w < kMedWaves; ++w) wcount[w][d] = static_cast<uint16_t>(wcount[w][d] + local_start);
+                __syncthreads();
+#pragma unroll
+                for (int i = 0; i < kMedItems; ++i) {
+                    if (i >= items) continue;
+                    const uint32_t p = wave * wave_len + i * kWave + lane;
+                    if (p < survivors) sk[wcount[wave][(k[i] >> shift) & (kMedRadix - 1)] + ranks[i]] = k[i];
+                }
+                __syncthreads();
+            }
+            // drop duplicates (equal keys are adjacent now) and write the
+            // pairs back over the tile's input in order, with the unique count
+            uint64_t uniq[kMedItems];
+            uint32_t k[kMedItems];
+            uint32_t wave_u = 0;
+#pragma unroll
+            for (int i = 0; i < kMedItems; ++i) {
+                uniq[i] = 0;
+                k[i] = 0;
+                if (i >= items) continue;
+                const uint32_t p = wave * wave_len + i * kWave + lane;
+                const bool valid = p < survivors;
+                k[i] = valid ? sk[p] : 0u;
+                const uint32_t prev = valid && p > 0 ? sk[p - 1] : ~k[i];
+                uniq[i] = ballot(valid && prev != k[i]);
+                wave_u += static_cast<uint32_t>(__popcll(uniq[i]));
+            }
+            if (lane == 0) wtot[wave] = wave_u;
+            __syncthreads();
+            uint32_t upos = 0, unique = 0;
+#pragma unroll
+            for (int w = 0; w < kMedWaves; ++w) {
+                if (w < static_cast<int>(wave)) upos += wtot[w];
+                unique += wtot[w];
+            }
+#pragma unroll
+            for (int i = 0; i < kMedItems; ++i) {
+                if (i >= items) continue;
+                if ((uniq[i] >> lane) & 1ull) {
+                    const uint32_t major = majors[(k[i] >> mw) & ((1u << kSegRankBits) - 1)];
+                    const uint32_t minor = fixed | ((k[i] & field) << mlb);
+                    keys[b0 + upos + lanes_below(uniq[i])] = (static_cast<uint64_t>(major) << 32) | minor;
+                }
+                upos += static_cast<uint32_t>(__popcll(uniq[i]));
+            }
+            if (threadIdx.x == 0) {
+                side_tile[t] = 0;
+                counts[t] = unique;
+                atomicAdd(reinterpret_cast<unsigned long long*>(&group_sums[t / kSegGroup]),
+                          static_cast<unsigned long long>(unique));
+            }
+        }
+        __syncthreads();  // LDS is reused by the block's next tile
+    }
+}
+
 // Exclusive scan, in place, of the tile groups' unique counts (one block,
 // n >= 1), and the total.
 constexpr int kSegScanThreads = 1024;
@@ -591,7 +912,7 @@ __global__ __launch_bounds__(kBlock) void seg_write_kernel(const uint64_t* keys,
                                                            const uint64_t* side_sorted, const uint64_t* counts,
                                                            const uint64_t* group_offsets, const SegHeader* h,
                                                            size_t tiles, uint32_t minor_bits, PairOutput out,
-                                                           uint32_t* xidx_out) {
+                                                           uint32_t* xidx_out, const uint8_t* side_tile) {
     __shared__ uint64_t offset_slot;
     const uint64_t minor_mask = (1ull << minor_bits) - 1;
     const uint32_t lane = lane_id();
@@ -608,7 +929,7 @@ __global__ __launch_bounds__(kBlock) void seg_write_kernel(const uint64_t* keys,
         __syncthreads();
         const uint64_t o = offset_slot;
         const size_t b0 = bounds[t], b1 = bounds[t + 1];
-        const bool big = b1 - b0 > static_cast<size_t>(kSegCap);
+        const bool big = b1 - b0 > static_cast<size_t>(kSegCap) && side_tile[t];
         const uint64_t* src = keys + b0;
         if (big) {
             // its unique keys are the side output's rows of its first values
@@ -657,7 +978,7 @@ __global__ __launch_bounds__(kBlock) void seg_write_kernel(const uint64_t* keys,
 
 size_t segmented_distinct_meta_bytes(size_t rows, uint32_t tile_rows) { return seg_layout(rows, tile_rows).total; }
 
-hipError_t segmented_distinct_prepare(const uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t* side,
+hipError_t segmented_distinct_prepare(uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t* side,
                                       void* meta, size_t meta_bytes, uint64_t* side_rows, hipStream_t stream,
                                       const uint64_t* found, uint32_t tile_rows) {
     *side_rows = 0;
@@ -672,10 +993,19 @@ hipError_t segmented_distinct_prepare(const uint64_t* keys, size_t rows, uint32_
     uint64_t* groups = reinterpret_cast<uint64_t*>(base + l.groups_off);
     seg_partition_kernel<<<small_grid, kBlock, 0, stream>>>(keys, rows, minor_bits, bounds, l.tiles, h, tile_rows,
                                                             groups, l.groups);
+    uint32_t* list = reinterpret_cast<uint32_t*>(base + l.list_off);
+    uint8_t* side_tile = reinterpret_cast<uint8_t*>(base + l.side_off);
     seg_gather_kernel<<<static_cast<unsigned>(l.tiles < 1024 ? l.tiles : 1024), kBlock, 0, stream>>>(
-        keys, bounds, l.tiles, side, h, found);
+        keys, bounds, l.tiles, side, h, found, list, side_tile);
+    // medium tiles: at most rows / (kSegCap + 1) of them
+    if (rows > static_cast<size_t>(kSegCap)) {
+        const size_t most = rows / (kSegCap + 1);
+        seg_medium_kernel<<<static_cast<unsigned>(most < kMedGrid ? most : kMedGrid), kMedBlock, 0, stream>>>(
+            keys, minor_bits, bounds, list, h, side, side_tile, reinterpret_cast<uint64_t*>(base + l.status_off), found,
+            groups);
+    }
     hipError_t st = hipGetLastError();
-    // runs longer than a tile (usually none): one 8-byte read sizes their sort
+    // rows for the global sort (usually none): one 8-byte read sizes it
     if (st == hipSuccess) st = hipMemcpyAsync(side_rows, &h->side_rows, sizeof(uint64_t), hipMemcpyDeviceToHost, stream);
     if (st == hipSuccess) st = hipStreamSynchronize(stream);
     return st;
@@ -695,6 +1025,7 @@ hipError_t segmented_distinct_finish(uint64_t* keys, size_t rows, uint32_t minor
     uint64_t* status = reinterpret_cast<uint64_t*>(base + l.status_off);
     uint64_t* groups = reinterpret_cast<uint64_t*>(base + l.groups_off);
     size_t* bounds = reinterpret_cast<size_t*>(base + l.bounds_off);
+    const uint8_t* side_tile = reinterpret_cast<const uint8_t*>(base + l.side_off);
     hipError_t st = hipSuccess;
     uint64_t* side_unique_out = side_a;
     if (side_rows) {
@@ -712,14 +1043,14 @@ hipError_t segmented_distinct_finish(uint64_t* keys, size_t rows, uint32_t minor
     // rank bits: at most log2(kSegCap) = 12 (a run per key)
     if (minor_bits + 12 <= 32)
         seg_tile_kernel<uint32_t><<<static_cast<unsigned>(l.tiles), kSegBlock, 0, stream>>>(
-            keys, rows, minor_bits, bounds, side_unique_out, status, h, found, tile_rows, groups);
+            keys, rows, minor_bits, bounds, side_unique_out, status, h, found, tile_rows, groups, side_tile);
     else
         seg_tile_kernel<uint64_t><<<static_cast<unsigned>(l.tiles), kSegBlock, 0, stream>>>(
-            keys, rows, minor_bits, bounds, side_unique_out, status, h, found, tile_rows, groups);
+            keys, rows, minor_bits, bounds, side_unique_out, status, h, found, tile_rows, groups, side_tile);
     seg_scan_kernel<<<1, kSegScanThreads, 0, stream>>>(groups, l.groups, unique_rows);
     const unsigned write_grid = static_cast<unsigned>(l.tiles < 65536 ? l.tiles : 65536);
     seg_write_kernel<<<write_grid, kBlock, 0, stream>>>(keys, bounds, side_unique_out, status, groups, h, l.tiles,
-                                                        minor_bits, out, xidx_out);
+                                                        minor_bits, out, xidx_out, side_tile);
     timer_stop(timer, stream);
     return hipGetLastError();
 }

@@ -193,6 +193,30 @@ def test_mixed_run_lengths(sparse_probe_rows, stream):
 
 
 @pytest.mark.parametrize("sparse_probe_rows", [None, 0])
+def test_medium_candidate_runs(sparse_probe_rows, stream):
+    # sources with 1,025..4,097 first-layer successors, each with 4 successors
+    # drawn from 6,000 targets (so many candidates repeat): candidate runs of
+    # 4,100..16,388 rows, from just past the small LDS tile to just past the
+    # largest tile one block sorts; a chain behind some targets adds
+    # iterations whose candidates meet FULL
+    rng = np.random.default_rng(0x3ED)
+    fans = rng.permutation(np.repeat([1025, 1500, 2048, 3000, 4095, 4096, 4097], 2))
+    e = set()
+    for src, fan in enumerate(fans):
+        mids = 100 + rng.choice(12_000, size=int(fan), replace=False)
+        e.update((src, int(m)) for m in mids)
+    for m in range(100, 12_100):
+        e.update((m, 20_000 + int(t)) for t in rng.choice(6000, size=4, replace=False))
+    e.update((t, t + 1) for t in range(20_000, 20_030))
+    edges = cols(sorted(e))
+    store, summ = _tc(edges, stream, sparse_probe_rows=sparse_probe_rows)
+    ref, rsum = _oracle_run({0: edges}, {1: (edges, edges)}, [rule(1, 1, 0)], [1])
+    full, expect = store.full(R(1)), ref.get(1)
+    assert np.array_equal(full[0], expect[0]) and np.array_equal(full[1], expect[1])
+    assert summ.iterations == rsum.iterations and summ.total_new_rows == rsum.total_new_rows
+
+
+@pytest.mark.parametrize("sparse_probe_rows", [None, 0])
 def test_huge_segment_probe(sparse_probe_rows, stream):
     # node 0 reaches 600,000 nodes in one step: FULL's segment of first value
     # 0 is longer than a tile streams (2^19 rows), so the candidates of 0 in
