@@ -8,11 +8,13 @@
 // write per key:
 //   1. partition: tile boundaries are moved back to the start of the run
 //      that contains them (one thread per boundary), so no run crosses a tile;
-//   2. tiles of 4,097..16,384 keys (a long run: one source's candidates in
-//      one iteration) are sorted in LDS by a persistent kernel, one
-//      1,024-thread block a tile; longer tiles go, gathered, through the
-//      global radix sort;
-//   3. every other tile sorts its keys in LDS by (local run rank, minor) with LSD
+//   2. a tile longer than kSegCap starts with a long run (one source's
+//      candidates in one iteration): its keys set bits of a bitmap of their
+//      minors in LDS, whose set bits are the run's sorted unique minors
+//      (a run whose minors differ above their low 18 bits sends the tile,
+//      gathered, through the global radix sort instead);
+//   3. every tile (a long tile: its tail after the first run) sorts its keys
+//      in LDS by (local run rank, minor) with LSD
 //      passes over the bit range where its keys differ (a 12-bit digit, then
 //      stable 9-bit digits: up to 30 varying bits take three passes), drops
 //      duplicates, and writes the expanded pairs (major << 32 | minor) back
@@ -36,29 +38,18 @@ constexpr int kSegRadix = 1 << kSegDigitBits;
 constexpr int kSegFirstBits = 12;             // digit of the first (unstable) pass
 constexpr int kSegFirstRadix = 1 << kSegFirstBits;
 
-// Tiles longer than kSegCap keys and at most kMedCap are sorted in LDS as
-// well, by a persistent kernel of kMedBlock-thread blocks (seg_medium_kernel,
-// one tile a block at a time); longer tiles go through the global sort.
-// Every tile, whatever its length, holds at most as many runs as the nominal
-// tile has keys (the run containing its nominal start, then runs that start
-// before the next nominal boundary), so a run rank fits kSegRankBits bits.
-// A medium tile's sort key is (rank, the minor's bits in the range where the
-// tile's minors differ): 32 bits when that range is at most 20 bits wide;
-// a wider tile goes to the global sort after all.
-constexpr int kMedCap = 16384;
-constexpr int kMedBlock = 1024;
-constexpr int kMedWaves = kMedBlock / kWave;    // 16
-constexpr int kMedItems = kMedCap / kMedBlock;  // 16 keys per thread
-constexpr int kMedDigitBits = 10;               // one digit per thread
-constexpr int kMedRadix = 1 << kMedDigitBits;
-constexpr unsigned kMedGrid = 256;              // one block a CU (LDS bound)
-constexpr uint32_t kSegRankBits = 12;
-static_assert(kSegCap <= (1 << kSegRankBits), "run ranks fit");
-static_assert(kMedRadix == kMedBlock, "one digit per thread");
-
-__host__ __device__ inline bool seg_medium(size_t size) {
-    return size > static_cast<size_t>(kSegCap) && size <= static_cast<size_t>(kMedCap);
-}
+// A tile longer than kSegCap holds one long run (its first: the run that
+// contains the tile's nominal start) and a tail of shorter runs that lies
+// within one nominal tile (the runs after the first start after the nominal
+// start, and end before the next nominal boundary), so the tail is at most
+// tile_rows - 1 <= kSegCap keys. The first run has one major: it is
+// deduplicated through a bitmap of its minors' low bits in LDS when its
+// minors agree above them (seg_run_kernel), and the tail is sorted like a
+// small tile. A tile whose first run spans a wider range goes to the global
+// sort whole. The bitmap covers 20 bits (128 KB of LDS, one block a CU) when
+// minors have 19 or 20 bits, so no run is too wide; else 18 (32 KB, four
+// blocks a CU).
+constexpr int kRunItems = 8;  // keys a thread loads per chunk
 
 __device__ __forceinline__ void store_pair(const PairOutput& out, size_t i, uint64_t k) {
     if (out.packed) out.packed[i] = k;
@@ -78,11 +69,11 @@ struct SegHeader {
     uint32_t counter;
     uint64_t side_rows;     // keys gathered for the global sort
     uint64_t side_unique;   // their unique count
-    uint64_t medium_tiles;  // tiles listed for seg_medium_kernel
+    uint64_t big_tiles;     // tiles longer than kSegCap, listed for seg_run_kernel
 };
 
 struct SegLayout {
-    size_t status_off, groups_off, bounds_off, list_off, side_off, total, tiles, groups;
+    size_t status_off, groups_off, bounds_off, list_off, side_off, run_off, total, tiles, groups;
 };
 
 // Tiles of a group: a tile adds its unique count to its group's sum, a
@@ -91,8 +82,9 @@ struct SegLayout {
 constexpr uint32_t kSegGroup = 64;
 
 // Metadata of one segmented distinct: header, per-tile unique counts, group
-// sums (then group offsets), tile bounds, the list of medium tiles and a
-// flag per tile longer than kSegCap: 1 when its keys went to the side sort.
+// sums (then group offsets), tile bounds, the list of tiles longer than
+// kSegCap and, for each of those, a flag (1: its keys went to the global
+// sort) and the length of its first run.
 SegLayout seg_layout(size_t rows, uint32_t tile) {
     SegLayout l{};
     l.tiles = seg_tiles(rows, tile);
@@ -108,6 +100,8 @@ SegLayout seg_layout(size_t rows, uint32_t tile) {
     off += seg_align((l.tiles ? l.tiles : 1) * sizeof(uint32_t));
     l.side_off = off;
     off += seg_align((l.tiles ? l.tiles : 1) * sizeof(uint8_t));
+    l.run_off = off;
+    off += seg_align((l.tiles ? l.tiles : 1) * sizeof(uint64_t));
     l.total = off;
     return l;
 }
@@ -126,7 +120,7 @@ __global__ void seg_partition_kernel(const uint64_t* keys, size_t rows, uint32_t
         h->counter = 0;
         h->side_rows = 0;
         h->side_unique = 0;
-        h->medium_tiles = 0;
+        h->big_tiles = 0;
     }
     for (size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; t <= tiles; t += stride) {
         const size_t p = t * tile;
@@ -151,68 +145,12 @@ __device__ __forceinline__ bool dropped(const uint64_t* found, size_t i) {
     return found != nullptr && ((found[i >> 6] >> (i & 63)) & 1ull);
 }
 
-// Copy the keys of tiles longer than kMedCap to the side buffer (any order:
-// they are sorted globally afterwards), except keys whose found bit is set,
-// and list the medium tiles (any order).
-// A block takes a chunk of kGatherItems keys a thread at a time: per item
-// the wave's kept keys are compacted by ballot, the waves' totals are
-// scanned once per chunk, and one atomic on the side count places the chunk
-// (it had scanned the whole block every 256 keys, twice, which made the
-// gather of the batched fixpoint's long runs cost as much as their sort).
-constexpr int kGatherItems = 16;
-__global__ __launch_bounds__(kBlock) void seg_gather_kernel(const uint64_t* keys, const size_t* bounds, size_t tiles,
-                                                            uint64_t* side, SegHeader* h, const uint64_t* found,
-                                                            uint32_t* medium_list, uint8_t* side_tile) {
-    __shared__ uint64_t slot;
-    __shared__ uint32_t wave_tot[kWavesPerBlock];
-    const uint32_t lane = lane_id(), wave = wave_id();
-    for (size_t t = blockIdx.x; t < tiles; t += gridDim.x) {
-        const size_t b0 = bounds[t], b1 = bounds[t + 1];
-        if (b1 - b0 <= static_cast<size_t>(kSegCap)) continue;  // block-uniform
-        if (seg_medium(b1 - b0)) {
-            if (threadIdx.x == 0) {
-                const unsigned long long at =
-                    atomicAdd(reinterpret_cast<unsigned long long*>(&h->medium_tiles), 1ull);
-                medium_list[at] = static_cast<uint32_t>(t);
-            }
-            continue;
-        }
-        if (threadIdx.x == 0) side_tile[t] = 1;
-        for (size_t c = b0; c < b1; c += static_cast<size_t>(kBlock) * kGatherItems) {
-            uint64_t k[kGatherItems], m[kGatherItems];
-            uint32_t mine = 0;  // the wave's kept keys (every lane holds the same)
-#pragma unroll
-            for (int q = 0; q < kGatherItems; ++q) {
-                const size_t i = c + static_cast<size_t>(q) * kBlock + threadIdx.x;
-                k[q] = keys[i < b1 ? i : b1 - 1];
-            }
-#pragma unroll
-            for (int q = 0; q < kGatherItems; ++q) {
-                const size_t i = c + static_cast<size_t>(q) * kBlock + threadIdx.x;
-                m[q] = ballot(i < b1 && !dropped(found, i));
-                mine += static_cast<uint32_t>(__popcll(m[q]));
-            }
-            if (lane == 0) wave_tot[wave] = mine;
-            __syncthreads();
-            uint32_t before = 0, total = 0;
-#pragma unroll
-            for (int w = 0; w < kWavesPerBlock; ++w) {
-                if (w < static_cast<int>(wave)) before += wave_tot[w];
-                total += wave_tot[w];
-            }
-            if (threadIdx.x == 0)
-                slot = atomicAdd(reinterpret_cast<unsigned long long*>(&h->side_rows),
-                                 static_cast<unsigned long long>(total));
-            __syncthreads();
-            uint64_t pos = slot + before;
-#pragma unroll
-            for (int q = 0; q < kGatherItems; ++q) {
-                if ((m[q] >> lane) & 1ull) side[pos + lanes_below(m[q])] = k[q];
-                pos += static_cast<uint32_t>(__popcll(m[q]));
-            }
-            __syncthreads();  // wave_tot and slot are rewritten by the next chunk
-        }
-    }
+// List the tiles longer than kSegCap (any order), one thread a tile.
+__global__ void seg_list_kernel(const size_t* bounds, size_t tiles, SegHeader* h, uint32_t* list) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < tiles; t += stride)
+        if (bounds[t + 1] - bounds[t] > static_cast<size_t>(kSegCap))
+            list[atomicAdd(reinterpret_cast<unsigned long long*>(&h->big_tiles), 1ull)] = static_cast<uint32_t>(t);
 }
 
 // One tile (whole runs, at most kSegCap keys) per block. K is the in-LDS
@@ -252,7 +190,7 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) 
                                                           const size_t* bounds, const uint64_t* side_sorted,
                                                           uint64_t* counts, SegHeader* h, const uint64_t* found,
                                                           uint32_t tile_rows, uint64_t* group_sums,
-                                                          const uint8_t* side_tile) {
+                                                          const uint8_t* side_tile, const uint64_t* run_rows) {
     // sort keys and run majors; before ranking, the same bytes hold the
     // tile's surviving raw keys (u64) when some keys are dropped
     __shared__ uint64_t seg_buf[(kSegCap * (sizeof(K) + sizeof(uint32_t)) + 7) / 8];
@@ -277,11 +215,17 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) 
     const size_t tiles = seg_tiles(rows, tile_rows);
     if (tile >= tiles) return;
     const uint32_t lane = lane_id(), wave = wave_id();
-    const size_t b0 = bounds[tile], b1 = bounds[tile + 1];
+    const size_t tb0 = bounds[tile], b1 = bounds[tile + 1];
+    const bool long_tile = b1 - tb0 > static_cast<size_t>(kSegCap);
+    const bool big = long_tile && side_tile[tile];  // its keys went to the global sort
+    // a long tile's first run is deduplicated already (seg_run_kernel), its
+    // unique rows at the tile's start: this block sorts the tail after it
+    // (less than a nominal tile) and writes the tail's rows after them
+    const uint64_t base_count = long_tile && !big ? counts[tile] : 0;
+    const size_t b0 = long_tile && !big ? tb0 + run_rows[tile] : tb0;  // the first key sorted here
+    const size_t out0 = tb0 + base_count;                              // where its unique rows go
     const size_t size = b1 - b0;
     const uint64_t minor_mask = (1ull << minor_bits) - 1;
-    const bool big = size > static_cast<size_t>(kSegCap);
-    if (big && !side_tile[tile]) return;  // seg_medium_kernel sorted it
     // each wave owns a contiguous share of the tile: items = ceil(size / 512)
     int items = static_cast<int>((size + kSegBlock - 1) / kSegBlock);
     uint32_t wave_len = static_cast<uint32_t>(items) * kWave;
@@ -598,9 +542,9 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) 
     // the slice is done: the write needs no other tile), expanded to pairs;
     // seg_write_kernel moves them to their offsets once every count is known
     if (threadIdx.x == 0) {
-        counts[tile] = count;
+        counts[tile] = base_count + count;
         atomicAdd(reinterpret_cast<unsigned long long*>(&group_sums[tile / kSegGroup]),
-                  static_cast<unsigned long long>(count));
+                  static_cast<unsigned long long>(base_count + count));
     }
     if (!big) {
         uint32_t pos = wave_start;
@@ -611,7 +555,7 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) 
             if ((kept_i >> lane) & 1ull) {
                 const uint32_t rank = static_cast<uint32_t>(k[i] >> minor_bits);
                 const uint32_t major = major_of[rank];
-                keys[b0 + pos + lanes_below(kept_i)] =
+                keys[out0 + pos + lanes_below(kept_i)] =
                     (static_cast<uint64_t>(major) << 32) | (static_cast<uint64_t>(k[i]) & minor_mask);
             }
             pos += static_cast<uint32_t>(__popcll(kept_i));
@@ -619,278 +563,155 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) 
     }
 }
 
-// Medium tiles (ids listed by seg_gather_kernel), one at a time per block
-// of a persistent grid: drops the keys whose found bit is set, sorts the
-// rest in LDS by (local run rank, minor) with stable LSD passes on 10-bit
-// digits over the bits where they differ, drops duplicates, and writes the
-// expanded pairs back over the tile's input slice with its unique count, as
-// seg_tile_kernel does for tiles of at most kSegCap keys. A tile whose
-// minors differ over more than 32 - kSegRankBits bits copies its kept keys
-// to the side buffer instead and is flagged in side_tile. Wave w owns
-// positions [w * 1024, w * 1024 + 1024) of a tile; item i of lane j is
-// position w * 1024 + i * 64 + j.
-__global__ __launch_bounds__(kMedBlock) void seg_medium_kernel(uint64_t* keys, uint32_t minor_bits,
-                                                              const size_t* bounds, const uint32_t* list,
-                                                              SegHeader* h, uint64_t* side, uint8_t* side_tile,
-                                                              uint64_t* counts, const uint64_t* found,
-                                                              uint64_t* group_sums) {
-    __shared__ uint32_t sk[kMedCap];                    // sort keys
-    __shared__ uint32_t majors[1 << kSegRankBits];      // major of each local run
-    __shared__ uint16_t wcount[kMedWaves][kMedRadix];   // per-wave digit counts, then starts
-    __shared__ uint32_t wtot[kMedWaves], wkeep[kMedWaves], scan_lds[kMedWaves];
-    __shared__ uint32_t minor_or[kMedWaves], minor_and[kMedWaves], red_or[kMedWaves], red_and[kMedWaves];
+// The first runs of the long tiles (listed by seg_list_kernel), one tile at
+// a time per block of a persistent grid. A run has one major; its kept keys
+// (found bit clear) set bit (minor & (2^18 - 1)) of a bitmap in LDS while
+// the block checks that their minors agree above bit 18. If they do, the
+// bitmap's set bits, in order, are the run's sorted unique minors: they are
+// written, expanded, over the start of the tile (every key of the tile is
+// read by then), their count to counts[tile] and the run's length (kept or
+// not) to run_rows[tile], for seg_tile_kernel to sort the tail after it.
+// Otherwise every kept key of the tile is copied to the side buffer for the
+// global sort and the tile is flagged in side_tile.
+template <uint32_t kLowBits>
+__global__ __launch_bounds__(kSegBlock) void seg_run_kernel(uint64_t* keys, uint32_t minor_bits, const size_t* bounds,
+                                                           const uint32_t* list, SegHeader* h, uint64_t* side,
+                                                           uint8_t* side_tile, uint64_t* counts, uint64_t* run_rows,
+                                                           const uint64_t* found) {
+    constexpr uint32_t kRunWords = (1u << kLowBits) / 32;
+    __shared__ uint32_t bm[kRunWords];
+    __shared__ uint32_t red_len[kSegWaves], red_or[kSegWaves], red_and[kSegWaves], red_low[kSegWaves];
+    __shared__ uint32_t scan_lds[kSegWaves];
     __shared__ uint64_t side_slot;
-    const uint64_t n_tiles = h->medium_tiles;
+    constexpr uint32_t kChunk = kSegBlock * kRunItems;
+    const uint64_t n_list = h->big_tiles;
+    const uint32_t lane = lane_id(), wave = wave_id();
     const uint32_t minor_mask = static_cast<uint32_t>((1ull << minor_bits) - 1);
-    uint32_t* const words = reinterpret_cast<uint32_t*>(&wcount[0][0]);
-    for (uint64_t j = blockIdx.x; j < n_tiles; j += gridDim.x) {
-        // the thread's positions are recomputed per tile (an empty asm hides
-        // the lane from loop-invariant code motion, which kept 16 positions
-        // and their addresses live across the whole loop and spilled them)
-        uint32_t lane = lane_id();
-        asm volatile("" : "+v"(lane));
-        const uint32_t wave = wave_id();
-        const uint32_t p_wave = wave * kMedItems * kWave;
+    const uint32_t low_mask = (1u << kLowBits) - 1;
+    const uint32_t* const found32 = reinterpret_cast<const uint32_t*>(found);
+    for (uint64_t j = blockIdx.x; j < n_list; j += gridDim.x) {
         const uint32_t t = list[j];
         const size_t b0 = bounds[t];
-        const uint32_t n = static_cast<uint32_t>(bounds[t + 1] - b0);  // in (kSegCap, kMedCap]
-        // loads address the tile from its (uniform) base by 32-bit byte
-        // offsets: one register an address, recomputed rather than kept
-        const char* const tile_keys = reinterpret_cast<const char*>(keys + b0);
-        const auto key_at = [tile_keys, n](uint32_t p) {
-            return *reinterpret_cast<const uint64_t*>(tile_keys + (p < n ? p : n - 1) * 8u);
-        };
-        // every load is issued before any is used (positions clamped to the tile)
-        uint64_t raw[kMedItems];
-        uint32_t fw[kMedItems];  // the 32-bit half of the found bitmap holding each key's bit
+        const size_t size = bounds[t + 1] - b0;  // > kSegCap
+        const uint64_t major0 = keys[b0] >> minor_bits;
+        for (uint32_t i = threadIdx.x; i < kRunWords; i += kSegBlock) bm[i] = 0;
+        __syncthreads();
+        // the run is a prefix of the tile: count its keys, set the kept
+        // keys' bits, and gather the OR/AND of their high minor bits and the
+        // OR of their low ones; chunks of kChunk keys, every load of a chunk
+        // issued before any is used
+        uint32_t len = 0, hi_or = 0, hi_and = 0xFFFFFFFFu, lo_or = 0;
+        for (size_t c = 0; c < size; c += kChunk) {
+            uint64_t k[kRunItems];
+            uint32_t fw[kRunItems];
 #pragma unroll
-        for (int i = 0; i < kMedItems; ++i) raw[i] = key_at(p_wave + i * kWave + lane);
-        if (found != nullptr) {
-            // word ((b0 & 31) + p) / 32 of the bitmap from the tile's first word
-            const char* const tile_found = reinterpret_cast<const char*>(found) + (b0 >> 5) * 4;
-            const uint32_t first_bit = static_cast<uint32_t>(b0 & 31);
-#pragma unroll
-            for (int i = 0; i < kMedItems; ++i) {
-                const uint32_t p = p_wave + i * kWave + lane;
-                fw[i] = *reinterpret_cast<const uint32_t*>(tile_found + ((first_bit + (p < n ? p : n - 1)) >> 5) * 4u);
+            for (int i = 0; i < kRunItems; ++i) {
+                const size_t p = c + static_cast<size_t>(i) * kSegBlock + threadIdx.x;
+                const size_t g = b0 + (p < size ? p : size - 1);
+                k[i] = keys[g];
+                fw[i] = found32 != nullptr ? found32[g >> 5] : 0u;
             }
-        } else {
 #pragma unroll
-            for (int i = 0; i < kMedItems; ++i) fw[i] = 0;
-        }
-        // the major before an item's first lane: the key before the wave's
-        // share for item 0, then lane 63's of the previous item
-        uint32_t last_major = 0;
-        if (lane == 0 && p_wave > 0 && p_wave < n) last_major = static_cast<uint32_t>(key_at(p_wave - 1) >> minor_bits);
-        // run starts over every position, kept or not (a key whose major
-        // differs from its predecessor's), the kept keys and their minors' OR/AND
-        uint64_t starts[kMedItems], keep[kMedItems];
-        uint32_t wave_starts = 0, wave_keep = 0, mor = 0, mand = 0xFFFFFFFFu;
-#pragma unroll
-        for (int i = 0; i < kMedItems; ++i) {
-            const uint32_t p = p_wave + i * kWave + lane;
-            const bool valid = p < n;
-            const uint32_t major = static_cast<uint32_t>(raw[i] >> minor_bits);
-            const uint32_t up = __shfl_up(major, 1, kWave);
-            const uint32_t prev = lane > 0 ? up : last_major;
-            last_major = __shfl(major, kWave - 1, kWave);
-            starts[i] = ballot(valid && (p == 0 || prev != major));
-            wave_starts += static_cast<uint32_t>(__popcll(starts[i]));
-            const bool kept = valid && !((fw[i] >> ((b0 + p) & 31)) & 1u);
-            keep[i] = ballot(kept);
-            wave_keep += static_cast<uint32_t>(__popcll(keep[i]));
-            if (kept) {
-                mor |= static_cast<uint32_t>(raw[i]) & minor_mask;
-                mand &= static_cast<uint32_t>(raw[i]) & minor_mask;
+            for (int i = 0; i < kRunItems; ++i) {
+                const size_t p = c + static_cast<size_t>(i) * kSegBlock + threadIdx.x;
+                const bool in_run = p < size && (k[i] >> minor_bits) == major0;
+                len += static_cast<uint32_t>(__popcll(ballot(in_run)));
+                if (in_run && !((fw[i] >> ((b0 + p) & 31)) & 1u)) {
+                    const uint32_t minor = static_cast<uint32_t>(k[i]) & minor_mask;
+                    atomicOr(&bm[(minor & low_mask) >> 5], 1u << (minor & 31));
+                    hi_or |= minor >> kLowBits;
+                    hi_and &= minor >> kLowBits;
+                    lo_or |= minor & low_mask;
+                }
             }
         }
 #pragma unroll
         for (int off = kWave / 2; off > 0; off >>= 1) {
-            mor |= __shfl_xor(mor, off, kWave);
-            mand &= __shfl_xor(mand, off, kWave);
+            hi_or |= __shfl_xor(hi_or, off, kWave);
+            hi_and &= __shfl_xor(hi_and, off, kWave);
+            lo_or |= __shfl_xor(lo_or, off, kWave);
         }
         if (lane == 0) {
-            wtot[wave] = wave_starts;
-            wkeep[wave] = wave_keep;
-            minor_or[wave] = mor;
-            minor_and[wave] = mand;
+            red_len[wave] = len;  // wave-uniform
+            red_or[wave] = hi_or;
+            red_and[wave] = hi_and;
+            red_low[wave] = lo_or;
         }
         __syncthreads();
-        uint32_t rank_base = 0, pos = 0, survivors = 0, mo = 0, ma = 0xFFFFFFFFu;
+        uint32_t run_len = 0, ho = 0, ha = 0xFFFFFFFFu, lo = 0;
 #pragma unroll
-        for (int w = 0; w < kMedWaves; ++w) {
-            if (w < static_cast<int>(wave)) {
-                rank_base += wtot[w];
-                pos += wkeep[w];
-            }
-            survivors += wkeep[w];
-            mo |= minor_or[w];
-            ma &= minor_and[w];
+        for (int w = 0; w < kSegWaves; ++w) {
+            run_len += red_len[w];
+            ho |= red_or[w];
+            ha &= red_and[w];
+            lo |= red_low[w];
         }
-        // the bit range [mlb, mlb + mw) where the kept minors differ
-        const uint32_t mvary = survivors ? (mo ^ ma) : 0u;
-        const int mlb = mvary ? __builtin_ctz(mvary) : 0;
-        const int mw = mvary ? 32 - __builtin_clz(mvary) - mlb : 0;
-        // the tile's keys again (from cache: it was read just now), so that
-        // they need no registers across the barrier
+        if (ho & ~ha) {  // block-uniform: kept minors differ above their low kLowBits bits
+            // the whole tile's kept keys to the side buffer (any order), a
+            // chunk at a time: ballot compaction per item, one atomic a chunk
+            for (size_t c = 0; c < size; c += kChunk) {
+                uint64_t k[kRunItems], m[kRunItems];
+                uint32_t mine = 0;
 #pragma unroll
-        for (int i = 0; i < kMedItems; ++i) raw[i] = key_at(p_wave + i * kWave + lane);
-        if (mw + static_cast<int>(kSegRankBits) > 32) {  // block-uniform
-            // too wide for 32-bit sort keys: the kept keys go to the global sort
-            if (threadIdx.x == 0) {
-                side_slot = atomicAdd(reinterpret_cast<unsigned long long*>(&h->side_rows),
-                                      static_cast<unsigned long long>(survivors));
-                side_tile[t] = 1;
-            }
-            __syncthreads();
-            uint64_t at = side_slot + pos;
+                for (int i = 0; i < kRunItems; ++i) {
+                    const size_t p = c + static_cast<size_t>(i) * kSegBlock + threadIdx.x;
+                    k[i] = keys[b0 + (p < size ? p : size - 1)];
+                }
 #pragma unroll
-            for (int i = 0; i < kMedItems; ++i) {
-                if ((keep[i] >> lane) & 1ull) side[at + lanes_below(keep[i])] = raw[i];
-                at += static_cast<uint32_t>(__popcll(keep[i]));
+                for (int i = 0; i < kRunItems; ++i) {
+                    const size_t p = c + static_cast<size_t>(i) * kSegBlock + threadIdx.x;
+                    m[i] = ballot(p < size && !dropped(found, b0 + p));
+                    mine += static_cast<uint32_t>(__popcll(m[i]));
+                }
+                if (lane == 0) red_len[wave] = mine;
+                __syncthreads();
+                uint32_t before = 0, total = 0;
+#pragma unroll
+                for (int w = 0; w < kSegWaves; ++w) {
+                    if (w < static_cast<int>(wave)) before += red_len[w];
+                    total += red_len[w];
+                }
+                if (threadIdx.x == 0)
+                    side_slot = atomicAdd(reinterpret_cast<unsigned long long*>(&h->side_rows),
+                                          static_cast<unsigned long long>(total));
+                __syncthreads();
+                uint64_t at = side_slot + before;
+#pragma unroll
+                for (int i = 0; i < kRunItems; ++i) {
+                    if ((m[i] >> lane) & 1ull) side[at + lanes_below(m[i])] = k[i];
+                    at += static_cast<uint32_t>(__popcll(m[i]));
+                }
+                __syncthreads();  // red_len and side_slot are rewritten by the next chunk
             }
+            if (threadIdx.x == 0) side_tile[t] = 1;
         } else {
-            const uint32_t field = (1u << mw) - 1u;               // mw <= 20
-            const uint32_t fixed = ma & ~(field << mlb);         // the minor bits every kept key shares
-            // rank = run starts at or before the position, minus one; the
-            // majors go to majors[rank], the kept keys' (rank, field) to sk
-            // in position order
-            uint32_t or_acc = 0, and_acc = 0xFFFFFFFFu;
-#pragma unroll
-            for (int i = 0; i < kMedItems; ++i) {
-                const uint32_t p = p_wave + i * kWave + lane;
-                const bool start = (starts[i] >> lane) & 1ull;
-                const uint32_t rank = rank_base + lanes_below(starts[i]) + (start ? 1u : 0u) - 1u;
-                rank_base += static_cast<uint32_t>(__popcll(starts[i]));
-                if (start && p < n && rank < (1u << kSegRankBits))
-                    majors[rank] = static_cast<uint32_t>(raw[i] >> minor_bits);
-                if ((keep[i] >> lane) & 1ull) {
-                    const uint32_t key = (rank << mw) | ((static_cast<uint32_t>(raw[i]) >> mlb) & field);
-                    sk[pos + lanes_below(keep[i])] = key;
-                    or_acc |= key;
-                    and_acc &= key;
+            // the set bits in order: thread i takes words [i * per, i * per
+            // + per) up to the highest set bit (lo: the OR of the set bits'
+            // positions), counts them, and after one scan writes them
+            const uint64_t high = (static_cast<uint64_t>(major0) << 32) | (static_cast<uint64_t>(ho) << kLowBits);
+            const uint32_t words_used = lo ? (lo >> 5) + 1 : 0;
+            const uint32_t per = (words_used + kSegBlock - 1) / kSegBlock;
+            const uint32_t w0 = threadIdx.x * per;
+            uint32_t mine = 0;
+            for (uint32_t q = 0; q < per; ++q)
+                if (w0 + q < words_used) mine += static_cast<uint32_t>(__popc(bm[w0 + q]));
+            uint32_t out;
+            uint32_t at = block_exclusive_sum<uint32_t, kSegWaves>(mine, &out, scan_lds);
+            for (uint32_t q = 0; q < per; ++q) {
+                const uint32_t w = w0 + q;
+                uint32_t bits = w < words_used ? bm[w] : 0u;
+                while (bits) {
+                    keys[b0 + at++] = high | ((w << 5) | static_cast<uint32_t>(__builtin_ctz(bits)));
+                    bits &= bits - 1;
                 }
-                pos += static_cast<uint32_t>(__popcll(keep[i]));
-            }
-#pragma unroll
-            for (int off = kWave / 2; off > 0; off >>= 1) {
-                or_acc |= __shfl_xor(or_acc, off, kWave);
-                and_acc &= __shfl_xor(and_acc, off, kWave);
-            }
-            if (lane == 0) {
-                red_or[wave] = or_acc;
-                red_and[wave] = and_acc;
-            }
-            __syncthreads();
-            uint32_t vary = 0;
-            {
-                uint32_t o = 0, a = 0xFFFFFFFFu;
-#pragma unroll
-                for (int w = 0; w < kMedWaves; ++w) {
-                    o |= red_or[w];
-                    a &= red_and[w];
-                }
-                vary = survivors ? (o ^ a) : 0u;
-            }
-            const int lb = vary ? __builtin_ctz(vary) : 0;
-            const int width = vary ? 32 - __builtin_clz(vary) - lb : 0;
-            const int passes = (width + kMedDigitBits - 1) / kMedDigitBits;
-            // the survivors, shared out again: items = ceil(survivors / 1024) a thread
-            const int items = static_cast<int>((survivors + kMedBlock - 1) / kMedBlock);
-            const uint32_t wave_len = static_cast<uint32_t>(items) * kWave;
-            for (int pass = 0; pass < passes; ++pass) {  // block-uniform
-                const int shift = lb + pass * kMedDigitBits;
-                // (every entry is defined: items past `items` would
-                // otherwise leave undefined values that the compiler keeps)
-                uint32_t k[kMedItems], ranks[kMedItems];
-#pragma unroll
-                for (int i = 0; i < kMedItems; ++i) {
-                    k[i] = 0;
-                    ranks[i] = 0;
-                    if (i >= items) continue;
-                    const uint32_t p = wave * wave_len + i * kWave + lane;
-                    k[i] = p < survivors ? sk[p] : 0u;
-                }
-                for (int i = threadIdx.x; i < kMedWaves * kMedRadix / 2; i += kMedBlock) words[i] = 0;
-                __syncthreads();
-#pragma unroll
-                for (int i = 0; i < kMedItems; ++i) {
-                    if (i >= items) continue;
-                    const uint32_t p = wave * wave_len + i * kWave + lane;
-                    const bool valid = p < survivors;
-                    const uint32_t digit = (k[i] >> shift) & (kMedRadix - 1);
-                    const uint64_t peers = match_digit<kMedDigitBits>(digit, valid);
-                    const uint32_t before = wcount[wave][digit];
-                    const uint32_t below = lanes_below(peers);
-                    ranks[i] = before + below;
-                    if (valid && below == 0) wcount[wave][digit] = static_cast<uint16_t>(before + __popcll(peers));
-                }
-                __syncthreads();
-                // thread d owns digit d: its start per wave, then over the tile
-                const uint32_t d = threadIdx.x;
-                uint32_t tc = 0;
-#pragma unroll
-                for (int w = 0; w < kMedWaves; ++w) {
-                    const uint32_t c = wcount[w][d];
-                    wcount[w][d] = static_cast<uint16_t>(tc);
-                    tc += c;
-                }
-                uint32_t total_unused;
-                const uint32_t local_start = block_exclusive_sum<uint32_t, kMedWaves>(tc, &total_unused, scan_lds);
-#pragma unroll
-                for (int w = 0; w < kMedWaves; ++w) wcount[w][d] = static_cast<uint16_t>(wcount[w][d] + local_start);
-                __syncthreads();
-#pragma unroll
-                for (int i = 0; i < kMedItems; ++i) {
-                    if (i >= items) continue;
-                    const uint32_t p = wave * wave_len + i * kWave + lane;
-                    if (p < survivors) sk[wcount[wave][(k[i] >> shift) & (kMedRadix - 1)] + ranks[i]] = k[i];
-                }
-                __syncthreads();
-            }
-            // drop duplicates (equal keys are adjacent now) and write the
-            // pairs back over the tile's input in order, with the unique count
-            uint64_t uniq[kMedItems];
-            uint32_t k[kMedItems];
-            uint32_t wave_u = 0;
-#pragma unroll
-            for (int i = 0; i < kMedItems; ++i) {
-                uniq[i] = 0;
-                k[i] = 0;
-                if (i >= items) continue;
-                const uint32_t p = wave * wave_len + i * kWave + lane;
-                const bool valid = p < survivors;
-                k[i] = valid ? sk[p] : 0u;
-                const uint32_t prev = valid && p > 0 ? sk[p - 1] : ~k[i];
-                uniq[i] = ballot(valid && prev != k[i]);
-                wave_u += static_cast<uint32_t>(__popcll(uniq[i]));
-            }
-            if (lane == 0) wtot[wave] = wave_u;
-            __syncthreads();
-            uint32_t upos = 0, unique = 0;
-#pragma unroll
-            for (int w = 0; w < kMedWaves; ++w) {
-                if (w < static_cast<int>(wave)) upos += wtot[w];
-                unique += wtot[w];
-            }
-#pragma unroll
-            for (int i = 0; i < kMedItems; ++i) {
-                if (i >= items) continue;
-                if ((uniq[i] >> lane) & 1ull) {
-                    const uint32_t major = majors[(k[i] >> mw) & ((1u << kSegRankBits) - 1)];
-                    const uint32_t minor = fixed | ((k[i] & field) << mlb);
-                    keys[b0 + upos + lanes_below(uniq[i])] = (static_cast<uint64_t>(major) << 32) | minor;
-                }
-                upos += static_cast<uint32_t>(__popcll(uniq[i]));
             }
             if (threadIdx.x == 0) {
                 side_tile[t] = 0;
-                counts[t] = unique;
-                atomicAdd(reinterpret_cast<unsigned long long*>(&group_sums[t / kSegGroup]),
-                          static_cast<unsigned long long>(unique));
+                counts[t] = out;
+                run_rows[t] = run_len;
             }
         }
-        __syncthreads();  // LDS is reused by the block's next tile
+        __syncthreads();  // the bitmap and the reductions are reused by the block's next tile
     }
 }
 
@@ -995,14 +816,21 @@ hipError_t segmented_distinct_prepare(uint64_t* keys, size_t rows, uint32_t mino
                                                             groups, l.groups);
     uint32_t* list = reinterpret_cast<uint32_t*>(base + l.list_off);
     uint8_t* side_tile = reinterpret_cast<uint8_t*>(base + l.side_off);
-    seg_gather_kernel<<<static_cast<unsigned>(l.tiles < 1024 ? l.tiles : 1024), kBlock, 0, stream>>>(
-        keys, bounds, l.tiles, side, h, found, list, side_tile);
-    // medium tiles: at most rows / (kSegCap + 1) of them
+    seg_list_kernel<<<small_grid, kBlock, 0, stream>>>(bounds, l.tiles, h, list);
+    // long tiles: at most rows / (kSegCap + 1) of them
     if (rows > static_cast<size_t>(kSegCap)) {
         const size_t most = rows / (kSegCap + 1);
-        seg_medium_kernel<<<static_cast<unsigned>(most < kMedGrid ? most : kMedGrid), kMedBlock, 0, stream>>>(
-            keys, minor_bits, bounds, list, h, side, side_tile, reinterpret_cast<uint64_t*>(base + l.status_off), found,
-            groups);
+        uint64_t* counts = reinterpret_cast<uint64_t*>(base + l.status_off);
+        uint64_t* run_rows = reinterpret_cast<uint64_t*>(base + l.run_off);
+        if (minor_bits > 18 && minor_bits <= 20) {
+            const unsigned grid = static_cast<unsigned>(most < 256 ? most : 256);  // one block a CU
+            seg_run_kernel<20><<<grid, kSegBlock, 0, stream>>>(keys, minor_bits, bounds, list, h, side, side_tile,
+                                                               counts, run_rows, found);
+        } else {
+            const unsigned grid = static_cast<unsigned>(most < 1024 ? most : 1024);  // four blocks a CU
+            seg_run_kernel<18><<<grid, kSegBlock, 0, stream>>>(keys, minor_bits, bounds, list, h, side, side_tile,
+                                                               counts, run_rows, found);
+        }
     }
     hipError_t st = hipGetLastError();
     // rows for the global sort (usually none): one 8-byte read sizes it
@@ -1026,6 +854,7 @@ hipError_t segmented_distinct_finish(uint64_t* keys, size_t rows, uint32_t minor
     uint64_t* groups = reinterpret_cast<uint64_t*>(base + l.groups_off);
     size_t* bounds = reinterpret_cast<size_t*>(base + l.bounds_off);
     const uint8_t* side_tile = reinterpret_cast<const uint8_t*>(base + l.side_off);
+    const uint64_t* run_rows = reinterpret_cast<const uint64_t*>(base + l.run_off);
     hipError_t st = hipSuccess;
     uint64_t* side_unique_out = side_a;
     if (side_rows) {
@@ -1043,10 +872,10 @@ hipError_t segmented_distinct_finish(uint64_t* keys, size_t rows, uint32_t minor
     // rank bits: at most log2(kSegCap) = 12 (a run per key)
     if (minor_bits + 12 <= 32)
         seg_tile_kernel<uint32_t><<<static_cast<unsigned>(l.tiles), kSegBlock, 0, stream>>>(
-            keys, rows, minor_bits, bounds, side_unique_out, status, h, found, tile_rows, groups, side_tile);
+            keys, rows, minor_bits, bounds, side_unique_out, status, h, found, tile_rows, groups, side_tile, run_rows);
     else
         seg_tile_kernel<uint64_t><<<static_cast<unsigned>(l.tiles), kSegBlock, 0, stream>>>(
-            keys, rows, minor_bits, bounds, side_unique_out, status, h, found, tile_rows, groups, side_tile);
+            keys, rows, minor_bits, bounds, side_unique_out, status, h, found, tile_rows, groups, side_tile, run_rows);
     seg_scan_kernel<<<1, kSegScanThreads, 0, stream>>>(groups, l.groups, unique_rows);
     const unsigned write_grid = static_cast<unsigned>(l.tiles < 65536 ? l.tiles : 65536);
     seg_write_kernel<<<write_grid, kBlock, 0, stream>>>(keys, bounds, side_unique_out, status, groups, h, l.tiles,

@@ -192,13 +192,15 @@ def test_mixed_run_lengths(sparse_probe_rows, stream):
     assert summ.iterations == rsum.iterations and summ.total_new_rows == rsum.total_new_rows
 
 
-@pytest.mark.parametrize("sparse_probe_rows", [None, 0])
-def test_medium_candidate_runs(sparse_probe_rows, stream):
+@pytest.mark.parametrize("sparse_probe_rows,spread", [(None, False), (0, False), (None, True)])
+def test_medium_candidate_runs(sparse_probe_rows, spread, stream):
     # sources with 1,025..4,097 first-layer successors, each with 4 successors
     # drawn from 6,000 targets (so many candidates repeat): candidate runs of
     # 4,100..16,388 rows, from just past the small LDS tile to just past the
     # largest tile one block sorts; a chain behind some targets adds
-    # iterations whose candidates meet FULL
+    # iterations whose candidates meet FULL. spread: ids scaled by 65,537,
+    # so a run's second columns differ over more than 20 bits (too wide for
+    # the in-LDS sort keys: such tiles go to the global sort)
     rng = np.random.default_rng(0x3ED)
     fans = rng.permutation(np.repeat([1025, 1500, 2048, 3000, 4095, 4096, 4097], 2))
     e = set()
@@ -208,6 +210,8 @@ def test_medium_candidate_runs(sparse_probe_rows, stream):
     for m in range(100, 12_100):
         e.update((m, 20_000 + int(t)) for t in rng.choice(6000, size=4, replace=False))
     e.update((t, t + 1) for t in range(20_000, 20_030))
+    if spread:
+        e = {(a * 65_537, b * 65_537) for a, b in e}
     edges = cols(sorted(e))
     store, summ = _tc(edges, stream, sparse_probe_rows=sparse_probe_rows)
     ref, rsum = _oracle_run({0: edges}, {1: (edges, edges)}, [rule(1, 1, 0)], [1])
