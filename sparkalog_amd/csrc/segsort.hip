@@ -37,6 +37,7 @@ constexpr int kSegDigitBits = 9;              // digit of a tile pass
 constexpr int kSegRadix = 1 << kSegDigitBits;
 constexpr int kSegFirstBits = 12;             // digit of the first (unstable) pass
 constexpr int kSegFirstRadix = 1 << kSegFirstBits;
+constexpr uint32_t kSegRankBits = 12;  // a tile holds fewer runs than kSegCap
 
 // A tile longer than kSegCap holds one long run (its first: the run that
 // contains the tile's nominal start) and a tail of shorter runs that lies
@@ -70,6 +71,7 @@ struct SegHeader {
     uint64_t side_rows;     // keys gathered for the global sort
     uint64_t side_unique;   // their unique count
     uint64_t big_tiles;     // tiles longer than kSegCap, listed for seg_run_kernel
+    uint64_t deferred;      // tiles too wide for compressed u32 sort keys
 };
 
 struct SegLayout {
@@ -121,6 +123,7 @@ __global__ void seg_partition_kernel(const uint64_t* keys, size_t rows, uint32_t
         h->side_rows = 0;
         h->side_unique = 0;
         h->big_tiles = 0;
+        h->deferred = 0;
     }
     for (size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; t <= tiles; t += stride) {
         const size_t p = t * tile;
@@ -182,38 +185,39 @@ __device__ __forceinline__ size_t tile_search(size_t lo, size_t hi, Pred before,
     return lo;
 }
 
-// sort key (local run rank << minor_bits | minor): u32 when rank and minor
-// fit in 32 bits (minor_bits <= 20), else u64.
-template <typename K>
-__global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) void seg_tile_kernel(
-    uint64_t* keys, size_t rows, uint32_t minor_bits,
-                                                          const size_t* bounds, const uint64_t* side_sorted,
-                                                          uint64_t* counts, SegHeader* h, const uint64_t* found,
-                                                          uint32_t tile_rows, uint64_t* group_sums,
-                                                          const uint8_t* side_tile, const uint64_t* run_rows) {
-    // sort keys and run majors; before ranking, the same bytes hold the
-    // tile's surviving raw keys (u64) when some keys are dropped
-    __shared__ uint64_t seg_buf[(kSegCap * (sizeof(K) + sizeof(uint32_t)) + 7) / 8];
-    static_assert(sizeof(seg_buf) >= kSegCap * sizeof(uint64_t), "raw keys fit");
+// One tile (at most kSegCap keys, or a long tile's tail) per call, with the
+// block's LDS. The sort key is (local run rank << S | minor field): with
+// kCompress the field is the bit range [mlb, mlb + mw) where the tile's
+// minors differ (S = mw; the bits outside it are shared and put back when
+// writing), else the whole minor (S = minor_bits). K is u32 when the rank
+// (below kSegCap: 12 bits) and field fit; a compressing block whose field
+// is wider than 20 bits lists its tile in `defer` for the u64 list kernel
+// and returns.
+template <typename K, bool kCompress>
+__device__ __forceinline__ void seg_tile_body(uint32_t tile, uint64_t* keys, uint32_t minor_bits,
+                                              const size_t* bounds, const uint64_t* side_sorted, uint64_t* counts,
+                                              SegHeader* h, const uint64_t* found, uint64_t* group_sums,
+                                              const uint8_t* side_tile, const uint64_t* run_rows, uint32_t* defer,
+                                              uint64_t* seg_buf, uint16_t (*wave_count)[kSegRadix]) {
+    // seg_buf: sort keys and run majors; before ranking, the same bytes hold
+    // the tile's surviving raw keys (u64) when some keys are dropped
     K* const tile_keys = reinterpret_cast<K*>(seg_buf);
     uint32_t* const major_of = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(seg_buf) + kSegCap * sizeof(K));
-    // digit counts per wave (a wave ranks at most 512 keys: u16), 40 KB in
-    // all for u32 sort keys, so 4 blocks fit a CU. The small arrays alias
-    // space idle while they are used: per-wave totals and the OR/AND of the
-    // keys live in wave_count outside the passes, the scan's scratch in
-    // tile_keys during a pass's digit scan (the keys are in registers then).
-    __shared__ __attribute__((aligned(16))) uint16_t wave_count[kSegWaves][kSegRadix];
+    // wave_count: digit counts per wave (a wave ranks at most 512 keys: u16),
+    // 40 KB in all for u32 sort keys, so 4 blocks fit a CU. The small arrays
+    // alias space idle while they are used: per-wave totals and the OR/AND
+    // of the minors and then of the keys live in wave_count outside the
+    // passes, the scan's scratch in tile_keys during a pass's digit scan (the
+    // keys are in registers then).
     uint32_t* const wave_tot = reinterpret_cast<uint32_t*>(&wave_count[0][0]);  // kept keys, run starts, unique keys
     K* const red_or = reinterpret_cast<K*>(&wave_count[0][0]) + 2 * kSegWaves;
     K* const red_and = red_or + kSegWaves;
     uint32_t* const scan_lds = reinterpret_cast<uint32_t*>(tile_keys);
     static_assert(sizeof(uint32_t) * kSegWaves + 2 * sizeof(K) * kSegWaves + 2 * sizeof(K) * kSegWaves <=
                       sizeof(uint16_t) * kSegWaves * kSegRadix, "aliases fit");
-    static_assert(sizeof(uint16_t) * kSegFirstRadix == sizeof(wave_count), "first-pass counts fit");
-    // tiles are independent (no look-back), so dispatch order needs no ticket
-    const uint32_t tile = blockIdx.x;
-    const size_t tiles = seg_tiles(rows, tile_rows);
-    if (tile >= tiles) return;
+    static_assert(sizeof(uint16_t) * kSegFirstRadix == sizeof(uint16_t) * kSegWaves * kSegRadix,
+                  "first-pass counts fit");
+    static_assert(!kCompress || sizeof(K) == sizeof(uint32_t), "compressed keys are u32");
     const uint32_t lane = lane_id(), wave = wave_id();
     const size_t tb0 = bounds[tile], b1 = bounds[tile + 1];
     const bool long_tile = b1 - tb0 > static_cast<size_t>(kSegCap);
@@ -236,6 +240,10 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) 
     size_t side0 = 0;          // big tile: its slice of the side output
     uint64_t kept[kSegItems];  // ballot masks of the unique keys (wave-uniform)
     uint32_t wave_start = 0;   // unique keys in earlier waves
+    // sort key = rank << key_shift | ((minor >> mlb) & field); the minor is
+    // fixed | ((key & field) << mlb)
+    uint32_t key_shift = minor_bits, mlb = 0;
+    uint32_t field = static_cast<uint32_t>(minor_mask), fixed = 0;
     if (!big) {
         // load; k = (run starts at or before the position within this wave)
         // << minor_bits | minor; majors of run starts are staged in LDS (in
@@ -311,6 +319,49 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) 
             if (p0 > 0 && p0 < n) prev_raw = seg_buf[p0 - 1];
             __syncthreads();  // seg_buf now holds the sort keys and run majors
         }
+        // with kCompress: the bit range where the kept minors differ, from
+        // their OR and AND (through red_or / red_and, which the keys' own
+        // reduction reuses only after the run starts' barrier below)
+        if constexpr (kCompress) {
+            uint32_t o = 0, a = 0xFFFFFFFFu;
+#pragma unroll
+            for (int i = 0; i < kSegItems; ++i) {
+                if (i >= items) continue;
+                const uint32_t p = wave * wave_len + i * kWave + lane;
+                if (p < n) {
+                    const uint32_t m = static_cast<uint32_t>(rawv[i]) & static_cast<uint32_t>(minor_mask);
+                    o |= m;
+                    a &= m;
+                }
+            }
+#pragma unroll
+            for (int off = kWave / 2; off > 0; off >>= 1) {
+                o |= __shfl_xor(o, off, kWave);
+                a &= __shfl_xor(a, off, kWave);
+            }
+            if (lane == 0) {
+                red_or[wave] = o;
+                red_and[wave] = a;
+            }
+            __syncthreads();
+            uint32_t mo = 0, ma = 0xFFFFFFFFu;
+#pragma unroll
+            for (int w = 0; w < kSegWaves; ++w) {
+                mo |= red_or[w];
+                ma &= red_and[w];
+            }
+            const uint32_t vary = n ? (mo ^ ma) : 0u;
+            mlb = vary ? static_cast<uint32_t>(__builtin_ctz(vary)) : 0u;
+            const uint32_t mw = vary ? 32u - static_cast<uint32_t>(__builtin_clz(vary)) - mlb : 0u;
+            if (mw + kSegRankBits > 32) {  // block-uniform: the u64 list kernel sorts this tile
+                if (threadIdx.x == 0)
+                    defer[atomicAdd(reinterpret_cast<unsigned long long*>(&h->deferred), 1ull)] = tile;
+                return;
+            }
+            key_shift = mw;
+            field = (1u << mw) - 1u;
+            fixed = ma & ~(field << mlb);  // the minor bits every kept key shares
+        }
         uint32_t* staged = reinterpret_cast<uint32_t*>(tile_keys) + wave * (kSegCap / kSegWaves);
         uint32_t starts_before = 0;  // run starts in earlier items of this wave
         uint32_t last_major = 0;     // major of lane 63 of the previous item
@@ -337,7 +388,7 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) 
             const uint64_t m = ballot(start);
             const uint32_t local = starts_before + lanes_below(m) + (start ? 1u : 0u);
             if (start) staged[local - 1] = major;
-            k[i] = (static_cast<K>(local) << minor_bits) | static_cast<K>(raw & minor_mask);
+            k[i] = (static_cast<K>(local) << key_shift) | static_cast<K>((static_cast<uint32_t>(raw) >> mlb) & field);
             starts_before += static_cast<uint32_t>(__popcll(m));
         }
         if (lane == 0) wave_tot[wave] = starts_before;
@@ -346,7 +397,7 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) 
         for (int w = 0; w < static_cast<int>(wave); ++w) wave_base += wave_tot[w];
         for (uint32_t j = lane; j < starts_before; j += kWave) major_of[wave_base + j] = staged[j];
         // rank = wave_base + local - 1 (unsigned wrap-around is exact)
-        const K rank_add = static_cast<K>(static_cast<K>(static_cast<K>(wave_base) - static_cast<K>(1)) << minor_bits);
+        const K rank_add = static_cast<K>(static_cast<K>(static_cast<K>(wave_base) - static_cast<K>(1)) << key_shift);
         K or_acc = 0, and_acc = static_cast<K>(~static_cast<K>(0));
 #pragma unroll
         for (int i = 0; i < kSegItems; ++i) {
@@ -553,20 +604,54 @@ __global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) 
             if (i >= items) continue;
             const uint64_t kept_i = kept[i];
             if ((kept_i >> lane) & 1ull) {
-                const uint32_t rank = static_cast<uint32_t>(k[i] >> minor_bits);
+                const uint32_t rank = static_cast<uint32_t>(k[i] >> key_shift);
                 const uint32_t major = major_of[rank];
                 keys[out0 + pos + lanes_below(kept_i)] =
-                    (static_cast<uint64_t>(major) << 32) | (static_cast<uint64_t>(k[i]) & minor_mask);
+                    (static_cast<uint64_t>(major) << 32) | (fixed | ((static_cast<uint32_t>(k[i]) & field) << mlb));
             }
             pos += static_cast<uint32_t>(__popcll(kept_i));
         }
     }
 }
 
+// One tile per block (blockIdx.x < tiles).
+template <typename K, bool kCompress>
+__global__ __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(8))) void seg_tile_kernel(
+    uint64_t* keys, size_t tiles, uint32_t minor_bits, const size_t* bounds, const uint64_t* side_sorted,
+    uint64_t* counts, SegHeader* h, const uint64_t* found, uint64_t* group_sums, const uint8_t* side_tile,
+    const uint64_t* run_rows, uint32_t* defer) {
+    __shared__ uint64_t seg_buf[(kSegCap * (sizeof(K) + sizeof(uint32_t)) + 7) / 8];
+    static_assert(sizeof(seg_buf) >= kSegCap * sizeof(uint64_t), "raw keys fit");
+    __shared__ __attribute__((aligned(16))) uint16_t wave_count[kSegWaves][kSegRadix];
+    // tiles are independent (no look-back), so dispatch order needs no ticket
+    if (blockIdx.x >= tiles) return;
+    seg_tile_body<K, kCompress>(blockIdx.x, keys, minor_bits, bounds, side_sorted, counts, h, found, group_sums,
+                                side_tile, run_rows, defer, seg_buf, wave_count);
+}
+
+// The tiles a compressing launch deferred (u64 sort keys), one at a time per
+// block of a persistent grid.
+__global__ __launch_bounds__(kSegBlock) void seg_tile_list_kernel(uint64_t* keys, uint32_t minor_bits,
+                                                                 const size_t* bounds, const uint64_t* side_sorted,
+                                                                 uint64_t* counts, SegHeader* h,
+                                                                 const uint64_t* found, uint64_t* group_sums,
+                                                                 const uint8_t* side_tile, const uint64_t* run_rows,
+                                                                 const uint32_t* list) {
+    __shared__ uint64_t seg_buf[(kSegCap * (sizeof(uint64_t) + sizeof(uint32_t)) + 7) / 8];
+    __shared__ __attribute__((aligned(16))) uint16_t wave_count[kSegWaves][kSegRadix];
+    const uint64_t n_list = h->deferred;
+    for (uint64_t j = blockIdx.x; j < n_list; j += gridDim.x) {
+        seg_tile_body<uint64_t, false>(list[j], keys, minor_bits, bounds, side_sorted, counts, h, found, group_sums,
+                                       side_tile, run_rows, nullptr, seg_buf, wave_count);
+        __syncthreads();  // LDS is reused by the block's next tile
+    }
+}
+
 // The first runs of the long tiles (listed by seg_list_kernel), one tile at
 // a time per block of a persistent grid. A run has one major; its kept keys
-// (found bit clear) set bit (minor & (2^18 - 1)) of a bitmap in LDS while
-// the block checks that their minors agree above bit 18. If they do, the
+// (found bit clear) set the bit of their minor's low kLowBits bits in a
+// bitmap in LDS while the block checks that their minors agree above those
+// bits. If they do, the
 // bitmap's set bits, in order, are the run's sorted unique minors: they are
 // written, expanded, over the start of the tile (every key of the tile is
 // read by then), their count to counts[tile] and the run's length (kept or
@@ -869,13 +954,19 @@ hipError_t segmented_distinct_finish(uint64_t* keys, size_t rows, uint32_t minor
         if (st != hipSuccess) return st;
     }
     timer_start(timer, stream);
-    // rank bits: at most log2(kSegCap) = 12 (a run per key)
-    if (minor_bits + 12 <= 32)
-        seg_tile_kernel<uint32_t><<<static_cast<unsigned>(l.tiles), kSegBlock, 0, stream>>>(
-            keys, rows, minor_bits, bounds, side_unique_out, status, h, found, tile_rows, groups, side_tile, run_rows);
-    else
-        seg_tile_kernel<uint64_t><<<static_cast<unsigned>(l.tiles), kSegBlock, 0, stream>>>(
-            keys, rows, minor_bits, bounds, side_unique_out, status, h, found, tile_rows, groups, side_tile, run_rows);
+    // rank bits: at most log2(kSegCap) = 12 (a run per key); wider minors
+    // are compressed to the range where each tile's differ, and the tiles
+    // where that is still too wide are sorted with u64 keys after
+    uint32_t* defer = reinterpret_cast<uint32_t*>(base + l.list_off);  // the long-tile list is used up
+    if (minor_bits + kSegRankBits <= 32) {
+        seg_tile_kernel<uint32_t, false><<<static_cast<unsigned>(l.tiles), kSegBlock, 0, stream>>>(
+            keys, l.tiles, minor_bits, bounds, side_unique_out, status, h, found, groups, side_tile, run_rows, defer);
+    } else {
+        seg_tile_kernel<uint32_t, true><<<static_cast<unsigned>(l.tiles), kSegBlock, 0, stream>>>(
+            keys, l.tiles, minor_bits, bounds, side_unique_out, status, h, found, groups, side_tile, run_rows, defer);
+        seg_tile_list_kernel<<<static_cast<unsigned>(l.tiles < 512 ? l.tiles : 512), kSegBlock, 0, stream>>>(
+            keys, minor_bits, bounds, side_unique_out, status, h, found, groups, side_tile, run_rows, defer);
+    }
     seg_scan_kernel<<<1, kSegScanThreads, 0, stream>>>(groups, l.groups, unique_rows);
     const unsigned write_grid = static_cast<unsigned>(l.tiles < 65536 ? l.tiles : 65536);
     seg_write_kernel<<<write_grid, kBlock, 0, stream>>>(keys, bounds, side_unique_out, status, groups, h, l.tiles,
