@@ -81,11 +81,15 @@ hipError_t unique_sorted(const uint64_t* keys, const uint64_t* rows_dev, size_t 
 // Distinct of compressed keys (major << minor_bits | minor) whose major part
 // is non-decreasing along the input; writes sorted unique expanded pairs
 // (major << 32 | minor). Two phases: prepare partitions the input into tiles
-// aligned to runs, sorts the tiles of 4,097..16,384 keys (one block each),
-// copies the keys of longer tiles to `side` and returns their row count on
-// the host; finish sorts those (side/side_b, sort_temp sized by
-// distinct_temp_bytes(side_rows); unused when side_rows is 0) and every
-// other tile, overwriting `keys` (each tile's unique rows are staged over
+// aligned to runs, deduplicates the long first run of each tile over
+// kSegCap keys through a bitmap of its minors, copies the keys of a long
+// tile whose first run is too wide for the bitmap (minors of more than 20
+// bits that differ above their low 18) to `side` and returns their row
+// count on the host (with minor_bits <= 20 there are none: prepare returns
+// 0 without waiting for the device); finish sorts
+// those (side/side_b, sort_temp sized by distinct_temp_bytes(side_rows);
+// unused when side_rows is 0) and every tile (a long tile's tail after its
+// first run), overwriting `keys` (each tile's unique rows are staged over
 // its input).
 // `meta` (segmented_distinct_meta_bytes(rows)) must persist between phases.
 // With `found` (probe_grouped's words over the same keys), keys whose found

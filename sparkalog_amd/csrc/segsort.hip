@@ -918,6 +918,9 @@ hipError_t segmented_distinct_prepare(uint64_t* keys, size_t rows, uint32_t mino
         }
     }
     hipError_t st = hipGetLastError();
+    // minors of at most 20 bits fit the run bitmap whole: no key goes to
+    // the global sort, and the host need not wait
+    if (minor_bits <= 20) return st;
     // rows for the global sort (usually none): one 8-byte read sizes it
     if (st == hipSuccess) st = hipMemcpyAsync(side_rows, &h->side_rows, sizeof(uint64_t), hipMemcpyDeviceToHost, stream);
     if (st == hipSuccess) st = hipStreamSynchronize(stream);
