@@ -665,7 +665,8 @@ __global__ __launch_bounds__(kSegBlock) void seg_run_kernel(uint64_t* keys, uint
                                                            const uint64_t* found) {
     constexpr uint32_t kRunWords = (1u << kLowBits) / 32;
     __shared__ uint32_t bm[kRunWords];
-    __shared__ uint32_t red_len[kSegWaves], red_or[kSegWaves], red_and[kSegWaves], red_low[kSegWaves];
+    __shared__ uint32_t red_len[kSegWaves], red_or[kSegWaves], red_and[kSegWaves], red_min[kSegWaves],
+        red_max[kSegWaves];
     __shared__ uint32_t scan_lds[kSegWaves];
     __shared__ uint64_t side_slot;
     constexpr uint32_t kChunk = kSegBlock * kRunItems;
@@ -683,9 +684,9 @@ __global__ __launch_bounds__(kSegBlock) void seg_run_kernel(uint64_t* keys, uint
         __syncthreads();
         // the run is a prefix of the tile: count its keys, set the kept
         // keys' bits, and gather the OR/AND of their high minor bits and the
-        // OR of their low ones; chunks of kChunk keys, every load of a chunk
-        // issued before any is used
-        uint32_t len = 0, hi_or = 0, hi_and = 0xFFFFFFFFu, lo_or = 0;
+        // least and greatest set bit; chunks of kChunk keys, every load of a
+        // chunk issued before any is used
+        uint32_t len = 0, hi_or = 0, hi_and = 0xFFFFFFFFu, lo_min = 0xFFFFFFFFu, lo_max = 0;
         for (size_t c = 0; c < size; c += kChunk) {
             uint64_t k[kRunItems];
             uint32_t fw[kRunItems];
@@ -706,7 +707,8 @@ __global__ __launch_bounds__(kSegBlock) void seg_run_kernel(uint64_t* keys, uint
                     atomicOr(&bm[(minor & low_mask) >> 5], 1u << (minor & 31));
                     hi_or |= minor >> kLowBits;
                     hi_and &= minor >> kLowBits;
-                    lo_or |= minor & low_mask;
+                    lo_min = min(lo_min, minor & low_mask);
+                    lo_max = max(lo_max, minor & low_mask);
                 }
             }
         }
@@ -714,22 +716,25 @@ __global__ __launch_bounds__(kSegBlock) void seg_run_kernel(uint64_t* keys, uint
         for (int off = kWave / 2; off > 0; off >>= 1) {
             hi_or |= __shfl_xor(hi_or, off, kWave);
             hi_and &= __shfl_xor(hi_and, off, kWave);
-            lo_or |= __shfl_xor(lo_or, off, kWave);
+            lo_min = min(lo_min, static_cast<uint32_t>(__shfl_xor(lo_min, off, kWave)));
+            lo_max = max(lo_max, static_cast<uint32_t>(__shfl_xor(lo_max, off, kWave)));
         }
         if (lane == 0) {
             red_len[wave] = len;  // wave-uniform
             red_or[wave] = hi_or;
             red_and[wave] = hi_and;
-            red_low[wave] = lo_or;
+            red_min[wave] = lo_min;
+            red_max[wave] = lo_max;
         }
         __syncthreads();
-        uint32_t run_len = 0, ho = 0, ha = 0xFFFFFFFFu, lo = 0;
+        uint32_t run_len = 0, ho = 0, ha = 0xFFFFFFFFu, lo = 0xFFFFFFFFu, hi = 0;
 #pragma unroll
         for (int w = 0; w < kSegWaves; ++w) {
             run_len += red_len[w];
             ho |= red_or[w];
             ha &= red_and[w];
-            lo |= red_low[w];
+            lo = min(lo, red_min[w]);
+            hi = max(hi, red_max[w]);
         }
         if (ho & ~ha) {  // block-uniform: kept minors differ above their low kLowBits bits
             // the whole tile's kept keys to the side buffer (any order), a
@@ -770,13 +775,14 @@ __global__ __launch_bounds__(kSegBlock) void seg_run_kernel(uint64_t* keys, uint
             }
             if (threadIdx.x == 0) side_tile[t] = 1;
         } else {
-            // the set bits in order: thread i takes words [i * per, i * per
-            // + per) up to the highest set bit (lo: the OR of the set bits'
-            // positions), counts them, and after one scan writes them
+            // the set bits in order: of the words from the least set bit's
+            // to the greatest's, thread i takes [i * per, i * per + per),
+            // counts their bits, and after one scan writes them
             const uint64_t high = (static_cast<uint64_t>(major0) << 32) | (static_cast<uint64_t>(ho) << kLowBits);
-            const uint32_t words_used = lo ? (lo >> 5) + 1 : 0;
-            const uint32_t per = (words_used + kSegBlock - 1) / kSegBlock;
-            const uint32_t w0 = threadIdx.x * per;
+            const uint32_t w_first = lo <= hi ? lo >> 5 : 0;
+            const uint32_t words_used = lo <= hi ? (hi >> 5) + 1 : 0;  // one past the last word
+            const uint32_t per = (words_used - w_first + kSegBlock - 1) / kSegBlock;
+            const uint32_t w0 = w_first + threadIdx.x * per;
             uint32_t mine = 0;
             for (uint32_t q = 0; q < per; ++q)
                 if (w0 + q < words_used) mine += static_cast<uint32_t>(__popc(bm[w0 + q]));

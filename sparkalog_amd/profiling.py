@@ -45,14 +45,36 @@ def short_name(name: str) -> str:
     return base.rsplit("::", 1)[-1]
 
 
+def templated_name(name: str) -> str:
+    """short_name with the template arguments kept ("k<unsigned int, true>"),
+    to tell instantiations of one kernel apart."""
+    base = short_name(name)
+    name = name.strip().replace("(anonymous namespace)::", "")
+    at = name.find(base + "<")
+    if at < 0:
+        return base
+    depth = 0
+    for i in range(at + len(base), len(name)):
+        depth += {"<": 1, ">": -1}.get(name[i], 0)
+        if depth == 0:
+            return name[at:i + 1]
+    return base
+
+
 def stats_markdown(stats_csv: Path, title: str, command: str, top: int = 40) -> str:
     rows = list(csv.DictReader(open(stats_csv)))
     total = sum(float(r["TotalDurationNs"]) for r in rows)
+    seen = defaultdict(int)
+    for r in rows:
+        seen[short_name(r["Name"])] += 1
+    # instantiations of one template are listed with their arguments
+    label = {r["Name"]: templated_name(r["Name"]) if seen[short_name(r["Name"])] > 1 else short_name(r["Name"])
+             for r in rows}
     out = [f"# {title}", "", "Command (run from the repository root):", "", "```", command, "```", "",
            f"Total kernel time: {total / 1e6:.3f} ms over {sum(int(r['Calls']) for r in rows)} dispatches", "",
            "| Kernel | Calls | Total ms | Avg us | Min us | Max us | % |", "|---|---|---|---|---|---|---|"]
     for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"]))[:top]:
-        out.append(f"| {short_name(r['Name'])} | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
+        out.append(f"| {label[r['Name']]} | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
                    f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['MinNs']) / 1e3:.2f} | "
                    f"{float(r['MaxNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
     return "\n".join(out) + "\n"
