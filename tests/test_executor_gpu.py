@@ -220,6 +220,28 @@ def test_medium_candidate_runs(sparse_probe_rows, spread, stream):
     assert summ.iterations == rsum.iterations and summ.total_new_rows == rsum.total_new_rows
 
 
+@pytest.mark.parametrize("spread", [1, 1 << 17])
+def test_long_run_with_tail_at_high_ids(spread, stream):
+    # ids above 2^30 (32-bit minors): hub b reaches 5,000 nodes b+1.. in one
+    # step, each of those reaches one node b+6000+i, which reaches
+    # c + i*spread. The first iteration's candidates are b's run of 5,000
+    # (minors agreeing above their low 18 bits: the run bitmap) followed, in
+    # the same long tile, by one-row runs whose minors differ over 30 bits
+    # with spread 2^17 (tail tiles too wide for 32-bit sort keys) or over 13
+    # bits with spread 1 (compressed keys); the second iteration gives b a
+    # run of 5,000 over those same minors (the global sort when wide)
+    b, c, n = 1 << 30, 1 << 31, 5000
+    e = {(b, b + 1 + i) for i in range(n)}
+    e |= {(b + 1 + i, b + 6000 + i) for i in range(n)}
+    e |= {(b + 6000 + i, c + i * spread) for i in range(n)}
+    edges = cols(sorted(e))
+    store, summ = _tc(edges, stream)
+    ref, rsum = _oracle_run({0: edges}, {1: (edges, edges)}, [rule(1, 1, 0)], [1])
+    full, expect = store.full(R(1)), ref.get(1)
+    assert np.array_equal(full[0], expect[0]) and np.array_equal(full[1], expect[1])
+    assert summ.iterations == rsum.iterations and summ.total_new_rows == rsum.total_new_rows
+
+
 @pytest.mark.parametrize("sparse_probe_rows", [None, 0])
 def test_huge_segment_probe(sparse_probe_rows, stream):
     # node 0 reaches 600,000 nodes in one step: FULL's segment of first value
