@@ -85,8 +85,9 @@ constexpr uint32_t kSegGroup = 64;
 
 // Metadata of one segmented distinct: header, per-tile unique counts, group
 // sums (then group offsets), tile bounds, the list of tiles longer than
-// kSegCap and, for each of those, a flag (1: its keys went to the global
-// sort) and the length of its first run.
+// kSegCap (in finish, of the tiles deferred to u64 sort keys) and, for each
+// long tile, a flag (1: its keys went to the global sort) and the length of
+// its first run.
 SegLayout seg_layout(size_t rows, uint32_t tile) {
     SegLayout l{};
     l.tiles = seg_tiles(rows, tile);
@@ -156,7 +157,6 @@ __global__ void seg_list_kernel(const size_t* bounds, size_t tiles, SegHeader* h
             list[atomicAdd(reinterpret_cast<unsigned long long*>(&h->big_tiles), 1ull)] = static_cast<uint32_t>(t);
 }
 
-// One tile (whole runs, at most kSegCap keys) per block. K is the in-LDS
 // block_search (first index in [lo, hi) where `before` turns false, for a
 // predicate true on a prefix) for the 512-thread tile kernel, reducing
 // through `red` (kSegWaves u32 of LDS the caller has free) instead of
@@ -651,13 +651,12 @@ __global__ __launch_bounds__(kSegBlock) void seg_tile_list_kernel(uint64_t* keys
 // a time per block of a persistent grid. A run has one major; its kept keys
 // (found bit clear) set the bit of their minor's low kLowBits bits in a
 // bitmap in LDS while the block checks that their minors agree above those
-// bits. If they do, the
-// bitmap's set bits, in order, are the run's sorted unique minors: they are
-// written, expanded, over the start of the tile (every key of the tile is
-// read by then), their count to counts[tile] and the run's length (kept or
-// not) to run_rows[tile], for seg_tile_kernel to sort the tail after it.
-// Otherwise every kept key of the tile is copied to the side buffer for the
-// global sort and the tile is flagged in side_tile.
+// bits. If they do, the bitmap's set bits, in order, are the run's sorted
+// unique minors: they are written, expanded, over the start of the tile
+// (every key of the tile is read by then), their count to counts[tile] and
+// the run's length (kept or not) to run_rows[tile], for seg_tile_kernel to
+// sort the tail after it. Otherwise every kept key of the tile is copied to
+// the side buffer for the global sort and the tile is flagged in side_tile.
 template <uint32_t kLowBits>
 __global__ __launch_bounds__(kSegBlock) void seg_run_kernel(uint64_t* keys, uint32_t minor_bits, const size_t* bounds,
                                                            const uint32_t* list, SegHeader* h, uint64_t* side,
