@@ -774,6 +774,7 @@ struct TargetRuntime {
     uint64_t contrib_bound = 0;
     int anti_slot = -1;            // profiling event of this iteration's anti-join
     bool fused = false;            // this iteration's distinct wrote NEWT (candidates in FULL dropped first)
+    bool instances_marked = false; // and marked the instances of NEWT's rows (instance tracking)
     std::vector<int> union_slots;  // profiling events of this iteration's contribution unions
 };
 
@@ -2264,6 +2265,7 @@ void session_step(sparkalog_hip_store* s) {
     bool status_cleared = false;
     for (auto& t : targets) {
         t.fused = false;
+        t.instances_marked = false;
         t.csr_run = false;
         t.anti_slot = -1;
     }
@@ -2519,10 +2521,18 @@ void session_step(sparkalog_hip_store* s) {
             uint32_t* xidx_out = nullptr;
             if (fuse >= 0 && targets[static_cast<size_t>(fuse)].csr_run)
                 xidx_out = targets[static_cast<size_t>(fuse)].new_xidx.ptr;
+            // NEWT written here is the next DELTA: its instances are marked
+            // as it is written (instead of a pass over DELTA after the step)
+            InstanceMarks marks;
+            if (fuse >= 0 && ss.instance_bits) {
+                marks = InstanceMarks{ss.last_new.ptr, ss.instance_bits, ss.instances,
+                                      static_cast<uint32_t>(iterations + 1)};
+                targets[static_cast<size_t>(fuse)].instances_marked = true;
+            }
             check(segmented_distinct_finish(rt.cand_a.ptr, M, rt.shift, known_compressed, rt.cand_b.ptr,
                                             side_rows ? rt.cand_c.ptr : nullptr, side_rows, out, count,
                                             rt.seg_meta.ptr, mb, temp, tb, s->stream, s->kernel_timer(ev), found,
-                                            xidx_out, s->seg_tile));
+                                            xidx_out, s->seg_tile, marks));
             if (xidx_out) {
                 // the new run's index: entries of first values without new rows
                 const size_t ftb = xindex_fill_temp_bytes(n_x);
@@ -2758,7 +2768,7 @@ void session_step(sparkalog_hip_store* s) {
     if (ss.instance_bits) {
         for (uint32_t i = 0; i < n_relations; ++i) {
             const RelationState& ts = *find(s, relations[i]);
-            if (ts.delta.rows)
+            if (ts.delta.rows && !targets[i].instances_marked)
                 track_instances_kernel<<<grid_for(ts.delta.rows), kBlock, 0, s->stream>>>(
                     ts.delta.keys.ptr, ts.delta.rows, ss.instance_bits, ss.instances,
                     static_cast<uint32_t>(iterations + 1), ss.last_new.ptr);

@@ -100,6 +100,15 @@ hipError_t unique_sorted(const uint64_t* keys, const uint64_t* rows_dev, size_t 
 // `tile_rows` is the nominal tile (512..4096 keys, 3,072 by default): tile
 // boundaries sit every tile_rows keys before they move back to run starts.
 constexpr uint32_t kSegTileDefault = 3072;
+// Instance tracking for a batch of instances in one id space (instance =
+// first value >> bits): finish raises last_new[instance] to `value` for every
+// instance with an output row (at most `instances`; last_new null: none).
+struct InstanceMarks {
+    uint32_t* last_new = nullptr;
+    uint32_t bits = 0;
+    uint32_t instances = 0;
+    uint32_t value = 0;
+};
 size_t segmented_distinct_meta_bytes(size_t rows, uint32_t tile_rows = kSegTileDefault);
 hipError_t segmented_distinct_prepare(uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t* side,
                                       void* meta, size_t meta_bytes, uint64_t* side_rows, hipStream_t stream,
@@ -109,7 +118,7 @@ hipError_t segmented_distinct_finish(uint64_t* keys, size_t rows, uint32_t minor
                                      uint64_t* unique_rows, void* meta, size_t meta_bytes, void* sort_temp,
                                      size_t sort_bytes, hipStream_t stream, const KernelTimer* timer = nullptr,
                                      const uint64_t* found = nullptr, uint32_t* xidx_out = nullptr,
-                                     uint32_t tile_rows = kSegTileDefault);
+                                     uint32_t tile_rows = kSegTileDefault, InstanceMarks marks = {});
 
 // ---- sorted set operators ----
 size_t union_temp_bytes(size_t left_rows, size_t right_rows);

@@ -823,7 +823,8 @@ __global__ __launch_bounds__(kBlock) void seg_write_kernel(const uint64_t* keys,
                                                            const uint64_t* side_sorted, const uint64_t* counts,
                                                            const uint64_t* group_offsets, const SegHeader* h,
                                                            size_t tiles, uint32_t minor_bits, PairOutput out,
-                                                           uint32_t* xidx_out, const uint8_t* side_tile) {
+                                                           uint32_t* xidx_out, const uint8_t* side_tile,
+                                                           InstanceMarks marks) {
     __shared__ uint64_t offset_slot;
     const uint64_t minor_mask = (1ull << minor_bits) - 1;
     const uint32_t lane = lane_id();
@@ -879,6 +880,14 @@ __global__ __launch_bounds__(kBlock) void seg_write_kernel(const uint64_t* keys,
                 // segment of the output's first-column index
                 if (xidx_out && (i == 0 || (pk >> 32) != (k >> 32)))
                     xidx_out[k >> 32] = static_cast<uint32_t>(o + i);
+                // a tile's first row and each row whose instance differs from
+                // its predecessor's mark their instance (rows are sorted)
+                if (marks.last_new) {
+                    const uint32_t inst = static_cast<uint32_t>(k >> 32) >> marks.bits;
+                    if (inst < marks.instances &&
+                        (i == 0 || (static_cast<uint32_t>(pk >> 32) >> marks.bits) != inst))
+                        atomicMax(&marks.last_new[inst], marks.value);
+                }
             }
         }
         __syncthreads();  // offset_slot is rewritten for the block's next tile
@@ -936,7 +945,8 @@ hipError_t segmented_distinct_finish(uint64_t* keys, size_t rows, uint32_t minor
                                      uint64_t* side_a, uint64_t* side_b, uint64_t side_rows, PairOutput out,
                                      uint64_t* unique_rows, void* meta, size_t meta_bytes, void* sort_temp,
                                      size_t sort_bytes, hipStream_t stream, const KernelTimer* timer,
-                                     const uint64_t* found, uint32_t* xidx_out, uint32_t tile_rows) {
+                                     const uint64_t* found, uint32_t* xidx_out, uint32_t tile_rows,
+                                     InstanceMarks marks) {
     if (rows == 0) return set_u64(unique_rows, 0, stream);
     if (tile_rows < 512 || tile_rows > static_cast<uint32_t>(kSegCap)) return hipErrorInvalidValue;
     const SegLayout l = seg_layout(rows, tile_rows);
@@ -978,7 +988,7 @@ hipError_t segmented_distinct_finish(uint64_t* keys, size_t rows, uint32_t minor
     seg_scan_kernel<<<1, kSegScanThreads, 0, stream>>>(groups, l.groups, unique_rows);
     const unsigned write_grid = static_cast<unsigned>(l.tiles < 65536 ? l.tiles : 65536);
     seg_write_kernel<<<write_grid, kBlock, 0, stream>>>(keys, bounds, side_unique_out, status, groups, h, l.tiles,
-                                                        minor_bits, out, xidx_out, side_tile);
+                                                        minor_bits, out, xidx_out, side_tile, marks);
     timer_stop(timer, stream);
     return hipGetLastError();
 }
