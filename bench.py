@@ -253,6 +253,8 @@ def batched_summary(budget_pairs: float, plain_stride: int, full_log: bool = Fal
            "results_equal_plain": same, "max_iterations": int(res.iterations.max())}
     if full_log:
         out["wave_log"] = res.waves
+        out["cold_wave_log"] = cold.waves
+    store.close()
     del store
     torch.cuda.empty_cache()
     return out
@@ -335,6 +337,9 @@ def served_summary(budget_pairs: float, requests: int, utilizations: str, lane_b
             load["one_lane"] = serving.summarize(one, t_one)
         out["loads"].append(load)
     out["results_equal_alone_vs_mix"] = equal
+    store.close()
+    for lane in lanes:
+        lane.close()
     del store, lanes
     torch.cuda.empty_cache()
     return out
@@ -763,7 +768,10 @@ def main():
                            "tuples_per_s_pcie_inclusive": round(host_full[0].size / (t2 - t0), 1)}
         del host_full
     if not args.no_batched:
-        # the batched configuration (4,096 instances) beside the plain path
+        # the batched configuration (4,096 instances) beside the plain path;
+        # the closure's store returns its pools first (close, not only del:
+        # other references may keep the object and its device memory alive)
+        store.close()
         del store
         torch.cuda.empty_cache()
         line["batched"] = batched_summary(args.budget_pairs, args.bench_plain_stride, graph=_graph_arg(args),
