@@ -674,13 +674,16 @@ __global__ __launch_bounds__(kSegBlock) void seg_run_kernel(uint64_t* keys, uint
     const uint32_t minor_mask = static_cast<uint32_t>((1ull << minor_bits) - 1);
     const uint32_t low_mask = (1u << kLowBits) - 1;
     const uint32_t* const found32 = reinterpret_cast<const uint32_t*>(found);
+    // the bitmap is cleared once here, and after each tile only between the
+    // words of its least and greatest set bit
+    if (blockIdx.x < n_list)
+        for (uint32_t i = threadIdx.x; i < kRunWords; i += kSegBlock) bm[i] = 0;
+    __syncthreads();
     for (uint64_t j = blockIdx.x; j < n_list; j += gridDim.x) {
         const uint32_t t = list[j];
         const size_t b0 = bounds[t];
         const size_t size = bounds[t + 1] - b0;  // > kSegCap
         const uint64_t major0 = keys[b0] >> minor_bits;
-        for (uint32_t i = threadIdx.x; i < kRunWords; i += kSegBlock) bm[i] = 0;
-        __syncthreads();
         // the run is a prefix of the tile: count its keys, set the kept
         // keys' bits, and gather the OR/AND of their high minor bits and the
         // least and greatest set bit; chunks of kChunk keys, every load of a
@@ -801,6 +804,9 @@ __global__ __launch_bounds__(kSegBlock) void seg_run_kernel(uint64_t* keys, uint
                 run_rows[t] = run_len;
             }
         }
+        __syncthreads();  // every read of the bitmap is done
+        if (lo <= hi)
+            for (uint32_t w = (lo >> 5) + threadIdx.x; w <= (hi >> 5); w += kSegBlock) bm[w] = 0;
         __syncthreads();  // the bitmap and the reductions are reused by the block's next tile
     }
 }
