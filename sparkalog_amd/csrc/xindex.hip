@@ -435,9 +435,10 @@ __global__ __launch_bounds__(kBlock) void probe_sparse_kernel(const uint64_t* ke
 // run, [xstart[x_lo], xstart[x_hi + 1]), stream through LDS in chunks of
 // kPStage second values, and a candidate (x, y) searches y only in the part
 // of its own segment that the chunk holds. A tile with a wide range and more
-// than kPSeg / 2 groups (fewer than 4 candidates per first value), or whose
+// than kPSeg / 2 groups (fewer than 4 candidates per first value), whose
 // rows of some run exceed kPStreamLimit (few first values with huge
-// segments), is listed for probe_xsec_deferred_kernel instead, which searches
+// segments), or whose runs hold more than kPRowsPerCandidate rows per
+// candidate, is listed for probe_xsec_deferred_kernel instead, which searches
 // every run in place (keeping that path out of this kernel keeps it at 64
 // VGPRs).
 constexpr int kPTile = 2048;                   // candidates per tile
@@ -446,6 +447,7 @@ constexpr int kPWaveRows = kPItems * kWave;    // 512 per wave
 constexpr int kPStage = 4096;                  // second values per LDS chunk (16 KB)
 constexpr int kPStageVecs = kPStage / 4 / kBlock;  // 16-byte loads per thread and chunk
 constexpr size_t kPStreamLimit = size_t{1} << 19;
+constexpr size_t kPRowsPerCandidate = 256;
 constexpr int kPSeg = 512;                     // LDS index entries of a tile's segment bounds (19.6 KB of LDS: 8 blocks a CU)
 
 __global__ __launch_bounds__(kBlock) void probe_xsec_kernel(const uint64_t* keys, size_t rows, uint32_t minor_bits,
@@ -538,10 +540,17 @@ __global__ __launch_bounds__(kBlock) void probe_xsec_kernel(const uint64_t* keys
     // dense: the range's index entries fit; else one pair per group (block-uniform)
     const uint32_t span_x = x_hi - x_lo + 2;
     const bool dense = span_x <= static_cast<uint32_t>(kPSeg);
-    // every run's range must stream, and the bounds must fit (block-uniform)
+    // every run's range must stream, and the bounds must fit; a tile whose
+    // runs hold more than kPRowsPerCandidate rows per candidate searches in
+    // place instead (a binary search per candidate reads fewer bytes than
+    // streaming the rows) (block-uniform)
     bool stream = dense || n_groups <= static_cast<uint32_t>(kPSeg / 2);
-    for (uint32_t q = 0; q < rights.count; ++q)
+    size_t tile_rows = 0;
+    for (uint32_t q = 0; q < rights.count; ++q) {
         stream = stream && static_cast<size_t>(run_hi[q] - run_lo[q]) <= kPStreamLimit;
+        tile_rows += run_hi[q] - run_lo[q];
+    }
+    stream = stream && tile_rows <= kPRowsPerCandidate * n;
     if (!stream) {
         // deferred[0] counts the listed tiles, deferred[1 ..] lists them
         if (threadIdx.x == 0) deferred[1 + atomicAdd(&deferred[0], 1u)] = blockIdx.x;
