@@ -1486,7 +1486,6 @@ __global__ __launch_bounds__(kBlock) void instance_stats_kernel(const uint64_t* 
     if (r0 >= n) return;  // wave-uniform
     const size_t r1 = r0 + kStatsWaveRows < n ? r0 + kStatsWaveRows : n;
     const uint32_t lane = lane_id();
-    const uint32_t mask = bits >= 32 ? 0xFFFFFFFFu : ((1u << bits) - 1u);
     uint32_t cur = 0xFFFFFFFFu;     // instance being summed (wave-uniform)
     unsigned long long acc = 0;     // this lane's digest share of it
     unsigned long long acc_rows = 0;
@@ -1512,10 +1511,7 @@ __global__ __launch_bounds__(kBlock) void instance_stats_kernel(const uint64_t* 
             const bool valid = i < r1;
             const uint32_t x = static_cast<uint32_t>(kk[j] >> 32), z = static_cast<uint32_t>(kk[j]);
             const uint32_t inst = valid ? (bits >= 32 ? 0u : x >> bits) : 0xFFFFFFFFu;
-            uint64_t zz = ((static_cast<uint64_t>(x & mask) << 32) | (z & mask)) + 0x9E3779B97F4A7C15ull;
-            zz = (zz ^ (zz >> 30)) * 0xBF58476D1CE4E5B9ull;
-            zz = (zz ^ (zz >> 27)) * 0x94D049BB133111EBull;
-            const unsigned long long c = valid ? (zz ^ (zz >> 31)) : 0ull;
+            const unsigned long long c = valid ? instance_pair_hash(x, z, bits) : 0ull;
             const uint64_t valid_mask = ballot(valid);
             if (valid_mask == 0) continue;  // wave-uniform
             const uint32_t lead = __builtin_amdgcn_readfirstlane(inst);
@@ -2110,6 +2106,14 @@ struct Session {
     uint64_t* inst_delta_host = nullptr;
     uint32_t inst_delta_cap = 0;
     bool inst_delta_valid = false;
+    // per-instance rows and digests of the one tracked relation's FULL,
+    // accumulated from its initial FULL and every NEWT the distinct writes
+    // (stat_live while every step's NEWT was counted); after the session
+    // they answer instance_stats while the relation keeps stat_version
+    DevBuf<uint64_t> stat_acc;
+    bool stat_live = false, stat_valid = false;
+    uint32_t stat_rel = 0, stat_bits = 0, stat_instances = 0;
+    uint64_t stat_version = 0;
 };
 
 // Rows of each instance in sorted packed keys (count *rows_dev <= bound):
@@ -2210,6 +2214,32 @@ int session_begin(sparkalog_hip_store* s, const uint32_t* relations, uint32_t n_
         }
     }
     ss.inst_delta_valid = false;
+    ss.stat_live = ss.stat_valid = false;
+    if (ss.instance_bits && ss.instances && n_relations == 1 && !ss.graph) {
+        // the initial FULL's statistics, when all of it is packed keys
+        RelationState& st = *find(s, relations[0]);
+        bool packed = !st.full.csr;
+        for (const auto& r : st.runs) packed = packed && !r.csr;
+        if (packed) {
+            // rows, then digests, each kInstanceStatSlots copies of every instance
+            const size_t n_acc = 2 * static_cast<size_t>(kInstanceStatSlots) * ss.instances;
+            ss.stat_acc.reserve(n_acc);
+            unsigned long long* acc = reinterpret_cast<unsigned long long*>(ss.stat_acc.ptr);
+            check(fill_u64(ss.stat_acc.ptr, n_acc, 0, s->stream));
+            const auto add = [&](const DevRelation& part) {
+                if (!part.rows) return;
+                const size_t waves = (part.rows + kStatsWaveRows - 1) / kStatsWaveRows;
+                const size_t blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+                instance_stats_kernel<<<static_cast<unsigned>(blocks), kBlock, 0, s->stream>>>(
+                    part.keys.ptr, part.rows, ss.instance_bits, ss.instances, acc,
+                    acc + static_cast<size_t>(kInstanceStatSlots) * ss.instances);
+            };
+            add(st.full);
+            for (const auto& r : st.runs) add(r);
+            check(hipGetLastError());
+            ss.stat_live = true;
+        }
+    }
     ss.active = true;
     return SPARKALOG_HIP_OK;
 }
@@ -2527,6 +2557,10 @@ void session_step(sparkalog_hip_store* s) {
             if (fuse >= 0 && ss.instance_bits) {
                 marks = InstanceMarks{ss.last_new.ptr, ss.instance_bits, ss.instances,
                                       static_cast<uint32_t>(iterations + 1)};
+                if (ss.stat_live) {
+                    marks.rows_acc = reinterpret_cast<unsigned long long*>(ss.stat_acc.ptr);
+                    marks.digest_acc = marks.rows_acc + static_cast<size_t>(kInstanceStatSlots) * ss.instances;
+                }
                 targets[static_cast<size_t>(fuse)].instances_marked = true;
             }
             check(segmented_distinct_finish(rt.cand_a.ptr, M, rt.shift, known_compressed, rt.cand_b.ptr,
@@ -2768,10 +2802,12 @@ void session_step(sparkalog_hip_store* s) {
     if (ss.instance_bits) {
         for (uint32_t i = 0; i < n_relations; ++i) {
             const RelationState& ts = *find(s, relations[i]);
-            if (ts.delta.rows && !targets[i].instances_marked)
+            if (ts.delta.rows && !targets[i].instances_marked) {
+                ss.stat_live = false;  // this NEWT was not counted
                 track_instances_kernel<<<grid_for(ts.delta.rows), kBlock, 0, s->stream>>>(
                     ts.delta.keys.ptr, ts.delta.rows, ss.instance_bits, ss.instances,
                     static_cast<uint32_t>(iterations + 1), ss.last_new.ptr);
+            }
         }
         check(hipGetLastError());
     }
@@ -3124,6 +3160,14 @@ int session_end(sparkalog_hip_store* s, sparkalog_hip_summary* summary, int rc) 
     Session& ss = session_of(s);
     // FULL leaves the run as one sorted array
     for (uint32_t r : ss.relations) ss.algo_bytes += settle_runs(s, *find(s, r), true);
+    if (ss.stat_live && rc == SPARKALOG_HIP_OK && ss.relations.size() == 1) {
+        ss.stat_valid = true;
+        ss.stat_rel = ss.relations[0];
+        ss.stat_version = find(s, ss.stat_rel)->version;
+        ss.stat_bits = ss.instance_bits;
+        ss.stat_instances = ss.instances;
+    }
+    ss.stat_live = false;
     check(hipEventRecord(ss.t_end, s->stream));
     check(hipEventSynchronize(ss.t_end));
     float ms = 0.f;
@@ -3195,6 +3239,28 @@ extern "C" int sparkalog_hip_store_instance_stats(sparkalog_hip_store* s, uint32
     RelationState* st = find(s, id);
     if (!st) return SPARKALOG_HIP_ERR_MISSING_RELATION;
     SPK_GUARD_BEGIN
+    Session& ss = session_of(s);
+    if (ss.stat_valid && !ss.active && id == ss.stat_rel && st->version == ss.stat_version &&
+        instance_bits == ss.stat_bits && instances == ss.stat_instances) {
+        // accumulated by the last session, and the relation is unchanged:
+        // the sum of each instance's copies
+        const size_t n_acc = 2 * static_cast<size_t>(kInstanceStatSlots) * instances;
+        std::vector<uint64_t> copies(n_acc);
+        if (n_acc)
+            check(hipMemcpyAsync(copies.data(), ss.stat_acc.ptr, n_acc * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                 s->stream));
+        check(hipStreamSynchronize(s->stream));
+        for (uint32_t i = 0; i < instances; ++i) {
+            uint64_t r = 0, d = 0;
+            for (uint32_t j = 0; j < kInstanceStatSlots; ++j) {
+                r += copies[static_cast<size_t>(j) * instances + i];
+                d += copies[(static_cast<size_t>(kInstanceStatSlots) + j) * instances + i];
+            }
+            rows[i] = r;
+            digests[i] = d;
+        }
+        return SPARKALOG_HIP_OK;
+    }
     settle_all(s, *st);
     unsigned long long* acc =
         reinterpret_cast<unsigned long long*>(s->probes(2 * static_cast<size_t>(instances) + 2));

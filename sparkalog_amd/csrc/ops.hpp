@@ -103,12 +103,33 @@ constexpr uint32_t kSegTileDefault = 3072;
 // Instance tracking for a batch of instances in one id space (instance =
 // first value >> bits): finish raises last_new[instance] to `value` for every
 // instance with an output row (at most `instances`; last_new null: none).
+// With rows_acc / digest_acc, finish also adds each instance's output rows
+// and the sum of instance_pair_hash over them (the per-instance statistics
+// of the batched fixpoint, accumulated as NEWT is written), spread over
+// kInstanceStatSlots copies (slot-major: copy j of instance i at
+// j * instances + i; a block adds to copy blockIdx % slots) so that the
+// atomics of many blocks do not all meet on one counter; the reader sums
+// the copies.
+constexpr uint32_t kInstanceStatSlots = 16;
 struct InstanceMarks {
     uint32_t* last_new = nullptr;
     uint32_t bits = 0;
     uint32_t instances = 0;
     uint32_t value = 0;
+    unsigned long long* rows_acc = nullptr;
+    unsigned long long* digest_acc = nullptr;
 };
+
+// Digest term of a pair (x, z) of an instance: its local ids (the low `bits`
+// bits of each) through the splitmix64 finalizer (Steele, Lea and Flood,
+// 2014); an instance's digest is the sum over its pairs (order-independent).
+__host__ __device__ inline uint64_t instance_pair_hash(uint32_t x, uint32_t z, uint32_t bits) {
+    const uint32_t mask = bits >= 32 ? 0xFFFFFFFFu : ((1u << bits) - 1u);
+    uint64_t zz = ((static_cast<uint64_t>(x & mask) << 32) | (z & mask)) + 0x9E3779B97F4A7C15ull;
+    zz = (zz ^ (zz >> 30)) * 0xBF58476D1CE4E5B9ull;
+    zz = (zz ^ (zz >> 27)) * 0x94D049BB133111EBull;
+    return zz ^ (zz >> 31);
+}
 size_t segmented_distinct_meta_bytes(size_t rows, uint32_t tile_rows = kSegTileDefault);
 hipError_t segmented_distinct_prepare(uint64_t* keys, size_t rows, uint32_t minor_bits, uint64_t* side,
                                       void* meta, size_t meta_bytes, uint64_t* side_rows, hipStream_t stream,

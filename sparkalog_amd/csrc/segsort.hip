@@ -834,6 +834,18 @@ __global__ __launch_bounds__(kBlock) void seg_write_kernel(const uint64_t* keys,
     __shared__ uint64_t offset_slot;
     const uint64_t minor_mask = (1ull << minor_bits) - 1;
     const uint32_t lane = lane_id();
+    // the instance this wave is summing, and this lane's share of its sums
+    uint32_t stat_inst = 0xFFFFFFFFu;
+    unsigned long long stat_digest = 0, stat_rows = 0;
+    const size_t stat_slot = static_cast<size_t>(blockIdx.x % kInstanceStatSlots) * marks.instances;
+    const auto flush_stats = [&]() {  // wave-uniform
+        const unsigned long long d = wave_sum(stat_digest), r = wave_sum(stat_rows);
+        if (lane == 0 && stat_inst < marks.instances && r) {
+            atomicAdd(&marks.digest_acc[stat_slot + stat_inst], d);
+            atomicAdd(&marks.rows_acc[stat_slot + stat_inst], r);
+        }
+        stat_digest = stat_rows = 0;
+    };
     for (size_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         const uint64_t c = counts[t];
         if (c == 0) continue;  // block-uniform
@@ -895,6 +907,44 @@ __global__ __launch_bounds__(kBlock) void seg_write_kernel(const uint64_t* keys,
                         atomicMax(&marks.last_new[inst], marks.value);
                 }
             }
+            if (marks.rows_acc) {
+                // per-instance rows and digest terms: a wave sums an item's
+                // 64 rows while they belong to the instance it is summing
+                // (valid lanes are a prefix of the wave), and adds the sums
+                // when the instance changes or the tile ends
+#pragma unroll
+                for (int q = 0; q < kSegWriteItems; ++q) {
+                    const uint64_t i = c0 + threadIdx.x + static_cast<uint64_t>(q) * kBlock;
+                    const bool valid = i < c;
+                    uint64_t k = v[q];
+                    if (big) k = ((k >> minor_bits) << 32) | (k & minor_mask);
+                    const uint32_t x = static_cast<uint32_t>(k >> 32);
+                    const uint32_t inst = valid ? x >> marks.bits : 0xFFFFFFFFu;
+                    if (ballot(valid) == 0) continue;  // wave-uniform
+                    const uint32_t lead = __builtin_amdgcn_readfirstlane(inst);
+                    const unsigned long long term =
+                        valid ? instance_pair_hash(x, static_cast<uint32_t>(k), marks.bits) : 0ull;
+                    if (ballot(valid && inst != lead) == 0) {
+                        if (lead != stat_inst) {
+                            flush_stats();
+                            stat_inst = lead;
+                        }
+                        stat_digest += term;
+                        stat_rows += valid ? 1ull : 0ull;
+                    } else {
+                        flush_stats();
+                        stat_inst = 0xFFFFFFFFu;
+                        if (valid && inst < marks.instances) {
+                            atomicAdd(&marks.digest_acc[stat_slot + inst], term);
+                            atomicAdd(&marks.rows_acc[stat_slot + inst], 1ull);
+                        }
+                    }
+                }
+            }
+        }
+        if (marks.rows_acc) {
+            flush_stats();
+            stat_inst = 0xFFFFFFFFu;
         }
         __syncthreads();  // offset_slot is rewritten for the block's next tile
     }
