@@ -45,6 +45,34 @@ def test_batched_matches_bfs_and_plain(stream):
     assert np.array_equal(plain.iterations, res.iterations)
 
 
+def test_instance_stats_follow_a_change_to_the_relation(stream):
+    # the statistics a run accumulates answer instance_stats only while the
+    # relation is unchanged: a pair added to one instance's FULL afterwards
+    # shows in that instance's rows and digest, and nowhere else
+    from sparkalog_amd.recursion import RecursiveExecutor
+
+    s, d, offs, counts = workload.batch_graphs(seed=0x5C, instances=24, n_max=512)
+    ref = bfs_per_instance(s, d, offs, counts)
+    bits = batch.local_bits_for(counts)
+    members = list(range(counts.size))
+    gs, gd = batch._wave_edges(s, d, offs, members, bits)
+    store = RelationStore(stream)
+    store.insert_static(batch.EDGE, (gs, gd))
+    store.insert_recursive(batch.PATH, (gs, gd), (gs, gd))
+    store.set_instance_tracking(bits, len(members))
+    RecursiveExecutor.compile(batch.TC_PLAN, store).run(store, 100_000)
+    rows, digests = store.instance_stats(batch.PATH, bits, len(members))
+    assert np.array_equal(rows, ref[:, 0]) and np.array_equal(digests, ref[:, 1])
+    first, second = store.full(batch.PATH)
+    have = set(zip(first.tolist(), second.tolist()))
+    n0 = int(counts[0])
+    pair = next((x, z) for x in range(n0) for z in range(n0) if (x, z) not in have)
+    store.merge_rows(batch.PATH, 0, (np.array([pair[0]], np.uint32), np.array([pair[1]], np.uint32)))
+    rows2, digests2 = store.instance_stats(batch.PATH, bits, len(members))
+    assert rows2[0] == rows[0] + 1 and digests2[0] != digests[0]
+    assert np.array_equal(rows2[1:], rows[1:]) and np.array_equal(digests2[1:], digests[1:])
+
+
 def test_full_batch_matches_fixture(stream):
     exp = np.load(GOLDEN / "batch_expected.npz")
     s, d, offs, counts = workload.batch_graphs()
