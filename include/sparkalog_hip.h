@@ -389,7 +389,11 @@ int sparkalog_hip_instance_iterations(sparkalog_hip_store* store, uint32_t* last
  * (host copy only; valid after a step of a session begun with tracking). */
 int sparkalog_hip_instance_delta_rows(sparkalog_hip_store* store, uint64_t* rows, uint32_t instances);
 
-/* Per instance: FULL rows and digest of its local pairs. */
+/* Per instance: FULL rows and digest of its local pairs (the digest: the
+ * sum over the pairs of splitmix64's finalizer applied to the local pair).
+ * Right after a session that tracked these instances of this relation, and
+ * while the relation is unchanged, the answer is the statistics the session
+ * accumulated as it wrote each NEWT; otherwise FULL is read. */
 int sparkalog_hip_store_instance_stats(sparkalog_hip_store* store, uint32_t id, uint32_t instance_bits,
                                        uint32_t instances, uint64_t* rows, uint64_t* digests);
 
