@@ -2,7 +2,7 @@
 uneven size run as one semi-naive loop.
 
 Instance i's local node v becomes global id (slot << b) | v, b = bits of the
-largest local id, so the batch is the disjoint union of the instances and its
+largest local id in its wave, so the batch is the disjoint union of the instances and its
 closure is the union of their closures. Sort keys of the union are
 (slot, x, y) packed into one u64; the executor's segmented distinct and
 tiered FULL apply unchanged, and per-instance results come from one
@@ -113,7 +113,6 @@ def run_batched(src, dst, offsets, node_counts, store: RelationStore, budget_pai
                 max_iterations: int = 100_000) -> BatchResult:
     n = len(node_counts)
     edge_counts = np.diff(np.asarray(offsets, dtype=np.int64))
-    bits = local_bits_for(node_counts)
     rows = np.zeros(n, np.uint64)
     digests = np.zeros(n, np.uint64)
     iterations = np.zeros(n, np.uint32)
@@ -122,13 +121,16 @@ def run_batched(src, dst, offsets, node_counts, store: RelationStore, budget_pai
     log = []
     for members in waves:
         tw = time.perf_counter()
-        gs, gd = _wave_edges(src, dst, offsets, members, bits)
+        # a wave's local ids take the bits of its largest member, so a wave
+        # of small instances has a small id space (and small indexes)
+        wbits = local_bits_for(np.asarray(node_counts)[members])
+        gs, gd = _wave_edges(src, dst, offsets, members, wbits)
         store.clear()
         store.insert_static(EDGE, (gs, gd))
         store.insert_recursive(PATH, (gs, gd), (gs, gd))
-        store.set_instance_tracking(bits, len(members))
+        store.set_instance_tracking(wbits, len(members))
         summary = RecursiveExecutor.compile(TC_PLAN, store).run(store, max_iterations)
-        r, d = store.instance_stats(PATH, bits, len(members))
+        r, d = store.instance_stats(PATH, wbits, len(members))
         last = store.instance_iterations(len(members))
         idx = np.asarray(members)
         rows[idx], digests[idx] = r, d
